@@ -1,0 +1,244 @@
+/*
+ * pgk_force.h - C ABI of the MI355X force-evaluation library (libpgkforce.so),
+ * a drop-in for peachgk_md's force path.
+ *
+ * The reference evaluates forces in two Fortran drivers that read all system
+ * data from module globals (md_global.inc:143-470) and take the box, cutoffs
+ * and flags as arguments:
+ *   calforce     peachgk-md/calforce.F90:8-1073      (interface_mdtech.F90:358)
+ *   calforce_hf  peachgk-md/calforce_hf.F90:8-1451   (interface_mdtech.F90:609)
+ * and rebuild their Verlet list with
+ *   mklist2a     peachgk-md/mklist.F90:8-234
+ *   mklist_cell  peachgk-md/mklist_cell.F90:8-352
+ * This header replaces those entry points with plain-pointer C functions that a
+ * Fortran host binds with iso_c_binding (see the pgk_force_mod module shipped
+ * beside the library).  All arrays are host arrays in the reference's own
+ * layout and units: positions and forces are Fortran (3,natom) column-major
+ * doubles, atom indices in topology arrays are 1-based, lengths are in the
+ * reference's non-dimensional unit (1 A), energies in eref, and charges are
+ * already multiplied by 1/sqrt(4*pi*eps0) (prepmd.F90:239-242).
+ *
+ * Every function returns PGK_OK (0) or a negative PGK_E* code; the message of
+ * the last failure on the calling thread is returned by pgk_last_error().
+ */
+#ifndef PGK_FORCE_H
+#define PGK_FORCE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGK_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+  PGK_OK = 0,
+  PGK_EINVAL = -1,     /* bad argument (size, index, flag)                    */
+  PGK_EDEVICE = -2,    /* HIP runtime failure or no usable gfx950 device      */
+  PGK_ENOMEM = -3,     /* device allocation failed                            */
+  PGK_ESTATE = -4,     /* call out of order (e.g. no list built yet)          */
+  PGK_EOVERFLOW = -5   /* list or cell capacity exceeded (mklist_cell.F90:121-145,342-346) */
+};
+
+/* ---- arithmetic forms ---------------------------------------------------- */
+enum {
+  PGK_PREC_FP64 = 0,   /* FP64 throughout, as the reference                    */
+  PGK_PREC_MIXED = 1,  /* FP32 pair/spline/FFT arithmetic, 64-bit fixed-point
+                          force/virial/heat-flux sums, FP64 reductions of
+                          energies and the Ewald self/net-charge terms          */
+  PGK_PREC_FP32 = 2    /* FP32 everywhere inside the force evaluation         */
+};
+
+/* ---- MTS levels (md_global.inc:138-140) ---------------------------------- */
+enum { PGK_MTS_LONG = 1, PGK_MTS_MED = 2, PGK_MTS_SHORT = 3 };
+
+/* ---- heat-flux virial modes (calvirihf.F90:8-405) ------------------------ */
+enum {
+  PGK_HF_VOLUME = 0,   /* default build: z-fragment weighted column 3 (ifhfvol) */
+  PGK_HF_SURFACE = 1,  /* ifhfvol = .false.: plane-crossing (method of planes)  */
+  PGK_HF_BULK = 2      /* -D_HF_BULK: full 3x3, minimum image on all axes       */
+};
+enum { PGK_HFTYP_ATOM = 1, PGK_HFTYP_MOLE = 2 };   /* md_global.inc:142-143 */
+
+/* ---- energy record: index into pgk_force_out.ene ------------------------- */
+enum {
+  PGK_E_TOT = 0, PGK_E_NONBON, PGK_E_VDW, PGK_E_ELC, PGK_E_EWK, PGK_E_EWC,
+  PGK_E_EWNQ, PGK_E_VDW14, PGK_E_ELC14, PGK_E_BOND, PGK_E_ANGL, PGK_E_TORS,
+  PGK_E_TORSRB,
+  PGK_V_COUL,          /* pot_viri_coul */
+  PGK_V_LJ,            /* pot_viri_lj   */
+  PGK_V_BOND,          /* atm_viri_bond */
+  PGK_V_ANGL,          /* atm_viri_angl */
+  PGK_V_TORS,          /* atm_viri_tors (periodic + RB) */
+  PGK_V_14,            /* atm_viri_14   */
+  PGK_NENE
+};
+/* ---- virial tensors: 3x3 blocks (Fortran order, (force comp, r comp)) ----- */
+enum {
+  PGK_T_COUL = 0,      /* pot_virit_coul (real + excluded + reciprocal + netq) */
+  PGK_T_LJ,            /* pot_virit_lj   */
+  PGK_T_BOND,          /* atm_virit_bond */
+  PGK_T_ANGL,          /* atm_virit_angl */
+  PGK_T_TORS,          /* atm_virit_tors */
+  PGK_T_14,            /* atm_virit_14   */
+  PGK_NTENS
+};
+/* ---- per-atom heat-flux classes (calforce_hf.F90:300-330) ---------------- */
+enum {
+  PGK_HF_BO = 0,       /* pot_bo_atm   / viribot_atm   */
+  PGK_HF_AN,           /* pot_an_atm   / viriant_atm   */
+  PGK_HF_TO,           /* pot_to_atm   / viritot_atm   */
+  PGK_HF_14,           /* pot_14_atm   / viri14t_atm   */
+  PGK_HF_ELIN,         /* pot_elin_atm / virielint_atm */
+  PGK_HF_LJIN,         /* pot_ljin_atm / viriljint_atm */
+  PGK_HF_ELC,          /* pot_elc_atm  / virielct_atm  */
+  PGK_HF_VDW,          /* pot_vdw_atm  / viriljt_atm   */
+  PGK_NHF
+};
+
+/*
+ * System description: the part of md_global / spme_global the force path
+ * reads (md_global.inc:143-470, spme_global.inc:24-43).  Copied at create
+ * time; the caller keeps ownership of every array.
+ */
+typedef struct pgk_system_desc {
+  int32_t natom;
+  int32_t ntype;               /* LJ tables are ntype x ntype                 */
+  const double *charge;        /* [natom] atmchrg (scaled)                     */
+  const int32_t *atype;        /* [natom] atmindex, 1..ntype                   */
+  const int32_t *molid;        /* [natom] irmolept_list, 1-based               */
+  const double *lj_eps;        /* [ntype*ntype] vdw_welij (column-major)       */
+  const double *lj_rmin;       /* [ntype*ntype] vdw_radij (R_min)              */
+  const int32_t *inttyp;       /* [ntype*ntype] inter_inttyp (1 = INTTYPE_VDW) */
+  /* exclusion CSR as left by mkexcl (mkexcl.F90:193-222)                      */
+  int32_t nexcl;               /* length of excl_list                          */
+  const int32_t *excl_index;   /* [natom+1] 1-based start of each atom's run   */
+  const int32_t *excl_list;    /* [nexcl] partners j > i, or a single 0        */
+  /* harmonic bonds (calbond.F90)                                              */
+  int32_t nbond, nbondtyp;
+  const int32_t *bond_ij;      /* [2*nbond] (i,j) pairs, 1-based               */
+  const int32_t *bond_type;    /* [nbond] 1-based                              */
+  const double *bond_k, *bond_r0;            /* [nbondtyp]                     */
+  /* harmonic angles (calangl.F90)                                             */
+  int32_t nangl, nangltyp;
+  const int32_t *angl_ijk;     /* [3*nangl]                                    */
+  const int32_t *angl_type;
+  const double *angl_k, *angl_theta0;        /* [nangltyp], theta0 in rad      */
+  /* periodic torsions (caltors.F90) and their 1-4 pairs (calnon14.F90)        */
+  int32_t ntors, ntorstyp;
+  const int32_t *tors_ijkl;    /* [4*ntors]                                    */
+  const int32_t *tors_type;
+  const double *tors_ring14;   /* [ntors] ring14                               */
+  const int32_t *tors_divfac;  /* [ntorstyp] para_divfac                       */
+  const double *tors_barhig, *tors_phsang, *tors_period;   /* [ntorstyp]       */
+  const double *tors_divfac_vdw, *tors_divfac_elc;         /* [ntorstyp]       */
+  const int32_t *tors_if14;    /* [ntorstyp] ifcalnonb14 (0/1)                 */
+  /* Ryckaert-Bellemans torsions (caltorsrb.F90) and their 1-4 pairs           */
+  int32_t ntorsrb, ntorsrbtyp;
+  const int32_t *torsrb_ijkl;
+  const int32_t *torsrb_type;
+  const double *torsrb_ring14;
+  const double *torsrb_c;      /* [6*ntorsrbtyp] para_barhigrb(1:6,t)          */
+  const double *torsrb_divfac_vdw, *torsrb_divfac_elc;
+  const int32_t *torsrb_if14;
+  /* Ewald / SPME parameters (already non-dimensional)                        */
+  double alpha;                /* Ewald alpha [1/A]                            */
+  double rrcut;                /* Ewald real-space cutoff                      */
+  double rcut;                 /* LJ cutoff                                    */
+  int32_t nspltbl;             /* spline table points (spline_interp.F90:74)   */
+  int32_t nfft1, nfft2, nfft3; /* SPME grid                                    */
+  int32_t pme_order;           /* B-spline order, 3..10                        */
+  int32_t ifnetqcorrp;         /* net-charge correction for pressure (0/1)     */
+  const double *box;           /* [3] initial box: fixes rkcut (erf_corr_cutoff.F90:28-62) */
+} pgk_system_desc;
+
+/* MTS level of each force term, as the mts_* arguments of calforce. A term is
+ * evaluated when its level equals the mts_flag of the call; 0 disables it.  */
+typedef struct pgk_mts {
+  int32_t bond, angl, tors, torsrb, vdw, ewk, nonb14;
+} pgk_mts;
+
+/* Outputs of one calforce call.  force and ene are required; the rest may be
+ * NULL.  Semantics follow calforce.F90:307-345 (everything zeroed on entry).  */
+typedef struct pgk_force_out {
+  double *force;               /* [3*natom]                                    */
+  double *ene;                 /* [PGK_NENE]                                   */
+  double *virt;                /* [9*PGK_NTENS]                                */
+  double *for_viri_coul;       /* [3*natom] Coulomb part of the force          */
+  double *for_viri_lj;         /* [3*natom] LJ part of the force               */
+} pgk_force_out;
+
+/* Heat-flux region set-up: the extra arguments of calforce_hf. */
+typedef struct pgk_hf_desc {
+  int32_t mode;                /* PGK_HF_VOLUME / _SURFACE / _BULK              */
+  int32_t nregion;
+  const double *z1, *z2;       /* [nregion] hfzpos1 / hfzpos2                  */
+  const int32_t *hftyp;        /* [natom] hftyp_atm                            */
+  int32_t nmol;
+  const double *molecom;       /* [3*nmol] molecule centres of mass (calcom)   */
+} pgk_hf_desc;
+
+/* Per-atom heat-flux outputs: PGK_NHF classes, each natom energies and a
+ * Fortran (3,3,natom,nregion) virial array.                                   */
+typedef struct pgk_hf_out {
+  double *pot_atm;             /* [PGK_NHF*natom]                               */
+  double *viri_atm;            /* [PGK_NHF*9*natom*nregion]                     */
+} pgk_hf_out;
+
+typedef struct pgk_ctx pgk_ctx;
+
+/* Library version and the last error message of the calling thread. */
+int32_t pgk_abi_version(void);
+const char *pgk_last_error(void);
+
+/* Create a context on the current HIP device: copies the description, builds
+ * the erfc spline tables (spline_interp.F90:14-136), the B-spline moduli and
+ * rkcut (fft_pme_init.F90:14-182, erf_corr_cutoff.F90:28-62) and the Ewald
+ * self energy.  Replaces the module set-up done by main_md.F90:1101-1187.   */
+int pgk_create(const pgk_system_desc *desc, int32_t precision, pgk_ctx **out);
+int pgk_destroy(pgk_ctx *ctx);
+
+/* Box lengths [3] (xcel, ycel, zcel).  rkcut keeps its creation-time value,
+ * as in the reference.                                                        */
+int pgk_set_box(pgk_ctx *ctx, const double *box);
+/* Copy positions (Fortran atmcor(3,natom)) from host memory. */
+int pgk_set_positions(pgk_ctx *ctx, const double *pos);
+/* Same from device memory (no host round trip). */
+int pgk_set_positions_device(pgk_ctx *ctx, const double *d_pos);
+
+/* Verlet list rebuild: the pair set of mklist2a / mklist_cell (i<j, r^2 <
+ * rcut_book^2 in FP64 minimum image, exclusions removed), stored as cluster
+ * pairs with per-pair masks.  Replaces mklist2a / mklist_cell.              */
+int pgk_build_list(pgk_ctx *ctx, double rcut_book);
+/* Number of listed pairs (nlistnball without the 0 sentinels). */
+int pgk_list_npairs(pgk_ctx *ctx, int64_t *npairs);
+/* Export the listed pair set as sorted (i,j) 1-based pairs, i<j [2*npairs]. */
+int pgk_list_pairs(pgk_ctx *ctx, int32_t *pairs, int64_t capacity);
+
+/* One calforce call (calforce.F90:8-1073): zero outputs, evaluate every term
+ * whose MTS level equals mts_flag, add the net-charge terms on MTS_LONG, sum
+ * the energies.  ifcalpreatom selects the virial-producing variants
+ * (ewaldrp, calbondp, ...).  Positions must have been set.                  */
+int pgk_calforce(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
+                 int32_t ifcalpreatom, pgk_force_out *out);
+/* One calforce_hf call on a heat-flux step (calforce_hf.F90:429-1451). */
+int pgk_calforce_hf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
+                    const pgk_hf_desc *hf, pgk_force_out *out,
+                    pgk_hf_out *hfo);
+
+/* Device-resident variant used by GPU-resident drivers: same as pgk_calforce
+ * but the force stays on the device (d_force, [3*natom] doubles, may be NULL)
+ * and only the energy/virial records are copied to the host.                */
+int pgk_calforce_device(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
+                        int32_t ifcalpreatom, double *d_force, double *ene,
+                        double *virt);
+
+/* Wait for all work queued by the context. */
+int pgk_synchronize(pgk_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGK_FORCE_H */
