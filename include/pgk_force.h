@@ -238,6 +238,17 @@ int pgk_calforce_device(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
 /* Wait for all work queued by the context. */
 int pgk_synchronize(pgk_ctx *ctx);
 
+/* Stage timing (no reference counterpart: instrumentation for measurement).
+ * enable != 0 turns HIP-event timing of each stage on (and clears the
+ * totals); ms[PGK_NPROF_STAGES] and calls[PGK_NPROF_STAGES] receive the
+ * accumulated device time and launch count of each stage.                    */
+enum {
+  PGK_PROF_PAIRS = 0, PGK_PROF_EXCL, PGK_PROF_SPREAD, PGK_PROF_FFT,
+  PGK_PROF_GATHER, PGK_PROF_BONDED, PGK_PROF_LIST, PGK_PROF_OTHER,
+  PGK_NPROF_STAGES
+};
+int pgk_profile(pgk_ctx *ctx, int32_t enable, double *ms, int64_t *calls);
+
 #ifdef __cplusplus
 }
 #endif

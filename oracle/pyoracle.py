@@ -78,6 +78,14 @@ def fft3d(grid_complex: np.ndarray, isign: int) -> np.ndarray:
     return g
 
 
+def _list_mode(sysd, rcut_book, mode):
+    """The reference switches the cell index off when an axis holds fewer than
+    three cells (rdscript.F90:1596-1610); mirror that choice."""
+    if mode == 1 and min(int(b / rcut_book) for b in sysd["box"]) < 3:
+        return 0
+    return mode
+
+
 class _Outputs:
     def __init__(self, n, nreg=0):
         self.force = np.zeros((n, 3))
@@ -158,6 +166,7 @@ class Oracle:
         return self.L.pgko_pot_ewc(self.h)
 
     def mklist(self, rcut_book, mode=0):
+        mode = _list_mode(self.sysd, rcut_book, mode)
         nl = self.L.pgko_mklist(self.h, float(rcut_book), int(mode))
         idx = np.zeros(self.n, np.int32)
         lst = np.zeros(max(nl, 1), np.int32)
@@ -345,6 +354,7 @@ class Reference:
         self.L.pgkref_set_box(C.c_void_p(b.ctypes.data))
 
     def mklist(self, rcut_book, mode=0):
+        mode = _list_mode(self.sysd, rcut_book, mode)
         nl = _big_stack_call(self.L.pgkref_mklist, float(rcut_book), int(mode))
         idx = np.zeros(self.n, np.int32)
         lst = np.zeros(max(nl, 1), np.int32)

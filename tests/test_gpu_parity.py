@@ -1,0 +1,132 @@
+"""GPU parity: libpgkforce (through its C ABI) against the oracle.
+
+Bars: the listed pair set is bit-exact; FP64 forces within rms relative error
+1e-5 (observed ~1e-13) and energies/virials to 1e-9 relative; the mixed and
+FP32 forms within rms relative force error 1e-4.
+"""
+import numpy as np
+import pytest
+
+from thrid_amd import abi, systems
+from helpers import rms_rel, ene, water648, small_slab, butane_box, pairs_from_csr
+
+pytestmark = pytest.mark.gpu
+
+ENE_CHECK = ["pot_vdw", "pot_elc", "pot_ewk", "pot_ewc", "pot_ewnq", "pot_vdw14", "pot_elc14",
+             "pot_bond", "pot_angl", "pot_tors", "pot_torsrb", "pot_tot", "pot_viri_coul",
+             "pot_viri_lj", "atm_viri_bond", "atm_viri_angl", "atm_viri_tors", "atm_viri_14"]
+
+
+def _ctx(sysd, prec="fp64"):
+    from thrid_amd.pgk import ForceContext
+    return ForceContext(sysd, prec)
+
+
+def _oracle(oracle_mod, sysd, mode=1):
+    o = oracle_mod.Oracle(sysd)
+    o.mklist(sysd["rcut_book"], mode)
+    return o
+
+
+def _compare(got, ref, ftol, etol, vtol):
+    err = rms_rel(got["force"], ref["force"])
+    assert err <= ftol, f"rms rel force error {err:.3e} > {ftol}"
+    scale = max(1.0, float(np.max(np.abs(ref["ene"]))))
+    for name in ENE_CHECK:
+        k = abi.ENE_NAMES.index(name)
+        d = abs(got["ene"][k] - ref["ene"][k])
+        assert d <= etol * max(abs(ref["ene"][k]), 1e-3 * scale), \
+            f"{name}: {got['ene'][k]!r} vs {ref['ene'][k]!r}"
+    vs = max(1.0, float(np.max(np.abs(ref["virt"]))))
+    dv = float(np.max(np.abs(got["virt"] - ref["virt"])))
+    assert dv <= vtol * vs, f"virial tensors differ by {dv:.3e} (scale {vs:.3e})"
+    return err
+
+
+@pytest.mark.parametrize("which,mode", [("water", 0), ("slab", 1), ("butane", 1)])
+def test_list_pair_set_bitexact(oracle_mod, which, mode):
+    s = {"water": water648, "slab": small_slab, "butane": butane_box}[which]()
+    o = oracle_mod.Oracle(s)
+    idx, lst = o.mklist(s["rcut_book"], mode)
+    want = pairs_from_csr(idx, lst, s["natom"])
+    c = _ctx(s)
+    c.mklist(s["rcut_book"])
+    got = c.list_pairs()
+    print(f"{which}: {len(want)} pairs")
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("which", ["water", "slab", "slab_periodic", "butane"])
+def test_calforce_fp64(oracle_mod, which):
+    s = {"water": water648, "slab": small_slab, "butane": butane_box,
+         "slab_periodic": lambda: small_slab("periodic")}[which]()
+    ref = _oracle(oracle_mod, s).calforce(abi.MTS_LONG)
+    c = _ctx(s)
+    c.mklist(s["rcut_book"])
+    got = c.calforce(abi.MTS_LONG)
+    err = _compare(got, ref, 1e-5, 1e-9, 1e-9)
+    print(f"{which}: FP64 rms rel force err {err:.3e}")
+
+
+def test_calforce_split_and_mts(oracle_mod):
+    """for_viri_coul / for_viri_lj and the MTS gating (short = bonded only)."""
+    s = small_slab()
+    o = _oracle(oracle_mod, s)
+    c = _ctx(s)
+    c.mklist(s["rcut_book"])
+    ref = o.calforce(abi.MTS_LONG)
+    got = c.calforce(abi.MTS_LONG, split=True)
+    assert rms_rel(got["for_viri_coul"], ref["for_viri_coul"]) < 1e-9
+    assert rms_rel(got["for_viri_lj"], ref["for_viri_lj"]) < 1e-9
+    m = abi.Mts(abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_LONG,
+                abi.MTS_LONG, abi.MTS_LONG)
+    for flag in (abi.MTS_SHORT, abi.MTS_LONG):
+        r = o.calforce(flag, mts=m)
+        g = c.calforce(flag, mts=m)
+        assert rms_rel(g["force"], r["force"]) < 1e-9
+        for name in ("pot_bond", "pot_angl", "pot_torsrb", "pot_vdw", "pot_ewk", "pot_ewnq"):
+            assert abs(ene(g, name) - ene(r, name)) <= 1e-9 * max(1.0, abs(ene(r, name))), name
+
+
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-4), ("fp32", 1e-4)])
+@pytest.mark.parametrize("which", ["water", "slab"])
+def test_calforce_reduced_precision(oracle_mod, prec, tol, which):
+    s = {"water": water648, "slab": small_slab}[which]()
+    ref = _oracle(oracle_mod, s).calforce(abi.MTS_LONG)
+    c = _ctx(s, prec)
+    c.mklist(s["rcut_book"])
+    got = c.calforce(abi.MTS_LONG)
+    err = rms_rel(got["force"], ref["force"])
+    print(f"{which} {prec}: rms rel force err {err:.3e}")
+    assert err <= tol
+    for name in ("pot_elc", "pot_vdw", "pot_ewk", "pot_tot"):
+        assert abs(ene(got, name) - ene(ref, name)) <= 1e-4 * max(1.0, abs(ene(ref, name))), name
+
+
+def test_mixed_is_deterministic():
+    s = small_slab()
+    c = _ctx(s, "mixed")
+    c.mklist(s["rcut_book"])
+    a = c.calforce(abi.MTS_LONG)
+    b = c.calforce(abi.MTS_LONG)
+    c2 = _ctx(s, "mixed")
+    c2.mklist(s["rcut_book"])
+    d = c2.calforce(abi.MTS_LONG)
+    assert np.array_equal(a["force"], b["force"]) and np.array_equal(a["force"], d["force"])
+    assert np.array_equal(a["ene"], d["ene"])
+
+
+def test_rebuild_after_motion(oracle_mod):
+    """List rebuilt at new positions; pair set and forces still match."""
+    s = dict(water648())
+    rng = np.random.default_rng(3)
+    s["pos"] = s["pos"] + rng.normal(scale=0.3, size=s["pos"].shape)
+    c = _ctx(water648())
+    c.mklist(s["rcut_book"])
+    c.set_positions(s["pos"])
+    c.mklist(s["rcut_book"])
+    o = oracle_mod.Oracle(s)
+    idx, lst = o.mklist(s["rcut_book"], 0)
+    assert np.array_equal(c.list_pairs(), pairs_from_csr(idx, lst, s["natom"]))
+    assert rms_rel(c.calforce()["force"], o.calforce()["force"]) < 1e-9

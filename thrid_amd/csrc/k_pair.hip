@@ -1,0 +1,343 @@
+// k_pair.hip - real-space Ewald + Lennard-Jones pair kernel and the
+// excluded-pair correction.
+//
+// Restates ewaldrp (ewaldr.F90:404-838) for the listed pairs:
+//   r^2 <= max(rrcut^2, rcut^2), then
+//   Coulomb if r <= rrcut:   f = qq*S_f(r)*r_ij,  E = qq*S_e(r)
+//     (S_f, S_e: the reference's natural cubic splines of
+//      erfc(ar)/r^3 + 2a/sqrt(pi) e^{-a^2r^2}/r^2 and erfc(ar)/r,
+//      spline_interp.F90:96-111, evaluated as SPLINE_INTERPFUNC :196-242)
+//   LJ if r^2 <= rcut^2 and inter_inttyp == VDW:
+//     E = eps[(R/r)^12 - 2(R/r)^6],  f = 12 eps[(R/r)^12 - (R/r)^6]/r^2 * r_ij
+//   virial tensors W(a,n) += f_a r_n per class, scalar LJ virial f.r.
+// and the excluded-pair correction (ewaldr.F90:744-828): for every exclusion
+//   E -= qq*S'_e(r), f = -qq*S'_f(r)*r_ij with the erf tables, no cutoff.
+//
+// Mapping: one 64-lane wavefront per 8-atom i-cluster; lane = ii + 8*jj
+// handles the pair (i-slot ii, j-slot jj) of the current j-cluster, so a
+// cluster-pair mask bit is exactly the lane's predicate.  j-forces are summed
+// over the 8 i-lanes with cross-lane shuffles and added once per j-atom; the
+// i-forces stay in registers over the whole j list.
+#include "pgk_device.h"
+
+template <int PREC>
+struct PairArgs {
+  using R = typename Prec<PREC>::real;
+  using R4 = typename Prec<PREC>::real4;
+  using A = typename Prec<PREC>::acc;
+  const R4 *xq;
+  const int32_t *ct;
+  const int32_t *ci_start, *cj;
+  const uint64_t *mask;
+  const R *spl;          // nspl x 8: force a,b,c,d | energy a,b,c,d
+  const R *lj;           // ntype^2 x 2: eps, rmin^2 (column-major type pairs)
+  const int32_t *inttyp;
+  A *fcl, *fcl_c;
+  double *red;
+  int ncl, ntype, nspl;
+  R L[3], Linv[3];
+  R sqcut, sqcut2, rrcut;
+  R h, hinv, xmax;
+};
+
+template <class R>
+__device__ __forceinline__ void spline2(const R *__restrict__ tb, R x, R h, R hinv, R xmax,
+                                        int nspl, R &sf, R &se) {
+  if (!(x < xmax)) {
+    sf = se = R(0);
+    return;
+  }
+  int i = (int)(x * hinv);
+  i = max(0, min(i, nspl - 1));
+  R d = x - (R)i * h;
+  const R *t = tb + 8 * i;
+  sf = t[0] + d * (t[1] + d * (t[2] + d * t[3]));
+  se = t[4] + d * (t[5] + d * (t[6] + d * t[7]));
+}
+
+// |r|^2 without contraction so the cutoff decision matches the reference
+__device__ __forceinline__ double r2_nc(double x, double y, double z) {
+#pragma clang fp contract(off)
+  return x * x + y * y + z * z;
+}
+__device__ __forceinline__ float r2_nc(float x, float y, float z) {
+#pragma clang fp contract(off)
+  return x * x + y * y + z * z;
+}
+
+template <int PREC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
+  using R = typename Prec<PREC>::real;
+  using R4 = typename Prec<PREC>::real4;
+  // per-lane force sums: FP64 in every form (exact order, no cancellation loss)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + wave;
+  double ev[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) ev[k] = 0.0;
+  if (ci < a.ncl) {
+    const int ii = lane & 7, jj = lane >> 3;
+    const R4 xi = a.xq[ci * PGK_CL + ii];
+    const int ti = a.ct[ci * PGK_CL + ii];
+    double fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
+    R e_elc = 0, e_vdw = 0;
+    R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
+    const int e0 = a.ci_start[ci], e1 = a.ci_start[ci + 1];
+    for (int e = e0; e < e1; e++) {
+      const int cj = a.cj[e];
+      const uint64_t m = a.mask[e];
+      const R4 xj = a.xq[cj * PGK_CL + jj];
+      const int tj = a.ct[cj * PGK_CL + jj];
+      R fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+      if ((m >> lane) & 1ull) {
+        R rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        rx = rx - a.L[0] * fnint(rx * a.Linv[0]);
+        ry = ry - a.L[1] * fnint(ry * a.Linv[1]);
+        rz = rz - a.L[2] * fnint(rz * a.Linv[2]);
+        const R r2 = r2_nc(rx, ry, rz);
+        if (r2 <= a.sqcut) {
+          const R rinv2 = R(1) / r2;
+          const R r = sqrt(r2);
+          if (r <= a.rrcut) {
+            R sf, se;
+            spline2(a.spl, r, a.h, a.hinv, a.xmax, a.nspl, sf, se);
+            const R qq = xi.w * xj.w;
+            const R ft = sf * qq;
+            e_elc += se * qq;
+            cx = ft * rx;
+            cy = ft * ry;
+            cz = ft * rz;
+            tc[0] += cx * rx; tc[1] += cy * ry; tc[2] += cz * rz;
+            tc[3] += cx * ry; tc[4] += cx * rz; tc[5] += cy * rz;
+          }
+          if (r2 <= a.sqcut2) {
+            const int tp = (ti - 1) + a.ntype * (tj - 1);
+            if (a.inttyp[tp] == 1) {
+              const R eps = a.lj[2 * tp], rm2 = a.lj[2 * tp + 1];
+              const R t2 = rinv2 * rm2;
+              const R t6 = t2 * t2 * t2;
+              const R t12 = t6 * t6;
+              e_vdw += eps * (t12 - R(2) * t6);
+              const R ft = R(12) * eps * (t12 - t6) * rinv2;
+              const R lx = ft * rx, ly = ft * ry, lz = ft * rz;
+              tl[0] += lx * rx; tl[1] += ly * ry; tl[2] += lz * rz;
+              tl[3] += lx * ry; tl[4] += lx * rz; tl[5] += ly * rz;
+              fx = lx;
+              fy = ly;
+              fz = lz;
+            }
+          }
+          fx += cx;
+          fy += cy;
+          fz += cz;
+        }
+      }
+      fix += (double)fx;
+      fiy += (double)fy;
+      fiz += (double)fz;
+      if (SPLIT) {
+        cix += (double)cx;
+        ciy += (double)cy;
+        ciz += (double)cz;
+      }
+      // j side: sum over the 8 i-lanes that share this j-slot
+      R gx = -fx, gy = -fy, gz = -fz;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        gx += __shfl_xor(gx, o, 64);
+        gy += __shfl_xor(gy, o, 64);
+        gz += __shfl_xor(gz, o, 64);
+      }
+      R hx = 0, hy = 0, hz = 0;
+      if (SPLIT) {
+        hx = -cx; hy = -cy; hz = -cz;
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+          hx += __shfl_xor(hx, o, 64);
+          hy += __shfl_xor(hy, o, 64);
+          hz += __shfl_xor(hz, o, 64);
+        }
+      }
+      if (ii == 0 && ((m >> (8 * jj)) & 0xffull)) {
+        const size_t s = (size_t)(cj * PGK_CL + jj) * 3;
+        acc_add(&a.fcl[s + 0], gx);
+        acc_add(&a.fcl[s + 1], gy);
+        acc_add(&a.fcl[s + 2], gz);
+        if (SPLIT) {
+          acc_add(&a.fcl_c[s + 0], hx);
+          acc_add(&a.fcl_c[s + 1], hy);
+          acc_add(&a.fcl_c[s + 2], hz);
+        }
+      }
+    }
+    // i side: sum over the 8 j-lanes that share this i-slot
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) {
+      fix += __shfl_xor(fix, o, 64);
+      fiy += __shfl_xor(fiy, o, 64);
+      fiz += __shfl_xor(fiz, o, 64);
+      if (SPLIT) {
+        cix += __shfl_xor(cix, o, 64);
+        ciy += __shfl_xor(ciy, o, 64);
+        ciz += __shfl_xor(ciz, o, 64);
+      }
+    }
+    if (jj == 0) {
+      const size_t s = (size_t)(ci * PGK_CL + ii) * 3;
+      acc_add(&a.fcl[s + 0], fix);
+      acc_add(&a.fcl[s + 1], fiy);
+      acc_add(&a.fcl[s + 2], fiz);
+      if (SPLIT) {
+        acc_add(&a.fcl_c[s + 0], cix);
+        acc_add(&a.fcl_c[s + 1], ciy);
+        acc_add(&a.fcl_c[s + 2], ciz);
+      }
+    }
+    ev[0] = e_elc;
+    ev[1] = e_vdw;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      ev[2 + k] = tc[k];
+      ev[8 + k] = tl[k];
+    }
+  }
+  block_record<14, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+}
+
+// ---- excluded pairs (FP64 in every form) -----------------------------------
+template <int PREC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_excl(int npair, const int32_t *__restrict__ pairs,
+                                              const double *__restrict__ pos,
+                                              const double *__restrict__ q,
+                                              const double *__restrict__ splx, double h,
+                                              double hinv, double xmax, int nspl, double3 L,
+                                              double3 Li,
+                                              typename Prec<PREC>::acc *fat,
+                                              typename Prec<PREC>::acc *fat_c,
+                                              double *red) {
+  double ev[7] = {0, 0, 0, 0, 0, 0, 0};
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < npair) {
+    int i = pairs[2 * k], j = pairs[2 * k + 1];
+    double rx = pos[3 * i] - pos[3 * j], ry = pos[3 * i + 1] - pos[3 * j + 1],
+           rz = pos[3 * i + 2] - pos[3 * j + 2];
+    rx = rx - L.x * fnint(rx * Li.x);
+    ry = ry - L.y * fnint(ry * Li.y);
+    rz = rz - L.z * fnint(rz * Li.z);
+    double r = sqrt(r2_nc(rx, ry, rz));
+    double sf, se;
+    spline2(splx, r, h, hinv, xmax, nspl, sf, se);
+    double qq = q[i] * q[j];
+    double ft = -sf * qq;
+    double pot = se * qq;
+    double fx = ft * rx, fy = ft * ry, fz = ft * rz;
+    acc_add(&fat[3 * i + 0], fx);
+    acc_add(&fat[3 * i + 1], fy);
+    acc_add(&fat[3 * i + 2], fz);
+    acc_add(&fat[3 * j + 0], -fx);
+    acc_add(&fat[3 * j + 1], -fy);
+    acc_add(&fat[3 * j + 2], -fz);
+    if (SPLIT) {
+      acc_add(&fat_c[3 * i + 0], fx);
+      acc_add(&fat_c[3 * i + 1], fy);
+      acc_add(&fat_c[3 * i + 2], fz);
+      acc_add(&fat_c[3 * j + 0], -fx);
+      acc_add(&fat_c[3 * j + 1], -fy);
+      acc_add(&fat_c[3 * j + 2], -fz);
+    }
+    ev[0] = -pot;
+    ev[1] = fx * rx; ev[2] = fy * ry; ev[3] = fz * rz;
+    ev[4] = fx * ry; ev[5] = fx * rz; ev[6] = fy * rz;
+  }
+  block_record<7, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
+}
+
+template <int PREC>
+static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
+  using R = typename Prec<PREC>::real;
+  using R4 = typename Prec<PREC>::real4;
+  using A = typename Prec<PREC>::acc;
+  PairArgs<PREC> a;
+  a.xq = reinterpret_cast<const R4 *>(c->d_xq);
+  a.ct = c->d_ct;
+  a.ci_start = c->d_ci_start;
+  a.cj = c->d_cj;
+  a.mask = c->d_mask;
+  if (PREC == PGK_PREC_FP64) {
+    a.spl = reinterpret_cast<const R *>(c->d_spl);
+    a.lj = reinterpret_cast<const R *>(c->d_ljp64);
+  } else {
+    a.spl = reinterpret_cast<const R *>(c->d_splf);
+    a.lj = reinterpret_cast<const R *>(c->d_ljp32);
+  }
+  a.inttyp = c->d_inttyp;
+  a.fcl = reinterpret_cast<A *>(c->d_fcl);
+  a.fcl_c = reinterpret_cast<A *>(c->d_fcl_c);
+  a.ncl = c->ncl;
+  a.ntype = c->ntype;
+  a.nspl = c->nspl;
+  for (int d = 0; d < 3; d++) {
+    a.L[d] = (R)c->box[d];
+    a.Linv[d] = (R)(1.0 / c->box[d]);
+  }
+  double sq = c->rrcut * c->rrcut, sq2 = c->rcut * c->rcut;
+  if (sq2 > sq) sq = sq2;
+  a.sqcut = (R)sq;
+  a.sqcut2 = (R)sq2;
+  a.rrcut = (R)c->rrcut;
+  a.h = (R)c->spl_int;
+  a.hinv = (R)(1.0 / c->spl_int);
+  a.xmax = (R)(c->spl_int * (double)(c->nspl - 1));
+  int nblk = pgk_grid_blocks(c->ncl, 4);
+  *off = c->red_used;
+  *nrec = nblk;
+  a.red = c->d_red + (size_t)c->red_used * PGK_REC;
+  c->red_used += nblk;
+  if (split)
+    k_pairs<PREC, true><<<nblk, 256, 0, c->stream>>>(a);
+  else
+    k_pairs<PREC, false><<<nblk, 256, 0, c->stream>>>(a);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+int pgk_pairs_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_pairs<PGK_PREC_FP64>(c, split, off, nrec);
+    case PGK_PREC_MIXED: return launch_pairs<PGK_PREC_MIXED>(c, split, off, nrec);
+    default: return launch_pairs<PGK_PREC_FP32>(c, split, off, nrec);
+  }
+}
+
+template <int PREC>
+static int launch_excl(pgk_ctx *c, bool split, int *off, int *nrec) {
+  using A = typename Prec<PREC>::acc;
+  int n = c->nexclpair;
+  int nblk = pgk_grid_blocks(n > 0 ? n : 1, 256);
+  *off = c->red_used;
+  *nrec = nblk;
+  double *red = c->d_red + (size_t)c->red_used * PGK_REC;
+  c->red_used += nblk;
+  double3 L = make_double3(c->box[0], c->box[1], c->box[2]);
+  double3 Li = make_double3(1.0 / c->box[0], 1.0 / c->box[1], 1.0 / c->box[2]);
+  double xmax = c->spl_int * (double)(c->nspl - 1);
+  A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
+  if (split)
+    k_excl<PREC, true><<<nblk, 256, 0, c->stream>>>(n, c->d_exclpair, c->d_pos, c->d_q,
+                                                    c->d_splx, c->spl_int, 1.0 / c->spl_int,
+                                                    xmax, c->nspl, L, Li, fat, fatc, red);
+  else
+    k_excl<PREC, false><<<nblk, 256, 0, c->stream>>>(n, c->d_exclpair, c->d_pos, c->d_q,
+                                                     c->d_splx, c->spl_int, 1.0 / c->spl_int,
+                                                     xmax, c->nspl, L, Li, fat, fatc, red);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+int pgk_excl_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_excl<PGK_PREC_FP64>(c, split, off, nrec);
+    case PGK_PREC_MIXED: return launch_excl<PGK_PREC_MIXED>(c, split, off, nrec);
+    default: return launch_excl<PGK_PREC_FP32>(c, split, off, nrec);
+  }
+}

@@ -1,0 +1,640 @@
+// k_pme.hip - smooth particle-mesh Ewald reciprocal part.
+//
+// Restates fft_pmep (fft_pme.F90:155-270):
+//   get_scaled_fractionals (:273-313)  fr_d = K_d (w - floor w), w = x_d/L_d
+//   fill_bspline (fft_pme_init.F90:216-260) at fr - int(fr)
+//   fill_charge_grid (:359-422)        Q(int(fr)-p+1 .. int(fr)) += q th1 th2 th3
+//   fft_back  (pubz3d isign -1)        forward transform (e^{-2 pi i jk/n})
+//   scalar_sum (:544-690)              eterm = exp(-pi^2 m^2/a^2)/(pi V B m^2),
+//                                      zero outside |m| <= rkcut and at m = 0
+//                                      (the m = 0 value is left untouched),
+//                                      E = 1/2 sum eterm |S|^2, virial, Q *= eterm
+//   fft_forward (pubz3d isign +1)      unnormalised backward transform
+//   grad_sum (:872-937)                F = -q sum Q (K1 dth1 th2 th3, ...) / L
+//
+// MI355X form: the charge grid is real, so the x transform is real-to-complex
+// (two grid rows packed into one complex line) and only the half spectrum
+// h = 0..K1/2 is kept.  Modes off the h = 0 and h = K1/2 planes stand for
+// themselves and their mirror image; their energy counts twice and each
+// off-diagonal virial product counts (1 + s_a s_b) times, where s_d = -1
+// unless index d sits on its Nyquist plane (whose mirror image keeps the
+// reference's negative representative m_d = -K_d/2).  The y and z transforms
+// are batched Stockham radix-2/3/4/5 FFTs in LDS; the convolution is fused
+// between the forward and backward z transforms, so the spectrum makes one
+// round trip through HBM.
+#include "pgk_device.h"
+
+namespace {
+
+template <class R> struct C2T;
+template <> struct C2T<double> { using type = double2; };
+template <> struct C2T<float> { using type = float2; };
+
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  C r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) {
+  C r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+// multiply by -i (forward) or +i (inverse): sgn = -1 forward
+template <class C>
+__device__ __forceinline__ C cmuli(C a, int sgn) {
+  C r;
+  if (sgn < 0) {
+    r.x = a.y;
+    r.y = -a.x;
+  } else {
+    r.x = -a.y;
+    r.y = a.x;
+  }
+  return r;
+}
+
+// in-register DFT of size R with sign sgn (-1 forward, +1 inverse)
+template <class C>
+__device__ __forceinline__ void dft(C *v, int R, int sgn) {
+  using T = decltype(v[0].x);
+  if (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if (R == 4) {
+    C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    C b0 = cadd(v[1], v[3]), b1 = cmuli(csub(v[1], v[3]), sgn);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+  } else if (R == 3) {
+    const T c1 = T(-0.5), s1 = T(0.86602540378443864676) * T(sgn);
+    C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    C m1;
+    m1.x = v[0].x + c1 * t1.x;
+    m1.y = v[0].y + c1 * t1.y;
+    C m2;
+    m2.x = -s1 * t2.y;
+    m2.y = s1 * t2.x;
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m1, m2);
+    v[2] = csub(m1, m2);
+  } else if (R == 5) {
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(0.95105651629515357212) * T(sgn), s2 = T(0.58778525229247312917) * T(sgn);
+    C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    C a1, a2, b1, b2;
+    a1.x = v[0].x + c1 * t1.x + c2 * t2.x;
+    a1.y = v[0].y + c1 * t1.y + c2 * t2.y;
+    a2.x = v[0].x + c2 * t1.x + c1 * t2.x;
+    a2.y = v[0].y + c2 * t1.y + c1 * t2.y;
+    // b = i*(s1 t3 + s2 t4) and i*(s2 t3 - s1 t4)
+    b1.x = -(s1 * t3.y + s2 * t4.y);
+    b1.y = s1 * t3.x + s2 * t4.x;
+    b2.x = -(s2 * t3.y - s1 * t4.y);
+    b2.y = s2 * t3.x - s1 * t4.x;
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+}
+
+struct FftPlan {
+  int n;
+  int nrad;
+  int rad[24];
+};
+
+// Batched Stockham FFT of nl lines of length p.n held in LDS (line stride n).
+// Result ends in *pa (the buffers are swapped pass by pass).
+template <class C>
+__device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *__restrict__ tw,
+                        int sgn) {
+  const int n = p.n;
+  int Ns = 1;
+  for (int s = 0; s < p.nrad; s++) {
+    const int R = p.rad[s];
+    const int nb = n / R;
+    const int tstride = n / (Ns * R);
+    for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
+      const int line = t / nb, j = t - line * nb;
+      const C *src = pa + line * n;
+      C *dst = pb + line * n;
+      const int k = j % Ns;
+      C v[5];
+      for (int q = 0; q < R; q++) {
+        C x = src[j + q * nb];
+        if (q > 0) {
+          C w = tw[(q * k * tstride) % n];
+          if (sgn > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[q] = x;
+      }
+      dft(v, R, sgn);
+      const int idxD = (j / Ns) * Ns * R + k;
+      for (int q = 0; q < R; q++) dst[idxD + q * Ns] = v[q];
+    }
+    __syncthreads();
+    C *t = pa;
+    pa = pb;
+    pb = t;
+    Ns *= R;
+  }
+}
+
+__device__ __forceinline__ double grid_in(const double *g, size_t i) { return g[i]; }
+__device__ __forceinline__ float grid_in(const float *g, size_t i) { return g[i]; }
+__device__ __forceinline__ float grid_in_fx(const unsigned long long *g, size_t i) {
+  return (float)((double)(long long)g[i] * PGK_GRID_FX_INV);
+}
+
+// B-spline weights of order p at w (fill_bspline, fft_pme_init.F90:216-260)
+template <class R>
+__device__ __forceinline__ void bspline(R w, int order, R *A, R *D) {
+  // A, D are 1-based views of length order
+  A[order] = R(0);
+  A[2] = w;
+  A[1] = R(1) - w;
+  for (int k = 3; k <= order - 1; k++) {
+    R div = R(1) / R(k - 1);
+    A[k] = div * w * A[k - 1];
+    for (int j = 1; j <= k - 2; j++)
+      A[k - j] = div * ((w + R(j)) * A[k - j - 1] + (R(k - j) - w) * A[k - j]);
+    A[1] = div * (R(1) - w) * A[1];
+  }
+  D[1] = -A[1];
+  for (int j = 2; j <= order; j++) D[j] = A[j - 1] - A[j];
+  R div = R(1) / R(order - 1);
+  A[order] = div * w * A[order - 1];
+  for (int j = 1; j <= order - 2; j++)
+    A[order - j] = div * ((w + R(j)) * A[order - j - 1] + (R(order - j) - w) * A[order - j]);
+  A[1] = div * (R(1) - w) * A[1];
+}
+
+struct PmeGeom {
+  int K[3];
+  int H;            // K1/2 + 1
+  double recip[3];
+  int order;
+};
+
+template <class R>
+__device__ __forceinline__ void atom_frac(const double *pos, int i, const PmeGeom &g, int *i0,
+                                          R (*th)[PGK_MAXORD + 1], R (*dth)[PGK_MAXORD + 1]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double w = pos[3 * i + d] * g.recip[d];
+    w = w - floor(w);
+    double fr = (double)g.K[d] * w;
+    // w - floor(w) may round to 1.0 and non-finite input must not index the
+    // grid: keep 0 <= int(fr) < K
+    if (!(fr >= 0.0 && fr < (double)g.K[d])) fr = (fr >= (double)g.K[d]) ? (double)g.K[d] - 1.0 : 0.0;
+    int ifr = (int)fr;
+    R ww = (R)(fr - (double)ifr);
+    bspline<R>(ww, g.order, th[d], dth[d]);
+    i0[d] = ifr - g.order;   // first index (before +1) as fill_charge_grid
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict__ pos,
+                                                const double *__restrict__ q, PmeGeom g,
+                                                void *grid) {
+  using R = typename Prec<PREC>::real;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
+  int i0[3];
+  atom_frac<R>(pos, i, g, i0, th, dth);
+  const R qi = (R)q[i];
+  const int p = g.order;
+  for (int a3 = 1; a3 <= p; a3++) {
+    int k = i0[2] + a3;
+    if (k < 0) k += g.K[2];
+    R p1 = th[2][a3] * qi;
+    for (int a2 = 1; a2 <= p; a2++) {
+      int j = i0[1] + a2;
+      if (j < 0) j += g.K[1];
+      R pr = th[1][a2] * p1;
+      size_t row = ((size_t)k * g.K[1] + j) * g.K[0];
+      for (int a1 = 1; a1 <= p; a1++) {
+        int ii = i0[0] + a1;
+        if (ii < 0) ii += g.K[0];
+        R v = th[0][a1] * pr;
+        if (PREC == PGK_PREC_FP64) {
+          atomicAdd(reinterpret_cast<double *>(grid) + row + ii, (double)v);
+        } else if (PREC == PGK_PREC_MIXED) {
+          long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
+          atomicAdd(reinterpret_cast<unsigned long long *>(grid) + row + ii, (unsigned long long)x);
+        } else {
+          atomicAdd(reinterpret_cast<float *>(grid) + row + ii, (float)v);
+        }
+      }
+    }
+  }
+}
+
+// x pass, forward: pairs of real rows -> half spectra.  One block handles NL
+// complex lines (2*NL rows).
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid, int nrow,
+                                                    int NL, FftPlan p, const void *twv, int H) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n;
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + NL * n;
+  const C *tw = reinterpret_cast<const C *>(twv);
+  const int row0 = blockIdx.x * 2 * NL;
+  for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
+    int l = t / n, k = t - l * n;
+    int ra = row0 + 2 * l, rb = ra + 1;
+    C z;
+    if (PREC == PGK_PREC_MIXED) {
+      const unsigned long long *gi = reinterpret_cast<const unsigned long long *>(grid);
+      z.x = (ra < nrow) ? grid_in_fx(gi, (size_t)ra * n + k) : R(0);
+      z.y = (rb < nrow) ? grid_in_fx(gi, (size_t)rb * n + k) : R(0);
+    } else {
+      const R *gr = reinterpret_cast<const R *>(grid);
+      z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
+      z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
+    }
+    pa[t] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, NL, p, tw, -1);
+  C *cg = reinterpret_cast<C *>(cgrid);
+  for (int t = threadIdx.x; t < NL * H; t += blockDim.x) {
+    int l = t / H, k = t - l * H;
+    int ra = row0 + 2 * l, rb = ra + 1;
+    C zk = pa[l * n + k];
+    C zn = pa[l * n + ((n - k) % n)];
+    // A = (Z_k + conj Z_{n-k})/2,  B = (Z_k - conj Z_{n-k})/(2i)
+    C A, B;
+    A.x = R(0.5) * (zk.x + zn.x);
+    A.y = R(0.5) * (zk.y - zn.y);
+    B.x = R(0.5) * (zk.y + zn.y);
+    B.y = R(0.5) * (zn.x - zk.x);
+    if (ra < nrow) cg[(size_t)ra * H + k] = A;
+    if (rb < nrow) cg[(size_t)rb * H + k] = B;
+  }
+}
+
+// x pass, backward: half spectra of two rows -> two real rows (C2R)
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_x_bwd(const void *cgrid, void *grid, int nrow,
+                                                    int NL, FftPlan p, const void *twv, int H) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n;
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + NL * n;
+  const C *tw = reinterpret_cast<const C *>(twv);
+  const C *cg = reinterpret_cast<const C *>(cgrid);
+  const int row0 = blockIdx.x * 2 * NL;
+  for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
+    int l = t / n, k = t - l * n;
+    int ra = row0 + 2 * l, rb = ra + 1;
+    C A, B;
+    A.x = A.y = B.x = B.y = R(0);
+    int kk = (k < H) ? k : n - k;
+    bool conj = (k >= H);
+    if (ra < nrow) A = cg[(size_t)ra * H + kk];
+    if (rb < nrow) B = cg[(size_t)rb * H + kk];
+    if (conj) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    // Z = A + i B
+    C z;
+    z.x = A.x - B.y;
+    z.y = A.y + B.x;
+    pa[t] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, NL, p, tw, +1);
+  R *gr = reinterpret_cast<R *>(grid);
+  for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
+    int l = t / n, k = t - l * n;
+    int ra = row0 + 2 * l, rb = ra + 1;
+    C z = pa[t];
+    if (ra < nrow) gr[(size_t)ra * n + k] = z.x;
+    if (rb < nrow) gr[(size_t)rb * n + k] = z.y;
+  }
+}
+
+// y pass (both directions): lines along k2 for a tile of TW consecutive h at one k3
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_y(void *cgrid, int H, int K2, int TW, FftPlan p,
+                                                const void *twv, int sgn) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n;   // == K2
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + TW * n;
+  const C *tw = reinterpret_cast<const C *>(twv);
+  C *cg = reinterpret_cast<C *>(cgrid);
+  const int ntile = (H + TW - 1) / TW;
+  const int k3 = blockIdx.x / ntile;
+  const int h0 = (blockIdx.x - k3 * ntile) * TW;
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int k2 = t / TW, l = t - k2 * TW;
+    C z;
+    z.x = z.y = R(0);
+    if (h0 + l < H) z = cg[((size_t)k3 * K2 + k2) * H + h0 + l];
+    pa[l * n + k2] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, p, tw, sgn);
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int k2 = t / TW, l = t - k2 * TW;
+    if (h0 + l < H) cg[((size_t)k3 * K2 + k2) * H + h0 + l] = pa[l * n + k2];
+  }
+}
+
+struct ConvArgs {
+  const double *bsp;    // K1 | K2 | K3 moduli
+  double recip[3];
+  double fac, rk2, pivol, netq;
+  int K[3], H;
+  int ifnetq;
+};
+
+// z pass with the influence-function convolution between the forward and the
+// backward transform (scalar_sum, fft_pme.F90:544-690)
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan p,
+                                                     const void *twv, ConvArgs ca,
+                                                     double *red) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n;   // == K3
+  const int H = ca.H, K2 = ca.K[1];
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + TW * n;
+  const C *tw = reinterpret_cast<const C *>(twv);
+  C *cg = reinterpret_cast<C *>(cgrid);
+  const int ntile = (H + TW - 1) / TW;
+  const int k2 = blockIdx.x / ntile;
+  const int h0 = (blockIdx.x - k2 * ntile) * TW;
+  const size_t zstride = (size_t)K2 * H;
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int k3 = t / TW, l = t - k3 * TW;
+    C z;
+    z.x = z.y = R(0);
+    if (h0 + l < H) z = cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l];
+    pa[l * n + k3] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, p, tw, -1);
+  // convolution, energy and virial
+  double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // E, vsca, xx yy zz xy xz yz
+  const int K1 = ca.K[0], K3 = ca.K[2];
+  const int nf1 = (K1 + 1) / 2, nf2 = (K2 + 1) / 2, nf3 = (K3 + 1) / 2;
+  const int m2 = (k2 < nf2) ? k2 : k2 - K2;
+  const double mh2 = ca.recip[1] * m2;
+  const double s2 = ((K2 & 1) == 0 && k2 == K2 / 2) ? 1.0 : -1.0;
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int l = t / n, k3 = t - l * n;
+    int h = h0 + l;
+    if (h >= H) continue;
+    if (h == 0 && k2 == 0 && k3 == 0) continue;   // m = 0 left untouched
+    int m1 = (h < nf1) ? h : h - K1;
+    int m3 = (k3 < nf3) ? k3 : k3 - K3;
+    double mh1 = ca.recip[0] * m1, mh3 = ca.recip[2] * m3;
+    double msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
+    C z = pa[t];
+    double eterm = 0.0;
+    if (msq <= ca.rk2) {
+      double denom = ca.pivol * ca.bsp[h] * ca.bsp[K1 + k2] * ca.bsp[K1 + K2 + k3] * msq;
+      double ex = exp(-ca.fac * msq);
+      eterm = ex / denom;
+      double vterm = 2.0 * (ca.fac * msq + 1.0) / msq;
+      double struc2 = (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+      bool single = (h == 0) || ((K1 & 1) == 0 && h == K1 / 2);
+      double w = single ? 1.0 : 2.0;
+      double s3 = ((K3 & 1) == 0 && k3 == K3 / 2) ? 1.0 : -1.0;
+      double w12 = single ? 1.0 : 1.0 - s2;        // s1 = -1
+      double w13 = single ? 1.0 : 1.0 - s3;
+      double w23 = single ? 1.0 : 1.0 + s2 * s3;
+      double wgt = eterm * struc2;
+      if (ca.ifnetq) {
+        double e2 = ex * ca.netq / (ca.pivol * msq);
+        ev[1] += w * (e2 * (-1.0 + 2.0 * ca.fac * msq));
+        wgt = wgt - e2;
+      }
+      ev[0] += w * eterm * struc2;
+      ev[2] += w * wgt * (vterm * mh1 * mh1 - 1.0);
+      ev[3] += w * wgt * (vterm * mh2 * mh2 - 1.0);
+      ev[4] += w * wgt * (vterm * mh3 * mh3 - 1.0);
+      ev[5] += w12 * wgt * (vterm * mh1 * mh2);
+      ev[6] += w13 * wgt * (vterm * mh1 * mh3);
+      ev[7] += w23 * wgt * (vterm * mh2 * mh3);
+    }
+    z.x = (R)(eterm * (double)z.x);
+    z.y = (R)(eterm * (double)z.y);
+    pa[t] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, p, tw, +1);
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int k3 = t / TW, l = t - k3 * TW;
+    if (h0 + l < H) cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l] = pa[l * n + k3];
+  }
+  block_record<8, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
+}
+
+template <int PREC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_gather(int n, const double *__restrict__ pos,
+                                                const double *__restrict__ q, PmeGeom g,
+                                                const void *gridv,
+                                                typename Prec<PREC>::acc *fat,
+                                                typename Prec<PREC>::acc *fat_c) {
+  using R = typename Prec<PREC>::real;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R *grid = reinterpret_cast<const R *>(gridv);
+  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
+  int i0[3];
+  atom_frac<R>(pos, i, g, i0, th, dth);
+  const int p = g.order;
+  R f1 = 0, f2 = 0, f3 = 0;
+  const R qi = (R)q[i];
+  const R K1 = (R)g.K[0], K2 = (R)g.K[1], K3 = (R)g.K[2];
+  for (int a3 = 1; a3 <= p; a3++) {
+    int k = i0[2] + a3;
+    if (k < 0) k += g.K[2];
+    for (int a2 = 1; a2 <= p; a2++) {
+      int j = i0[1] + a2;
+      if (j < 0) j += g.K[1];
+      size_t row = ((size_t)k * g.K[1] + j) * g.K[0];
+      for (int a1 = 1; a1 <= p; a1++) {
+        int ii = i0[0] + a1;
+        if (ii < 0) ii += g.K[0];
+        R term = qi * grid[row + ii];
+        f1 = f1 - K1 * term * dth[0][a1] * th[1][a2] * th[2][a3];
+        f2 = f2 - K2 * term * th[0][a1] * dth[1][a2] * th[2][a3];
+        f3 = f3 - K3 * term * th[0][a1] * th[1][a2] * dth[2][a3];
+      }
+    }
+  }
+  double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
+  acc_add(&fat[3 * i + 0], fx);
+  acc_add(&fat[3 * i + 1], fy);
+  acc_add(&fat[3 * i + 2], fz);
+  if (SPLIT) {
+    acc_add(&fat_c[3 * i + 0], fx);
+    acc_add(&fat_c[3 * i + 1], fy);
+    acc_add(&fat_c[3 * i + 2], fz);
+  }
+}
+
+FftPlan make_plan(int n) {
+  FftPlan p;
+  p.n = n;
+  p.nrad = 0;
+  int m = n;
+  while (m % 4 == 0) { p.rad[p.nrad++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.rad[p.nrad++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.nrad++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.rad[p.nrad++] = 5; m /= 5; }
+  if (m != 1) p.nrad = -1;
+  return p;
+}
+
+int lines_for(int n, size_t csize, int maxl) {
+  int nl = maxl;
+  while (nl > 1 && (size_t)2 * nl * n * csize > 65536) nl >>= 1;
+  return nl;
+}
+
+template <int PREC>
+int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  using A = typename Prec<PREC>::acc;
+  hipStream_t st = c->stream;
+  const int K1 = c->nfft[0], K2 = c->nfft[1], K3 = c->nfft[2];
+  const int H = K1 / 2 + 1;
+  PmeGeom g;
+  for (int d = 0; d < 3; d++) {
+    g.K[d] = c->nfft[d];
+    g.recip[d] = 1.0 / c->box[d];
+  }
+  g.H = H;
+  g.order = c->order;
+  size_t G = (size_t)K1 * K2 * K3;
+  size_t gel = (PREC == PGK_PREC_FP32) ? 4 : 8;
+  pgk_prof_begin(c, PGK_PROF_SPREAD);
+  HIP_CHECK(hipMemsetAsync(c->d_grid, 0, G * gel, st));
+  const int n = c->natom;
+  k_spread<PREC><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g, c->d_grid);
+  HIP_CHECK(hipGetLastError());
+  pgk_prof_end(c, PGK_PROF_SPREAD);
+  pgk_prof_begin(c, PGK_PROF_FFT);
+  const void *tw = (PREC == PGK_PREC_FP64) ? (const void *)c->d_tw : (const void *)c->d_twf;
+  size_t csz = sizeof(C);
+  const C *twc = reinterpret_cast<const C *>(tw);
+  FftPlan p1 = make_plan(K1), p2 = make_plan(K2), p3 = make_plan(K3);
+  int nrow = K2 * K3;
+  int nlx = lines_for(K1, csz, 16);
+  int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
+  k_fft_x_fwd<PREC><<<bx, 256, 2 * nlx * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow, nlx, p1,
+                                                           twc + c->tw_off[0], H);
+  HIP_CHECK(hipGetLastError());
+  int twy = lines_for(K2, csz, 16);
+  int by = K3 * ((H + twy - 1) / twy);
+  k_fft_y<PREC><<<by, 256, 2 * twy * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
+                                                       twc + c->tw_off[1], -1);
+  HIP_CHECK(hipGetLastError());
+  ConvArgs ca;
+  ca.bsp = c->d_bsp;
+  for (int d = 0; d < 3; d++) {
+    ca.recip[d] = 1.0 / c->box[d];
+    ca.K[d] = c->nfft[d];
+  }
+  double pi = 3.14159265358979323846;
+  ca.fac = pi * pi / (c->alpha * c->alpha);
+  ca.rk2 = c->rkcut * c->rkcut;
+  ca.pivol = pi * c->box[0] * c->box[1] * c->box[2];
+  ca.netq = c->netchrgsq;
+  ca.H = H;
+  ca.ifnetq = c->ifnetq;
+  int twz = lines_for(K3, csz, 16);
+  int bz = K2 * ((H + twz - 1) / twz);
+  *off = c->red_used;
+  *nrec = bz;
+  double *red = c->d_red + (size_t)c->red_used * PGK_REC;
+  c->red_used += bz;
+  k_fft_z_conv<PREC><<<bz, 256, 2 * twz * K3 * csz, st>>>(c->d_cgrid, twz, p3,
+                                                            twc + c->tw_off[2], ca, red);
+  HIP_CHECK(hipGetLastError());
+  k_fft_y<PREC><<<by, 256, 2 * twy * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
+                                                       twc + c->tw_off[1], +1);
+  // potential grid back into the real grid buffer (compute type)
+  k_fft_x_bwd<PREC><<<bx, 256, 2 * nlx * K1 * csz, st>>>(c->d_cgrid, c->d_grid, nrow, nlx, p1,
+                                                           twc + c->tw_off[0], H);
+  HIP_CHECK(hipGetLastError());
+  pgk_prof_end(c, PGK_PROF_FFT);
+  pgk_prof_begin(c, PGK_PROF_GATHER);
+  A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
+  if (split)
+    k_gather<PREC, true><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g,
+                                                                  c->d_grid, fat, fatc);
+  else
+    k_gather<PREC, false><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g,
+                                                                   c->d_grid, fat, fatc);
+  HIP_CHECK(hipGetLastError());
+  pgk_prof_end(c, PGK_PROF_GATHER);
+  return PGK_OK;
+}
+
+}  // namespace
+
+int pgk_pme_prepare(pgk_ctx *c) {
+  for (int d = 0; d < 3; d++) {
+    FftPlan p = make_plan(c->nfft[d]);
+    if (p.nrad < 0) {
+      pgk_set_error("SPME grid sizes must be products of 2, 3 and 5");
+      return PGK_EINVAL;
+    }
+    if (p.nrad > 24) {
+      pgk_set_error("SPME grid size too large");
+      return PGK_EINVAL;
+    }
+  }
+  if (c->order < 3 || c->order > PGK_MAXORD) {
+    pgk_set_error("pme_order must be in 3..10");
+    return PGK_EINVAL;
+  }
+  return PGK_OK;
+}
+
+int pgk_pme_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_pme<PGK_PREC_FP64>(c, split, off, nrec);
+    case PGK_PREC_MIXED: return launch_pme<PGK_PREC_MIXED>(c, split, off, nrec);
+    default: return launch_pme<PGK_PREC_FP32>(c, split, off, nrec);
+  }
+}

@@ -1,0 +1,939 @@
+// pgk_api.hip - C ABI of libpgkforce.so (declared in include/pgk_force.h).
+//
+// Host-side set-up restates the reference's one-time preparation:
+//   spline_interp_init / cal_spl_coeff  spline_interp.F90:14-190
+//   fft_pme_init (moduli, self energy)  fft_pme_init.F90:14-182, 263-289
+//   erf_corr_cutoff (rkcut)             erf_corr_cutoff.F90:28-62
+//   netchrgsq                           prepmd.F90:244-249
+// and pgk_calforce orchestrates the device kernels in calforce's order with
+// its output semantics (calforce.F90:307-1068).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "pgk_device.h"
+
+static thread_local std::string g_err;
+void pgk_set_error(const std::string &msg) { g_err = msg; }
+
+void pgk_prof_begin(pgk_ctx *c, int k) {
+  if (!c->prof) return;
+  if (!c->ev[2 * k]) {
+    (void)hipEventCreate(&c->ev[2 * k]);
+    (void)hipEventCreate(&c->ev[2 * k + 1]);
+  }
+  (void)hipEventRecord(c->ev[2 * k], c->stream);
+}
+void pgk_prof_end(pgk_ctx *c, int k) {
+  if (!c->prof) return;
+  (void)hipEventRecord(c->ev[2 * k + 1], c->stream);
+  c->ev_used[k] = true;
+}
+// call after the stream has been synchronised
+void pgk_prof_collect(pgk_ctx *c) {
+  if (!c->prof) return;
+  for (int k = 0; k < PGK_NPROF; k++)
+    if (c->ev_used[k]) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess) {
+        c->prof_ms[k] += ms;
+        c->prof_calls[k] += 1;
+      }
+      c->ev_used[k] = false;
+    }
+}
+
+extern int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off,
+                              int *seg_n, int *seg_kind, int *nseg);
+
+namespace {
+
+enum {
+  SEG_BOND = 1, SEG_ANGL = 2, SEG_TORS = 3, SEG_TORSRB = 4, SEG_14 = 5,
+  SEG_PAIR = 10, SEG_EXCL = 11, SEG_PME = 12
+};
+
+// ---- host set-up mirrors of the reference's initialisation -----------------
+void spline_coeff(int n, double h, const double *x, const double *y, double *b, double *c,
+                  double *d) {
+  std::vector<double> al(n + 1), l(n + 1), mu(n + 1), z(n + 1);
+  for (int i = 2; i <= n - 1; i++)
+    al[i] = 3.0 / h * ((y[i] - y[i - 1]) - (y[i - 1] - y[i - 2]));
+  l[1] = 1.0;
+  mu[1] = 0.0;
+  z[1] = 0.0;
+  for (int i = 2; i <= n - 1; i++) {
+    l[i] = 2.0 * (x[i] - x[i - 2]) - h * mu[i - 1];
+    mu[i] = h / l[i];
+    z[i] = (al[i] - h * z[i - 1]) / l[i];
+  }
+  c[n - 1] = 0.0;
+  for (int i = n - 1; i >= 1; i--) {
+    c[i - 1] = z[i] - mu[i] * c[i];
+    b[i - 1] = (y[i] - y[i - 1]) / h - h * (c[i] + 2.0 * c[i - 1]) / 3.0;
+    d[i - 1] = (c[i] - c[i - 1]) / (3.0 * h);
+  }
+}
+
+void build_splines(pgk_ctx *c, std::vector<double> &real8, std::vector<double> &excs8) {
+  const int n = c->nspl;
+  const double pie = 2.0 / sqrt(acos(-1.0)) * c->alpha;
+  c->spl_int = (c->rrcut + 2.0) / (double)n;
+  std::vector<double> x(n), tb[4], co[12];
+  for (int k = 0; k < 4; k++) tb[k].assign(n, 0.0);
+  for (int k = 0; k < 12; k++) co[k].assign(n, 0.0);
+  for (int i = 0; i < n; i++) x[i] = (double)i * c->spl_int;
+  tb[0][0] = tb[1][0] = tb[2][0] = tb[3][0] = 1.0;
+  for (int i = 1; i < n; i++) {
+    double r = x[i], inv = 1.0 / r, arg = c->alpha * r;
+    tb[0][i] = inv * (pie * exp(-arg * arg) * inv + erfc(arg) * inv * inv);
+    tb[1][i] = erfc(arg) * inv;
+    tb[2][i] = inv * (-pie * exp(-arg * arg) * inv + (1.0 - erfc(arg)) * inv * inv);
+    tb[3][i] = (1.0 - erfc(arg)) * inv;
+  }
+  for (int t = 0; t < 4; t++)
+    spline_coeff(n, c->spl_int, x.data(), tb[t].data(), co[3 * t].data(), co[3 * t + 1].data(),
+                 co[3 * t + 2].data());
+  real8.assign((size_t)8 * n, 0.0);
+  excs8.assign((size_t)8 * n, 0.0);
+  for (int i = 0; i < n; i++) {
+    for (int h = 0; h < 2; h++) {
+      int t = h;       // 0 real (force), 1 realpot
+      real8[8 * i + 4 * h + 0] = tb[t][i];
+      real8[8 * i + 4 * h + 1] = co[3 * t][i];
+      real8[8 * i + 4 * h + 2] = co[3 * t + 1][i];
+      real8[8 * i + 4 * h + 3] = co[3 * t + 2][i];
+      t = 2 + h;       // 2 excs (force), 3 excspot
+      excs8[8 * i + 4 * h + 0] = tb[t][i];
+      excs8[8 * i + 4 * h + 1] = co[3 * t][i];
+      excs8[8 * i + 4 * h + 2] = co[3 * t + 1][i];
+      excs8[8 * i + 4 * h + 3] = co[3 * t + 2][i];
+    }
+  }
+}
+
+void host_bspline(double w, int order, double *A, double *D) {   // 1-based views
+  A[order] = 0.0;
+  A[2] = w;
+  A[1] = 1.0 - w;
+  for (int k = 3; k <= order - 1; k++) {
+    double div = 1.0 / (double)(k - 1);
+    A[k] = div * w * A[k - 1];
+    for (int j = 1; j <= k - 2; j++)
+      A[k - j] = div * ((w + j) * A[k - j - 1] + (k - j - w) * A[k - j]);
+    A[1] = div * (1 - w) * A[1];
+  }
+  D[1] = -A[1];
+  for (int j = 2; j <= order; j++) D[j] = A[j - 1] - A[j];
+  double div = 1.0 / (double)(order - 1);
+  A[order] = div * w * A[order - 1];
+  for (int j = 1; j <= order - 2; j++)
+    A[order - j] = div * ((w + j) * A[order - j - 1] + (order - j - w) * A[order - j]);
+  A[1] = div * (1 - w) * A[1];
+}
+
+void dftmod(std::vector<double> &mod, const std::vector<double> &arr, int n) {
+  const double twopi = 2.0 * 3.14159265358979323846, tiny = 1.0e-7;
+  mod.assign(n, 0.0);
+  for (int k = 1; k <= n; k++) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 1; j <= n; j++) {
+      double arg = twopi * (k - 1) * (j - 1) / n;
+      s1 = s1 + arr[j - 1] * cos(arg);
+      s2 = s2 + arr[j - 1] * sin(arg);
+    }
+    mod[k - 1] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 1; k <= n; k++)
+    if (mod[k - 1] < tiny) {
+      double lo = (k >= 2) ? mod[k - 2] : mod[k - 1];
+      double hi = (k < n) ? mod[k] : mod[k - 1];
+      mod[k - 1] = 0.5 * (lo + hi);
+    }
+}
+
+template <class T>
+int upload(T **d, const std::vector<T> &h) {
+  size_t n = h.size() ? h.size() : 1;
+  if (*d) (void)hipFree(*d);
+  *d = nullptr;
+  HIP_CHECK(hipMalloc((void **)d, n * sizeof(T)));
+  if (!h.empty()) HIP_CHECK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PGK_OK;
+}
+
+template <class T>
+std::vector<T> copyv(const T *p, size_t n) {
+  return (p && n) ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+
+// ---- small kernels ---------------------------------------------------------
+template <class R4>
+__global__ void k_pack(int nslot, const int32_t *__restrict__ cl_atom,
+                       const double *__restrict__ pos, const double *__restrict__ q,
+                       const int32_t *__restrict__ type, R4 *__restrict__ xq,
+                       int32_t *__restrict__ ct) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  int a = cl_atom[s];
+  R4 v;
+  if (a >= 0) {
+    v.x = pos[3 * a];
+    v.y = pos[3 * a + 1];
+    v.z = pos[3 * a + 2];
+    v.w = q[a];
+    ct[s] = type[a];
+  } else {
+    v.x = v.y = v.z = v.w = 0;
+    ct[s] = 0;
+  }
+  xq[s] = v;
+}
+
+template <class A>
+__global__ void k_combine(int n, const int32_t *__restrict__ slot_of, int use_cl,
+                          const A *__restrict__ fcl, const A *__restrict__ fat,
+                          const A *__restrict__ fclc, const A *__restrict__ fatc,
+                          double *__restrict__ force, double *__restrict__ fvc,
+                          double *__restrict__ fvl) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int s = use_cl ? slot_of[i] : -1;
+  for (int d = 0; d < 3; d++) {
+    double fp = (s >= 0) ? acc_load(&fcl[3 * s + d]) : 0.0;
+    force[3 * i + d] = fp + acc_load(&fat[3 * i + d]);
+    if (fvc) {
+      double fc = (s >= 0) ? acc_load(&fclc[3 * s + d]) : 0.0;
+      fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]);
+      fvl[3 * i + d] = fp - fc;
+    }
+  }
+}
+
+// deterministic segment sums: one block per segment, fixed-order tree
+__global__ __launch_bounds__(256) void k_reduce_segments(const double *__restrict__ red,
+                                                         const int *__restrict__ seg_off,
+                                                         const int *__restrict__ seg_n,
+                                                         double *__restrict__ out) {
+  __shared__ double s[256];
+  int sg = blockIdx.x;
+  const double *r = red + (size_t)seg_off[sg] * PGK_REC;
+  int n = seg_n[sg];
+  for (int f = 0; f < PGK_REC; f++) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v += r[(size_t)k * PGK_REC + f];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[sg * PGK_REC + f] = s[0];
+    __syncthreads();
+  }
+}
+
+int set_device_ctx(pgk_ctx *c) {
+  HIP_CHECK(hipSetDevice(c->device));
+  return PGK_OK;
+}
+
+void sym6_to9(const double *v6, double *t9, double s) {
+  // v6: xx yy zz xy xz yz  -> Fortran (a,n): idx a + 3n
+  t9[0] += s * v6[0];
+  t9[4] += s * v6[1];
+  t9[8] += s * v6[2];
+  t9[3] += s * v6[3];   // (x,y)
+  t9[1] += s * v6[3];   // (y,x)
+  t9[6] += s * v6[4];   // (x,z)
+  t9[2] += s * v6[4];
+  t9[7] += s * v6[5];   // (y,z)
+  t9[5] += s * v6[5];
+}
+
+int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom, bool split,
+                 double *d_force_out, double *d_fvc, double *d_fvl, double *ene, double *virt) {
+  if (!c->have_pos) {
+    pgk_set_error("pgk_calforce: positions not set");
+    return PGK_ESTATE;
+  }
+  const bool do_pair = (m->vdw == mts_flag);
+  if (do_pair && !c->have_list) {
+    pgk_set_error("pgk_calforce: neighbour list not built");
+    return PGK_ESTATE;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  const int n = c->natom;
+  const size_t ab = (c->precision == PGK_PREC_FP32) ? 4 : 8;
+  // capacity of the record buffer: every launch appends its blocks
+  int need = 16;
+  if (do_pair) need += pgk_grid_blocks(c->ncl, 4) + pgk_grid_blocks(c->nexclpair, 256) + 1;
+  {
+    int H = c->nfft[0] / 2 + 1;
+    need += c->nfft[1] * H + 16;
+  }
+  need += pgk_grid_blocks((int64_t)c->bond_type.size(), 256) +
+          pgk_grid_blocks((int64_t)c->angl_type.size(), 256) +
+          pgk_grid_blocks((int64_t)c->tors_type.size(), 256) +
+          pgk_grid_blocks((int64_t)c->rb_type.size(), 256) + pgk_grid_blocks(c->n14, 256) + 8;
+  if (need > c->red_cap) {
+    if (c->d_red) (void)hipFree(c->d_red);
+    c->d_red = nullptr;
+    HIP_CHECK(hipMalloc(&c->d_red, (size_t)need * PGK_REC * sizeof(double)));
+    c->red_cap = need;
+  }
+  c->red_used = 0;
+  int seg_off[16], seg_n[16], seg_kind[16], nseg = 0;
+
+  HIP_CHECK(hipMemsetAsync(c->d_fat, 0, (size_t)n * 3 * ab, st));
+  if (split) HIP_CHECK(hipMemsetAsync(c->d_fat_c, 0, (size_t)n * 3 * ab, st));
+  if (do_pair) {
+    size_t ns = (size_t)c->ncl * PGK_CL;
+    HIP_CHECK(hipMemsetAsync(c->d_fcl, 0, ns * 3 * ab, st));
+    if (split) HIP_CHECK(hipMemsetAsync(c->d_fcl_c, 0, ns * 3 * ab, st));
+    if (c->precision == PGK_PREC_FP64)
+      k_pack<double4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
+          (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (double4 *)c->d_xq, c->d_ct);
+    else
+      k_pack<float4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
+          (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (float4 *)c->d_xq, c->d_ct);
+    HIP_CHECK(hipGetLastError());
+    pgk_prof_begin(c, PGK_PROF_PAIRS);
+    if ((rc = pgk_pairs_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+    pgk_prof_end(c, PGK_PROF_PAIRS);
+    seg_kind[nseg++] = SEG_PAIR;
+    if (c->nexclpair > 0) {
+      pgk_prof_begin(c, PGK_PROF_EXCL);
+      if ((rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+      pgk_prof_end(c, PGK_PROF_EXCL);
+      seg_kind[nseg++] = SEG_EXCL;
+    }
+  }
+  if (m->ewk == mts_flag) {
+    if ((rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+    seg_kind[nseg++] = SEG_PME;
+  }
+  pgk_prof_begin(c, PGK_PROF_BONDED);
+  if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg))) return rc;
+  pgk_prof_end(c, PGK_PROF_BONDED);
+  // forces
+  double *fout = d_force_out ? d_force_out : c->d_force;
+  {
+    int g = pgk_grid_blocks(n, 256);
+    double *fvc = split ? d_fvc : nullptr, *fvl = split ? d_fvl : nullptr;
+    switch (c->precision) {
+      case PGK_PREC_FP64:
+        k_combine<double><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
+                                             (const double *)c->d_fcl, (const double *)c->d_fat,
+                                             (const double *)c->d_fcl_c,
+                                             (const double *)c->d_fat_c, fout, fvc, fvl);
+        break;
+      case PGK_PREC_MIXED:
+        k_combine<unsigned long long><<<g, 256, 0, st>>>(
+            n, c->d_slot_of, do_pair ? 1 : 0, (const unsigned long long *)c->d_fcl,
+            (const unsigned long long *)c->d_fat, (const unsigned long long *)c->d_fcl_c,
+            (const unsigned long long *)c->d_fat_c, fout, fvc, fvl);
+        break;
+      default:
+        k_combine<float><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
+                                            (const float *)c->d_fcl, (const float *)c->d_fat,
+                                            (const float *)c->d_fcl_c, (const float *)c->d_fat_c,
+                                            fout, fvc, fvl);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+  // energies
+  double segsum[16][PGK_REC];
+  if (nseg > 0) {
+    int *dseg = reinterpret_cast<int *>(c->d_sum + 512);
+    int *hseg = reinterpret_cast<int *>(c->h_sum + 512);   // pinned staging
+    for (int k = 0; k < nseg; k++) {
+      hseg[k] = seg_off[k];
+      hseg[16 + k] = seg_n[k];
+    }
+    HIP_CHECK(hipMemcpyAsync(dseg, hseg, 32 * sizeof(int), hipMemcpyHostToDevice, st));
+    k_reduce_segments<<<nseg, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
+                             hipMemcpyDeviceToHost, st));
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+  pgk_prof_collect(c);
+  for (int k = 0; k < nseg; k++) memcpy(segsum[k], c->h_sum + k * PGK_REC, sizeof(segsum[k]));
+
+  double E[PGK_NENE];
+  double T[PGK_NTENS][9];
+  memcpy(E, ene, sizeof(E));   // energies not evaluated keep the caller's values
+  memset(T, 0, sizeof(T));
+  E[PGK_V_COUL] = E[PGK_V_LJ] = 0.0;
+  E[PGK_V_BOND] = E[PGK_V_ANGL] = E[PGK_V_TORS] = E[PGK_V_14] = 0.0;
+  bool tors_done = false;
+  for (int k = 0; k < nseg; k++) {
+    const double *s = segsum[k];
+    switch (seg_kind[k]) {
+      case SEG_PAIR:
+        E[PGK_E_ELC] = s[0];
+        E[PGK_E_VDW] = s[1];
+        E[PGK_V_COUL] += s[0];
+        sym6_to9(s + 2, T[PGK_T_COUL], 1.0);
+        sym6_to9(s + 8, T[PGK_T_LJ], 1.0);
+        E[PGK_V_LJ] = s[8] + s[9] + s[10];
+        break;
+      case SEG_EXCL:
+        E[PGK_E_ELC] += s[0];
+        E[PGK_V_COUL] += s[0];
+        sym6_to9(s + 1, T[PGK_T_COUL], 1.0);
+        break;
+      case SEG_PME: {
+        E[PGK_E_EWK] = 0.5 * s[0];
+        double vsca = 0.5 * s[1];
+        E[PGK_V_COUL] += E[PGK_E_EWK] + vsca;
+        sym6_to9(s + 2, T[PGK_T_COUL], -0.5);
+        break;
+      }
+      case SEG_BOND:
+        E[PGK_E_BOND] = s[0];
+        E[PGK_V_BOND] += s[1];
+        for (int q = 0; q < 9; q++) T[PGK_T_BOND][q] += s[2 + q];
+        break;
+      case SEG_ANGL:
+        E[PGK_E_ANGL] = s[0];
+        E[PGK_V_ANGL] += s[1];
+        for (int q = 0; q < 9; q++) T[PGK_T_ANGL][q] += s[2 + q];
+        break;
+      case SEG_TORS:
+      case SEG_TORSRB:
+        E[seg_kind[k] == SEG_TORS ? PGK_E_TORS : PGK_E_TORSRB] = s[0];
+        E[PGK_V_TORS] += s[1];
+        for (int q = 0; q < 9; q++) T[PGK_T_TORS][q] += s[2 + q];
+        tors_done = true;
+        break;
+      case SEG_14:
+        E[PGK_E_ELC14] = s[0];
+        E[PGK_E_VDW14] = s[1];
+        E[PGK_V_14] += s[2];
+        for (int q = 0; q < 9; q++) T[PGK_T_14][q] += s[3 + q];
+        break;
+    }
+  }
+  (void)tors_done;
+  // terms whose MTS level matches but that have no entries still report zero
+  if (m->bond == mts_flag && c->bond_type.empty()) E[PGK_E_BOND] = 0.0;
+  if (m->angl == mts_flag && c->angl_type.empty()) E[PGK_E_ANGL] = 0.0;
+  if (m->tors == mts_flag && c->tors_type.empty()) E[PGK_E_TORS] = 0.0;
+  if (m->torsrb == mts_flag && c->rb_type.empty()) E[PGK_E_TORSRB] = 0.0;
+  if (m->nonb14 == mts_flag && c->n14 == 0) E[PGK_E_ELC14] = E[PGK_E_VDW14] = 0.0;
+  if (do_pair && c->nexclpair == 0) { /* pot_elc already set from the pairs */ }
+  E[PGK_E_EWC] = c->pot_ewc;
+  if (mts_flag == PGK_MTS_LONG) {   // calforce.F90:1018-1044
+    double pi = acos(-1.0);
+    double a1 = -pi / (c->alpha * c->alpha * c->box[0] * c->box[1] * c->box[2]);
+    E[PGK_E_EWNQ] = 0.5 * a1 * c->netchrgsq;
+    if (!c->ifnetq) {
+      E[PGK_V_COUL] += a1 * c->netchrgsq;
+      for (int d = 0; d < 3; d++) T[PGK_T_COUL][4 * d] += 0.5 * a1 * c->netchrgsq;
+    } else {
+      E[PGK_V_COUL] -= 0.5 * a1 * c->netchrgsq;
+    }
+  }
+  E[PGK_E_NONBON] = E[PGK_E_VDW] + E[PGK_E_ELC] + E[PGK_E_EWK] + E[PGK_E_EWC] + E[PGK_E_EWNQ] +
+                    E[PGK_E_VDW14] + E[PGK_E_ELC14];
+  E[PGK_E_TOT] = E[PGK_E_BOND] + E[PGK_E_ANGL] + E[PGK_E_TORS] + E[PGK_E_TORSRB] +
+                 E[PGK_E_NONBON];
+  (void)preatom;
+  memcpy(ene, E, sizeof(E));
+  if (virt) memcpy(virt, T, sizeof(T));
+  return PGK_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+extern "C" {
+
+int32_t pgk_abi_version(void) { return PGK_ABI_VERSION; }
+const char *pgk_last_error(void) { return g_err.c_str(); }
+
+int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
+  if (!d || !out) {
+    pgk_set_error("pgk_create: null argument");
+    return PGK_EINVAL;
+  }
+  *out = nullptr;
+  if (d->natom <= 1 || d->ntype <= 0 || !d->charge || !d->atype || !d->molid || !d->lj_eps ||
+      !d->lj_rmin || !d->inttyp || !d->excl_index || !d->box) {
+    pgk_set_error("pgk_create: incomplete system description");
+    return PGK_EINVAL;
+  }
+  if (precision < PGK_PREC_FP64 || precision > PGK_PREC_FP32) {
+    pgk_set_error("pgk_create: unknown precision");
+    return PGK_EINVAL;
+  }
+  if (d->nspltbl < 4 || d->nfft1 < 4 || d->nfft2 < 4 || d->nfft3 < 4) {
+    pgk_set_error("pgk_create: spline table or SPME grid too small");
+    return PGK_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    pgk_set_error("pgk_create: no HIP device available");
+    return PGK_EDEVICE;
+  }
+  const int n = d->natom, nt = d->ntype;
+  for (int i = 0; i < n; i++)
+    if (d->atype[i] < 1 || d->atype[i] > nt) {
+      pgk_set_error("pgk_create: atom type out of range");
+      return PGK_EINVAL;
+    }
+  pgk_ctx *c = new pgk_ctx();
+  c->precision = precision;
+  (void)hipGetDevice(&c->device);
+  c->natom = n;
+  c->ntype = nt;
+  c->charge = copyv(d->charge, n);
+  c->atype = copyv(d->atype, n);
+  c->molid = copyv(d->molid, n);
+  c->lj_eps = copyv(d->lj_eps, (size_t)nt * nt);
+  c->lj_rmin = copyv(d->lj_rmin, (size_t)nt * nt);
+  c->inttyp = copyv(d->inttyp, (size_t)nt * nt);
+  c->excl_index = copyv(d->excl_index, (size_t)n + 1);
+  c->excl_list = copyv(d->excl_list, (size_t)d->nexcl);
+  c->bond_ij = copyv(d->bond_ij, (size_t)2 * d->nbond);
+  c->bond_type = copyv(d->bond_type, (size_t)d->nbond);
+  c->bond_k = copyv(d->bond_k, (size_t)d->nbondtyp);
+  c->bond_r0 = copyv(d->bond_r0, (size_t)d->nbondtyp);
+  c->angl_ijk = copyv(d->angl_ijk, (size_t)3 * d->nangl);
+  c->angl_type = copyv(d->angl_type, (size_t)d->nangl);
+  c->angl_k = copyv(d->angl_k, (size_t)d->nangltyp);
+  c->angl_t0 = copyv(d->angl_theta0, (size_t)d->nangltyp);
+  c->tors_ijkl = copyv(d->tors_ijkl, (size_t)4 * d->ntors);
+  c->tors_type = copyv(d->tors_type, (size_t)d->ntors);
+  c->tors_ring = copyv(d->tors_ring14, (size_t)d->ntors);
+  c->tors_divfac = copyv(d->tors_divfac, (size_t)d->ntorstyp);
+  c->tors_barhig = copyv(d->tors_barhig, (size_t)d->ntorstyp);
+  c->tors_phsang = copyv(d->tors_phsang, (size_t)d->ntorstyp);
+  c->tors_period = copyv(d->tors_period, (size_t)d->ntorstyp);
+  c->tors_dvdw = copyv(d->tors_divfac_vdw, (size_t)d->ntorstyp);
+  c->tors_delc = copyv(d->tors_divfac_elc, (size_t)d->ntorstyp);
+  c->tors_if14 = copyv(d->tors_if14, (size_t)d->ntorstyp);
+  c->rb_ijkl = copyv(d->torsrb_ijkl, (size_t)4 * d->ntorsrb);
+  c->rb_type = copyv(d->torsrb_type, (size_t)d->ntorsrb);
+  c->rb_ring = copyv(d->torsrb_ring14, (size_t)d->ntorsrb);
+  c->rb_c = copyv(d->torsrb_c, (size_t)6 * d->ntorsrbtyp);
+  c->rb_dvdw = copyv(d->torsrb_divfac_vdw, (size_t)d->ntorsrbtyp);
+  c->rb_delc = copyv(d->torsrb_divfac_elc, (size_t)d->ntorsrbtyp);
+  c->rb_if14 = copyv(d->torsrb_if14, (size_t)d->ntorsrbtyp);
+  c->alpha = d->alpha;
+  c->rrcut = d->rrcut;
+  c->rcut = d->rcut;
+  c->nspl = d->nspltbl;
+  c->nfft[0] = d->nfft1;
+  c->nfft[1] = d->nfft2;
+  c->nfft[2] = d->nfft3;
+  c->order = d->pme_order;
+  c->ifnetq = d->ifnetqcorrp;
+  for (int k = 0; k < 3; k++) c->box[k] = d->box[k];
+  // validate topology indices
+  auto bad = [&](const std::vector<int32_t> &v) {
+    for (int32_t x : v)
+      if (x < 1 || x > n) return true;
+    return false;
+  };
+  auto badt = [&](const std::vector<int32_t> &v, int ntyp) {
+    for (int32_t x : v)
+      if (x < 1 || x > ntyp) return true;
+    return false;
+  };
+  bool bad_csr = (int)c->excl_index.size() != n + 1 || c->excl_index[0] != 1 ||
+                 c->excl_index[n] - 1 != d->nexcl;
+  for (int i = 0; i < n && !bad_csr; i++)
+    if (c->excl_index[i + 1] < c->excl_index[i]) bad_csr = true;
+  for (int32_t x : c->excl_list)
+    if (x < 0 || x > n) bad_csr = true;
+  if (bad(c->bond_ij) || bad(c->angl_ijk) || bad(c->tors_ijkl) || bad(c->rb_ijkl) ||
+      badt(c->bond_type, d->nbondtyp) || badt(c->angl_type, d->nangltyp) ||
+      badt(c->tors_type, d->ntorstyp) || badt(c->rb_type, d->ntorsrbtyp) || bad_csr ||
+      (int)c->tors_ring.size() != d->ntors || (int)c->rb_ring.size() != d->ntorsrb) {
+    pgk_set_error("pgk_create: topology index, type or exclusion CSR out of range");
+    delete c;
+    return PGK_EINVAL;
+  }
+  for (int i = 0; i < n; i++)
+    if (c->molid[i] < 1) {
+      pgk_set_error("pgk_create: molecule index out of range");
+      delete c;
+      return PGK_EINVAL;
+    }
+  int rc = pgk_pme_prepare(c);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  // netchrgsq, self energy, moduli, rkcut
+  double qs = 0.0, q2 = 0.0;
+  for (int i = 0; i < n; i++) {
+    qs = qs + c->charge[i];
+    q2 = q2 + c->charge[i] * c->charge[i];
+  }
+  c->netchrgsq = qs * qs;
+  c->pot_ewc = -1.0 / sqrt(acos(-1.0)) * c->alpha * q2;
+  {
+    double A[PGK_MAXORD + 2], D[PGK_MAXORD + 2];
+    host_bspline(0.0, c->order, A, D);
+    int maxn = std::max(c->nfft[0], std::max(c->nfft[1], c->nfft[2]));
+    std::vector<double> arr(maxn + PGK_MAXORD + 2, 0.0);
+    for (int i = 2; i <= c->order + 1; i++) arr[i - 1] = A[i - 1];
+    for (int dd = 0; dd < 3; dd++) dftmod(c->bsp_mod[dd], arr, c->nfft[dd]);
+    double m1 = 0.5 * (c->nfft[0] / d->box[0]);
+    double rk = m1;
+    double m2 = 0.5 * (c->nfft[1] / d->box[1]);
+    if (m2 < m1) rk = m2;
+    double m3 = 0.5 * (c->nfft[2] / d->box[2]);
+    if (m3 < m2) rk = m3;
+    c->rkcut = 0.9 * rk;
+  }
+  hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    pgk_set_error(std::string("pgk_create: stream: ") + hipGetErrorString(he));
+    delete c;
+    return PGK_EDEVICE;
+  }
+  // ---- device uploads ----
+  std::vector<double> real8, excs8;
+  build_splines(c, real8, excs8);
+  std::vector<float> realf(real8.begin(), real8.end());
+  std::vector<double> ljp64((size_t)2 * nt * nt);
+  std::vector<float> ljp32((size_t)2 * nt * nt);
+  for (size_t k = 0; k < (size_t)nt * nt; k++) {
+    ljp64[2 * k] = c->lj_eps[k];
+    ljp64[2 * k + 1] = c->lj_rmin[k] * c->lj_rmin[k];
+    ljp32[2 * k] = (float)ljp64[2 * k];
+    ljp32[2 * k + 1] = (float)ljp64[2 * k + 1];
+  }
+  std::vector<int32_t> exclpair;
+  for (int i = 0; i < n; i++)
+    for (int k = c->excl_index[i] - 1; k < c->excl_index[i + 1] - 1; k++) {
+      int j = c->excl_list[k];
+      if (j > 0) {
+        exclpair.push_back(i);
+        exclpair.push_back(j - 1);
+      }
+    }
+  c->nexclpair = (int)(exclpair.size() / 2);
+  // bonded: 0-based indices, parameters packed
+  std::vector<int32_t> hb, ha, ht, hr, hp14;
+  std::vector<double> par, p14f;
+  c->off_bk = (int)par.size();
+  par.insert(par.end(), c->bond_k.begin(), c->bond_k.end());
+  c->off_br = (int)par.size();
+  par.insert(par.end(), c->bond_r0.begin(), c->bond_r0.end());
+  c->off_ak = (int)par.size();
+  par.insert(par.end(), c->angl_k.begin(), c->angl_k.end());
+  c->off_at = (int)par.size();
+  par.insert(par.end(), c->angl_t0.begin(), c->angl_t0.end());
+  c->off_tdiv = (int)par.size();
+  for (int32_t x : c->tors_divfac) par.push_back((double)x);
+  c->off_tb = (int)par.size();
+  par.insert(par.end(), c->tors_barhig.begin(), c->tors_barhig.end());
+  c->off_tp = (int)par.size();
+  par.insert(par.end(), c->tors_phsang.begin(), c->tors_phsang.end());
+  c->off_tper = (int)par.size();
+  par.insert(par.end(), c->tors_period.begin(), c->tors_period.end());
+  c->off_rbc = (int)par.size();
+  par.insert(par.end(), c->rb_c.begin(), c->rb_c.end());
+  for (size_t k = 0; k < c->bond_type.size(); k++) {
+    hb.push_back(c->bond_ij[2 * k] - 1);
+    hb.push_back(c->bond_ij[2 * k + 1] - 1);
+    hb.push_back(c->bond_type[k] - 1);
+  }
+  for (size_t k = 0; k < c->angl_type.size(); k++) {
+    for (int q = 0; q < 3; q++) ha.push_back(c->angl_ijk[3 * k + q] - 1);
+    ha.push_back(c->angl_type[k] - 1);
+  }
+  for (size_t k = 0; k < c->tors_type.size(); k++) {
+    int np = (int)lround(fabs(c->tors_period[c->tors_type[k] - 1]));
+    if (np < 1 || np > 6) {
+      pgk_set_error("pgk_create: torsion periodicity not supported (caltors.F90:414-435)");
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return PGK_EINVAL;
+    }
+    for (int q = 0; q < 4; q++) ht.push_back(c->tors_ijkl[4 * k + q] - 1);
+    ht.push_back(c->tors_type[k] - 1);
+  }
+  for (size_t k = 0; k < c->rb_type.size(); k++) {
+    for (int q = 0; q < 4; q++) hr.push_back(c->rb_ijkl[4 * k + q] - 1);
+    hr.push_back(c->rb_type[k] - 1);
+  }
+  // 1-4 pairs: periodic torsions first, then RB (calnon14.F90:274-509)
+  for (size_t k = 0; k < c->tors_type.size(); k++) {
+    int t = c->tors_type[k] - 1;
+    if (!c->tors_if14[t]) continue;
+    int i = c->tors_ijkl[4 * k], l = c->tors_ijkl[4 * k + 3];
+    if (i == l) continue;
+    hp14.push_back(i - 1);
+    hp14.push_back(l - 1);
+    p14f.push_back(1.0 / c->tors_dvdw[t] * c->tors_ring[k]);
+    p14f.push_back(1.0 / c->tors_delc[t] * c->tors_ring[k]);
+  }
+  for (size_t k = 0; k < c->rb_type.size(); k++) {
+    int t = c->rb_type[k] - 1;
+    if (!c->rb_if14[t]) continue;
+    int i = c->rb_ijkl[4 * k], l = c->rb_ijkl[4 * k + 3];
+    if (i == l) continue;
+    hp14.push_back(i - 1);
+    hp14.push_back(l - 1);
+    p14f.push_back(1.0 / c->rb_dvdw[t] * c->rb_ring[k]);
+    p14f.push_back(1.0 / c->rb_delc[t] * c->rb_ring[k]);
+  }
+  c->n14 = (int)(hp14.size() / 2);
+  // twiddles exp(-2 pi i k/n)
+  std::vector<double> tw;
+  std::vector<float> twf;
+  for (int dd = 0; dd < 3; dd++) {
+    c->tw_off[dd] = (int)(tw.size() / 2);
+    int nn = c->nfft[dd];
+    for (int k = 0; k < nn; k++) {
+      double a = 2.0 * 3.14159265358979323846 * (double)k / (double)nn;
+      tw.push_back(cos(a));
+      tw.push_back(-sin(a));
+    }
+  }
+  twf.assign(tw.begin(), tw.end());
+  std::vector<double> bsp;
+  for (int dd = 0; dd < 3; dd++) bsp.insert(bsp.end(), c->bsp_mod[dd].begin(), c->bsp_mod[dd].end());
+
+  if ((rc = upload(&c->d_q, c->charge)) || (rc = upload(&c->d_type, c->atype)) ||
+      (rc = upload(&c->d_molid, c->molid)) || (rc = upload(&c->d_eps, c->lj_eps)) ||
+      (rc = upload(&c->d_rmin, c->lj_rmin)) || (rc = upload(&c->d_inttyp, c->inttyp)) ||
+      (rc = upload(&c->d_excl_index, c->excl_index)) ||
+      (rc = upload(&c->d_excl_list, c->excl_list)) || (rc = upload(&c->d_exclpair, exclpair)) ||
+      (rc = upload(&c->d_spl, real8)) || (rc = upload(&c->d_splx, excs8)) ||
+      (rc = upload(&c->d_splf, realf)) || (rc = upload(&c->d_ljp64, ljp64)) ||
+      (rc = upload(&c->d_ljp32, ljp32)) || (rc = upload(&c->d_bsp, bsp)) ||
+      (rc = upload(&c->d_bond, hb)) || (rc = upload(&c->d_angl, ha)) ||
+      (rc = upload(&c->d_tors, ht)) || (rc = upload(&c->d_rb, hr)) ||
+      (rc = upload(&c->d_p14, hp14)) || (rc = upload(&c->d_p14f, p14f)) ||
+      (rc = upload(&c->d_bpar, par)) || (rc = upload(&c->d_tw, tw)) ||
+      (rc = upload(&c->d_twf, twf))) {
+    pgk_destroy(c);
+    return rc;
+  }
+  size_t ab = (precision == PGK_PREC_FP32) ? 4 : 8;
+  size_t G = (size_t)c->nfft[0] * c->nfft[1] * c->nfft[2];
+  size_t Hc = (size_t)(c->nfft[0] / 2 + 1) * c->nfft[1] * c->nfft[2];
+  size_t csz = (precision == PGK_PREC_FP64) ? 16 : 8;
+  he = hipMalloc(&c->d_pos, sizeof(double) * 3 * n);
+  if (he == hipSuccess) he = hipMalloc(&c->d_fat, ab * 3 * n);
+  if (he == hipSuccess) he = hipMalloc(&c->d_fat_c, ab * 3 * n);
+  if (he == hipSuccess) he = hipMalloc((void **)&c->d_force, sizeof(double) * 3 * n);
+  if (he == hipSuccess) he = hipMalloc((void **)&c->d_fvc, sizeof(double) * 3 * n);
+  if (he == hipSuccess) he = hipMalloc((void **)&c->d_fvl, sizeof(double) * 3 * n);
+  if (he == hipSuccess) he = hipMalloc(&c->d_grid, G * 8);
+  if (he == hipSuccess) he = hipMalloc(&c->d_cgrid, Hc * csz);
+  if (he == hipSuccess) he = hipMalloc((void **)&c->d_sum, sizeof(double) * 1024);
+  if (he == hipSuccess) he = hipHostMalloc((void **)&c->h_sum, sizeof(double) * 1024);
+  if (he != hipSuccess) {
+    pgk_set_error(std::string("pgk_create: device allocation failed: ") + hipGetErrorString(he));
+    pgk_destroy(c);
+    return PGK_ENOMEM;
+  }
+  *out = c;
+  return PGK_OK;
+}
+
+int pgk_destroy(pgk_ctx *c) {
+  if (!c) return PGK_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->d_pos, c->d_q, c->d_type, c->d_molid, c->d_eps, c->d_rmin, c->d_inttyp,
+                  c->d_excl_index, c->d_excl_list, c->d_exclpair, c->d_spl, c->d_splx,
+                  c->d_splf, c->d_ljp64, c->d_ljp32, c->d_bsp, c->d_bond, c->d_angl, c->d_tors,
+                  c->d_rb, c->d_p14, c->d_p14f, c->d_bpar, c->d_bin_count, c->d_bin_start,
+                  c->d_atom_bin, c->d_atom_slot, c->d_sorted, c->d_col_ncl, c->d_col_clstart,
+                  c->d_cl_atom, c->d_cl_col, c->d_slot_of, c->d_bbox, c->d_tmp_cj,
+                  c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
+                  c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
+                  c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
+                  c->d_grid, c->d_cgrid, c->d_tw, c->d_twf};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_sum) (void)hipHostFree(c->h_sum);
+  for (int k = 0; k < 2 * PGK_NPROF; k++)
+    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PGK_OK;
+}
+
+int pgk_set_box(pgk_ctx *c, const double *box) {
+  if (!c || !box || !(box[0] > 0) || !(box[1] > 0) || !(box[2] > 0)) {
+    pgk_set_error("pgk_set_box: invalid box");
+    return PGK_EINVAL;
+  }
+  for (int d = 0; d < 3; d++) c->box[d] = box[d];
+  return PGK_OK;
+}
+
+int pgk_set_positions(pgk_ctx *c, const double *pos) {
+  if (!c || !pos) {
+    pgk_set_error("pgk_set_positions: null argument");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpyAsync(c->d_pos, pos, sizeof(double) * 3 * c->natom, hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->have_pos = true;
+  return PGK_OK;
+}
+
+int pgk_set_positions_device(pgk_ctx *c, const double *d_pos) {
+  if (!c || !d_pos) {
+    pgk_set_error("pgk_set_positions_device: null argument");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpyAsync(c->d_pos, d_pos, sizeof(double) * 3 * c->natom,
+                           hipMemcpyDeviceToDevice, c->stream));
+  c->have_pos = true;
+  return PGK_OK;
+}
+
+int pgk_build_list(pgk_ctx *c, double rcut_book) {
+  if (!c || !(rcut_book > 0)) {
+    pgk_set_error("pgk_build_list: invalid argument");
+    return PGK_EINVAL;
+  }
+  for (int d = 0; d < 3; d++)
+    if (rcut_book > 0.5 * c->box[d]) {
+      // prepmd.F90:374-382 stops the same way
+      pgk_set_error("pgk_build_list: rcut_book exceeds half the box");
+      return PGK_EINVAL;
+    }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  pgk_prof_begin(c, PGK_PROF_LIST);
+  rc = pgk_list_build(c, rcut_book);
+  pgk_prof_end(c, PGK_PROF_LIST);
+  if (c->prof) {
+    (void)hipStreamSynchronize(c->stream);
+    pgk_prof_collect(c);
+  }
+  return rc;
+}
+
+int pgk_list_npairs(pgk_ctx *c, int64_t *np) {
+  if (!c || !np) return PGK_EINVAL;
+  if (!c->have_list) {
+    pgk_set_error("pgk_list_npairs: no list");
+    return PGK_ESTATE;
+  }
+  *np = c->npairs;
+  return PGK_OK;
+}
+
+int pgk_list_pairs(pgk_ctx *c, int32_t *pairs, int64_t capacity) {
+  if (!c || !pairs) return PGK_EINVAL;
+  if (!c->have_list) {
+    pgk_set_error("pgk_list_pairs: no list");
+    return PGK_ESTATE;
+  }
+  if (capacity < c->npairs) {
+    pgk_set_error("pgk_list_pairs: capacity too small");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  std::vector<int32_t> start(c->ncl + 1), cj(c->npair_cl), atoms((size_t)c->ncl * PGK_CL);
+  std::vector<uint64_t> mask(c->npair_cl);
+  HIP_CHECK(hipMemcpy(start.data(), c->d_ci_start, sizeof(int32_t) * (c->ncl + 1),
+                      hipMemcpyDeviceToHost));
+  if (c->npair_cl) {
+    HIP_CHECK(hipMemcpy(cj.data(), c->d_cj, sizeof(int32_t) * c->npair_cl, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(mask.data(), c->d_mask, sizeof(uint64_t) * c->npair_cl,
+                        hipMemcpyDeviceToHost));
+  }
+  HIP_CHECK(hipMemcpy(atoms.data(), c->d_cl_atom, sizeof(int32_t) * atoms.size(),
+                      hipMemcpyDeviceToHost));
+  std::vector<std::pair<int32_t, int32_t>> v;
+  v.reserve(c->npairs);
+  for (int ci = 0; ci < c->ncl; ci++)
+    for (int e = start[ci]; e < start[ci + 1]; e++)
+      for (int b = 0; b < 64; b++)
+        if ((mask[e] >> b) & 1ull) {
+          int ai = atoms[(size_t)ci * PGK_CL + (b & 7)], aj = atoms[(size_t)cj[e] * PGK_CL + (b >> 3)];
+          v.emplace_back(std::min(ai, aj) + 1, std::max(ai, aj) + 1);
+        }
+  std::sort(v.begin(), v.end());
+  for (size_t k = 0; k < v.size(); k++) {
+    pairs[2 * k] = v[k].first;
+    pairs[2 * k + 1] = v[k].second;
+  }
+  return PGK_OK;
+}
+
+int pgk_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, int32_t ifcalpreatom,
+                 pgk_force_out *out) {
+  if (!c || !mts || !out || !out->force || !out->ene) {
+    pgk_set_error("pgk_calforce: null argument");
+    return PGK_EINVAL;
+  }
+  bool split = out->for_viri_coul && out->for_viri_lj;
+  int rc = run_calforce(c, mts_flag, mts, ifcalpreatom, split, c->d_force, c->d_fvc, c->d_fvl,
+                        out->ene, out->virt);
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * 3 * c->natom;
+  HIP_CHECK(hipMemcpy(out->force, c->d_force, nb, hipMemcpyDeviceToHost));
+  if (split) {
+    HIP_CHECK(hipMemcpy(out->for_viri_coul, c->d_fvc, nb, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(out->for_viri_lj, c->d_fvl, nb, hipMemcpyDeviceToHost));
+  }
+  return PGK_OK;
+}
+
+int pgk_calforce_device(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, int32_t ifcalpreatom,
+                        double *d_force, double *ene, double *virt) {
+  if (!c || !mts || !ene) {
+    pgk_set_error("pgk_calforce_device: null argument");
+    return PGK_EINVAL;
+  }
+  return run_calforce(c, mts_flag, mts, ifcalpreatom, false, d_force ? d_force : c->d_force,
+                      nullptr, nullptr, ene, virt);
+}
+
+int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
+                    pgk_force_out *out, pgk_hf_out *hfo) {
+  (void)c; (void)mts_flag; (void)mts; (void)hf; (void)out; (void)hfo;
+  pgk_set_error("pgk_calforce_hf: not available in this build");
+  return PGK_EINVAL;
+}
+
+int pgk_profile(pgk_ctx *c, int32_t enable, double *ms, int64_t *calls) {
+  if (!c) return PGK_EINVAL;
+  if (ms)
+    for (int k = 0; k < PGK_NPROF; k++) ms[k] = c->prof_ms[k];
+  if (calls)
+    for (int k = 0; k < PGK_NPROF; k++) calls[k] = c->prof_calls[k];
+  if (enable >= 0) {
+    c->prof = enable != 0;
+    for (int k = 0; k < PGK_NPROF; k++) {
+      c->prof_ms[k] = 0.0;
+      c->prof_calls[k] = 0;
+    }
+  }
+  return PGK_OK;
+}
+
+int pgk_synchronize(pgk_ctx *c) {
+  if (!c) return PGK_EINVAL;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return PGK_OK;
+}
+
+}  // extern "C"

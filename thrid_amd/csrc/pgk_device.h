@@ -1,0 +1,79 @@
+// pgk_device.h - device helpers shared by the force kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pgk_internal.h"
+
+// ---- arithmetic forms ------------------------------------------------------
+// FP64:  compute double, accumulate double (hardware FP64 atomics)
+// MIXED: compute float, accumulate int64 fixed point (2^-32) - deterministic
+// FP32:  compute float, accumulate float
+template <int PREC> struct Prec;
+template <> struct Prec<PGK_PREC_FP64> {
+  using real = double;
+  using real4 = double4;
+  using acc = double;
+};
+template <> struct Prec<PGK_PREC_MIXED> {
+  using real = float;
+  using real4 = float4;
+  using acc = unsigned long long;
+};
+template <> struct Prec<PGK_PREC_FP32> {
+  using real = float;
+  using real4 = float4;
+  using acc = float;
+};
+
+__device__ __forceinline__ void acc_add(double *p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void acc_add(float *p, float v) { atomicAdd(p, v); }
+__device__ __forceinline__ void acc_add(float *p, double v) { atomicAdd(p, (float)v); }
+__device__ __forceinline__ void acc_add(unsigned long long *p, double v) {
+  long long x = __double2ll_rn(v * PGK_FX_SCALE);
+  atomicAdd(p, (unsigned long long)x);
+}
+__device__ __forceinline__ void acc_add(unsigned long long *p, float v) {
+  long long x = __double2ll_rn((double)v * PGK_FX_SCALE);
+  atomicAdd(p, (unsigned long long)x);
+}
+
+__device__ __forceinline__ double acc_load(const double *p) { return *p; }
+__device__ __forceinline__ double acc_load(const float *p) { return (double)*p; }
+__device__ __forceinline__ double acc_load(const unsigned long long *p) {
+  return (double)(long long)(*p) * PGK_FX_INV;
+}
+
+// Fortran dnint/anint: round half away from zero
+__device__ __forceinline__ double fnint(double x) { return round(x); }
+__device__ __forceinline__ float fnint(float x) { return roundf(x); }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-level deterministic record write: each wave reduces NF values, the
+// waves combine in LDS in fixed order, thread 0 writes the record.
+template <int NF, int NW>
+__device__ __forceinline__ void block_record(double (&v)[NF], double *rec) {
+  __shared__ double s[NW][NF];
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    double t = wave_sum(v[f]);
+    if (lane == 0) s[wave][f] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NF) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) t += s[w][threadIdx.x];
+    rec[threadIdx.x] = t;
+  }
+}
+
+#define PGK_REC 16   // doubles per reduction record

@@ -1,0 +1,186 @@
+// pgk_internal.h - device data layout and kernel entry points of libpgkforce.
+//
+// Layout in HBM (one context = one system):
+//   atom order (as the host):   pos  double[3N] (Fortran atmcor(3,N))
+//                               q    double[N], type int[N], molid int[N]
+//   cluster order (list build): 8-atom clusters along z inside x-y columns;
+//                               cl_atom int[ncl*8] (-1 = padding),
+//                               slot_of int[N] (atom -> cluster slot)
+//   pair list:                  ci_start int[ncl+1], cj int[npair],
+//                               mask uint64[npair]  (bit ii + 8*jj: pair of
+//                               i-slot ii and j-slot jj is in the list)
+//   per call:                   xq  (x,y,z,q) per slot in the compute type,
+//                               force accumulators (FP64, or int64 fixed point
+//                               in the mixed form), reduction records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "pgk_force.h"
+
+#define PGK_CL 8             // atoms per cluster
+#define PGK_MAXJ 2048        // max j-clusters per i-cluster before compaction
+#define PGK_MAXORD 10        // max B-spline order
+#define PGK_NPROF 8          // profiled stages: see enum PGK_PROF_* in pgk_force.h
+
+// fixed-point scale of the deterministic accumulators (mixed form)
+#define PGK_FX_SCALE 4294967296.0            // 2^32
+#define PGK_FX_INV (1.0 / 4294967296.0)
+#define PGK_GRID_FX_SCALE 1099511627776.0    // 2^40 for the PME charge grid
+#define PGK_GRID_FX_INV (1.0 / 1099511627776.0)
+
+// reduction record layout (doubles): one record per block of each kernel
+enum {
+  R_ELC = 0, R_VDW, R_VCOUL, R_VLJ,
+  R_TC = 4,    // 6: xx yy zz xy xz yz  (Coulomb tensor)
+  R_TL = 10,   // 6: LJ tensor
+  R_EWK = 16, R_VSCA, R_TK = 18,   // 6: reciprocal tensor
+  R_BOND = 24, R_ANGL, R_TORS, R_TORSRB, R_ELC14, R_VDW14,
+  R_VBOND = 30, R_VANGL, R_VTORS, R_V14,
+  R_TB = 34, R_TA = 40, R_TT = 46, R_T14 = 52,   // 9 each (full tensors)
+  R_N = 64
+};
+
+#define HIP_CHECK(x)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                 \
+      pgk_set_error(std::string("HIP error ") + hipGetErrorString(e_) +     \
+                    " at " __FILE__ ":" + std::to_string(__LINE__));        \
+      return PGK_EDEVICE;                                                   \
+    }                                                                       \
+  } while (0)
+
+void pgk_set_error(const std::string &msg);
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+};
+
+struct pgk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = PGK_PREC_FP64;
+
+  // system (host copies)
+  int natom = 0, ntype = 0;
+  std::vector<double> charge, lj_eps, lj_rmin;
+  std::vector<int32_t> atype, molid, inttyp, excl_index, excl_list;
+  std::vector<int32_t> bond_ij, bond_type, angl_ijk, angl_type;
+  std::vector<double> bond_k, bond_r0, angl_k, angl_t0;
+  std::vector<int32_t> tors_ijkl, tors_type, tors_divfac, tors_if14;
+  std::vector<double> tors_ring, tors_barhig, tors_phsang, tors_period, tors_dvdw,
+      tors_delc;
+  std::vector<int32_t> rb_ijkl, rb_type, rb_if14;
+  std::vector<double> rb_ring, rb_c, rb_dvdw, rb_delc;
+  double alpha = 0, rrcut = 0, rcut = 0;
+  int nspl = 0, nfft[3] = {0, 0, 0}, order = 4, ifnetq = 0;
+  double box[3] = {0, 0, 0};
+  double netchrgsq = 0, pot_ewc = 0, rkcut = 0, spl_int = 0;
+  std::vector<double> bsp_mod[3];
+  bool have_pos = false, have_list = false;
+  double list_box[3] = {0, 0, 0};
+
+  // device: atom order
+  double *d_pos = nullptr;      // 3N
+  double *d_q = nullptr;        // N
+  int32_t *d_type = nullptr;    // N
+  int32_t *d_molid = nullptr;   // N
+  double *d_eps = nullptr, *d_rmin = nullptr;   // ntype^2 (column-major)
+  int32_t *d_inttyp = nullptr;
+  int32_t *d_excl_index = nullptr, *d_excl_list = nullptr;
+  int32_t nexclpair = 0;
+  int32_t *d_exclpair = nullptr;   // 2 * nexclpair, 0-based (i<j)
+  double *d_spl = nullptr;         // nspl * 8 (real a,b,c,d | realpot a,b,c,d)
+  double *d_splx = nullptr;        // nspl * 8 (excs | excspot)
+  float *d_splf = nullptr;         // nspl * 8 float copy (real | realpot)
+  double *d_ljp64 = nullptr;       // ntype^2 x (eps, rmin^2)
+  float *d_ljp32 = nullptr;
+  double *d_bsp = nullptr;         // n1+n2+n3
+
+  // bonded (device, 0-based indices)
+  int32_t *d_bond = nullptr;       // 3 per bond: i j type
+  int32_t *d_angl = nullptr;       // 4 per angle
+  int32_t *d_tors = nullptr;       // 5 per torsion
+  int32_t *d_rb = nullptr;         // 5 per RB torsion
+  int32_t *d_p14 = nullptr;        // 2 per 1-4 pair
+  double *d_p14f = nullptr;        // 2 per 1-4 pair: vdw factor, elc factor
+  double *d_bpar = nullptr;        // bond k, r0 | angle k, t0 | tors ... packed
+  int n14 = 0;
+  int off_bk = 0, off_br = 0, off_ak = 0, off_at = 0, off_tb = 0, off_tp = 0,
+      off_tper = 0, off_tdiv = 0, off_rbc = 0;
+
+  // clusters / list
+  int ncol[2] = {1, 1}, nzb = 1, nbin = 0;
+  int ncl = 0;                     // number of clusters
+  int32_t *d_bin_count = nullptr, *d_bin_start = nullptr, *d_atom_bin = nullptr,
+          *d_atom_slot = nullptr, *d_sorted = nullptr;
+  int32_t *d_col_ncl = nullptr, *d_col_clstart = nullptr;  // per column
+  int32_t *d_cl_atom = nullptr;    // ncl*8
+  int32_t *d_cl_col = nullptr;     // ncl: column of each cluster
+  int32_t *d_slot_of = nullptr;    // N
+  float *d_bbox = nullptr;         // ncl * 8: lo xyz, hi xyz (wrapped frame)
+  int32_t *d_tmp_cj = nullptr;     // ncl * PGK_MAXJ
+  uint64_t *d_tmp_mask = nullptr;
+  int32_t *d_ci_count = nullptr, *d_ci_start = nullptr;
+  int32_t *d_cj = nullptr;
+  uint64_t *d_mask = nullptr;
+  int64_t npair_cl = 0, cap_pair_cl = 0;
+  int64_t npairs = 0;              // atom pairs in the list
+  int32_t *d_overflow = nullptr;
+  void *d_scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  int ncl_cap = 0, nbin_cap = 0, ncolumn_cap = 0;
+
+  // per call
+  void *d_xq = nullptr;            // ncl*8 of double4 or float4
+  int32_t *d_ct = nullptr;         // ncl*8 types (0 for padding)
+  void *d_fcl = nullptr;           // ncl*8*3 force accumulators (slot order)
+  void *d_fat = nullptr;           // N*3 force accumulators (atom order)
+  void *d_fcl_c = nullptr;         // slot-order Coulomb part (for_viri_coul)
+  void *d_fat_c = nullptr;         // atom-order Coulomb part
+  double *d_force = nullptr;       // N*3 final force (double)
+  double *d_fvc = nullptr, *d_fvl = nullptr;   // per-class forces (optional)
+  double *d_red = nullptr;         // reduction records
+  int red_cap = 0;
+  double *d_sum = nullptr;         // R_N final sums
+  double *h_sum = nullptr;         // pinned R_N
+
+  // PME
+  void *d_grid = nullptr;          // real grid (n1*n2*n3) in compute type, or int64
+  void *d_cgrid = nullptr;         // complex half grid ((n1/2+1)*n2*n3)
+  double *d_tw = nullptr;          // twiddles: 3 dims, each n complex (FP64)
+  float *d_twf = nullptr;
+  int tw_off[3] = {0, 0, 0};
+
+  // reduction segment bookkeeping for the current call
+  int red_used = 0;
+
+  // optional per-stage timing with HIP events on the context's stream
+  bool prof = false;
+  hipEvent_t ev[2 * PGK_NPROF] = {};
+  bool ev_used[PGK_NPROF] = {};
+  double prof_ms[PGK_NPROF] = {};
+  int64_t prof_calls[PGK_NPROF] = {};
+};
+
+// stage markers (record before/after a stage when profiling is on)
+void pgk_prof_begin(pgk_ctx *c, int k);
+void pgk_prof_end(pgk_ctx *c, int k);
+void pgk_prof_collect(pgk_ctx *c);
+
+// ---- kernels / launchers (return hipError_t of the launch) ----------------
+int pgk_list_build(pgk_ctx *c, double rcut_book);
+int pgk_pairs_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
+int pgk_excl_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
+int pgk_pme_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
+int pgk_pme_prepare(pgk_ctx *c);
+
+static inline int pgk_grid_blocks(int64_t n, int bs) {
+  return (int)((n + bs - 1) / bs);
+}

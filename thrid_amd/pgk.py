@@ -1,0 +1,182 @@
+"""Python binding of libpgkforce.so (the MI355X force path).
+
+Mirrors the reference's force interface: a ForceContext owns one system and
+offers mklist (mklist2a / mklist_cell), calforce (calforce.F90:8-1073) and
+calforce_hf (calforce_hf.F90:8-1451) with the reference's output records.
+There is no CPU fallback: if the HIP library is missing or no GPU is present
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgkforce.so")
+
+_lib = None
+
+
+class PgkError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libpgkforce.so and declare its C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PgkError(f"libpgkforce.so not built ({path}); run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.pgk_abi_version.restype = C.c_int32
+    L.pgk_last_error.restype = C.c_char_p
+    L.pgk_create.argtypes = [C.POINTER(abi.SystemDesc), C.c_int32, C.POINTER(vp)]
+    L.pgk_destroy.argtypes = [vp]
+    L.pgk_set_box.argtypes = [vp, vp]
+    L.pgk_set_positions.argtypes = [vp, vp]
+    L.pgk_set_positions_device.argtypes = [vp, vp]
+    L.pgk_build_list.argtypes = [vp, C.c_double]
+    L.pgk_list_npairs.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pgk_list_pairs.argtypes = [vp, vp, C.c_int64]
+    L.pgk_calforce.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32,
+                               C.POINTER(abi.ForceOut)]
+    L.pgk_calforce_hf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
+                                  C.POINTER(abi.ForceOut), C.POINTER(abi.HfOut)]
+    L.pgk_calforce_device.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32, vp, vp, vp]
+    L.pgk_synchronize.argtypes = [vp]
+    for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
+              "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
+              "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
+              "pgk_synchronize"):
+        getattr(L, f).restype = C.c_int
+    if L.pgk_abi_version() != abi.ABI_VERSION:
+        raise PgkError("libpgkforce.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = _lib.pgk_last_error().decode(errors="replace") if _lib else ""
+        raise PgkError(f"{what} failed ({rc}): {msg}")
+
+
+class ForceContext:
+    """One system on the current GPU.
+
+    precision: 'fp64' (the reference's arithmetic), 'mixed' (FP32 pair/FFT,
+    64-bit fixed-point force sums) or 'fp32'."""
+
+    def __init__(self, sysd, precision="fp64", excl=None, ifnetqcorrp=0):
+        self.L = load_library()
+        self.sysd = sysd
+        self.n = int(sysd["natom"])
+        self.precision = precision
+        self._desc, self._keep = abi.make_desc(sysd, excl, ifnetqcorrp)
+        h = C.c_void_p()
+        _check(self.L.pgk_create(C.byref(self._desc), abi.PREC[precision], C.byref(h)),
+               "pgk_create")
+        self.h = h
+        self.set_box(sysd["box"])
+        self.set_positions(sysd["pos"])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pgk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_box(self, box):
+        b = np.ascontiguousarray(box, np.float64)
+        _check(self.L.pgk_set_box(self.h, b.ctypes.data), "pgk_set_box")
+
+    def set_positions(self, pos):
+        p = np.ascontiguousarray(pos, np.float64)
+        if p.shape != (self.n, 3):
+            raise ValueError("positions must be (natom, 3)")
+        _check(self.L.pgk_set_positions(self.h, p.ctypes.data), "pgk_set_positions")
+
+    def set_positions_device(self, dev_ptr: int):
+        _check(self.L.pgk_set_positions_device(self.h, C.c_void_p(dev_ptr)),
+               "pgk_set_positions_device")
+
+    def mklist(self, rcut_book=None):
+        rcb = float(rcut_book if rcut_book is not None else self.sysd["rcut_book"])
+        _check(self.L.pgk_build_list(self.h, rcb), "pgk_build_list")
+
+    def npairs(self):
+        v = C.c_int64()
+        _check(self.L.pgk_list_npairs(self.h, C.byref(v)), "pgk_list_npairs")
+        return v.value
+
+    def list_pairs(self):
+        """Listed pairs as a sorted (npairs, 2) array of 1-based (i, j), i < j."""
+        n = self.npairs()
+        out = np.zeros((max(n, 1), 2), np.int32)
+        _check(self.L.pgk_list_pairs(self.h, out.ctypes.data, n), "pgk_list_pairs")
+        return out[:n]
+
+    def calforce(self, mts_flag=abi.MTS_LONG, mts=None, ifcalpreatom=1, split=False, ene=None):
+        mts = mts or abi.mts_all(mts_flag)
+        f = np.zeros((self.n, 3))
+        e = np.zeros(abi.NENE) if ene is None else np.array(ene, np.float64)
+        v = np.zeros((abi.NTENS, 9))
+        fc = np.zeros((self.n, 3)) if split else None
+        fl = np.zeros((self.n, 3)) if split else None
+        o = abi.ForceOut(abi.dptr(f), abi.dptr(e), abi.dptr(v), abi.dptr(fc), abi.dptr(fl))
+        _check(self.L.pgk_calforce(self.h, mts_flag, C.byref(mts), ifcalpreatom, C.byref(o)),
+               "pgk_calforce")
+        r = dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy())
+        if split:
+            r["for_viri_coul"], r["for_viri_lj"] = fc, fl
+        return r
+
+    def calforce_device(self, d_force=0, mts_flag=abi.MTS_LONG, mts=None, ifcalpreatom=1):
+        mts = mts or abi.mts_all(mts_flag)
+        e = np.zeros(abi.NENE)
+        v = np.zeros((abi.NTENS, 9))
+        _check(self.L.pgk_calforce_device(self.h, mts_flag, C.byref(mts), ifcalpreatom,
+                                          C.c_void_p(d_force), e.ctypes.data, v.ctypes.data),
+               "pgk_calforce_device")
+        return e, v
+
+    def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None):
+        mts = mts or abi.mts_all(mts_flag)
+        keep = {}
+        h = abi.HfDesc()
+        h.mode = int(hf.get("mode", abi.HF_VOLUME))
+        for k in ("z1", "z2"):
+            keep[k] = np.ascontiguousarray(hf[k], np.float64)
+        h.nregion = len(keep["z1"])
+        h.z1, h.z2 = abi.dptr(keep["z1"]), abi.dptr(keep["z2"])
+        keep["ht"] = np.ascontiguousarray(hf.get("hftyp", np.ones(self.n, np.int32)), np.int32)
+        h.hftyp = abi.iptr(keep["ht"])
+        keep["mc"] = np.ascontiguousarray(hf.get("molecom", np.zeros((self.sysd["nmol"], 3))),
+                                          np.float64)
+        h.nmol = keep["mc"].shape[0]
+        h.molecom = abi.dptr(keep["mc"])
+        f = np.zeros((self.n, 3))
+        e = np.zeros(abi.NENE)
+        v = np.zeros((abi.NTENS, 9))
+        pe = np.zeros((abi.NHF, self.n))
+        wv = np.zeros((abi.NHF, h.nregion, self.n, 9))
+        o = abi.ForceOut(abi.dptr(f), abi.dptr(e), abi.dptr(v), None, None)
+        ho = abi.HfOut(abi.dptr(pe), abi.dptr(wv))
+        _check(self.L.pgk_calforce_hf(self.h, mts_flag, C.byref(mts), C.byref(h), C.byref(o),
+                                      C.byref(ho)), "pgk_calforce_hf")
+        return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
+                    pot_atm=pe, viri_atm=wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+
+    def synchronize(self):
+        _check(self.L.pgk_synchronize(self.h), "pgk_synchronize")

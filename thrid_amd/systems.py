@@ -268,6 +268,31 @@ class _Builder:
         self.nmol = 0
         self.natom = 0
 
+    def add_checked(self, tmpl, centre, rng, box, dmin=2.2, tries=200):
+        """Add with a random orientation that keeps every atom at least `dmin`
+        (minimum image) from the atoms already placed."""
+        placed = np.concatenate(self.pos, axis=0) if self.pos else np.zeros((0, 3))
+        box = np.asarray(box, float)
+        local = tmpl["pos"] - tmpl["pos"].mean(axis=0)
+        best, bestd = None, -1.0
+        near = np.zeros((0, 3))
+        if placed.shape[0]:
+            d0 = placed - centre
+            d0 -= box * np.round(d0 / box)
+            near = d0[np.all(np.abs(d0) < 8.0, axis=1)]
+        for _ in range(tries):
+            rot = _random_rotation(rng)
+            if near.shape[0] == 0:
+                best = rot
+                break
+            dr = (local @ rot.T)[:, None, :] - near[None, :, :]
+            dm = np.sqrt((dr ** 2).sum(-1)).min()
+            if dm > bestd:
+                best, bestd = rot, dm
+            if dm >= dmin:
+                break
+        self.add(tmpl, centre, best)
+
     def add(self, tmpl, centre, rot):
         base = self.natom
         self.nmol += 1
@@ -409,17 +434,17 @@ def propanol_water_slab(nwater=4096, nprop=512, box=(50.0, 50.0, 150.0), *, seed
         b.add(wt, c, _random_rotation(rng))
     pt = _propanol_template()
     per_side = nprop // 2
-    sp = 6.5                                   # propanol lattice spacing (A)
+    sp = (1.0 / 0.0075) ** (1.0 / 3.0)         # propanol lattice spacing (A)
     npx = max(1, int(round(lx / sp)))
     npy = max(1, int(round(ly / sp)))
     npz = max(1, int(math.ceil(per_side / (npx * npy))))
     pthick = npz * sp
     for side in (0, 1):
-        zlo = z0 - pthick - 1.0 if side == 0 else z0 + wthick + 1.0
+        zlo = z0 - pthick - 2.5 if side == 0 else z0 + wthick + 2.5
         ps = _lattice_sites(npx, npy, npz, (0, 0, zlo), (lx, ly, zlo + pthick), rng, 0.3)
         ps = ps[rng.permutation(len(ps))][:per_side]
         for c in ps:
-            b.add(pt, c, _random_rotation(rng))
+            b.add_checked(pt, c, rng, box)
     return _finish(b, box, alpha=alpha, rrcut=rcut, rcut=rcut, rcut_book=rcut_book,
                    nfft=nfft, order=order, torsion_form=torsion_form, name=name, seed=seed)
 
@@ -442,6 +467,8 @@ def mixture_box(n_target, composition, *, seed=4, density_gcc=0.997, rcut=10.0,
     kinds = kinds[:nmol] + [tm[0][0]] * max(0, nmol - len(kinds))
     kinds = [kinds[i] for i in rng.permutation(len(kinds))]
     mass = sum(sum(ATOM_TYPES[TYPE_INDEX[t] - 1].mass for t in tmpls[k]["types"]) for k in kinds)
+    if composition == "butane":
+        density_gcc = min(density_gcc, 0.60)         # liquid n-butane
     vol = mass / 6.0221367e23 / density_gcc * 1e24   # A^3
     box = vol ** (1.0 / 3.0)
     box = max(box, 2.0 * rcut_book + 0.5)
@@ -449,8 +476,21 @@ def mixture_box(n_target, composition, *, seed=4, density_gcc=0.997, rcut=10.0,
     sites = _lattice_sites(nsite, nsite, nsite, (0, 0, 0), (box, box, box), rng, 0.2)
     sites = sites[rng.permutation(len(sites))][:nmol]
     b = _Builder()
-    for k, c in zip(kinds, sites):
-        b.add(tmpls[k], c, _random_rotation(rng))
+    bx = np.array([box, box, box])
+    # larger molecules first (orientation search against what is placed), then
+    # waters, dropping any water that would sit on top of them
+    big = [(k, c) for k, c in zip(kinds, sites) if k != "water"]
+    small = [c for k, c in zip(kinds, sites) if k == "water"]
+    for k, c in big:
+        b.add_checked(tmpls[k], c, rng, bx, dmin=2.5, tries=400)
+    heavy = np.concatenate(b.pos, axis=0) if b.pos else np.zeros((0, 3))
+    for c in small:
+        if heavy.shape[0]:
+            d = heavy - c
+            d -= bx * np.round(d / bx)
+            if np.min((d ** 2).sum(-1)) < 2.6 ** 2:
+                continue
+        b.add(tmpls["water"], c, _random_rotation(rng))
     g = fft_friendly(int(math.ceil(box / spacing)))
     return _finish(b, (box, box, box), alpha=alpha, rrcut=rcut, rcut=rcut,
                    rcut_book=rcut_book, nfft=(g, g, g), order=order,
