@@ -41,15 +41,13 @@ def pair_counts(sysd):
     d -= box * np.round(d / box)
     r = np.sqrt((d * d).sum(1))
     idx, lst = exclusions(sysd)
-    ex = set()
-    for i in range(sysd["natom"]):
-        for k in range(idx[i] - 1, idx[i + 1] - 1):
-            if lst[k] > 0:
-                ex.add((i, lst[k] - 1))
-    a = np.minimum(pr[:, 0], pr[:, 1])
-    b = np.maximum(pr[:, 0], pr[:, 1])
-    keys = a.astype(np.int64) * sysd["natom"] + b
-    exk = np.array([i * sysd["natom"] + j for i, j in ex], np.int64)
+    n = sysd["natom"]
+    owner = np.repeat(np.arange(n, dtype=np.int64), np.diff(idx).astype(np.int64))
+    sel = lst > 0
+    exk = owner[sel] * n + (lst[sel].astype(np.int64) - 1)
+    a = np.minimum(pr[:, 0], pr[:, 1]).astype(np.int64)
+    b = np.maximum(pr[:, 0], pr[:, 1]).astype(np.int64)
+    keys = a * n + b
     keep = ~np.isin(keys, exk)
     ncoul = int(np.count_nonzero(keep & (r <= sysd["rrcut"])))
     ti = sysd["atype"][a] - 1
@@ -60,19 +58,16 @@ def pair_counts(sysd):
 
 
 def run_gpu(sysd, precision, steps, warmup):
-    import ctypes as C
-    from thrid_amd import abi
     from thrid_amd.pgk import ForceContext
     ctx = ForceContext(sysd, precision)
-    L = ctx.L
-    L.pgk_profile.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     ctx.mklist(sysd["rcut_book"])
+    info = ctx.list_info()
     for k in range(warmup):
         if k % NSTEP_BOOK == 0:
             ctx.mklist(sysd["rcut_book"])
         e, _ = ctx.calforce_device()
     ctx.synchronize()
-    L.pgk_profile(ctx.h, 1, None, None)
+    ctx.profile(1)
     t0 = time.perf_counter()
     for k in range(steps):
         if k % NSTEP_BOOK == 0:
@@ -80,14 +75,12 @@ def run_gpu(sysd, precision, steps, warmup):
         e, _ = ctx.calforce_device()
     ctx.synchronize()
     t1 = time.perf_counter()
-    ms = np.zeros(8)
-    calls = np.zeros(8, np.int64)
-    L.pgk_profile(ctx.h, 0, ms.ctypes.data, calls.ctypes.data)
+    ms, calls = ctx.profile(0)
     ene = e.copy()
     ctx.close()
     stages = ["pairs", "excl", "spread", "fft", "gather", "bonded", "list", "other"]
     prof = {s: (float(ms[k] / calls[k]) if calls[k] else 0.0) for k, s in enumerate(stages)}
-    return (t1 - t0) / steps, prof, ene
+    return (t1 - t0) / steps, prof, ene, info
 
 
 def cpu_baseline(sysd, budget_s=20.0):
@@ -135,8 +128,8 @@ def main():
     n = sysd["natom"]
     res = {}
     for prec in ("fp64", "mixed"):
-        sec, prof, ene = run_gpu(sysd, prec, args.steps, args.warmup)
-        res[prec] = dict(sec=sec, prof=prof, ene=ene)
+        sec, prof, ene, info = run_gpu(sysd, prec, args.steps, args.warmup)
+        res[prec] = dict(sec=sec, prof=prof, ene=ene, info=info)
         print(f"# {prec}: {sec * 1e3:.3f} ms/step, stages(ms) "
               + " ".join(f"{k}={v:.3f}" for k, v in prof.items()), file=sys.stderr, flush=True)
     ncoul, nlj = pair_counts(sysd)
@@ -171,6 +164,9 @@ def main():
                      "frac": achieved / PEAK["mixed"], "traffic": None,
                      "flops_per_launch": flops, "pairs_coulomb": ncoul, "pairs_lj": nlj,
                      "avg_launch_ms": mix["prof"]["pairs"]},
+        "list": {"clusters": res["mixed"]["info"][0], "cluster_pairs": res["mixed"]["info"][1],
+                 "listed_pairs": res["mixed"]["info"][2],
+                 "lane_efficiency": ncoul / (64.0 * res["mixed"]["info"][1])},
         "roofline_fp64_pairs": {"achieved": flops / (res["fp64"]["prof"]["pairs"] * 1e-3) / 1e12
                                 if res["fp64"]["prof"]["pairs"] > 0 else 0.0,
                                 "peak": PEAK["fp64"], "unit": "TFLOP/s"},

@@ -248,6 +248,10 @@ enum {
   PGK_NPROF_STAGES
 };
 int pgk_profile(pgk_ctx *ctx, int32_t enable, double *ms, int64_t *calls);
+/* Shape of the current list (no reference counterpart: instrumentation):
+ * info[0] clusters, info[1] cluster pairs, info[2] listed atom pairs,
+ * info[3] atoms per cluster.                                                  */
+int pgk_list_info(pgk_ctx *ctx, int64_t *info);
 
 #ifdef __cplusplus
 }

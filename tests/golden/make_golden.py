@@ -1,0 +1,70 @@
+"""Write the golden fixtures of tests/test_golden.py from the reference.
+
+Runs the reference peachgk_md force path compiled from source
+(oracle/build_ref.sh -> oracle/_ref/libpgkref.so) on small seeded systems and
+stores inputs and outputs as compressed .npz files (data only).
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from thrid_amd import abi, systems  # noqa: E402
+import pyoracle  # noqa: E402
+from helpers import water648, small_slab  # noqa: E402
+
+
+def system_arrays(s):
+    keys = [k for k, v in s.items() if isinstance(v, np.ndarray)]
+    out = {f"sys_{k}": s[k] for k in keys}
+    for k in ("natom", "nmol", "alpha", "rrcut", "rcut", "rcut_book", "nspltbl", "pme_order"):
+        out[f"sys_{k}"] = np.array(s[k])
+    return out
+
+
+def tiny_hf_slab():
+    s = systems.propanol_water_slab(nwater=60, nprop=8, box=(16.0, 16.0, 40.0), nfft=(16, 16, 40),
+                                    rcut=7.5, rcut_book=8.0, alpha=0.4, seed=21)
+    rng = np.random.default_rng(21)
+    s["pos"] = s["pos"] + rng.normal(scale=0.05, size=s["pos"].shape)
+    return s
+
+
+def main():
+    if not pyoracle.reference_available():
+        raise SystemExit("build the reference first: oracle/build_ref.sh")
+    cases = {"water648": (water648(), 0), "slab_rb": (small_slab("rb"), 1),
+             "slab_periodic": (small_slab("periodic"), 1)}
+    for name, (s, mode) in cases.items():
+        r = pyoracle.Reference(s)
+        idx, lst = r.mklist(s["rcut_book"], mode)
+        out = r.calforce(abi.MTS_LONG)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **system_arrays(s),
+                            list_mode=mode, nb_index=idx, nb_list=lst, force=out["force"],
+                            ene=out["ene"], virt=out["virt"],
+                            for_viri_coul=out["for_viri_coul"], for_viri_lj=out["for_viri_lj"])
+        print(name, s["natom"], len(lst))
+    s = tiny_hf_slab()
+    r = pyoracle.Reference(s)
+    idx, lst = r.mklist(s["rcut_book"], 0)
+    z = np.linspace(0.0, 40.0, 4)
+    res = {}
+    for mode in (abi.HF_VOLUME, abi.HF_SURFACE):
+        o = r.calforce_hf(dict(z1=z[:-1], z2=z[1:], mode=mode))
+        for k in ("force", "ene", "virt", "pot_atm", "viri_atm"):
+            res[f"m{mode}_{k}"] = o[k]
+    np.savez_compressed(os.path.join(HERE, "hf_slab.npz"), **system_arrays(s), z=z,
+                        nb_index=idx, nb_list=lst, **res)
+    print("hf_slab", s["natom"])
+
+
+if __name__ == "__main__":
+    main()

@@ -394,6 +394,7 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
     HIP_CHECK(hipMalloc(&c->d_fcl, ns * 3 * fb));
     HIP_CHECK(hipMalloc(&c->d_fcl_c, ns * 3 * fb));
     c->ncl_cap = ncl;
+    c->acc_dirty = true;   // new accumulators start zeroed by the next call
   }
   c->ncl = ncl;
   c->ncol[0] = g.ncol[0];

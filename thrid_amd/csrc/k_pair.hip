@@ -30,11 +30,10 @@ struct PairArgs {
   const int32_t *ci_start, *cj;
   const uint64_t *mask;
   const R *spl;          // nspl x 8: force a,b,c,d | energy a,b,c,d
-  const R *lj;           // ntype^2 x 2: eps, rmin^2 (column-major type pairs)
-  const int32_t *inttyp;
+  const R *lj;           // ntype^2 x 2: eps (0 unless INTTYPE_VDW), rmin^2
   A *fcl, *fcl_c;
   double *red;
-  int ncl, ntype, nspl;
+  int ncl, ntype, nspl_lds, lj_lds;
   R L[3], Linv[3];
   R sqcut, sqcut2, rrcut;
   R h, hinv, xmax;
@@ -65,13 +64,68 @@ __device__ __forceinline__ float r2_nc(float x, float y, float z) {
   return x * x + y * y + z * z;
 }
 
-template <int PREC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
+// nearest-image shift; listed pairs are shorter than half the box, where
+// round-half-even and the reference's round-half-away agree
+__device__ __forceinline__ double nimg(double x) { return rint(x); }
+__device__ __forceinline__ float nimg(float x) { return rintf(x); }
+
+// ---- cross-lane sums over the 8 lanes of an aligned group (DPP, no LDS) ----
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppd(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// quad_perm[1,0,3,2] = 0xB1, quad_perm[2,3,0,1] = 0x4E, row_half_mirror = 0x141
+__device__ __forceinline__ float sum8(float v) {
+  v += dppf<0xB1>(v);
+  v += dppf<0x4E>(v);
+  v += dppf<0x141>(v);
+  return v;
+}
+__device__ __forceinline__ double sum8(double v) {
+  v += dppd<0xB1>(v);
+  v += dppd<0x4E>(v);
+  v += dppd<0x141>(v);
+  return v;
+}
+
+template <class R>
+__device__ __forceinline__ void rinv_pair(R r2, R &rinv2, R &r);
+template <>
+__device__ __forceinline__ void rinv_pair<double>(double r2, double &rinv2, double &r) {
+  rinv2 = 1.0 / r2;
+  r = sqrt(r2);
+}
+template <>
+__device__ __forceinline__ void rinv_pair<float>(float r2, float &rinv2, float &r) {
+  float ri = __frsqrt_rn(r2);
+  rinv2 = ri * ri;
+  r = r2 * ri;
+}
+
+template <int PREC, bool SPLIT, int NW>
+__global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
-  // per-lane force sums: FP64 in every form (exact order, no cancellation loss)
+  extern __shared__ __align__(16) unsigned char smem[];
+  R *s_spl = reinterpret_cast<R *>(smem);
+  R *s_lj = s_spl + 8 * a.nspl_lds;
+  // stage the erfc spline and LJ tables in LDS
+  for (int k = threadIdx.x; k < 8 * a.nspl_lds; k += blockDim.x) s_spl[k] = a.spl[k];
+  const int nlj = 2 * a.ntype * a.ntype;
+  if (a.lj_lds)
+    for (int k = threadIdx.x; k < nlj; k += blockDim.x) s_lj[k] = a.lj[k];
+  __syncthreads();
+  const R *ljt = a.lj_lds ? s_lj : a.lj;
+
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
+  const int ci = blockIdx.x * NW + wave;
   double ev[14];
 #pragma unroll
   for (int k = 0; k < 14; k++) ev[k] = 0.0;
@@ -79,28 +133,45 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
     const int ii = lane & 7, jj = lane >> 3;
     const R4 xi = a.xq[ci * PGK_CL + ii];
     const int ti = a.ct[ci * PGK_CL + ii];
+    const int tib = (ti - 1);
     double fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
     const int e0 = a.ci_start[ci], e1 = a.ci_start[ci + 1];
+    int cj_n = 0;
+    uint64_t m_n = 0;
+    R4 xj_n;
+    int tj_n = 0;
+    if (e0 < e1) {
+      cj_n = a.cj[e0];
+      m_n = a.mask[e0];
+      xj_n = a.xq[cj_n * PGK_CL + jj];
+      tj_n = a.ct[cj_n * PGK_CL + jj];
+    }
     for (int e = e0; e < e1; e++) {
-      const int cj = a.cj[e];
-      const uint64_t m = a.mask[e];
-      const R4 xj = a.xq[cj * PGK_CL + jj];
-      const int tj = a.ct[cj * PGK_CL + jj];
+      const int cj = cj_n;
+      const uint64_t m = m_n;
+      const R4 xj = xj_n;
+      const int tj = tj_n;
+      if (e + 1 < e1) {   // prefetch the next j-cluster
+        cj_n = a.cj[e + 1];
+        m_n = a.mask[e + 1];
+        xj_n = a.xq[cj_n * PGK_CL + jj];
+        tj_n = a.ct[cj_n * PGK_CL + jj];
+      }
       R fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
       if ((m >> lane) & 1ull) {
         R rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        rx = rx - a.L[0] * fnint(rx * a.Linv[0]);
-        ry = ry - a.L[1] * fnint(ry * a.Linv[1]);
-        rz = rz - a.L[2] * fnint(rz * a.Linv[2]);
+        rx = rx - a.L[0] * nimg(rx * a.Linv[0]);
+        ry = ry - a.L[1] * nimg(ry * a.Linv[1]);
+        rz = rz - a.L[2] * nimg(rz * a.Linv[2]);
         const R r2 = r2_nc(rx, ry, rz);
         if (r2 <= a.sqcut) {
-          const R rinv2 = R(1) / r2;
-          const R r = sqrt(r2);
+          R rinv2, r;
+          rinv_pair<R>(r2, rinv2, r);
           if (r <= a.rrcut) {
             R sf, se;
-            spline2(a.spl, r, a.h, a.hinv, a.xmax, a.nspl, sf, se);
+            spline2(s_spl, r, a.h, a.hinv, a.xmax, a.nspl_lds, sf, se);
             const R qq = xi.w * xj.w;
             const R ft = sf * qq;
             e_elc += se * qq;
@@ -111,21 +182,19 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
             tc[3] += cx * ry; tc[4] += cx * rz; tc[5] += cy * rz;
           }
           if (r2 <= a.sqcut2) {
-            const int tp = (ti - 1) + a.ntype * (tj - 1);
-            if (a.inttyp[tp] == 1) {
-              const R eps = a.lj[2 * tp], rm2 = a.lj[2 * tp + 1];
-              const R t2 = rinv2 * rm2;
-              const R t6 = t2 * t2 * t2;
-              const R t12 = t6 * t6;
-              e_vdw += eps * (t12 - R(2) * t6);
-              const R ft = R(12) * eps * (t12 - t6) * rinv2;
-              const R lx = ft * rx, ly = ft * ry, lz = ft * rz;
-              tl[0] += lx * rx; tl[1] += ly * ry; tl[2] += lz * rz;
-              tl[3] += lx * ry; tl[4] += lx * rz; tl[5] += ly * rz;
-              fx = lx;
-              fy = ly;
-              fz = lz;
-            }
+            const int tp = tib + a.ntype * (tj - 1);
+            const R eps = ljt[2 * tp], rm2 = ljt[2 * tp + 1];
+            const R t2 = rinv2 * rm2;
+            const R t6 = t2 * t2 * t2;
+            const R t12 = t6 * t6;
+            e_vdw += eps * (t12 - R(2) * t6);
+            const R ft = R(12) * eps * (t12 - t6) * rinv2;
+            const R lx = ft * rx, ly = ft * ry, lz = ft * rz;
+            tl[0] += lx * rx; tl[1] += ly * ry; tl[2] += lz * rz;
+            tl[3] += lx * ry; tl[4] += lx * rz; tl[5] += ly * rz;
+            fx = lx;
+            fy = ly;
+            fz = lz;
           }
           fx += cx;
           fy += cy;
@@ -140,33 +209,17 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
         ciy += (double)cy;
         ciz += (double)cz;
       }
-      // j side: sum over the 8 i-lanes that share this j-slot
-      R gx = -fx, gy = -fy, gz = -fz;
-#pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
-        gx += __shfl_xor(gx, o, 64);
-        gy += __shfl_xor(gy, o, 64);
-        gz += __shfl_xor(gz, o, 64);
-      }
-      R hx = 0, hy = 0, hz = 0;
-      if (SPLIT) {
-        hx = -cx; hy = -cy; hz = -cz;
-#pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-          hx += __shfl_xor(hx, o, 64);
-          hy += __shfl_xor(hy, o, 64);
-          hz += __shfl_xor(hz, o, 64);
-        }
-      }
-      if (ii == 0 && ((m >> (8 * jj)) & 0xffull)) {
+      // j side: every lane of the 8-lane group gets the group sum; lanes
+      // ii = 0, 1, 2 add the x, y, z component in one atomic instruction
+      if ((m >> (8 * jj)) & 0xffull) {
+        const R gx = sum8(-fx), gy = sum8(-fy), gz = sum8(-fz);
+        const R g = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
         const size_t s = (size_t)(cj * PGK_CL + jj) * 3;
-        acc_add(&a.fcl[s + 0], gx);
-        acc_add(&a.fcl[s + 1], gy);
-        acc_add(&a.fcl[s + 2], gz);
+        if (ii < 3) acc_add(&a.fcl[s + ii], g);
         if (SPLIT) {
-          acc_add(&a.fcl_c[s + 0], hx);
-          acc_add(&a.fcl_c[s + 1], hy);
-          acc_add(&a.fcl_c[s + 2], hz);
+          const R hx = sum8(-cx), hy = sum8(-cy), hz = sum8(-cz);
+          const R hh = (ii == 0) ? hx : ((ii == 1) ? hy : hz);
+          if (ii < 3) acc_add(&a.fcl_c[s + ii], hh);
         }
       }
     }
@@ -182,15 +235,13 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
         ciz += __shfl_xor(ciz, o, 64);
       }
     }
-    if (jj == 0) {
+    if (jj < 3) {   // lanes jj = 0, 1, 2 hold x, y, z of slot ii
       const size_t s = (size_t)(ci * PGK_CL + ii) * 3;
-      acc_add(&a.fcl[s + 0], fix);
-      acc_add(&a.fcl[s + 1], fiy);
-      acc_add(&a.fcl[s + 2], fiz);
+      const double v = (jj == 0) ? fix : ((jj == 1) ? fiy : fiz);
+      acc_add(&a.fcl[s + jj], v);
       if (SPLIT) {
-        acc_add(&a.fcl_c[s + 0], cix);
-        acc_add(&a.fcl_c[s + 1], ciy);
-        acc_add(&a.fcl_c[s + 2], ciz);
+        const double w = (jj == 0) ? cix : ((jj == 1) ? ciy : ciz);
+        acc_add(&a.fcl_c[s + jj], w);
       }
     }
     ev[0] = e_elc;
@@ -201,7 +252,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs<PREC> a) {
       ev[8 + k] = tl[k];
     }
   }
-  block_record<14, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  block_record<14, NW>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 // ---- excluded pairs (FP64 in every form) -----------------------------------
@@ -257,6 +308,7 @@ static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
   using A = typename Prec<PREC>::acc;
+  constexpr int NW = 8;
   PairArgs<PREC> a;
   a.xq = reinterpret_cast<const R4 *>(c->d_xq);
   a.ct = c->d_ct;
@@ -270,12 +322,17 @@ static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
     a.spl = reinterpret_cast<const R *>(c->d_splf);
     a.lj = reinterpret_cast<const R *>(c->d_ljp32);
   }
-  a.inttyp = c->d_inttyp;
   a.fcl = reinterpret_cast<A *>(c->d_fcl);
   a.fcl_c = reinterpret_cast<A *>(c->d_fcl_c);
   a.ncl = c->ncl;
   a.ntype = c->ntype;
-  a.nspl = c->nspl;
+  // only the table rows up to rrcut are ever read
+  int nused = (int)(c->rrcut / c->spl_int) + 2;
+  a.nspl_lds = nused < c->nspl ? nused : c->nspl;
+  size_t lds = (size_t)8 * a.nspl_lds * sizeof(R);
+  size_t ljb = (size_t)2 * c->ntype * c->ntype * sizeof(R);
+  a.lj_lds = (lds + ljb <= 96 * 1024) ? 1 : 0;
+  if (a.lj_lds) lds += ljb;
   for (int d = 0; d < 3; d++) {
     a.L[d] = (R)c->box[d];
     a.Linv[d] = (R)(1.0 / c->box[d]);
@@ -288,15 +345,22 @@ static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
   a.h = (R)c->spl_int;
   a.hinv = (R)(1.0 / c->spl_int);
   a.xmax = (R)(c->spl_int * (double)(c->nspl - 1));
-  int nblk = pgk_grid_blocks(c->ncl, 4);
+  int nblk = pgk_grid_blocks(c->ncl, NW);
   *off = c->red_used;
   *nrec = nblk;
   a.red = c->d_red + (size_t)c->red_used * PGK_REC;
   c->red_used += nblk;
-  if (split)
-    k_pairs<PREC, true><<<nblk, 256, 0, c->stream>>>(a);
-  else
-    k_pairs<PREC, false><<<nblk, 256, 0, c->stream>>>(a);
+  if (split) {
+    auto kf = k_pairs<PREC, true, NW>;
+    HIP_CHECK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    kf<<<nblk, 64 * NW, lds, c->stream>>>(a);
+  } else {
+    auto kf = k_pairs<PREC, false, NW>;
+    HIP_CHECK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    kf<<<nblk, 64 * NW, lds, c->stream>>>(a);
+  }
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

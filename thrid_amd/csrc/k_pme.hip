@@ -121,35 +121,57 @@ struct FftPlan {
   int rad[24];
 };
 
+// q = t / d, r = t % d for 0 <= t < 2^22 and 1 <= d < 2^16 without integer
+// division: a float reciprocal estimate and one correction step
+__device__ __forceinline__ int fdiv(int t, int d, float rd) {
+  int q = (int)((float)t * rd);
+  int r = t - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
+// One Stockham pass of radix R over nl lines (compile-time R keeps the
+// butterfly operands in registers).
+template <int R, class C>
+__device__ __forceinline__ void stockham_pass(const C *pa, C *pb, int nl, int n, int Ns,
+                                              const C *tw, int sgn) {
+  const int nb = n / R;
+  const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
+  const int tstride = n / (Ns * R);
+  for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
+    const int line = fdiv(t, nb, rnb), j = t - line * nb;
+    const C *src = pa + line * n;
+    C *dst = pb + line * n;
+    const int jq = fdiv(j, Ns, rns), k = j - jq * Ns;
+    C v[R];
+    v[0] = src[j];
+    const int tk = k * tstride;   // (q k tstride) < n: no wrap needed
+#pragma unroll
+    for (int q = 1; q < R; q++) {
+      C w = tw[q * tk];
+      if (sgn > 0) w.y = -w.y;
+      v[q] = cmul(src[j + q * nb], w);
+    }
+    dft(v, R, sgn);
+    const int idxD = jq * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; q++) dst[idxD + q * Ns] = v[q];
+  }
+}
+
 // Batched Stockham FFT of nl lines of length p.n held in LDS (line stride n).
 // Result ends in *pa (the buffers are swapped pass by pass).
 template <class C>
-__device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *__restrict__ tw,
-                        int sgn) {
+__device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *tw, int sgn) {
   const int n = p.n;
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int R = p.rad[s];
-    const int nb = n / R;
-    const int tstride = n / (Ns * R);
-    for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
-      const int line = t / nb, j = t - line * nb;
-      const C *src = pa + line * n;
-      C *dst = pb + line * n;
-      const int k = j % Ns;
-      C v[5];
-      for (int q = 0; q < R; q++) {
-        C x = src[j + q * nb];
-        if (q > 0) {
-          C w = tw[(q * k * tstride) % n];
-          if (sgn > 0) w.y = -w.y;
-          x = cmul(x, w);
-        }
-        v[q] = x;
-      }
-      dft(v, R, sgn);
-      const int idxD = (j / Ns) * Ns * R + k;
-      for (int q = 0; q < R; q++) dst[idxD + q * Ns] = v[q];
+    switch (R) {
+      case 4: stockham_pass<4>(pa, pb, nl, n, Ns, tw, sgn); break;
+      case 2: stockham_pass<2>(pa, pb, nl, n, Ns, tw, sgn); break;
+      case 3: stockham_pass<3>(pa, pb, nl, n, Ns, tw, sgn); break;
+      default: stockham_pass<5>(pa, pb, nl, n, Ns, tw, sgn); break;
     }
     __syncthreads();
     C *t = pa;
@@ -159,8 +181,6 @@ __device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *__res
   }
 }
 
-__device__ __forceinline__ double grid_in(const double *g, size_t i) { return g[i]; }
-__device__ __forceinline__ float grid_in(const float *g, size_t i) { return g[i]; }
 __device__ __forceinline__ float grid_in_fx(const unsigned long long *g, size_t i) {
   return (float)((double)(long long)g[i] * PGK_GRID_FX_INV);
 }
@@ -185,6 +205,33 @@ __device__ __forceinline__ void bspline(R w, int order, R *A, R *D) {
   A[order] = div * w * A[order - 1];
   for (int j = 1; j <= order - 2; j++)
     A[order - j] = div * ((w + R(j)) * A[order - j - 1] + (R(order - j) - w) * A[order - j]);
+  A[1] = div * (R(1) - w) * A[1];
+}
+
+// B-spline weights of compile-time order P (same recursion as bspline())
+template <class R, int P>
+__device__ __forceinline__ void bspline_t(R w, R *A, R *D) {
+  // A[1..P], D[1..P]
+  A[P] = R(0);
+  A[2] = w;
+  A[1] = R(1) - w;
+#pragma unroll
+  for (int k = 3; k <= P - 1; k++) {
+    const R div = R(1) / R(k - 1);
+    A[k] = div * w * A[k - 1];
+#pragma unroll
+    for (int j = 1; j <= k - 2; j++)
+      A[k - j] = div * ((w + R(j)) * A[k - j - 1] + (R(k - j) - w) * A[k - j]);
+    A[1] = div * (R(1) - w) * A[1];
+  }
+  D[1] = -A[1];
+#pragma unroll
+  for (int j = 2; j <= P; j++) D[j] = A[j - 1] - A[j];
+  const R div = R(1) / R(P - 1);
+  A[P] = div * w * A[P - 1];
+#pragma unroll
+  for (int j = 1; j <= P - 2; j++)
+    A[P - j] = div * ((w + R(j)) * A[P - j - 1] + (R(P - j) - w) * A[P - j]);
   A[1] = div * (R(1) - w) * A[1];
 }
 
@@ -213,36 +260,248 @@ __device__ __forceinline__ void atom_frac(const double *pos, int i, const PmeGeo
   }
 }
 
+// B-spline weights of every atom, in the spatially sorted cluster order when
+// a list exists (order[k] = atom or -1).  th: [k][6][p] = th1 th2 th3 dth1
+// dth2 dth3; base: [k] first grid index per dimension (int(fr) - p + 1 as in
+// fill_charge_grid, possibly negative before wrapping).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict__ pos,
-                                                const double *__restrict__ q, PmeGeom g,
-                                                void *grid) {
+__global__ __launch_bounds__(256) void k_bspline(int nitem, const int32_t *__restrict__ order,
+                                                 const double *__restrict__ pos, PmeGeom g,
+                                                 typename Prec<PREC>::real *__restrict__ th,
+                                                 int4 *__restrict__ base) {
   using R = typename Prec<PREC>::real;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitem) return;
+  int a = order ? order[k] : k;
+  int4 b;
+  b.w = a;
+  if (a < 0) {
+    b.x = b.y = b.z = 0;
+    base[k] = b;
+    return;
+  }
+  R t[3][PGK_MAXORD + 1], dt[3][PGK_MAXORD + 1];
   int i0[3];
-  atom_frac<R>(pos, i, g, i0, th, dth);
-  const R qi = (R)q[i];
+  atom_frac<R>(pos, a, g, i0, t, dt);
   const int p = g.order;
-  for (int a3 = 1; a3 <= p; a3++) {
-    int k = i0[2] + a3;
-    if (k < 0) k += g.K[2];
-    R p1 = th[2][a3] * qi;
-    for (int a2 = 1; a2 <= p; a2++) {
-      int j = i0[1] + a2;
-      if (j < 0) j += g.K[1];
-      R pr = th[1][a2] * p1;
-      size_t row = ((size_t)k * g.K[1] + j) * g.K[0];
-      for (int a1 = 1; a1 <= p; a1++) {
-        int ii = i0[0] + a1;
+  for (int d = 0; d < 3; d++)
+    for (int q = 0; q < p; q++) th[(size_t)(d * p + q) * nitem + k] = t[d][q + 1];
+  b.x = i0[0] + 1;
+  b.y = i0[1] + 1;
+  b.z = i0[2] + 1;
+  base[k] = b;
+}
+
+// one wavefront per atom, one lane per stencil point (x fastest)
+template <class R, int P>
+__device__ __forceinline__ void atom_frac_t(const double *pos, int i, const PmeGeom &g, int *i0,
+                                            R (*th)[P + 1], R (*dth)[P + 1]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double w = pos[3 * i + d] * g.recip[d];
+    w = w - floor(w);
+    double fr = (double)g.K[d] * w;
+    if (!(fr >= 0.0 && fr < (double)g.K[d])) fr = (fr >= (double)g.K[d]) ? (double)g.K[d] - 1.0 : 0.0;
+    int ifr = (int)fr;
+    R ww = (R)(fr - (double)ifr);
+    bspline_t<R, P>(ww, th[d], dth[d]);
+    i0[d] = ifr - P;
+  }
+}
+
+template <int PREC, int P>
+__global__ __launch_bounds__(256) void k_bspline_t(int nitem, const int32_t *__restrict__ order,
+                                                   const double *__restrict__ pos, PmeGeom g,
+                                                   typename Prec<PREC>::real *__restrict__ th,
+                                                   int4 *__restrict__ base) {
+  using R = typename Prec<PREC>::real;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitem) return;
+  int a = order ? order[k] : k;
+  int4 b;
+  b.w = a;
+  if (a < 0) {
+    b.x = b.y = b.z = 0;
+    base[k] = b;
+    return;
+  }
+  R t[3][P + 1], dt[3][P + 1];
+  int i0[3];
+  atom_frac_t<R, P>(pos, a, g, i0, t, dt);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int q = 0; q < P; q++) th[(size_t)(d * P + q) * nitem + k] = t[d][q + 1];
+  b.x = i0[0] + 1;
+  b.y = i0[1] + 1;
+  b.z = i0[2] + 1;
+  base[k] = b;
+}
+
+template <int PREC, bool SPLIT, int P>
+__global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__restrict__ order,
+                                                  const double *__restrict__ pos,
+                                                  const double *__restrict__ q, PmeGeom g,
+                                                  const void *gridv,
+                                                  typename Prec<PREC>::acc *fat,
+                                                  typename Prec<PREC>::acc *fat_c) {
+  using R = typename Prec<PREC>::real;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitem) return;
+  const int i = order ? order[k] : k;
+  if (i < 0) return;
+  const R *grid = reinterpret_cast<const R *>(gridv);
+  R th[3][P + 1], dth[3][P + 1];
+  int i0[3];
+  atom_frac_t<R, P>(pos, i, g, i0, th, dth);
+  R f1 = 0, f2 = 0, f3 = 0;
+  const R qi = (R)q[i];
+  const R K1 = (R)g.K[0], K2 = (R)g.K[1], K3 = (R)g.K[2];
+  int ix[P];
+#pragma unroll
+  for (int a1 = 1; a1 <= P; a1++) {
+    int ii = i0[0] + a1;
+    ix[a1 - 1] = (ii < 0) ? ii + g.K[0] : ii;
+  }
+#pragma unroll
+  for (int a3 = 1; a3 <= P; a3++) {
+    int kk = i0[2] + a3;
+    if (kk < 0) kk += g.K[2];
+#pragma unroll
+    for (int a2 = 1; a2 <= P; a2++) {
+      int jj = i0[1] + a2;
+      if (jj < 0) jj += g.K[1];
+      const R *row = grid + ((size_t)kk * g.K[1] + jj) * g.K[0];
+      const R t23 = th[1][a2] * th[2][a3];
+      const R d23 = dth[1][a2] * th[2][a3];
+      const R t2d3 = th[1][a2] * dth[2][a3];
+#pragma unroll
+      for (int a1 = 1; a1 <= P; a1++) {
+        const R term = qi * row[ix[a1 - 1]];
+        f1 = f1 - K1 * term * dth[0][a1] * t23;
+        f2 = f2 - K2 * term * th[0][a1] * d23;
+        f3 = f3 - K3 * term * th[0][a1] * t2d3;
+      }
+    }
+  }
+  double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
+  acc_add(&fat[3 * i + 0], fx);
+  acc_add(&fat[3 * i + 1], fy);
+  acc_add(&fat[3 * i + 2], fz);
+  if (SPLIT) {
+    acc_add(&fat_c[3 * i + 0], fx);
+    acc_add(&fat_c[3 * i + 1], fy);
+    acc_add(&fat_c[3 * i + 2], fz);
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_spread_w(int nitem, const double *__restrict__ q,
+                                                  PmeGeom g,
+                                                  const typename Prec<PREC>::real *__restrict__ th,
+                                                  const int4 *__restrict__ base, void *grid) {
+  using R = typename Prec<PREC>::real;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= nitem) return;
+  const int4 b = base[k];
+  if (b.w < 0) return;
+  const int p = g.order, p2 = p * p, p3 = p2 * p;
+  const R qi = (R)q[b.w];
+  const R *t = th + k;
+  const size_t st = (size_t)nitem;
+  for (int pt = lane; pt < p3; pt += 64) {
+    const int a3 = pt / p2, rem = pt - a3 * p2;
+    const int a2 = rem / p, a1 = rem - a2 * p;
+    int kk = b.z + a3, jj = b.y + a2, ii = b.x + a1;
+    if (kk < 0) kk += g.K[2];
+    if (jj < 0) jj += g.K[1];
+    if (ii < 0) ii += g.K[0];
+    const R v = t[a1 * st] * (t[(p + a2) * st] * (t[(2 * p + a3) * st] * qi));
+    const size_t idx = ((size_t)kk * g.K[1] + jj) * g.K[0] + ii;
+    if (PREC == PGK_PREC_FP64) {
+      atomicAdd(reinterpret_cast<double *>(grid) + idx, (double)v);
+    } else if (PREC == PGK_PREC_MIXED) {
+      long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
+      atomicAdd(reinterpret_cast<unsigned long long *>(grid) + idx, (unsigned long long)x);
+    } else {
+      atomicAdd(reinterpret_cast<float *>(grid) + idx, (float)v);
+    }
+  }
+}
+
+// four atoms per wavefront, 16 lanes per atom (one x-y plane of the stencil
+// per step over a3); p = 4 only
+template <int PREC>
+__global__ __launch_bounds__(256) void k_spread_q(int nitem, const double *__restrict__ q,
+                                                  PmeGeom g,
+                                                  const typename Prec<PREC>::real *__restrict__ th,
+                                                  const int4 *__restrict__ base, void *grid) {
+  using R = typename Prec<PREC>::real;
+  const int lane = threadIdx.x & 63;
+  const int k = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
+  if (k >= nitem) return;
+  const int4 b = base[k];
+  if (b.w < 0) return;
+  const int sub = lane & 15, a1 = sub & 3, a2 = sub >> 2;
+  const R qi = (R)q[b.w];
+  const size_t st = (size_t)nitem;
+  const R *t = th + k;
+  int ii = b.x + a1, jj = b.y + a2;
+  if (ii < 0) ii += g.K[0];
+  if (jj < 0) jj += g.K[1];
+  const R w12 = t[a1 * st] * t[(4 + a2) * st] * qi;
+#pragma unroll
+  for (int a3 = 0; a3 < 4; a3++) {
+    int kk = b.z + a3;
+    if (kk < 0) kk += g.K[2];
+    const R v = w12 * t[(8 + a3) * st];
+    const size_t idx = ((size_t)kk * g.K[1] + jj) * g.K[0] + ii;
+    if (PREC == PGK_PREC_FP64) {
+      atomicAdd(reinterpret_cast<double *>(grid) + idx, (double)v);
+    } else if (PREC == PGK_PREC_MIXED) {
+      long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
+      atomicAdd(reinterpret_cast<unsigned long long *>(grid) + idx, (unsigned long long)x);
+    } else {
+      atomicAdd(reinterpret_cast<float *>(grid) + idx, (float)v);
+    }
+  }
+}
+
+// thread per atom in the sorted order, all p^3 points in sequence
+template <int PREC>
+__global__ __launch_bounds__(256) void k_spread_t(int nitem, const double *__restrict__ q,
+                                                  PmeGeom g,
+                                                  const typename Prec<PREC>::real *__restrict__ th,
+                                                  const int4 *__restrict__ base, void *grid) {
+  using R = typename Prec<PREC>::real;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitem) return;
+  const int4 b = base[k];
+  if (b.w < 0) return;
+  const int p = g.order;
+  const R qi = (R)q[b.w];
+  const size_t st = (size_t)nitem;
+  const R *t = th + k;
+  for (int a3 = 0; a3 < p; a3++) {
+    int kk = b.z + a3;
+    if (kk < 0) kk += g.K[2];
+    const R p3 = t[(2 * p + a3) * st] * qi;
+    for (int a2 = 0; a2 < p; a2++) {
+      int jj = b.y + a2;
+      if (jj < 0) jj += g.K[1];
+      const R p2 = t[(p + a2) * st] * p3;
+      const size_t row = ((size_t)kk * g.K[1] + jj) * g.K[0];
+      for (int a1 = 0; a1 < p; a1++) {
+        int ii = b.x + a1;
         if (ii < 0) ii += g.K[0];
-        R v = th[0][a1] * pr;
+        const R v = t[a1 * st] * p2;
         if (PREC == PGK_PREC_FP64) {
           atomicAdd(reinterpret_cast<double *>(grid) + row + ii, (double)v);
         } else if (PREC == PGK_PREC_MIXED) {
           long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
-          atomicAdd(reinterpret_cast<unsigned long long *>(grid) + row + ii, (unsigned long long)x);
+          atomicAdd(reinterpret_cast<unsigned long long *>(grid) + row + ii,
+                    (unsigned long long)x);
         } else {
           atomicAdd(reinterpret_cast<float *>(grid) + row + ii, (float)v);
         }
@@ -251,10 +510,59 @@ __global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict_
   }
 }
 
+// thread per atom (sorted order), weights from k_bspline (grad_sum)
+template <int PREC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_gather_s(int nitem, const int32_t *__restrict__ order,
+                                                  const double *__restrict__ pos,
+                                                  const double *__restrict__ q, PmeGeom g,
+                                                  const void *gridv,
+                                                  typename Prec<PREC>::acc *fat,
+                                                  typename Prec<PREC>::acc *fat_c) {
+  using R = typename Prec<PREC>::real;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitem) return;
+  const int i = order ? order[k] : k;
+  if (i < 0) return;
+  const R *grid = reinterpret_cast<const R *>(gridv);
+  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
+  int i0[3];
+  atom_frac<R>(pos, i, g, i0, th, dth);
+  const int p = g.order;
+  R f1 = 0, f2 = 0, f3 = 0;
+  const R qi = (R)q[i];
+  const R K1 = (R)g.K[0], K2 = (R)g.K[1], K3 = (R)g.K[2];
+  for (int a3 = 1; a3 <= p; a3++) {
+    int kk = i0[2] + a3;
+    if (kk < 0) kk += g.K[2];
+    for (int a2 = 1; a2 <= p; a2++) {
+      int jj = i0[1] + a2;
+      if (jj < 0) jj += g.K[1];
+      const size_t row = ((size_t)kk * g.K[1] + jj) * g.K[0];
+      for (int a1 = 1; a1 <= p; a1++) {
+        int ii = i0[0] + a1;
+        if (ii < 0) ii += g.K[0];
+        const R term = qi * grid[row + ii];
+        f1 = f1 - K1 * term * dth[0][a1] * th[1][a2] * th[2][a3];
+        f2 = f2 - K2 * term * th[0][a1] * dth[1][a2] * th[2][a3];
+        f3 = f3 - K3 * term * th[0][a1] * th[1][a2] * dth[2][a3];
+      }
+    }
+  }
+  double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
+  acc_add(&fat[3 * i + 0], fx);
+  acc_add(&fat[3 * i + 1], fy);
+  acc_add(&fat[3 * i + 2], fz);
+  if (SPLIT) {
+    acc_add(&fat_c[3 * i + 0], fx);
+    acc_add(&fat_c[3 * i + 1], fy);
+    acc_add(&fat_c[3 * i + 2], fz);
+  }
+}
+
 // x pass, forward: pairs of real rows -> half spectra.  One block handles NL
 // complex lines (2*NL rows).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid, int nrow,
+__global__ __launch_bounds__(256) void k_fft_x_fwd(void *grid, void *cgrid, int nrow,
                                                     int NL, FftPlan p, const void *twv, int H) {
   using R = typename Prec<PREC>::real;
   using C = typename C2T<R>::type;
@@ -262,28 +570,36 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid
   const int n = p.n;
   C *pa = reinterpret_cast<C *>(smem);
   C *pb = pa + NL * n;
-  const C *tw = reinterpret_cast<const C *>(twv);
+  C *tw = pb + NL * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   const int row0 = blockIdx.x * 2 * NL;
+  const float rn = 1.0f / (float)n;
   for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-    int l = t / n, k = t - l * n;
+    int l = fdiv(t, n, rn), k = t - l * n;
     int ra = row0 + 2 * l, rb = ra + 1;
     C z;
+    // the charge grid is consumed here and left zeroed for the next spread
     if (PREC == PGK_PREC_MIXED) {
-      const unsigned long long *gi = reinterpret_cast<const unsigned long long *>(grid);
+      unsigned long long *gi = reinterpret_cast<unsigned long long *>(grid);
       z.x = (ra < nrow) ? grid_in_fx(gi, (size_t)ra * n + k) : R(0);
       z.y = (rb < nrow) ? grid_in_fx(gi, (size_t)rb * n + k) : R(0);
+      if (ra < nrow) gi[(size_t)ra * n + k] = 0ull;
+      if (rb < nrow) gi[(size_t)rb * n + k] = 0ull;
     } else {
-      const R *gr = reinterpret_cast<const R *>(grid);
+      R *gr = reinterpret_cast<R *>(grid);
       z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
       z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
+      if (ra < nrow) gr[(size_t)ra * n + k] = R(0);
+      if (rb < nrow) gr[(size_t)rb * n + k] = R(0);
     }
     pa[t] = z;
   }
   __syncthreads();
   fft_lds<C>(pa, pb, NL, p, tw, -1);
   C *cg = reinterpret_cast<C *>(cgrid);
+  const float rH = 1.0f / (float)H;
   for (int t = threadIdx.x; t < NL * H; t += blockDim.x) {
-    int l = t / H, k = t - l * H;
+    int l = fdiv(t, H, rH), k = t - l * H;
     int ra = row0 + 2 * l, rb = ra + 1;
     C zk = pa[l * n + k];
     C zn = pa[l * n + ((n - k) % n)];
@@ -308,11 +624,13 @@ __global__ __launch_bounds__(256) void k_fft_x_bwd(const void *cgrid, void *grid
   const int n = p.n;
   C *pa = reinterpret_cast<C *>(smem);
   C *pb = pa + NL * n;
-  const C *tw = reinterpret_cast<const C *>(twv);
+  C *tw = pb + NL * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   const C *cg = reinterpret_cast<const C *>(cgrid);
   const int row0 = blockIdx.x * 2 * NL;
+  const float rn = 1.0f / (float)n;
   for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-    int l = t / n, k = t - l * n;
+    int l = fdiv(t, n, rn), k = t - l * n;
     int ra = row0 + 2 * l, rb = ra + 1;
     C A, B;
     A.x = A.y = B.x = B.y = R(0);
@@ -334,7 +652,7 @@ __global__ __launch_bounds__(256) void k_fft_x_bwd(const void *cgrid, void *grid
   fft_lds<C>(pa, pb, NL, p, tw, +1);
   R *gr = reinterpret_cast<R *>(grid);
   for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-    int l = t / n, k = t - l * n;
+    int l = fdiv(t, n, rn), k = t - l * n;
     int ra = row0 + 2 * l, rb = ra + 1;
     C z = pa[t];
     if (ra < nrow) gr[(size_t)ra * n + k] = z.x;
@@ -352,13 +670,15 @@ __global__ __launch_bounds__(256) void k_fft_y(void *cgrid, int H, int K2, int T
   const int n = p.n;   // == K2
   C *pa = reinterpret_cast<C *>(smem);
   C *pb = pa + TW * n;
-  const C *tw = reinterpret_cast<const C *>(twv);
+  C *tw = pb + TW * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   C *cg = reinterpret_cast<C *>(cgrid);
   const int ntile = (H + TW - 1) / TW;
   const int k3 = blockIdx.x / ntile;
   const int h0 = (blockIdx.x - k3 * ntile) * TW;
+  const int lt = __ffs(TW) - 1;   // TW is a power of two
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k2 = t / TW, l = t - k2 * TW;
+    int k2 = t >> lt, l = t & (TW - 1);
     C z;
     z.x = z.y = R(0);
     if (h0 + l < H) z = cg[((size_t)k3 * K2 + k2) * H + h0 + l];
@@ -367,7 +687,7 @@ __global__ __launch_bounds__(256) void k_fft_y(void *cgrid, int H, int K2, int T
   __syncthreads();
   fft_lds<C>(pa, pb, TW, p, tw, sgn);
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k2 = t / TW, l = t - k2 * TW;
+    int k2 = t >> lt, l = t & (TW - 1);
     if (h0 + l < H) cg[((size_t)k3 * K2 + k2) * H + h0 + l] = pa[l * n + k2];
   }
 }
@@ -393,14 +713,16 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   const int H = ca.H, K2 = ca.K[1];
   C *pa = reinterpret_cast<C *>(smem);
   C *pb = pa + TW * n;
-  const C *tw = reinterpret_cast<const C *>(twv);
+  C *tw = pb + TW * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   C *cg = reinterpret_cast<C *>(cgrid);
   const int ntile = (H + TW - 1) / TW;
   const int k2 = blockIdx.x / ntile;
   const int h0 = (blockIdx.x - k2 * ntile) * TW;
   const size_t zstride = (size_t)K2 * H;
+  const int lt = __ffs(TW) - 1;   // TW is a power of two
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k3 = t / TW, l = t - k3 * TW;
+    int k3 = t >> lt, l = t & (TW - 1);
     C z;
     z.x = z.y = R(0);
     if (h0 + l < H) z = cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l];
@@ -415,8 +737,9 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   const int m2 = (k2 < nf2) ? k2 : k2 - K2;
   const double mh2 = ca.recip[1] * m2;
   const double s2 = ((K2 & 1) == 0 && k2 == K2 / 2) ? 1.0 : -1.0;
+  const float rn = 1.0f / (float)n;
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int l = t / n, k3 = t - l * n;
+    int l = fdiv(t, n, rn), k3 = t - l * n;
     int h = h0 + l;
     if (h >= H) continue;
     if (h == 0 && k2 == 0 && k3 == 0) continue;   // m = 0 left untouched
@@ -459,55 +782,10 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   __syncthreads();
   fft_lds<C>(pa, pb, TW, p, tw, +1);
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k3 = t / TW, l = t - k3 * TW;
+    int k3 = t >> lt, l = t & (TW - 1);
     if (h0 + l < H) cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l] = pa[l * n + k3];
   }
   block_record<8, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
-}
-
-template <int PREC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_gather(int n, const double *__restrict__ pos,
-                                                const double *__restrict__ q, PmeGeom g,
-                                                const void *gridv,
-                                                typename Prec<PREC>::acc *fat,
-                                                typename Prec<PREC>::acc *fat_c) {
-  using R = typename Prec<PREC>::real;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const R *grid = reinterpret_cast<const R *>(gridv);
-  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
-  int i0[3];
-  atom_frac<R>(pos, i, g, i0, th, dth);
-  const int p = g.order;
-  R f1 = 0, f2 = 0, f3 = 0;
-  const R qi = (R)q[i];
-  const R K1 = (R)g.K[0], K2 = (R)g.K[1], K3 = (R)g.K[2];
-  for (int a3 = 1; a3 <= p; a3++) {
-    int k = i0[2] + a3;
-    if (k < 0) k += g.K[2];
-    for (int a2 = 1; a2 <= p; a2++) {
-      int j = i0[1] + a2;
-      if (j < 0) j += g.K[1];
-      size_t row = ((size_t)k * g.K[1] + j) * g.K[0];
-      for (int a1 = 1; a1 <= p; a1++) {
-        int ii = i0[0] + a1;
-        if (ii < 0) ii += g.K[0];
-        R term = qi * grid[row + ii];
-        f1 = f1 - K1 * term * dth[0][a1] * th[1][a2] * th[2][a3];
-        f2 = f2 - K2 * term * th[0][a1] * dth[1][a2] * th[2][a3];
-        f3 = f3 - K3 * term * th[0][a1] * th[1][a2] * dth[2][a3];
-      }
-    }
-  }
-  double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
-  acc_add(&fat[3 * i + 0], fx);
-  acc_add(&fat[3 * i + 1], fy);
-  acc_add(&fat[3 * i + 2], fz);
-  if (SPLIT) {
-    acc_add(&fat_c[3 * i + 0], fx);
-    acc_add(&fat_c[3 * i + 1], fy);
-    acc_add(&fat_c[3 * i + 2], fz);
-  }
 }
 
 FftPlan make_plan(int n) {
@@ -525,7 +803,7 @@ FftPlan make_plan(int n) {
 
 int lines_for(int n, size_t csize, int maxl) {
   int nl = maxl;
-  while (nl > 1 && (size_t)2 * nl * n * csize > 65536) nl >>= 1;
+  while (nl > 1 && (size_t)(2 * nl + 1) * n * csize > 65536) nl >>= 1;
   return nl;
 }
 
@@ -547,9 +825,44 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   size_t G = (size_t)K1 * K2 * K3;
   size_t gel = (PREC == PGK_PREC_FP32) ? 4 : 8;
   pgk_prof_begin(c, PGK_PROF_SPREAD);
-  HIP_CHECK(hipMemsetAsync(c->d_grid, 0, G * gel, st));
+  (void)G;
+  (void)gel;
   const int n = c->natom;
-  k_spread<PREC><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g, c->d_grid);
+  // B-spline weights in the list's spatial order (fallback: atom order)
+  const bool sorted = c->have_list && c->d_cl_atom;
+  const int nitem = sorted ? c->ncl * PGK_CL : n;
+  {
+    size_t need = (size_t)nitem * 3 * c->order * sizeof(R) + (size_t)nitem * sizeof(int4);
+    if (need > c->theta_bytes) {
+      if (c->d_theta) (void)hipFree(c->d_theta);
+      c->d_theta = nullptr;
+      HIP_CHECK(hipMalloc(&c->d_theta, need));
+      c->theta_bytes = need;
+    }
+  }
+  R *th = reinterpret_cast<R *>(c->d_theta);
+  int4 *base = reinterpret_cast<int4 *>(reinterpret_cast<char *>(c->d_theta) +
+                                        (size_t)nitem * 3 * c->order * sizeof(R));
+  {
+    const int32_t *ord = sorted ? c->d_cl_atom : nullptr;
+    const int gb = pgk_grid_blocks(nitem, 256);
+    switch (c->order) {
+      case 4: k_bspline_t<PREC, 4><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
+      case 5: k_bspline_t<PREC, 5><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
+      case 6: k_bspline_t<PREC, 6><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
+      case 8: k_bspline_t<PREC, 8><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
+      default: k_bspline<PREC><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base);
+    }
+  }
+  if (c->spread_variant == 1 && c->order == 4)
+    k_spread_q<PREC><<<pgk_grid_blocks(nitem, 16), 256, 0, st>>>(nitem, c->d_q, g, th, base,
+                                                                 c->d_grid);
+  else if (c->spread_variant == 2)
+    k_spread_t<PREC><<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, c->d_q, g, th, base,
+                                                                  c->d_grid);
+  else
+    k_spread_w<PREC><<<pgk_grid_blocks(nitem, 4), 256, 0, st>>>(nitem, c->d_q, g, th, base,
+                                                                c->d_grid);
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_SPREAD);
   pgk_prof_begin(c, PGK_PROF_FFT);
@@ -560,12 +873,12 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   int nrow = K2 * K3;
   int nlx = lines_for(K1, csz, 16);
   int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
-  k_fft_x_fwd<PREC><<<bx, 256, 2 * nlx * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow, nlx, p1,
+  k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow, nlx, p1,
                                                            twc + c->tw_off[0], H);
   HIP_CHECK(hipGetLastError());
   int twy = lines_for(K2, csz, 16);
   int by = K3 * ((H + twy - 1) / twy);
-  k_fft_y<PREC><<<by, 256, 2 * twy * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
+  k_fft_y<PREC><<<by, 256, (2 * twy + 1) * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
                                                        twc + c->tw_off[1], -1);
   HIP_CHECK(hipGetLastError());
   ConvArgs ca;
@@ -587,24 +900,47 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   *nrec = bz;
   double *red = c->d_red + (size_t)c->red_used * PGK_REC;
   c->red_used += bz;
-  k_fft_z_conv<PREC><<<bz, 256, 2 * twz * K3 * csz, st>>>(c->d_cgrid, twz, p3,
+  k_fft_z_conv<PREC><<<bz, 256, (2 * twz + 1) * K3 * csz, st>>>(c->d_cgrid, twz, p3,
                                                             twc + c->tw_off[2], ca, red);
   HIP_CHECK(hipGetLastError());
-  k_fft_y<PREC><<<by, 256, 2 * twy * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
+  k_fft_y<PREC><<<by, 256, (2 * twy + 1) * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
                                                        twc + c->tw_off[1], +1);
   // potential grid back into the real grid buffer (compute type)
-  k_fft_x_bwd<PREC><<<bx, 256, 2 * nlx * K1 * csz, st>>>(c->d_cgrid, c->d_grid, nrow, nlx, p1,
+  k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_pgrid, nrow, nlx, p1,
                                                            twc + c->tw_off[0], H);
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_FFT);
   pgk_prof_begin(c, PGK_PROF_GATHER);
   A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
-  if (split)
-    k_gather<PREC, true><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g,
-                                                                  c->d_grid, fat, fatc);
-  else
-    k_gather<PREC, false><<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_pos, c->d_q, g,
-                                                                   c->d_grid, fat, fatc);
+  {
+    const int32_t *ord = sorted ? c->d_cl_atom : nullptr;
+    const int gb = pgk_grid_blocks(nitem, 256);
+#define PGK_GATHER(SP, P)                                                                 \
+  k_gather_t<PREC, SP, P><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid, \
+                                              fat, fatc)
+    if (split) {
+      switch (c->order) {
+        case 4: PGK_GATHER(true, 4); break;
+        case 5: PGK_GATHER(true, 5); break;
+        case 6: PGK_GATHER(true, 6); break;
+        case 8: PGK_GATHER(true, 8); break;
+        default:
+          k_gather_s<PREC, true><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid,
+                                                     fat, fatc);
+      }
+    } else {
+      switch (c->order) {
+        case 4: PGK_GATHER(false, 4); break;
+        case 5: PGK_GATHER(false, 5); break;
+        case 6: PGK_GATHER(false, 6); break;
+        case 8: PGK_GATHER(false, 8); break;
+        default:
+          k_gather_s<PREC, false><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid,
+                                                      fat, fatc);
+      }
+    }
+#undef PGK_GATHER
+  }
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_GATHER);
   return PGK_OK;
@@ -628,6 +964,12 @@ int pgk_pme_prepare(pgk_ctx *c) {
     pgk_set_error("pme_order must be in 3..10");
     return PGK_EINVAL;
   }
+  for (int d = 0; d < 3; d++)
+    if (c->nfft[d] < c->order) {
+      // fill_charge_grid wraps a stencil index at most once (fft_pme.F90:393-411)
+      pgk_set_error("SPME grid must have at least pme_order points per axis");
+      return PGK_EINVAL;
+    }
   return PGK_OK;
 }
 

@@ -8,6 +8,7 @@
 // and pgk_calforce orchestrates the device kernels in calforce's order with
 // its output semantics (calforce.F90:307-1068).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -194,45 +195,52 @@ __global__ void k_pack(int nslot, const int32_t *__restrict__ cl_atom,
 
 template <class A>
 __global__ void k_combine(int n, const int32_t *__restrict__ slot_of, int use_cl,
-                          const A *__restrict__ fcl, const A *__restrict__ fat,
-                          const A *__restrict__ fclc, const A *__restrict__ fatc,
-                          double *__restrict__ force, double *__restrict__ fvc,
-                          double *__restrict__ fvl) {
+                          A *__restrict__ fcl, A *__restrict__ fat, A *__restrict__ fclc,
+                          A *__restrict__ fatc, double *__restrict__ force,
+                          double *__restrict__ fvc, double *__restrict__ fvl) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int s = use_cl ? slot_of[i] : -1;
+  // every accumulator is read once here and left zeroed for the next call
   for (int d = 0; d < 3; d++) {
-    double fp = (s >= 0) ? acc_load(&fcl[3 * s + d]) : 0.0;
+    double fp = 0.0;
+    if (s >= 0) {
+      fp = acc_load(&fcl[3 * s + d]);
+      fcl[3 * s + d] = A(0);
+    }
     force[3 * i + d] = fp + acc_load(&fat[3 * i + d]);
+    fat[3 * i + d] = A(0);
     if (fvc) {
-      double fc = (s >= 0) ? acc_load(&fclc[3 * s + d]) : 0.0;
+      double fc = 0.0;
+      if (s >= 0) {
+        fc = acc_load(&fclc[3 * s + d]);
+        fclc[3 * s + d] = A(0);
+      }
       fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]);
+      fatc[3 * i + d] = A(0);
       fvl[3 * i + d] = fp - fc;
     }
   }
 }
 
-// deterministic segment sums: one block per segment, fixed-order tree
+// deterministic segment sums: one block per (segment, field), fixed-order tree
 __global__ __launch_bounds__(256) void k_reduce_segments(const double *__restrict__ red,
                                                          const int *__restrict__ seg_off,
                                                          const int *__restrict__ seg_n,
                                                          double *__restrict__ out) {
   __shared__ double s[256];
-  int sg = blockIdx.x;
-  const double *r = red + (size_t)seg_off[sg] * PGK_REC;
-  int n = seg_n[sg];
-  for (int f = 0; f < PGK_REC; f++) {
-    double v = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) v += r[(size_t)k * PGK_REC + f];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[sg * PGK_REC + f] = s[0];
+  const int sg = blockIdx.x / PGK_REC, f = blockIdx.x - sg * PGK_REC;
+  const double *r = red + (size_t)seg_off[sg] * PGK_REC + f;
+  const int n = seg_n[sg];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v += r[(size_t)k * PGK_REC];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[sg * PGK_REC + f] = s[0];
 }
 
 int set_device_ctx(pgk_ctx *c) {
@@ -289,12 +297,20 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   c->red_used = 0;
   int seg_off[16], seg_n[16], seg_kind[16], nseg = 0;
 
-  HIP_CHECK(hipMemsetAsync(c->d_fat, 0, (size_t)n * 3 * ab, st));
-  if (split) HIP_CHECK(hipMemsetAsync(c->d_fat_c, 0, (size_t)n * 3 * ab, st));
+  if (c->acc_dirty) {   // first call, a new list size, or an interrupted call
+    HIP_CHECK(hipMemsetAsync(c->d_fat, 0, (size_t)n * 3 * ab, st));
+    HIP_CHECK(hipMemsetAsync(c->d_fat_c, 0, (size_t)n * 3 * ab, st));
+    if (c->ncl_cap > 0) {
+      size_t nsc = (size_t)c->ncl_cap * PGK_CL;
+      HIP_CHECK(hipMemsetAsync(c->d_fcl, 0, nsc * 3 * ab, st));
+      HIP_CHECK(hipMemsetAsync(c->d_fcl_c, 0, nsc * 3 * ab, st));
+    }
+    size_t G = (size_t)c->nfft[0] * c->nfft[1] * c->nfft[2];
+    HIP_CHECK(hipMemsetAsync(c->d_grid, 0, G * 8, st));
+  }
+  c->acc_dirty = true;
   if (do_pair) {
     size_t ns = (size_t)c->ncl * PGK_CL;
-    HIP_CHECK(hipMemsetAsync(c->d_fcl, 0, ns * 3 * ab, st));
-    if (split) HIP_CHECK(hipMemsetAsync(c->d_fcl_c, 0, ns * 3 * ab, st));
     if (c->precision == PGK_PREC_FP64)
       k_pack<double4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
           (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (double4 *)c->d_xq, c->d_ct);
@@ -328,21 +344,21 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     switch (c->precision) {
       case PGK_PREC_FP64:
         k_combine<double><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
-                                             (const double *)c->d_fcl, (const double *)c->d_fat,
-                                             (const double *)c->d_fcl_c,
-                                             (const double *)c->d_fat_c, fout, fvc, fvl);
+                                             (double *)c->d_fcl, (double *)c->d_fat,
+                                             (double *)c->d_fcl_c, (double *)c->d_fat_c, fout,
+                                             fvc, fvl);
         break;
       case PGK_PREC_MIXED:
         k_combine<unsigned long long><<<g, 256, 0, st>>>(
-            n, c->d_slot_of, do_pair ? 1 : 0, (const unsigned long long *)c->d_fcl,
-            (const unsigned long long *)c->d_fat, (const unsigned long long *)c->d_fcl_c,
-            (const unsigned long long *)c->d_fat_c, fout, fvc, fvl);
+            n, c->d_slot_of, do_pair ? 1 : 0, (unsigned long long *)c->d_fcl,
+            (unsigned long long *)c->d_fat, (unsigned long long *)c->d_fcl_c,
+            (unsigned long long *)c->d_fat_c, fout, fvc, fvl);
         break;
       default:
         k_combine<float><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
-                                            (const float *)c->d_fcl, (const float *)c->d_fat,
-                                            (const float *)c->d_fcl_c, (const float *)c->d_fat_c,
-                                            fout, fvc, fvl);
+                                            (float *)c->d_fcl, (float *)c->d_fat,
+                                            (float *)c->d_fcl_c, (float *)c->d_fat_c, fout, fvc,
+                                            fvl);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -356,12 +372,13 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       hseg[16 + k] = seg_n[k];
     }
     HIP_CHECK(hipMemcpyAsync(dseg, hseg, 32 * sizeof(int), hipMemcpyHostToDevice, st));
-    k_reduce_segments<<<nseg, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
+    k_reduce_segments<<<nseg * PGK_REC, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
                              hipMemcpyDeviceToHost, st));
   }
   HIP_CHECK(hipStreamSynchronize(st));
+  c->acc_dirty = false;
   pgk_prof_collect(c);
   for (int k = 0; k < nseg; k++) memcpy(segsum[k], c->h_sum + k * PGK_REC, sizeof(segsum[k]));
 
@@ -490,6 +507,7 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
     }
   pgk_ctx *c = new pgk_ctx();
   c->precision = precision;
+  if (const char *v = getenv("PGK_SPREAD_VARIANT")) c->spread_variant = atoi(v);
   (void)hipGetDevice(&c->device);
   c->natom = n;
   c->ntype = nt;
@@ -608,7 +626,9 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   std::vector<double> ljp64((size_t)2 * nt * nt);
   std::vector<float> ljp32((size_t)2 * nt * nt);
   for (size_t k = 0; k < (size_t)nt * nt; k++) {
-    ljp64[2 * k] = c->lj_eps[k];
+    // a type pair that is not INTTYPE_VDW contributes no LJ (ewaldr.F90:260):
+    // fold that into a zero well depth
+    ljp64[2 * k] = (c->inttyp[k] == 1) ? c->lj_eps[k] : 0.0;
     ljp64[2 * k + 1] = c->lj_rmin[k] * c->lj_rmin[k];
     ljp32[2 * k] = (float)ljp64[2 * k];
     ljp32[2 * k + 1] = (float)ljp64[2 * k + 1];
@@ -733,6 +753,7 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   if (he == hipSuccess) he = hipMalloc((void **)&c->d_fvc, sizeof(double) * 3 * n);
   if (he == hipSuccess) he = hipMalloc((void **)&c->d_fvl, sizeof(double) * 3 * n);
   if (he == hipSuccess) he = hipMalloc(&c->d_grid, G * 8);
+  if (he == hipSuccess) he = hipMalloc(&c->d_pgrid, G * ab);
   if (he == hipSuccess) he = hipMalloc(&c->d_cgrid, Hc * csz);
   if (he == hipSuccess) he = hipMalloc((void **)&c->d_sum, sizeof(double) * 1024);
   if (he == hipSuccess) he = hipHostMalloc((void **)&c->h_sum, sizeof(double) * 1024);
@@ -758,7 +779,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
                   c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
-                  c->d_grid, c->d_cgrid, c->d_tw, c->d_twf};
+                  c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -927,6 +948,19 @@ int pgk_profile(pgk_ctx *c, int32_t enable, double *ms, int64_t *calls) {
       c->prof_calls[k] = 0;
     }
   }
+  return PGK_OK;
+}
+
+int pgk_list_info(pgk_ctx *c, int64_t *info) {
+  if (!c || !info) return PGK_EINVAL;
+  if (!c->have_list) {
+    pgk_set_error("pgk_list_info: no list");
+    return PGK_ESTATE;
+  }
+  info[0] = c->ncl;
+  info[1] = c->npair_cl;
+  info[2] = c->npairs;
+  info[3] = PGK_CL;
   return PGK_OK;
 }
 

@@ -152,14 +152,23 @@ struct pgk_ctx {
   double *h_sum = nullptr;         // pinned R_N
 
   // PME
-  void *d_grid = nullptr;          // real grid (n1*n2*n3) in compute type, or int64
+  void *d_grid = nullptr;          // charge grid (n1*n2*n3): double, int64 (mixed) or float;
+                                   // zero between calls (the x pass clears it)
+  void *d_pgrid = nullptr;         // potential grid (n1*n2*n3) in the compute type
   void *d_cgrid = nullptr;         // complex half grid ((n1/2+1)*n2*n3)
+  void *d_theta = nullptr;         // B-spline weights + base indices per atom
+  size_t theta_bytes = 0;
   double *d_tw = nullptr;          // twiddles: 3 dims, each n complex (FP64)
   float *d_twf = nullptr;
   int tw_off[3] = {0, 0, 0};
 
   // reduction segment bookkeeping for the current call
   int red_used = 0;
+  // accumulators (force sums, charge grid) are cleared by the kernels that
+  // consume them; set when a call did not run to completion
+  bool acc_dirty = true;
+  // kernel variant selectors (PGK_SPREAD_VARIANT): measurement aid
+  int spread_variant = 0;
 
   // optional per-stage timing with HIP events on the context's stream
   bool prof = false;

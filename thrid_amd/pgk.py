@@ -50,10 +50,12 @@ def load_library(path: str = LIB_PATH):
                                   C.POINTER(abi.ForceOut), C.POINTER(abi.HfOut)]
     L.pgk_calforce_device.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32, vp, vp, vp]
     L.pgk_synchronize.argtypes = [vp]
+    L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
+    L.pgk_list_info.argtypes = [vp, vp]
     for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
               "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
-              "pgk_synchronize"):
+              "pgk_synchronize", "pgk_profile", "pgk_list_info"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -177,6 +179,21 @@ class ForceContext:
                                       C.byref(ho)), "pgk_calforce_hf")
         return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
                     pot_atm=pe, viri_atm=wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+
+    def list_info(self):
+        """(clusters, cluster pairs, listed atom pairs, atoms per cluster)."""
+        v = np.zeros(4, np.int64)
+        _check(self.L.pgk_list_info(self.h, v.ctypes.data), "pgk_list_info")
+        return tuple(int(x) for x in v)
+
+    def profile(self, enable=-1):
+        """Stage times: returns (ms[8], calls[8]) accumulated since the last
+        enable; enable=1/0 switches timing on/off and clears the totals."""
+        ms = np.zeros(8)
+        calls = np.zeros(8, np.int64)
+        _check(self.L.pgk_profile(self.h, int(enable), ms.ctypes.data, calls.ctypes.data),
+               "pgk_profile")
+        return ms, calls
 
     def synchronize(self):
         _check(self.L.pgk_synchronize(self.h), "pgk_synchronize")
