@@ -378,6 +378,13 @@ int64_t pgko_mklist(pgko_sys *s, double rcut_book, int32_t mode) {
       csz[d] = s->box[d] / (double)cc[d];
     }
     int ncell = cc[0] * cc[1] * cc[2];
+    /* MOD(INT((x + L)/cellsize), ncell) indexes a cell only for x >= -L:
+     * coordinates below that are outside the reference's domain */
+    for (int i = 0; i < 3 * n; i++)
+      if (!(s->pos[i] + s->box[i % 3] >= 0.0)) {
+        free(lst); free(idx); free(tmp);
+        return -2;
+      }
     a2c = malloc(sizeof(int32_t) * 3 * n);
     int32_t *ccount = calloc(ncell + 1, sizeof(int32_t));
     int32_t *cid = malloc(sizeof(int32_t) * n);

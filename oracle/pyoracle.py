@@ -168,6 +168,9 @@ class Oracle:
     def mklist(self, rcut_book, mode=0):
         mode = _list_mode(self.sysd, rcut_book, mode)
         nl = self.L.pgko_mklist(self.h, float(rcut_book), int(mode))
+        if nl < 0:
+            raise ValueError(f"oracle mklist failed ({nl}): coordinates below -box "
+                             "are outside the cell index's domain")
         idx = np.zeros(self.n, np.int32)
         lst = np.zeros(max(nl, 1), np.int32)
         self.L.pgko_get_list(self.h, idx.ctypes.data, lst.ctypes.data)

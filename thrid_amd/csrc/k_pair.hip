@@ -35,8 +35,8 @@ struct PairArgs {
   double *red;
   int ncl, ntype, nspl_lds, lj_lds;
   R L[3], Linv[3];
-  R sqcut, sqcut2, rrcut;
-  R h, hinv, xmax;
+  R sqcut, sqcut2, rrsq;
+  R h, hinv;
 };
 
 template <class R>
@@ -95,12 +95,18 @@ __device__ __forceinline__ double sum8(double v) {
   return v;
 }
 
+// 1/r^2 and r.  FP64: hardware rsq seed and two Newton steps (relative error
+// a few ulp; the cutoff decisions never depend on it, see rrsq below).
 template <class R>
 __device__ __forceinline__ void rinv_pair(R r2, R &rinv2, R &r);
 template <>
 __device__ __forceinline__ void rinv_pair<double>(double r2, double &rinv2, double &r) {
-  rinv2 = 1.0 / r2;
-  r = sqrt(r2);
+  double y = __builtin_amdgcn_rsq(r2);
+  double h = 0.5 * r2;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  rinv2 = y * y;
+  r = r2 * y;
 }
 template <>
 __device__ __forceinline__ void rinv_pair<float>(float r2, float &rinv2, float &r) {
@@ -109,7 +115,45 @@ __device__ __forceinline__ void rinv_pair<float>(float r2, float &rinv2, float &
   r = r2 * ri;
 }
 
-template <int PREC, bool SPLIT, int NW>
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v) { return dppf<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v) { return dppd<CTRL>(v); }
+
+// x, y, z summed over each aligned group of 8 lanes in three exchange steps
+// (k = lane & 7): afterwards lanes k = 0|1 hold sum(x), k = 2|3 sum(y) and
+// k = 4|5 sum(z).  Step 1 pairs lane k with 7-k (row_half_mirror): the low
+// half keeps x and y, the high half z; step 2 pairs k with k^2, step 3 k^1.
+template <class R>
+__device__ __forceinline__ R sum3x8(R x, R y, R z, int k) {
+  const bool hi = (k & 4) != 0;
+  const R keep0 = hi ? z : x, send0 = hi ? x : z;
+  const R keep1 = hi ? R(0) : y, send1 = hi ? y : R(0);
+  const R a0 = keep0 + dpp_t<0x141>(send0);
+  const R a1 = keep1 + dpp_t<0x141>(send1);
+  const bool b1 = (k & 2) != 0;
+  const R b = (b1 ? a1 : a0) + dpp_t<0x4E>(b1 ? a0 : a1);
+  return b + dpp_t<0xB1>(b);
+}
+
+// float -> 2^-32 fixed point as hi:lo = floor(v) : (v - floor(v)) 2^32.
+// v - floor(v) is exact for |v| >= 1 (Sterbenz) and for 0 <= v < 1; for
+// -1 < v < 0 it is rounded once (error <= 2^-24).  A pure function of v, so
+// sums of the results are independent of the order of the atomic adds.
+__device__ __forceinline__ unsigned long long fx_of(float v) {
+  const float fl = floorf(v);
+  const float fr = fminf(v - fl, 0x1.fffffep-1f);
+  const unsigned lo = (unsigned)(fr * 4294967296.0f);
+  const int hi = (int)fl;
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)lo;
+}
+__device__ __forceinline__ void acc_add_j(unsigned long long *p, float v) {
+  atomicAdd(p, fx_of(v));
+}
+__device__ __forceinline__ void acc_add_j(double *p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void acc_add_j(float *p, float v) { atomicAdd(p, v); }
+
+template <int PREC, bool SPLIT, bool LJLDS, int NW>
 __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
@@ -119,10 +163,10 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   // stage the erfc spline and LJ tables in LDS
   for (int k = threadIdx.x; k < 8 * a.nspl_lds; k += blockDim.x) s_spl[k] = a.spl[k];
   const int nlj = 2 * a.ntype * a.ntype;
-  if (a.lj_lds)
+  if (LJLDS)
     for (int k = threadIdx.x; k < nlj; k += blockDim.x) s_lj[k] = a.lj[k];
   __syncthreads();
-  const R *ljt = a.lj_lds ? s_lj : a.lj;
+  const R *ljt = LJLDS ? s_lj : a.lj;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ci = blockIdx.x * NW + wave;
@@ -132,12 +176,14 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   if (ci < a.ncl) {
     const int ii = lane & 7, jj = lane >> 3;
     const R4 xi = a.xq[ci * PGK_CL + ii];
-    const int ti = a.ct[ci * PGK_CL + ii];
-    const int tib = (ti - 1);
+    const int ti = a.ct[ci * PGK_CL + ii];   // 0-based type (0 for padding)
+    const int nlast = a.nspl_lds - 1;
     double fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
     const int e0 = a.ci_start[ci], e1 = a.ci_start[ci + 1];
+    // j-cluster data of entry e+1 is loaded while entry e is computed; the
+    // index is clamped so the loads stay in bounds on the last entry
     int cj_n = 0;
     uint64_t m_n = 0;
     R4 xj_n;
@@ -148,79 +194,76 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       xj_n = a.xq[cj_n * PGK_CL + jj];
       tj_n = a.ct[cj_n * PGK_CL + jj];
     }
+#pragma unroll 2
     for (int e = e0; e < e1; e++) {
       const int cj = cj_n;
       const uint64_t m = m_n;
       const R4 xj = xj_n;
       const int tj = tj_n;
-      if (e + 1 < e1) {   // prefetch the next j-cluster
-        cj_n = a.cj[e + 1];
-        m_n = a.mask[e + 1];
+      {
+        const int en = min(e + 1, e1 - 1);
+        cj_n = a.cj[en];
+        m_n = a.mask[en];
         xj_n = a.xq[cj_n * PGK_CL + jj];
         tj_n = a.ct[cj_n * PGK_CL + jj];
       }
-      R fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
-      if ((m >> lane) & 1ull) {
-        R rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        rx = rx - a.L[0] * nimg(rx * a.Linv[0]);
-        ry = ry - a.L[1] * nimg(ry * a.Linv[1]);
-        rz = rz - a.L[2] * nimg(rz * a.Linv[2]);
-        const R r2 = r2_nc(rx, ry, rz);
-        if (r2 <= a.sqcut) {
-          R rinv2, r;
-          rinv_pair<R>(r2, rinv2, r);
-          if (r <= a.rrcut) {
-            R sf, se;
-            spline2(s_spl, r, a.h, a.hinv, a.xmax, a.nspl_lds, sf, se);
-            const R qq = xi.w * xj.w;
-            const R ft = sf * qq;
-            e_elc += se * qq;
-            cx = ft * rx;
-            cy = ft * ry;
-            cz = ft * rz;
-            tc[0] += cx * rx; tc[1] += cy * ry; tc[2] += cz * rz;
-            tc[3] += cx * ry; tc[4] += cx * rz; tc[5] += cy * rz;
-          }
-          if (r2 <= a.sqcut2) {
-            const int tp = tib + a.ntype * (tj - 1);
-            const R eps = ljt[2 * tp], rm2 = ljt[2 * tp + 1];
-            const R t2 = rinv2 * rm2;
-            const R t6 = t2 * t2 * t2;
-            const R t12 = t6 * t6;
-            e_vdw += eps * (t12 - R(2) * t6);
-            const R ft = R(12) * eps * (t12 - t6) * rinv2;
-            const R lx = ft * rx, ly = ft * ry, lz = ft * rz;
-            tl[0] += lx * rx; tl[1] += ly * ry; tl[2] += lz * rz;
-            tl[3] += lx * ry; tl[4] += lx * rz; tl[5] += ly * rz;
-            fx = lx;
-            fy = ly;
-            fz = lz;
-          }
-          fx += cx;
-          fy += cy;
-          fz += cz;
-        }
-      }
+      // every lane runs the same instructions; a lane whose pair is not
+      // listed or lies outside a cutoff contributes exact zeros
+      R rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      rx = rx - a.L[0] * nimg(rx * a.Linv[0]);
+      ry = ry - a.L[1] * nimg(ry * a.Linv[1]);
+      rz = rz - a.L[2] * nimg(rz * a.Linv[2]);
+      const R r2 = r2_nc(rx, ry, rz);
+      const bool in = ((m >> lane) & 1ull) && (r2 <= a.sqcut);
+      const R r2s = in ? r2 : R(1);
+      R rinv2, r;
+      rinv_pair<R>(r2s, rinv2, r);
+      // Coulomb (r <= rrcut: r2 <= rrsq decides it exactly)
+      const bool coul = in && (r2 <= a.rrsq);
+      int sk = (int)(r * a.hinv);
+      sk = min(sk, nlast);
+      const R d = r - (R)sk * a.h;
+      const R *t = s_spl + 8 * sk;
+      const R sf = t[0] + d * (t[1] + d * (t[2] + d * t[3]));
+      const R se = t[4] + d * (t[5] + d * (t[6] + d * t[7]));
+      const R qq = coul ? xi.w * xj.w : R(0);
+      const R ftc = sf * qq;
+      e_elc += se * qq;
+      // Lennard-Jones (eps is 0 unless the type pair interacts by VDW)
+      const bool ljon = in && (r2 <= a.sqcut2);
+      const int tp = ljon ? (ti + a.ntype * tj) : 0;
+      const R eps = ljon ? ljt[2 * tp] : R(0);
+      const R rm2 = ljt[2 * tp + 1];
+      const R t2 = rinv2 * rm2;
+      const R t6 = t2 * t2 * t2;
+      const R t12 = t6 * t6;
+      e_vdw += eps * (t12 - R(2) * t6);
+      const R ftl = R(12) * eps * (t12 - t6) * rinv2;
+      const R xx = rx * rx, yy = ry * ry, zz = rz * rz;
+      const R xy = rx * ry, xz = rx * rz, yz = ry * rz;
+      tc[0] += ftc * xx; tc[1] += ftc * yy; tc[2] += ftc * zz;
+      tc[3] += ftc * xy; tc[4] += ftc * xz; tc[5] += ftc * yz;
+      tl[0] += ftl * xx; tl[1] += ftl * yy; tl[2] += ftl * zz;
+      tl[3] += ftl * xy; tl[4] += ftl * xz; tl[5] += ftl * yz;
+      const R ft = ftc + ftl;
+      const R fx = ft * rx, fy = ft * ry, fz = ft * rz;
       fix += (double)fx;
       fiy += (double)fy;
       fiz += (double)fz;
       if (SPLIT) {
-        cix += (double)cx;
-        ciy += (double)cy;
-        ciz += (double)cz;
+        cix += (double)(ftc * rx);
+        ciy += (double)(ftc * ry);
+        ciz += (double)(ftc * rz);
       }
-      // j side: every lane of the 8-lane group gets the group sum; lanes
-      // ii = 0, 1, 2 add the x, y, z component in one atomic instruction
-      if ((m >> (8 * jj)) & 0xffull) {
-        const R gx = sum8(-fx), gy = sum8(-fy), gz = sum8(-fz);
-        const R g = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
-        const size_t s = (size_t)(cj * PGK_CL + jj) * 3;
-        if (ii < 3) acc_add(&a.fcl[s + ii], g);
-        if (SPLIT) {
-          const R hx = sum8(-cx), hy = sum8(-cy), hz = sum8(-cz);
-          const R hh = (ii == 0) ? hx : ((ii == 1) ? hy : hz);
-          if (ii < 3) acc_add(&a.fcl_c[s + ii], hh);
-        }
+      // j side: x, y, z of each j-slot summed over its 8 i-lanes; lanes
+      // ii = 0, 2, 4 add -sum to the x, y, z accumulator of the j atom
+      const R g = sum3x8(fx, fy, fz, ii);
+      R gc = R(0);
+      if (SPLIT) gc = sum3x8(ftc * rx, ftc * ry, ftc * rz, ii);
+      if (((m >> (8 * jj)) & 0xffull) && !(ii & 1) && ii < 6) {
+        const size_t s = (size_t)(cj * PGK_CL + jj) * 3 + (ii >> 1);
+        acc_add_j(&a.fcl[s], -g);
+        if (SPLIT) acc_add_j(&a.fcl_c[s], -gc);
       }
     }
     // i side: sum over the 8 j-lanes that share this i-slot
@@ -341,26 +384,27 @@ static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
   if (sq2 > sq) sq = sq2;
   a.sqcut = (R)sq;
   a.sqcut2 = (R)sq2;
-  a.rrcut = (R)c->rrcut;
+  a.rrsq = (R)c->rrsq_cut;
   a.h = (R)c->spl_int;
   a.hinv = (R)(1.0 / c->spl_int);
-  a.xmax = (R)(c->spl_int * (double)(c->nspl - 1));
   int nblk = pgk_grid_blocks(c->ncl, NW);
   *off = c->red_used;
   *nrec = nblk;
   a.red = c->d_red + (size_t)c->red_used * PGK_REC;
   c->red_used += nblk;
-  if (split) {
-    auto kf = k_pairs<PREC, true, NW>;
-    HIP_CHECK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
-    kf<<<nblk, 64 * NW, lds, c->stream>>>(a);
-  } else {
-    auto kf = k_pairs<PREC, false, NW>;
-    HIP_CHECK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
-    kf<<<nblk, 64 * NW, lds, c->stream>>>(a);
+#define PGK_LAUNCH_PAIRS(SP, LL)                                                        \
+  {                                                                                      \
+    auto kf = k_pairs<PREC, SP, LL, NW>;                                                 \
+    HIP_CHECK(hipFuncSetAttribute((const void *)kf,                                      \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    kf<<<nblk, 64 * NW, lds, c->stream>>>(a);                                            \
   }
+  if (split) {
+    if (a.lj_lds) PGK_LAUNCH_PAIRS(true, true) else PGK_LAUNCH_PAIRS(true, false)
+  } else {
+    if (a.lj_lds) PGK_LAUNCH_PAIRS(false, true) else PGK_LAUNCH_PAIRS(false, false)
+  }
+#undef PGK_LAUNCH_PAIRS
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

@@ -185,7 +185,7 @@ __global__ void k_pack(int nslot, const int32_t *__restrict__ cl_atom,
     v.y = pos[3 * a + 1];
     v.z = pos[3 * a + 2];
     v.w = q[a];
-    ct[s] = type[a];
+    ct[s] = type[a] - 1;   // 0-based LJ type index
   } else {
     v.x = v.y = v.z = v.w = 0;
     ct[s] = 0;
@@ -475,6 +475,15 @@ extern "C" {
 int32_t pgk_abi_version(void) { return PGK_ABI_VERSION; }
 const char *pgk_last_error(void) { return g_err.c_str(); }
 
+// Largest double T with sqrt(T) <= rc in IEEE arithmetic: since the
+// correctly rounded sqrt is monotone, r2 <= T decides sqrt(r2) <= rc exactly.
+static double sqrt_threshold(double rc) {
+  double t = rc * rc;
+  while (std::sqrt(t) > rc) t = std::nextafter(t, 0.0);
+  while (std::sqrt(std::nextafter(t, HUGE_VAL)) <= rc) t = std::nextafter(t, HUGE_VAL);
+  return t;
+}
+
 int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   if (!d || !out) {
     pgk_set_error("pgk_create: null argument");
@@ -546,6 +555,7 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   c->rb_if14 = copyv(d->torsrb_if14, (size_t)d->ntorsrbtyp);
   c->alpha = d->alpha;
   c->rrcut = d->rrcut;
+  c->rrsq_cut = sqrt_threshold(d->rrcut);
   c->rcut = d->rcut;
   c->nspl = d->nspltbl;
   c->nfft[0] = d->nfft1;

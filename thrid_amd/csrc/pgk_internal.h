@@ -79,6 +79,7 @@ struct pgk_ctx {
   std::vector<int32_t> rb_ijkl, rb_type, rb_if14;
   std::vector<double> rb_ring, rb_c, rb_dvdw, rb_delc;
   double alpha = 0, rrcut = 0, rcut = 0;
+  double rrsq_cut = 0;   // largest r^2 with sqrt(r^2) <= rrcut (correctly rounded)
   int nspl = 0, nfft[3] = {0, 0, 0}, order = 4, ifnetq = 0;
   double box[3] = {0, 0, 0};
   double netchrgsq = 0, pot_ewc = 0, rkcut = 0, spl_int = 0;
