@@ -58,6 +58,9 @@ def pair_counts(sysd):
 
 
 def run_gpu(sysd, precision, steps, warmup):
+    """Time `steps` force steps (list rebuilt every NSTEP_BOOK steps) after
+    `warmup` untimed ones; returns (s/step, stage ms, energies, list info,
+    forces at the benchmark positions)."""
     from thrid_amd.pgk import ForceContext
     ctx = ForceContext(sysd, precision)
     ctx.mklist(sysd["rcut_book"])
@@ -77,39 +80,87 @@ def run_gpu(sysd, precision, steps, warmup):
     t1 = time.perf_counter()
     ms, calls = ctx.profile(0)
     ene = e.copy()
+    force = ctx.calforce()["force"]          # untimed, same positions
     ctx.close()
     stages = ["pairs", "excl", "spread", "fft", "gather", "bonded", "list", "other"]
     prof = {s: (float(ms[k] / calls[k]) if calls[k] else 0.0) for k, s in enumerate(stages)}
-    return (t1 - t0) / steps, prof, ene, info
+    return (t1 - t0) / steps, prof, ene, info, force
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (scale for the HBM byte counters)."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: {reps} copies of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel_prefix):
+    """HBM bytes per launch of the kernel from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d["kernels"][kernel_prefix]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(sysd, budget_s=20.0):
     """The reference's own calforce (compiled from source, oracle/_ref) on one
-    core, timed on a bounded number of calls; falls back to the C restatement."""
+    core, timed on a bounded number of calls; falls back to the C restatement.
+    Also returns its forces at the benchmark positions (the FP64 checker)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from thrid_amd import abi
     kind = "reference" if pyoracle.reference_available() else "port"
-    if kind == "reference":
-        m = pyoracle.Reference(sysd)
-        t0 = time.perf_counter()
-        m.mklist(sysd["rcut_book"], 1)
-        t_list = time.perf_counter() - t0
-    else:
-        m = pyoracle.Oracle(sysd)
-        t0 = time.perf_counter()
-        m.mklist(sysd["rcut_book"], 1)
-        t_list = time.perf_counter() - t0
-    ncall, t_force = 0, 0.0
+    m = pyoracle.Reference(sysd) if kind == "reference" else pyoracle.Oracle(sysd)
+    t0 = time.perf_counter()
+    m.mklist(sysd["rcut_book"], 1)
+    t_list = time.perf_counter() - t0
+    ncall, t_force, force = 0, 0.0, None
     while t_force < budget_s and ncall < 50:
         t0 = time.perf_counter()
-        m.calforce(abi.MTS_LONG)
+        r = m.calforce(abi.MTS_LONG)
         t_force += time.perf_counter() - t0
         ncall += 1
+        force = r["force"]
     per_step = t_force / ncall + t_list / NSTEP_BOOK
-    return dict(value=sysd["natom"] / per_step, unit="atom-steps/s", cores=1, kind=kind,
-                sample=f"{ncall} calforce calls + 1 list build (amortised over "
-                       f"{NSTEP_BOOK} steps) of the same 100,001-atom box, serial")
+    base = dict(value=sysd["natom"] / per_step, unit="atom-steps/s", cores=1, kind=kind,
+                sample=f"{ncall} calforce calls + 1 cell-list build (amortised over "
+                       f"{NSTEP_BOOK} steps) of the same 100,001-atom box, serial, "
+                       f"list {t_list:.2f} s, calforce {t_force / ncall:.2f} s/call")
+    return base, force
+
+
+def cpu_leg_subprocess():
+    """Run cpu_baseline in a child process: the compiled reference writes its
+    own progress lines to stdout, which must not mix with the JSON line."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        log = os.path.join(d, "cpu.log")
+        with open(log, "w") as lf:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", d],
+                               stdout=lf, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            sys.stderr.write(open(log).read()[-4000:])
+            raise RuntimeError(f"CPU baseline leg failed ({r.returncode})")
+        with open(os.path.join(d, "cpu.json")) as f:
+            base = json.load(f)
+        fref = np.load(os.path.join(d, "force.npy"))
+    return base, fref
+
+
+def force_error(f, ref):
+    d = np.sqrt(((f - ref) ** 2).sum(1))
+    rms_ref = np.sqrt(np.mean((ref ** 2).sum(1)))
+    return {"rms_rel": float(np.sqrt(np.mean(d ** 2)) / rms_ref),
+            "max_rel": float(d.max() / rms_ref)}
 
 
 def main():
@@ -118,62 +169,83 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg:   # child process of the CPU baseline leg (no GPU use)
+        from thrid_amd import systems
+        base, fref = cpu_baseline(systems.water_ion_box_100k())
+        with open(os.path.join(args.cpu_leg, "cpu.json"), "w") as f:
+            json.dump(base, f)
+        np.save(os.path.join(args.cpu_leg, "force.npy"), fref)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the force path on a single "
               "MI355X (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     from thrid_amd import systems
+    if args.copy_calib:
+        copy_calibration()
     sysd = systems.water_ion_box_100k()
     n = sysd["natom"]
     res = {}
-    for prec in ("fp64", "mixed"):
-        sec, prof, ene, info = run_gpu(sysd, prec, args.steps, args.warmup)
-        res[prec] = dict(sec=sec, prof=prof, ene=ene, info=info)
+    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    for prec in precs:
+        sec, prof, ene, info, force = run_gpu(sysd, prec, args.steps, args.warmup)
+        res[prec] = dict(sec=sec, prof=prof, ene=ene, info=info, force=force)
         print(f"# {prec}: {sec * 1e3:.3f} ms/step, stages(ms) "
               + " ".join(f"{k}={v:.3f}" for k, v in prof.items()), file=sys.stderr, flush=True)
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    hp = "mixed" if "mixed" in res else "fp64"
     ncoul, nlj = pair_counts(sysd)
     flops = ncoul * (36 + 12) + nlj * 13
-    mix = res["mixed"]
-    t_pair = mix["prof"]["pairs"] * 1e-3
+    t_pair = head["prof"]["pairs"] * 1e-3
     achieved = flops / t_pair / 1e12 if t_pair > 0 else 0.0
     out = {
         "metric": "atom-steps/s (mixed SPME force step), 100k water",
-        "value": n / mix["sec"],
+        "value": n / head["sec"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mix["sec"] * 1e3,
+        "ms_per_step": head["sec"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (pairs, spline, FFT) + int64 fixed-point sums",
+        "dtype": "f32 (pairs, spline, FFT) + int64 fixed-point sums" if hp == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": "100,001-atom water + Na+/Cl-, 100.1 A box, SPME 128^3 p=4, "
                                "alpha 0.32, rc 10 A, rcut_book 11 A, list every 20 steps",
-                   "natom": n, "precision": "mixed"},
-        "ns_per_day": 86400.0 / mix["sec"] * 1e-6,
-        "fp64": {"atom_steps_per_s": n / res["fp64"]["sec"],
-                 "ms_per_step": res["fp64"]["sec"] * 1e3,
-                 "ns_per_day": 86400.0 / res["fp64"]["sec"] * 1e-6},
-        "speedup_mixed_vs_fp64": res["fp64"]["sec"] / mix["sec"],
+                   "natom": n, "precision": hp},
+        "ns_per_day": 86400.0 / head["sec"] * 1e-6,
         "stage_ms": {p: res[p]["prof"] for p in res},
-        "roofline": {"bound": "valu", "kernel": "k_pairs (mixed)", "achieved": achieved,
-                     "peak": PEAK["mixed"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["mixed"], "traffic": None,
+        "roofline": {"bound": "valu", "kernel": f"k_pairs ({hp})", "achieved": achieved,
+                     "peak": PEAK[hp], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[hp],
+                     "traffic": traffic_from_profiles(f"k_pairs_{hp}"),
                      "flops_per_launch": flops, "pairs_coulomb": ncoul, "pairs_lj": nlj,
-                     "avg_launch_ms": mix["prof"]["pairs"]},
-        "list": {"clusters": res["mixed"]["info"][0], "cluster_pairs": res["mixed"]["info"][1],
-                 "listed_pairs": res["mixed"]["info"][2],
-                 "lane_efficiency": ncoul / (64.0 * res["mixed"]["info"][1])},
-        "roofline_fp64_pairs": {"achieved": flops / (res["fp64"]["prof"]["pairs"] * 1e-3) / 1e12
-                                if res["fp64"]["prof"]["pairs"] > 0 else 0.0,
-                                "peak": PEAK["fp64"], "unit": "TFLOP/s"},
+                     "avg_launch_ms": head["prof"]["pairs"]},
+        "list": {"clusters": head["info"][0], "cluster_pairs": head["info"][1],
+                 "listed_pairs": head["info"][2],
+                 "lane_efficiency": ncoul / (64.0 * head["info"][1])},
     }
-    out["roofline_fp64_pairs"]["frac"] = out["roofline_fp64_pairs"]["achieved"] / PEAK["fp64"]
+    if "fp64" in res and "mixed" in res:
+        f64 = res["fp64"]
+        out["fp64"] = {"atom_steps_per_s": n / f64["sec"], "ms_per_step": f64["sec"] * 1e3,
+                       "ns_per_day": 86400.0 / f64["sec"] * 1e-6,
+                       "roofline_pairs": {
+                           "achieved": flops / (f64["prof"]["pairs"] * 1e-3) / 1e12,
+                           "peak": PEAK["fp64"], "unit": "TFLOP/s"}}
+        out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
+                                                 / PEAK["fp64"])
+        out["speedup_mixed_vs_fp64"] = f64["sec"] / res["mixed"]["sec"]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sysd)
+        base, fref = cpu_leg_subprocess()
+        out["cpu_baseline"] = base
+        out["accuracy"] = {p: force_error(res[p]["force"], fref) for p in res}
+        out["accuracy"]["against"] = f"{base['kind']} FP64 calforce on the same positions"
     print(json.dumps(out))
 
 

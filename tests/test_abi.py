@@ -1,0 +1,123 @@
+"""The C ABI of libpgkforce.so (include/pgk_force.h) without a GPU.
+
+The library must load, export every function the header declares, agree with
+the ctypes mirror (thrid_amd/abi.py) on every struct layout, validate its
+arguments before touching a device and fail loudly (never fall back to a CPU
+path) when no device is present.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from thrid_amd import abi, pgk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pgk_force.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pgk_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_force_interface():
+    names = declared_functions()
+    for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions", "pgk_build_list",
+              "pgk_calforce", "pgk_calforce_hf", "pgk_last_error", "pgk_abi_version"):
+        assert f in names
+
+
+def test_library_exports_every_declared_function():
+    lib = pgk.load_library()
+    missing = [f for f in declared_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+    assert lib.pgk_abi_version() == abi.ABI_VERSION
+
+
+_STRUCTS = {
+    "pgk_system_desc": abi.SystemDesc,
+    "pgk_mts": abi.Mts,
+    "pgk_force_out": abi.ForceOut,
+    "pgk_hf_desc": abi.HfDesc,
+    "pgk_hf_out": abi.HfOut,
+}
+
+
+def test_struct_layout_matches_ctypes(tmp_path):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pgk_force.h"',
+             "int main(void) {"]
+    for cname, py in _STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _t in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for ln in out.splitlines():
+        s, f, v = ln.split()
+        got[(s, f)] = int(v)
+    for cname, py in _STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _t in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_constants_match_header(tmp_path):
+    names = {"PGK_ABI_VERSION": abi.ABI_VERSION, "PGK_OK": abi.PGK_OK,
+             "PGK_EINVAL": abi.PGK_EINVAL, "PGK_EDEVICE": abi.PGK_EDEVICE,
+             "PGK_ENOMEM": abi.PGK_ENOMEM, "PGK_ESTATE": abi.PGK_ESTATE,
+             "PGK_PREC_FP64": abi.PREC["fp64"], "PGK_PREC_MIXED": abi.PREC["mixed"],
+             "PGK_PREC_FP32": abi.PREC["fp32"], "PGK_NENE": abi.NENE,
+             "PGK_NTENS": abi.NTENS, "PGK_NHF": abi.NHF,
+             "PGK_HF_VOLUME": abi.HF_VOLUME, "PGK_HF_SURFACE": abi.HF_SURFACE,
+             "PGK_HF_BULK": abi.HF_BULK}
+    lines = ["#include <stdio.h>", '#include "pgk_force.h"', "int main(void) {"]
+    for k in names:
+        lines.append(f'  printf("{k} %lld\\n", (long long)({k}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "consts.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "consts"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+    assert got == names
+
+
+def test_create_validates_arguments_before_the_device():
+    lib = pgk.load_library()
+    h = C.c_void_p()
+    assert lib.pgk_create(None, 0, C.byref(h)) == abi.PGK_EINVAL
+    assert b"null" in lib.pgk_last_error()
+    d = abi.SystemDesc()          # natom 0: incomplete
+    assert lib.pgk_create(C.byref(d), 0, C.byref(h)) == abi.PGK_EINVAL
+    assert b"incomplete" in lib.pgk_last_error()
+    assert lib.pgk_destroy(None) in (abi.PGK_OK, abi.PGK_EINVAL)
+    assert lib.pgk_calforce(None, 1, None, 1, None) == abi.PGK_EINVAL
+
+
+def test_missing_library_raises(tmp_path):
+    saved = pgk._lib
+    pgk._lib = None
+    try:
+        with pytest.raises(pgk.PgkError):
+            pgk.load_library(str(tmp_path / "absent.so"))
+    finally:
+        pgk._lib = saved
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from helpers import water648
+    with pytest.raises(pgk.PgkError, match="device"):
+        pgk.ForceContext(water648(), "fp64")

@@ -8,7 +8,7 @@
 //                                       force through the Chebyshev form of calijkl
 //   caltorsrbp  caltorsrb.F90:130-459   E = sum_k (-1)^k C_k cos^k phi (Horner)
 //   calnon14p   calnon14.F90:249-521    scaled Coulomb + LJ on torsion end atoms
-#include "pgk_device.h"
+#include "pgk_hf.h"
 
 namespace {
 
@@ -42,9 +42,9 @@ struct BArgs {
 };
 
 // rec: E, V, T(9 in Fortran (a,n) order)
-template <class A>
+template <class A, class AH, bool HF>
 __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restrict__ bl, int ok,
-                                               int orr, A *fat) {
+                                               int orr, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -62,6 +62,12 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
       double f[3] = {fa * r[0] / ra, fa * r[1] / ra, fa * r[2] / ra};
       addf(fat, i, f, 1.0);
       addf(fat, j, f, -1.0);
+      if (HF) {   // calbondp_hf (calbond_hf.F90:6-164)
+        const double fj[3] = {-f[0], -f[1], -f[2]};
+        hf_pe_add<AH>(h, PGK_HF_BO, i, 0.5 * ev[0]);
+        hf_pe_add<AH>(h, PGK_HF_BO, j, 0.5 * ev[0]);
+        calvirihf_dev<AH>(h, i, j, f, fj, 0.5, PGK_HF_BO);
+      }
       ev[1] = dot3(f, r);
       for (int nn = 0; nn < 3; nn++)
         for (int aa = 0; aa < 3; aa++) ev[2 + aa + 3 * nn] = f[aa] * r[nn];
@@ -70,9 +76,9 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
   block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
-template <class A>
+template <class A, class AH, bool HF>
 __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restrict__ al, int ok,
-                                                int ot, A *fat) {
+                                                int ot, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -106,6 +112,15 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
       addf(fat, i, fi, 1.0);
       addf(fat, j, fj, 1.0);
       addf(fat, k, fk, 1.0);
+      if (HF) {   // calanglp_hf (calangl_hf.F90:6-255): 1/3 per atom, three scatters
+        const double c3 = 1.0 / 3.0;
+        hf_pe_add<AH>(h, PGK_HF_AN, i, c3 * ev[0]);
+        hf_pe_add<AH>(h, PGK_HF_AN, j, c3 * ev[0]);
+        hf_pe_add<AH>(h, PGK_HF_AN, k, c3 * ev[0]);
+        calvirihf_dev<AH>(h, i, j, fi, fj, c3, PGK_HF_AN);
+        calvirihf_dev<AH>(h, j, k, fj, fk, c3, PGK_HF_AN);
+        calvirihf_dev<AH>(h, k, i, fk, fi, c3, PGK_HF_AN);
+      }
       const double *pi = a.pos + 3 * i, *pj = a.pos + 3 * j, *pk = a.pos + 3 * k;
       ev[1] = dot3(fi, pi) + dot3(fj, pj) + dot3(fk, pk);
       for (int nn = 0; nn < 3; nn++)
@@ -175,9 +190,9 @@ __device__ __forceinline__ void tforces(TGeo &g, double fa, double *fi, double *
 }
 
 // RB = true: C_0..C_5 at par[oc + 6*ty]; false: periodic (div, barhig, phase, period)
-template <class A, bool RB>
+template <class A, class AH, bool HF, bool RB>
 __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict__ tl, int o0,
-                                              int o1, int o2, int o3, A *fat) {
+                                              int o1, int o2, int o3, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -219,6 +234,19 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
     addf(fat, j, fj, 1.0);
     addf(fat, k, fk, 1.0);
     addf(fat, l, fl, 1.0);
+    if (HF) {   // caltors(rb)p_hf: 1/4 per atom, six pair scatters
+      const double c4 = 0.25;
+      hf_pe_add<AH>(h, PGK_HF_TO, i, c4 * dp);
+      hf_pe_add<AH>(h, PGK_HF_TO, j, c4 * dp);
+      hf_pe_add<AH>(h, PGK_HF_TO, k, c4 * dp);
+      hf_pe_add<AH>(h, PGK_HF_TO, l, c4 * dp);
+      calvirihf_dev<AH>(h, i, j, fi, fj, c4, PGK_HF_TO);
+      calvirihf_dev<AH>(h, i, k, fi, fk, c4, PGK_HF_TO);
+      calvirihf_dev<AH>(h, i, l, fi, fl, c4, PGK_HF_TO);
+      calvirihf_dev<AH>(h, j, k, fj, fk, c4, PGK_HF_TO);
+      calvirihf_dev<AH>(h, j, l, fj, fl, c4, PGK_HF_TO);
+      calvirihf_dev<AH>(h, k, l, fk, fl, c4, PGK_HF_TO);
+    }
     const double *P[4] = {a.pos + 3 * i, a.pos + 3 * j, a.pos + 3 * k, a.pos + 3 * l};
     const double *F[4] = {fi, fj, fk, fl};
     ev[0] = dp;
@@ -234,14 +262,14 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
 }
 
 // 1-4 pairs; rec: E_elc14, E_vdw14, V, T(9)
-template <class A>
+template <class A, class AH, bool HF>
 __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restrict__ pl,
                                                const double *__restrict__ pf,
                                                const double *__restrict__ q,
                                                const int32_t *__restrict__ type,
                                                const double *__restrict__ eps,
                                                const double *__restrict__ rmin, int ntype,
-                                               A *fat) {
+                                               A *fat, HfDev h) {
   double ev[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) ev[k] = 0.0;
@@ -267,6 +295,17 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
     double f[3] = {fe[0] + ft * r[0], fe[1] + ft * r[1], fe[2] + ft * r[2]};
     addf(fat, i, f, 1.0);
     addf(fat, j, f, -1.0);
+    if (HF) {   // calnon14p_hf (calnon14_hf.F90:8-386): Coulomb and LJ parts apart
+      const double fej[3] = {-fe[0], -fe[1], -fe[2]};
+      const double fv[3] = {ft * r[0], ft * r[1], ft * r[2]};
+      const double fvj[3] = {-fv[0], -fv[1], -fv[2]};
+      hf_pe_add<AH>(h, PGK_HF_14, i, 0.5 * pel);
+      hf_pe_add<AH>(h, PGK_HF_14, j, 0.5 * pel);
+      calvirihf_dev<AH>(h, i, j, fe, fej, 0.5, PGK_HF_14);
+      hf_pe_add<AH>(h, PGK_HF_14, i, 0.5 * pv);
+      hf_pe_add<AH>(h, PGK_HF_14, j, 0.5 * pv);
+      calvirihf_dev<AH>(h, i, j, fv, fvj, 0.5, PGK_HF_14);
+    }
     ev[0] = pel;
     ev[1] = pv;
     ev[2] = dot3(f, r);
@@ -279,14 +318,14 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
 }  // namespace
 
 // Launch every bonded term on the MTS level mts_flag.  Records are appended
-// per term; the segment table (kind per segment) is written to segs.
-int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
-                       int *seg_kind, int *nseg);
-
-template <int PREC>
+// per term; the segment table (kind per segment) is written to segs.  With
+// hf the calforce_hf variants also scatter the per-atom heat-flux terms.
+template <int PREC, bool HF>
 static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
                          int *seg_kind, int *nseg) {
   using A = typename Prec<PREC>::acc;
+  using AH = typename HfAcc<PREC>::type;
+  const HfDev &h = c->hfdev;
   A *fat = reinterpret_cast<A *>(c->d_fat);
   BArgs a;
   a.pos = c->d_pos;
@@ -307,46 +346,50 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     a.n = nb;
     int g = pgk_grid_blocks(nb, 256);
     seg(g, 1);
-    k_bonds<A><<<g, 256, 0, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat);
+    k_bonds<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat, h);
   }
   int na = (int)(c->angl_type.size());
   if (m->angl == mts_flag && na > 0) {
     a.n = na;
     int g = pgk_grid_blocks(na, 256);
     seg(g, 2);
-    k_angles<A><<<g, 256, 0, st>>>(a, c->d_angl, c->off_ak, c->off_at, fat);
+    k_angles<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_angl, c->off_ak, c->off_at, fat, h);
   }
   int nt = (int)(c->tors_type.size());
   if (m->tors == mts_flag && nt > 0) {
     a.n = nt;
     int g = pgk_grid_blocks(nt, 256);
     seg(g, 3);
-    k_tors<A, false><<<g, 256, 0, st>>>(a, c->d_tors, c->off_tdiv, c->off_tb, c->off_tp,
-                                        c->off_tper, fat);
+    k_tors<A, AH, HF, false><<<g, 256, 0, st>>>(a, c->d_tors, c->off_tdiv, c->off_tb,
+                                                c->off_tp, c->off_tper, fat, h);
   }
   int nr = (int)(c->rb_type.size());
   if (m->torsrb == mts_flag && nr > 0) {
     a.n = nr;
     int g = pgk_grid_blocks(nr, 256);
     seg(g, 4);
-    k_tors<A, true><<<g, 256, 0, st>>>(a, c->d_rb, c->off_rbc, 0, 0, 0, fat);
+    k_tors<A, AH, HF, true><<<g, 256, 0, st>>>(a, c->d_rb, c->off_rbc, 0, 0, 0, fat, h);
   }
   if (m->nonb14 == mts_flag && c->n14 > 0) {
     a.n = c->n14;
     int g = pgk_grid_blocks(c->n14, 256);
     seg(g, 5);
-    k_non14<A><<<g, 256, 0, st>>>(a, c->d_p14, c->d_p14f, c->d_q, c->d_type, c->d_eps,
-                                   c->d_rmin, c->ntype, fat);
+    k_non14<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_p14, c->d_p14f, c->d_q, c->d_type, c->d_eps,
+                                           c->d_rmin, c->ntype, fat, h);
   }
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
 
 int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
-                       int *seg_kind, int *nseg) {
+                       int *seg_kind, int *nseg, bool hf) {
+#define PGK_BL(P)                                                                      \
+  return hf ? launch_bonded<P, true>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)   \
+            : launch_bonded<P, false>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)
   switch (c->precision) {
-    case PGK_PREC_FP64: return launch_bonded<PGK_PREC_FP64>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg);
-    case PGK_PREC_MIXED: return launch_bonded<PGK_PREC_MIXED>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg);
-    default: return launch_bonded<PGK_PREC_FP32>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg);
+    case PGK_PREC_FP64: PGK_BL(PGK_PREC_FP64);
+    case PGK_PREC_MIXED: PGK_BL(PGK_PREC_MIXED);
+    default: PGK_BL(PGK_PREC_FP32);
   }
+#undef PGK_BL
 }

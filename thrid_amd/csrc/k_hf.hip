@@ -1,0 +1,174 @@
+// k_hf.hip - heat-flux pass of calforce_hf over the listed pairs, and the
+// export of the per-atom arrays.
+//
+// Restates the per-atom part of ewaldrp_hf (ewaldr_hf.F90:8-670) in the
+// default build: for every listed pair with r <= rrcut the bare Coulomb
+// energy qq/r and force qq/r^3 r_ij (ewaldr_hf.F90:336-343), for r^2 <= rcut^2
+// and INTTYPE_VDW the LJ energy and force; half of each energy goes to each
+// atom, class ELIN/LJIN for pairs inside one molecule and ELC/VDW otherwise
+// (ewaldr_hf.F90:347-371, 438-462), and calvirihf scatters the virial with
+// weight 1/2.  Excluded pairs and the reciprocal sum add no per-atom terms in
+// this build (ewaldr_hf.F90:637-653 only under _HF_BULK_EWK).
+//
+// Arithmetic is FP64 in every precision form (the cutoff tests are the
+// reference's: no contraction, r = sqrt(r^2), r <= rrcut).  The pair terms
+// are symmetric under i <-> j, so the cluster list's orientation of a pair
+// does not matter.  One 64-lane wave per i-cluster, lane = ii + 8 jj.
+#include "pgk_hf.h"
+
+namespace {
+
+struct HfPairArgs {
+  const int32_t *ci_start, *cj, *cl_atom;
+  const uint64_t *mask;
+  const double *q;
+  const int32_t *type, *molid, *inttyp;
+  const double *eps, *rmin;
+  int ncl, ntype;
+  double sqcut, sqcut2, rrcut;
+};
+
+__device__ __forceinline__ double r2_exact(const double *r) {
+#pragma clang fp contract(off)
+  return r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+}
+
+template <class AH>
+__global__ __launch_bounds__(256) void k_hf_pairs(HfPairArgs p, HfDev h) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + wave;
+  if (ci >= p.ncl) return;
+  const int ii = lane & 7, jj = lane >> 3;
+  const int ai = p.cl_atom[ci * PGK_CL + ii];
+  const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
+  for (int e = e0; e < e1; e++) {
+    const uint64_t m = p.mask[e];
+    if (!((m >> lane) & 1ull)) continue;
+    const int aj = p.cl_atom[p.cj[e] * PGK_CL + jj];
+    double r[3];
+    for (int d = 0; d < 3; d++) {
+      r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
+      r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
+    }
+    const double r2 = r2_exact(r);
+    if (r2 > p.sqcut) continue;
+    const double rinv2 = 1.0 / r2;
+    const double ra = sqrt(r2);
+    const double rinv1 = 1.0 / ra;
+    const bool same = p.molid[ai] == p.molid[aj];
+    if (ra <= p.rrcut) {
+      const double qq = p.q[ai] * p.q[aj];
+      const double ph = qq * rinv1;
+      const double fh = ph * rinv2;
+      const double fi[3] = {fh * r[0], fh * r[1], fh * r[2]};
+      const double fj[3] = {-fi[0], -fi[1], -fi[2]};
+      const int cls = same ? PGK_HF_ELIN : PGK_HF_ELC;
+      hf_pe_add<AH>(h, cls, ai, 0.5 * ph);
+      hf_pe_add<AH>(h, cls, aj, 0.5 * ph);
+      calvirihf_dev<AH>(h, ai, aj, fi, fj, 0.5, cls);
+    }
+    if (r2 <= p.sqcut2) {
+      const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
+      if (p.inttyp[tij] != 1) continue;
+      const double rad = p.rmin[tij];
+      const double t2 = rinv2 * (rad * rad);
+      const double t6 = t2 * t2 * t2;
+      const double t12 = t6 * t6;
+      const double ep = p.eps[tij];
+      const double pot = ep * (t12 - 2.0 * t6);
+      const double ft = 12.0 * ep * (t12 - t6) * rinv2;
+      const double fi[3] = {ft * r[0], ft * r[1], ft * r[2]};
+      const double fj[3] = {-fi[0], -fi[1], -fi[2]};
+      const int cls = same ? PGK_HF_LJIN : PGK_HF_VDW;
+      hf_pe_add<AH>(h, cls, ai, 0.5 * pot);
+      hf_pe_add<AH>(h, cls, aj, 0.5 * pot);
+      calvirihf_dev<AH>(h, ai, aj, fi, fj, 0.5, cls);
+    }
+  }
+}
+
+// accumulators -> doubles in the caller's layout: pot_atm[class][atom] and the
+// Fortran (3,3,natom,nregion) array per class
+template <class AH>
+__global__ __launch_bounds__(256) void k_hf_export(int natom, int nreg, int nw,
+                                                   const AH *__restrict__ pe,
+                                                   const AH *__restrict__ w,
+                                                   double *__restrict__ pe_out,
+                                                   double *__restrict__ w_out) {
+  const size_t per_cls = (size_t)nreg * natom;
+  const size_t total = (size_t)PGK_NHF * per_cls;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    // k = (cls, region, atom); 9 outputs each
+    double *o = w_out + 9 * k;
+    const AH *src = w + (size_t)nw * k;
+    if (nw == 9) {
+      for (int q = 0; q < 9; q++) o[q] = acc_load(&src[q]);
+    } else {
+      for (int q = 0; q < 6; q++) o[q] = 0.0;
+      for (int a = 0; a < 3; a++) o[6 + a] = acc_load(&src[a]);
+    }
+    if (k < (size_t)PGK_NHF * natom) pe_out[k] = acc_load(&pe[k]);
+  }
+}
+
+template <int PREC>
+int launch_hf(pgk_ctx *c, bool do_pairs) {
+  using AH = typename HfAcc<PREC>::type;
+  hipStream_t st = c->stream;
+  HfDev h = c->hfdev;
+  if (do_pairs && c->ncl > 0) {
+    HfPairArgs p;
+    p.ci_start = c->d_ci_start;
+    p.cj = c->d_cj;
+    p.cl_atom = c->d_cl_atom;
+    p.mask = c->d_mask;
+    p.q = c->d_q;
+    p.type = c->d_type;
+    p.molid = c->d_molid;
+    p.inttyp = c->d_inttyp;
+    p.eps = c->d_eps;
+    p.rmin = c->d_rmin;
+    p.ncl = c->ncl;
+    p.ntype = c->ntype;
+    double sq = c->rrcut * c->rrcut, sq2 = c->rcut * c->rcut;
+    p.sqcut = sq2 > sq ? sq2 : sq;
+    p.sqcut2 = sq2;
+    p.rrcut = c->rrcut;
+    k_hf_pairs<AH><<<pgk_grid_blocks(c->ncl, 4), 256, 0, st>>>(p, h);
+    HIP_CHECK(hipGetLastError());
+  }
+  return PGK_OK;
+}
+
+template <int PREC>
+int export_hf(pgk_ctx *c) {
+  using AH = typename HfAcc<PREC>::type;
+  const HfDev &h = c->hfdev;
+  size_t items = (size_t)PGK_NHF * h.nreg * h.natom;
+  int nb = (int)std::min<size_t>((items + 255) / 256, 65536);
+  k_hf_export<AH><<<nb, 256, 0, c->stream>>>(h.natom, h.nreg, h.nw,
+                                             reinterpret_cast<const AH *>(h.pe),
+                                             reinterpret_cast<const AH *>(h.w), c->d_hf_pe_out,
+                                             c->d_hf_w_out);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+}  // namespace
+
+int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs) {
+  switch (c->precision) {
+    case PGK_PREC_MIXED: return launch_hf<PGK_PREC_MIXED>(c, do_pairs);
+    case PGK_PREC_FP64: return launch_hf<PGK_PREC_FP64>(c, do_pairs);
+    default: return launch_hf<PGK_PREC_FP32>(c, do_pairs);
+  }
+}
+
+int pgk_hf_export(pgk_ctx *c) {
+  switch (c->precision) {
+    case PGK_PREC_MIXED: return export_hf<PGK_PREC_MIXED>(c);
+    case PGK_PREC_FP64: return export_hf<PGK_PREC_FP64>(c);
+    default: return export_hf<PGK_PREC_FP32>(c);
+  }
+}

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       xj_n = a.xq[cj_n * PGK_CL + jj];
       tj_n = a.ct[cj_n * PGK_CL + jj];
     }
-#pragma unroll 2
+
     for (int e = e0; e < e1; e++) {
       const int cj = cj_n;
       const uint64_t m = m_n;

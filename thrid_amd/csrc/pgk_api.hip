@@ -46,9 +46,6 @@ void pgk_prof_collect(pgk_ctx *c) {
     }
 }
 
-extern int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off,
-                              int *seg_n, int *seg_kind, int *nseg);
-
 namespace {
 
 enum {
@@ -261,8 +258,13 @@ void sym6_to9(const double *v6, double *t9, double s) {
   t9[5] += s * v6[5];
 }
 
+// One force call on the context's stream.  With hf (calforce_hf) the bonded
+// kernels also scatter the per-atom heat-flux terms, the pair heat-flux pass
+// runs over the list and the per-atom arrays are exported to the staging
+// buffers before the final synchronisation.
 int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom, bool split,
-                 double *d_force_out, double *d_fvc, double *d_fvl, double *ene, double *virt) {
+                 double *d_force_out, double *d_fvc, double *d_fvl, double *ene, double *virt,
+                 bool hf = false) {
   if (!c->have_pos) {
     pgk_set_error("pgk_calforce: positions not set");
     return PGK_ESTATE;
@@ -334,8 +336,9 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     seg_kind[nseg++] = SEG_PME;
   }
   pgk_prof_begin(c, PGK_PROF_BONDED);
-  if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg))) return rc;
+  if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
+  if (hf && do_pair && (rc = pgk_hf_launch_pairs(c, true))) return rc;
   // forces
   double *fout = d_force_out ? d_force_out : c->d_force;
   {
@@ -377,6 +380,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
                              hipMemcpyDeviceToHost, st));
   }
+  if (hf && (rc = pgk_hf_export(c))) return rc;
   HIP_CHECK(hipStreamSynchronize(st));
   c->acc_dirty = false;
   pgk_prof_collect(c);
@@ -789,7 +793,9 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
                   c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
-                  c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta};
+                  c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta,
+                  c->d_hf_z, c->d_hf_typ, c->d_hf_com, c->d_hf_pe, c->d_hf_w,
+                  c->d_hf_pe_out, c->d_hf_w_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -938,11 +944,103 @@ int pgk_calforce_device(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, int32_
                       nullptr, nullptr, ene, virt);
 }
 
+// grow a device buffer to at least `need` bytes (contents not kept)
+static int grow_dev(void **p, size_t *cap, size_t need) {
+  if (need <= *cap && *p) return PGK_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_CHECK(hipMalloc(p, need));
+  *cap = need;
+  return PGK_OK;
+}
+
 int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
                     pgk_force_out *out, pgk_hf_out *hfo) {
-  (void)c; (void)mts_flag; (void)mts; (void)hf; (void)out; (void)hfo;
-  pgk_set_error("pgk_calforce_hf: not available in this build");
-  return PGK_EINVAL;
+  if (!c || !mts || !hf || !out || !out->force || !out->ene || !hfo || !hfo->pot_atm ||
+      !hfo->viri_atm) {
+    pgk_set_error("pgk_calforce_hf: null argument");
+    return PGK_EINVAL;
+  }
+  if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
+      !hf->z2 || !hf->hftyp) {
+    pgk_set_error("pgk_calforce_hf: invalid heat-flux description");
+    return PGK_EINVAL;
+  }
+  const int n = c->natom, nreg = hf->nregion;
+  bool need_com = false;
+  for (int i = 0; i < n; i++) {
+    if (hf->hftyp[i] != PGK_HFTYP_ATOM && hf->hftyp[i] != PGK_HFTYP_MOLE) {
+      pgk_set_error("pgk_calforce_hf: hftyp must be ATOM (1) or MOLE (2)");
+      return PGK_EINVAL;
+    }
+    if (hf->hftyp[i] == PGK_HFTYP_MOLE) need_com = true;
+  }
+  int maxmol = 0;
+  for (int i = 0; i < n; i++) maxmol = std::max(maxmol, c->molid[i]);
+  if (need_com && (!hf->molecom || hf->nmol < maxmol)) {
+    pgk_set_error("pgk_calforce_hf: MOLE atoms need molecom for every molecule");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  const size_t nw = (hf->mode == PGK_HF_BULK) ? 9 : 3;
+  const size_t pe_b = sizeof(double) * PGK_NHF * (size_t)n;
+  const size_t w_b = sizeof(double) * PGK_NHF * (size_t)nreg * n * nw;
+  const size_t wout_b = sizeof(double) * PGK_NHF * (size_t)nreg * n * 9;
+  size_t typ_cap = c->d_hf_typ ? sizeof(int32_t) * n : 0;
+  size_t pe_cap = c->d_hf_pe ? pe_b : 0, peo_cap = c->d_hf_pe_out ? pe_b : 0;
+  if ((rc = grow_dev((void **)&c->d_hf_z, &c->hf_z_cap, sizeof(double) * 2 * nreg)) ||
+      (rc = grow_dev((void **)&c->d_hf_typ, &typ_cap, sizeof(int32_t) * n)) ||
+      (rc = grow_dev(&c->d_hf_pe, &pe_cap, pe_b)) ||
+      (rc = grow_dev(&c->d_hf_w, &c->hf_w_cap, w_b)) ||
+      (rc = grow_dev((void **)&c->d_hf_pe_out, &peo_cap, pe_b)) ||
+      (rc = grow_dev((void **)&c->d_hf_w_out, &c->hf_wout_cap, wout_b)))
+    return rc;
+  if (need_com &&
+      (rc = grow_dev((void **)&c->d_hf_com, &c->hf_com_cap, sizeof(double) * 3 * hf->nmol)))
+    return rc;
+  HIP_CHECK(hipMemcpyAsync(c->d_hf_z, hf->z1, sizeof(double) * nreg, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(c->d_hf_z + nreg, hf->z2, sizeof(double) * nreg,
+                           hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(c->d_hf_typ, hf->hftyp, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                           st));
+  if (need_com)
+    HIP_CHECK(hipMemcpyAsync(c->d_hf_com, hf->molecom, sizeof(double) * 3 * hf->nmol,
+                             hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(c->d_hf_pe, 0, pe_b, st));
+  HIP_CHECK(hipMemsetAsync(c->d_hf_w, 0, w_b, st));
+  HfDev &h = c->hfdev;
+  h.mode = hf->mode;
+  h.nreg = nreg;
+  h.natom = n;
+  h.nw = (int)nw;
+  h.z1 = c->d_hf_z;
+  h.z2 = c->d_hf_z + nreg;
+  h.hftyp = c->d_hf_typ;
+  h.molid = c->d_molid;
+  h.molecom = need_com ? c->d_hf_com : nullptr;
+  h.pos = c->d_pos;
+  for (int d = 0; d < 3; d++) {
+    h.L[d] = c->box[d];
+    h.Li[d] = 1.0 / c->box[d];
+  }
+  h.pe = c->d_hf_pe;
+  h.w = c->d_hf_w;
+  const bool split = out->for_viri_coul && out->for_viri_lj;
+  rc = run_calforce(c, mts_flag, mts, 1, split, c->d_force, c->d_fvc, c->d_fvl, out->ene,
+                    out->virt, true);
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * 3 * n;
+  HIP_CHECK(hipMemcpy(out->force, c->d_force, nb, hipMemcpyDeviceToHost));
+  if (split) {
+    HIP_CHECK(hipMemcpy(out->for_viri_coul, c->d_fvc, nb, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(out->for_viri_lj, c->d_fvl, nb, hipMemcpyDeviceToHost));
+  }
+  HIP_CHECK(hipMemcpy(hfo->pot_atm, c->d_hf_pe_out, pe_b, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(hfo->viri_atm, c->d_hf_w_out, wout_b, hipMemcpyDeviceToHost));
+  return PGK_OK;
 }
 
 int pgk_profile(pgk_ctx *c, int32_t enable, double *ms, int64_t *calls) {

@@ -57,6 +57,21 @@ enum {
 
 void pgk_set_error(const std::string &msg);
 
+// device view of one calforce_hf call's heat-flux set-up and accumulators
+// (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
+// else 3 = the z column only)
+struct HfDev {
+  int mode = 0, nreg = 0, natom = 0, nw = 3;
+  const double *z1 = nullptr, *z2 = nullptr;   // [nreg]
+  const int32_t *hftyp = nullptr;              // [natom] 1 = ATOM, 2 = MOLE
+  const int32_t *molid = nullptr;              // [natom] 1-based
+  const double *molecom = nullptr;             // [3*nmol]
+  const double *pos = nullptr;                 // [3*natom]
+  double L[3] = {0, 0, 0}, Li[3] = {0, 0, 0};
+  void *pe = nullptr;
+  void *w = nullptr;
+};
+
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
@@ -163,6 +178,15 @@ struct pgk_ctx {
   float *d_twf = nullptr;
   int tw_off[3] = {0, 0, 0};
 
+  // heat flux (calforce_hf): inputs, accumulators and export staging
+  HfDev hfdev;
+  double *d_hf_z = nullptr;        // z1 | z2
+  int32_t *d_hf_typ = nullptr;     // N
+  double *d_hf_com = nullptr;      // 3*nmol
+  void *d_hf_pe = nullptr, *d_hf_w = nullptr;
+  double *d_hf_pe_out = nullptr, *d_hf_w_out = nullptr;
+  size_t hf_z_cap = 0, hf_com_cap = 0, hf_w_cap = 0, hf_wout_cap = 0;
+
   // reduction segment bookkeeping for the current call
   int red_used = 0;
   // accumulators (force sums, charge grid) are cleared by the kernels that
@@ -190,6 +214,10 @@ int pgk_pairs_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_excl_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_pme_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_pme_prepare(pgk_ctx *c);
+int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
+                       int *seg_kind, int *nseg, bool hf);
+int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
+int pgk_hf_export(pgk_ctx *c);
 
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);
