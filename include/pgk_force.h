@@ -235,6 +235,18 @@ int pgk_calforce_device(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
                         int32_t ifcalpreatom, double *d_force, double *ene,
                         double *virt);
 
+/* Queue one calforce on the context's stream and return without waiting
+ * (no reference counterpart: the split of calforce for drivers that keep the
+ * step on the device).  The force goes to d_force ([3*natom] doubles on the
+ * device, NULL = the context's buffer); energies and virials are reduced on
+ * the device and read with pgk_energies.  Calls queue in order.             */
+int pgk_calforce_async(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
+                       int32_t ifcalpreatom, double *d_force);
+/* Wait for the context's work and return the energy record and virial
+ * tensors of the last calforce call, with pgk_calforce's semantics (ene is
+ * inout: energies of terms the call did not evaluate keep their values).    */
+int pgk_energies(pgk_ctx *ctx, double *ene, double *virt);
+
 /* Wait for all work queued by the context. */
 int pgk_synchronize(pgk_ctx *ctx);
 

@@ -130,3 +130,20 @@ def test_rebuild_after_motion(oracle_mod):
     idx, lst = o.mklist(s["rcut_book"], 0)
     assert np.array_equal(c.list_pairs(), pairs_from_csr(idx, lst, s["natom"]))
     assert rms_rel(c.calforce()["force"], o.calforce()["force"]) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_async_calls_match_sync(prec):
+    """pgk_calforce_async + pgk_energies give the synchronous call's record."""
+    s = small_slab()
+    c = _ctx(s, prec)
+    c.mklist(s["rcut_book"])
+    ref = c.calforce(abi.MTS_LONG)
+    for _ in range(3):
+        c.calforce_async()
+    e, v = c.energies()
+    if prec == "mixed":
+        assert np.array_equal(e, ref["ene"]) and np.array_equal(v, ref["virt"])
+    else:
+        np.testing.assert_allclose(e, ref["ene"], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(v, ref["virt"], rtol=1e-12, atol=1e-12)

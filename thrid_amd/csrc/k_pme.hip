@@ -22,6 +22,9 @@
 // are batched Stockham radix-2/3/4/5 FFTs in LDS; the convolution is fused
 // between the forward and backward z transforms, so the spectrum makes one
 // round trip through HBM.
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+
 #include "pgk_device.h"
 
 namespace {
@@ -181,9 +184,6 @@ __device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *tw, i
   }
 }
 
-__device__ __forceinline__ float grid_in_fx(const unsigned long long *g, size_t i) {
-  return (float)((double)(long long)g[i] * PGK_GRID_FX_INV);
-}
 
 // B-spline weights of order p at w (fill_bspline, fft_pme_init.F90:216-260)
 template <class R>
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
                                                   typename Prec<PREC>::acc *fat,
                                                   typename Prec<PREC>::acc *fat_c) {
   using R = typename Prec<PREC>::real;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nitem) return;
   const int i = order ? order[k] : k;
   if (i < 0) return;
@@ -395,118 +395,178 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
   }
 }
 
-template <int PREC>
-__global__ __launch_bounds__(256) void k_spread_w(int nitem, const double *__restrict__ q,
-                                                  PmeGeom g,
-                                                  const typename Prec<PREC>::real *__restrict__ th,
-                                                  const int4 *__restrict__ base, void *grid) {
-  using R = typename Prec<PREC>::real;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (k >= nitem) return;
-  const int4 b = base[k];
-  if (b.w < 0) return;
-  const int p = g.order, p2 = p * p, p3 = p2 * p;
-  const R qi = (R)q[b.w];
-  const R *t = th + k;
-  const size_t st = (size_t)nitem;
-  for (int pt = lane; pt < p3; pt += 64) {
-    const int a3 = pt / p2, rem = pt - a3 * p2;
-    const int a2 = rem / p, a1 = rem - a2 * p;
-    int kk = b.z + a3, jj = b.y + a2, ii = b.x + a1;
-    if (kk < 0) kk += g.K[2];
-    if (jj < 0) jj += g.K[1];
-    if (ii < 0) ii += g.K[0];
-    const R v = t[a1 * st] * (t[(p + a2) * st] * (t[(2 * p + a3) * st] * qi));
-    const size_t idx = ((size_t)kk * g.K[1] + jj) * g.K[0] + ii;
-    if (PREC == PGK_PREC_FP64) {
-      atomicAdd(reinterpret_cast<double *>(grid) + idx, (double)v);
-    } else if (PREC == PGK_PREC_MIXED) {
-      long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
-      atomicAdd(reinterpret_cast<unsigned long long *>(grid) + idx, (unsigned long long)x);
-    } else {
-      atomicAdd(reinterpret_cast<float *>(grid) + idx, (float)v);
-    }
-  }
+// ---- charge spreading (fill_charge_grid, fft_pme.F90:359-422) -------------
+// Owner-computes: the grid is cut into equal tiles of T0 x T1 x T2 points
+// (T = largest divisor of K that is <= 8).  Atoms are binned by the tile of
+// their first stencil point; a stencil of p points reaches nb = (T+p-2)/T
+// further tiles, so tile t reads the bins t, t-1, ..., t-nb of each axis.
+// One workgroup per tile accumulates the points it owns in LDS (int64 fixed
+// point in the mixed form: order-independent) and writes each grid point
+// once, in the compute type.  No device atomics, no clearing pass.
+#define PGK_MAXSRC 125   // source bins per tile (5 per axis)
+
+struct TileGeom {
+  int T[3], nt[3], nsrc[3];   // tile size, tiles per axis, source bins per axis
+  int ntile;
+};
+
+__device__ __forceinline__ int tile_home(const int4 &b, const PmeGeom &g, const TileGeom &tg) {
+  const int bx = b.x < 0 ? b.x + g.K[0] : b.x;
+  const int by = b.y < 0 ? b.y + g.K[1] : b.y;
+  const int bz = b.z < 0 ? b.z + g.K[2] : b.z;
+  return ((bz / tg.T[2]) * tg.nt[1] + by / tg.T[1]) * tg.nt[0] + bx / tg.T[0];
 }
 
-// four atoms per wavefront, 16 lanes per atom (one x-y plane of the stencil
-// per step over a3); p = 4 only
-template <int PREC>
-__global__ __launch_bounds__(256) void k_spread_q(int nitem, const double *__restrict__ q,
-                                                  PmeGeom g,
-                                                  const typename Prec<PREC>::real *__restrict__ th,
-                                                  const int4 *__restrict__ base, void *grid) {
-  using R = typename Prec<PREC>::real;
-  const int lane = threadIdx.x & 63;
-  const int k = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
-  if (k >= nitem) return;
-  const int4 b = base[k];
-  if (b.w < 0) return;
-  const int sub = lane & 15, a1 = sub & 3, a2 = sub >> 2;
-  const R qi = (R)q[b.w];
-  const size_t st = (size_t)nitem;
-  const R *t = th + k;
-  int ii = b.x + a1, jj = b.y + a2;
-  if (ii < 0) ii += g.K[0];
-  if (jj < 0) jj += g.K[1];
-  const R w12 = t[a1 * st] * t[(4 + a2) * st] * qi;
-#pragma unroll
-  for (int a3 = 0; a3 < 4; a3++) {
-    int kk = b.z + a3;
-    if (kk < 0) kk += g.K[2];
-    const R v = w12 * t[(8 + a3) * st];
-    const size_t idx = ((size_t)kk * g.K[1] + jj) * g.K[0] + ii;
-    if (PREC == PGK_PREC_FP64) {
-      atomicAdd(reinterpret_cast<double *>(grid) + idx, (double)v);
-    } else if (PREC == PGK_PREC_MIXED) {
-      long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
-      atomicAdd(reinterpret_cast<unsigned long long *>(grid) + idx, (unsigned long long)x);
-    } else {
-      atomicAdd(reinterpret_cast<float *>(grid) + idx, (float)v);
-    }
-  }
+__global__ __launch_bounds__(256) void k_tile_zero(int n, int32_t *__restrict__ count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) count[k] = 0;
 }
 
-// thread per atom in the sorted order, all p^3 points in sequence
-template <int PREC>
-__global__ __launch_bounds__(256) void k_spread_t(int nitem, const double *__restrict__ q,
-                                                  PmeGeom g,
-                                                  const typename Prec<PREC>::real *__restrict__ th,
-                                                  const int4 *__restrict__ base, void *grid) {
-  using R = typename Prec<PREC>::real;
+// rank of every item inside its home bin.  Items of one block are spatially
+// close, so the block first counts in LDS (open addressing on the tile id)
+// and takes one global atomic per distinct tile.
+__global__ __launch_bounds__(256) void k_tile_count(int nitem, const int4 *__restrict__ base,
+                                                    PmeGeom g, TileGeom tg,
+                                                    int32_t *__restrict__ count,
+                                                    int32_t *__restrict__ rank) {
+  __shared__ int s_key[512], s_cnt[512], s_base[512];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    s_key[i] = -1;
+    s_cnt[i] = 0;
+  }
+  __syncthreads();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = -1, slot = -1, r = -1;
+  if (k < nitem) {
+    const int4 b = base[k];
+    if (b.w >= 0) t = tile_home(b, g, tg);
+  }
+  if (t >= 0) {
+    int h = (int)(((unsigned)t * 2654435761u) >> 23);   // 9-bit hash
+    for (int probe = 0; probe < 512; probe++) {
+      const int old = atomicCAS(&s_key[h], -1, t);
+      if (old == -1 || old == t) {
+        slot = h;
+        break;
+      }
+      h = (h + 1) & 511;
+    }
+    r = atomicAdd(&s_cnt[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x)
+    if (s_key[i] >= 0) s_base[i] = atomicAdd(&count[s_key[i]], s_cnt[i]);
+  __syncthreads();
+  if (k < nitem) rank[k] = (t >= 0) ? s_base[slot] + r : -1;
+}
+
+__global__ __launch_bounds__(256) void k_tile_fill(int nitem, const int4 *__restrict__ base,
+                                                   PmeGeom g, TileGeom tg,
+                                                   const int32_t *__restrict__ start,
+                                                   const int32_t *__restrict__ rank,
+                                                   int32_t *__restrict__ items) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nitem) return;
-  const int4 b = base[k];
-  if (b.w < 0) return;
-  const int p = g.order;
-  const R qi = (R)q[b.w];
+  const int r = rank[k];
+  if (r < 0) return;
+  items[start[tile_home(base[k], g, tg)] + r] = k;
+}
+
+template <int PREC> struct SpreadAcc { using type = typename Prec<PREC>::real; };
+template <> struct SpreadAcc<PGK_PREC_MIXED> { using type = unsigned long long; };
+
+template <int PREC, int P>
+__global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__restrict__ q,
+                                                     PmeGeom g, TileGeom tg,
+                                                     const int32_t *__restrict__ start,
+                                                     const int32_t *__restrict__ items,
+                                                     const typename Prec<PREC>::real *__restrict__ th,
+                                                     const int4 *__restrict__ base,
+                                                     typename Prec<PREC>::real *__restrict__ grid) {
+  using R = typename Prec<PREC>::real;
+  using S = typename SpreadAcc<PREC>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  S *acc = reinterpret_cast<S *>(smem);
+  __shared__ int s_lo[PGK_MAXSRC], s_n[PGK_MAXSRC + 1];
+  const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
+  const int npt = T0 * T1 * T2;
+  const int tid = xcd_block();
+  if (tid >= tg.ntile) return;
+  const int tx = tid % tg.nt[0], ty = (tid / tg.nt[0]) % tg.nt[1], tz = tid / (tg.nt[0] * tg.nt[1]);
+  const int ox = tx * T0, oy = ty * T1, oz = tz * T2;
+  for (int i = threadIdx.x; i < npt; i += blockDim.x) acc[i] = S(0);
+  // the source bins of this tile, flattened into one index space
+  const int nsrc = tg.nsrc[0] * tg.nsrc[1] * tg.nsrc[2];
+  if (threadIdx.x == 0) {
+    int tot = 0, u = 0;
+    for (int dz = 0; dz < tg.nsrc[2]; dz++)
+      for (int dy = 0; dy < tg.nsrc[1]; dy++)
+        for (int dx = 0; dx < tg.nsrc[0]; dx++) {
+          const int sx = (tx - dx + tg.nt[0]) % tg.nt[0];
+          const int sy = (ty - dy + tg.nt[1]) % tg.nt[1];
+          const int sz = (tz - dz + tg.nt[2]) % tg.nt[2];
+          const int bin = (sz * tg.nt[1] + sy) * tg.nt[0] + sx;
+          s_lo[u] = start[bin];
+          s_n[u] = tot;
+          tot += start[bin + 1] - start[bin];
+          u++;
+        }
+    s_n[nsrc] = tot;
+  }
+  __syncthreads();
+  const int total = s_n[nsrc];
   const size_t st = (size_t)nitem;
-  const R *t = th + k;
-  for (int a3 = 0; a3 < p; a3++) {
-    int kk = b.z + a3;
-    if (kk < 0) kk += g.K[2];
-    const R p3 = t[(2 * p + a3) * st] * qi;
-    for (int a2 = 0; a2 < p; a2++) {
-      int jj = b.y + a2;
-      if (jj < 0) jj += g.K[1];
-      const R p2 = t[(p + a2) * st] * p3;
-      const size_t row = ((size_t)kk * g.K[1] + jj) * g.K[0];
-      for (int a1 = 0; a1 < p; a1++) {
-        int ii = b.x + a1;
-        if (ii < 0) ii += g.K[0];
-        const R v = t[a1 * st] * p2;
-        if (PREC == PGK_PREC_FP64) {
-          atomicAdd(reinterpret_cast<double *>(grid) + row + ii, (double)v);
-        } else if (PREC == PGK_PREC_MIXED) {
-          long long x = __double2ll_rn((double)v * PGK_GRID_FX_SCALE);
-          atomicAdd(reinterpret_cast<unsigned long long *>(grid) + row + ii,
-                    (unsigned long long)x);
-        } else {
-          atomicAdd(reinterpret_cast<float *>(grid) + row + ii, (float)v);
+  // one thread per atom visit: its weights in registers, then the p^3 points
+  for (int v = threadIdx.x; v < total; v += blockDim.x) {
+    int u = 0;
+    while (u + 1 < nsrc && v >= s_n[u + 1]) u++;
+    const int k = items[s_lo[u] + (v - s_n[u])];
+    const int4 b = base[k];
+    const R qi = (R)q[b.w];
+    R w1[P], w2[P], w3[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      w1[a] = th[(size_t)a * st + k];
+      w2[a] = th[(size_t)(P + a) * st + k];
+      w3[a] = th[(size_t)(2 * P + a) * st + k] * qi;
+    }
+#pragma unroll
+    for (int a3 = 0; a3 < P; a3++) {
+      int iz = b.z + a3;
+      if (iz < 0) iz += g.K[2];
+      const unsigned lz = (unsigned)(iz - oz);
+      if (lz >= (unsigned)T2) continue;
+#pragma unroll
+      for (int a2 = 0; a2 < P; a2++) {
+        int iy = b.y + a2;
+        if (iy < 0) iy += g.K[1];
+        const unsigned ly = (unsigned)(iy - oy);
+        if (ly >= (unsigned)T1) continue;
+        const R w23 = w2[a2] * w3[a3];
+#pragma unroll
+        for (int a1 = 0; a1 < P; a1++) {
+          int ix = b.x + a1;
+          if (ix < 0) ix += g.K[0];
+          const unsigned lx = (unsigned)(ix - ox);
+          if (lx >= (unsigned)T0) continue;
+          const R val = w1[a1] * w23;
+          S *dst = &acc[(lz * T1 + ly) * T0 + lx];
+          if (PREC == PGK_PREC_MIXED)
+            atomicAdd(dst, (S)__double2ll_rn((double)val * PGK_GRID_FX_SCALE));
+          else
+            atomicAdd(dst, (S)val);
         }
       }
     }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < npt; i += blockDim.x) {
+    const int lx = i % T0, ly = (i / T0) % T1, lz = i / (T0 * T1);
+    const size_t idx = ((size_t)(oz + lz) * g.K[1] + (oy + ly)) * g.K[0] + (ox + lx);
+    if (PREC == PGK_PREC_MIXED)
+      grid[idx] = (R)((double)(long long)acc[i] * PGK_GRID_FX_INV);
+    else
+      grid[idx] = (R)acc[i];
   }
 }
 
@@ -562,7 +622,7 @@ __global__ __launch_bounds__(256) void k_gather_s(int nitem, const int32_t *__re
 // x pass, forward: pairs of real rows -> half spectra.  One block handles NL
 // complex lines (2*NL rows).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(void *grid, void *cgrid, int nrow,
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid, int nrow,
                                                     int NL, FftPlan p, const void *twv, int H) {
   using R = typename Prec<PREC>::real;
   using C = typename C2T<R>::type;
@@ -578,20 +638,9 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(void *grid, void *cgrid, int 
     int l = fdiv(t, n, rn), k = t - l * n;
     int ra = row0 + 2 * l, rb = ra + 1;
     C z;
-    // the charge grid is consumed here and left zeroed for the next spread
-    if (PREC == PGK_PREC_MIXED) {
-      unsigned long long *gi = reinterpret_cast<unsigned long long *>(grid);
-      z.x = (ra < nrow) ? grid_in_fx(gi, (size_t)ra * n + k) : R(0);
-      z.y = (rb < nrow) ? grid_in_fx(gi, (size_t)rb * n + k) : R(0);
-      if (ra < nrow) gi[(size_t)ra * n + k] = 0ull;
-      if (rb < nrow) gi[(size_t)rb * n + k] = 0ull;
-    } else {
-      R *gr = reinterpret_cast<R *>(grid);
-      z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
-      z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
-      if (ra < nrow) gr[(size_t)ra * n + k] = R(0);
-      if (rb < nrow) gr[(size_t)rb * n + k] = R(0);
-    }
+    const R *gr = reinterpret_cast<const R *>(grid);
+    z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
+    z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
     pa[t] = z;
   }
   __syncthreads();
@@ -854,15 +903,65 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
       default: k_bspline<PREC><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base);
     }
   }
-  if (c->spread_variant == 1 && c->order == 4)
-    k_spread_q<PREC><<<pgk_grid_blocks(nitem, 16), 256, 0, st>>>(nitem, c->d_q, g, th, base,
-                                                                 c->d_grid);
-  else if (c->spread_variant == 2)
-    k_spread_t<PREC><<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, c->d_q, g, th, base,
-                                                                  c->d_grid);
-  else
-    k_spread_w<PREC><<<pgk_grid_blocks(nitem, 4), 256, 0, st>>>(nitem, c->d_q, g, th, base,
-                                                                c->d_grid);
+  {
+    TileGeom tg;
+    tg.ntile = 1;
+    for (int d = 0; d < 3; d++) {
+      int T = 8;
+      while (c->nfft[d] % T) T--;
+      tg.T[d] = T;
+      tg.nt[d] = c->nfft[d] / T;
+      const int nb = (T + c->order - 2) / T;
+      tg.nsrc[d] = std::min(nb + 1, tg.nt[d]);
+      tg.ntile *= tg.nt[d];
+    }
+    if (tg.nsrc[0] * tg.nsrc[1] * tg.nsrc[2] > PGK_MAXSRC) {
+      pgk_set_error("SPME grid tiles too small for this pme_order");
+      return PGK_EINVAL;
+    }
+    // bins: count[ntile + 1] | start[ntile + 1] | rank[nitem] | items[nitem]
+    size_t need = sizeof(int32_t) * (2 * (size_t)(tg.ntile + 1) + 2 * (size_t)nitem);
+    size_t scan_bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t *)nullptr,
+                                           (int32_t *)nullptr, tg.ntile + 1, st);
+    need += scan_bytes + 256;
+    if (need > c->tile_bytes) {
+      if (c->d_tile) (void)hipFree(c->d_tile);
+      c->d_tile = nullptr;
+      c->tile_bytes = 0;
+      HIP_CHECK(hipMalloc(&c->d_tile, need));
+      c->tile_bytes = need;
+    }
+    int32_t *count = reinterpret_cast<int32_t *>(c->d_tile);
+    int32_t *start = count + tg.ntile + 1;
+    int32_t *rank = start + tg.ntile + 1;
+    int32_t *items = rank + nitem;
+    void *scan_tmp = reinterpret_cast<char *>(c->d_tile) +
+                     ((sizeof(int32_t) * (2 * (size_t)(tg.ntile + 1) + 2 * (size_t)nitem) + 255) &
+                      ~(size_t)255);
+    k_tile_zero<<<pgk_grid_blocks(tg.ntile + 1, 256), 256, 0, st>>>(tg.ntile + 1, count);
+    k_tile_count<<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, base, g, tg, count, rank);
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, count, start,
+                                               tg.ntile + 1, st));
+    k_tile_fill<<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, base, g, tg, start, rank,
+                                                             items);
+    const size_t lds = (size_t)tg.T[0] * tg.T[1] * tg.T[2] * sizeof(typename SpreadAcc<PREC>::type);
+    R *gr = reinterpret_cast<R *>(c->d_grid);
+#define PGK_SPREAD(P)                                                                         \
+  k_spread_tile<PREC, P><<<xcd_grid(tg.ntile), 256, lds, st>>>(nitem, c->d_q, g, tg, start,   \
+                                                                items, th, base, gr)
+    switch (c->order) {
+      case 3: PGK_SPREAD(3); break;
+      case 4: PGK_SPREAD(4); break;
+      case 5: PGK_SPREAD(5); break;
+      case 6: PGK_SPREAD(6); break;
+      case 7: PGK_SPREAD(7); break;
+      case 8: PGK_SPREAD(8); break;
+      case 9: PGK_SPREAD(9); break;
+      default: PGK_SPREAD(10); break;
+    }
+#undef PGK_SPREAD
+  }
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_SPREAD);
   pgk_prof_begin(c, PGK_PROF_FFT);
@@ -915,8 +1014,9 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   {
     const int32_t *ord = sorted ? c->d_cl_atom : nullptr;
     const int gb = pgk_grid_blocks(nitem, 256);
+    const int gbx = xcd_grid(gb);
 #define PGK_GATHER(SP, P)                                                                 \
-  k_gather_t<PREC, SP, P><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid, \
+  k_gather_t<PREC, SP, P><<<gbx, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid, \
                                               fat, fatc)
     if (split) {
       switch (c->order) {

@@ -21,29 +21,31 @@ void pgk_set_error(const std::string &msg) { g_err = msg; }
 
 void pgk_prof_begin(pgk_ctx *c, int k) {
   if (!c->prof) return;
-  if (!c->ev[2 * k]) {
-    (void)hipEventCreate(&c->ev[2 * k]);
-    (void)hipEventCreate(&c->ev[2 * k + 1]);
+  const int i = c->pn++;
+  while ((int)c->pev.size() < 2 * (i + 1)) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    c->pev.push_back(e);
   }
-  (void)hipEventRecord(c->ev[2 * k], c->stream);
+  if ((int)c->pstage.size() < i + 1) c->pstage.resize(i + 1);
+  c->pstage[i] = k;
+  c->popen[k] = i;
+  (void)hipEventRecord(c->pev[2 * i], c->stream);
 }
 void pgk_prof_end(pgk_ctx *c, int k) {
   if (!c->prof) return;
-  (void)hipEventRecord(c->ev[2 * k + 1], c->stream);
-  c->ev_used[k] = true;
+  (void)hipEventRecord(c->pev[2 * c->popen[k] + 1], c->stream);
 }
 // call after the stream has been synchronised
 void pgk_prof_collect(pgk_ctx *c) {
-  if (!c->prof) return;
-  for (int k = 0; k < PGK_NPROF; k++)
-    if (c->ev_used[k]) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess) {
-        c->prof_ms[k] += ms;
-        c->prof_calls[k] += 1;
-      }
-      c->ev_used[k] = false;
+  for (int i = 0; i < c->pn; i++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->pev[2 * i], c->pev[2 * i + 1]) == hipSuccess) {
+      c->prof_ms[c->pstage[i]] += ms;
+      c->prof_calls[c->pstage[i]] += 1;
     }
+  }
+  c->pn = 0;
 }
 
 namespace {
@@ -262,9 +264,11 @@ void sym6_to9(const double *v6, double *t9, double s) {
 // kernels also scatter the per-atom heat-flux terms, the pair heat-flux pass
 // runs over the list and the per-atom arrays are exported to the staging
 // buffers before the final synchronisation.
+int finalize_energies(pgk_ctx *c, double *ene, double *virt);
+
 int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom, bool split,
                  double *d_force_out, double *d_fvc, double *d_fvl, double *ene, double *virt,
-                 bool hf = false) {
+                 bool hf = false, bool wait = true) {
   if (!c->have_pos) {
     pgk_set_error("pgk_calforce: positions not set");
     return PGK_ESTATE;
@@ -307,8 +311,6 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipMemsetAsync(c->d_fcl, 0, nsc * 3 * ab, st));
       HIP_CHECK(hipMemsetAsync(c->d_fcl_c, 0, nsc * 3 * ab, st));
     }
-    size_t G = (size_t)c->nfft[0] * c->nfft[1] * c->nfft[2];
-    HIP_CHECK(hipMemsetAsync(c->d_grid, 0, G * 8, st));
   }
   c->acc_dirty = true;
   if (do_pair) {
@@ -365,25 +367,56 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     }
     HIP_CHECK(hipGetLastError());
   }
-  // energies
-  double segsum[16][PGK_REC];
+  // energies: fixed-order segment sums into d_sum, copied to pinned h_sum
   if (nseg > 0) {
     int *dseg = reinterpret_cast<int *>(c->d_sum + 512);
     int *hseg = reinterpret_cast<int *>(c->h_sum + 512);   // pinned staging
+    int tab[32] = {};
     for (int k = 0; k < nseg; k++) {
-      hseg[k] = seg_off[k];
-      hseg[16 + k] = seg_n[k];
+      tab[k] = seg_off[k];
+      tab[16 + k] = seg_n[k];
     }
-    HIP_CHECK(hipMemcpyAsync(dseg, hseg, 32 * sizeof(int), hipMemcpyHostToDevice, st));
+    if (c->seg_up_n != nseg || memcmp(tab, c->seg_up, sizeof(tab)) != 0) {
+      // an earlier queued upload may still read the staging area
+      HIP_CHECK(hipStreamSynchronize(st));
+      memcpy(hseg, tab, sizeof(tab));
+      HIP_CHECK(hipMemcpyAsync(dseg, hseg, 32 * sizeof(int), hipMemcpyHostToDevice, st));
+      memcpy(c->seg_up, tab, sizeof(tab));
+      c->seg_up_n = nseg;
+    }
     k_reduce_segments<<<nseg * PGK_REC, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
                              hipMemcpyDeviceToHost, st));
   }
   if (hf && (rc = pgk_hf_export(c))) return rc;
-  HIP_CHECK(hipStreamSynchronize(st));
-  c->acc_dirty = false;
+  c->last_nseg = nseg;
+  for (int k = 0; k < nseg; k++) c->last_seg_kind[k] = seg_kind[k];
+  c->last_mts_flag = mts_flag;
+  c->last_mts = *m;
+  c->last_pending = true;
+  c->acc_dirty = false;   // the queued kernels consume and clear every accumulator
+  if (!wait) return PGK_OK;
+  return finalize_energies(c, ene, virt);
+}
+
+// Wait for the last call and turn its segment sums into the energy record
+// and virial tensors (calforce.F90:1018-1068 for the net-charge terms and
+// the totals).  ene is inout: energies of terms not evaluated keep their
+// values.
+int finalize_energies(pgk_ctx *c, double *ene, double *virt) {
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   pgk_prof_collect(c);
+  if (!c->last_pending) {
+    pgk_set_error("no calforce call to read energies from");
+    return PGK_ESTATE;
+  }
+  const int nseg = c->last_nseg;
+  const int *seg_kind = c->last_seg_kind;
+  const pgk_mts *m = &c->last_mts;
+  const int32_t mts_flag = c->last_mts_flag;
+  const bool do_pair = (m->vdw == mts_flag);
+  double segsum[16][PGK_REC];
   for (int k = 0; k < nseg; k++) memcpy(segsum[k], c->h_sum + k * PGK_REC, sizeof(segsum[k]));
 
   double E[PGK_NENE];
@@ -465,7 +498,6 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
                     E[PGK_E_VDW14] + E[PGK_E_ELC14];
   E[PGK_E_TOT] = E[PGK_E_BOND] + E[PGK_E_ANGL] + E[PGK_E_TORS] + E[PGK_E_TORSRB] +
                  E[PGK_E_NONBON];
-  (void)preatom;
   memcpy(ene, E, sizeof(E));
   if (virt) memcpy(virt, T, sizeof(T));
   return PGK_OK;
@@ -520,7 +552,6 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
     }
   pgk_ctx *c = new pgk_ctx();
   c->precision = precision;
-  if (const char *v = getenv("PGK_SPREAD_VARIANT")) c->spread_variant = atoi(v);
   (void)hipGetDevice(&c->device);
   c->natom = n;
   c->ntype = nt;
@@ -793,14 +824,14 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
                   c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
-                  c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta,
+                  c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta, c->d_tile,
                   c->d_hf_z, c->d_hf_typ, c->d_hf_com, c->d_hf_pe, c->d_hf_w,
                   c->d_hf_pe_out, c->d_hf_w_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
-  for (int k = 0; k < 2 * PGK_NPROF; k++)
-    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  for (hipEvent_t e : c->pev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PGK_OK;
@@ -955,6 +986,26 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
   return PGK_OK;
 }
 
+int pgk_calforce_async(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, int32_t ifcalpreatom,
+                       double *d_force) {
+  if (!c || !mts) {
+    pgk_set_error("pgk_calforce_async: null argument");
+    return PGK_EINVAL;
+  }
+  return run_calforce(c, mts_flag, mts, ifcalpreatom, false, d_force ? d_force : c->d_force,
+                      nullptr, nullptr, nullptr, nullptr, false, false);
+}
+
+int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
+  if (!c || !ene) {
+    pgk_set_error("pgk_energies: null argument");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  return finalize_energies(c, ene, virt);
+}
+
 int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
                     pgk_force_out *out, pgk_hf_out *hfo) {
   if (!c || !mts || !hf || !out || !out->force || !out->ene || !hfo || !hfo->pot_atm ||
@@ -1045,6 +1096,10 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
 
 int pgk_profile(pgk_ctx *c, int32_t enable, double *ms, int64_t *calls) {
   if (!c) return PGK_EINVAL;
+  if (c->pn > 0) {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    pgk_prof_collect(c);
+  }
   if (ms)
     for (int k = 0; k < PGK_NPROF; k++) ms[k] = c->prof_ms[k];
   if (calls)
@@ -1075,6 +1130,7 @@ int pgk_list_info(pgk_ctx *c, int64_t *info) {
 int pgk_synchronize(pgk_ctx *c) {
   if (!c) return PGK_EINVAL;
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  pgk_prof_collect(c);
   return PGK_OK;
 }
 

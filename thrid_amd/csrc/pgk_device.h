@@ -45,6 +45,17 @@ __device__ __forceinline__ double acc_load(const unsigned long long *p) {
   return (double)(long long)(*p) * PGK_FX_INV;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8)
+// and every XCD has its own L2.  This remaps the block index so that each XCD
+// walks one contiguous eighth of the work, which for spatially sorted data is
+// a compact region.  The grid must be launched with a multiple of 8 blocks
+// (xcd_grid); logical blocks past the real work must return.
+static inline int xcd_grid(int nblk) { return (nblk + 7) & ~7; }
+__device__ __forceinline__ int xcd_block() {
+  const int per = (int)(gridDim.x >> 3);
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
 // Fortran dnint/anint: round half away from zero
 __device__ __forceinline__ double fnint(double x) { return round(x); }
 __device__ __forceinline__ float fnint(float x) { return roundf(x); }

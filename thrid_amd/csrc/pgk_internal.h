@@ -168,12 +168,14 @@ struct pgk_ctx {
   double *h_sum = nullptr;         // pinned R_N
 
   // PME
-  void *d_grid = nullptr;          // charge grid (n1*n2*n3): double, int64 (mixed) or float;
-                                   // zero between calls (the x pass clears it)
+  void *d_grid = nullptr;          // charge grid (n1*n2*n3) in the compute type, written
+                                   // point by point by the tile spread
   void *d_pgrid = nullptr;         // potential grid (n1*n2*n3) in the compute type
   void *d_cgrid = nullptr;         // complex half grid ((n1/2+1)*n2*n3)
   void *d_theta = nullptr;         // B-spline weights + base indices per atom
   size_t theta_bytes = 0;
+  void *d_tile = nullptr;          // spreading bins: counts, starts, ranks, items, scan space
+  size_t tile_bytes = 0;
   double *d_tw = nullptr;          // twiddles: 3 dims, each n complex (FP64)
   float *d_twf = nullptr;
   int tw_off[3] = {0, 0, 0};
@@ -192,15 +194,26 @@ struct pgk_ctx {
   // accumulators (force sums, charge grid) are cleared by the kernels that
   // consume them; set when a call did not run to completion
   bool acc_dirty = true;
-  // kernel variant selectors (PGK_SPREAD_VARIANT): measurement aid
-  int spread_variant = 0;
 
-  // optional per-stage timing with HIP events on the context's stream
+  // optional per-stage timing: every stage instance records a begin/end pair
+  // of HIP events from a pool; pairs are read at the next synchronisation
   bool prof = false;
-  hipEvent_t ev[2 * PGK_NPROF] = {};
-  bool ev_used[PGK_NPROF] = {};
+  std::vector<hipEvent_t> pev;     // 2 per pending stage instance
+  std::vector<int> pstage;
+  int pn = 0;                      // pending stage instances
+  int popen[PGK_NPROF] = {};
   double prof_ms[PGK_NPROF] = {};
   int64_t prof_calls[PGK_NPROF] = {};
+
+  // the last calforce call, finalised by pgk_energies (or at once when the
+  // call waits): its reduction segments and the inputs of the energy record
+  int last_nseg = 0;
+  int last_seg_kind[16] = {};
+  int seg_up[32] = {};             // segment table currently on the device
+  int seg_up_n = -1;
+  int32_t last_mts_flag = 0;
+  pgk_mts last_mts = {};
+  bool last_pending = false;
 };
 
 // stage markers (record before/after a stage when profiling is on)

@@ -162,6 +162,25 @@ module pgk_force_mod
        integer(c_int) :: rc
      end function pgk_calforce_hf
 
+     function pgk_calforce_async(ctx, mts_flag, mts, ifcalpreatom, d_force) &
+          & bind(C, name='pgk_calforce_async') result(rc)
+       import :: c_ptr, c_int, c_int32_t, pgk_mts
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mts_flag
+       type(pgk_mts), intent(in) :: mts
+       integer(c_int32_t), value :: ifcalpreatom
+       type(c_ptr), value :: d_force
+       integer(c_int) :: rc
+     end function pgk_calforce_async
+
+     function pgk_energies(ctx, ene, virt) bind(C, name='pgk_energies') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: ene(*)
+       real(c_double), intent(out) :: virt(*)
+       integer(c_int) :: rc
+     end function pgk_energies
+
      function pgk_synchronize(ctx) bind(C, name='pgk_synchronize') result(rc)
        import :: c_ptr, c_int
        type(c_ptr), value :: ctx

@@ -49,13 +49,16 @@ def load_library(path: str = LIB_PATH):
     L.pgk_calforce_hf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
                                   C.POINTER(abi.ForceOut), C.POINTER(abi.HfOut)]
     L.pgk_calforce_device.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32, vp, vp, vp]
+    L.pgk_calforce_async.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32, vp]
+    L.pgk_energies.argtypes = [vp, vp, vp]
     L.pgk_synchronize.argtypes = [vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
     for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
               "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
-              "pgk_synchronize", "pgk_profile", "pgk_list_info"):
+              "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
+              "pgk_list_info"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -152,6 +155,20 @@ class ForceContext:
                                           C.c_void_p(d_force), e.ctypes.data, v.ctypes.data),
                "pgk_calforce_device")
         return e, v
+
+    def calforce_async(self, d_force=0, mts_flag=abi.MTS_LONG, mts=None, ifcalpreatom=1):
+        """Queue one force call without waiting; read energies with energies()."""
+        mts = mts or abi.mts_all(mts_flag)
+        self._mts_keep = mts
+        _check(self.L.pgk_calforce_async(self.h, mts_flag, C.byref(mts), ifcalpreatom,
+                                         C.c_void_p(d_force)), "pgk_calforce_async")
+
+    def energies(self, ene=None):
+        """Wait, then the energy record and virial tensors of the last call."""
+        e = np.zeros(abi.NENE) if ene is None else np.array(ene, np.float64)
+        v = np.zeros((abi.NTENS, 9))
+        _check(self.L.pgk_energies(self.h, e.ctypes.data, v.ctypes.data), "pgk_energies")
+        return e, v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy()
 
     def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None):
         mts = mts or abi.mts_all(mts_flag)
