@@ -147,3 +147,17 @@ def test_async_calls_match_sync(prec):
     else:
         np.testing.assert_allclose(e, ref["ene"], rtol=1e-12, atol=1e-12)
         np.testing.assert_allclose(v, ref["virt"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [3, 5, 6, 8, 10])
+def test_spme_orders_and_tiles(oracle_mod, order):
+    """B-spline orders 3..10 on a 24 x 20 x 27 grid (tile edges 8, 5, 3)."""
+    s = systems.water_box(nfft=(24, 20, 27), order=order)
+    ref = _oracle(oracle_mod, s, mode=0).calforce(abi.MTS_LONG)
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
+        c = _ctx(s, prec)
+        c.mklist(s["rcut_book"])
+        got = c.calforce(abi.MTS_LONG)
+        err = rms_rel(got["force"], ref["force"])
+        assert err <= tol, (prec, err)
+        assert abs(ene(got, "pot_ewk") - ene(ref, "pot_ewk")) <= 10 * tol * abs(ene(ref, "pot_ewk"))

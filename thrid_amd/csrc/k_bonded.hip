@@ -341,14 +341,14 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     a.red = c->d_red + (size_t)c->red_used * PGK_REC;
     c->red_used += nblk;
   };
-  int nb = (int)(c->bond_type.size());
+  int nb = c->nbond_act;
   if (m->bond == mts_flag && nb > 0) {
     a.n = nb;
     int g = pgk_grid_blocks(nb, 256);
     seg(g, 1);
     k_bonds<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat, h);
   }
-  int na = (int)(c->angl_type.size());
+  int na = c->nangl_act;
   if (m->angl == mts_flag && na > 0) {
     a.n = na;
     int g = pgk_grid_blocks(na, 256);

@@ -298,52 +298,88 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   block_record<14, NW>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
-// ---- excluded pairs (FP64 in every form) -----------------------------------
-template <int PREC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_excl(int npair, const int32_t *__restrict__ pairs,
-                                              const double *__restrict__ pos,
-                                              const double *__restrict__ q,
-                                              const double *__restrict__ splx, double h,
-                                              double hinv, double xmax, int nspl, double3 L,
-                                              double3 Li,
-                                              typename Prec<PREC>::acc *fat,
-                                              typename Prec<PREC>::acc *fat_c,
-                                              double *red) {
+// ---- force combination + excluded pairs ------------------------------------
+// One thread per atom: the excluded-pair correction (ewaldr.F90:744-828:
+// E -= qq S'_e(r), f = -qq S'_f(r) r_ij with the erf tables, no cutoff) over
+// the atom's symmetric exclusion list, then the slot-order pair forces and
+// the atom-order accumulators (PME, bonded) are read, summed into the FP64
+// output and cleared for the next call.  Each pair is evaluated in the
+// reference's orientation (lower index first), so both atoms see the same
+// value; the lower atom records the pair's energy and virial.  FP64 in every
+// form; no atomics.
+struct CombineArgs {
+  int n, use_cl, do_excl;
+  const int32_t *slot_of;
+  const int32_t *xs_start, *xs_j;   // symmetric exclusion CSR, 0-based
+  const double *pos, *q, *splx;
+  double h, hinv, xmax;
+  int nspl;
+  double3 L, Li;
+  double *red;
+};
+
+template <class A, bool SPLIT>
+__global__ __launch_bounds__(256) void k_combine(CombineArgs a, A *__restrict__ fcl,
+                                                 A *__restrict__ fat, A *__restrict__ fclc,
+                                                 A *__restrict__ fatc, double *__restrict__ force,
+                                                 double *__restrict__ fvc,
+                                                 double *__restrict__ fvl) {
   double ev[7] = {0, 0, 0, 0, 0, 0, 0};
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < npair) {
-    int i = pairs[2 * k], j = pairs[2 * k + 1];
-    double rx = pos[3 * i] - pos[3 * j], ry = pos[3 * i + 1] - pos[3 * j + 1],
-           rz = pos[3 * i + 2] - pos[3 * j + 2];
-    rx = rx - L.x * fnint(rx * Li.x);
-    ry = ry - L.y * fnint(ry * Li.y);
-    rz = rz - L.z * fnint(rz * Li.z);
-    double r = sqrt(r2_nc(rx, ry, rz));
-    double sf, se;
-    spline2(splx, r, h, hinv, xmax, nspl, sf, se);
-    double qq = q[i] * q[j];
-    double ft = -sf * qq;
-    double pot = se * qq;
-    double fx = ft * rx, fy = ft * ry, fz = ft * rz;
-    acc_add(&fat[3 * i + 0], fx);
-    acc_add(&fat[3 * i + 1], fy);
-    acc_add(&fat[3 * i + 2], fz);
-    acc_add(&fat[3 * j + 0], -fx);
-    acc_add(&fat[3 * j + 1], -fy);
-    acc_add(&fat[3 * j + 2], -fz);
-    if (SPLIT) {
-      acc_add(&fat_c[3 * i + 0], fx);
-      acc_add(&fat_c[3 * i + 1], fy);
-      acc_add(&fat_c[3 * i + 2], fz);
-      acc_add(&fat_c[3 * j + 0], -fx);
-      acc_add(&fat_c[3 * j + 1], -fy);
-      acc_add(&fat_c[3 * j + 2], -fz);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) {
+    double fe[3] = {0.0, 0.0, 0.0};
+    if (a.do_excl) {
+      for (int k = a.xs_start[i]; k < a.xs_start[i + 1]; k++) {
+        const int j = a.xs_j[k];
+        const int lo = min(i, j), hi = max(i, j);
+        double rx = a.pos[3 * lo] - a.pos[3 * hi], ry = a.pos[3 * lo + 1] - a.pos[3 * hi + 1],
+               rz = a.pos[3 * lo + 2] - a.pos[3 * hi + 2];
+        rx = rx - a.L.x * fnint(rx * a.Li.x);
+        ry = ry - a.L.y * fnint(ry * a.Li.y);
+        rz = rz - a.L.z * fnint(rz * a.Li.z);
+        const double r = sqrt(r2_nc(rx, ry, rz));
+        double sf, se;
+        spline2(a.splx, r, a.h, a.hinv, a.xmax, a.nspl, sf, se);
+        const double qq = a.q[lo] * a.q[hi];
+        const double ft = -sf * qq;
+        const double fx = ft * rx, fy = ft * ry, fz = ft * rz;
+        if (i == lo) {
+          fe[0] += fx;
+          fe[1] += fy;
+          fe[2] += fz;
+          ev[0] -= se * qq;
+          ev[1] += fx * rx; ev[2] += fy * ry; ev[3] += fz * rz;
+          ev[4] += fx * ry; ev[5] += fx * rz; ev[6] += fy * rz;
+        } else {
+          fe[0] -= fx;
+          fe[1] -= fy;
+          fe[2] -= fz;
+        }
+      }
     }
-    ev[0] = -pot;
-    ev[1] = fx * rx; ev[2] = fy * ry; ev[3] = fz * rz;
-    ev[4] = fx * ry; ev[5] = fx * rz; ev[6] = fy * rz;
+    const int s = a.use_cl ? a.slot_of[i] : -1;
+    // every accumulator is read once here and left zeroed for the next call
+    for (int d = 0; d < 3; d++) {
+      double fp = 0.0;
+      if (s >= 0) {
+        fp = acc_load(&fcl[3 * s + d]);
+        fcl[3 * s + d] = A(0);
+      }
+      force[3 * i + d] = fp + acc_load(&fat[3 * i + d]) + fe[d];
+      fat[3 * i + d] = A(0);
+      if (SPLIT) {
+        double fc = 0.0;
+        if (s >= 0) {
+          fc = acc_load(&fclc[3 * s + d]);
+          fclc[3 * s + d] = A(0);
+        }
+        fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]) + fe[d];
+        fatc[3 * i + d] = A(0);
+        fvl[3 * i + d] = fp - fc;
+      }
+    }
   }
-  block_record<7, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
+  if (a.do_excl) block_record<7, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 template <int PREC>
@@ -417,35 +453,55 @@ int pgk_pairs_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
   }
 }
 
-template <int PREC>
-static int launch_excl(pgk_ctx *c, bool split, int *off, int *nrec) {
-  using A = typename Prec<PREC>::acc;
-  int n = c->nexclpair;
-  int nblk = pgk_grid_blocks(n > 0 ? n : 1, 256);
-  *off = c->red_used;
-  *nrec = nblk;
-  double *red = c->d_red + (size_t)c->red_used * PGK_REC;
-  c->red_used += nblk;
-  double3 L = make_double3(c->box[0], c->box[1], c->box[2]);
-  double3 Li = make_double3(1.0 / c->box[0], 1.0 / c->box[1], 1.0 / c->box[2]);
-  double xmax = c->spl_int * (double)(c->nspl - 1);
-  A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
+template <class A>
+static int launch_combine(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
+                          double *fvl, int *off, int *nrec) {
+  CombineArgs a;
+  a.n = c->natom;
+  a.use_cl = do_pair ? 1 : 0;
+  a.do_excl = (do_pair && c->nexclpair > 0) ? 1 : 0;
+  a.slot_of = c->d_slot_of;
+  a.xs_start = c->d_xs_start;
+  a.xs_j = c->d_xs_j;
+  a.pos = c->d_pos;
+  a.q = c->d_q;
+  a.splx = c->d_splx;
+  a.h = c->spl_int;
+  a.hinv = 1.0 / c->spl_int;
+  a.xmax = c->spl_int * (double)(c->nspl - 1);
+  a.nspl = c->nspl;
+  a.L = make_double3(c->box[0], c->box[1], c->box[2]);
+  a.Li = make_double3(1.0 / c->box[0], 1.0 / c->box[1], 1.0 / c->box[2]);
+  const int nblk = pgk_grid_blocks(c->natom, 256);
+  *nrec = 0;
+  a.red = nullptr;
+  if (a.do_excl) {
+    *off = c->red_used;
+    *nrec = nblk;
+    a.red = c->d_red + (size_t)c->red_used * PGK_REC;
+    c->red_used += nblk;
+  }
+  A *fcl = reinterpret_cast<A *>(c->d_fcl), *fat = reinterpret_cast<A *>(c->d_fat);
+  A *fclc = reinterpret_cast<A *>(c->d_fcl_c), *fatc = reinterpret_cast<A *>(c->d_fat_c);
   if (split)
-    k_excl<PREC, true><<<nblk, 256, 0, c->stream>>>(n, c->d_exclpair, c->d_pos, c->d_q,
-                                                    c->d_splx, c->spl_int, 1.0 / c->spl_int,
-                                                    xmax, c->nspl, L, Li, fat, fatc, red);
+    k_combine<A, true><<<nblk, 256, 0, c->stream>>>(a, fcl, fat, fclc, fatc, fout, fvc, fvl);
   else
-    k_excl<PREC, false><<<nblk, 256, 0, c->stream>>>(n, c->d_exclpair, c->d_pos, c->d_q,
-                                                     c->d_splx, c->spl_int, 1.0 / c->spl_int,
-                                                     xmax, c->nspl, L, Li, fat, fatc, red);
+    k_combine<A, false><<<nblk, 256, 0, c->stream>>>(a, fcl, fat, fclc, fatc, fout, fvc, fvl);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
 
-int pgk_excl_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+// Sum every force accumulator into fout (and the per-class forces when
+// split), adding the excluded-pair correction when the pair terms ran; the
+// correction's energy/virial records are appended when *nrec > 0.
+int pgk_combine_launch(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
+                       double *fvl, int *off, int *nrec) {
   switch (c->precision) {
-    case PGK_PREC_FP64: return launch_excl<PGK_PREC_FP64>(c, split, off, nrec);
-    case PGK_PREC_MIXED: return launch_excl<PGK_PREC_MIXED>(c, split, off, nrec);
-    default: return launch_excl<PGK_PREC_FP32>(c, split, off, nrec);
+    case PGK_PREC_FP64:
+      return launch_combine<double>(c, do_pair, split, fout, fvc, fvl, off, nrec);
+    case PGK_PREC_MIXED:
+      return launch_combine<unsigned long long>(c, do_pair, split, fout, fvc, fvl, off, nrec);
+    default:
+      return launch_combine<float>(c, do_pair, split, fout, fvc, fvl, off, nrec);
   }
 }

@@ -185,30 +185,7 @@ __device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *tw, i
 }
 
 
-// B-spline weights of order p at w (fill_bspline, fft_pme_init.F90:216-260)
-template <class R>
-__device__ __forceinline__ void bspline(R w, int order, R *A, R *D) {
-  // A, D are 1-based views of length order
-  A[order] = R(0);
-  A[2] = w;
-  A[1] = R(1) - w;
-  for (int k = 3; k <= order - 1; k++) {
-    R div = R(1) / R(k - 1);
-    A[k] = div * w * A[k - 1];
-    for (int j = 1; j <= k - 2; j++)
-      A[k - j] = div * ((w + R(j)) * A[k - j - 1] + (R(k - j) - w) * A[k - j]);
-    A[1] = div * (R(1) - w) * A[1];
-  }
-  D[1] = -A[1];
-  for (int j = 2; j <= order; j++) D[j] = A[j - 1] - A[j];
-  R div = R(1) / R(order - 1);
-  A[order] = div * w * A[order - 1];
-  for (int j = 1; j <= order - 2; j++)
-    A[order - j] = div * ((w + R(j)) * A[order - j - 1] + (R(order - j) - w) * A[order - j]);
-  A[1] = div * (R(1) - w) * A[1];
-}
-
-// B-spline weights of compile-time order P (same recursion as bspline())
+// B-spline weights of order P at w (fill_bspline, fft_pme_init.F90:216-260)
 template <class R, int P>
 __device__ __forceinline__ void bspline_t(R w, R *A, R *D) {
   // A[1..P], D[1..P]
@@ -235,6 +212,11 @@ __device__ __forceinline__ void bspline_t(R w, R *A, R *D) {
   A[1] = div * (R(1) - w) * A[1];
 }
 
+__device__ __forceinline__ double msq_nc(double a, double b, double c) {
+#pragma clang fp contract(off)
+  return a * a + b * b + c * c;
+}
+
 struct PmeGeom {
   int K[3];
   int H;            // K1/2 + 1
@@ -242,57 +224,21 @@ struct PmeGeom {
   int order;
 };
 
-template <class R>
-__device__ __forceinline__ void atom_frac(const double *pos, int i, const PmeGeom &g, int *i0,
-                                          R (*th)[PGK_MAXORD + 1], R (*dth)[PGK_MAXORD + 1]) {
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    double w = pos[3 * i + d] * g.recip[d];
-    w = w - floor(w);
-    double fr = (double)g.K[d] * w;
-    // w - floor(w) may round to 1.0 and non-finite input must not index the
-    // grid: keep 0 <= int(fr) < K
-    if (!(fr >= 0.0 && fr < (double)g.K[d])) fr = (fr >= (double)g.K[d]) ? (double)g.K[d] - 1.0 : 0.0;
-    int ifr = (int)fr;
-    R ww = (R)(fr - (double)ifr);
-    bspline<R>(ww, g.order, th[d], dth[d]);
-    i0[d] = ifr - g.order;   // first index (before +1) as fill_charge_grid
-  }
+#define PGK_MAXSRC 125   // source bins per tile (5 per axis)
+
+struct TileGeom {
+  int T[3], nt[3], nsrc[3];   // tile size, tiles per axis, source bins per axis
+  int ntile;
+};
+
+__device__ __forceinline__ int tile_home(const int4 &b, const PmeGeom &g, const TileGeom &tg) {
+  const int bx = b.x < 0 ? b.x + g.K[0] : b.x;
+  const int by = b.y < 0 ? b.y + g.K[1] : b.y;
+  const int bz = b.z < 0 ? b.z + g.K[2] : b.z;
+  return ((bz / tg.T[2]) * tg.nt[1] + by / tg.T[1]) * tg.nt[0] + bx / tg.T[0];
 }
 
-// B-spline weights of every atom, in the spatially sorted cluster order when
-// a list exists (order[k] = atom or -1).  th: [k][6][p] = th1 th2 th3 dth1
-// dth2 dth3; base: [k] first grid index per dimension (int(fr) - p + 1 as in
-// fill_charge_grid, possibly negative before wrapping).
-template <int PREC>
-__global__ __launch_bounds__(256) void k_bspline(int nitem, const int32_t *__restrict__ order,
-                                                 const double *__restrict__ pos, PmeGeom g,
-                                                 typename Prec<PREC>::real *__restrict__ th,
-                                                 int4 *__restrict__ base) {
-  using R = typename Prec<PREC>::real;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nitem) return;
-  int a = order ? order[k] : k;
-  int4 b;
-  b.w = a;
-  if (a < 0) {
-    b.x = b.y = b.z = 0;
-    base[k] = b;
-    return;
-  }
-  R t[3][PGK_MAXORD + 1], dt[3][PGK_MAXORD + 1];
-  int i0[3];
-  atom_frac<R>(pos, a, g, i0, t, dt);
-  const int p = g.order;
-  for (int d = 0; d < 3; d++)
-    for (int q = 0; q < p; q++) th[(size_t)(d * p + q) * nitem + k] = t[d][q + 1];
-  b.x = i0[0] + 1;
-  b.y = i0[1] + 1;
-  b.z = i0[2] + 1;
-  base[k] = b;
-}
 
-// one wavefront per atom, one lane per stencil point (x fastest)
 template <class R, int P>
 __device__ __forceinline__ void atom_frac_t(const double *pos, int i, const PmeGeom &g, int *i0,
                                             R (*th)[P + 1], R (*dth)[P + 1]) {
@@ -309,33 +255,90 @@ __device__ __forceinline__ void atom_frac_t(const double *pos, int i, const PmeG
   }
 }
 
-template <int PREC, int P>
-__global__ __launch_bounds__(256) void k_bspline_t(int nitem, const int32_t *__restrict__ order,
-                                                   const double *__restrict__ pos, PmeGeom g,
-                                                   typename Prec<PREC>::real *__restrict__ th,
-                                                   int4 *__restrict__ base) {
+// Per-slot preparation ahead of the force kernels, in the list's spatial order
+// (order[k] = atom, or -1 for padding; identity without a list):
+//   - B-spline weights th[(d*P + q)*nitem + k] and the first stencil index
+//     base[k] = int(fr) - P + 1 per axis (get_bspline_coeffs and
+//     fill_charge_grid, fft_pme.F90:316-422), base.w = atom;
+//   - the slot's rank in its home tile bin: the block counts in LDS (open
+//     addressing on the tile id) and takes one global atomic per tile;
+//   - with PACK, the pair kernel's (x, y, z, q) and 0-based LJ type per slot.
+template <int PREC, int P, bool PACK>
+__global__ __launch_bounds__(256) void k_prep(int nitem, const int32_t *__restrict__ order,
+                                              const double *__restrict__ pos,
+                                              const double *__restrict__ q,
+                                              const int32_t *__restrict__ type, PmeGeom g,
+                                              TileGeom tg,
+                                              typename Prec<PREC>::real *__restrict__ th,
+                                              int4 *__restrict__ base,
+                                              int32_t *__restrict__ count,
+                                              int32_t *__restrict__ rank,
+                                              typename Prec<PREC>::real4 *__restrict__ xq,
+                                              int32_t *__restrict__ ct) {
   using R = typename Prec<PREC>::real;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nitem) return;
-  int a = order ? order[k] : k;
-  int4 b;
-  b.w = a;
-  if (a < 0) {
-    b.x = b.y = b.z = 0;
-    base[k] = b;
-    return;
+  using R4 = typename Prec<PREC>::real4;
+  __shared__ int s_key[512], s_cnt[512], s_base[512];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    s_key[i] = -1;
+    s_cnt[i] = 0;
   }
-  R t[3][P + 1], dt[3][P + 1];
-  int i0[3];
-  atom_frac_t<R, P>(pos, a, g, i0, t, dt);
+  __syncthreads();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = -1;
+  if (k < nitem) {
+    const int a = order ? order[k] : k;
+    int4 b;
+    b.w = a;
+    if (a < 0) {
+      b.x = b.y = b.z = 0;
+      if (PACK) {
+        R4 z;
+        z.x = z.y = z.z = z.w = R(0);
+        xq[k] = z;
+        ct[k] = 0;
+      }
+    } else {
+      R tt[3][P + 1], dt[3][P + 1];
+      int i0[3];
+      atom_frac_t<R, P>(pos, a, g, i0, tt, dt);
 #pragma unroll
-  for (int d = 0; d < 3; d++)
+      for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int q = 0; q < P; q++) th[(size_t)(d * P + q) * nitem + k] = t[d][q + 1];
-  b.x = i0[0] + 1;
-  b.y = i0[1] + 1;
-  b.z = i0[2] + 1;
-  base[k] = b;
+        for (int qq = 0; qq < P; qq++) th[(size_t)(d * P + qq) * nitem + k] = tt[d][qq + 1];
+      b.x = i0[0] + 1;
+      b.y = i0[1] + 1;
+      b.z = i0[2] + 1;
+      t = tile_home(b, g, tg);
+      if (PACK) {
+        R4 v;
+        v.x = (R)pos[3 * a];
+        v.y = (R)pos[3 * a + 1];
+        v.z = (R)pos[3 * a + 2];
+        v.w = (R)q[a];
+        xq[k] = v;
+        ct[k] = type[a] - 1;   // 0-based LJ type index
+      }
+    }
+    base[k] = b;
+  }
+  int slot = -1, r = -1;
+  if (t >= 0) {
+    int h = (int)(((unsigned)t * 2654435761u) >> 23);   // 9-bit hash
+    for (int probe = 0; probe < 512; probe++) {
+      const int old = atomicCAS(&s_key[h], -1, t);
+      if (old == -1 || old == t) {
+        slot = h;
+        break;
+      }
+      h = (h + 1) & 511;
+    }
+    r = atomicAdd(&s_cnt[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x)
+    if (s_key[i] >= 0) s_base[i] = atomicAdd(&count[s_key[i]], s_cnt[i]);
+  __syncthreads();
+  if (k < nitem) rank[k] = (t >= 0) ? s_base[slot] + r : -1;
 }
 
 template <int PREC, bool SPLIT, int P>
@@ -403,69 +406,41 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
 // One workgroup per tile accumulates the points it owns in LDS (int64 fixed
 // point in the mixed form: order-independent) and writes each grid point
 // once, in the compute type.  No device atomics, no clearing pass.
-#define PGK_MAXSRC 125   // source bins per tile (5 per axis)
-
-struct TileGeom {
-  int T[3], nt[3], nsrc[3];   // tile size, tiles per axis, source bins per axis
-  int ntile;
-};
-
-__device__ __forceinline__ int tile_home(const int4 &b, const PmeGeom &g, const TileGeom &tg) {
-  const int bx = b.x < 0 ? b.x + g.K[0] : b.x;
-  const int by = b.y < 0 ? b.y + g.K[1] : b.y;
-  const int bz = b.z < 0 ? b.z + g.K[2] : b.z;
-  return ((bz / tg.T[2]) * tg.nt[1] + by / tg.T[1]) * tg.nt[0] + bx / tg.T[0];
+// exclusive scan of count[0..n) into start[0..n] (start[n] = total), one
+// block: each thread sums a contiguous chunk, the block scans the chunk sums
+__global__ __launch_bounds__(1024) void k_scan_start(int n, const int32_t *__restrict__ count,
+                                                     int32_t *__restrict__ start) {
+  __shared__ int s[1024];
+  const int per = (n + 1023) / 1024;
+  const int lo = min((int)threadIdx.x * per, n), hi = min(lo + per, n);
+  int sum = 0;
+  for (int i = lo; i < hi; i++) sum += count[i];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - sum;
+  for (int i = lo; i < hi; i++) {
+    start[i] = run;
+    run += count[i];
+  }
+  if (threadIdx.x == 1023) start[n] = s[1023];
 }
 
-__global__ __launch_bounds__(256) void k_tile_zero(int n, int32_t *__restrict__ count) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) count[k] = 0;
-}
-
-// rank of every item inside its home bin.  Items of one block are spatially
-// close, so the block first counts in LDS (open addressing on the tile id)
-// and takes one global atomic per distinct tile.
-__global__ __launch_bounds__(256) void k_tile_count(int nitem, const int4 *__restrict__ base,
-                                                    PmeGeom g, TileGeom tg,
-                                                    int32_t *__restrict__ count,
-                                                    int32_t *__restrict__ rank) {
-  __shared__ int s_key[512], s_cnt[512], s_base[512];
-  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
-    s_key[i] = -1;
-    s_cnt[i] = 0;
-  }
-  __syncthreads();
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = -1, slot = -1, r = -1;
-  if (k < nitem) {
-    const int4 b = base[k];
-    if (b.w >= 0) t = tile_home(b, g, tg);
-  }
-  if (t >= 0) {
-    int h = (int)(((unsigned)t * 2654435761u) >> 23);   // 9-bit hash
-    for (int probe = 0; probe < 512; probe++) {
-      const int old = atomicCAS(&s_key[h], -1, t);
-      if (old == -1 || old == t) {
-        slot = h;
-        break;
-      }
-      h = (h + 1) & 511;
-    }
-    r = atomicAdd(&s_cnt[slot], 1);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 512; i += blockDim.x)
-    if (s_key[i] >= 0) s_base[i] = atomicAdd(&count[s_key[i]], s_cnt[i]);
-  __syncthreads();
-  if (k < nitem) rank[k] = (t >= 0) ? s_base[slot] + r : -1;
-}
-
+// items of every bin in rank order; the counts (already scanned) are left
+// zeroed for the next call's k_prep
 __global__ __launch_bounds__(256) void k_tile_fill(int nitem, const int4 *__restrict__ base,
                                                    PmeGeom g, TileGeom tg,
+                                                   int32_t *__restrict__ count,
                                                    const int32_t *__restrict__ start,
                                                    const int32_t *__restrict__ rank,
                                                    int32_t *__restrict__ items) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int z = k; z < tg.ntile; z += gridDim.x * blockDim.x) count[z] = 0;
   if (k >= nitem) return;
   const int r = rank[k];
   if (r < 0) return;
@@ -570,54 +545,6 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
   }
 }
 
-// thread per atom (sorted order), weights from k_bspline (grad_sum)
-template <int PREC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_gather_s(int nitem, const int32_t *__restrict__ order,
-                                                  const double *__restrict__ pos,
-                                                  const double *__restrict__ q, PmeGeom g,
-                                                  const void *gridv,
-                                                  typename Prec<PREC>::acc *fat,
-                                                  typename Prec<PREC>::acc *fat_c) {
-  using R = typename Prec<PREC>::real;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nitem) return;
-  const int i = order ? order[k] : k;
-  if (i < 0) return;
-  const R *grid = reinterpret_cast<const R *>(gridv);
-  R th[3][PGK_MAXORD + 1], dth[3][PGK_MAXORD + 1];
-  int i0[3];
-  atom_frac<R>(pos, i, g, i0, th, dth);
-  const int p = g.order;
-  R f1 = 0, f2 = 0, f3 = 0;
-  const R qi = (R)q[i];
-  const R K1 = (R)g.K[0], K2 = (R)g.K[1], K3 = (R)g.K[2];
-  for (int a3 = 1; a3 <= p; a3++) {
-    int kk = i0[2] + a3;
-    if (kk < 0) kk += g.K[2];
-    for (int a2 = 1; a2 <= p; a2++) {
-      int jj = i0[1] + a2;
-      if (jj < 0) jj += g.K[1];
-      const size_t row = ((size_t)kk * g.K[1] + jj) * g.K[0];
-      for (int a1 = 1; a1 <= p; a1++) {
-        int ii = i0[0] + a1;
-        if (ii < 0) ii += g.K[0];
-        const R term = qi * grid[row + ii];
-        f1 = f1 - K1 * term * dth[0][a1] * th[1][a2] * th[2][a3];
-        f2 = f2 - K2 * term * th[0][a1] * dth[1][a2] * th[2][a3];
-        f3 = f3 - K3 * term * th[0][a1] * th[1][a2] * dth[2][a3];
-      }
-    }
-  }
-  double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
-  acc_add(&fat[3 * i + 0], fx);
-  acc_add(&fat[3 * i + 1], fy);
-  acc_add(&fat[3 * i + 2], fz);
-  if (SPLIT) {
-    acc_add(&fat_c[3 * i + 0], fx);
-    acc_add(&fat_c[3 * i + 1], fy);
-    acc_add(&fat_c[3 * i + 2], fz);
-  }
-}
 
 // x pass, forward: pairs of real rows -> half spectra.  One block handles NL
 // complex lines (2*NL rows).
@@ -795,7 +722,9 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
     int m1 = (h < nf1) ? h : h - K1;
     int m3 = (k3 < nf3) ? k3 : k3 - K3;
     double mh1 = ca.recip[0] * m1, mh3 = ca.recip[2] * m3;
-    double msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
+    // the rkcut test decides modes that lie on the sphere (e.g. integer
+    // |k|^2 = (rkcut L)^2): |m|^2 in the reference's rounding, no contraction
+    double msq = msq_nc(mh1, mh2, mh3);
     C z = pa[t];
     double eterm = 0.0;
     if (msq <= ca.rk2) {
@@ -856,6 +785,103 @@ int lines_for(int n, size_t csize, int maxl) {
   return nl;
 }
 
+// Device layout of the per-slot PME data: slots are the list's spatial order
+// when a list exists (ncl*8 with padding), else atom order.
+struct PmeLayout {
+  PmeGeom g;
+  TileGeom tg;
+  int nitem;
+  const int32_t *order;
+  void *th;
+  int4 *base;
+  int32_t *count, *start, *rank, *items;
+};
+
+static int pme_layout(pgk_ctx *c, size_t rsize, PmeLayout *L) {
+  for (int d = 0; d < 3; d++) {
+    L->g.K[d] = c->nfft[d];
+    L->g.recip[d] = 1.0 / c->box[d];
+  }
+  L->g.H = c->nfft[0] / 2 + 1;
+  L->g.order = c->order;
+  TileGeom &tg = L->tg;
+  tg.ntile = 1;
+  for (int d = 0; d < 3; d++) {
+    int T = 8;
+    while (c->nfft[d] % T) T--;
+    tg.T[d] = T;
+    tg.nt[d] = c->nfft[d] / T;
+    const int nb = (T + c->order - 2) / T;
+    tg.nsrc[d] = std::min(nb + 1, tg.nt[d]);
+    tg.ntile *= tg.nt[d];
+  }
+  if (tg.nsrc[0] * tg.nsrc[1] * tg.nsrc[2] > PGK_MAXSRC) {
+    pgk_set_error("SPME grid tiles too small for this pme_order");
+    return PGK_EINVAL;
+  }
+  const bool sorted = c->have_list && c->d_cl_atom;
+  L->nitem = sorted ? c->ncl * PGK_CL : c->natom;
+  L->order = sorted ? c->d_cl_atom : nullptr;
+  const size_t nitem = (size_t)L->nitem;
+  size_t need = nitem * 3 * c->order * rsize + nitem * sizeof(int4);
+  if (need > c->theta_bytes) {
+    if (c->d_theta) (void)hipFree(c->d_theta);
+    c->d_theta = nullptr;
+    c->theta_bytes = 0;
+    HIP_CHECK(hipMalloc(&c->d_theta, need));
+    c->theta_bytes = need;
+  }
+  L->th = c->d_theta;
+  L->base = reinterpret_cast<int4 *>(reinterpret_cast<char *>(c->d_theta) +
+                                     nitem * 3 * c->order * rsize);
+  // bins: count[ntile] | start[ntile + 1] | rank[nitem] | items[nitem]
+  size_t tneed = sizeof(int32_t) * (2 * (size_t)tg.ntile + 1 + 2 * nitem);
+  if (tneed > c->tile_bytes || c->tile_ntile != tg.ntile) {
+    if (c->d_tile) (void)hipFree(c->d_tile);
+    c->d_tile = nullptr;
+    c->tile_bytes = 0;
+    HIP_CHECK(hipMalloc(&c->d_tile, tneed));
+    c->tile_bytes = tneed;
+    c->tile_ntile = tg.ntile;
+    // counts start at zero; afterwards k_tile_fill leaves them zeroed
+    HIP_CHECK(hipMemsetAsync(c->d_tile, 0, sizeof(int32_t) * tg.ntile, c->stream));
+  }
+  L->count = reinterpret_cast<int32_t *>(c->d_tile);
+  L->start = L->count + tg.ntile;
+  L->rank = L->start + tg.ntile + 1;
+  L->items = L->rank + nitem;
+  return PGK_OK;
+}
+
+template <int PREC, bool PACK>
+int launch_prep(pgk_ctx *c) {
+  using R = typename Prec<PREC>::real;
+  using R4 = typename Prec<PREC>::real4;
+  PmeLayout L;
+  int rc = pme_layout(c, sizeof(R), &L);
+  if (rc) return rc;
+  const int gb = pgk_grid_blocks(L.nitem, 256);
+  R *th = reinterpret_cast<R *>(L.th);
+  R4 *xq = reinterpret_cast<R4 *>(c->d_xq);
+#define PGK_PREP(P)                                                                          \
+  k_prep<PREC, P, PACK><<<gb, 256, 0, c->stream>>>(L.nitem, L.order, c->d_pos, c->d_q,      \
+                                                   c->d_type, L.g, L.tg, th, L.base, L.count, \
+                                                   L.rank, xq, c->d_ct)
+  switch (c->order) {
+    case 3: PGK_PREP(3); break;
+    case 4: PGK_PREP(4); break;
+    case 5: PGK_PREP(5); break;
+    case 6: PGK_PREP(6); break;
+    case 7: PGK_PREP(7); break;
+    case 8: PGK_PREP(8); break;
+    case 9: PGK_PREP(9); break;
+    default: PGK_PREP(10); break;
+  }
+#undef PGK_PREP
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
 template <int PREC>
 int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   using R = typename Prec<PREC>::real;
@@ -864,89 +890,22 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   hipStream_t st = c->stream;
   const int K1 = c->nfft[0], K2 = c->nfft[1], K3 = c->nfft[2];
   const int H = K1 / 2 + 1;
-  PmeGeom g;
-  for (int d = 0; d < 3; d++) {
-    g.K[d] = c->nfft[d];
-    g.recip[d] = 1.0 / c->box[d];
-  }
-  g.H = H;
-  g.order = c->order;
-  size_t G = (size_t)K1 * K2 * K3;
-  size_t gel = (PREC == PGK_PREC_FP32) ? 4 : 8;
+  PmeLayout L;
+  int rc = pme_layout(c, sizeof(R), &L);
+  if (rc) return rc;
+  const PmeGeom g = L.g;
+  const TileGeom tg = L.tg;
+  const int nitem = L.nitem;
+  R *th = reinterpret_cast<R *>(L.th);
+  int4 *base = L.base;
   pgk_prof_begin(c, PGK_PROF_SPREAD);
-  (void)G;
-  (void)gel;
-  const int n = c->natom;
-  // B-spline weights in the list's spatial order (fallback: atom order)
-  const bool sorted = c->have_list && c->d_cl_atom;
-  const int nitem = sorted ? c->ncl * PGK_CL : n;
   {
-    size_t need = (size_t)nitem * 3 * c->order * sizeof(R) + (size_t)nitem * sizeof(int4);
-    if (need > c->theta_bytes) {
-      if (c->d_theta) (void)hipFree(c->d_theta);
-      c->d_theta = nullptr;
-      HIP_CHECK(hipMalloc(&c->d_theta, need));
-      c->theta_bytes = need;
-    }
-  }
-  R *th = reinterpret_cast<R *>(c->d_theta);
-  int4 *base = reinterpret_cast<int4 *>(reinterpret_cast<char *>(c->d_theta) +
-                                        (size_t)nitem * 3 * c->order * sizeof(R));
-  {
-    const int32_t *ord = sorted ? c->d_cl_atom : nullptr;
-    const int gb = pgk_grid_blocks(nitem, 256);
-    switch (c->order) {
-      case 4: k_bspline_t<PREC, 4><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
-      case 5: k_bspline_t<PREC, 5><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
-      case 6: k_bspline_t<PREC, 6><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
-      case 8: k_bspline_t<PREC, 8><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base); break;
-      default: k_bspline<PREC><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, g, th, base);
-    }
-  }
-  {
-    TileGeom tg;
-    tg.ntile = 1;
-    for (int d = 0; d < 3; d++) {
-      int T = 8;
-      while (c->nfft[d] % T) T--;
-      tg.T[d] = T;
-      tg.nt[d] = c->nfft[d] / T;
-      const int nb = (T + c->order - 2) / T;
-      tg.nsrc[d] = std::min(nb + 1, tg.nt[d]);
-      tg.ntile *= tg.nt[d];
-    }
-    if (tg.nsrc[0] * tg.nsrc[1] * tg.nsrc[2] > PGK_MAXSRC) {
-      pgk_set_error("SPME grid tiles too small for this pme_order");
-      return PGK_EINVAL;
-    }
-    // bins: count[ntile + 1] | start[ntile + 1] | rank[nitem] | items[nitem]
-    size_t need = sizeof(int32_t) * (2 * (size_t)(tg.ntile + 1) + 2 * (size_t)nitem);
-    size_t scan_bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t *)nullptr,
-                                           (int32_t *)nullptr, tg.ntile + 1, st);
-    need += scan_bytes + 256;
-    if (need > c->tile_bytes) {
-      if (c->d_tile) (void)hipFree(c->d_tile);
-      c->d_tile = nullptr;
-      c->tile_bytes = 0;
-      HIP_CHECK(hipMalloc(&c->d_tile, need));
-      c->tile_bytes = need;
-    }
-    int32_t *count = reinterpret_cast<int32_t *>(c->d_tile);
-    int32_t *start = count + tg.ntile + 1;
-    int32_t *rank = start + tg.ntile + 1;
-    int32_t *items = rank + nitem;
-    void *scan_tmp = reinterpret_cast<char *>(c->d_tile) +
-                     ((sizeof(int32_t) * (2 * (size_t)(tg.ntile + 1) + 2 * (size_t)nitem) + 255) &
-                      ~(size_t)255);
-    k_tile_zero<<<pgk_grid_blocks(tg.ntile + 1, 256), 256, 0, st>>>(tg.ntile + 1, count);
-    k_tile_count<<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, base, g, tg, count, rank);
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, count, start,
-                                               tg.ntile + 1, st));
-    k_tile_fill<<<pgk_grid_blocks(nitem, 256), 256, 0, st>>>(nitem, base, g, tg, start, rank,
-                                                             items);
+    k_scan_start<<<1, 1024, 0, st>>>(tg.ntile, L.count, L.start);
+    k_tile_fill<<<pgk_grid_blocks(std::max(nitem, tg.ntile), 256), 256, 0, st>>>(
+        nitem, base, g, tg, L.count, L.start, L.rank, L.items);
     const size_t lds = (size_t)tg.T[0] * tg.T[1] * tg.T[2] * sizeof(typename SpreadAcc<PREC>::type);
     R *gr = reinterpret_cast<R *>(c->d_grid);
+    const int32_t *start = L.start, *items = L.items;
 #define PGK_SPREAD(P)                                                                         \
   k_spread_tile<PREC, P><<<xcd_grid(tg.ntile), 256, lds, st>>>(nitem, c->d_q, g, tg, start,   \
                                                                 items, th, base, gr)
@@ -1012,31 +971,33 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   pgk_prof_begin(c, PGK_PROF_GATHER);
   A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
   {
-    const int32_t *ord = sorted ? c->d_cl_atom : nullptr;
     const int gb = pgk_grid_blocks(nitem, 256);
     const int gbx = xcd_grid(gb);
 #define PGK_GATHER(SP, P)                                                                 \
   k_gather_t<PREC, SP, P><<<gbx, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid, \
                                               fat, fatc)
+    const int32_t *ord = L.order;
     if (split) {
       switch (c->order) {
+        case 3: PGK_GATHER(true, 3); break;
         case 4: PGK_GATHER(true, 4); break;
         case 5: PGK_GATHER(true, 5); break;
         case 6: PGK_GATHER(true, 6); break;
+        case 7: PGK_GATHER(true, 7); break;
         case 8: PGK_GATHER(true, 8); break;
-        default:
-          k_gather_s<PREC, true><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid,
-                                                     fat, fatc);
+        case 9: PGK_GATHER(true, 9); break;
+        default: PGK_GATHER(true, 10); break;
       }
     } else {
       switch (c->order) {
+        case 3: PGK_GATHER(false, 3); break;
         case 4: PGK_GATHER(false, 4); break;
         case 5: PGK_GATHER(false, 5); break;
         case 6: PGK_GATHER(false, 6); break;
+        case 7: PGK_GATHER(false, 7); break;
         case 8: PGK_GATHER(false, 8); break;
-        default:
-          k_gather_s<PREC, false><<<gb, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid,
-                                                      fat, fatc);
+        case 9: PGK_GATHER(false, 9); break;
+        default: PGK_GATHER(false, 10); break;
       }
     }
 #undef PGK_GATHER
@@ -1047,6 +1008,16 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
 }
 
 }  // namespace
+
+// Queue the per-slot preparation (B-spline weights, tile ranks and, with
+// pack, the pair kernel's packed positions) ahead of the force kernels.
+int pgk_pme_prep(pgk_ctx *c, bool pack) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return pack ? launch_prep<PGK_PREC_FP64, true>(c) : launch_prep<PGK_PREC_FP64, false>(c);
+    case PGK_PREC_MIXED: return pack ? launch_prep<PGK_PREC_MIXED, true>(c) : launch_prep<PGK_PREC_MIXED, false>(c);
+    default: return pack ? launch_prep<PGK_PREC_FP32, true>(c) : launch_prep<PGK_PREC_FP32, false>(c);
+  }
+}
 
 int pgk_pme_prepare(pgk_ctx *c) {
   for (int d = 0; d < 3; d++) {

@@ -192,36 +192,6 @@ __global__ void k_pack(int nslot, const int32_t *__restrict__ cl_atom,
   xq[s] = v;
 }
 
-template <class A>
-__global__ void k_combine(int n, const int32_t *__restrict__ slot_of, int use_cl,
-                          A *__restrict__ fcl, A *__restrict__ fat, A *__restrict__ fclc,
-                          A *__restrict__ fatc, double *__restrict__ force,
-                          double *__restrict__ fvc, double *__restrict__ fvl) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int s = use_cl ? slot_of[i] : -1;
-  // every accumulator is read once here and left zeroed for the next call
-  for (int d = 0; d < 3; d++) {
-    double fp = 0.0;
-    if (s >= 0) {
-      fp = acc_load(&fcl[3 * s + d]);
-      fcl[3 * s + d] = A(0);
-    }
-    force[3 * i + d] = fp + acc_load(&fat[3 * i + d]);
-    fat[3 * i + d] = A(0);
-    if (fvc) {
-      double fc = 0.0;
-      if (s >= 0) {
-        fc = acc_load(&fclc[3 * s + d]);
-        fclc[3 * s + d] = A(0);
-      }
-      fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]);
-      fatc[3 * i + d] = A(0);
-      fvl[3 * i + d] = fp - fc;
-    }
-  }
-}
-
 // deterministic segment sums: one block per (segment, field), fixed-order tree
 __global__ __launch_bounds__(256) void k_reduce_segments(const double *__restrict__ red,
                                                          const int *__restrict__ seg_off,
@@ -285,13 +255,13 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   const size_t ab = (c->precision == PGK_PREC_FP32) ? 4 : 8;
   // capacity of the record buffer: every launch appends its blocks
   int need = 16;
-  if (do_pair) need += pgk_grid_blocks(c->ncl, 4) + pgk_grid_blocks(c->nexclpair, 256) + 1;
+  if (do_pair) need += pgk_grid_blocks(c->ncl, 4) + pgk_grid_blocks(c->natom, 256) + 1;
   {
     int H = c->nfft[0] / 2 + 1;
     need += c->nfft[1] * H + 16;
   }
-  need += pgk_grid_blocks((int64_t)c->bond_type.size(), 256) +
-          pgk_grid_blocks((int64_t)c->angl_type.size(), 256) +
+  need += pgk_grid_blocks((int64_t)c->nbond_act, 256) +
+          pgk_grid_blocks((int64_t)c->nangl_act, 256) +
           pgk_grid_blocks((int64_t)c->tors_type.size(), 256) +
           pgk_grid_blocks((int64_t)c->rb_type.size(), 256) + pgk_grid_blocks(c->n14, 256) + 8;
   if (need > c->red_cap) {
@@ -311,29 +281,33 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipMemsetAsync(c->d_fcl, 0, nsc * 3 * ab, st));
       HIP_CHECK(hipMemsetAsync(c->d_fcl_c, 0, nsc * 3 * ab, st));
     }
+    if (c->d_tile && c->tile_ntile > 0)   // spreading bin counts
+      HIP_CHECK(hipMemsetAsync(c->d_tile, 0, sizeof(int32_t) * c->tile_ntile, st));
   }
   c->acc_dirty = true;
+  const bool do_ewk = (m->ewk == mts_flag);
+  if (do_ewk) {   // B-spline weights and tile ranks; packs the pair data too
+    pgk_prof_begin(c, PGK_PROF_OTHER);
+    if ((rc = pgk_pme_prep(c, do_pair))) return rc;
+    pgk_prof_end(c, PGK_PROF_OTHER);
+  }
   if (do_pair) {
     size_t ns = (size_t)c->ncl * PGK_CL;
-    if (c->precision == PGK_PREC_FP64)
-      k_pack<double4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
-          (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (double4 *)c->d_xq, c->d_ct);
-    else
-      k_pack<float4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
-          (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (float4 *)c->d_xq, c->d_ct);
-    HIP_CHECK(hipGetLastError());
+    if (!do_ewk) {
+      if (c->precision == PGK_PREC_FP64)
+        k_pack<double4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
+            (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (double4 *)c->d_xq, c->d_ct);
+      else
+        k_pack<float4><<<pgk_grid_blocks(ns, 256), 256, 0, st>>>(
+            (int)ns, c->d_cl_atom, c->d_pos, c->d_q, c->d_type, (float4 *)c->d_xq, c->d_ct);
+      HIP_CHECK(hipGetLastError());
+    }
     pgk_prof_begin(c, PGK_PROF_PAIRS);
     if ((rc = pgk_pairs_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
     pgk_prof_end(c, PGK_PROF_PAIRS);
     seg_kind[nseg++] = SEG_PAIR;
-    if (c->nexclpair > 0) {
-      pgk_prof_begin(c, PGK_PROF_EXCL);
-      if ((rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
-      pgk_prof_end(c, PGK_PROF_EXCL);
-      seg_kind[nseg++] = SEG_EXCL;
-    }
   }
-  if (m->ewk == mts_flag) {
+  if (do_ewk) {
     if ((rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
     seg_kind[nseg++] = SEG_PME;
   }
@@ -341,31 +315,20 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
   if (hf && do_pair && (rc = pgk_hf_launch_pairs(c, true))) return rc;
-  // forces
+  // forces: accumulators + excluded-pair correction -> FP64 output
   double *fout = d_force_out ? d_force_out : c->d_force;
   {
-    int g = pgk_grid_blocks(n, 256);
-    double *fvc = split ? d_fvc : nullptr, *fvl = split ? d_fvl : nullptr;
-    switch (c->precision) {
-      case PGK_PREC_FP64:
-        k_combine<double><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
-                                             (double *)c->d_fcl, (double *)c->d_fat,
-                                             (double *)c->d_fcl_c, (double *)c->d_fat_c, fout,
-                                             fvc, fvl);
-        break;
-      case PGK_PREC_MIXED:
-        k_combine<unsigned long long><<<g, 256, 0, st>>>(
-            n, c->d_slot_of, do_pair ? 1 : 0, (unsigned long long *)c->d_fcl,
-            (unsigned long long *)c->d_fat, (unsigned long long *)c->d_fcl_c,
-            (unsigned long long *)c->d_fat_c, fout, fvc, fvl);
-        break;
-      default:
-        k_combine<float><<<g, 256, 0, st>>>(n, c->d_slot_of, do_pair ? 1 : 0,
-                                            (float *)c->d_fcl, (float *)c->d_fat,
-                                            (float *)c->d_fcl_c, (float *)c->d_fat_c, fout, fvc,
-                                            fvl);
+    pgk_prof_begin(c, PGK_PROF_EXCL);
+    int eo = 0, en = 0;
+    if ((rc = pgk_combine_launch(c, do_pair, split, fout, split ? d_fvc : nullptr,
+                                 split ? d_fvl : nullptr, &eo, &en)))
+      return rc;
+    pgk_prof_end(c, PGK_PROF_EXCL);
+    if (en > 0) {
+      seg_off[nseg] = eo;
+      seg_n[nseg] = en;
+      seg_kind[nseg++] = SEG_EXCL;
     }
-    HIP_CHECK(hipGetLastError());
   }
   // energies: fixed-order segment sums into d_sum, copied to pinned h_sum
   if (nseg > 0) {
@@ -476,8 +439,8 @@ int finalize_energies(pgk_ctx *c, double *ene, double *virt) {
   }
   (void)tors_done;
   // terms whose MTS level matches but that have no entries still report zero
-  if (m->bond == mts_flag && c->bond_type.empty()) E[PGK_E_BOND] = 0.0;
-  if (m->angl == mts_flag && c->angl_type.empty()) E[PGK_E_ANGL] = 0.0;
+  if (m->bond == mts_flag && c->nbond_act == 0) E[PGK_E_BOND] = 0.0;
+  if (m->angl == mts_flag && c->nangl_act == 0) E[PGK_E_ANGL] = 0.0;
   if (m->tors == mts_flag && c->tors_type.empty()) E[PGK_E_TORS] = 0.0;
   if (m->torsrb == mts_flag && c->rb_type.empty()) E[PGK_E_TORSRB] = 0.0;
   if (m->nonb14 == mts_flag && c->n14 == 0) E[PGK_E_ELC14] = E[PGK_E_VDW14] = 0.0;
@@ -688,6 +651,17 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
       }
     }
   c->nexclpair = (int)(exclpair.size() / 2);
+  std::vector<int32_t> xs_start(n + 1, 0), xs_j(exclpair.size());
+  for (size_t k = 0; k < exclpair.size(); k++) xs_start[exclpair[k] + 1]++;
+  for (int i = 0; i < n; i++) xs_start[i + 1] += xs_start[i];
+  {
+    std::vector<int32_t> fill(xs_start.begin(), xs_start.end() - 1);
+    for (size_t k = 0; k < exclpair.size(); k += 2) {
+      const int i = exclpair[k], j = exclpair[k + 1];
+      xs_j[fill[i]++] = j;
+      xs_j[fill[j]++] = i;
+    }
+  }
   // bonded: 0-based indices, parameters packed
   std::vector<int32_t> hb, ha, ht, hr, hp14;
   std::vector<double> par, p14f;
@@ -709,15 +683,21 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   par.insert(par.end(), c->tors_period.begin(), c->tors_period.end());
   c->off_rbc = (int)par.size();
   par.insert(par.end(), c->rb_c.begin(), c->rb_c.end());
+  // terms the reference skips for a vanishing force constant (|K| < 1e-16
+  // for bonds, < 1e-32 for angles) are left out of the device lists
   for (size_t k = 0; k < c->bond_type.size(); k++) {
+    if (fabs(c->bond_k[c->bond_type[k] - 1]) < 1.0e-16) continue;
     hb.push_back(c->bond_ij[2 * k] - 1);
     hb.push_back(c->bond_ij[2 * k + 1] - 1);
     hb.push_back(c->bond_type[k] - 1);
   }
   for (size_t k = 0; k < c->angl_type.size(); k++) {
+    if (fabs(c->angl_k[c->angl_type[k] - 1]) < 1.0e-32) continue;
     for (int q = 0; q < 3; q++) ha.push_back(c->angl_ijk[3 * k + q] - 1);
     ha.push_back(c->angl_type[k] - 1);
   }
+  c->nbond_act = (int)(hb.size() / 3);
+  c->nangl_act = (int)(ha.size() / 4);
   for (size_t k = 0; k < c->tors_type.size(); k++) {
     int np = (int)lround(fabs(c->tors_period[c->tors_type[k] - 1]));
     if (np < 1 || np > 6) {
@@ -776,6 +756,7 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
       (rc = upload(&c->d_rmin, c->lj_rmin)) || (rc = upload(&c->d_inttyp, c->inttyp)) ||
       (rc = upload(&c->d_excl_index, c->excl_index)) ||
       (rc = upload(&c->d_excl_list, c->excl_list)) || (rc = upload(&c->d_exclpair, exclpair)) ||
+      (rc = upload(&c->d_xs_start, xs_start)) || (rc = upload(&c->d_xs_j, xs_j)) ||
       (rc = upload(&c->d_spl, real8)) || (rc = upload(&c->d_splx, excs8)) ||
       (rc = upload(&c->d_splf, realf)) || (rc = upload(&c->d_ljp64, ljp64)) ||
       (rc = upload(&c->d_ljp32, ljp32)) || (rc = upload(&c->d_bsp, bsp)) ||
@@ -816,7 +797,8 @@ int pgk_destroy(pgk_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->d_pos, c->d_q, c->d_type, c->d_molid, c->d_eps, c->d_rmin, c->d_inttyp,
-                  c->d_excl_index, c->d_excl_list, c->d_exclpair, c->d_spl, c->d_splx,
+                  c->d_excl_index, c->d_excl_list, c->d_exclpair, c->d_xs_start, c->d_xs_j,
+                  c->d_spl, c->d_splx,
                   c->d_splf, c->d_ljp64, c->d_ljp32, c->d_bsp, c->d_bond, c->d_angl, c->d_tors,
                   c->d_rb, c->d_p14, c->d_p14f, c->d_bpar, c->d_bin_count, c->d_bin_start,
                   c->d_atom_bin, c->d_atom_slot, c->d_sorted, c->d_col_ncl, c->d_col_clstart,

@@ -112,6 +112,8 @@ struct pgk_ctx {
   int32_t *d_excl_index = nullptr, *d_excl_list = nullptr;
   int32_t nexclpair = 0;
   int32_t *d_exclpair = nullptr;   // 2 * nexclpair, 0-based (i<j)
+  int32_t *d_xs_start = nullptr;   // N+1: symmetric exclusion CSR (both directions)
+  int32_t *d_xs_j = nullptr;       // 2 * nexclpair partners, 0-based
   double *d_spl = nullptr;         // nspl * 8 (real a,b,c,d | realpot a,b,c,d)
   double *d_splx = nullptr;        // nspl * 8 (excs | excspot)
   float *d_splf = nullptr;         // nspl * 8 float copy (real | realpot)
@@ -128,6 +130,9 @@ struct pgk_ctx {
   double *d_p14f = nullptr;        // 2 per 1-4 pair: vdw factor, elc factor
   double *d_bpar = nullptr;        // bond k, r0 | angle k, t0 | tors ... packed
   int n14 = 0;
+  // bonds / angles whose force constant passes the reference's skip test
+  // (calbond.F90:70, calangl.F90): terms below it never contribute
+  int nbond_act = 0, nangl_act = 0;
   int off_bk = 0, off_br = 0, off_ak = 0, off_at = 0, off_tb = 0, off_tp = 0,
       off_tper = 0, off_tdiv = 0, off_rbc = 0;
 
@@ -176,6 +181,7 @@ struct pgk_ctx {
   size_t theta_bytes = 0;
   void *d_tile = nullptr;          // spreading bins: counts, starts, ranks, items, scan space
   size_t tile_bytes = 0;
+  int tile_ntile = 0;
   double *d_tw = nullptr;          // twiddles: 3 dims, each n complex (FP64)
   float *d_twf = nullptr;
   int tw_off[3] = {0, 0, 0};
@@ -224,9 +230,11 @@ void pgk_prof_collect(pgk_ctx *c);
 // ---- kernels / launchers (return hipError_t of the launch) ----------------
 int pgk_list_build(pgk_ctx *c, double rcut_book);
 int pgk_pairs_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
-int pgk_excl_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
+int pgk_combine_launch(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
+                       double *fvl, int *red_off, int *red_n);
 int pgk_pme_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_pme_prepare(pgk_ctx *c);
+int pgk_pme_prep(pgk_ctx *c, bool pack);
 int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
                        int *seg_kind, int *nseg, bool hf);
 int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
