@@ -250,6 +250,31 @@ int pgk_energies(pgk_ctx *ctx, double *ene, double *virt);
 /* Wait for all work queued by the context. */
 int pgk_synchronize(pgk_ctx *ctx);
 
+/* ---- device-resident NVE step (no reference counterpart in the force path:
+ * the integrator of moldyn.F90:1465-1771 with the RATTLE of rattle.F90:6-272,
+ * for drivers that keep the whole step on the GPU) ------------------------- */
+typedef struct pgk_md_desc {
+  const double *mass;          /* [natom] amu                                  */
+  int32_t ncons;               /* rigid bonds                                  */
+  const int32_t *cons_ij;      /* [2*ncons] 1-based atom pairs                 */
+  const double *cons_d;        /* [ncons] lengths (A)                          */
+  int32_t ngroup;              /* coupled constraint groups (molecules)        */
+  const int32_t *group_start;  /* [ngroup+1] 0-based offsets into the list    */
+  double dt;                   /* time step (fs)                               */
+  double tol;                  /* relative constraint tolerance                */
+} pgk_md_desc;
+int pgk_md_setup(pgk_ctx *ctx, const pgk_md_desc *md);
+/* Velocities [3*natom] (A/fs); the constraint velocity projection is applied. */
+int pgk_md_set_velocities(pgk_ctx *ctx, const double *vel);
+int pgk_md_get_state(pgk_ctx *ctx, double *pos, double *vel);
+/* Run nstep velocity-Verlet steps from the current positions and velocities
+ * (every force term on MTS_LONG; the Verlet list rebuilt every nstep_book
+ * steps with rcut_book).  Every nsample steps (and at step 0) samples[3*k..]
+ * receives kinetic, potential (pot_tot) and total energy (eref); samples
+ * holds nstep/nsample + 1 rows.                                              */
+int pgk_md_run(pgk_ctx *ctx, int32_t nstep, int32_t nstep_book, double rcut_book,
+               int32_t nsample, double *samples);
+
 /* Stage timing (no reference counterpart: instrumentation for measurement).
  * enable != 0 turns HIP-event timing of each stage on (and clears the
  * totals); ms[PGK_NPROF_STAGES] and calls[PGK_NPROF_STAGES] receive the

@@ -78,6 +78,12 @@ class HfOut(C.Structure):
     _fields_ = [("pot_atm", _dp), ("viri_atm", _dp)]
 
 
+class MdDesc(C.Structure):
+    _fields_ = [("mass", _dp), ("ncons", C.c_int32), ("cons_ij", _ip), ("cons_d", _dp),
+                ("ngroup", C.c_int32), ("group_start", _ip), ("dt", C.c_double),
+                ("tol", C.c_double)]
+
+
 def dptr(a):
     return None if a is None else a.ctypes.data_as(_dp)
 

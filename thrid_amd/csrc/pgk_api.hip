@@ -808,7 +808,9 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
                   c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta, c->d_tile,
                   c->d_hf_z, c->d_hf_typ, c->d_hf_com, c->d_hf_pe, c->d_hf_w,
-                  c->d_hf_pe_out, c->d_hf_w_out};
+                  c->d_hf_pe_out, c->d_hf_w_out, c->d_md_gstart, c->d_md_cij, c->d_md_fail,
+                  c->d_md_small,
+                  c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_ke};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -1073,6 +1075,187 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
   }
   HIP_CHECK(hipMemcpy(hfo->pot_atm, c->d_hf_pe_out, pe_b, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(hfo->viri_atm, c->d_hf_w_out, wout_b, hipMemcpyDeviceToHost));
+  return PGK_OK;
+}
+
+// ---- device-resident NVE step ---------------------------------------------
+int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
+  if (!c || !md || !md->mass || !(md->dt > 0) || md->ncons < 0 || md->ngroup < 0 ||
+      (md->ncons > 0 && (!md->cons_ij || !md->cons_d || !md->group_start))) {
+    pgk_set_error("pgk_md_setup: invalid description");
+    return PGK_EINVAL;
+  }
+  const int n = c->natom;
+  std::vector<double> minv(n);
+  for (int i = 0; i < n; i++) {
+    if (!(md->mass[i] > 0)) {
+      pgk_set_error("pgk_md_setup: masses must be positive");
+      return PGK_EINVAL;
+    }
+    minv[i] = 1.0 / md->mass[i];
+  }
+  std::vector<int32_t> gs, cij;
+  std::vector<double> d2;
+  if (md->ncons > 0) {
+    gs.assign(md->group_start, md->group_start + md->ngroup + 1);
+    if (gs[0] != 0 || gs[md->ngroup] != md->ncons) {
+      pgk_set_error("pgk_md_setup: group_start must run from 0 to ncons");
+      return PGK_EINVAL;
+    }
+    for (int g = 0; g < md->ngroup; g++)
+      if (gs[g + 1] < gs[g]) {
+        pgk_set_error("pgk_md_setup: group_start not monotone");
+        return PGK_EINVAL;
+      }
+    for (int k = 0; k < md->ncons; k++) {
+      const int i = md->cons_ij[2 * k], j = md->cons_ij[2 * k + 1];
+      if (i < 1 || i > n || j < 1 || j > n || i == j || !(md->cons_d[k] > 0)) {
+        pgk_set_error("pgk_md_setup: constraint out of range");
+        return PGK_EINVAL;
+      }
+      cij.push_back(i - 1);
+      cij.push_back(j - 1);
+      d2.push_back(md->cons_d[k] * md->cons_d[k]);
+    }
+  }
+  // triangle groups (3 atoms, 3 constraints) are reordered to (a,b), (a,c),
+  // (b,c) for the register path of the constraint kernels
+  const int ng = md->ncons > 0 ? md->ngroup : 0;
+  std::vector<int32_t> small(std::max(ng, 1), 0);
+  for (int g = 0; g < ng; g++) {
+    if (gs[g + 1] - gs[g] != 3) continue;
+    const int k0 = gs[g];
+    int at[3] = {cij[2 * k0], cij[2 * k0 + 1], -1};
+    for (int k = k0 + 1; k < k0 + 3; k++)
+      for (int e = 0; e < 2; e++) {
+        const int x = cij[2 * k + e];
+        if (x != at[0] && x != at[1]) at[2] = x;
+      }
+    if (at[2] < 0) continue;
+    int kk[3] = {-1, -1, -1};   // constraint index of (0,1), (0,2), (1,2)
+    const int want[3][2] = {{at[0], at[1]}, {at[0], at[2]}, {at[1], at[2]}};
+    for (int q = 0; q < 3; q++)
+      for (int k = k0; k < k0 + 3; k++) {
+        const int x = cij[2 * k], y = cij[2 * k + 1];
+        if ((x == want[q][0] && y == want[q][1]) || (x == want[q][1] && y == want[q][0]))
+          kk[q] = k;
+      }
+    if (kk[0] < 0 || kk[1] < 0 || kk[2] < 0 || kk[0] == kk[1] || kk[0] == kk[2] ||
+        kk[1] == kk[2])
+      continue;
+    const double dd[3] = {d2[kk[0]], d2[kk[1]], d2[kk[2]]};
+    for (int q = 0; q < 3; q++) {
+      cij[2 * (k0 + q)] = want[q][0];
+      cij[2 * (k0 + q) + 1] = want[q][1];
+      d2[k0 + q] = dd[q];
+    }
+    small[g] = 1;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if ((rc = upload(&c->d_md_minv, minv)) || (rc = upload(&c->d_md_gstart, gs)) ||
+      (rc = upload(&c->d_md_cij, cij)) || (rc = upload(&c->d_md_d2, d2)) ||
+      (rc = upload(&c->d_md_small, small)))
+    return rc;
+  std::vector<double> zero((size_t)3 * n, 0.0);
+  std::vector<int32_t> one(1, 0);
+  if ((rc = upload(&c->d_md_vel, zero)) || (rc = upload(&c->d_md_pos_old, zero)) ||
+      (rc = upload(&c->d_md_fail, one)))
+    return rc;
+  std::vector<double> rec((size_t)pgk_grid_blocks(n, 256) * PGK_REC, 0.0);
+  if ((rc = upload(&c->d_md_ke, rec))) return rc;
+  c->md_ngroup = md->ncons > 0 ? md->ngroup : 0;
+  c->md_ncons = md->ncons;
+  c->md_dt = md->dt;
+  c->md_tol = md->tol > 0 ? md->tol : 1e-10;
+  c->md_ready = true;
+  return PGK_OK;
+}
+
+int pgk_md_set_velocities(pgk_ctx *c, const double *vel) {
+  if (!c || !vel || !c->md_ready) {
+    pgk_set_error("pgk_md_set_velocities: no MD set-up or null argument");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpyAsync(c->d_md_vel, vel, sizeof(double) * 3 * c->natom,
+                           hipMemcpyHostToDevice, c->stream));
+  if ((rc = pgk_md_rattle_v(c))) return rc;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return PGK_OK;
+}
+
+int pgk_md_get_state(pgk_ctx *c, double *pos, double *vel) {
+  if (!c || !c->md_ready) {
+    pgk_set_error("pgk_md_get_state: no MD set-up");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * 3 * c->natom;
+  if (pos) HIP_CHECK(hipMemcpyAsync(pos, c->d_pos, nb, hipMemcpyDeviceToHost, c->stream));
+  if (vel) HIP_CHECK(hipMemcpyAsync(vel, c->d_md_vel, nb, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return PGK_OK;
+}
+
+static int md_sample(pgk_ctx *c, double *row) {
+  int rc = pgk_md_ke(c, c->d_md_ke);
+  if (rc) return rc;
+  double ene[PGK_NENE] = {};
+  if ((rc = finalize_energies(c, ene, nullptr))) return rc;   // waits for the stream
+  const int nb = pgk_grid_blocks(c->natom, 256);
+  std::vector<double> rec((size_t)nb * PGK_REC);
+  HIP_CHECK(hipMemcpy(rec.data(), c->d_md_ke, sizeof(double) * rec.size(),
+                      hipMemcpyDeviceToHost));
+  double ke = 0.0;
+  for (int b = 0; b < nb; b++) ke += rec[(size_t)b * PGK_REC];
+  int32_t fail = 0;
+  HIP_CHECK(hipMemcpy(&fail, c->d_md_fail, sizeof(fail), hipMemcpyDeviceToHost));
+  if (fail) {
+    pgk_set_error("pgk_md_run: constraints did not converge (RATTLE)");
+    return PGK_EOVERFLOW;
+  }
+  row[0] = ke;
+  row[1] = ene[PGK_E_TOT];
+  row[2] = ke + ene[PGK_E_TOT];
+  return PGK_OK;
+}
+
+int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
+               int32_t nsample, double *samples) {
+  if (!c || !c->md_ready || nstep < 0 || nstep_book < 1 || nsample < 1 || !samples) {
+    pgk_set_error("pgk_md_run: no MD set-up or invalid argument");
+    return PGK_EINVAL;
+  }
+  if (!c->have_pos) {
+    pgk_set_error("pgk_md_run: positions not set");
+    return PGK_ESTATE;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  const pgk_mts m = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
+                     PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+  if ((rc = pgk_build_list(c, rcut_book))) return rc;
+  if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr, nullptr,
+                         nullptr, false, false)))
+    return rc;
+  if ((rc = md_sample(c, samples))) return rc;
+  int row = 1;
+  for (int s = 1; s <= nstep; s++) {
+    if ((rc = pgk_md_kick_drift(c))) return rc;
+    if (s % nstep_book == 0 && (rc = pgk_build_list(c, rcut_book))) return rc;
+    if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr,
+                           nullptr, nullptr, false, false)))
+      return rc;
+    if ((rc = pgk_md_kick(c))) return rc;
+    if (s % nsample == 0) {
+      if ((rc = md_sample(c, samples + 3 * row))) return rc;
+      row++;
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return PGK_OK;
 }
 

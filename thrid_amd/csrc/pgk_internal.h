@@ -195,6 +195,15 @@ struct pgk_ctx {
   double *d_hf_pe_out = nullptr, *d_hf_w_out = nullptr;
   size_t hf_z_cap = 0, hf_com_cap = 0, hf_w_cap = 0, hf_wout_cap = 0;
 
+  // device-resident MD step (velocity Verlet + RATTLE)
+  bool md_ready = false;
+  int md_ngroup = 0, md_ncons = 0;
+  double md_dt = 1.0, md_tol = 1e-10;
+  int32_t *d_md_gstart = nullptr, *d_md_cij = nullptr, *d_md_fail = nullptr;
+  int32_t *d_md_small = nullptr;
+  double *d_md_d2 = nullptr, *d_md_minv = nullptr, *d_md_vel = nullptr, *d_md_pos_old = nullptr;
+  double *d_md_ke = nullptr;       // kinetic-energy block records
+
   // reduction segment bookkeeping for the current call
   int red_used = 0;
   // accumulators (force sums, charge grid) are cleared by the kernels that
@@ -239,6 +248,10 @@ int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off,
                        int *seg_kind, int *nseg, bool hf);
 int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_export(pgk_ctx *c);
+int pgk_md_kick_drift(pgk_ctx *c);
+int pgk_md_kick(pgk_ctx *c);
+int pgk_md_rattle_v(pgk_ctx *c);
+int pgk_md_ke(pgk_ctx *c, double *rec);
 
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);

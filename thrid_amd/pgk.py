@@ -52,13 +52,18 @@ def load_library(path: str = LIB_PATH):
     L.pgk_calforce_async.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.c_int32, vp]
     L.pgk_energies.argtypes = [vp, vp, vp]
     L.pgk_synchronize.argtypes = [vp]
+    L.pgk_md_setup.argtypes = [vp, C.POINTER(abi.MdDesc)]
+    L.pgk_md_set_velocities.argtypes = [vp, vp]
+    L.pgk_md_get_state.argtypes = [vp, vp, vp]
+    L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
     for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
               "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
-              "pgk_list_info"):
+              "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
+              "pgk_md_run"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -196,6 +201,36 @@ class ForceContext:
                                       C.byref(ho)), "pgk_calforce_hf")
         return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
                     pot_atm=pe, viri_atm=wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+
+    # ---- device-resident NVE step ------------------------------------------
+    def md_setup(self, mass, cons_ij, cons_d, group_start, dt, tol=1e-10):
+        """mass [natom] amu; cons_ij (ncons,2) 1-based; cons_d (ncons,) A;
+        group_start (ngroup+1,) 0-based offsets; dt in fs."""
+        k = self._md_keep = dict(m=np.ascontiguousarray(mass, np.float64),
+                                 ij=np.ascontiguousarray(cons_ij, np.int32).reshape(-1),
+                                 d=np.ascontiguousarray(cons_d, np.float64),
+                                 g=np.ascontiguousarray(group_start, np.int32))
+        d = abi.MdDesc(abi.dptr(k["m"]), len(k["d"]), abi.iptr(k["ij"]), abi.dptr(k["d"]),
+                       max(len(k["g"]) - 1, 0), abi.iptr(k["g"]), float(dt), float(tol))
+        _check(self.L.pgk_md_setup(self.h, C.byref(d)), "pgk_md_setup")
+
+    def md_set_velocities(self, vel):
+        v = np.ascontiguousarray(vel, np.float64)
+        _check(self.L.pgk_md_set_velocities(self.h, v.ctypes.data), "pgk_md_set_velocities")
+
+    def md_state(self):
+        p = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        _check(self.L.pgk_md_get_state(self.h, p.ctypes.data, v.ctypes.data), "pgk_md_get_state")
+        return p, v
+
+    def md_run(self, nstep, nstep_book, rcut_book, nsample):
+        """Velocity Verlet + RATTLE; returns (nstep//nsample + 1, 3) rows of
+        kinetic, potential and total energy (eref)."""
+        out = np.zeros((nstep // nsample + 1, 3))
+        _check(self.L.pgk_md_run(self.h, int(nstep), int(nstep_book), float(rcut_book),
+                                 int(nsample), out.ctypes.data), "pgk_md_run")
+        return out
 
     def list_info(self):
         """(clusters, cluster pairs, listed atom pairs, atoms per cluster)."""
