@@ -1,15 +1,18 @@
-"""Benchmark: force-evaluation steps of the 100,001-atom SPME water box.
+"""Benchmark: MD steps of the 100,001-atom SPME water box on one MI355X.
 
-Workload (one step = one calforce call with every term on MTS_LONG and the
-virial on, positions resident in HBM; the Verlet list is rebuilt every
-NSTEP_BOOK steps inside the timed region): 33,333 rigid 3-site waters plus one
-Na+/Cl- pair in a 100.1 A cubic box, SPME 128^3 order 4, alpha 0.32/A, 10 A
-real-space and LJ cutoff, rcut_book 11 A.
+Workload: 33,333 rigid 3-site waters plus one Na+/Cl- pair in a 100.1 A
+cubic box, SPME 128^3 order 4, alpha 0.32/A, 10 A real-space and LJ cutoff,
+rcut_book 11 A, dt = 1 fs.  One step = one device-resident NVE step
+(velocity Verlet, every force term on MTS_LONG, RATTLE on the waters); the
+Verlet list is rebuilt every NSTEP_BOOK steps inside the timed region.  The
+start is the generated lattice relaxed by a short untimed FP64 run with
+velocity rescaling to 300 K.
 
-Prints one JSON line.  value = atom-steps/s of the mixed form; the FP64
-form's rate, the speed-up, ns/day (dt = 1 fs), the pair kernel's roofline
-(HIP events on the library stream) and the reference CPU baseline are
-reported beside it.
+Prints one JSON line.  value = atom-steps/s of the mixed form; beside it the
+FP64 form, ns/day, the force-only step rate, the pair kernel's roofline (HIP
+events on the library stream), each form's force error against the
+reference's FP64 forces, NVE energy drift per form and the reference CPU
+baseline.
 """
 import argparse
 import json
@@ -23,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 NSTEP_BOOK = 20
+DT_FS = 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3, "fp32": 157.3}   # TFLOP/s, MI355X vector
 
 
@@ -57,42 +61,78 @@ def pair_counts(sysd):
     return ncoul, nlj
 
 
-def run_gpu(sysd, precision, steps, warmup, nsync=20):
-    """Time `steps` force steps after `warmup` untimed ones.  Each step queues
-    one full calforce (every term, virial on; energies and virials reduced on
-    the device) without a host round trip, as a device-resident MD driver
-    does; the list is rebuilt every NSTEP_BOOK steps inside the timed region
-    and the last step's energies are read at the end.  Also times the
-    synchronous per-call form (energies copied to the host every call).
-    Returns (s/step, stage ms, energies, list info, forces, sync s/step)."""
+def equilibrated_state(sysd, seed=5):
+    """Relax the generated start: 2000 FP64 steps with velocity rescaling to
+    300 K (untimed).  Returns positions, velocities and the constraints."""
+    from thrid_amd import md
+    from thrid_amd.pgk import ForceContext
+    ij, d, gs = md.rigid_constraints(sysd)
+    ctx = ForceContext(sysd, "fp64")
+    ctx.md_setup(sysd["mass"], ij, d, gs, DT_FS)
+    ctx.md_set_velocities(md.maxwell_boltzmann(sysd, 300.0, seed=seed))
+    pos, vel = md.equilibrate(ctx, sysd, 300.0, 10, 200, NSTEP_BOOK, len(d))
+    ctx.close()
+    return pos, vel, (ij, d, gs)
+
+
+def run_md(sysd, state, precision, steps, warmup, ndrift):
+    """Time exactly `steps` MD steps (after `warmup`), then an NVE run of
+    `ndrift` steps for the energy drift.  Returns (s/step, stage ms, list
+    info, drift eref/ns/atom, energy rows of the drift run)."""
+    from thrid_amd import md
+    from thrid_amd.pgk import ForceContext
+    pos, vel, (ij, d, gs) = state
+    s2 = dict(sysd)
+    s2["pos"] = pos
+    ctx = ForceContext(s2, precision)
+    ctx.md_setup(sysd["mass"], ij, d, gs, DT_FS)
+    ctx.md_set_velocities(vel)
+    if warmup:
+        ctx.md_run(warmup, NSTEP_BOOK, sysd["rcut_book"], warmup)
+    ctx.synchronize()
+    ctx.profile_stages(1)          # HIP events around the pair kernel only
+    ctx.profile(1)
+    t0 = time.perf_counter()
+    ctx.md_run(steps, NSTEP_BOOK, sysd["rcut_book"], steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms, calls = ctx.profile(0)
+    info = ctx.list_info()
+    # stage breakdown from a separate profiled run of the same length
+    ctx.profile_stages(0xff)
+    ctx.profile(1)
+    ctx.md_run(steps, NSTEP_BOOK, sysd["rcut_book"], steps)
+    ms_all, calls_all = ctx.profile(0)
+    ctx.set_positions(pos)
+    ctx.md_set_velocities(vel)
+    rows = ctx.md_run(ndrift, NSTEP_BOOK, sysd["rcut_book"], 50)
+    ctx.close()
+    stages = ["pairs", "excl", "spread", "fft", "gather", "bonded", "list", "other"]
+    prof = {s: (float(ms_all[k] / calls_all[k]) if calls_all[k] else 0.0)
+            for k, s in enumerate(stages)}
+    prof["pairs_timed_region"] = float(ms[0] / calls[0]) if calls[0] else 0.0
+    return (t1 - t0) / steps, prof, info, md.drift_per_ns(rows, 50, DT_FS, sysd["natom"]), rows
+
+
+def run_force_only(sysd, precision, steps):
+    """Force-only steps at the generated positions (queued without host
+    round trips) and the forces there, for the accuracy check."""
     from thrid_amd.pgk import ForceContext
     ctx = ForceContext(sysd, precision)
     ctx.mklist(sysd["rcut_book"])
-    info = ctx.list_info()
-    for k in range(warmup):
-        if k % NSTEP_BOOK == 0:
-            ctx.mklist(sysd["rcut_book"])
+    for _ in range(3):
         ctx.calforce_async()
     ctx.synchronize()
-    ctx.profile(1)
     t0 = time.perf_counter()
     for k in range(steps):
         if k % NSTEP_BOOK == 0:
             ctx.mklist(sysd["rcut_book"])
         ctx.calforce_async()
-    e, _ = ctx.energies()
+    ctx.energies()
     t1 = time.perf_counter()
-    ms, calls = ctx.profile(0)
-    ene = e.copy()
-    t2 = time.perf_counter()
-    for k in range(nsync):
-        ctx.calforce_device()
-    t3 = time.perf_counter()
-    force = ctx.calforce()["force"]          # untimed, same positions
+    force = ctx.calforce()["force"]
     ctx.close()
-    stages = ["pairs", "excl", "spread", "fft", "gather", "bonded", "list", "other"]
-    prof = {s: (float(ms[k] / calls[k]) if calls[k] else 0.0) for k, s in enumerate(stages)}
-    return (t1 - t0) / steps, prof, ene, info, force, (t3 - t2) / nsync
+    return (t1 - t0) / steps, force
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -178,6 +218,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--drift-steps", type=int, default=3000,
+                    help="NVE steps per form for the energy drift")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
@@ -198,21 +240,25 @@ def main():
         copy_calibration()
     sysd = systems.water_ion_box_100k()
     n = sysd["natom"]
+    state = equilibrated_state(sysd)
     res = {}
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
     for prec in precs:
-        sec, prof, ene, info, force, sync = run_gpu(sysd, prec, args.steps, args.warmup)
-        res[prec] = dict(sec=sec, prof=prof, ene=ene, info=info, force=force, sync=sync)
-        print(f"# {prec}: {sec * 1e3:.3f} ms/step (per-call sync form {sync * 1e3:.3f}), stages(ms) "
+        sec, prof, info, drift, _rows = run_md(sysd, state, prec, args.steps, args.warmup,
+                                               args.drift_steps)
+        fsec, force = run_force_only(sysd, prec, 100)
+        res[prec] = dict(sec=sec, prof=prof, info=info, drift=drift, fsec=fsec, force=force)
+        print(f"# {prec}: {sec * 1e3:.3f} ms/MD step (force only {fsec * 1e3:.3f}), drift "
+              f"{drift:.3e} eref/ns/atom, stages(ms) "
               + " ".join(f"{k}={v:.3f}" for k, v in prof.items()), file=sys.stderr, flush=True)
     head = res["mixed"] if "mixed" in res else res["fp64"]
     hp = "mixed" if "mixed" in res else "fp64"
     ncoul, nlj = pair_counts(sysd)
     flops = ncoul * (36 + 12) + nlj * 13
-    t_pair = head["prof"]["pairs"] * 1e-3
+    t_pair = head["prof"]["pairs_timed_region"] * 1e-3
     achieved = flops / t_pair / 1e12 if t_pair > 0 else 0.0
     out = {
-        "metric": "atom-steps/s (mixed SPME force step), 100k water",
+        "metric": "atom-steps/s (mixed SPME MD step), 100k water",
         "value": n / head["sec"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
@@ -225,17 +271,19 @@ def main():
         "dtype": "f32 (pairs, spline, FFT) + int64 fixed-point sums" if hp == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": "100,001-atom water + Na+/Cl-, 100.1 A box, SPME 128^3 p=4, "
-                               "alpha 0.32, rc 10 A, rcut_book 11 A, list every 20 steps",
+                               "alpha 0.32, rc 10 A, rcut_book 11 A, list every 20 steps, "
+                               "NVE dt 1 fs, RATTLE",
                    "natom": n, "precision": hp},
-        "ns_per_day": 86400.0 / head["sec"] * 1e-6,
-        "sync_call_ms_per_step": {p: res[p]["sync"] * 1e3 for p in res},
+        "ns_per_day": 86400.0 / head["sec"] * DT_FS * 1e-6,
+        "force_only_ms_per_step": {p: res[p]["fsec"] * 1e3 for p in res},
+        "nve_drift_eref_per_ns_per_atom": {p: res[p]["drift"] for p in res},
         "stage_ms": {p: res[p]["prof"] for p in res},
         "roofline": {"bound": "valu", "kernel": f"k_pairs ({hp})", "achieved": achieved,
                      "peak": PEAK[hp], "unit": "TFLOP/s",
                      "frac": achieved / PEAK[hp],
                      "traffic": traffic_from_profiles(f"k_pairs_{hp}"),
                      "flops_per_launch": flops, "pairs_coulomb": ncoul, "pairs_lj": nlj,
-                     "avg_launch_ms": head["prof"]["pairs"]},
+                     "avg_launch_ms": head["prof"]["pairs_timed_region"]},
         "list": {"clusters": head["info"][0], "cluster_pairs": head["info"][1],
                  "listed_pairs": head["info"][2],
                  "lane_efficiency": ncoul / (64.0 * head["info"][1])},
@@ -243,9 +291,9 @@ def main():
     if "fp64" in res and "mixed" in res:
         f64 = res["fp64"]
         out["fp64"] = {"atom_steps_per_s": n / f64["sec"], "ms_per_step": f64["sec"] * 1e3,
-                       "ns_per_day": 86400.0 / f64["sec"] * 1e-6,
+                       "ns_per_day": 86400.0 / f64["sec"] * DT_FS * 1e-6,
                        "roofline_pairs": {
-                           "achieved": flops / (f64["prof"]["pairs"] * 1e-3) / 1e12,
+                           "achieved": flops / (f64["prof"]["pairs_timed_region"] * 1e-3) / 1e12,
                            "peak": PEAK["fp64"], "unit": "TFLOP/s"}}
         out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
                                                  / PEAK["fp64"])

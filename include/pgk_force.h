@@ -285,6 +285,9 @@ enum {
   PGK_NPROF_STAGES
 };
 int pgk_profile(pgk_ctx *ctx, int32_t enable, double *ms, int64_t *calls);
+/* Stages that record events while timing is on: bit k for stage k (default
+ * all).  Timing fewer stages keeps the host's enqueue rate up.               */
+int pgk_profile_stages(pgk_ctx *ctx, uint32_t mask);
 /* Shape of the current list (no reference counterpart: instrumentation):
  * info[0] clusters, info[1] cluster pairs, info[2] listed atom pairs,
  * info[3] atoms per cluster.                                                  */

@@ -20,7 +20,7 @@ static thread_local std::string g_err;
 void pgk_set_error(const std::string &msg) { g_err = msg; }
 
 void pgk_prof_begin(pgk_ctx *c, int k) {
-  if (!c->prof) return;
+  if (!c->prof || !((c->prof_mask >> k) & 1u)) return;
   const int i = c->pn++;
   while ((int)c->pev.size() < 2 * (i + 1)) {
     hipEvent_t e = nullptr;
@@ -33,7 +33,7 @@ void pgk_prof_begin(pgk_ctx *c, int k) {
   (void)hipEventRecord(c->pev[2 * i], c->stream);
 }
 void pgk_prof_end(pgk_ctx *c, int k) {
-  if (!c->prof) return;
+  if (!c->prof || !((c->prof_mask >> k) & 1u)) return;
   (void)hipEventRecord(c->pev[2 * c->popen[k] + 1], c->stream);
 }
 // call after the stream has been synchronised
@@ -1276,6 +1276,12 @@ int pgk_profile(pgk_ctx *c, int32_t enable, double *ms, int64_t *calls) {
       c->prof_calls[k] = 0;
     }
   }
+  return PGK_OK;
+}
+
+int pgk_profile_stages(pgk_ctx *c, uint32_t mask) {
+  if (!c) return PGK_EINVAL;
+  c->prof_mask = mask;
   return PGK_OK;
 }
 

@@ -213,6 +213,7 @@ struct pgk_ctx {
   // optional per-stage timing: every stage instance records a begin/end pair
   // of HIP events from a pool; pairs are read at the next synchronisation
   bool prof = false;
+  uint32_t prof_mask = 0xffffffffu;   // stages that record events
   std::vector<hipEvent_t> pev;     // 2 per pending stage instance
   std::vector<int> pstage;
   int pn = 0;                      // pending stage instances

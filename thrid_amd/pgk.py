@@ -58,12 +58,13 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
+    L.pgk_profile_stages.argtypes = [vp, C.c_uint32]
     for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
               "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
-              "pgk_md_run"):
+              "pgk_md_run", "pgk_profile_stages"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -246,6 +247,10 @@ class ForceContext:
         _check(self.L.pgk_profile(self.h, int(enable), ms.ctypes.data, calls.ctypes.data),
                "pgk_profile")
         return ms, calls
+
+    def profile_stages(self, mask):
+        """Stages timed while profiling is on: bit k = stage k (PAIRS = 0)."""
+        _check(self.L.pgk_profile_stages(self.h, int(mask)), "pgk_profile_stages")
 
     def synchronize(self):
         _check(self.L.pgk_synchronize(self.h), "pgk_synchronize")
