@@ -57,6 +57,65 @@ def test_list_pair_set_bitexact(oracle_mod, which, mode):
     assert np.array_equal(got, want)
 
 
+def test_list_pair_set_bitexact_100k(oracle_mod):
+    """100k-atom water/ion box (100 A cube, rcut_book 11 A): columns of ~23
+    clusters, so the search uses plain and wrapped z windows."""
+    s = systems.water_ion_box_100k()
+    o = oracle_mod.Oracle(s)
+    idx, lst = o.mklist(s["rcut_book"], 1)
+    idx = np.asarray(idx, np.int64)
+    lst = np.asarray(lst)
+    n = s["natom"]
+    # row i (1-based, i < n) holds lst[idx[i-1]-1 : idx[i]-1]; the last atom has none
+    rows = np.repeat(np.arange(1, n), np.diff(idx))
+    ent = lst[idx[0] - 1: idx[n - 1] - 1]
+    keep = ent > 0
+    want = np.stack([rows[keep], ent[keep]], 1).astype(np.int32)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    c = _ctx(s, "mixed")
+    c.mklist(s["rcut_book"])
+    got = c.list_pairs()
+    print(f"100k: {len(want)} pairs")
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+def _downward_exclusions(sysd):
+    """Every exclusion stored on the higher atom's row (pointing down)."""
+    idx, lst = systems.exclusions(sysd)
+    n = sysd["natom"]
+    rows = [[] for _ in range(n)]
+    for i in range(n):
+        for k in range(idx[i] - 1, idx[i + 1] - 1):
+            if lst[k] > 0:
+                rows[lst[k] - 1].append(i + 1)
+    nidx = np.zeros(n + 1, np.int32)
+    nlst = []
+    for i in range(n):
+        nidx[i] = len(nlst) + 1
+        nlst += sorted(rows[i]) if rows[i] else [0]
+    nidx[n] = len(nlst) + 1
+    return nidx, np.array(nlst, np.int32)
+
+
+def test_list_ignores_downward_exclusions(oracle_mod):
+    """mklist2a (mklist.F90:157-175) tests only the entries of atom i's own row
+    against j > i, so entries pointing to lower atoms remove nothing."""
+    from thrid_amd.pgk import ForceContext
+    s = butane_box()
+    excl = _downward_exclusions(s)
+    o = oracle_mod.Oracle(s, excl=excl)
+    idx, lst = o.mklist(s["rcut_book"], 0)
+    want = pairs_from_csr(idx, lst, s["natom"])
+    o2 = oracle_mod.Oracle(s)
+    idx2, lst2 = o2.mklist(s["rcut_book"], 0)
+    assert len(want) > len(pairs_from_csr(idx2, lst2, s["natom"]))
+    c = ForceContext(s, "fp64", excl=excl)
+    c.mklist(s["rcut_book"])
+    got = c.list_pairs()
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("which", ["water", "slab", "slab_periodic", "butane"])
 def test_calforce_fp64(oracle_mod, which):
     s = {"water": water648, "slab": small_slab, "butane": butane_box,

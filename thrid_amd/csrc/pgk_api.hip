@@ -662,6 +662,25 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
       xs_j[fill[j]++] = i;
     }
   }
+  // the list search removes a pair i<j only when j is on i's own list
+  // (mklist2a), so its symmetric copy keeps just the entries above the row
+  std::vector<int32_t> xl_start(n + 1, 0), xl_j;
+  {
+    std::vector<int32_t> up;
+    for (size_t k = 0; k < exclpair.size(); k += 2)
+      if (exclpair[k + 1] > exclpair[k]) {
+        up.push_back(exclpair[k]);
+        up.push_back(exclpair[k + 1]);
+      }
+    xl_j.resize(up.size());
+    for (size_t k = 0; k < up.size(); k++) xl_start[up[k] + 1]++;
+    for (int i = 0; i < n; i++) xl_start[i + 1] += xl_start[i];
+    std::vector<int32_t> fill(xl_start.begin(), xl_start.end() - 1);
+    for (size_t k = 0; k < up.size(); k += 2) {
+      xl_j[fill[up[k]]++] = up[k + 1];
+      xl_j[fill[up[k + 1]]++] = up[k];
+    }
+  }
   // bonded: 0-based indices, parameters packed
   std::vector<int32_t> hb, ha, ht, hr, hp14;
   std::vector<double> par, p14f;
@@ -757,6 +776,7 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
       (rc = upload(&c->d_excl_index, c->excl_index)) ||
       (rc = upload(&c->d_excl_list, c->excl_list)) || (rc = upload(&c->d_exclpair, exclpair)) ||
       (rc = upload(&c->d_xs_start, xs_start)) || (rc = upload(&c->d_xs_j, xs_j)) ||
+      (rc = upload(&c->d_xl_start, xl_start)) || (rc = upload(&c->d_xl_j, xl_j)) ||
       (rc = upload(&c->d_spl, real8)) || (rc = upload(&c->d_splx, excs8)) ||
       (rc = upload(&c->d_splf, realf)) || (rc = upload(&c->d_ljp64, ljp64)) ||
       (rc = upload(&c->d_ljp32, ljp32)) || (rc = upload(&c->d_bsp, bsp)) ||
@@ -798,11 +818,13 @@ int pgk_destroy(pgk_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->d_pos, c->d_q, c->d_type, c->d_molid, c->d_eps, c->d_rmin, c->d_inttyp,
                   c->d_excl_index, c->d_excl_list, c->d_exclpair, c->d_xs_start, c->d_xs_j,
+                  c->d_xl_start, c->d_xl_j,
                   c->d_spl, c->d_splx,
                   c->d_splf, c->d_ljp64, c->d_ljp32, c->d_bsp, c->d_bond, c->d_angl, c->d_tors,
                   c->d_rb, c->d_p14, c->d_p14f, c->d_bpar, c->d_bin_count, c->d_bin_start,
                   c->d_atom_bin, c->d_atom_slot, c->d_sorted, c->d_col_ncl, c->d_col_clstart,
-                  c->d_cl_atom, c->d_cl_col, c->d_slot_of, c->d_bbox, c->d_tmp_cj,
+                  c->d_cl_atom, c->d_cl_col, c->d_slot_of, c->d_bbox, c->d_posc, c->d_tmp_cj,
+                  c->d_list_meta,
                   c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
                   c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
@@ -814,6 +836,7 @@ int pgk_destroy(pgk_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
+  if (c->h_list_meta) (void)hipHostFree(c->h_list_meta);
   for (hipEvent_t e : c->pev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -886,6 +909,8 @@ int pgk_list_npairs(pgk_ctx *c, int64_t *np) {
     pgk_set_error("pgk_list_npairs: no list");
     return PGK_ESTATE;
   }
+  int rc = set_device_ctx(c);
+  if (rc || (rc = pgk_list_count_pairs(c))) return rc;
   *np = c->npairs;
   return PGK_OK;
 }
@@ -896,12 +921,12 @@ int pgk_list_pairs(pgk_ctx *c, int32_t *pairs, int64_t capacity) {
     pgk_set_error("pgk_list_pairs: no list");
     return PGK_ESTATE;
   }
+  int rc = set_device_ctx(c);
+  if (rc || (rc = pgk_list_count_pairs(c))) return rc;
   if (capacity < c->npairs) {
     pgk_set_error("pgk_list_pairs: capacity too small");
     return PGK_EINVAL;
   }
-  int rc = set_device_ctx(c);
-  if (rc) return rc;
   std::vector<int32_t> start(c->ncl + 1), cj(c->npair_cl), atoms((size_t)c->ncl * PGK_CL);
   std::vector<uint64_t> mask(c->npair_cl);
   HIP_CHECK(hipMemcpy(start.data(), c->d_ci_start, sizeof(int32_t) * (c->ncl + 1),
@@ -1291,6 +1316,8 @@ int pgk_list_info(pgk_ctx *c, int64_t *info) {
     pgk_set_error("pgk_list_info: no list");
     return PGK_ESTATE;
   }
+  int rc = set_device_ctx(c);
+  if (rc || (rc = pgk_list_count_pairs(c))) return rc;
   info[0] = c->ncl;
   info[1] = c->npair_cl;
   info[2] = c->npairs;

@@ -114,6 +114,8 @@ struct pgk_ctx {
   int32_t *d_exclpair = nullptr;   // 2 * nexclpair, 0-based (i<j)
   int32_t *d_xs_start = nullptr;   // N+1: symmetric exclusion CSR (both directions)
   int32_t *d_xs_j = nullptr;       // 2 * nexclpair partners, 0-based
+  int32_t *d_xl_start = nullptr;   // N+1: the same over entries j > i only (list search)
+  int32_t *d_xl_j = nullptr;
   double *d_spl = nullptr;         // nspl * 8 (real a,b,c,d | realpot a,b,c,d)
   double *d_splx = nullptr;        // nspl * 8 (excs | excspot)
   float *d_splf = nullptr;         // nspl * 8 float copy (real | realpot)
@@ -146,12 +148,15 @@ struct pgk_ctx {
   int32_t *d_cl_col = nullptr;     // ncl: column of each cluster
   int32_t *d_slot_of = nullptr;    // N
   float *d_bbox = nullptr;         // ncl * 8: lo xyz, hi xyz (wrapped frame)
+  double4 *d_posc = nullptr;       // ncl*8: FP64 position + atom index per slot
   int32_t *d_tmp_cj = nullptr;     // ncl * PGK_MAXJ
   uint64_t *d_tmp_mask = nullptr;
   int32_t *d_ci_count = nullptr, *d_ci_start = nullptr;
   int32_t *d_cj = nullptr;
   uint64_t *d_mask = nullptr;
   int64_t npair_cl = 0, cap_pair_cl = 0;
+  int32_t *d_list_meta = nullptr;  // cluster count, overflow flags, entry count
+  int32_t *h_list_meta = nullptr;  // pinned copy read once per rebuild
   int64_t npairs = 0;              // atom pairs in the list
   int32_t *d_overflow = nullptr;
   void *d_scan_tmp = nullptr;
@@ -239,6 +244,8 @@ void pgk_prof_collect(pgk_ctx *c);
 
 // ---- kernels / launchers (return hipError_t of the launch) ----------------
 int pgk_list_build(pgk_ctx *c, double rcut_book);
+int pgk_list_count_pairs(pgk_ctx *c);   // fills c->npairs (counted on request)
+int pgk_list_count_pairs(pgk_ctx *c);   // fills c->npairs (counted lazily)
 int pgk_pairs_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_combine_launch(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
                        double *fvl, int *red_off, int *red_n);
