@@ -220,3 +220,22 @@ def test_spme_orders_and_tiles(oracle_mod, order):
         err = rms_rel(got["force"], ref["force"])
         assert err <= tol, (prec, err)
         assert abs(ene(got, "pot_ewk") - ene(ref, "pot_ewk")) <= 10 * tol * abs(ene(ref, "pot_ewk"))
+
+
+@pytest.mark.parametrize("nfft", [(32, 64, 128), (256, 32, 64), (128, 24, 256), (20, 128, 30)])
+def test_spme_register_fft(oracle_mod, nfft):
+    """Grid lines of 32, 64, 128 and 256 points use the register FFT, other
+    lengths the LDS Stockham FFT, axis by axis; both against the oracle's
+    FFTPACK transform."""
+    s = systems.water_box(nfft=nfft)
+    ref = _oracle(oracle_mod, s, mode=0).calforce(abi.MTS_LONG)
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
+        c = _ctx(s, prec)
+        c.mklist(s["rcut_book"])
+        got = c.calforce(abi.MTS_LONG)
+        err = rms_rel(got["force"], ref["force"])
+        assert err <= tol, (prec, err)
+        for name in ("pot_ewk", "pot_tot"):
+            assert abs(ene(got, name) - ene(ref, name)) <= 10 * tol * abs(ene(ref, name)), name
+        vs = float(np.max(np.abs(ref["virt"])))
+        assert float(np.max(np.abs(got["virt"] - ref["virt"]))) <= 100 * tol * vs

@@ -153,6 +153,17 @@ __device__ __forceinline__ void acc_add_j(unsigned long long *p, float v) {
 __device__ __forceinline__ void acc_add_j(double *p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void acc_add_j(float *p, float v) { atomicAdd(p, v); }
 
+// a value every lane holds alike, moved to scalar registers so that the
+// arithmetic on it runs on the scalar unit
+__device__ __forceinline__ int uni32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+// bit `lane` of a wave-uniform 64-bit mask, used directly as the lane predicate
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 template <int PREC, bool SPLIT, bool LJLDS, int NW>
 __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   using R = typename Prec<PREC>::real;
@@ -169,19 +180,20 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   const R *ljt = LJLDS ? s_lj : a.lj;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * NW + wave;
+  const int ci = uni32(blockIdx.x * NW + wave);
   double ev[14];
 #pragma unroll
   for (int k = 0; k < 14; k++) ev[k] = 0.0;
   if (ci < a.ncl) {
     const int ii = lane & 7, jj = lane >> 3;
+    const unsigned ujj = (unsigned)jj;
     const R4 xi = a.xq[ci * PGK_CL + ii];
     const int ti = a.ct[ci * PGK_CL + ii];   // 0-based type (0 for padding)
     const int nlast = a.nspl_lds - 1;
     double fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
-    const int e0 = a.ci_start[ci], e1 = a.ci_start[ci + 1];
+    const int e0 = uni32(a.ci_start[ci]), e1 = uni32(a.ci_start[ci + 1]);
     // j-cluster data of entry e+1 is loaded while entry e is computed; the
     // index is clamped so the loads stay in bounds on the last entry
     int cj_n = 0;
@@ -189,10 +201,10 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
     R4 xj_n;
     int tj_n = 0;
     if (e0 < e1) {
-      cj_n = a.cj[e0];
-      m_n = a.mask[e0];
-      xj_n = a.xq[cj_n * PGK_CL + jj];
-      tj_n = a.ct[cj_n * PGK_CL + jj];
+      cj_n = uni32(a.cj[e0]);
+      m_n = uni64(a.mask[e0]);
+      xj_n = (a.xq + (size_t)cj_n * PGK_CL)[ujj];
+      tj_n = (a.ct + (size_t)cj_n * PGK_CL)[ujj];
     }
 
     for (int e = e0; e < e1; e++) {
@@ -202,10 +214,11 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       const int tj = tj_n;
       {
         const int en = min(e + 1, e1 - 1);
-        cj_n = a.cj[en];
-        m_n = a.mask[en];
-        xj_n = a.xq[cj_n * PGK_CL + jj];
-        tj_n = a.ct[cj_n * PGK_CL + jj];
+        cj_n = uni32(a.cj[en]);
+        m_n = uni64(a.mask[en]);
+        // scalar base of the j-cluster plus the lane's fixed slot offset
+        xj_n = (a.xq + (size_t)cj_n * PGK_CL)[ujj];
+        tj_n = (a.ct + (size_t)cj_n * PGK_CL)[ujj];
       }
       // every lane runs the same instructions; a lane whose pair is not
       // listed or lies outside a cutoff contributes exact zeros
@@ -214,7 +227,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       ry = ry - a.L[1] * nimg(ry * a.Linv[1]);
       rz = rz - a.L[2] * nimg(rz * a.Linv[2]);
       const R r2 = r2_nc(rx, ry, rz);
-      const bool in = ((m >> lane) & 1ull) && (r2 <= a.sqcut);
+      const bool in = lane_bit(m) && (r2 <= a.sqcut);
       const R r2s = in ? r2 : R(1);
       R rinv2, r;
       rinv_pair<R>(r2s, rinv2, r);
@@ -231,7 +244,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       e_elc += se * qq;
       // Lennard-Jones (eps is 0 unless the type pair interacts by VDW)
       const bool ljon = in && (r2 <= a.sqcut2);
-      const int tp = ljon ? (ti + a.ntype * tj) : 0;
+      const int tp = ti + __mul24(a.ntype, tj);
       const R eps = ljon ? ljt[2 * tp] : R(0);
       const R rm2 = ljt[2 * tp + 1];
       const R t2 = rinv2 * rm2;
@@ -260,10 +273,15 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       const R g = sum3x8(fx, fy, fz, ii);
       R gc = R(0);
       if (SPLIT) gc = sum3x8(ftc * rx, ftc * ry, ftc * rz, ii);
-      if (((m >> (8 * jj)) & 0xffull) && !(ii & 1) && ii < 6) {
-        const size_t s = (size_t)(cj * PGK_CL + jj) * 3 + (ii >> 1);
-        acc_add_j(&a.fcl[s], -g);
-        if (SPLIT) acc_add_j(&a.fcl_c[s], -gc);
+      // lanes ii = 0, 2, 4 of every j-slot that has a listed pair: bit 8 jj of
+      // mb is the OR of byte jj of m
+      uint64_t mb = m | (m >> 4);
+      mb |= mb >> 2;
+      mb |= mb >> 1;
+      if (lane_bit((mb & 0x0101010101010101ull) * 0x15ull)) {
+        const unsigned s = (unsigned)(jj * 3 + (ii >> 1));
+        acc_add_j(&(a.fcl + (size_t)cj * (3 * PGK_CL))[s], -g);
+        if (SPLIT) acc_add_j(&(a.fcl_c + (size_t)cj * (3 * PGK_CL))[s], -gc);
       }
     }
     // i side: sum over the 8 j-lanes that share this i-slot

@@ -24,6 +24,7 @@
 // round trip through HBM.
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
+#include <type_traits>
 
 #include "pgk_device.h"
 
@@ -136,15 +137,15 @@ __device__ __forceinline__ int fdiv(int t, int d, float rd) {
 // One Stockham pass of radix R over nl lines (compile-time R keeps the
 // butterfly operands in registers).
 template <int R, class C>
-__device__ __forceinline__ void stockham_pass(const C *pa, C *pb, int nl, int n, int Ns,
+__device__ __forceinline__ void stockham_pass(const C *pa, C *pb, int nl, int n, int ls, int Ns,
                                               const C *tw, int sgn) {
   const int nb = n / R;
   const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
   const int tstride = n / (Ns * R);
   for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
     const int line = fdiv(t, nb, rnb), j = t - line * nb;
-    const C *src = pa + line * n;
-    C *dst = pb + line * n;
+    const C *src = pa + line * ls;
+    C *dst = pb + line * ls;
     const int jq = fdiv(j, Ns, rns), k = j - jq * Ns;
     C v[R];
     v[0] = src[j];
@@ -162,19 +163,21 @@ __device__ __forceinline__ void stockham_pass(const C *pa, C *pb, int nl, int n,
   }
 }
 
-// Batched Stockham FFT of nl lines of length p.n held in LDS (line stride n).
-// Result ends in *pa (the buffers are swapped pass by pass).
+// Batched Stockham FFT of nl lines of length p.n held in LDS with line
+// stride ls (n + 1 where lines are filled column-wise, so that lanes walking
+// across lines hit different banks).  Result ends in *pa (the buffers are
+// swapped pass by pass).
 template <class C>
-__device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *tw, int sgn) {
+__device__ void fft_lds(C *&pa, C *&pb, int nl, int ls, const FftPlan &p, const C *tw, int sgn) {
   const int n = p.n;
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int R = p.rad[s];
     switch (R) {
-      case 4: stockham_pass<4>(pa, pb, nl, n, Ns, tw, sgn); break;
-      case 2: stockham_pass<2>(pa, pb, nl, n, Ns, tw, sgn); break;
-      case 3: stockham_pass<3>(pa, pb, nl, n, Ns, tw, sgn); break;
-      default: stockham_pass<5>(pa, pb, nl, n, Ns, tw, sgn); break;
+      case 4: stockham_pass<4>(pa, pb, nl, n, ls, Ns, tw, sgn); break;
+      case 2: stockham_pass<2>(pa, pb, nl, n, ls, Ns, tw, sgn); break;
+      case 3: stockham_pass<3>(pa, pb, nl, n, ls, Ns, tw, sgn); break;
+      default: stockham_pass<5>(pa, pb, nl, n, ls, Ns, tw, sgn); break;
     }
     __syncthreads();
     C *t = pa;
@@ -184,6 +187,96 @@ __device__ void fft_lds(C *&pa, C *&pb, int nl, const FftPlan &p, const C *tw, i
   }
 }
 
+
+// ---- register FFT for line lengths N = N1 * N2 in {32, 64, 128, 256} ----
+// N2 lanes share a line; lane t starts with elements t + N2 j (j < N1),
+// transforms them with an in-register N1-point DFT, applies the twiddles
+// W_N^(t k1), swaps through LDS and finishes with N1/N2 in-register N2-point
+// DFTs.  It ends holding elements (t + N2 q) + N1 k2.  One wavefront holds
+// 64/N2 lines and needs no barrier wider than itself.
+
+// cos and sin of 2 pi k / 16, k < 8
+__host__ __device__ constexpr double cos16(int k) {
+  return k == 0 ? 1.0
+       : k == 1 ? 0.92387953251128675613
+       : k == 2 ? 0.70710678118654752440
+       : k == 3 ? 0.38268343236508977173
+       : k == 4 ? 0.0
+       : k == 5 ? -0.38268343236508977173
+       : k == 6 ? -0.70710678118654752440
+                : -0.92387953251128675613;
+}
+__host__ __device__ constexpr double sin16(int k) { return k <= 4 ? cos16(4 - k) : cos16(k - 4); }
+
+// a * exp(SGN 2 pi i k / M), M <= 16
+template <int M, int SGN, class C>
+__device__ __forceinline__ C twc(C a, int k) {
+  using T = decltype(a.x);
+  const int idx = k * (16 / M);
+  if (idx == 0) return a;
+  if (idx == 4) return cmuli(a, SGN);
+  const T c = (T)cos16(idx), sn = (T)(SGN * sin16(idx));
+  C r;
+  r.x = a.x * c - a.y * sn;
+  r.y = a.x * sn + a.y * c;
+  return r;
+}
+
+// in-register M-point DFT (radix-2, decimation in time), natural order
+template <int M, int SGN, class C>
+__device__ __forceinline__ void dft_r(C *v) {
+  if constexpr (M > 1) {
+    C e[M / 2], o[M / 2];
+#pragma unroll
+    for (int i = 0; i < M / 2; i++) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    dft_r<M / 2, SGN>(e);
+    dft_r<M / 2, SGN>(o);
+#pragma unroll
+    for (int k = 0; k < M / 2; k++) {
+      const C tk = twc<M, SGN>(o[k], k);
+      v[k] = cadd(e[k], tk);
+      v[k + M / 2] = csub(e[k], tk);
+    }
+  }
+}
+
+template <int N1_, int N2_>
+struct RegFft {
+  static constexpr int N1 = N1_, N2 = N2_, N = N1_ * N2_;
+  static constexpr int LPW = 64 / N2_;          // lines per wavefront
+  static constexpr int XS = 65;                 // exchange row stride (complex)
+  static constexpr int NS = N + N2_;            // natural-order line stride (complex)
+  static constexpr int BUF = (N1_ * XS > LPW * NS) ? N1_ * XS : LPW * NS;
+};
+
+// tw: W_N^m = exp(-2 pi i m / N), m < N, in LDS; ex: the wave's exchange
+// buffer (RegFft::BUF complex)
+template <class F, int SGN, class C>
+__device__ __forceinline__ void fft_reg(C (&x)[F::N1], C *ex, const C *tw, int t, int l) {
+  dft_r<F::N1, SGN>(x);
+#pragma unroll
+  for (int k1 = 1; k1 < F::N1; k1++) {
+    C w = tw[t * k1];
+    if (SGN > 0) w.y = -w.y;
+    x[k1] = cmul(x[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < F::N1; k1++) ex[k1 * F::XS + l * F::N2 + t] = x[k1];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < F::N1 / F::N2; q++) {
+    C v[F::N2];
+#pragma unroll
+    for (int t2 = 0; t2 < F::N2; t2++) v[t2] = ex[(t + F::N2 * q) * F::XS + l * F::N2 + t2];
+    dft_r<F::N2, SGN>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < F::N2; k2++) x[q * F::N2 + k2] = v[k2];
+  }
+  __syncthreads();
+}
 
 // B-spline weights of order P at w (fill_bspline, fft_pme_init.F90:216-260)
 template <class R, int P>
@@ -546,11 +639,41 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
 }
 
 
+// Global -> LDS staging: each thread issues PGK_STAGE loads before the first
+// LDS store, so their memory latencies overlap instead of adding up.
+#define PGK_STAGE 8
+template <class V, class Ld, class St>
+__device__ __forceinline__ void stage_in(int total, Ld ld, St st) {
+  for (int t0 = threadIdx.x; t0 < total; t0 += PGK_STAGE * blockDim.x) {
+    V v[PGK_STAGE];
+#pragma unroll
+    for (int u = 0; u < PGK_STAGE; u++) {
+      const int t = t0 + u * blockDim.x;
+      if (t < total) v[u] = ld(t);
+    }
+#pragma unroll
+    for (int u = 0; u < PGK_STAGE; u++) {
+      const int t = t0 + u * blockDim.x;
+      if (t < total) st(t, v[u]);
+    }
+  }
+}
+
+// The half spectrum is stored [k3][h][k2] (k2 fastest): row r = k3 K2 + k2 of
+// the real grid puts its mode h at cidx(r, h).  Lines along k2 are then
+// contiguous, and lines along k3 and the x pass's row pairs read whole
+// cache lines.
+__device__ __forceinline__ size_t cidx(int r, int h, int H, int K2) {
+  const int k3 = r / K2;
+  return ((size_t)k3 * H + h) * K2 + (r - k3 * K2);
+}
+
 // x pass, forward: pairs of real rows -> half spectra.  One block handles NL
 // complex lines (2*NL rows).
 template <int PREC>
 __global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid, int nrow,
-                                                    int NL, FftPlan p, const void *twv, int H) {
+                                                    int NL, FftPlan p, const void *twv, int H,
+                                                    int K2) {
   using R = typename Prec<PREC>::real;
   using C = typename C2T<R>::type;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -561,17 +684,20 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid
   for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   const int row0 = blockIdx.x * 2 * NL;
   const float rn = 1.0f / (float)n;
-  for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-    int l = fdiv(t, n, rn), k = t - l * n;
-    int ra = row0 + 2 * l, rb = ra + 1;
-    C z;
-    const R *gr = reinterpret_cast<const R *>(grid);
-    z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
-    z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
-    pa[t] = z;
-  }
+  const R *gr = reinterpret_cast<const R *>(grid);
+  stage_in<C>(
+      NL * n,
+      [&](int t) {
+        const int l = fdiv(t, n, rn), k = t - l * n;
+        const int ra = row0 + 2 * l, rb = ra + 1;
+        C z;
+        z.x = (ra < nrow) ? gr[(size_t)ra * n + k] : R(0);
+        z.y = (rb < nrow) ? gr[(size_t)rb * n + k] : R(0);
+        return z;
+      },
+      [&](int t, C z) { pa[t] = z; });
   __syncthreads();
-  fft_lds<C>(pa, pb, NL, p, tw, -1);
+  fft_lds<C>(pa, pb, NL, n, p, tw, -1);
   C *cg = reinterpret_cast<C *>(cgrid);
   const float rH = 1.0f / (float)H;
   for (int t = threadIdx.x; t < NL * H; t += blockDim.x) {
@@ -585,15 +711,16 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const void *grid, void *cgrid
     A.y = R(0.5) * (zk.y - zn.y);
     B.x = R(0.5) * (zk.y + zn.y);
     B.y = R(0.5) * (zn.x - zk.x);
-    if (ra < nrow) cg[(size_t)ra * H + k] = A;
-    if (rb < nrow) cg[(size_t)rb * H + k] = B;
+    if (ra < nrow) cg[cidx(ra, k, H, K2)] = A;
+    if (rb < nrow) cg[cidx(rb, k, H, K2)] = B;
   }
 }
 
 // x pass, backward: half spectra of two rows -> two real rows (C2R)
 template <int PREC>
 __global__ __launch_bounds__(256) void k_fft_x_bwd(const void *cgrid, void *grid, int nrow,
-                                                    int NL, FftPlan p, const void *twv, int H) {
+                                                    int NL, FftPlan p, const void *twv, int H,
+                                                    int K2) {
   using R = typename Prec<PREC>::real;
   using C = typename C2T<R>::type;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -605,27 +732,29 @@ __global__ __launch_bounds__(256) void k_fft_x_bwd(const void *cgrid, void *grid
   const C *cg = reinterpret_cast<const C *>(cgrid);
   const int row0 = blockIdx.x * 2 * NL;
   const float rn = 1.0f / (float)n;
-  for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-    int l = fdiv(t, n, rn), k = t - l * n;
-    int ra = row0 + 2 * l, rb = ra + 1;
-    C A, B;
-    A.x = A.y = B.x = B.y = R(0);
-    int kk = (k < H) ? k : n - k;
-    bool conj = (k >= H);
-    if (ra < nrow) A = cg[(size_t)ra * H + kk];
-    if (rb < nrow) B = cg[(size_t)rb * H + kk];
-    if (conj) {
-      A.y = -A.y;
-      B.y = -B.y;
-    }
-    // Z = A + i B
-    C z;
-    z.x = A.x - B.y;
-    z.y = A.y + B.x;
-    pa[t] = z;
-  }
+  stage_in<C>(
+      NL * n,
+      [&](int t) {
+        const int l = fdiv(t, n, rn), k = t - l * n;
+        const int ra = row0 + 2 * l, rb = ra + 1;
+        C A, B;
+        A.x = A.y = B.x = B.y = R(0);
+        const int kk = (k < H) ? k : n - k;
+        if (ra < nrow) A = cg[cidx(ra, kk, H, K2)];
+        if (rb < nrow) B = cg[cidx(rb, kk, H, K2)];
+        if (k >= H) {
+          A.y = -A.y;
+          B.y = -B.y;
+        }
+        // Z = A + i B
+        C z;
+        z.x = A.x - B.y;
+        z.y = A.y + B.x;
+        return z;
+      },
+      [&](int t, C z) { pa[t] = z; });
   __syncthreads();
-  fft_lds<C>(pa, pb, NL, p, tw, +1);
+  fft_lds<C>(pa, pb, NL, n, p, tw, +1);
   R *gr = reinterpret_cast<R *>(grid);
   for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
     int l = fdiv(t, n, rn), k = t - l * n;
@@ -644,37 +773,321 @@ __global__ __launch_bounds__(256) void k_fft_y(void *cgrid, int H, int K2, int T
   using C = typename C2T<R>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = p.n;   // == K2
+  const int ls = n + 1;
   C *pa = reinterpret_cast<C *>(smem);
-  C *pb = pa + TW * n;
-  C *tw = pb + TW * n;
+  C *pb = pa + TW * ls;
+  C *tw = pb + TW * ls;
   for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   C *cg = reinterpret_cast<C *>(cgrid);
   const int ntile = (H + TW - 1) / TW;
   const int k3 = blockIdx.x / ntile;
   const int h0 = (blockIdx.x - k3 * ntile) * TW;
   const int lt = __ffs(TW) - 1;   // TW is a power of two
+  stage_in<C>(
+      TW * n,
+      [&](int t) {
+        const int k2 = t >> lt, l = t & (TW - 1);
+        C z;
+        z.x = z.y = R(0);
+        if (h0 + l < H) z = cg[((size_t)k3 * H + h0 + l) * K2 + k2];
+        return z;
+      },
+      [&](int t, C z) { pa[(t & (TW - 1)) * ls + (t >> lt)] = z; });
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, ls, p, tw, sgn);
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
     int k2 = t >> lt, l = t & (TW - 1);
-    C z;
-    z.x = z.y = R(0);
-    if (h0 + l < H) z = cg[((size_t)k3 * K2 + k2) * H + h0 + l];
-    pa[l * n + k2] = z;
+    if (h0 + l < H) cg[((size_t)k3 * H + h0 + l) * K2 + k2] = pa[l * ls + k2];
+  }
+}
+
+// y pass with the register FFT: one wavefront per block, LPW lines (h fastest
+// across lines, then k3), so consecutive lanes read consecutive h
+template <int PREC, class F, int SGN>
+__global__ __launch_bounds__(64) void k_fft_y_reg(void *cgrid, int H, int K3,
+                                                  const void *twv) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  __shared__ C tw[F::N];
+  __shared__ C ex[F::BUF];
+  const int lane = threadIdx.x, t = lane % F::N2, l = lane / F::N2;
+  for (int k = lane; k < F::N; k += 64) tw[k] = reinterpret_cast<const C *>(twv)[k];
+  // line (k3, h) = k3 H + h is contiguous along k2
+  const int line = blockIdx.x * F::LPW + l;
+  const bool valid = line < H * K3;
+  C *cg = reinterpret_cast<C *>(cgrid);
+  const size_t base = (size_t)(valid ? line : 0) * F::N;
+  C x[F::N1];
+#pragma unroll
+  for (int j = 0; j < F::N1; j++) {
+    x[j].x = x[j].y = R(0);
+    if (valid) x[j] = cg[base + t + F::N2 * j];
   }
   __syncthreads();
-  fft_lds<C>(pa, pb, TW, p, tw, sgn);
-  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k2 = t >> lt, l = t & (TW - 1);
-    if (h0 + l < H) cg[((size_t)k3 * K2 + k2) * H + h0 + l] = pa[l * n + k2];
+  fft_reg<F, SGN>(x, ex, tw, t, l);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < F::N1 / F::N2; q++)
+#pragma unroll
+      for (int k2 = 0; k2 < F::N2; k2++)
+        cg[base + t + F::N2 * q + F::N1 * k2] = x[q * F::N2 + k2];
   }
+}
+
+// x pass, forward, register FFT: complex line p packs real rows 2p and 2p+1
+template <int PREC, class F>
+__global__ __launch_bounds__(64) void k_fft_x_fwd_reg(const void *grid, void *cgrid, int nrow,
+                                                      const void *twv, int H, int K2) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  __shared__ C tw[F::N];
+  __shared__ C ex[F::BUF];
+  const int lane = threadIdx.x, t = lane % F::N2, l = lane / F::N2;
+  for (int k = lane; k < F::N; k += 64) tw[k] = reinterpret_cast<const C *>(twv)[k];
+  const int ra = 2 * (blockIdx.x * F::LPW + l), rb = ra + 1;
+  const R *gr = reinterpret_cast<const R *>(grid);
+  C x[F::N1];
+#pragma unroll
+  for (int j = 0; j < F::N1; j++) {
+    const int k = t + F::N2 * j;
+    x[j].x = (ra < nrow) ? gr[(size_t)ra * F::N + k] : R(0);
+    x[j].y = (rb < nrow) ? gr[(size_t)rb * F::N + k] : R(0);
+  }
+  __syncthreads();
+  fft_reg<F, -1>(x, ex, tw, t, l);
+  // natural order, then split the two real rows' spectra
+#pragma unroll
+  for (int q = 0; q < F::N1 / F::N2; q++)
+#pragma unroll
+    for (int k2 = 0; k2 < F::N2; k2++)
+      ex[l * F::NS + t + F::N2 * q + F::N1 * k2] = x[q * F::N2 + k2];
+  __syncthreads();
+  C *cg = reinterpret_cast<C *>(cgrid);
+  for (int k = t; k < H; k += F::N2) {
+    const C zk = ex[l * F::NS + k];
+    const C zn = ex[l * F::NS + ((F::N - k) & (F::N - 1))];
+    // A = (Z_k + conj Z_{n-k})/2,  B = (Z_k - conj Z_{n-k})/(2i)
+    C A, B;
+    A.x = R(0.5) * (zk.x + zn.x);
+    A.y = R(0.5) * (zk.y - zn.y);
+    B.x = R(0.5) * (zk.y + zn.y);
+    B.y = R(0.5) * (zn.x - zk.x);
+    if (ra < nrow) cg[cidx(ra, k, H, K2)] = A;
+    if (rb < nrow) cg[cidx(rb, k, H, K2)] = B;
+  }
+}
+
+// x pass, backward (C2R), register FFT
+template <int PREC, class F>
+__global__ __launch_bounds__(64) void k_fft_x_bwd_reg(const void *cgrid, void *grid, int nrow,
+                                                      const void *twv, int H, int K2) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  __shared__ C tw[F::N];
+  __shared__ C ex[F::BUF];
+  const int lane = threadIdx.x, t = lane % F::N2, l = lane / F::N2;
+  for (int k = lane; k < F::N; k += 64) tw[k] = reinterpret_cast<const C *>(twv)[k];
+  const int ra = 2 * (blockIdx.x * F::LPW + l), rb = ra + 1;
+  const C *cg = reinterpret_cast<const C *>(cgrid);
+  C x[F::N1];
+#pragma unroll
+  for (int j = 0; j < F::N1; j++) {
+    const int k = t + F::N2 * j;
+    const int kk = (k < H) ? k : F::N - k;
+    C A, B;
+    A.x = A.y = B.x = B.y = R(0);
+    if (ra < nrow) A = cg[cidx(ra, kk, H, K2)];
+    if (rb < nrow) B = cg[cidx(rb, kk, H, K2)];
+    if (k >= H) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    // Z = A + i B
+    x[j].x = A.x - B.y;
+    x[j].y = A.y + B.x;
+  }
+  __syncthreads();
+  fft_reg<F, +1>(x, ex, tw, t, l);
+  R *gr = reinterpret_cast<R *>(grid);
+#pragma unroll
+  for (int q = 0; q < F::N1 / F::N2; q++)
+#pragma unroll
+    for (int k2 = 0; k2 < F::N2; k2++) {
+      const int k = t + F::N2 * q + F::N1 * k2;
+      if (ra < nrow) gr[(size_t)ra * F::N + k] = x[q * F::N2 + k2].x;
+      if (rb < nrow) gr[(size_t)rb * F::N + k] = x[q * F::N2 + k2].y;
+    }
 }
 
 struct ConvArgs {
   const double *bsp;    // K1 | K2 | K3 moduli
   double recip[3];
-  double fac, rk2, pivol, netq;
+  double fac, rk2, pivol, rpivol, netq;
   int K[3], H;
   int ifnetq;
 };
+
+// Influence function of mode (h, k2, k3) (scalar_sum, fft_pme.F90:544-690):
+// returns eterm (0 outside |m| <= rkcut) and adds the mode's energy and
+// virial terms, weighted for the half spectrum, to ev.  mh2 = m2/L2 and s2
+// (+1 on the k2 Nyquist plane, else -1) belong to k2; b1 b2 b3 are the
+// B-spline moduli of h, k2, k3.
+__device__ __forceinline__ double conv_mode(const ConvArgs &ca, int h, int k3, double mh2,
+                                            double s2, double b1, double b2, double b3,
+                                            double zx, double zy, double (&ev)[8]) {
+  const int K1 = ca.K[0], K3 = ca.K[2];
+  const int nf1 = (K1 + 1) / 2, nf3 = (K3 + 1) / 2;
+  const int m1 = (h < nf1) ? h : h - K1;
+  const int m3 = (k3 < nf3) ? k3 : k3 - K3;
+  const double mh1 = ca.recip[0] * m1, mh3 = ca.recip[2] * m3;
+  // the rkcut test decides modes that lie on the sphere (e.g. integer
+  // |k|^2 = (rkcut L)^2): |m|^2 in the reference's rounding, no contraction
+  const double msq = msq_nc(mh1, mh2, mh3);
+  double eterm = 0.0;
+  if (msq <= ca.rk2) {
+    const double denom = ca.pivol * b1 * b2 * b3 * msq;
+    const double ex = exp(-ca.fac * msq);
+    eterm = ex / denom;
+    const double vterm = 2.0 * (ca.fac * msq + 1.0) / msq;
+    const double struc2 = zx * zx + zy * zy;
+    const bool single = (h == 0) || ((K1 & 1) == 0 && h == K1 / 2);
+    const double w = single ? 1.0 : 2.0;
+    const double s3 = ((K3 & 1) == 0 && k3 == K3 / 2) ? 1.0 : -1.0;
+    const double w12 = single ? 1.0 : 1.0 - s2;        // s1 = -1
+    const double w13 = single ? 1.0 : 1.0 - s3;
+    const double w23 = single ? 1.0 : 1.0 + s2 * s3;
+    double wgt = eterm * struc2;
+    if (ca.ifnetq) {
+      const double e2 = ex * ca.netq / (ca.pivol * msq);
+      ev[1] += w * (e2 * (-1.0 + 2.0 * ca.fac * msq));
+      wgt = wgt - e2;
+    }
+    ev[0] += w * eterm * struc2;
+    ev[2] += w * wgt * (vterm * mh1 * mh1 - 1.0);
+    ev[3] += w * wgt * (vterm * mh2 * mh2 - 1.0);
+    ev[4] += w * wgt * (vterm * mh3 * mh3 - 1.0);
+    ev[5] += w12 * wgt * (vterm * mh1 * mh2);
+    ev[6] += w13 * wgt * (vterm * mh1 * mh3);
+    ev[7] += w23 * wgt * (vterm * mh2 * mh3);
+  }
+  return eterm;
+}
+
+// The same mode with exp(-fac |m|^2) taken as the product of per-axis factors
+// exp(-fac mh_d^2) and the moduli as reciprocals, so that a mode costs one
+// division.  e12 = exp(-fac (mh1^2 + mh2^2)) as a product, r12 = 1/(pi V b1 b2),
+// e3 = exp(-fac mh3^2), r3 = 1/b3; the cutoff test is the reference's.
+__device__ __forceinline__ double conv_mode_f(const ConvArgs &ca, int h, int k3, double mh1,
+                                              double mh2, double s2, double e12, double r12,
+                                              double e3, double r3, double zx, double zy,
+                                              double (&ev)[8]) {
+  const int K1 = ca.K[0], K3 = ca.K[2];
+  const int nf3 = (K3 + 1) / 2;
+  const int m3 = (k3 < nf3) ? k3 : k3 - K3;
+  const double mh3 = ca.recip[2] * m3;
+  const double msq = msq_nc(mh1, mh2, mh3);
+  double eterm = 0.0;
+  if (msq <= ca.rk2) {
+    const double rm = 1.0 / msq;
+    const double ex = e12 * e3;
+    eterm = ex * (r12 * r3) * rm;
+    const double vterm = 2.0 * (ca.fac * msq + 1.0) * rm;
+    const double struc2 = zx * zx + zy * zy;
+    const bool single = (h == 0) || ((K1 & 1) == 0 && h == K1 / 2);
+    const double w = single ? 1.0 : 2.0;
+    const double s3 = ((K3 & 1) == 0 && k3 == K3 / 2) ? 1.0 : -1.0;
+    const double w12 = single ? 1.0 : 1.0 - s2;        // s1 = -1
+    const double w13 = single ? 1.0 : 1.0 - s3;
+    const double w23 = single ? 1.0 : 1.0 + s2 * s3;
+    double wgt = eterm * struc2;
+    if (ca.ifnetq) {
+      const double e2 = ex * ca.netq * ca.rpivol * rm;
+      ev[1] += w * (e2 * (-1.0 + 2.0 * ca.fac * msq));
+      wgt = wgt - e2;
+    }
+    ev[0] += w * eterm * struc2;
+    ev[2] += w * wgt * (vterm * mh1 * mh1 - 1.0);
+    ev[3] += w * wgt * (vterm * mh2 * mh2 - 1.0);
+    ev[4] += w * wgt * (vterm * mh3 * mh3 - 1.0);
+    ev[5] += w12 * wgt * (vterm * mh1 * mh2);
+    ev[6] += w13 * wgt * (vterm * mh1 * mh3);
+    ev[7] += w23 * wgt * (vterm * mh2 * mh3);
+  }
+  return eterm;
+}
+
+// z pass with the register FFT and the convolution between the forward and
+// the backward transform; LPW lines per wavefront, h fastest across lines
+template <int PREC, class F>
+__global__ __launch_bounds__(64) void k_fft_z_conv_reg(void *cgrid, const void *twv, ConvArgs ca,
+                                                       double *red) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  __shared__ C tw[F::N];
+  __shared__ C ex[F::BUF];
+  __shared__ double se3[F::N], sr3[F::N];
+  const int lane = threadIdx.x, t = lane % F::N2, l = lane / F::N2;
+  const int H = ca.H, K2 = ca.K[1], K1 = ca.K[0];
+  for (int k = lane; k < F::N; k += 64) {
+    tw[k] = reinterpret_cast<const C *>(twv)[k];
+    const int m3 = (k < (F::N + 1) / 2) ? k : k - F::N;
+    const double mh3 = ca.recip[2] * m3;
+    se3[k] = exp(-ca.fac * (mh3 * mh3));
+    sr3[k] = 1.0 / ca.bsp[K1 + K2 + k];
+  }
+  // line (h, k2) = h K2 + k2: neighbouring lanes hold neighbouring k2
+  const int line = blockIdx.x * F::LPW + l;
+  const bool valid = line < H * K2;
+  const int h = valid ? line / K2 : 0, k2 = valid ? line - h * K2 : 0;
+  C *cg = reinterpret_cast<C *>(cgrid);
+  const size_t zstride = (size_t)K2 * H;
+  const size_t base = (size_t)(valid ? line : 0);
+  C x[F::N1];
+#pragma unroll
+  for (int j = 0; j < F::N1; j++) {
+    x[j].x = x[j].y = R(0);
+    if (valid) x[j] = cg[base + (size_t)(t + F::N2 * j) * zstride];
+  }
+  __syncthreads();
+  fft_reg<F, -1>(x, ex, tw, t, l);
+  double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // E, vsca, xx yy zz xy xz yz
+  const int nf1 = (K1 + 1) / 2, nf2 = (K2 + 1) / 2;
+  const int m1 = (h < nf1) ? h : h - K1;
+  const int m2 = (k2 < nf2) ? k2 : k2 - K2;
+  const double mh1 = ca.recip[0] * m1, mh2 = ca.recip[1] * m2;
+  const double s2 = ((K2 & 1) == 0 && k2 == K2 / 2) ? 1.0 : -1.0;
+  const double b1 = valid ? ca.bsp[h] : 1.0, b2 = valid ? ca.bsp[K1 + k2] : 1.0;
+  const double e12 = exp(-ca.fac * (mh1 * mh1)) * exp(-ca.fac * (mh2 * mh2));
+  const double r12 = 1.0 / (ca.pivol * b1 * b2);
+#pragma unroll
+  for (int q = 0; q < F::N1 / F::N2; q++)
+#pragma unroll
+    for (int kk = 0; kk < F::N2; kk++) {
+      const int k3 = t + F::N2 * q + F::N1 * kk;
+      C &z = x[q * F::N2 + kk];
+      if (valid && !(h == 0 && k2 == 0 && k3 == 0)) {   // m = 0 left untouched
+        const double eterm = conv_mode_f(ca, h, k3, mh1, mh2, s2, e12, r12, se3[k3], sr3[k3],
+                                         (double)z.x, (double)z.y, ev);
+        z.x = (R)(eterm * (double)z.x);
+        z.y = (R)(eterm * (double)z.y);
+      }
+      // natural order for the inverse transform's input layout
+      ex[k3 * F::LPW + l] = z;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < F::N1; j++) x[j] = ex[(t + F::N2 * j) * F::LPW + l];
+  __syncthreads();
+  fft_reg<F, +1>(x, ex, tw, t, l);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < F::N1 / F::N2; q++)
+#pragma unroll
+      for (int kk = 0; kk < F::N2; kk++)
+        cg[base + (size_t)(t + F::N2 * q + F::N1 * kk) * zstride] = x[q * F::N2 + kk];
+  }
+  block_record<8, 1>(ev, red + (size_t)blockIdx.x * PGK_REC);
+}
 
 // z pass with the influence-function convolution between the forward and the
 // backward transform (scalar_sum, fft_pme.F90:544-690)
@@ -686,10 +1099,11 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   using C = typename C2T<R>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = p.n;   // == K3
+  const int ls = n + 1;
   const int H = ca.H, K2 = ca.K[1];
   C *pa = reinterpret_cast<C *>(smem);
-  C *pb = pa + TW * n;
-  C *tw = pb + TW * n;
+  C *pb = pa + TW * ls;
+  C *tw = pb + TW * ls;
   for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
   C *cg = reinterpret_cast<C *>(cgrid);
   const int ntile = (H + TW - 1) / TW;
@@ -697,71 +1111,51 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   const int h0 = (blockIdx.x - k2 * ntile) * TW;
   const size_t zstride = (size_t)K2 * H;
   const int lt = __ffs(TW) - 1;   // TW is a power of two
-  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
-    int k3 = t >> lt, l = t & (TW - 1);
-    C z;
-    z.x = z.y = R(0);
-    if (h0 + l < H) z = cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l];
-    pa[l * n + k3] = z;
+  // B-spline moduli of the tile's h, and of every k3 (k2 is fixed)
+  double *sb1 = reinterpret_cast<double *>(tw + n);
+  double *sb3 = sb1 + TW;
+  for (int t = threadIdx.x; t < TW + n; t += blockDim.x) {
+    if (t < TW) sb1[t] = (h0 + t < H) ? ca.bsp[h0 + t] : 1.0;
+    else sb3[t - TW] = ca.bsp[ca.K[0] + K2 + (t - TW)];
   }
+  stage_in<C>(
+      TW * n,
+      [&](int t) {
+        const int k3 = t >> lt, l = t & (TW - 1);
+        C z;
+        z.x = z.y = R(0);
+        if (h0 + l < H) z = cg[(size_t)k3 * zstride + (size_t)(h0 + l) * K2 + k2];
+        return z;
+      },
+      [&](int t, C z) { pa[(t & (TW - 1)) * ls + (t >> lt)] = z; });
   __syncthreads();
-  fft_lds<C>(pa, pb, TW, p, tw, -1);
+  fft_lds<C>(pa, pb, TW, ls, p, tw, -1);
   // convolution, energy and virial
   double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // E, vsca, xx yy zz xy xz yz
-  const int K1 = ca.K[0], K3 = ca.K[2];
-  const int nf1 = (K1 + 1) / 2, nf2 = (K2 + 1) / 2, nf3 = (K3 + 1) / 2;
+  const int K1 = ca.K[0];
+  const int nf2 = (K2 + 1) / 2;
   const int m2 = (k2 < nf2) ? k2 : k2 - K2;
   const double mh2 = ca.recip[1] * m2;
   const double s2 = ((K2 & 1) == 0 && k2 == K2 / 2) ? 1.0 : -1.0;
+  const double bsp2 = ca.bsp[K1 + k2];
   const float rn = 1.0f / (float)n;
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
     int l = fdiv(t, n, rn), k3 = t - l * n;
     int h = h0 + l;
     if (h >= H) continue;
     if (h == 0 && k2 == 0 && k3 == 0) continue;   // m = 0 left untouched
-    int m1 = (h < nf1) ? h : h - K1;
-    int m3 = (k3 < nf3) ? k3 : k3 - K3;
-    double mh1 = ca.recip[0] * m1, mh3 = ca.recip[2] * m3;
-    // the rkcut test decides modes that lie on the sphere (e.g. integer
-    // |k|^2 = (rkcut L)^2): |m|^2 in the reference's rounding, no contraction
-    double msq = msq_nc(mh1, mh2, mh3);
-    C z = pa[t];
-    double eterm = 0.0;
-    if (msq <= ca.rk2) {
-      double denom = ca.pivol * ca.bsp[h] * ca.bsp[K1 + k2] * ca.bsp[K1 + K2 + k3] * msq;
-      double ex = exp(-ca.fac * msq);
-      eterm = ex / denom;
-      double vterm = 2.0 * (ca.fac * msq + 1.0) / msq;
-      double struc2 = (double)z.x * (double)z.x + (double)z.y * (double)z.y;
-      bool single = (h == 0) || ((K1 & 1) == 0 && h == K1 / 2);
-      double w = single ? 1.0 : 2.0;
-      double s3 = ((K3 & 1) == 0 && k3 == K3 / 2) ? 1.0 : -1.0;
-      double w12 = single ? 1.0 : 1.0 - s2;        // s1 = -1
-      double w13 = single ? 1.0 : 1.0 - s3;
-      double w23 = single ? 1.0 : 1.0 + s2 * s3;
-      double wgt = eterm * struc2;
-      if (ca.ifnetq) {
-        double e2 = ex * ca.netq / (ca.pivol * msq);
-        ev[1] += w * (e2 * (-1.0 + 2.0 * ca.fac * msq));
-        wgt = wgt - e2;
-      }
-      ev[0] += w * eterm * struc2;
-      ev[2] += w * wgt * (vterm * mh1 * mh1 - 1.0);
-      ev[3] += w * wgt * (vterm * mh2 * mh2 - 1.0);
-      ev[4] += w * wgt * (vterm * mh3 * mh3 - 1.0);
-      ev[5] += w12 * wgt * (vterm * mh1 * mh2);
-      ev[6] += w13 * wgt * (vterm * mh1 * mh3);
-      ev[7] += w23 * wgt * (vterm * mh2 * mh3);
-    }
+    C z = pa[l * ls + k3];
+    const double eterm = conv_mode(ca, h, k3, mh2, s2, sb1[l], bsp2, sb3[k3], (double)z.x,
+                                   (double)z.y, ev);
     z.x = (R)(eterm * (double)z.x);
     z.y = (R)(eterm * (double)z.y);
-    pa[t] = z;
+    pa[l * ls + k3] = z;
   }
   __syncthreads();
-  fft_lds<C>(pa, pb, TW, p, tw, +1);
+  fft_lds<C>(pa, pb, TW, ls, p, tw, +1);
   for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
     int k3 = t >> lt, l = t & (TW - 1);
-    if (h0 + l < H) cg[(size_t)k3 * zstride + (size_t)k2 * H + h0 + l] = pa[l * n + k3];
+    if (h0 + l < H) cg[(size_t)k3 * zstride + (size_t)(h0 + l) * K2 + k2] = pa[l * ls + k3];
   }
   block_record<8, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
 }
@@ -779,11 +1173,25 @@ FftPlan make_plan(int n) {
   return p;
 }
 
+// calls fn(RegFft<N1, N2>{}) for line lengths with a register FFT
+template <class Fn>
+bool with_reg_fft(int n, Fn fn) {
+  switch (n) {
+    case 32: fn(RegFft<8, 4>{}); return true;
+    case 64: fn(RegFft<8, 8>{}); return true;
+    case 128: fn(RegFft<16, 8>{}); return true;
+    case 256: fn(RegFft<16, 16>{}); return true;
+    default: return false;
+  }
+}
+
+// lines per block: two line buffers of stride n + 1 and the twiddles fit in 64 KiB
 int lines_for(int n, size_t csize, int maxl) {
   int nl = maxl;
-  while (nl > 1 && (size_t)(2 * nl + 1) * n * csize > 65536) nl >>= 1;
+  while (nl > 1 && (size_t)(2 * nl * (n + 1) + n) * csize > 65536) nl >>= 1;
   return nl;
 }
+size_t fft_lds_bytes(int nl, int n, size_t csize) { return (size_t)(2 * nl * (n + 1) + n) * csize; }
 
 // Device layout of the per-slot PME data: slots are the list's spatial order
 // when a list exists (ncl*8 with padding), else atom order.
@@ -929,16 +1337,6 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   const C *twc = reinterpret_cast<const C *>(tw);
   FftPlan p1 = make_plan(K1), p2 = make_plan(K2), p3 = make_plan(K3);
   int nrow = K2 * K3;
-  int nlx = lines_for(K1, csz, 16);
-  int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
-  k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow, nlx, p1,
-                                                           twc + c->tw_off[0], H);
-  HIP_CHECK(hipGetLastError());
-  int twy = lines_for(K2, csz, 16);
-  int by = K3 * ((H + twy - 1) / twy);
-  k_fft_y<PREC><<<by, 256, (2 * twy + 1) * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
-                                                       twc + c->tw_off[1], -1);
-  HIP_CHECK(hipGetLastError());
   ConvArgs ca;
   ca.bsp = c->d_bsp;
   for (int d = 0; d < 3; d++) {
@@ -949,23 +1347,67 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   ca.fac = pi * pi / (c->alpha * c->alpha);
   ca.rk2 = c->rkcut * c->rkcut;
   ca.pivol = pi * c->box[0] * c->box[1] * c->box[2];
+  ca.rpivol = 1.0 / ca.pivol;
   ca.netq = c->netchrgsq;
   ca.H = H;
   ca.ifnetq = c->ifnetq;
-  int twz = lines_for(K3, csz, 16);
-  int bz = K2 * ((H + twz - 1) / twz);
-  *off = c->red_used;
-  *nrec = bz;
-  double *red = c->d_red + (size_t)c->red_used * PGK_REC;
-  c->red_used += bz;
-  k_fft_z_conv<PREC><<<bz, 256, (2 * twz + 1) * K3 * csz, st>>>(c->d_cgrid, twz, p3,
-                                                            twc + c->tw_off[2], ca, red);
+  // register FFT where the line length has one (32, 64, 128, 256), batched
+  // Stockham in LDS otherwise; each axis chooses on its own
+  const void *tw1 = twc + c->tw_off[0], *tw2 = twc + c->tw_off[1], *tw3 = twc + c->tw_off[2];
+  const int nlx = lines_for(K1, csz, 16);
+  const int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
+  const bool rx = with_reg_fft(K1, [&](auto f) {
+    using F = decltype(f);
+    const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+    k_fft_x_fwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_grid, c->d_cgrid, nrow, tw1, H, K2);
+  });
+  if (!rx)
+    k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow,
+                                                                 nlx, p1, tw1, H, K2);
   HIP_CHECK(hipGetLastError());
-  k_fft_y<PREC><<<by, 256, (2 * twy + 1) * K2 * csz, st>>>(c->d_cgrid, H, K2, twy, p2,
-                                                       twc + c->tw_off[1], +1);
+  const int twy = lines_for(K2, csz, 16);
+  const int by = K3 * ((H + twy - 1) / twy);
+  auto fft_y = [&](auto sgn) {
+    constexpr int SGN = decltype(sgn)::value;
+    const bool ry = with_reg_fft(K2, [&](auto f) {
+      using F = decltype(f);
+      const int nb = (H * K3 + F::LPW - 1) / F::LPW;
+      k_fft_y_reg<PREC, F, SGN><<<nb, 64, 0, st>>>(c->d_cgrid, H, K3, tw2);
+    });
+    if (!ry)
+      k_fft_y<PREC><<<by, 256, fft_lds_bytes(twy, K2, csz), st>>>(c->d_cgrid, H, K2, twy, p2,
+                                                                  tw2, SGN);
+  };
+  fft_y(std::integral_constant<int, -1>{});
+  HIP_CHECK(hipGetLastError());
+  *off = c->red_used;
+  double *red = c->d_red + (size_t)c->red_used * PGK_REC;
+  const bool rz = with_reg_fft(K3, [&](auto f) {
+    using F = decltype(f);
+    const int nb = (H * K2 + F::LPW - 1) / F::LPW;
+    *nrec = nb;
+    k_fft_z_conv_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_cgrid, tw3, ca, red);
+  });
+  if (!rz) {
+    const int twz = lines_for(K3, csz, 16);
+    const int bz = K2 * ((H + twz - 1) / twz);
+    *nrec = bz;
+    k_fft_z_conv<PREC><<<bz, 256, fft_lds_bytes(twz, K3, csz) + (twz + K3) * sizeof(double),
+                         st>>>(c->d_cgrid, twz, p3, tw3, ca, red);
+  }
+  c->red_used += *nrec;
+  HIP_CHECK(hipGetLastError());
+  fft_y(std::integral_constant<int, +1>{});
   // potential grid back into the real grid buffer (compute type)
-  k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_pgrid, nrow, nlx, p1,
-                                                           twc + c->tw_off[0], H);
+  if (!rx)
+    k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_pgrid, nrow,
+                                                                 nlx, p1, tw1, H, K2);
+  else
+    with_reg_fft(K1, [&](auto f) {
+      using F = decltype(f);
+      const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+      k_fft_x_bwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_cgrid, c->d_pgrid, nrow, tw1, H, K2);
+    });
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_FFT);
   pgk_prof_begin(c, PGK_PROF_GATHER);
