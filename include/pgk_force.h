@@ -251,8 +251,9 @@ int pgk_energies(pgk_ctx *ctx, double *ene, double *virt);
 int pgk_synchronize(pgk_ctx *ctx);
 
 /* ---- device-resident NVE step (no reference counterpart in the force path:
- * the integrator of moldyn.F90:1465-1771 with the RATTLE of rattle.F90:6-272,
- * for drivers that keep the whole step on the GPU) ------------------------- */
+ * the integrator of moldyn.F90:1465-1771 with the RATTLE of rattle.F90:6-272
+ * (rattle_c) and rattle_mtk.F90:278-542 (rattlep_v), for drivers that keep the
+ * whole step on the GPU) --------------------------------------------------- */
 typedef struct pgk_md_desc {
   const double *mass;          /* [natom] amu                                  */
   int32_t ncons;               /* rigid bonds                                  */
@@ -261,7 +262,8 @@ typedef struct pgk_md_desc {
   int32_t ngroup;              /* coupled constraint groups (molecules)        */
   const int32_t *group_start;  /* [ngroup+1] 0-based offsets into the list    */
   double dt;                   /* time step (fs)                               */
-  double tol;                  /* relative constraint tolerance                */
+  double tol;                  /* eps_rattle: rattle_c stops at |r^2-d^2|/d^2  *
+                                * < tol, rattlep_v at (r.v)^2/r^2 v^2 < tol^2 */
 } pgk_md_desc;
 int pgk_md_setup(pgk_ctx *ctx, const pgk_md_desc *md);
 /* Velocities [3*natom] (A/fs); the constraint velocity projection is applied. */

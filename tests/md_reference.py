@@ -1,7 +1,9 @@
-"""NumPy velocity Verlet + RATTLE (Andersen 1983) driven by the oracle's forces:
-the checker of the device-resident NVE step.  Same units as thrid_amd.md
-(amu, A, fs, eref); constraint groups iterated until every |r|^2 is within
-tol * d^2 and every |r.v_ij| within tol * d^2."""
+"""NumPy velocity Verlet + RATTLE driven by the oracle's forces: the checker
+of the device-resident NVE step.  Same units as thrid_amd.md (amu, A, fs,
+eref).  The constraint stages restate the reference's: positions as rattle_c
+(rattle.F90:180-226), velocities as rattlep_v (rattle_mtk.F90:455-490), each
+group iterated in the given constraint order until an iteration moves nothing.
+Python floats keep the reference's left-to-right, unfused evaluation."""
 import numpy as np
 
 from thrid_amd import abi
@@ -12,43 +14,57 @@ def _groups(cons_ij, group_start):
     return [np.arange(group_start[g], group_start[g + 1]) for g in range(len(group_start) - 1)]
 
 
-def shake(pos, vel, old, minv, cons_ij, d2, groups, dt, tol=1e-10, maxit=500):
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def shake(pos, vel, old, minv, cons_ij, d2, groups, dt, eps=1e-10, maxit=500):
     for ks in groups:
         for _ in range(maxit):
             done = True
             for k in ks:
                 i, j = cons_ij[k, 0] - 1, cons_ij[k, 1] - 1
-                r = pos[i] - pos[j]
-                ro = old[i] - old[j]
-                diff = d2[k] - r @ r
-                if abs(diff) <= 2 * tol * d2[k]:
+                r = [pos[i, c] - pos[j, c] for c in range(3)]
+                rr2 = _dot(r, r)
+                if abs(rr2 - d2[k]) / d2[k] < eps:
                     continue
                 done = False
-                g = diff / (2 * (minv[i] + minv[j]) * (r @ ro))
-                pos[i] += g * minv[i] * ro
-                pos[j] -= g * minv[j] * ro
-                vel[i] += g * minv[i] * ro / dt
-                vel[j] -= g * minv[j] * ro / dt
+                ro = [old[i, c] - old[j, c] for c in range(3)]
+                g = (rr2 - d2[k]) / (2.0 * _dot(r, ro) * (minv[i] + minv[j]) * dt)
+                for c in range(3):
+                    gr = g * ro[c]
+                    pos[i, c] = pos[i, c] - minv[i] * gr * dt
+                    pos[j, c] = pos[j, c] + minv[j] * gr * dt
+                    vel[i, c] = vel[i, c] - minv[i] * gr
+                    vel[j, c] = vel[j, c] + minv[j] * gr
             if done:
                 break
+        else:
+            raise RuntimeError("rattle_c did not converge")
 
 
-def rattle_v(pos, vel, minv, cons_ij, d2, groups, tol=1e-10, maxit=500):
+def rattle_v(pos, vel, minv, cons_ij, d2, groups, eps=1e-10, maxit=500):
+    eps2 = eps * eps
     for ks in groups:
         for _ in range(maxit):
             done = True
             for k in ks:
                 i, j = cons_ij[k, 0] - 1, cons_ij[k, 1] - 1
-                r = pos[i] - pos[j]
-                rv = r @ (vel[i] - vel[j])
-                if abs(rv) <= tol * d2[k]:
+                r = [pos[i, c] - pos[j, c] for c in range(3)]
+                v = [vel[i, c] - vel[j, c] for c in range(3)]
+                rv = _dot(r, v)
+                if rv * rv / _dot(r, r) * _dot(v, v) < eps2:
                     continue
                 done = False
                 kk = rv / (d2[k] * (minv[i] + minv[j]))
-                vel[i] -= kk * minv[i] * r
-                vel[j] += kk * minv[j] * r
+                for c in range(3):
+                    kr = kk * r[c]
+                    vel[i, c] = vel[i, c] - minv[i] * kr
+                    vel[j, c] = vel[j, c] + minv[j] * kr
             if done:
                 break
+        else:
+            raise RuntimeError("rattlep_v did not converge")
 
 
 def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep):

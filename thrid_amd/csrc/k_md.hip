@@ -10,8 +10,12 @@
 // = 6.9511042e-21 J / 1.66053907e-17 J.
 //
 // Constraints are grouped by molecule (CSR over the constraint list); one
-// thread iterates a group's constraints until every |r|^2 is within
-// tol * d^2 (positions) or |r.v| <= tol * d^2 / dt-scale (velocities).
+// thread iterates a group's constraints in the host's order until an
+// iteration moves nothing, with the reference's updates and tests:
+// positions as rattle_c (rattle.F90:180-226), velocities as rattlep_v
+// (rattle_mtk.F90:455-490), evaluated without contraction like the
+// reference's build; divisions by quantities that stay constant through the
+// iteration are multiplications by reciprocals taken once.
 #include "pgk_device.h"
 
 #define PGK_MD_CONV (6.9511042e-21 / 1.66053907e-17)
@@ -52,15 +56,15 @@ struct ConsArgs {
   const double *minv;      // [natom]
   double *pos, *vel;
   const double *pos_old;
-  double dt, tol;
+  double dt, tol;            // tol: eps_rattle, a relative tolerance on |r|^2
   int maxit;
   int32_t *fail;           // set when a group does not converge
 };
 
 // Triangle groups (three atoms held by three constraints: the rigid 3-site
-// water of the reference's topologies) are iterated in registers with the
-// constraints reordered at set-up to (0,1), (0,2), (1,2) of the group's
-// atoms; any other group iterates on global memory.
+// water of the reference's topologies) are iterated in registers; set-up
+// labels their atoms so that the host's constraint order reads (0,1), (0,2),
+// (1,2).  Any other group iterates on global memory.
 __device__ __forceinline__ double3 ld3(const double *p, int i) {
   return make_double3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
@@ -69,50 +73,64 @@ __device__ __forceinline__ void st3(double *p, int i, const double3 &v) {
   p[3 * i + 1] = v.y;
   p[3 * i + 2] = v.z;
 }
-__device__ __forceinline__ double dot(const double3 &a, const double3 &b) {
+__device__ __forceinline__ double dot_nc(const double3 &a, const double3 &b) {
+#pragma clang fp contract(off)
   return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 __device__ __forceinline__ double3 sub(const double3 &a, const double3 &b) {
   return make_double3(a.x - b.x, a.y - b.y, a.z - b.z);
 }
-__device__ __forceinline__ void axpy(double3 &x, double s, const double3 &v) {
-  x.x += s * v.x;
-  x.y += s * v.y;
-  x.z += s * v.z;
-}
 
-// one SHAKE update of constraint (i, j) with squared length d2
+// One rattle_c update of constraint (i, j) (rattle.F90:180-226):
+//   delta = |Rij^2 - d^2| / d^2, nothing to do when delta < eps;
+//   g = (Rij^2 - d^2) / (2 RR (1/mi + 1/mj) dt), RR = Rij . Rij_old;
+//   x_i -= (1/mi g Rij_old) dt, x_j += (1/mj g Rij_old) dt,
+//   v_i -= 1/mi g Rij_old,      v_j += 1/mj g Rij_old.
+// rd2 = 1/d^2: the division by the constant d^2 is a multiplication by its
+// reciprocal (within one rounding of the reference's quotient), so each
+// update keeps a single division.  Returns true when the constraint is
+// already satisfied.
 __device__ __forceinline__ bool shake_one(double3 &pi, double3 &pj, double3 &vi, double3 &vj,
-                                          const double3 &oi, const double3 &oj, double mi,
-                                          double mj, double d2, double tol, double dt) {
+                                          const double3 &ro, double mi, double mj, double d2,
+                                          double rd2, double eps, double dt) {
+#pragma clang fp contract(off)
   const double3 r = sub(pi, pj);
-  const double diff = d2 - dot(r, r);
-  if (fabs(diff) <= 2.0 * tol * d2) return true;
-  const double3 ro = sub(oi, oj);
-  const double gg = diff / (2.0 * (mi + mj) * dot(r, ro));
-  axpy(pi, gg * mi, ro);
-  axpy(pj, -gg * mj, ro);
-  axpy(vi, gg * mi / dt, ro);
-  axpy(vj, -gg * mj / dt, ro);
-  return false;
+  const double rr2 = dot_nc(r, r);
+  const bool ok = fabs(rr2 - d2) * rd2 < eps;
+  // branch-free: a satisfied constraint applies g = 0, which leaves every
+  // coordinate as it is
+  const double gq = (rr2 - d2) / (2.0 * dot_nc(r, ro) * (mi + mj) * dt);
+  const double g = ok ? 0.0 : gq;
+  const double3 gr = make_double3(g * ro.x, g * ro.y, g * ro.z);
+  pi = make_double3(pi.x - mi * gr.x * dt, pi.y - mi * gr.y * dt, pi.z - mi * gr.z * dt);
+  pj = make_double3(pj.x + mj * gr.x * dt, pj.y + mj * gr.y * dt, pj.z + mj * gr.z * dt);
+  vi = make_double3(vi.x - mi * gr.x, vi.y - mi * gr.y, vi.z - mi * gr.z);
+  vj = make_double3(vj.x + mj * gr.x, vj.y + mj * gr.y, vj.z + mj * gr.z);
+  return ok;
 }
 
-// one RATTLE velocity update of constraint (i, j)
-__device__ __forceinline__ bool rattle_one(const double3 &pi, const double3 &pj, double3 &vi,
-                                           double3 &vj, double mi, double mj, double d2,
-                                           double tol) {
-  const double3 r = sub(pi, pj);
-  const double rv = dot(r, sub(vi, vj));
-  if (fabs(rv) <= tol * d2) return true;
-  const double kk = rv / (d2 * (mi + mj));
-  axpy(vi, -kk * mi, r);
-  axpy(vj, kk * mj, r);
-  return false;
+// One rattlep_v update of constraint (i, j) (rattle_mtk.F90:455-490):
+//   phi = (Rij.Vij)^2 / Rij^2 * Vij^2, nothing to do when phi < eps^2;
+//   k = Rij.Vij / (d^2 (1/mi + 1/mj)), v_i -= 1/mi k Rij, v_j += 1/mj k Rij.
+// Positions do not move in this stage, so Rij^2 and d^2 (1/mi + 1/mj) are
+// constants and both divisions are multiplications by their reciprocals
+// (rr2i, rdm), each within one rounding of the reference's quotient.
+__device__ __forceinline__ bool rattle_one(const double3 &r, double rr2i, double3 &vi,
+                                           double3 &vj, double mi, double mj, double rdm,
+                                           double eps2) {
+#pragma clang fp contract(off)
+  const double3 v = sub(vi, vj);
+  const double rv = dot_nc(r, v);
+  const bool ok = rv * rv * rr2i * dot_nc(v, v) < eps2;
+  const double k = ok ? 0.0 : rv * rdm;   // branch-free, as in shake_one
+  const double3 kr = make_double3(k * r.x, k * r.y, k * r.z);
+  vi = make_double3(vi.x - mi * kr.x, vi.y - mi * kr.y, vi.z - mi * kr.z);
+  vj = make_double3(vj.x + mj * kr.x, vj.y + mj * kr.y, vj.z + mj * kr.z);
+  return ok;
 }
 
-// SHAKE on the positions (with the matching velocity correction): displace
-// along the old bond vector, g = (d^2 - r.r) / (2 (1/mi + 1/mj) r.r_old)
-__global__ __launch_bounds__(256) void k_shake(ConsArgs a) {
+// positions (with the matching velocity correction), one group per thread
+__global__ __launch_bounds__(64) void k_shake(ConsArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.ngroup) return;
   const int c0 = a.gstart[g], c1 = a.gstart[g + 1];
@@ -122,12 +140,14 @@ __global__ __launch_bounds__(256) void k_shake(ConsArgs a) {
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     const double3 o0 = ld3(a.pos_old, i0), o1 = ld3(a.pos_old, i1), o2 = ld3(a.pos_old, i2);
+    const double3 r01 = sub(o0, o1), r02 = sub(o0, o2), r12 = sub(o1, o2);
     const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
     const double d01 = a.d2[c0], d02 = a.d2[c0 + 1], d12 = a.d2[c0 + 2];
+    const double e01 = 1.0 / d01, e02 = 1.0 / d02, e12 = 1.0 / d12;
     for (int it = 0; it < a.maxit && !done; it++) {
-      done = shake_one(p0, p1, v0, v1, o0, o1, m0, m1, d01, a.tol, a.dt);
-      done = shake_one(p0, p2, v0, v2, o0, o2, m0, m2, d02, a.tol, a.dt) && done;
-      done = shake_one(p1, p2, v1, v2, o1, o2, m1, m2, d12, a.tol, a.dt) && done;
+      done = shake_one(p0, p1, v0, v1, r01, m0, m1, d01, e01, a.tol, a.dt);
+      done = shake_one(p0, p2, v0, v2, r02, m0, m2, d02, e02, a.tol, a.dt) && done;
+      done = shake_one(p1, p2, v1, v2, r12, m1, m2, d12, e12, a.tol, a.dt) && done;
     }
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
@@ -141,8 +161,9 @@ __global__ __launch_bounds__(256) void k_shake(ConsArgs a) {
       for (int k = c0; k < c1; k++) {
         const int i = a.cij[2 * k], j = a.cij[2 * k + 1];
         double3 pi = ld3(a.pos, i), pj = ld3(a.pos, j), vi = ld3(a.vel, i), vj = ld3(a.vel, j);
-        if (!shake_one(pi, pj, vi, vj, ld3(a.pos_old, i), ld3(a.pos_old, j), a.minv[i],
-                       a.minv[j], a.d2[k], a.tol, a.dt)) {
+        const double3 ro = sub(ld3(a.pos_old, i), ld3(a.pos_old, j));
+        if (!shake_one(pi, pj, vi, vj, ro, a.minv[i], a.minv[j], a.d2[k], 1.0 / a.d2[k], a.tol,
+                       a.dt)) {
           done = false;
           st3(a.pos, i, pi);
           st3(a.pos, j, pj);
@@ -155,23 +176,29 @@ __global__ __launch_bounds__(256) void k_shake(ConsArgs a) {
   if (!done) atomicOr(a.fail, 1);
 }
 
-// RATTLE velocity stage: remove the relative velocity along each bond,
-// k = r.(vi - vj) / (d^2 (1/mi + 1/mj))
-__global__ __launch_bounds__(256) void k_rattle_v(ConsArgs a) {
+// velocities, one group per thread
+__global__ __launch_bounds__(64) void k_rattle_v(ConsArgs a) {
+#pragma clang fp contract(off)
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.ngroup) return;
   const int c0 = a.gstart[g], c1 = a.gstart[g + 1];
+  const double eps2 = a.tol * a.tol;
   bool done = false;
   if (a.small[g]) {
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
-    const double d01 = a.d2[c0], d02 = a.d2[c0 + 1], d12 = a.d2[c0 + 2];
+    // positions are fixed here, so Rij and Rij^2 are the same every iteration
+    const double3 r01 = sub(p0, p1), r02 = sub(p0, p2), r12 = sub(p1, p2);
+    const double q01 = 1.0 / dot_nc(r01, r01), q02 = 1.0 / dot_nc(r02, r02),
+                 q12 = 1.0 / dot_nc(r12, r12);
+    const double dm01 = 1.0 / (a.d2[c0] * (m0 + m1)), dm02 = 1.0 / (a.d2[c0 + 1] * (m0 + m2)),
+                 dm12 = 1.0 / (a.d2[c0 + 2] * (m1 + m2));
     for (int it = 0; it < a.maxit && !done; it++) {
-      done = rattle_one(p0, p1, v0, v1, m0, m1, d01, a.tol);
-      done = rattle_one(p0, p2, v0, v2, m0, m2, d02, a.tol) && done;
-      done = rattle_one(p1, p2, v1, v2, m1, m2, d12, a.tol) && done;
+      done = rattle_one(r01, q01, v0, v1, m0, m1, dm01, eps2);
+      done = rattle_one(r02, q02, v0, v2, m0, m2, dm02, eps2) && done;
+      done = rattle_one(r12, q12, v1, v2, m1, m2, dm12, eps2) && done;
     }
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
@@ -182,8 +209,10 @@ __global__ __launch_bounds__(256) void k_rattle_v(ConsArgs a) {
       for (int k = c0; k < c1; k++) {
         const int i = a.cij[2 * k], j = a.cij[2 * k + 1];
         double3 vi = ld3(a.vel, i), vj = ld3(a.vel, j);
-        if (!rattle_one(ld3(a.pos, i), ld3(a.pos, j), vi, vj, a.minv[i], a.minv[j], a.d2[k],
-                        a.tol)) {
+        const double mi = a.minv[i], mj = a.minv[j];
+        const double3 r = sub(ld3(a.pos, i), ld3(a.pos, j));
+        if (!rattle_one(r, 1.0 / dot_nc(r, r), vi, vj, mi, mj, 1.0 / (a.d2[k] * (mi + mj)),
+                        eps2)) {
           done = false;
           st3(a.vel, i, vi);
           st3(a.vel, j, vj);
@@ -235,7 +264,7 @@ int pgk_md_kick_drift(pgk_ctx *c) {
       n, c->d_pos, c->d_md_pos_old, c->d_md_vel, c->d_force, c->d_md_minv, c->md_dt);
   HIP_CHECK(hipGetLastError());
   if (c->md_ngroup > 0) {
-    k_shake<<<pgk_grid_blocks(c->md_ngroup, 256), 256, 0, c->stream>>>(cons_args(c));
+    k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(cons_args(c));
     HIP_CHECK(hipGetLastError());
   }
   return PGK_OK;
@@ -251,7 +280,7 @@ int pgk_md_kick(pgk_ctx *c) {
 
 int pgk_md_rattle_v(pgk_ctx *c) {
   if (c->md_ngroup > 0) {
-    k_rattle_v<<<pgk_grid_blocks(c->md_ngroup, 256), 256, 0, c->stream>>>(cons_args(c));
+    k_rattle_v<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(cons_args(c));
     HIP_CHECK(hipGetLastError());
   }
   return PGK_OK;

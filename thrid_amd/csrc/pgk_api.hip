@@ -1143,36 +1143,27 @@ int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
       d2.push_back(md->cons_d[k] * md->cons_d[k]);
     }
   }
-  // triangle groups (3 atoms, 3 constraints) are reordered to (a,b), (a,c),
-  // (b,c) for the register path of the constraint kernels
+  // triangle groups (3 atoms, 3 constraints) keep the host's constraint order;
+  // their atoms are labelled a = the atom shared by the first two constraints,
+  // b and c the others, so the order reads (a,b), (a,c), (b,c).  Swapping the
+  // two atoms of a constraint changes no result (both vectors change sign).
   const int ng = md->ncons > 0 ? md->ngroup : 0;
   std::vector<int32_t> small(std::max(ng, 1), 0);
   for (int g = 0; g < ng; g++) {
     if (gs[g + 1] - gs[g] != 3) continue;
     const int k0 = gs[g];
-    int at[3] = {cij[2 * k0], cij[2 * k0 + 1], -1};
-    for (int k = k0 + 1; k < k0 + 3; k++)
-      for (int e = 0; e < 2; e++) {
-        const int x = cij[2 * k + e];
-        if (x != at[0] && x != at[1]) at[2] = x;
-      }
-    if (at[2] < 0) continue;
-    int kk[3] = {-1, -1, -1};   // constraint index of (0,1), (0,2), (1,2)
-    const int want[3][2] = {{at[0], at[1]}, {at[0], at[2]}, {at[1], at[2]}};
-    for (int q = 0; q < 3; q++)
-      for (int k = k0; k < k0 + 3; k++) {
-        const int x = cij[2 * k], y = cij[2 * k + 1];
-        if ((x == want[q][0] && y == want[q][1]) || (x == want[q][1] && y == want[q][0]))
-          kk[q] = k;
-      }
-    if (kk[0] < 0 || kk[1] < 0 || kk[2] < 0 || kk[0] == kk[1] || kk[0] == kk[2] ||
-        kk[1] == kk[2])
-      continue;
-    const double dd[3] = {d2[kk[0]], d2[kk[1]], d2[kk[2]]};
+    const int x0 = cij[2 * k0], y0 = cij[2 * k0 + 1];
+    const int x1 = cij[2 * k0 + 2], y1 = cij[2 * k0 + 3];
+    const int x2 = cij[2 * k0 + 4], y2 = cij[2 * k0 + 5];
+    const int a = (x0 == x1 || x0 == y1) ? x0 : ((y0 == x1 || y0 == y1) ? y0 : -1);
+    if (a < 0) continue;
+    const int b = (x0 == a) ? y0 : x0, cc = (x1 == a) ? y1 : x1;
+    if (b == a || cc == a || cc == b) continue;
+    if (!((x2 == b && y2 == cc) || (x2 == cc && y2 == b))) continue;
+    const int lab[3][2] = {{a, b}, {a, cc}, {b, cc}};
     for (int q = 0; q < 3; q++) {
-      cij[2 * (k0 + q)] = want[q][0];
-      cij[2 * (k0 + q) + 1] = want[q][1];
-      d2[k0 + q] = dd[q];
+      cij[2 * (k0 + q)] = lab[q][0];
+      cij[2 * (k0 + q) + 1] = lab[q][1];
     }
     small[g] = 1;
   }
