@@ -284,9 +284,9 @@ def main():
                      "traffic": traffic_from_profiles(f"k_pairs_{hp}"),
                      "flops_per_launch": flops, "pairs_coulomb": ncoul, "pairs_lj": nlj,
                      "avg_launch_ms": head["prof"]["pairs_timed_region"]},
-        "list": {"clusters": head["info"][0], "cluster_pairs": head["info"][1],
+        "list": {"clusters": head["info"][0], "entries_8x4": head["info"][1],
                  "listed_pairs": head["info"][2],
-                 "lane_efficiency": ncoul / (64.0 * head["info"][1])},
+                 "lane_efficiency": ncoul / (float(head["info"][3]) * head["info"][1])},
     }
     if "fp64" in res and "mixed" in res:
         f64 = res["fp64"]

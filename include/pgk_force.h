@@ -291,8 +291,8 @@ int pgk_profile(pgk_ctx *ctx, int32_t enable, double *ms, int64_t *calls);
  * all).  Timing fewer stages keeps the host's enqueue rate up.               */
 int pgk_profile_stages(pgk_ctx *ctx, uint32_t mask);
 /* Shape of the current list (no reference counterpart: instrumentation):
- * info[0] clusters, info[1] cluster pairs, info[2] listed atom pairs,
- * info[3] atoms per cluster.                                                  */
+ * info[0] 8-atom clusters, info[1] list entries (an i-cluster against half a
+ * j-cluster), info[2] listed atom pairs, info[3] pair slots per entry.       */
 int pgk_list_info(pgk_ctx *ctx, int64_t *info);
 
 #ifdef __cplusplus

@@ -31,6 +31,34 @@ def test_md_trajectory_matches_reference(oracle_mod):
     np.testing.assert_allclose(rows, rrows, rtol=1e-9, atol=1e-9)
 
 
+def test_md_general_groups_match_reference(oracle_mod):
+    """Every second water keeps only its two O-H constraints: those groups
+    are not triangles, so the step takes the separate kick/drift and
+    constraint kernels with the global-memory group path."""
+    s = water648()
+    ij, d, gs = md.rigid_constraints(s)
+    keep, starts = [], [0]
+    for g in range(len(gs) - 1):
+        ks = list(range(gs[g], gs[g + 1]))
+        if g % 2 == 1:
+            ks = [k for k in ks if s["atype"][ij[k, 0] - 1] != s["atype"][ij[k, 1] - 1]]
+        keep += ks
+        starts.append(len(keep))
+    ij2, d2, gs2 = ij[keep], d[keep], np.asarray(starts, np.int32)
+    assert len(ij2) < len(ij)
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=4)
+    from thrid_amd.pgk import ForceContext
+    c = ForceContext(s, "fp64")
+    c.md_setup(s["mass"], ij2, d2, gs2, 1.0)
+    c.md_set_velocities(v0)
+    rows = c.md_run(5, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    rp, rv, rrows = md_reference.run(oracle_mod.Oracle(s), s, v0, ij2, d2, gs2, 1.0, 5)
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    np.testing.assert_allclose(rows, rrows, rtol=1e-9, atol=1e-9)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_md_nve_conserves_energy_and_constraints(prec):
     s = water648()

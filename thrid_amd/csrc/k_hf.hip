@@ -19,8 +19,8 @@
 namespace {
 
 struct HfPairArgs {
-  const int32_t *ci_start, *cj, *cl_atom;
-  const uint64_t *mask;
+  const int32_t *ci_start, *cj, *cl_atom;   // cj: half-cluster index of each entry
+  const uint32_t *mask;
   const double *q;
   const int32_t *type, *molid, *inttyp;
   const double *eps, *rmin;
@@ -38,13 +38,16 @@ __global__ __launch_bounds__(256) void k_hf_pairs(HfPairArgs p, HfDev h) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ci = blockIdx.x * 4 + wave;
   if (ci >= p.ncl) return;
-  const int ii = lane & 7, jj = lane >> 3;
+  // lane = ii + 8 jq + 32 hs: i-slot ii, j-slot jq of entry e + hs
+  const int ii = lane & 7, jq = (lane >> 3) & 3, hs = lane >> 5;
   const int ai = p.cl_atom[ci * PGK_CL + ii];
   const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
-  for (int e = e0; e < e1; e++) {
-    const uint64_t m = p.mask[e];
-    if (!((m >> lane) & 1ull)) continue;
-    const int aj = p.cl_atom[p.cj[e] * PGK_CL + jj];
+  for (int e2 = e0; e2 < e1; e2 += 2) {
+    const int e = e2 + hs;
+    if (e >= e1) continue;
+    const uint32_t m = p.mask[e];
+    if (!((m >> (lane & 31)) & 1u)) continue;
+    const int aj = p.cl_atom[p.cj[e] * 4 + jq];
     double r[3];
     for (int d = 0; d < 3; d++) {
       r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];

@@ -8,10 +8,11 @@
 //
 // Storage is MI355X-shaped rather than the reference's per-atom CSR: atoms are
 // grouped into 8-atom clusters (sorted along z inside x-y columns), candidate
-// cluster pairs are found with bounding boxes, and each kept cluster pair
-// carries a 64-bit mask whose bit (ii + 8*jj) records the reference's decision
-// for the pair (i-slot ii, j-slot jj).  The pair kernel evaluates exactly the
-// masked pairs.
+// cluster pairs are found with bounding boxes, and every half j-cluster (j
+// slots 0-3 or 4-7) with a listed pair becomes an entry: half-cluster index
+// 2 cj + h and a 32-bit mask whose bit (ii + 8 jq) records the reference's
+// decision for the pair (i-slot ii, j-slot 4 h + jq).  The pair kernel
+// evaluates exactly the masked pairs, two entries per wavefront pass.
 #include <hipcub/hipcub.hpp>
 
 #include "pgk_internal.h"
@@ -177,7 +178,7 @@ struct ListArgs {
   const float *bbox;
   const int32_t *xidx, *xlst;
   int32_t *tmp_cj;
-  uint64_t *tmp_mask;
+  uint32_t *tmp_mask;
   int32_t *ci_count;
   int32_t *overflow;
   int ncolumn, ncl_max;
@@ -362,16 +363,22 @@ __global__ __launch_bounds__(256) void k_find_pairs(ListArgs a) {
           }
         }
         const uint64_t m = __ballot(in);
-        if (m) {
-          if (lane == 0) {
-            if (count < PGK_MAXJ) {
-              a.tmp_cj[(size_t)ci * PGK_MAXJ + count] = cj;
-              a.tmp_mask[(size_t)ci * PGK_MAXJ + count] = m;
-            } else {
-              ovf = true;
+        // one entry per half j-cluster with a listed pair: slots jj = 0..3
+        // (mask bits 0..31) and jj = 4..7 (bits 32..63)
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+          const uint32_t mh = (uint32_t)(m >> (32 * hh));
+          if (mh) {
+            if (lane == 0) {
+              if (count < PGK_MAXJ) {
+                a.tmp_cj[(size_t)ci * PGK_MAXJ + count] = 2 * cj + hh;
+                a.tmp_mask[(size_t)ci * PGK_MAXJ + count] = mh;
+              } else {
+                ovf = true;
+              }
             }
+            count++;
           }
-          count++;
         }
       }
     }
@@ -387,8 +394,8 @@ __global__ __launch_bounds__(256) void k_find_pairs(ListArgs a) {
 __global__ void k_compact_pairs(int ncl_max, const int32_t *__restrict__ ci_count,
                                 const int32_t *__restrict__ ci_start,
                                 const int32_t *__restrict__ tmp_cj,
-                                const uint64_t *__restrict__ tmp_mask,
-                                int32_t *__restrict__ cj, uint64_t *__restrict__ mask,
+                                const uint32_t *__restrict__ tmp_mask,
+                                int32_t *__restrict__ cj, uint32_t *__restrict__ mask,
                                 int64_t cap, int32_t *__restrict__ overflow) {
   const int ci = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ci >= ncl_max) return;
@@ -412,13 +419,13 @@ __global__ void k_list_meta(const int32_t *__restrict__ ncl, const int32_t *__re
 }
 
 // number of listed atom pairs (bits set over all masks)
-__global__ void k_count_pairs(int64_t n, const uint64_t *__restrict__ mask,
+__global__ void k_count_pairs(int64_t n, const uint32_t *__restrict__ mask,
                               unsigned long long *__restrict__ out) {
   __shared__ unsigned long long s_part[4];
   unsigned long long cnt = 0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x)
-    cnt += __popcll(mask[k]);
+    cnt += __popc(mask[k]);
   for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_down(cnt, o);
   if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = cnt;
   __syncthreads();
@@ -603,8 +610,8 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
   HIP_CHECK(hipStreamSynchronize(st));
   const int32_t ncl = c->h_list_meta[0], ovf = c->h_list_meta[1], tot = c->h_list_meta[2];
   if (ovf & 1) {
-    pgk_set_error("pgk_build_list: more than 2048 neighbour clusters for one cluster "
-                  "(rcut_book too large for the cluster list)");
+    pgk_set_error("pgk_build_list: more than 2048 neighbour candidates or half-cluster "
+                  "entries for one cluster (rcut_book too large for the cluster list)");
     return PGK_EOVERFLOW;
   }
   if (ovf & 2) {

@@ -22,6 +22,14 @@
 
 namespace {
 
+// half kick v += dt/2 F/m and drift x += dt v of one coordinate; every
+// kernel that moves atoms uses these, so the arithmetic is the same
+__device__ __forceinline__ double kick1(double v, double f, double minv, double dt) {
+  const double h = 0.5 * dt * PGK_MD_CONV * minv;
+  return v + h * f;
+}
+__device__ __forceinline__ double drift1(double x, double v, double dt) { return x + dt * v; }
+
 __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict__ pos,
                                                        double *__restrict__ pos_old,
                                                        double *__restrict__ vel,
@@ -29,12 +37,11 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict
                                                        const double *__restrict__ minv, double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double h = 0.5 * dt * PGK_MD_CONV * minv[i];
   for (int d = 0; d < 3; d++) {
-    const double v = vel[3 * i + d] + h * force[3 * i + d];
+    const double v = kick1(vel[3 * i + d], force[3 * i + d], minv[i], dt);
     vel[3 * i + d] = v;
     pos_old[3 * i + d] = pos[3 * i + d];
-    pos[3 * i + d] = pos[3 * i + d] + dt * v;
+    pos[3 * i + d] = drift1(pos[3 * i + d], v, dt);
   }
 }
 
@@ -43,8 +50,7 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double *__restrict__ vel
                                                  const double *__restrict__ minv, double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double h = 0.5 * dt * PGK_MD_CONV * minv[i];
-  for (int d = 0; d < 3; d++) vel[3 * i + d] += h * force[3 * i + d];
+  for (int d = 0; d < 3; d++) vel[3 * i + d] = kick1(vel[3 * i + d], force[3 * i + d], minv[i], dt);
 }
 
 struct ConsArgs {
@@ -129,6 +135,129 @@ __device__ __forceinline__ bool rattle_one(const double3 &r, double rr2i, double
   return ok;
 }
 
+// a triangle group's position stage in registers: constraints (0,1), (0,2),
+// (1,2) in that order until an iteration moves nothing
+__device__ __forceinline__ bool shake_triangle(const ConsArgs &a, int c0, int i0, int i1, int i2,
+                                               double3 &p0, double3 &p1, double3 &p2,
+                                               double3 &v0, double3 &v1, double3 &v2,
+                                               const double3 &o0, const double3 &o1,
+                                               const double3 &o2) {
+  const double3 r01 = sub(o0, o1), r02 = sub(o0, o2), r12 = sub(o1, o2);
+  const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
+  const double d01 = a.d2[c0], d02 = a.d2[c0 + 1], d12 = a.d2[c0 + 2];
+  const double e01 = 1.0 / d01, e02 = 1.0 / d02, e12 = 1.0 / d12;
+  bool done = false;
+  for (int it = 0; it < a.maxit && !done; it++) {
+    done = shake_one(p0, p1, v0, v1, r01, m0, m1, d01, e01, a.tol, a.dt);
+    done = shake_one(p0, p2, v0, v2, r02, m0, m2, d02, e02, a.tol, a.dt) && done;
+    done = shake_one(p1, p2, v1, v2, r12, m1, m2, d12, e12, a.tol, a.dt) && done;
+  }
+  return done;
+}
+
+// a triangle group's velocity stage in registers
+__device__ __forceinline__ bool rattle_triangle(const ConsArgs &a, int c0, int i0, int i1,
+                                                int i2, const double3 &p0, const double3 &p1,
+                                                const double3 &p2, double3 &v0, double3 &v1,
+                                                double3 &v2) {
+#pragma clang fp contract(off)
+  const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
+  const double eps2 = a.tol * a.tol;
+  // positions are fixed here, so Rij and Rij^2 are the same every iteration
+  const double3 r01 = sub(p0, p1), r02 = sub(p0, p2), r12 = sub(p1, p2);
+  const double q01 = 1.0 / dot_nc(r01, r01), q02 = 1.0 / dot_nc(r02, r02),
+               q12 = 1.0 / dot_nc(r12, r12);
+  const double dm01 = 1.0 / (a.d2[c0] * (m0 + m1)), dm02 = 1.0 / (a.d2[c0 + 1] * (m0 + m2)),
+               dm12 = 1.0 / (a.d2[c0 + 2] * (m1 + m2));
+  bool done = false;
+  for (int it = 0; it < a.maxit && !done; it++) {
+    done = rattle_one(r01, q01, v0, v1, m0, m1, dm01, eps2);
+    done = rattle_one(r02, q02, v0, v2, m0, m2, dm02, eps2) && done;
+    done = rattle_one(r12, q12, v1, v2, m1, m2, dm12, eps2) && done;
+  }
+  return done;
+}
+
+// kick and drift of the three atoms of a triangle group, in registers
+__device__ __forceinline__ void kick_drift3(const double *force, const double *minv, int i,
+                                            double dt, double3 &p, double3 &v, double3 &o) {
+  const double m = minv[i];
+  o = p;
+  v.x = kick1(v.x, force[3 * i], m, dt);
+  v.y = kick1(v.y, force[3 * i + 1], m, dt);
+  v.z = kick1(v.z, force[3 * i + 2], m, dt);
+  p.x = drift1(p.x, v.x, dt);
+  p.y = drift1(p.y, v.y, dt);
+  p.z = drift1(p.z, v.z, dt);
+}
+__device__ __forceinline__ void kick3(const double *force, const double *minv, int i, double dt,
+                                      double3 &v) {
+  const double m = minv[i];
+  v.x = kick1(v.x, force[3 * i], m, dt);
+  v.y = kick1(v.y, force[3 * i + 1], m, dt);
+  v.z = kick1(v.z, force[3 * i + 2], m, dt);
+}
+
+// When every group is a triangle: the first half step (kick, drift, position
+// constraints) in one pass, one thread per group, then one thread per atom
+// outside every group.  Same arithmetic as k_md_kick_drift + k_shake.
+__global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *__restrict__ force,
+                                                      const int32_t *__restrict__ free_atoms,
+                                                      int nfree) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < a.ngroup) {
+    const int c0 = a.gstart[g];
+    const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
+    double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
+    double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
+    double3 o0, o1, o2;
+    kick_drift3(force, a.minv, i0, a.dt, p0, v0, o0);
+    kick_drift3(force, a.minv, i1, a.dt, p1, v1, o1);
+    kick_drift3(force, a.minv, i2, a.dt, p2, v2, o2);
+    const bool done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
+    st3(a.pos, i0, p0);
+    st3(a.pos, i1, p1);
+    st3(a.pos, i2, p2);
+    st3(a.vel, i0, v0);
+    st3(a.vel, i1, v1);
+    st3(a.vel, i2, v2);
+    if (!done) atomicOr(a.fail, 1);
+  } else if (g < a.ngroup + nfree) {
+    const int i = free_atoms[g - a.ngroup];
+    double3 p = ld3(a.pos, i), v = ld3(a.vel, i), o;
+    kick_drift3(force, a.minv, i, a.dt, p, v, o);
+    st3(a.pos, i, p);
+    st3(a.vel, i, v);
+  }
+}
+
+// the second half step (kick, velocity constraints) in one pass
+__global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
+                                                       const double *__restrict__ force,
+                                                       const int32_t *__restrict__ free_atoms,
+                                                       int nfree) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < a.ngroup) {
+    const int c0 = a.gstart[g];
+    const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
+    const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
+    double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
+    kick3(force, a.minv, i0, a.dt, v0);
+    kick3(force, a.minv, i1, a.dt, v1);
+    kick3(force, a.minv, i2, a.dt, v2);
+    const bool done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
+    st3(a.vel, i0, v0);
+    st3(a.vel, i1, v1);
+    st3(a.vel, i2, v2);
+    if (!done) atomicOr(a.fail, 2);
+  } else if (g < a.ngroup + nfree) {
+    const int i = free_atoms[g - a.ngroup];
+    double3 v = ld3(a.vel, i);
+    kick3(force, a.minv, i, a.dt, v);
+    st3(a.vel, i, v);
+  }
+}
+
 // positions (with the matching velocity correction), one group per thread
 __global__ __launch_bounds__(64) void k_shake(ConsArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,15 +269,7 @@ __global__ __launch_bounds__(64) void k_shake(ConsArgs a) {
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     const double3 o0 = ld3(a.pos_old, i0), o1 = ld3(a.pos_old, i1), o2 = ld3(a.pos_old, i2);
-    const double3 r01 = sub(o0, o1), r02 = sub(o0, o2), r12 = sub(o1, o2);
-    const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
-    const double d01 = a.d2[c0], d02 = a.d2[c0 + 1], d12 = a.d2[c0 + 2];
-    const double e01 = 1.0 / d01, e02 = 1.0 / d02, e12 = 1.0 / d12;
-    for (int it = 0; it < a.maxit && !done; it++) {
-      done = shake_one(p0, p1, v0, v1, r01, m0, m1, d01, e01, a.tol, a.dt);
-      done = shake_one(p0, p2, v0, v2, r02, m0, m2, d02, e02, a.tol, a.dt) && done;
-      done = shake_one(p1, p2, v1, v2, r12, m1, m2, d12, e12, a.tol, a.dt) && done;
-    }
+    done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
     st3(a.pos, i2, p2);
@@ -188,18 +309,7 @@ __global__ __launch_bounds__(64) void k_rattle_v(ConsArgs a) {
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
-    const double m0 = a.minv[i0], m1 = a.minv[i1], m2 = a.minv[i2];
-    // positions are fixed here, so Rij and Rij^2 are the same every iteration
-    const double3 r01 = sub(p0, p1), r02 = sub(p0, p2), r12 = sub(p1, p2);
-    const double q01 = 1.0 / dot_nc(r01, r01), q02 = 1.0 / dot_nc(r02, r02),
-                 q12 = 1.0 / dot_nc(r12, r12);
-    const double dm01 = 1.0 / (a.d2[c0] * (m0 + m1)), dm02 = 1.0 / (a.d2[c0 + 1] * (m0 + m2)),
-                 dm12 = 1.0 / (a.d2[c0 + 2] * (m1 + m2));
-    for (int it = 0; it < a.maxit && !done; it++) {
-      done = rattle_one(r01, q01, v0, v1, m0, m1, dm01, eps2);
-      done = rattle_one(r02, q02, v0, v2, m0, m2, dm02, eps2) && done;
-      done = rattle_one(r12, q12, v1, v2, m1, m2, dm12, eps2) && done;
-    }
+    done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
     st3(a.vel, i2, v2);
@@ -260,6 +370,13 @@ ConsArgs cons_args(pgk_ctx *c) {
 // ---- launchers used by the C ABI (pgk_api.hip) ----------------------------
 int pgk_md_kick_drift(pgk_ctx *c) {
   const int n = c->natom;
+  if (c->md_all_tri) {
+    const int nt = c->md_ngroup + c->md_nfree;
+    k_md_first_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(cons_args(c), c->d_force,
+                                                                   c->d_md_free, c->md_nfree);
+    HIP_CHECK(hipGetLastError());
+    return PGK_OK;
+  }
   k_md_kick_drift<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(
       n, c->d_pos, c->d_md_pos_old, c->d_md_vel, c->d_force, c->d_md_minv, c->md_dt);
   HIP_CHECK(hipGetLastError());
@@ -272,6 +389,13 @@ int pgk_md_kick_drift(pgk_ctx *c) {
 
 int pgk_md_kick(pgk_ctx *c) {
   const int n = c->natom;
+  if (c->md_all_tri) {
+    const int nt = c->md_ngroup + c->md_nfree;
+    k_md_second_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(cons_args(c), c->d_force,
+                                                                    c->d_md_free, c->md_nfree);
+    HIP_CHECK(hipGetLastError());
+    return PGK_OK;
+  }
   k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_force,
                                                             c->d_md_minv, c->md_dt);
   HIP_CHECK(hipGetLastError());

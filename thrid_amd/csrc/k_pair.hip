@@ -27,8 +27,8 @@ struct PairArgs {
   using A = typename Prec<PREC>::acc;
   const R4 *xq;
   const int32_t *ct;
-  const int32_t *ci_start, *cj;
-  const uint64_t *mask;
+  const int32_t *ci_start, *cj;   // entries: half-cluster index 2 cj + h
+  const uint32_t *mask;            // entries: bit ii + 8 jq
   const R *spl;          // nspl x 8: force a,b,c,d | energy a,b,c,d
   const R *lj;           // ntype^2 x 2: eps (0 unless INTTYPE_VDW), rmin^2
   A *fcl, *fcl_c;
@@ -185,8 +185,9 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
 #pragma unroll
   for (int k = 0; k < 14; k++) ev[k] = 0.0;
   if (ci < a.ncl) {
-    const int ii = lane & 7, jj = lane >> 3;
-    const unsigned ujj = (unsigned)jj;
+    // lane = ii + 8 jq + 32 hs: i-slot ii against j-slot jq of the pass's
+    // first (hs = 0) or second (hs = 1) half-cluster entry
+    const int ii = lane & 7, jj = lane >> 3, jq = jj & 3, hs = lane >> 5;
     const R4 xi = a.xq[ci * PGK_CL + ii];
     const int ti = a.ct[ci * PGK_CL + ii];   // 0-based type (0 for padding)
     const int nlast = a.nspl_lds - 1;
@@ -194,32 +195,9 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
     const int e0 = uni32(a.ci_start[ci]), e1 = uni32(a.ci_start[ci + 1]);
-    // j-cluster data of entry e+1 is loaded while entry e is computed; the
-    // index is clamped so the loads stay in bounds on the last entry
-    int cj_n = 0;
-    uint64_t m_n = 0;
-    R4 xj_n;
-    int tj_n = 0;
-    if (e0 < e1) {
-      cj_n = uni32(a.cj[e0]);
-      m_n = uni64(a.mask[e0]);
-      xj_n = (a.xq + (size_t)cj_n * PGK_CL)[ujj];
-      tj_n = (a.ct + (size_t)cj_n * PGK_CL)[ujj];
-    }
-
-    for (int e = e0; e < e1; e++) {
-      const int cj = cj_n;
-      const uint64_t m = m_n;
-      const R4 xj = xj_n;
-      const int tj = tj_n;
-      {
-        const int en = min(e + 1, e1 - 1);
-        cj_n = uni32(a.cj[en]);
-        m_n = uni64(a.mask[en]);
-        // scalar base of the j-cluster plus the lane's fixed slot offset
-        xj_n = (a.xq + (size_t)cj_n * PGK_CL)[ujj];
-        tj_n = (a.ct + (size_t)cj_n * PGK_CL)[ujj];
-      }
+    // one pass: two half-cluster entries, masks combined as m (bits 0-31 the
+    // first, 32-63 the second), hc the lane's half-cluster, xj tj its j-slot
+    auto entry = [&](const int hc, const uint64_t m, const R4 xj, const int tj) {
       // every lane runs the same instructions; a lane whose pair is not
       // listed or lies outside a cutoff contributes exact zeros
       R rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -245,7 +223,8 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       // Lennard-Jones (eps is 0 unless the type pair interacts by VDW)
       const bool ljon = in && (r2 <= a.sqcut2);
       const int tp = ti + __mul24(a.ntype, tj);
-      const R eps = ljon ? ljt[2 * tp] : R(0);
+      const R eps0 = ljt[2 * tp];
+      const R eps = ljon ? eps0 : R(0);
       const R rm2 = ljt[2 * tp + 1];
       const R t2 = rinv2 * rm2;
       const R t6 = t2 * t2 * t2;
@@ -279,10 +258,60 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       mb |= mb >> 2;
       mb |= mb >> 1;
       if (lane_bit((mb & 0x0101010101010101ull) * 0x15ull)) {
-        const unsigned s = (unsigned)(jj * 3 + (ii >> 1));
-        acc_add_j(&(a.fcl + (size_t)cj * (3 * PGK_CL))[s], -g);
-        if (SPLIT) acc_add_j(&(a.fcl_c + (size_t)cj * (3 * PGK_CL))[s], -gc);
+        const size_t s = ((size_t)hc * 4 + jq) * 3 + (ii >> 1);
+        acc_add_j(&a.fcl[s], -g);
+        if (SPLIT) acc_add_j(&a.fcl_c[s], -gc);
       }
+    };
+    // The i-cluster's entries are read 64 at a time into registers (lane k
+    // holds entry k of the chunk, the next chunk is already in flight) and
+    // handed out with readlane, so a pass needs no memory round trip before
+    // its j-slot load is issued.
+    struct Ent {
+      int hc;
+      uint64_t m;
+      R4 xj;
+      int tj;
+    };
+    const int nent = e1 - e0;
+    int c_hc = (lane < nent) ? a.cj[e0 + lane] : 0;
+    uint32_t c_m = (lane < nent) ? a.mask[e0 + lane] : 0u;
+    int n_hc = (64 + lane < nent) ? a.cj[e0 + 64 + lane] : 0;
+    uint32_t n_m = (64 + lane < nent) ? a.mask[e0 + 64 + lane] : 0u;
+    for (int cb = 0; cb < nent; cb += 64) {
+      const int nin = min(64, nent - cb);
+      // the pass made of chunk entries k and k+1 (mask 0 past the end;
+      // the readlane index is clamped, the mask decides)
+      auto fetch = [&](Ent &d, int k) {
+        const int kk = min(k, 62);
+        const int ha = __builtin_amdgcn_readlane(c_hc, kk);
+        const int hb = __builtin_amdgcn_readlane(c_hc, kk + 1);
+        const uint32_t ma = (k < nin) ? (uint32_t)__builtin_amdgcn_readlane((int)c_m, kk) : 0u;
+        const uint32_t mb =
+            (k + 1 < nin) ? (uint32_t)__builtin_amdgcn_readlane((int)c_m, kk + 1) : 0u;
+        d.m = ((uint64_t)mb << 32) | ma;
+        d.hc = hs ? hb : ha;
+        const int slot = d.hc * 4 + jq;
+        d.xj = a.xq[slot];
+        d.tj = a.ct[slot];
+      };
+      // two passes in flight: each buffer is refilled right after its pass
+      // is computed, while the other buffer's pass is computed
+      Ent A, B;
+      fetch(A, 0);
+      fetch(B, 2);
+      for (int k = 0; k < nin; k += 4) {
+        entry(A.hc, A.m, A.xj, A.tj);
+        fetch(A, k + 4);
+        if (k + 2 < nin) {
+          entry(B.hc, B.m, B.xj, B.tj);
+          fetch(B, k + 6);
+        }
+      }
+      c_hc = n_hc;
+      c_m = n_m;
+      n_hc = (cb + 128 + lane < nent) ? a.cj[e0 + cb + 128 + lane] : 0;
+      n_m = (cb + 128 + lane < nent) ? a.mask[e0 + cb + 128 + lane] : 0u;
     }
     // i side: sum over the 8 j-lanes that share this i-slot
 #pragma unroll

@@ -238,7 +238,7 @@ int finalize_energies(pgk_ctx *c, double *ene, double *virt);
 
 int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom, bool split,
                  double *d_force_out, double *d_fvc, double *d_fvl, double *ene, double *virt,
-                 bool hf = false, bool wait = true) {
+                 bool hf = false, bool wait = true, bool energies = true) {
   if (!c->have_pos) {
     pgk_set_error("pgk_calforce: positions not set");
     return PGK_ESTATE;
@@ -331,7 +331,9 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     }
   }
   // energies: fixed-order segment sums into d_sum, copied to pinned h_sum
-  if (nseg > 0) {
+  // (a dynamics step that samples nothing skips them; the records are
+  // written either way)
+  if (nseg > 0 && energies) {
     int *dseg = reinterpret_cast<int *>(c->d_sum + 512);
     int *hseg = reinterpret_cast<int *>(c->h_sum + 512);   // pinned staging
     int tab[32] = {};
@@ -832,7 +834,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_hf_z, c->d_hf_typ, c->d_hf_com, c->d_hf_pe, c->d_hf_w,
                   c->d_hf_pe_out, c->d_hf_w_out, c->d_md_gstart, c->d_md_cij, c->d_md_fail,
                   c->d_md_small,
-                  c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_ke};
+                  c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -928,12 +930,12 @@ int pgk_list_pairs(pgk_ctx *c, int32_t *pairs, int64_t capacity) {
     return PGK_EINVAL;
   }
   std::vector<int32_t> start(c->ncl + 1), cj(c->npair_cl), atoms((size_t)c->ncl * PGK_CL);
-  std::vector<uint64_t> mask(c->npair_cl);
+  std::vector<uint32_t> mask(c->npair_cl);
   HIP_CHECK(hipMemcpy(start.data(), c->d_ci_start, sizeof(int32_t) * (c->ncl + 1),
                       hipMemcpyDeviceToHost));
   if (c->npair_cl) {
     HIP_CHECK(hipMemcpy(cj.data(), c->d_cj, sizeof(int32_t) * c->npair_cl, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(mask.data(), c->d_mask, sizeof(uint64_t) * c->npair_cl,
+    HIP_CHECK(hipMemcpy(mask.data(), c->d_mask, sizeof(uint32_t) * c->npair_cl,
                         hipMemcpyDeviceToHost));
   }
   HIP_CHECK(hipMemcpy(atoms.data(), c->d_cl_atom, sizeof(int32_t) * atoms.size(),
@@ -942,9 +944,10 @@ int pgk_list_pairs(pgk_ctx *c, int32_t *pairs, int64_t capacity) {
   v.reserve(c->npairs);
   for (int ci = 0; ci < c->ncl; ci++)
     for (int e = start[ci]; e < start[ci + 1]; e++)
-      for (int b = 0; b < 64; b++)
-        if ((mask[e] >> b) & 1ull) {
-          int ai = atoms[(size_t)ci * PGK_CL + (b & 7)], aj = atoms[(size_t)cj[e] * PGK_CL + (b >> 3)];
+      for (int b = 0; b < 32; b++)
+        if ((mask[e] >> b) & 1u) {
+          // entry cj[e] = 2 cluster + half; bit b = ii + 8 jq
+          int ai = atoms[(size_t)ci * PGK_CL + (b & 7)], aj = atoms[(size_t)cj[e] * 4 + (b >> 3)];
           v.emplace_back(std::min(ai, aj) + 1, std::max(ai, aj) + 1);
         }
   std::sort(v.begin(), v.end());
@@ -1142,6 +1145,19 @@ int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
       cij.push_back(j - 1);
       d2.push_back(md->cons_d[k] * md->cons_d[k]);
     }
+    // groups are independent molecules: one thread updates each group, so no
+    // atom may belong to two of them
+    std::vector<int32_t> owner(n, -1);
+    for (int g = 0; g < md->ngroup; g++)
+      for (int k = gs[g]; k < gs[g + 1]; k++)
+        for (int e = 0; e < 2; e++) {
+          int &o = owner[cij[2 * k + e]];
+          if (o >= 0 && o != g) {
+            pgk_set_error("pgk_md_setup: an atom belongs to two constraint groups");
+            return PGK_EINVAL;
+          }
+          o = g;
+        }
   }
   // triangle groups (3 atoms, 3 constraints) keep the host's constraint order;
   // their atoms are labelled a = the atom shared by the first two constraints,
@@ -1180,6 +1196,19 @@ int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
     return rc;
   std::vector<double> rec((size_t)pgk_grid_blocks(n, 256) * PGK_REC, 0.0);
   if ((rc = upload(&c->d_md_ke, rec))) return rc;
+  // every group a triangle: the half steps run as one kernel each, with the
+  // atoms outside every group listed for their own threads
+  bool all_tri = true;
+  for (int g = 0; g < ng; g++) all_tri = all_tri && small[g];
+  std::vector<int32_t> in_group(n, 0), free_atoms;
+  for (size_t k = 0; k < cij.size(); k++) in_group[cij[k]] = 1;
+  for (int i = 0; i < n; i++)
+    if (!in_group[i]) free_atoms.push_back(i);
+  if (free_atoms.empty()) free_atoms.push_back(0);   // upload needs one element
+  if ((rc = upload(&c->d_md_free, free_atoms))) return rc;
+  c->md_nfree = 0;
+  for (int i = 0; i < n; i++) c->md_nfree += in_group[i] ? 0 : 1;
+  c->md_all_tri = all_tri;
   c->md_ngroup = md->ncons > 0 ? md->ngroup : 0;
   c->md_ncons = md->ncons;
   c->md_dt = md->dt;
@@ -1262,8 +1291,10 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   for (int s = 1; s <= nstep; s++) {
     if ((rc = pgk_md_kick_drift(c))) return rc;
     if (s % nstep_book == 0 && (rc = pgk_build_list(c, rcut_book))) return rc;
+    // energies only where a sample (or the caller, after the last step) reads them
+    const bool ene = (s % nsample == 0) || s == nstep;
     if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr,
-                           nullptr, nullptr, false, false)))
+                           nullptr, nullptr, false, false, ene)))
       return rc;
     if ((rc = pgk_md_kick(c))) return rc;
     if (s % nsample == 0) {
@@ -1312,7 +1343,7 @@ int pgk_list_info(pgk_ctx *c, int64_t *info) {
   info[0] = c->ncl;
   info[1] = c->npair_cl;
   info[2] = c->npairs;
-  info[3] = PGK_CL;
+  info[3] = PGK_CL * 4;   // pair slots per entry (8 i-slots x 4 j-slots)
   return PGK_OK;
 }
 

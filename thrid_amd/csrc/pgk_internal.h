@@ -23,7 +23,7 @@
 #include "pgk_force.h"
 
 #define PGK_CL 8             // atoms per cluster
-#define PGK_MAXJ 2048        // max j-clusters per i-cluster before compaction
+#define PGK_MAXJ 2048        // max candidates / half-cluster entries per i-cluster
 #define PGK_MAXORD 10        // max B-spline order
 #define PGK_NPROF 8          // profiled stages: see enum PGK_PROF_* in pgk_force.h
 
@@ -150,10 +150,10 @@ struct pgk_ctx {
   float *d_bbox = nullptr;         // ncl * 8: lo xyz, hi xyz (wrapped frame)
   double4 *d_posc = nullptr;       // ncl*8: FP64 position + atom index per slot
   int32_t *d_tmp_cj = nullptr;     // ncl * PGK_MAXJ
-  uint64_t *d_tmp_mask = nullptr;
+  uint32_t *d_tmp_mask = nullptr;
   int32_t *d_ci_count = nullptr, *d_ci_start = nullptr;
-  int32_t *d_cj = nullptr;
-  uint64_t *d_mask = nullptr;
+  int32_t *d_cj = nullptr;         // entries: half-cluster index 2 cj + h
+  uint32_t *d_mask = nullptr;      // entries: bit ii + 8 jq for i-slot ii, j-slot 4 h + jq
   int64_t npair_cl = 0, cap_pair_cl = 0;
   int32_t *d_list_meta = nullptr;  // cluster count, overflow flags, entry count
   int32_t *h_list_meta = nullptr;  // pinned copy read once per rebuild
@@ -206,6 +206,9 @@ struct pgk_ctx {
   double md_dt = 1.0, md_tol = 1e-10;
   int32_t *d_md_gstart = nullptr, *d_md_cij = nullptr, *d_md_fail = nullptr;
   int32_t *d_md_small = nullptr;
+  bool md_all_tri = false;         // every group a triangle: fused half-step kernels
+  int md_nfree = 0;                // atoms in no constraint group
+  int32_t *d_md_free = nullptr;
   double *d_md_d2 = nullptr, *d_md_minv = nullptr, *d_md_vel = nullptr, *d_md_pos_old = nullptr;
   double *d_md_ke = nullptr;       // kinetic-energy block records
 
