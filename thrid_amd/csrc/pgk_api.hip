@@ -331,9 +331,9 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     }
   }
   // energies: fixed-order segment sums into d_sum, copied to pinned h_sum
-  // (a dynamics step that samples nothing skips them; the records are
-  // written either way)
-  if (nseg > 0 && energies) {
+  // (a dynamics step that samples nothing sums them on the device but skips
+  // the copy)
+  if (nseg > 0) {
     int *dseg = reinterpret_cast<int *>(c->d_sum + 512);
     int *hseg = reinterpret_cast<int *>(c->h_sum + 512);   // pinned staging
     int tab[32] = {};
@@ -351,8 +351,9 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     }
     k_reduce_segments<<<nseg * PGK_REC, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
-                             hipMemcpyDeviceToHost, st));
+    if (energies)
+      HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
+                               hipMemcpyDeviceToHost, st));
   }
   if (hf && (rc = pgk_hf_export(c))) return rc;
   c->last_nseg = nseg;
