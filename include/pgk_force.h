@@ -276,6 +276,10 @@ int pgk_md_get_state(pgk_ctx *ctx, double *pos, double *vel);
  * holds nstep/nsample + 1 rows.                                              */
 int pgk_md_run(pgk_ctx *ctx, int32_t nstep, int32_t nstep_book, double rcut_book,
                int32_t nsample, double *samples);
+/* Replay steps that rebuild no list and sample nothing from a captured HIP
+ * graph (enable != 0) or launch every step's kernels one by one (0, the
+ * default; no reference counterpart: a switch for drivers and tests).        */
+int pgk_md_use_graphs(pgk_ctx *ctx, int32_t enable);
 
 /* Stage timing (no reference counterpart: instrumentation for measurement).
  * enable != 0 turns HIP-event timing of each stage on (and clears the

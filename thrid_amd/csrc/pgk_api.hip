@@ -1289,21 +1289,84 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
     return rc;
   if ((rc = md_sample(c, samples))) return rc;
   int row = 1;
+  // A plain step (no list rebuild, no sample, no stage timing) launches the
+  // same kernels with the same arguments every time: it is captured once
+  // into a HIP graph after one plain step has run eagerly since the last
+  // rebuild (so no set-up work is left to do inside the capture) and then
+  // replayed.  A rebuild can reallocate the list buffers, so it drops the
+  // graph.
+  struct GraphExec {
+    hipGraphExec_t g = nullptr;
+    ~GraphExec() {
+      if (g) (void)hipGraphExecDestroy(g);
+    }
+  } ge;
+  hipGraphExec_t &gexec = ge.g;
+  bool primed = false;
+  auto drop_graph = [&]() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    primed = false;
+  };
+  auto md_step = [&](bool ene) -> int {
+    int r;
+    if ((r = pgk_md_kick_drift(c))) return r;
+    if ((r = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr, nullptr,
+                          nullptr, false, false, ene)))
+      return r;
+    return pgk_md_kick(c);
+  };
   for (int s = 1; s <= nstep; s++) {
-    if ((rc = pgk_md_kick_drift(c))) return rc;
-    if (s % nstep_book == 0 && (rc = pgk_build_list(c, rcut_book))) return rc;
+    const bool rebuild = (s % nstep_book == 0);
     // energies only where a sample (or the caller, after the last step) reads them
     const bool ene = (s % nsample == 0) || s == nstep;
-    if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr,
-                           nullptr, nullptr, false, false, ene)))
-      return rc;
-    if ((rc = pgk_md_kick(c))) return rc;
+    const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && c->md_graphs;
+    if (plain && gexec) {
+      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
+    } else if (plain && primed) {
+      hipGraph_t graph = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rcap = md_step(false);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rcap || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rcap) pgk_set_error(std::string("pgk_md_run: graph capture failed: ") +
+                                 hipGetErrorString(ec));
+        return rcap ? rcap : PGK_EDEVICE;
+      }
+      const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        gexec = nullptr;
+        pgk_set_error(std::string("pgk_md_run: graph instantiation failed: ") +
+                      hipGetErrorString(ei));
+        return PGK_EDEVICE;
+      }
+      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
+    } else {
+      if ((rc = pgk_md_kick_drift(c))) return rc;
+      if (rebuild) {
+        drop_graph();
+        if ((rc = pgk_build_list(c, rcut_book))) return rc;
+      }
+      if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr,
+                             nullptr, nullptr, false, false, ene)))
+        return rc;
+      if ((rc = pgk_md_kick(c))) return rc;
+      if (plain) primed = true;
+    }
     if (s % nsample == 0) {
       if ((rc = md_sample(c, samples + 3 * row))) return rc;
       row++;
     }
   }
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  return PGK_OK;
+}
+
+int pgk_md_use_graphs(pgk_ctx *c, int32_t enable) {
+  if (!c) return PGK_EINVAL;
+  c->md_graphs = enable != 0;
   return PGK_OK;
 }
 

@@ -56,6 +56,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_set_velocities.argtypes = [vp, vp]
     L.pgk_md_get_state.argtypes = [vp, vp, vp]
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
+    L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
     L.pgk_profile_stages.argtypes = [vp, C.c_uint32]
@@ -64,7 +65,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
-              "pgk_md_run", "pgk_profile_stages"):
+              "pgk_md_run", "pgk_md_use_graphs", "pgk_profile_stages"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -233,8 +234,12 @@ class ForceContext:
                                  int(nsample), out.ctypes.data), "pgk_md_run")
         return out
 
+    def md_use_graphs(self, enable=True):
+        """Replay plain MD steps from a captured HIP graph (off by default)."""
+        _check(self.L.pgk_md_use_graphs(self.h, int(bool(enable))), "pgk_md_use_graphs")
+
     def list_info(self):
-        """(clusters, cluster pairs, listed atom pairs, atoms per cluster)."""
+        """(clusters, list entries, listed atom pairs, pair slots per entry)."""
         v = np.zeros(4, np.int64)
         _check(self.L.pgk_list_info(self.h, v.ctypes.data), "pgk_list_info")
         return tuple(int(x) for x in v)
