@@ -211,6 +211,53 @@ def force_error(f, ref):
             "max_rel": float(d.max() / rms_ref)}
 
 
+def run_ensemble(reps, steps, warmup):
+    """The batched Green-Kubo ensemble leg: every replica relaxed from its
+    generated start, then `steps` mixed-precision MD steps of all replicas run
+    one replica after another and then stepped together (pgk_md_run_batch).
+    Returns the two throughputs in atom-steps/s."""
+    from thrid_amd import md
+    from thrid_amd.pgk import ForceContext, md_run_batch
+    ctxs, cons = [], []
+    t0 = time.perf_counter()
+    for k, s in enumerate(reps):
+        cn = md.rigid_constraints(s)
+        c = ForceContext(s, "mixed")
+        md.relax(c, s, cn)
+        c.md_setup(s["mass"], *cn, DT_FS)
+        c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=100 + k))
+        ctxs.append(c)
+        cons.append(cn)
+    t_setup = time.perf_counter() - t0
+    books = [s["rcut_book"] for s in reps]
+    natoms = sum(s["natom"] for s in reps)
+    for c, s in zip(ctxs, reps):
+        c.md_run(warmup, NSTEP_BOOK, s["rcut_book"], warmup)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for c, s in zip(ctxs, reps):
+        c.md_run(steps, NSTEP_BOOK, s["rcut_book"], steps)
+    for c in ctxs:
+        c.synchronize()
+    t_seq = time.perf_counter() - t0
+    md_run_batch(ctxs, warmup, NSTEP_BOOK, books, warmup)
+    t0 = time.perf_counter()
+    md_run_batch(ctxs, steps, NSTEP_BOOK, books, steps)
+    t_batch = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {"replicas": len(reps), "atoms": natoms,
+            "sizes": [int(min(s["natom"] for s in reps)), int(max(s["natom"] for s in reps))],
+            "steps": steps, "precision": "mixed",
+            "sequential_atom_steps_per_s": natoms * steps / t_seq,
+            "batched_atom_steps_per_s": natoms * steps / t_batch,
+            "batched_over_sequential": t_seq / t_batch,
+            "setup_s": t_setup,
+            "method": "one context and stream per replica; step s of every replica queued "
+                      "before step s+1 of any"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +269,9 @@ def main():
                     help="NVE steps per form for the energy drift")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--ensemble", type=int, default=64, metavar="NREP",
+                    help="replicas in the batched-ensemble leg (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg:   # child process of the CPU baseline leg (no GPU use)
@@ -236,6 +286,11 @@ def main():
               "MI355X (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     from thrid_amd import systems
+    reps = None
+    if args.ensemble > 0:
+        # built by forked workers, before anything in this process uses the GPU
+        reps = systems.replica_ensemble(nrep=args.ensemble,
+                                        workers=min(16, os.cpu_count() or 1))
     if args.copy_calib:
         copy_calibration()
     sysd = systems.water_ion_box_100k()
@@ -298,6 +353,12 @@ def main():
         out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
                                                  / PEAK["fp64"])
         out["speedup_mixed_vs_fp64"] = f64["sec"] / res["mixed"]["sec"]
+    if reps:
+        out["ensemble"] = run_ensemble(reps, args.ensemble_steps, 10)
+        e = out["ensemble"]
+        print(f"# ensemble: {e['replicas']} replicas, {e['atoms']} atoms: sequential "
+              f"{e['sequential_atom_steps_per_s']:.3e}, batched {e['batched_atom_steps_per_s']:.3e} "
+              f"atom-steps/s ({e['batched_over_sequential']:.2f}x)", file=sys.stderr, flush=True)
     if not args.no_cpu:
         base, fref = cpu_leg_subprocess()
         out["cpu_baseline"] = base

@@ -276,9 +276,20 @@ int pgk_md_get_state(pgk_ctx *ctx, double *pos, double *vel);
  * holds nstep/nsample + 1 rows.                                              */
 int pgk_md_run(pgk_ctx *ctx, int32_t nstep, int32_t nstep_book, double rcut_book,
                int32_t nsample, double *samples);
+/* Independent replicas stepped together (no reference counterpart: the
+ * batched Green-Kubo ensemble).  Every ctxs[r] has had pgk_md_setup and
+ * velocities; replica r uses rcut_book[r].  Step s of every replica is queued
+ * on the replicas' own streams before step s+1 of any, so small replicas run
+ * side by side.  samples: replica-major, (nstep/nsample + 1) rows of 3 per
+ * replica.  Each replica's trajectory is the one pgk_md_run gives.          */
+int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nstep_book,
+                     const double *rcut_book, int32_t nsample, double *samples);
 /* Replay steps that rebuild no list and sample nothing from a captured HIP
- * graph (enable != 0) or launch every step's kernels one by one (0, the
- * default; no reference counterpart: a switch for drivers and tests).        */
+ * graph instead of launching their kernels one by one (no reference
+ * counterpart: a switch for drivers and tests).  enable is a bit set: bit 0
+ * for pgk_md_run, bit 1 for this context's replica in pgk_md_run_batch.
+ * Both default off: on MI355X a graph launch cost more than the eager
+ * launches it replaced, for the 100k box and for the 64-replica batch.      */
 int pgk_md_use_graphs(pgk_ctx *ctx, int32_t enable);
 
 /* Stage timing (no reference counterpart: instrumentation for measurement).

@@ -70,7 +70,7 @@ def test_md_graph_replay_matches_eager(prec):
     out = []
     for graphs in (False, True):
         c, _cons = _ctx(s, prec, 1.0)
-        c.md_use_graphs(graphs)
+        c.md_use_graphs(single=graphs)
         c.md_set_velocities(v0)
         rows = c.md_run(47, 10, s["rcut_book"], 7)
         rows2 = c.md_run(13, 5, s["rcut_book"], 13)
@@ -98,3 +98,44 @@ def test_md_nve_conserves_energy_and_constraints(prec):
     p, _v = c.md_state()
     r = np.linalg.norm(p[ij[:, 0] - 1] - p[ij[:, 1] - 1], axis=1)
     assert np.max(np.abs(r - d)) < 1e-8
+
+
+def test_md_batch_matches_single_runs():
+    """Replicas stepped together (pgk_md_run_batch, interleaved streams, with
+    and without per-replica graph replay) give each replica the trajectory it
+    gets alone: bit-identical in the mixed form.  Three uneven replicas:
+    water, n-butane (no constraints) and a water/propanol mixture."""
+    from thrid_amd import systems
+    from thrid_amd.pgk import ForceContext, md_run_batch
+    reps = [systems.mixture_box(1500, "water", seed=11, name="w"),
+            systems.mixture_box(1800, "butane", seed=12, name="b"),
+            systems.mixture_box(2100, "mix", seed=13, alcohol_frac=0.1, name="m")]
+    for s in reps:   # relax the generated starts (huge initial forces)
+        c = ForceContext(s, "fp64")
+        s["pos"] = md.relax(c, s, md.rigid_constraints(s))
+        c.close()
+
+    def make(s, seed):
+        c = ForceContext(s, "mixed")
+        ij, d, gs = md.rigid_constraints(s)
+        c.md_setup(s["mass"], ij, d, gs, 1.0)
+        c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=seed))
+        return c
+
+    single = []
+    for k, s in enumerate(reps):
+        c = make(s, 20 + k)
+        rows = c.md_run(25, 10, s["rcut_book"], 5)
+        single.append((rows,) + c.md_state())
+        c.close()
+    for graphs in (False, True):   # replicas launched eagerly, then from graphs
+        ctxs = [make(s, 20 + k) for k, s in enumerate(reps)]
+        for c in ctxs:
+            c.md_use_graphs(single=False, batch=graphs)
+        rows = md_run_batch(ctxs, 25, 10, [s["rcut_book"] for s in reps], 5)
+        for k, c in enumerate(ctxs):
+            p, v = c.md_state()
+            assert np.array_equal(rows[k], single[k][0])
+            assert np.array_equal(p, single[k][1])
+            assert np.array_equal(v, single[k][2])
+            c.close()

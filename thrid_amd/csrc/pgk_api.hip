@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 
 #include "pgk_device.h"
@@ -1269,6 +1270,89 @@ static int md_sample(pgk_ctx *c, double *row) {
   return PGK_OK;
 }
 
+namespace {
+
+const pgk_mts kMdMts = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
+                        PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+
+// One context's MD loop, step by step.  A plain step (no list rebuild, no
+// sample, no stage timing) launches the same kernels with the same arguments
+// every time; with graphs on it is captured once into a HIP graph, after one
+// plain step has run eagerly since the last rebuild (so no set-up work is
+// left to do inside the capture), and then replayed.  A rebuild can
+// reallocate the list buffers, so it drops the graph.
+struct MdRunner {
+  pgk_ctx *c;
+  bool graphs;
+  hipGraphExec_t gexec = nullptr;
+  bool primed = false;
+  MdRunner(pgk_ctx *ctx, bool use_graphs) : c(ctx), graphs(use_graphs) {}
+  ~MdRunner() { drop(); }
+  void drop() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    primed = false;
+  }
+  int eager(bool ene) {
+    int r;
+    if ((r = pgk_md_kick_drift(c))) return r;
+    if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
+                          nullptr, nullptr, false, false, ene)))
+      return r;
+    return pgk_md_kick(c);
+  }
+  // step 0: list and forces at the starting positions
+  int begin(double rcut_book) {
+    int r;
+    if ((r = pgk_build_list(c, rcut_book))) return r;
+    return run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
+                        nullptr, nullptr, false, false);
+  }
+  int step(bool rebuild, bool ene, double rcut_book) {
+    int r;
+    const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && graphs;
+    if (plain && gexec) {
+      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
+      return PGK_OK;
+    }
+    if (plain && primed) {
+      hipGraph_t graph = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rcap = eager(false);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rcap || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rcap)
+          pgk_set_error(std::string("MD step: graph capture failed: ") + hipGetErrorString(ec));
+        return rcap ? rcap : PGK_EDEVICE;
+      }
+      const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        gexec = nullptr;
+        pgk_set_error(std::string("MD step: graph instantiation failed: ") +
+                      hipGetErrorString(ei));
+        return PGK_EDEVICE;
+      }
+      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
+      return PGK_OK;
+    }
+    if ((r = pgk_md_kick_drift(c))) return r;
+    if (rebuild) {
+      drop();
+      if ((r = pgk_build_list(c, rcut_book))) return r;
+    }
+    if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
+                          nullptr, nullptr, false, false, ene)))
+      return r;
+    if ((r = pgk_md_kick(c))) return r;
+    if (plain) primed = true;
+    return PGK_OK;
+  }
+};
+
+}  // namespace
+
 int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
                int32_t nsample, double *samples) {
   if (!c || !c->md_ready || nstep < 0 || nstep_book < 1 || nsample < 1 || !samples) {
@@ -1281,80 +1365,13 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   }
   int rc = set_device_ctx(c);
   if (rc) return rc;
-  const pgk_mts m = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
-                     PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
-  if ((rc = pgk_build_list(c, rcut_book))) return rc;
-  if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr, nullptr,
-                         nullptr, false, false)))
-    return rc;
-  if ((rc = md_sample(c, samples))) return rc;
+  MdRunner run(c, c->md_graphs);
+  if ((rc = run.begin(rcut_book)) || (rc = md_sample(c, samples))) return rc;
   int row = 1;
-  // A plain step (no list rebuild, no sample, no stage timing) launches the
-  // same kernels with the same arguments every time: it is captured once
-  // into a HIP graph after one plain step has run eagerly since the last
-  // rebuild (so no set-up work is left to do inside the capture) and then
-  // replayed.  A rebuild can reallocate the list buffers, so it drops the
-  // graph.
-  struct GraphExec {
-    hipGraphExec_t g = nullptr;
-    ~GraphExec() {
-      if (g) (void)hipGraphExecDestroy(g);
-    }
-  } ge;
-  hipGraphExec_t &gexec = ge.g;
-  bool primed = false;
-  auto drop_graph = [&]() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    primed = false;
-  };
-  auto md_step = [&](bool ene) -> int {
-    int r;
-    if ((r = pgk_md_kick_drift(c))) return r;
-    if ((r = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr, nullptr,
-                          nullptr, false, false, ene)))
-      return r;
-    return pgk_md_kick(c);
-  };
   for (int s = 1; s <= nstep; s++) {
-    const bool rebuild = (s % nstep_book == 0);
     // energies only where a sample (or the caller, after the last step) reads them
     const bool ene = (s % nsample == 0) || s == nstep;
-    const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && c->md_graphs;
-    if (plain && gexec) {
-      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
-    } else if (plain && primed) {
-      hipGraph_t graph = nullptr;
-      HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int rcap = md_step(false);
-      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-      if (rcap || ec != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        if (!rcap) pgk_set_error(std::string("pgk_md_run: graph capture failed: ") +
-                                 hipGetErrorString(ec));
-        return rcap ? rcap : PGK_EDEVICE;
-      }
-      const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ei != hipSuccess) {
-        gexec = nullptr;
-        pgk_set_error(std::string("pgk_md_run: graph instantiation failed: ") +
-                      hipGetErrorString(ei));
-        return PGK_EDEVICE;
-      }
-      HIP_CHECK(hipGraphLaunch(gexec, c->stream));
-    } else {
-      if ((rc = pgk_md_kick_drift(c))) return rc;
-      if (rebuild) {
-        drop_graph();
-        if ((rc = pgk_build_list(c, rcut_book))) return rc;
-      }
-      if ((rc = run_calforce(c, PGK_MTS_LONG, &m, 0, false, c->d_force, nullptr, nullptr,
-                             nullptr, nullptr, false, false, ene)))
-        return rc;
-      if ((rc = pgk_md_kick(c))) return rc;
-      if (plain) primed = true;
-    }
+    if ((rc = run.step(s % nstep_book == 0, ene, rcut_book))) return rc;
     if (s % nsample == 0) {
       if ((rc = md_sample(c, samples + 3 * row))) return rc;
       row++;
@@ -1364,9 +1381,62 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   return PGK_OK;
 }
 
+// Many independent replicas, each in its own context with its own stream:
+// step s of every replica is queued before step s+1 of any, so the GPU runs
+// the small replicas' kernels side by side instead of one system at a time
+// (plain steps can be replayed from one captured graph per replica, see
+// pgk_md_use_graphs).  Each replica runs exactly the single-context step, so
+// its trajectory is the one pgk_md_run gives.
+int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nstep_book,
+                     const double *rcut_book, int32_t nsample, double *samples) {
+  if (!ctxs || n < 1 || nstep < 0 || nstep_book < 1 || nsample < 1 || !samples || !rcut_book) {
+    pgk_set_error("pgk_md_run_batch: invalid argument");
+    return PGK_EINVAL;
+  }
+  for (int r = 0; r < n; r++) {
+    pgk_ctx *c = ctxs[r];
+    if (!c || !c->md_ready || !c->have_pos) {
+      pgk_set_error("pgk_md_run_batch: replica " + std::to_string(r) +
+                    " has no MD set-up or no positions");
+      return PGK_ESTATE;
+    }
+    for (int q = 0; q < r; q++)
+      if (ctxs[q] == c) {
+        pgk_set_error("pgk_md_run_batch: a context appears twice");
+        return PGK_EINVAL;
+      }
+  }
+  const int nrow = nstep / nsample + 1;
+  std::vector<std::unique_ptr<MdRunner>> run;
+  for (int r = 0; r < n; r++) run.emplace_back(new MdRunner(ctxs[r], ctxs[r]->md_batch_graphs));
+  int rc;
+  for (int r = 0; r < n; r++) {
+    if ((rc = set_device_ctx(ctxs[r])) || (rc = run[r]->begin(rcut_book[r]))) return rc;
+  }
+  for (int r = 0; r < n; r++)
+    if ((rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r))) return rc;
+  int row = 1;
+  for (int s = 1; s <= nstep; s++) {
+    const bool rebuild = (s % nstep_book == 0);
+    const bool ene = (s % nsample == 0) || s == nstep;
+    for (int r = 0; r < n; r++) {
+      if ((rc = set_device_ctx(ctxs[r])) || (rc = run[r]->step(rebuild, ene, rcut_book[r])))
+        return rc;
+    }
+    if (s % nsample == 0) {
+      for (int r = 0; r < n; r++)
+        if ((rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r + 3 * row))) return rc;
+      row++;
+    }
+  }
+  for (int r = 0; r < n; r++) HIP_CHECK(hipStreamSynchronize(ctxs[r]->stream));
+  return PGK_OK;
+}
+
 int pgk_md_use_graphs(pgk_ctx *c, int32_t enable) {
-  if (!c) return PGK_EINVAL;
-  c->md_graphs = enable != 0;
+  if (!c || enable < 0 || enable > 3) return PGK_EINVAL;
+  c->md_graphs = (enable & 1) != 0;
+  c->md_batch_graphs = (enable & 2) != 0;
   return PGK_OK;
 }
 

@@ -207,7 +207,8 @@ struct pgk_ctx {
   int32_t *d_md_gstart = nullptr, *d_md_cij = nullptr, *d_md_fail = nullptr;
   int32_t *d_md_small = nullptr;
   bool md_all_tri = false;         // every group a triangle: fused half-step kernels
-  bool md_graphs = false;          // replay plain MD steps from a captured graph
+  bool md_graphs = false;          // pgk_md_run: replay plain steps from a captured graph
+  bool md_batch_graphs = false;    // pgk_md_run_batch: the same, per replica
   int md_nfree = 0;                // atoms in no constraint group
   int32_t *d_md_free = nullptr;
   double *d_md_d2 = nullptr, *d_md_minv = nullptr, *d_md_vel = nullptr, *d_md_pos_old = nullptr;

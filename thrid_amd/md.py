@@ -66,3 +66,22 @@ def equilibrate(ctx, sysd, temperature, ncycle, nstep_cycle, nstep_book, ncons):
         t = kinetic_temperature(sysd, v, ncons)
         ctx.md_set_velocities(v * np.sqrt(temperature / max(t, 1e-6)))
     return ctx.md_state()
+
+
+RELAX_SCHEDULE = ((0.02, 40), (0.05, 40), (0.1, 40), (0.2, 40), (0.5, 40))
+
+
+def relax(ctx, sysd, cons, schedule=RELAX_SCHEDULE):
+    """Quench a generated start: single MD steps from zero velocity, so each
+    moves every atom by F dt^2 / 2m under the constraints (steepest descent
+    with a step that grows through `schedule` = ((dt_fs, steps), ...)).
+    Leaves the context set up for dt = the last schedule entry; returns the
+    relaxed positions."""
+    ij, d, gs = cons
+    zero = np.zeros((sysd["natom"], 3))
+    for dt, nstep in schedule:
+        ctx.md_setup(sysd["mass"], ij, d, gs, dt)
+        for _ in range(nstep):
+            ctx.md_set_velocities(zero)
+            ctx.md_run(1, 1, sysd["rcut_book"], 1)
+    return ctx.md_state()[0]

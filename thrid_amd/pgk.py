@@ -57,6 +57,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_get_state.argtypes = [vp, vp, vp]
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
+    L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
     L.pgk_profile_stages.argtypes = [vp, C.c_uint32]
@@ -65,7 +66,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
-              "pgk_md_run", "pgk_md_use_graphs", "pgk_profile_stages"):
+              "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -234,9 +235,11 @@ class ForceContext:
                                  int(nsample), out.ctypes.data), "pgk_md_run")
         return out
 
-    def md_use_graphs(self, enable=True):
-        """Replay plain MD steps from a captured HIP graph (off by default)."""
-        _check(self.L.pgk_md_use_graphs(self.h, int(bool(enable))), "pgk_md_use_graphs")
+    def md_use_graphs(self, single=True, batch=True):
+        """Replay plain MD steps from a captured HIP graph: `single` for md_run,
+        `batch` for this replica in md_run_batch (both off by default)."""
+        _check(self.L.pgk_md_use_graphs(self.h, int(bool(single)) | (int(bool(batch)) << 1)),
+               "pgk_md_use_graphs")
 
     def list_info(self):
         """(clusters, list entries, listed atom pairs, pair slots per entry)."""
@@ -259,3 +262,17 @@ class ForceContext:
 
     def synchronize(self):
         _check(self.L.pgk_synchronize(self.h), "pgk_synchronize")
+
+
+def md_run_batch(ctxs, nstep, nstep_book, rcut_books, nsample):
+    """Step independent replicas together (pgk_md_run_batch): every context
+    has had md_setup and md_set_velocities.  Returns (n, nstep//nsample + 1, 3)
+    rows of kinetic, potential and total energy per replica."""
+    n = len(ctxs)
+    L = ctxs[0].L
+    hs = (C.c_void_p * n)(*[c.h.value for c in ctxs])
+    rb = np.ascontiguousarray(rcut_books, np.float64)
+    out = np.zeros((n, nstep // nsample + 1, 3))
+    _check(L.pgk_md_run_batch(C.cast(hs, C.c_void_p), n, int(nstep), int(nstep_book),
+                              rb.ctypes.data, int(nsample), out.ctypes.data), "pgk_md_run_batch")
+    return out

@@ -510,17 +510,34 @@ def fft_friendly(n: int) -> int:
         m += 1
 
 
-def replica_ensemble(nrep=64, nmin=1500, nmax=12000, seed=4):
+def _replica(args):
+    n, kind, rseed, frac, name = args
+    return mixture_box(n, kind, seed=rseed, alcohol_frac=frac, name=name)
+
+
+def replica_ensemble(nrep=64, nmin=1500, nmax=12000, seed=4, workers=1):
     """Uneven Green-Kubo replicas: sizes log-uniform in [nmin, nmax], each a
-    water, n-butane or water/propanol (5-20 mol%) box with its own PME grid."""
+    water, n-butane or water/propanol (5-20 mol%) box with its own PME grid.
+    The replicas' parameters are drawn in order from one generator, so
+    `workers` > 1 (forked processes building replicas side by side) gives the
+    same replicas; fork before anything touches the GPU."""
     rng = np.random.default_rng(seed)
-    reps = []
+    jobs = []
     for r in range(nrep):
         n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
         kind = ["water", "butane", "mix"][int(rng.integers(0, 3))]
         frac = float(rng.uniform(0.05, 0.20))
-        reps.append(mixture_box(n, kind, seed=int(rng.integers(1 << 30)), alcohol_frac=frac,
-                                name=f"rep{r}-{kind}"))
+        jobs.append((n, kind, int(rng.integers(1 << 30)), frac, f"rep{r}-{kind}"))
+    if workers <= 1:
+        return [_replica(j) for j in jobs]
+    import multiprocessing as mp
+    # largest first, so the pool's tail is short
+    order = sorted(range(nrep), key=lambda r: -jobs[r][0])
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = pool.map(_replica, [jobs[r] for r in order], chunksize=1)
+    reps = [None] * nrep
+    for r, sysd in zip(order, built):
+        reps[r] = sysd
     return reps
 
 
