@@ -278,10 +278,13 @@ int pgk_md_run(pgk_ctx *ctx, int32_t nstep, int32_t nstep_book, double rcut_book
                int32_t nsample, double *samples);
 /* Independent replicas stepped together (no reference counterpart: the
  * batched Green-Kubo ensemble).  Every ctxs[r] has had pgk_md_setup and
- * velocities; replica r uses rcut_book[r].  Step s of every replica is queued
- * on the replicas' own streams before step s+1 of any, so small replicas run
- * side by side.  samples: replica-major, (nstep/nsample + 1) rows of 3 per
- * replica.  Each replica's trajectory is the one pgk_md_run gives.          */
+ * velocities; replica r uses rcut_book[r].  Up to 8 host threads share the
+ * replicas (balanced by atom count; one thread when graph replay is on for
+ * any replica); each queues step s of its replicas on
+ * their own streams before step s+1 of any, so small replicas run side by
+ * side.  samples: replica-major, (nstep/nsample + 1) rows of 3 per replica.
+ * Each replica's trajectory is the one pgk_md_run gives.  The contexts must
+ * not be used by other threads meanwhile.                                   */
 int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nstep_book,
                      const double *rcut_book, int32_t nsample, double *samples);
 /* Replay steps that rebuild no list and sample nothing from a captured HIP

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 #include "pgk_device.h"
 
@@ -1382,8 +1383,9 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
 }
 
 // Many independent replicas, each in its own context with its own stream:
-// step s of every replica is queued before step s+1 of any, so the GPU runs
-// the small replicas' kernels side by side instead of one system at a time
+// a host thread queues step s of each of its replicas before step s+1 of
+// any, so the GPU runs the small replicas' kernels side by side instead of
+// one system at a time
 // (plain steps can be replayed from one captured graph per replica, see
 // pgk_md_use_graphs).  Each replica runs exactly the single-context step, so
 // its trajectory is the one pgk_md_run gives.
@@ -1407,29 +1409,77 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
       }
   }
   const int nrow = nstep / nsample + 1;
-  std::vector<std::unique_ptr<MdRunner>> run;
-  for (int r = 0; r < n; r++) run.emplace_back(new MdRunner(ctxs[r], ctxs[r]->md_batch_graphs));
-  int rc;
-  for (int r = 0; r < n; r++) {
-    if ((rc = set_device_ctx(ctxs[r])) || (rc = run[r]->begin(rcut_book[r]))) return rc;
-  }
-  for (int r = 0; r < n; r++)
-    if ((rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r))) return rc;
-  int row = 1;
-  for (int s = 1; s <= nstep; s++) {
-    const bool rebuild = (s % nstep_book == 0);
-    const bool ene = (s % nsample == 0) || s == nstep;
-    for (int r = 0; r < n; r++) {
-      if ((rc = set_device_ctx(ctxs[r])) || (rc = run[r]->step(rebuild, ene, rcut_book[r])))
-        return rc;
+  // Host threads share the replicas (largest first, greedy by atom count),
+  // each queuing its replicas' steps in turn, so launch overhead and the
+  // host waits of list rebuilds and samples overlap across threads.
+  // (With graph replay on, one thread: a capture on one replica's stream
+  // must not overlap another thread's synchronous copies.)
+  bool any_graphs = false;
+  for (int r = 0; r < n; r++) any_graphs = any_graphs || ctxs[r]->md_batch_graphs;
+  const int nth = any_graphs ? 1 : std::min(n, 8);
+  std::vector<std::vector<int>> part(nth);
+  {
+    std::vector<int> order(n);
+    for (int r = 0; r < n; r++) order[r] = r;
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return ctxs[x]->natom > ctxs[y]->natom; });
+    std::vector<int64_t> load(nth, 0);
+    for (int r : order) {
+      const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      part[t].push_back(r);
+      load[t] += ctxs[r]->natom;
     }
-    if (s % nsample == 0) {
-      for (int r = 0; r < n; r++)
-        if ((rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r + 3 * row))) return rc;
-      row++;
-    }
   }
-  for (int r = 0; r < n; r++) HIP_CHECK(hipStreamSynchronize(ctxs[r]->stream));
+  std::vector<int> trc(nth, PGK_OK);
+  std::vector<std::string> terr(nth);
+  auto worker = [&](int t) {
+    const std::vector<int> &mine = part[t];
+    std::vector<std::unique_ptr<MdRunner>> run;
+    for (int r : mine) run.emplace_back(new MdRunner(ctxs[r], ctxs[r]->md_batch_graphs));
+    auto fail = [&](int rc) {
+      trc[t] = rc;
+      terr[t] = pgk_last_error();
+    };
+    int rc;
+    for (size_t k = 0; k < mine.size(); k++) {
+      const int r = mine[k];
+      if ((rc = set_device_ctx(ctxs[r])) || (rc = run[k]->begin(rcut_book[r])) ||
+          (rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r)))
+        return fail(rc);
+    }
+    int row = 1;
+    for (int s = 1; s <= nstep; s++) {
+      const bool rebuild = (s % nstep_book == 0);
+      const bool ene = (s % nsample == 0) || s == nstep;
+      for (size_t k = 0; k < mine.size(); k++) {
+        const int r = mine[k];
+        if ((rc = set_device_ctx(ctxs[r])) || (rc = run[k]->step(rebuild, ene, rcut_book[r])))
+          return fail(rc);
+      }
+      if (s % nsample == 0) {
+        for (int r : mine)
+          if ((rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r + 3 * row)))
+            return fail(rc);
+        row++;
+      }
+    }
+    for (int r : mine) {
+      const hipError_t e = hipStreamSynchronize(ctxs[r]->stream);
+      if (e != hipSuccess) {
+        pgk_set_error(std::string("pgk_md_run_batch: ") + hipGetErrorString(e));
+        return fail(PGK_EDEVICE);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; t++) th.emplace_back(worker, t);
+  worker(0);
+  for (auto &x : th) x.join();
+  for (int t = 0; t < nth; t++)
+    if (trc[t] != PGK_OK) {
+      pgk_set_error(terr[t]);
+      return trc[t];
+    }
   return PGK_OK;
 }
 
