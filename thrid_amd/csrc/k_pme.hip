@@ -585,6 +585,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
   const int total = s_n[nsrc];
   const size_t st = (size_t)nitem;
   // one thread per atom visit: its weights in registers, then the p^3 points
+  // (splitting a visit's z planes over threads measured slower)
   for (int v = threadIdx.x; v < total; v += blockDim.x) {
     int u = 0;
     while (u + 1 < nsrc && v >= s_n[u + 1]) u++;
@@ -620,7 +621,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
           const R val = w1[a1] * w23;
           S *dst = &acc[(lz * T1 + ly) * T0 + lx];
           if (PREC == PGK_PREC_MIXED)
-            atomicAdd(dst, (S)__double2ll_rn((double)val * PGK_GRID_FX_SCALE));
+            atomicAdd(dst, (S)fx_of(val * 256.0f));   // 2^-40 fixed point
           else
             atomicAdd(dst, (S)val);
         }

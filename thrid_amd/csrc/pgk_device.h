@@ -87,4 +87,16 @@ __device__ __forceinline__ void block_record(double (&v)[NF], double *rec) {
   }
 }
 
+// float -> 2^-32 fixed point as hi:lo = floor(v) : (v - floor(v)) 2^32.
+// v - floor(v) is exact for |v| >= 1 (Sterbenz) and for 0 <= v < 1; for
+// -1 < v < 0 it is rounded once (error <= 2^-24).  A pure function of v, so
+// sums of the results are independent of the order of the atomic adds.
+__device__ __forceinline__ unsigned long long fx_of(float v) {
+  const float fl = floorf(v);
+  const float fr = fminf(v - fl, 0x1.fffffep-1f);
+  const unsigned lo = (unsigned)(fr * 4294967296.0f);
+  const int hi = (int)fl;
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)lo;
+}
+
 #define PGK_REC 16   // doubles per reduction record
