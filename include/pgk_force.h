@@ -295,6 +295,30 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
  * launches it replaced, for the 100k box and for the 64-replica batch.      */
 int pgk_md_use_graphs(pgk_ctx *ctx, int32_t enable);
 
+/* Nose-Hoover chain thermostat for pgk_md_run / pgk_md_run_batch: the NVT
+ * step of moldyn_nhc.F90:1238-1690 with every term on the long step, i.e.
+ * nhcint (nhcint.F90:8-140), the thermostatted velocity update
+ * v = v exp(-dt vxi1/2) + F/m bbv (moldyn_nhc.F90:1285-1302, 1589-1590),
+ * drift, rattle_c, forces, the same velocity update, rattlep_v, nhcint.
+ * Q1 = nfree kT / tfreq^2, Qi = kT / tfreq^2 (prepmd.F90:515-518); the
+ * Yoshida-Suzuki weights of prepmd.F90:534-548.  With the chain on, the
+ * total-energy sample is the conserved quantity of calkin.F90:189-201
+ * (KE + PE + sum Q vxi^2/2 + nfree kT xi1 + kT sum_{i>=2} xi_i).          */
+typedef struct pgk_nhc_desc {
+  int32_t mchain;              /* chain length, 2..10 (maxnhc)                */
+  double temperature;          /* text_c (K)                                  */
+  double tfreq;                /* thermostat frequency (1/fs)                 */
+  double nfree;                /* degfree_all: degrees of freedom             */
+  int32_t nyosh;               /* Yoshida-Suzuki order: 1, 3 or 5             */
+  int32_t next;                /* nhcint sub-steps (next >= 1)                */
+  const double *xlogs;         /* [mchain] start xi (NULL = 0)                */
+  const double *vlogs;         /* [mchain] start vxi (1/fs, NULL = 0)         */
+} pgk_nhc_desc;
+/* nhc NULL switches the thermostat off (plain NVE).                          */
+int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
+/* Current chain state: xlogs[mchain], vlogs[mchain] (1/fs); NULL skips.      */
+int pgk_md_nhc_state(pgk_ctx *ctx, double *xlogs, double *vlogs);
+
 /* Stage timing (no reference counterpart: instrumentation for measurement).
  * enable != 0 turns HIP-event timing of each stage on (and clears the
  * totals); ms[PGK_NPROF_STAGES] and calls[PGK_NPROF_STAGES] receive the

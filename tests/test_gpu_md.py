@@ -104,7 +104,8 @@ def test_md_batch_matches_single_runs():
     """Replicas stepped together (pgk_md_run_batch, interleaved streams, with
     and without per-replica graph replay) give each replica the trajectory it
     gets alone: bit-identical in the mixed form.  Three uneven replicas:
-    water, n-butane (no constraints) and a water/propanol mixture."""
+    water, n-butane (no constraints) and a water/propanol mixture, the first
+    and last under a Nose-Hoover chain."""
     from thrid_amd import systems
     from thrid_amd.pgk import ForceContext, md_run_batch
     reps = [systems.mixture_box(1500, "water", seed=11, name="w"),
@@ -120,6 +121,8 @@ def test_md_batch_matches_single_runs():
         ij, d, gs = md.rigid_constraints(s)
         c.md_setup(s["mass"], ij, d, gs, 1.0)
         c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=seed))
+        if seed % 2 == 0:   # the first and last replica under a thermostat
+            c.md_nhc(320.0, 0.01, 3 * s["natom"] - len(d) - 3)
         return c
 
     single = []
@@ -139,3 +142,65 @@ def test_md_batch_matches_single_runs():
             assert np.array_equal(p, single[k][1])
             assert np.array_equal(v, single[k][2])
             c.close()
+
+
+def _nfree(s, d):
+    return 3 * s["natom"] - len(d) - 3
+
+
+def test_md_nhc_matches_reference(oracle_mod):
+    """The NVT step (Nose-Hoover chain of 4, Yoshida-Suzuki order 3, two
+    nhcint sub-steps) against the NumPy integrator on the oracle's forces:
+    positions, velocities, the chain's state and the energy samples (E the
+    conserved quantity)."""
+    s = water648()
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=8)
+    c, (ij, d, gs) = _ctx(s, "fp64", 1.0)
+    c.md_set_velocities(v0)
+    c.md_nhc(250.0, 0.01, _nfree(s, d), mchain=4, nyosh=3, next=2)
+    rows = c.md_run(5, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    x, vx = c.md_nhc_state()
+    nhc = md_reference.Nhc(4, 250.0, 0.01, _nfree(s, d), 3, 2)
+    mass = np.asarray(s["mass"], np.float64)
+    rrows = []
+
+    def record(pos, vel, pe):
+        ke = 0.5 * (mass[:, None] * vel * vel).sum() / md.MD_CONV
+        rrows.append([ke, pe, ke + pe + nhc.conserved_extra()])
+
+    rp, rv = md_reference.run_nhc(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.0, 5, nhc,
+                                  record=record)
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    assert np.max(np.abs(vx - nhc.v)) < 1e-9 * np.max(np.abs(nhc.v))
+    assert np.max(np.abs(x - nhc.x)) < 1e-9 * np.max(np.abs(nhc.x))
+    assert np.max(np.abs(nhc.v)) > 1e-4      # the chain did move
+    np.testing.assert_allclose(rows, np.array(rrows), rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_md_nhc_thermalises_and_conserves(prec):
+    """2000 NVT steps of the relaxed box from 300 K towards 360 K: the
+    extended energy stays put, the temperature settles at the target and the
+    constraints hold."""
+    from thrid_amd.pgk import ForceContext
+    s = water648()
+    c = ForceContext(s, "fp64")
+    s["pos"] = md.relax(c, s, md.rigid_constraints(s))
+    c.close()
+    c, (ij, d, gs) = _ctx(s, prec, 1.0)
+    c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=9))
+    c.md_nhc(360.0, 0.02, _nfree(s, d))
+    rows = c.md_run(2000, 20, s["rcut_book"], 10)
+    ke, h = rows[:, 0], rows[:, 2]
+    temp = 2.0 * ke / (_nfree(s, d) * md.KB_EREF)
+    print(prec, "H fluctuation / KE", np.std(h) / np.mean(ke), "T by 200 fs",
+          [round(temp[k:k + 20].mean(), 1) for k in range(0, 200, 20)])
+    assert np.std(h) < 5e-3 * np.mean(ke)
+    assert abs(h[-1] - h[0]) < 1e-2 * np.mean(ke)
+    assert abs(temp[100:].mean() - 360.0) < 0.05 * 360.0
+    p, _v = c.md_state()
+    r = np.linalg.norm(p[ij[:, 0] - 1] - p[ij[:, 1] - 1], axis=1)
+    assert np.max(np.abs(r - d)) < 1e-8
+

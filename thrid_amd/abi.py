@@ -84,6 +84,12 @@ class MdDesc(C.Structure):
                 ("tol", C.c_double)]
 
 
+class NhcDesc(C.Structure):
+    _fields_ = [("mchain", C.c_int32), ("temperature", C.c_double), ("tfreq", C.c_double),
+                ("nfree", C.c_double), ("nyosh", C.c_int32), ("next", C.c_int32),
+                ("xlogs", _dp), ("vlogs", _dp)]
+
+
 def dptr(a):
     return None if a is None else a.ctypes.data_as(_dp)
 

@@ -30,15 +30,42 @@ __device__ __forceinline__ double kick1(double v, double f, double minv, double 
 }
 __device__ __forceinline__ double drift1(double x, double v, double dt) { return x + dt * v; }
 
+// With the Nose-Hoover chain on, the half kick is the thermostatted update
+// v exp(-dt vxi1/2) + F/m bbv (moldyn_nhc.F90:1301-1302, 1589-1590); nhc
+// points at the two factors {exp(-dt vxi1/2), bbv} that k_nhc_half keeps on
+// the device, or is null (plain NVE).
+struct Kick {
+  bool nhc;
+  double s2, bbv;
+};
+__device__ __forceinline__ Kick kick_of(const double *nhc) {
+  Kick k{nhc != nullptr, 1.0, 0.0};
+  if (nhc) {
+    k.s2 = nhc[0];
+    k.bbv = nhc[1];
+  }
+  return k;
+}
+__device__ __forceinline__ double kick_nhc(double v, double f, double minv, const Kick &k) {
+#pragma clang fp contract(off)
+  return v * k.s2 + f * PGK_MD_CONV * minv * k.bbv;
+}
+__device__ __forceinline__ double kickv(double v, double f, double minv, double dt,
+                                        const Kick &k) {
+  return k.nhc ? kick_nhc(v, f, minv, k) : kick1(v, f, minv, dt);
+}
+
 __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict__ pos,
                                                        double *__restrict__ pos_old,
                                                        double *__restrict__ vel,
                                                        const double *__restrict__ force,
-                                                       const double *__restrict__ minv, double dt) {
+                                                       const double *__restrict__ minv, double dt,
+                                                       const double *__restrict__ nhc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const Kick k = kick_of(nhc);
   for (int d = 0; d < 3; d++) {
-    const double v = kick1(vel[3 * i + d], force[3 * i + d], minv[i], dt);
+    const double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
     vel[3 * i + d] = v;
     pos_old[3 * i + d] = pos[3 * i + d];
     pos[3 * i + d] = drift1(pos[3 * i + d], v, dt);
@@ -47,10 +74,13 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict
 
 __global__ __launch_bounds__(256) void k_md_kick(int n, double *__restrict__ vel,
                                                  const double *__restrict__ force,
-                                                 const double *__restrict__ minv, double dt) {
+                                                 const double *__restrict__ minv, double dt,
+                                                 const double *__restrict__ nhc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int d = 0; d < 3; d++) vel[3 * i + d] = kick1(vel[3 * i + d], force[3 * i + d], minv[i], dt);
+  const Kick k = kick_of(nhc);
+  for (int d = 0; d < 3; d++)
+    vel[3 * i + d] = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
 }
 
 struct ConsArgs {
@@ -65,6 +95,7 @@ struct ConsArgs {
   double dt, tol;            // tol: eps_rattle, a relative tolerance on |r|^2
   int maxit;
   int32_t *fail;           // set when a group does not converge
+  const double *nhc;       // thermostat factors of the half kicks, or null
 };
 
 // Triangle groups (three atoms held by three constraints: the rigid 3-site
@@ -180,22 +211,23 @@ __device__ __forceinline__ bool rattle_triangle(const ConsArgs &a, int c0, int i
 
 // kick and drift of the three atoms of a triangle group, in registers
 __device__ __forceinline__ void kick_drift3(const double *force, const double *minv, int i,
-                                            double dt, double3 &p, double3 &v, double3 &o) {
+                                            double dt, const Kick &k, double3 &p, double3 &v,
+                                            double3 &o) {
   const double m = minv[i];
   o = p;
-  v.x = kick1(v.x, force[3 * i], m, dt);
-  v.y = kick1(v.y, force[3 * i + 1], m, dt);
-  v.z = kick1(v.z, force[3 * i + 2], m, dt);
+  v.x = kickv(v.x, force[3 * i], m, dt, k);
+  v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
+  v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
   p.x = drift1(p.x, v.x, dt);
   p.y = drift1(p.y, v.y, dt);
   p.z = drift1(p.z, v.z, dt);
 }
 __device__ __forceinline__ void kick3(const double *force, const double *minv, int i, double dt,
-                                      double3 &v) {
+                                      const Kick &k, double3 &v) {
   const double m = minv[i];
-  v.x = kick1(v.x, force[3 * i], m, dt);
-  v.y = kick1(v.y, force[3 * i + 1], m, dt);
-  v.z = kick1(v.z, force[3 * i + 2], m, dt);
+  v.x = kickv(v.x, force[3 * i], m, dt, k);
+  v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
+  v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
 }
 
 // When every group is a triangle: the first half step (kick, drift, position
@@ -205,15 +237,16 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
                                                       const int32_t *__restrict__ free_atoms,
                                                       int nfree) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const Kick k = kick_of(a.nhc);
   if (g < a.ngroup) {
     const int c0 = a.gstart[g];
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     double3 o0, o1, o2;
-    kick_drift3(force, a.minv, i0, a.dt, p0, v0, o0);
-    kick_drift3(force, a.minv, i1, a.dt, p1, v1, o1);
-    kick_drift3(force, a.minv, i2, a.dt, p2, v2, o2);
+    kick_drift3(force, a.minv, i0, a.dt, k, p0, v0, o0);
+    kick_drift3(force, a.minv, i1, a.dt, k, p1, v1, o1);
+    kick_drift3(force, a.minv, i2, a.dt, k, p2, v2, o2);
     const bool done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
@@ -225,7 +258,7 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 p = ld3(a.pos, i), v = ld3(a.vel, i), o;
-    kick_drift3(force, a.minv, i, a.dt, p, v, o);
+    kick_drift3(force, a.minv, i, a.dt, k, p, v, o);
     st3(a.pos, i, p);
     st3(a.vel, i, v);
   }
@@ -237,14 +270,15 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
                                                        const int32_t *__restrict__ free_atoms,
                                                        int nfree) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const Kick k = kick_of(a.nhc);
   if (g < a.ngroup) {
     const int c0 = a.gstart[g];
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
-    kick3(force, a.minv, i0, a.dt, v0);
-    kick3(force, a.minv, i1, a.dt, v1);
-    kick3(force, a.minv, i2, a.dt, v2);
+    kick3(force, a.minv, i0, a.dt, k, v0);
+    kick3(force, a.minv, i1, a.dt, k, v1);
+    kick3(force, a.minv, i2, a.dt, k, v2);
     const bool done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
@@ -253,7 +287,7 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 v = ld3(a.vel, i);
-    kick3(force, a.minv, i, a.dt, v);
+    kick3(force, a.minv, i, a.dt, k, v);
     st3(a.vel, i, v);
   }
 }
@@ -347,6 +381,94 @@ __global__ __launch_bounds__(256) void k_md_ke(int n, const double *__restrict__
   block_record<1, 4>(ev, rec + (size_t)blockIdx.x * PGK_REC);
 }
 
+// ---- Nose-Hoover chain ----------------------------------------------------
+#define PGK_NHC_MAX 10   // maxnhc (md_global.F90:58)
+
+struct NhcArgs {
+  int m, nyosh, next, nrec;
+  double gkt, gnkt, dt;
+  double w2[5], w4[5], w8[5];   // yscoeff dt / next / 2, / 4, / 8 (nhcint.F90:56-58)
+};
+
+// the velocity update of chain thermostats 1 <= i <= m-2 (0-based), whose
+// neighbour i+1 damps it (nhcint.F90:86-92, 108-114)
+__device__ __forceinline__ double nhc_mid(const double (&q)[PGK_NHC_MAX],
+                                          const double (&v)[PGK_NHC_MAX], int i, double gkt,
+                                          double w4, double w8) {
+#pragma clang fp contract(off)
+  const double g = (q[i - 1] * v[i - 1] * v[i - 1] - gkt) / q[i];
+  const double e = exp(-1.0 * w8 * v[i + 1]);
+  return v[i] * e * e + w4 * g * e;
+}
+
+// One nhcint call (nhcint.F90:8-140) in one block: kint = sum m v^2 from the
+// kinetic-energy records (a fixed-order sum), then thread 0 integrates the
+// chain over next x nyosh sub-steps and refreshes the factors of the next
+// half kicks (moldyn_nhc.F90:1285-1298).  The arrays are sized for the
+// longest chain and indexed with constants, so they stay in registers;
+// links past the chain hold v = 0 and are never stored.  st: [3m+2] =
+// xlogs, vlogs, qmass, exp(-dt vxi1/2), bbv.
+__global__ __launch_bounds__(256) void k_nhc_half(NhcArgs p, const double *__restrict__ rec,
+                                                  double *__restrict__ st) {
+#pragma clang fp contract(off)
+  double t = 0.0;
+  for (int b = threadIdx.x; b < p.nrec; b += 256) t += rec[(size_t)b * PGK_REC];
+  __shared__ double part[4];
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double kint = 2.0 * (((part[0] + part[1]) + part[2]) + part[3]);   // records hold m v^2 / 2
+  const int m = p.m;
+  double x[PGK_NHC_MAX], v[PGK_NHC_MAX], q[PGK_NHC_MAX];
+#pragma unroll
+  for (int i = 0; i < PGK_NHC_MAX; i++) {
+    x[i] = i < m ? st[i] : 0.0;
+    v[i] = i < m ? st[m + i] : 0.0;
+    q[i] = i < m ? st[2 * m + i] : 1.0;
+  }
+  const double g1 = (kint - p.gnkt) / q[0];
+  for (int inc = 0; inc < p.next; inc++) {
+    for (int iy = 0; iy < p.nyosh; iy++) {
+      const double w2 = p.w2[iy], w4 = p.w4[iy], w8 = p.w8[iy];
+      // last thermostat, then m-1 down to 2 (1-based)
+#pragma unroll
+      for (int i = 1; i < PGK_NHC_MAX; i++)
+        if (i == m - 1) v[i] = v[i] + (q[i - 1] * v[i - 1] * v[i - 1] - p.gkt) / q[i] * w4;
+#pragma unroll
+      for (int i = PGK_NHC_MAX - 2; i >= 1; i--)
+        if (i <= m - 2) v[i] = nhc_mid(q, v, i, p.gkt, w4, w8);
+      // first thermostat, the positions, the first thermostat again with
+      // the same damping factor
+      const double e = exp(-1.0 * w8 * v[1]);
+      v[0] = v[0] * e * e + w4 * g1 * e;
+#pragma unroll
+      for (int i = 0; i < PGK_NHC_MAX; i++) x[i] = x[i] + v[i] * w2;
+      v[0] = v[0] * e * e + w4 * g1 * e;
+      // 2 up to m-1, then the last thermostat
+#pragma unroll
+      for (int i = 1; i <= PGK_NHC_MAX - 2; i++)
+        if (i <= m - 2) v[i] = nhc_mid(q, v, i, p.gkt, w4, w8);
+#pragma unroll
+      for (int i = 1; i < PGK_NHC_MAX; i++)
+        if (i == m - 1) v[i] = v[i] + (q[i - 1] * v[i - 1] * v[i - 1] - p.gkt) / q[i] * w4;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PGK_NHC_MAX; i++)
+    if (i < m) {
+      st[i] = x[i];
+      st[m + i] = v[i];
+    }
+  // expcdt2 = exp(-dt vxi1/4)^2, bbv = exp(-dt vxi1/4) sinh(dt vxi1/4)/(dt vxi1/4) dt/2
+  const double e2 = 1.0 / 6.0, e4 = e2 / 20.0, e6 = e4 / 42.0, e8 = e6 / 72.0;
+  const double ex4 = exp(-0.25 * p.dt * v[0]);
+  const double a2 = (0.25 * p.dt * v[0]) * (0.25 * p.dt * v[0]);
+  const double poly = 1.0 + a2 * (e2 + a2 * (e4 + a2 * (e6 + a2 * e8)));
+  st[3 * m] = ex4 * ex4;
+  st[3 * m + 1] = ex4 * poly * 0.5 * p.dt;
+}
+
 ConsArgs cons_args(pgk_ctx *c) {
   ConsArgs a;
   a.ngroup = c->md_ngroup;
@@ -362,6 +484,7 @@ ConsArgs cons_args(pgk_ctx *c) {
   a.tol = c->md_tol;
   a.maxit = 500;
   a.fail = c->d_md_fail;
+  a.nhc = c->md_nhc_m ? c->d_md_nhc + 3 * c->md_nhc_m : nullptr;
   return a;
 }
 
@@ -378,7 +501,8 @@ int pgk_md_kick_drift(pgk_ctx *c) {
     return PGK_OK;
   }
   k_md_kick_drift<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(
-      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, c->d_force, c->d_md_minv, c->md_dt);
+      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, c->d_force, c->d_md_minv, c->md_dt,
+      cons_args(c).nhc);
   HIP_CHECK(hipGetLastError());
   if (c->md_ngroup > 0) {
     k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(cons_args(c));
@@ -397,7 +521,8 @@ int pgk_md_kick(pgk_ctx *c) {
     return PGK_OK;
   }
   k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_force,
-                                                            c->d_md_minv, c->md_dt);
+                                                            c->d_md_minv, c->md_dt,
+                                                            cons_args(c).nhc);
   HIP_CHECK(hipGetLastError());
   return pgk_md_rattle_v(c);
 }
@@ -414,6 +539,26 @@ int pgk_md_rattle_v(pgk_ctx *c) {
 int pgk_md_ke(pgk_ctx *c, double *rec) {
   const int n = c->natom;
   k_md_ke<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_md_minv, rec);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+// one nhcint call on the kinetic-energy records already in d_md_ke
+int pgk_md_nhc_half(pgk_ctx *c) {
+  NhcArgs p;
+  p.m = c->md_nhc_m;
+  p.nyosh = c->md_nhc_nyosh;
+  p.next = c->md_nhc_next;
+  p.nrec = pgk_grid_blocks(c->natom, 256);
+  p.gkt = c->md_nhc_gkt;
+  p.gnkt = c->md_nhc_gnkt;
+  p.dt = c->md_dt;
+  for (int k = 0; k < 5; k++) {
+    p.w2[k] = c->md_nhc_ys[k] * c->md_dt / (double)c->md_nhc_next / 2.0;
+    p.w4[k] = p.w2[k] / 2.0;
+    p.w8[k] = p.w4[k] / 2.0;
+  }
+  k_nhc_half<<<1, 256, 0, c->stream>>>(p, c->d_md_ke, c->d_md_nhc);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

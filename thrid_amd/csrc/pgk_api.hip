@@ -837,7 +837,8 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_hf_z, c->d_hf_typ, c->d_hf_com, c->d_hf_pe, c->d_hf_w,
                   c->d_hf_pe_out, c->d_hf_w_out, c->d_md_gstart, c->d_md_cij, c->d_md_fail,
                   c->d_md_small,
-                  c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke};
+                  c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
+                  c->d_md_nhc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -1265,9 +1266,20 @@ static int md_sample(pgk_ctx *c, double *row) {
     pgk_set_error("pgk_md_run: constraints did not converge (RATTLE)");
     return PGK_EOVERFLOW;
   }
+  // with the chain on, E is the conserved quantity (calkin.F90:189-201)
+  double eth = 0.0;
+  const int m = c->md_nhc_m;
+  if (m > 0) {
+    std::vector<double> st((size_t)3 * m);
+    HIP_CHECK(hipMemcpy(st.data(), c->d_md_nhc, sizeof(double) * st.size(),
+                        hipMemcpyDeviceToHost));
+    for (int i = 0; i < m; i++) eth += 0.5 * st[2 * m + i] * st[m + i] * st[m + i];
+    eth += c->md_nhc_gnkt * st[0];
+    for (int i = 1; i < m; i++) eth += c->md_nhc_gkt * st[i];
+  }
   row[0] = ke;
   row[1] = ene[PGK_E_TOT];
-  row[2] = ke + ene[PGK_E_TOT];
+  row[2] = ke + ene[PGK_E_TOT] + eth;
   return PGK_OK;
 }
 
@@ -1294,18 +1306,32 @@ struct MdRunner {
     gexec = nullptr;
     primed = false;
   }
+  // With the Nose-Hoover chain on, a step opens and closes with nhcint.
+  // Both calls see the same atom velocities (the chain moves only its own
+  // variables; the atoms feel it through the half kicks), so the closing
+  // call's kinetic-energy records serve the next step's opening call, and
+  // begin() writes them for the first.
+  int nhc_open() { return c->md_nhc_m ? pgk_md_nhc_half(c) : PGK_OK; }
+  int nhc_close() {
+    int r;
+    if (!c->md_nhc_m) return PGK_OK;
+    if ((r = pgk_md_ke(c, c->d_md_ke))) return r;
+    return pgk_md_nhc_half(c);
+  }
   int eager(bool ene) {
     int r;
-    if ((r = pgk_md_kick_drift(c))) return r;
+    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c))) return r;
     if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                           nullptr, nullptr, false, false, ene)))
       return r;
-    return pgk_md_kick(c);
+    if ((r = pgk_md_kick(c))) return r;
+    return nhc_close();
   }
   // step 0: list and forces at the starting positions
   int begin(double rcut_book) {
     int r;
     if ((r = pgk_build_list(c, rcut_book))) return r;
+    if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_ke))) return r;
     return run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                         nullptr, nullptr, false, false);
   }
@@ -1338,7 +1364,7 @@ struct MdRunner {
       HIP_CHECK(hipGraphLaunch(gexec, c->stream));
       return PGK_OK;
     }
-    if ((r = pgk_md_kick_drift(c))) return r;
+    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c))) return r;
     if (rebuild) {
       drop();
       if ((r = pgk_build_list(c, rcut_book))) return r;
@@ -1346,7 +1372,7 @@ struct MdRunner {
     if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                           nullptr, nullptr, false, false, ene)))
       return r;
-    if ((r = pgk_md_kick(c))) return r;
+    if ((r = pgk_md_kick(c)) || (r = nhc_close())) return r;
     if (plain) primed = true;
     return PGK_OK;
   }
@@ -1480,6 +1506,70 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
       pgk_set_error(terr[t]);
       return trc[t];
     }
+  return PGK_OK;
+}
+
+int pgk_md_nhc(pgk_ctx *c, const pgk_nhc_desc *nhc) {
+  if (!c || !c->md_ready) {
+    pgk_set_error("pgk_md_nhc: no MD set-up");
+    return PGK_EINVAL;
+  }
+  if (!nhc) {
+    c->md_nhc_m = 0;
+    return PGK_OK;
+  }
+  const int m = nhc->mchain;
+  if (m < 2 || m > 10 || !(nhc->temperature > 0) || !(nhc->tfreq > 0) || !(nhc->nfree > 0) ||
+      (nhc->nyosh != 1 && nhc->nyosh != 3 && nhc->nyosh != 5) || nhc->next < 1) {
+    pgk_set_error("pgk_md_nhc: need 2 <= mchain <= 10, temperature, tfreq, nfree > 0, "
+                  "nyosh in {1, 3, 5}, next >= 1");
+    return PGK_EINVAL;
+  }
+  // kT in eref, masses in eref fs^2 (prepmd.F90:515-518)
+  const double gkt = 1.380649e-23 / 6.9511042e-21 * nhc->temperature;
+  std::vector<double> st((size_t)3 * m + 2, 0.0);
+  for (int i = 0; i < m; i++) {
+    st[i] = nhc->xlogs ? nhc->xlogs[i] : 0.0;
+    st[m + i] = nhc->vlogs ? nhc->vlogs[i] : 0.0;
+    st[2 * m + i] = (i == 0 ? nhc->nfree * gkt : gkt) / nhc->tfreq / nhc->tfreq;
+  }
+  // Yoshida-Suzuki weights (prepmd.F90:534-548)
+  double ys[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
+  if (nhc->nyosh == 3) {
+    ys[0] = ys[2] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0));
+    ys[1] = 1.0 - 2.0 * ys[0];
+  } else if (nhc->nyosh == 5) {
+    ys[0] = ys[1] = ys[3] = ys[4] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0));
+    ys[2] = 1.0 - 4.0 * ys[0];
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if ((rc = upload(&c->d_md_nhc, st))) return rc;
+  c->md_nhc_m = m;
+  c->md_nhc_nyosh = nhc->nyosh;
+  c->md_nhc_next = nhc->next;
+  c->md_nhc_gkt = gkt;
+  c->md_nhc_gnkt = gkt * nhc->nfree;
+  for (int k = 0; k < 5; k++) c->md_nhc_ys[k] = ys[k];
+  return PGK_OK;
+}
+
+int pgk_md_nhc_state(pgk_ctx *c, double *xlogs, double *vlogs) {
+  if (!c || !c->md_nhc_m) {
+    pgk_set_error("pgk_md_nhc_state: no thermostat set up");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int m = c->md_nhc_m;
+  std::vector<double> st((size_t)2 * m);
+  HIP_CHECK(hipMemcpy(st.data(), c->d_md_nhc, sizeof(double) * st.size(),
+                      hipMemcpyDeviceToHost));
+  for (int i = 0; i < m; i++) {
+    if (xlogs) xlogs[i] = st[i];
+    if (vlogs) vlogs[i] = st[m + i];
+  }
   return PGK_OK;
 }
 

@@ -213,6 +213,11 @@ struct pgk_ctx {
   int32_t *d_md_free = nullptr;
   double *d_md_d2 = nullptr, *d_md_minv = nullptr, *d_md_vel = nullptr, *d_md_pos_old = nullptr;
   double *d_md_ke = nullptr;       // kinetic-energy block records
+  // Nose-Hoover chain (pgk_md_nhc); off when md_nhc_m == 0
+  int md_nhc_m = 0, md_nhc_nyosh = 1, md_nhc_next = 1;
+  double md_nhc_gkt = 0.0, md_nhc_gnkt = 0.0;
+  double md_nhc_ys[5] = {};
+  double *d_md_nhc = nullptr;      // [3m+2]: xlogs, vlogs, qmass, exp(-dt vxi1/2), bbv
 
   // reduction segment bookkeeping for the current call
   int red_used = 0;
@@ -265,6 +270,7 @@ int pgk_md_kick_drift(pgk_ctx *c);
 int pgk_md_kick(pgk_ctx *c);
 int pgk_md_rattle_v(pgk_ctx *c);
 int pgk_md_ke(pgk_ctx *c, double *rec);
+int pgk_md_nhc_half(pgk_ctx *c);
 
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);

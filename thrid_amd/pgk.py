@@ -57,6 +57,8 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_get_state.argtypes = [vp, vp, vp]
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
+    L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
+    L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
     L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
@@ -66,7 +68,8 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
-              "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages"):
+              "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
+              "pgk_md_nhc", "pgk_md_nhc_state"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -240,6 +243,38 @@ class ForceContext:
         `batch` for this replica in md_run_batch (both off by default)."""
         _check(self.L.pgk_md_use_graphs(self.h, int(bool(single)) | (int(bool(batch)) << 1)),
                "pgk_md_use_graphs")
+
+    def md_nhc(self, temperature, tfreq=None, nfree=None, mchain=4, nyosh=3, next=1,
+               xlogs=None, vlogs=None):
+        """Nose-Hoover chain thermostat for md_run / md_run_batch
+        (moldyn_nhc.F90 with every term on the long step): temperature in K,
+        tfreq in 1/fs, nfree degrees of freedom; temperature=None switches it
+        off.  With the chain on, the total-energy column of md_run is the
+        conserved quantity."""
+        if temperature is None:
+            _check(self.L.pgk_md_nhc(self.h, None), "pgk_md_nhc")
+            self._nhc_m = 0
+            return
+        if tfreq is None or nfree is None:
+            raise ValueError("md_nhc needs tfreq and nfree")
+        k = self._nhc_keep = dict(
+            x=None if xlogs is None else np.ascontiguousarray(xlogs, np.float64),
+            v=None if vlogs is None else np.ascontiguousarray(vlogs, np.float64))
+        for a in k.values():
+            if a is not None and a.shape != (mchain,):
+                raise ValueError("xlogs / vlogs need mchain entries")
+        d = abi.NhcDesc(int(mchain), float(temperature), float(tfreq), float(nfree), int(nyosh),
+                        int(next), abi.dptr(k["x"]), abi.dptr(k["v"]))
+        _check(self.L.pgk_md_nhc(self.h, C.byref(d)), "pgk_md_nhc")
+        self._nhc_m = int(mchain)
+
+    def md_nhc_state(self):
+        """(xlogs, vlogs) of the chain, vlogs in 1/fs."""
+        m = getattr(self, "_nhc_m", 0)
+        x = np.zeros(max(m, 1))
+        v = np.zeros(max(m, 1))
+        _check(self.L.pgk_md_nhc_state(self.h, x.ctypes.data, v.ctypes.data), "pgk_md_nhc_state")
+        return x[:m], v[:m]
 
     def list_info(self):
         """(clusters, list entries, listed atom pairs, pair slots per entry)."""
