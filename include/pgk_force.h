@@ -180,8 +180,20 @@ typedef struct pgk_hf_desc {
   const double *molecom;       /* [3*nmol] molecule centres of mass (calcom)   */
 } pgk_hf_desc;
 
+/* ---- heat-flux terms of calheatf (calheatf.F90:1334-1463): index t of
+ * hf[(r*PGK_NHFJ + t)*3 + comp] --------------------------------------------- */
+enum {
+  PGK_HFJ_KIN = 0,                 /* hfkin_atm                                 */
+  PGK_HFJ_POT = 1,                 /* + PGK_HF_* class: hfpot*_atm              */
+  PGK_HFJ_VIRI = 1 + PGK_NHF,      /* + PGK_HF_* class: hfviri*_atm             */
+  PGK_HFJ_TOT = 1 + 2 * PGK_NHF,   /* hftot_atm                                 */
+  PGK_NHFJ
+};
+
 /* Per-atom heat-flux outputs: PGK_NHF classes, each natom energies and a
- * Fortran (3,3,natom,nregion) virial array.                                   */
+ * Fortran (3,3,natom,nregion) virial array.  Either pointer (or hfo itself
+ * in pgk_calforce_hf) may be NULL: that array then stays on the device only,
+ * for pgk_calheatf.                                                          */
 typedef struct pgk_hf_out {
   double *pot_atm;             /* [PGK_NHF*natom]                               */
   double *viri_atm;            /* [PGK_NHF*9*natom*nregion]                     */
@@ -227,6 +239,19 @@ int pgk_calforce(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
 int pgk_calforce_hf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
                     const pgk_hf_desc *hf, pgk_force_out *out,
                     pgk_hf_out *hfo);
+
+/* calheatf (calheatf.F90:8-1470) on the device, over the per-atom arrays and
+ * the region description of the last pgk_calforce_hf call (which may skip
+ * copying them out, see pgk_hf_out): per region r the kinetic and potential
+ * transport sum_{centre_i in r} e_i v_i (molecule centre and centre-of-mass
+ * velocity for MOLE atoms), the collision part sum_i sum_a W(a,n,i,r) v_i(a),
+ * each divided by the region volume Lx Ly (z2 - z1), and their total.
+ * Volume mode: z components only (the others are 0); bulk mode: all three.
+ * vel [3*natom] and mass [natom] in units where m v^2 / 2 is in the energy
+ * unit of the force path (the reference's non-dimensional atmvel, atmmass);
+ * mass NULL reuses the last masses given.  hf [nregion*PGK_NHFJ*3].  The
+ * surface mode keeps state across steps and is not offered.                 */
+int pgk_calheatf(pgk_ctx *ctx, const double *vel, const double *mass, double *hf);
 
 /* Device-resident variant used by GPU-resident drivers: same as pgk_calforce
  * but the force stays on the device (d_force, [3*natom] doubles, may be NULL)

@@ -1323,3 +1323,93 @@ int pgko_calforce_hf(pgko_sys *s, int32_t mts_flag, const pgk_mts *mts,
   }
   return drive(s, mts_flag, mts, 1, 1, &h, out);
 }
+
+/* ---- calheatf (calheatf.F90:8-1470) -------------------------------------
+ * Volume-based (ifhfvol) and bulk forms.  Centre of atom i: the atom
+ * (ATOM) or its molecule's centre of mass (MOLE), z wrapped once into
+ * [0, Lz) (calheatf.F90:650-655); velocity of the centre: the atom's, or
+ * the molecule's mass-weighted mean over its atoms in index order
+ * (calheatf.F90:548-570).  Transport part (:615-870), atoms outer, regions
+ * inner: for z1 <= z <= z2, hfkin += (m v^2 / 2) v_c and hfpot_c += e_c v_c.
+ * Collision part (:1027-1330), regions outer, atoms inner:
+ * hfviri_c(n) += W(1,n) v1 + W(2,n) v2 + W(3,n) v3.  Only the z component
+ * outside the bulk form.  Each term times 1/(Lx Ly (z2 - z1)) (:1334-1380),
+ * then hftot = the terms in the reference's order (:1420-1436; the classes
+ * the force path does not have are zero). */
+int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
+                  const double *mass, const double *pot_atm,
+                  const double *viri_atm, double *hfout) {
+  const int n = s->natom, nreg = hf->nregion;
+  const int bulk = hf->mode == PGK_HF_BULK;
+  if (hf->mode == PGK_HF_SURFACE) return PGK_EINVAL;
+  int nmol = 0;
+  for (int i = 0; i < n; i++)
+    if (s->molid[i] > nmol) nmol = s->molid[i];
+  double *mv = (double *)calloc((size_t)3 * (nmol + 1), sizeof(double));
+  double *mm = (double *)calloc((size_t)nmol + 1, sizeof(double));
+  for (int i = 0; i < n; i++) {
+    const int m = s->molid[i];
+    mm[m] = mm[m] + mass[i];
+    for (int d = 0; d < 3; d++) mv[3 * m + d] = mv[3 * m + d] + mass[i] * vel[3 * i + d];
+  }
+  for (int m = 1; m <= nmol; m++) {
+    const double inv = 1.0 / mm[m];
+    for (int d = 0; d < 3; d++) mv[3 * m + d] = mv[3 * m + d] * inv;
+  }
+  const int c0 = bulk ? 0 : 2;
+  double *acc = (double *)calloc((size_t)nreg * PGK_NHFJ * 3, sizeof(double));
+#define ACC(r, t, c) acc[((size_t)(r) * PGK_NHFJ + (t)) * 3 + (c)]
+  for (int i = 0; i < n; i++) {
+    double cz;
+    const double *pv;
+    if (hf->hftyp[i] == PGK_HFTYP_MOLE) {
+      cz = hf->molecom[3 * (s->molid[i] - 1) + 2];
+      pv = mv + 3 * s->molid[i];
+    } else {
+      cz = s->pos[3 * i + 2];
+      pv = vel + 3 * i;
+    }
+    if (cz < 0.0) cz = cz + s->box[2];
+    else if (cz >= s->box[2]) cz = cz - s->box[2];
+    const double *v = vel + 3 * i;
+    for (int r = 0; r < nreg; r++) {
+      if (!(hf->z1[r] <= cz && cz <= hf->z2[r])) continue;
+      const double ke = 0.5 * mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int c = c0; c < 3; c++) {
+        ACC(r, PGK_HFJ_KIN, c) = ACC(r, PGK_HFJ_KIN, c) + ke * pv[c];
+        for (int k = 0; k < PGK_NHF; k++)
+          ACC(r, PGK_HFJ_POT + k, c) =
+              ACC(r, PGK_HFJ_POT + k, c) + pot_atm[(size_t)k * n + i] * pv[c];
+      }
+    }
+  }
+  for (int r = 0; r < nreg; r++)
+    for (int i = 0; i < n; i++) {
+      const double *v = vel + 3 * i;
+      for (int k = 0; k < PGK_NHF; k++) {
+        /* Fortran (3,3,natom,nreg) per class: W(a,n) at a + 3n */
+        const double *w = viri_atm + (((size_t)k * nreg + r) * n + i) * 9;
+        for (int c = c0; c < 3; c++)
+          ACC(r, PGK_HFJ_VIRI + k, c) = ACC(r, PGK_HFJ_VIRI + k, c) + w[3 * c] * v[0] +
+                                        w[1 + 3 * c] * v[1] + w[2 + 3 * c] * v[2];
+      }
+    }
+  const double area_inv = (1.0 / s->box[0]) * (1.0 / s->box[1]);
+  for (int r = 0; r < nreg; r++) {
+    const double vol_inv = area_inv / (hf->z2[r] - hf->z1[r]);
+    for (int c = 0; c < 3; c++) {
+      double tot = 0.0;
+      for (int t = 0; t < PGK_HFJ_TOT; t++) {
+        ACC(r, t, c) = ACC(r, t, c) * vol_inv;
+        tot = t == 0 ? ACC(r, t, c) : tot + ACC(r, t, c);
+      }
+      ACC(r, PGK_HFJ_TOT, c) = tot;
+    }
+  }
+#undef ACC
+  memcpy(hfout, acc, sizeof(double) * (size_t)nreg * PGK_NHFJ * 3);
+  free(acc);
+  free(mv);
+  free(mm);
+  return PGK_OK;
+}

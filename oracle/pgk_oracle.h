@@ -40,6 +40,12 @@ int pgko_calforce_hf(pgko_sys *s, int32_t mts_flag, const pgk_mts *mts,
                      const pgk_hf_desc *hf, pgk_force_out *out,
                      pgk_hf_out *hfo);
 
+/* calheatf (calheatf.F90:8-1470), volume or bulk mode, over per-atom arrays
+ * in the pgk_hf_out layout: hf[nregion*PGK_NHFJ*3] (see pgk_calheatf). */
+int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
+                  const double *mass, const double *pot_atm,
+                  const double *viri_atm, double *hfout);
+
 /* spline tables [16*nspltbl] (same column order as the reference harness),
  * B-spline moduli [n1+n2+n3], rkcut */
 void pgko_get_tables(const pgko_sys *s, double *tbl, double *bsp,

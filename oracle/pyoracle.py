@@ -65,6 +65,8 @@ def _oracle_lib():
                                        C.POINTER(abi.HfDesc), C.POINTER(abi.ForceOut),
                                        C.POINTER(abi.HfOut)]
         L.pgko_get_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pgko_calheatf.restype = C.c_int
+        L.pgko_calheatf.argtypes = [C.c_void_p, C.POINTER(abi.HfDesc)] + [C.c_void_p] * 5
         L.pgko_fft3d.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         _lib = L
     return _lib
@@ -211,6 +213,21 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle calforce_hf failed: {rc}")
         return o.result(hf=True)
+
+    def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
+        """calheatf over calforce_hf's per-atom arrays (as returned by
+        calforce_hf); returns hf[nregion, PGK_NHFJ, 3]."""
+        hs, keep = _hf_struct(self.sysd, hf)
+        v = np.ascontiguousarray(vel, np.float64)
+        m = np.ascontiguousarray(mass, np.float64)
+        pe = np.ascontiguousarray(pot_atm, np.float64)
+        wv = np.ascontiguousarray(viri_atm, np.float64)
+        out = np.zeros((hs.nregion, abi.NHFJ, 3))
+        rc = self.L.pgko_calheatf(self.h, C.byref(hs), v.ctypes.data, m.ctypes.data,
+                                  pe.ctypes.data, wv.ctypes.data, out.ctypes.data)
+        if rc:
+            raise RuntimeError(f"oracle calheatf failed: {rc}")
+        return out
 
 
 def mkexcl(sysd):
@@ -416,3 +433,27 @@ class Reference:
                         C.c_void_p(ene.ctypes.data), C.c_void_p(vir.ctypes.data), C.c_void_p(pe.ctypes.data), C.c_void_p(wv.ctypes.data))
         return dict(force=f, ene=ene, virt=vir.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1),
                     pot_atm=pe, viri_atm=wv.reshape(abi.NHF, nreg, self.n, 3, 3))
+
+    def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
+        """The reference's calheatf (volume-based) on the given per-atom
+        arrays; returns hf[nregion, PGK_NHFJ, 3]."""
+        z1 = np.ascontiguousarray(hf["z1"], np.float64)
+        z2 = np.ascontiguousarray(hf["z2"], np.float64)
+        nreg = len(z1)
+        if int(hf.get("mode", abi.HF_VOLUME)) != abi.HF_VOLUME:
+            raise ValueError("the compiled reference's calheatf is the volume-based one")
+        ht = np.ascontiguousarray(hf.get("hftyp", np.ones(self.n, np.int32)), np.int32)
+        mc = np.asfortranarray(np.asarray(hf.get("molecom", np.zeros((self.sysd["nmol"], 3))),
+                                          np.float64).T)
+        v = np.ascontiguousarray(vel, np.float64)
+        m = np.ascontiguousarray(mass, np.float64)
+        pe = np.ascontiguousarray(pot_atm, np.float64)
+        wv = np.ascontiguousarray(viri_atm, np.float64)
+        out = np.zeros((nreg, abi.NHFJ, 3))
+        _big_stack_call(self.L.pgkref_calheatf, C.c_int32(nreg), C.c_void_p(z1.ctypes.data),
+                        C.c_void_p(z2.ctypes.data), C.c_void_p(ht.ctypes.data),
+                        C.c_int32(mc.shape[1]), C.c_void_p(mc.ctypes.data),
+                        C.c_void_p(m.ctypes.data), C.c_void_p(v.ctypes.data),
+                        C.c_void_p(pe.ctypes.data), C.c_void_p(wv.ctypes.data),
+                        C.c_void_p(out.ctypes.data))
+        return out

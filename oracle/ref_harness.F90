@@ -26,7 +26,7 @@ module pgkref_harness
   use spme_global
   use mpi_global
   use interface_mdtech, only: calforce, calforce_hf, mklist2a, mklist_cell, &
-       &                      erf_corr_cutoff
+       &                      erf_corr_cutoff, calheatf
   use interface_preproc, only: MPI_loopinit, malloc_larray, mkexcl, &
        &                       spline_interp_init, fft_pme_init
   implicit none
@@ -418,5 +418,89 @@ contains
     wv(:,:,:,:,1:8) = w(:,:,1:natom,1:nreg,1:8)
     deallocate(f, fv, fvx, pa, pax, w, wx)
   end subroutine pgkref_calforce_hf
+
+  ! One call of calheatf (calheatf.F90:8-1470), volume-based (ifhfvol), on
+  ! per-atom arrays in pgkref_calforce_hf's layout, all on the long level.
+  ! Every molecule is passed as a "poly" molecule (its centre-of-mass
+  ! velocity is mass weighted over its atoms, calheatf.F90:548-570).  The
+  ! routine stops unless every atom type is 110, 111 or 16
+  ! (calheatf.F90:620-630), so the call runs with atmindex set to 16 and
+  ! restores it afterwards.
+  ! hf(3,18,nreg): kin, pot bo an to 14 elin ljin elc lj, viri (same
+  ! order), tot.
+  subroutine pgkref_calheatf(nreg, z1, z2, hftyp, nmol, molecom, mass, vel, &
+       &                     pe, wv, hf) bind(C, name='pgkref_calheatf')
+    integer(c_int), value :: nreg, nmol
+    integer(c_int), intent(in) :: hftyp(*)
+    real(c_double), intent(in) :: z1(nreg), z2(nreg), molecom(3,nmol)
+    real(c_double), intent(in) :: mass(natom), vel(3,natom)
+    real(c_double), intent(in) :: pe(natom,8), wv(3,3,natom,nreg,8)
+    real(c_double), intent(out) :: hf(3,18,nreg)
+
+    integer, parameter :: nex = 3
+    real(8), allocatable :: z(:), zx(:,:), w0(:,:,:,:), wx(:,:,:,:,:)
+    real(8), allocatable :: hp(:,:,:), hx(:,:,:), hv(:,:,:), hz(:,:)
+    real(8), allocatable :: hkx(:,:,:), hpx(:,:,:), hvx(:,:,:), tot(:,:)
+    real(8), allocatable :: cor_hf(:,:), com_hf(:,:)
+    integer, allocatable :: typ_save(:)
+    integer :: i, im, k, c
+
+    allocate(z(natom), zx(natom,nex), w0(3,3,natom,nreg), wx(3,3,natom,nreg,nex))
+    allocate(hp(3,nreg,8), hv(3,nreg,8), hz(3,nreg), hx(3,nreg,10))
+    allocate(hkx(3,nreg,nex), hpx(3,nreg,nex), hvx(3,nreg,nex), tot(3,nreg))
+    allocate(cor_hf(3,natom), com_hf(3,max(nmol,1)), typ_save(natom))
+    z = 0.0d0
+    zx = 0.0d0
+    w0 = 0.0d0
+    wx = 0.0d0
+    cor_hf = 0.0d0
+    com_hf = 0.0d0
+    atmmass(1:natom) = mass(1:natom)
+    atmvel(1:3,1:natom) = vel(1:3,1:natom)
+    atmvel_strm(1:3,1:natom) = 0.0d0
+    ! molept lists: the atoms of each molecule in index order
+    nmole = maxval(irmolept_list(1:natom))
+    k = 1
+    do im = 1, nmole
+       molept_index(im) = k
+       do i = 1, natom
+          if (irmolept_list(i) == im) then
+             molept_list(k) = i
+             k = k + 1
+          end if
+       end do
+    end do
+    molept_index(nmole+1) = k
+    typ_save(1:natom) = atmindex(1:natom)
+    atmindex(1:natom) = 16
+    call calheatf(nmole, 0, 0, box_s(1), box_s(2), box_s(3), .true., &
+         & nreg, z1, z2, hftyp(1:natom), molecom, &
+         & pe(:,1), z, z, pe(:,2), z, z, pe(:,3), z, z, pe(:,4), z, z, &
+         & pe(:,5), z, z, pe(:,6), z, z, pe(:,7), z, z, pe(:,8), z, z, &
+         & z, z, z, z, z, z, z, z, z, z, z, z, z, z, z, zx, zx, zx, &
+         & wv(:,:,:,:,1), w0, w0, wv(:,:,:,:,2), w0, w0, wv(:,:,:,:,3), w0, w0, &
+         & wv(:,:,:,:,4), w0, w0, wv(:,:,:,:,5), w0, w0, wv(:,:,:,:,6), w0, w0, &
+         & wv(:,:,:,:,7), w0, w0, wv(:,:,:,:,8), w0, w0, &
+         & w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, w0, &
+         & wx, wx, wx, &
+         & hz, hkx, &
+         & hp(:,:,1), hp(:,:,2), hp(:,:,3), hp(:,:,4), hp(:,:,5), hp(:,:,6), &
+         & hp(:,:,7), hp(:,:,8), hx(:,:,1), hx(:,:,2), hx(:,:,3), hx(:,:,4), &
+         & hx(:,:,5), hpx, &
+         & hv(:,:,1), hv(:,:,2), hv(:,:,3), hv(:,:,4), hv(:,:,5), hv(:,:,6), &
+         & hv(:,:,7), hv(:,:,8), hx(:,:,6), hx(:,:,7), hx(:,:,8), hx(:,:,9), &
+         & hx(:,:,10), hvx, &
+         & nex, tot, cor_hf, com_hf, 1.0d0, 1, 2)
+    atmindex(1:natom) = typ_save(1:natom)
+    do k = 1, nreg
+       hf(:,1,k) = hz(:,k)
+       do c = 1, 8
+          hf(:,1+c,k) = hp(:,k,c)
+          hf(:,9+c,k) = hv(:,k,c)
+       end do
+       hf(:,18,k) = tot(:,k)
+    end do
+    deallocate(z, zx, w0, wx, hp, hv, hz, hx, hkx, hpx, hvx, tot, cor_hf, com_hf, typ_save)
+  end subroutine pgkref_calheatf
 
 end module pgkref_harness

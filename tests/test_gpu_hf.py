@@ -119,3 +119,68 @@ def test_calforce_hf_rejects_bad_input():
         c.calforce_hf(dict(mode=abi.HF_VOLUME, z1=z1, z2=z2, hftyp=bad))
     with pytest.raises(PgkError):
         c.calforce_hf(dict(mode=7, z1=z1, z2=z2))
+
+
+# ---- calheatf on the device ------------------------------------------------
+
+def _hf_centres(s):
+    hftyp = np.ones(s["natom"], np.int32)
+    hftyp[s["atype"] == systems.TYPE_INDEX["OB"]] = abi.HFTYP_MOLE
+    hftyp[s["atype"] == systems.TYPE_INDEX["HB"]] = abi.HFTYP_MOLE
+    return hftyp
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", [abi.HF_VOLUME, abi.HF_BULK])
+def test_calheatf_matches_oracle(oracle_mod, prec, mode):
+    """The device reduction against the oracle's calheatf on the same per-atom
+    arrays (the GPU's own: FP64 sums in another order, 1e-12 of the largest
+    term), then end to end against the oracle's arrays (FP64: the
+    calforce_hf bar; mixed: 1e-4 of each term's scale).  Regions overlap,
+    water atoms use molecule centres."""
+    s = small_slab("rb")
+    lz = s["box"][2]
+    hf = dict(mode=mode, z1=np.array([0.0, 0.3 * lz, 0.45 * lz, 0.2 * lz]),
+              z2=np.array([0.5 * lz, 0.6 * lz, lz, 0.7 * lz]), hftyp=_hf_centres(s),
+              molecom=molecom(s))
+    ref, got, c = _run(oracle_mod, s, hf, prec)
+    rng = np.random.default_rng(17)
+    vel = rng.normal(scale=0.02, size=(s["natom"], 3))
+    mass = s["mass"] * 0.37
+    o = oracle_mod.Oracle(s)
+    j = c.calheatf(vel, mass)
+    same = o.calheatf(hf, vel, mass, got["pot_atm"], got["viri_atm"].reshape(abi.NHF, 4, -1, 9))
+    scale = np.max(np.abs(same))
+    assert scale > 0
+    assert np.max(np.abs(j - same)) <= 1e-12 * scale
+    full = o.calheatf(hf, vel, mass, ref["pot_atm"], ref["viri_atm"].reshape(abi.NHF, 4, -1, 9))
+    tol = 1e-10 if prec == "fp64" else 1e-4
+    for t in range(abi.NHFJ):
+        ts = np.max(np.abs(full[:, t]))
+        assert np.max(np.abs(j[:, t] - full[:, t])) <= tol * max(ts, 1e-3 * scale), abi.HFJ_NAMES[t]
+    if mode == abi.HF_VOLUME:
+        assert np.all(j[:, :, :2] == 0.0)
+    # the per-atom arrays left on the device give the same flux, bit for bit,
+    # and masses are kept between calls
+    c.calforce_hf(hf, per_atom=False)
+    assert np.array_equal(c.calheatf(vel), j) if prec == "mixed" else \
+        np.max(np.abs(c.calheatf(vel) - j)) <= 1e-12 * scale
+
+
+def test_calheatf_order_of_calls():
+    from thrid_amd.pgk import ForceContext, PgkError
+    s = hf_slab()
+    c = ForceContext(s, "fp64")
+    c.mklist(s["rcut_book"])
+    vel = np.zeros((s["natom"], 3))
+    with pytest.raises(PgkError):
+        c.calheatf(vel, s["mass"])            # no calforce_hf yet
+    z1, z2 = regions(s["box"][2], 2)
+    c.calforce_hf(dict(mode=abi.HF_SURFACE, z1=z1, z2=z2), per_atom=False)
+    with pytest.raises(PgkError):
+        c.calheatf(vel, s["mass"])            # surface form not offered
+    c.calforce_hf(dict(mode=abi.HF_VOLUME, z1=z1, z2=z2), per_atom=False)
+    assert np.all(c.calheatf(vel, s["mass"]) == 0.0)   # atoms at rest carry nothing
+    c.set_positions(s["pos"])
+    with pytest.raises(PgkError):
+        c.calheatf(vel)                       # positions changed since the call

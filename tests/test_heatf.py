@@ -1,0 +1,101 @@
+"""calheatf's restatement in the oracle (oracle/pgk_oracle.c pgko_calheatf),
+pinned against the reference's own calheatf where it builds and against
+identities the heat flux must satisfy: in the bulk form with one uniform
+velocity the collision part is that velocity times the pair virial tensor
+that calforce computes on its own path, and slabs tiling the box add up to
+the bulk form."""
+import numpy as np
+import pytest
+
+from thrid_amd import abi, systems
+from helpers import small_slab
+
+
+def _molecom(s):
+    m = s["mass"]
+    com = np.zeros((int(s["nmol"]), 3))
+    tot = np.zeros(int(s["nmol"]))
+    np.add.at(com, s["molid"] - 1, s["pos"] * m[:, None])
+    np.add.at(tot, s["molid"] - 1, m)
+    return com / tot[:, None]
+
+
+def _mixed_centres(s):
+    ht = np.full(s["natom"], abi.HFTYP_ATOM, np.int32)
+    ht[s["atype"] == systems.TYPE_INDEX["OB"]] = abi.HFTYP_MOLE
+    ht[s["atype"] == systems.TYPE_INDEX["HB"]] = abi.HFTYP_MOLE
+    return ht
+
+
+def _velocities(s, seed=3):
+    rng = np.random.default_rng(seed)
+    return rng.normal(scale=0.02, size=(s["natom"], 3))
+
+
+def test_oracle_calheatf_matches_reference(oracle_mod):
+    if not oracle_mod.reference_available():
+        pytest.skip("reference library not built (oracle/build_ref.sh)")
+    s = small_slab("rb")
+    lz = s["box"][2]
+    hf = dict(mode=abi.HF_VOLUME, z1=np.array([0.0, 0.3 * lz, 0.45 * lz]),
+              z2=np.array([0.5 * lz, 0.6 * lz, lz]), hftyp=_mixed_centres(s),
+              molecom=_molecom(s))
+    ref = oracle_mod.Reference(s)
+    ref.mklist(s["rcut_book"], 1)
+    r = ref.calforce_hf(hf)
+    vel = _velocities(s)
+    mass = s["mass"] * 0.37
+    a = ref.calheatf(hf, vel, mass, r["pot_atm"], r["viri_atm"])
+    o = oracle_mod.Oracle(s)
+    b = o.calheatf(hf, vel, mass, r["pot_atm"], r["viri_atm"])
+    # every term present, and the restatement's arithmetic order is the reference's
+    assert np.all(np.abs(a[:, [abi.HFJ_KIN, abi.HFJ_POT + abi.HF_NAMES.index("bo"),
+                                abi.HFJ_VIRI + abi.HF_NAMES.index("vdw")], 2]) > 0)
+    np.testing.assert_array_equal(a, b)
+
+
+def _oracle_hf(oracle_mod, s, hf):
+    o = oracle_mod.Oracle(s)
+    o.mklist(s["rcut_book"], 1)
+    return o, o.calforce_hf(hf)
+
+
+def test_bulk_collision_part_is_the_pair_virial(oracle_mod):
+    s = small_slab("rb")
+    L = s["box"]
+    hf = dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([L[2]]))
+    o, r = _oracle_hf(oracle_mod, s, hf)
+    u = np.array([0.013, -0.021, 0.008])
+    vel = np.tile(u, (s["natom"], 1))
+    j = o.calheatf(hf, vel, s["mass"], r["pot_atm"], r["viri_atm"])[0]
+    lj = j[abi.HFJ_VIRI + abi.HF_NAMES.index("vdw")] + j[abi.HFJ_VIRI + abi.HF_NAMES.index("ljin")]
+    t = r["virt"][abi.TENS_NAMES.index("pot_virit_lj")]      # (force comp, r comp)
+    expect = u @ t / (L[0] * L[1] * L[2])
+    assert np.max(np.abs(expect)) > 0
+    np.testing.assert_allclose(lj, expect, rtol=1e-10, atol=1e-12 * np.max(np.abs(expect)))
+    # the transport part with one velocity: (sum of energies) u / V
+    ke = 0.5 * (s["mass"] * (vel ** 2).sum(1)).sum()
+    np.testing.assert_allclose(j[abi.HFJ_KIN], ke * u / (L[0] * L[1] * L[2]), rtol=1e-12)
+    for k, name in enumerate(abi.HF_NAMES):
+        np.testing.assert_allclose(j[abi.HFJ_POT + k], r["pot_atm"][k].sum() * u /
+                                   (L[0] * L[1] * L[2]), rtol=1e-11, atol=1e-30)
+
+
+def test_slabs_tiling_the_box_add_up_to_the_bulk_form(oracle_mod):
+    """No pair of the slab straddles the periodic z boundary, so the
+    z-fragment weights of slabs covering [0, Lz] add up to the full dz."""
+    s = small_slab("rb")
+    L = s["box"]
+    vel = _velocities(s, seed=5)
+    edges = np.linspace(0.0, L[2], 7)
+    hv = dict(mode=abi.HF_VOLUME, z1=edges[:-1], z2=edges[1:])
+    hb = dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([L[2]]))
+    o, rv = _oracle_hf(oracle_mod, s, hv)
+    jv = o.calheatf(hv, vel, s["mass"], rv["pot_atm"], rv["viri_atm"])
+    o, rb = _oracle_hf(oracle_mod, s, hb)
+    jb = o.calheatf(hb, vel, s["mass"], rb["pot_atm"], rb["viri_atm"])[0]
+    summed = (jv[:, :, 2] * np.diff(edges)[:, None]).sum(0) / L[2]
+    scale = np.max(np.abs(jb[:, 2]))
+    assert np.max(np.abs(summed - jb[:, 2])) < 1e-10 * scale
+    # the volume form has z components only
+    assert np.all(jv[:, :, :2] == 0.0)

@@ -27,6 +27,10 @@ TENS_NAMES = ["pot_virit_coul", "pot_virit_lj", "atm_virit_bond", "atm_virit_ang
 NTENS = len(TENS_NAMES)
 HF_NAMES = ["bo", "an", "to", "14", "elin", "ljin", "elc", "vdw"]
 NHF = len(HF_NAMES)
+# calheatf output terms: hf[region, term, comp]
+HFJ_NAMES = ["kin"] + ["pot_" + k for k in HF_NAMES] + ["viri_" + k for k in HF_NAMES] + ["tot"]
+NHFJ = len(HFJ_NAMES)
+HFJ_KIN, HFJ_POT, HFJ_VIRI, HFJ_TOT = 0, 1, 1 + NHF, 1 + 2 * NHF
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

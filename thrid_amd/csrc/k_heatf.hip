@@ -1,0 +1,214 @@
+// k_heatf.hip - calheatf (calheatf.F90:8-1470) on the device: the heat flux
+// of each z region from the per-atom energies and virial arrays that the
+// heat-flux pass of calforce_hf leaves in HBM (k_hf.hip, pgk_hf.h layout).
+//
+// Per region r, with the centre of atom i (the atom, or its molecule's
+// centre of mass for hftyp MOLE, z wrapped once into [0, Lz)) and the
+// velocity of that centre (the atom's, or the molecule's mass-weighted mean):
+//   transport   sum_{z1 <= z_i <= z2} (m_i v_i^2 / 2) vc_i   and   e_c(i) vc_i
+//   collision   sum_i sum_a W_c(a, n, i, r) v_i(a)
+// each times 1 / (Lx Ly (z2 - z1)), and their total in the reference's order.
+// The volume form has only z components (W holds column n = 3); the bulk
+// form all three.
+//
+// Every sum is deterministic: the atoms are cut into a fixed number of
+// chunks; one block reduces one (region or class-region row, chunk) pair
+// with a fixed-order tree into a partial record, and one thread per
+// (region, component) adds the partials in chunk order.  The collision part
+// streams the W rows once (HBM-bound: 8 nw bytes per atom per class-region
+// row); the transport part re-reads the atom data once per region.
+#include "pgk_hf.h"
+
+namespace {
+
+constexpr int kNterm = 1 + PGK_NHF;   // kinetic + potential classes
+
+struct HeatfArgs {
+  int natom, nreg, nw, bulk, nsplit, chunk;
+  const double *z1, *z2;
+  const int32_t *hftyp, *molid;
+  const double *molecom, *pos, *vel, *mass, *molevel;
+  const void *pe, *w;
+  double L[3];
+};
+
+// centre-of-mass velocity of each molecule, summed over its atoms in index
+// order (calheatf.F90:548-570); molecules are 0-based here
+__global__ __launch_bounds__(256) void k_molevel(int nmol, const int32_t *__restrict__ start,
+                                                 const int32_t *__restrict__ atoms,
+                                                 const double *__restrict__ mass,
+                                                 const double *__restrict__ vel,
+                                                 double *__restrict__ molevel) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  double mm = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  for (int k = start[m]; k < start[m + 1]; k++) {
+    const int i = atoms[k];
+    mm = mm + mass[i];
+    p0 = p0 + mass[i] * vel[3 * i];
+    p1 = p1 + mass[i] * vel[3 * i + 1];
+    p2 = p2 + mass[i] * vel[3 * i + 2];
+  }
+  const double inv = 1.0 / mm;
+  molevel[3 * m] = p0 * inv;
+  molevel[3 * m + 1] = p1 * inv;
+  molevel[3 * m + 2] = p2 * inv;
+}
+
+// transport part: block (chunk, region); record [kNterm][3]
+template <class AH>
+__global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__restrict__ part) {
+  const int r = blockIdx.y, split = blockIdx.x;
+  const int i0 = split * a.chunk;
+  const int i1 = min(a.natom, i0 + a.chunk);
+  const double z1 = a.z1[r], z2 = a.z2[r];
+  const AH *pe = reinterpret_cast<const AH *>(a.pe);
+  double acc[kNterm * 3];
+#pragma unroll
+  for (int t = 0; t < kNterm * 3; t++) acc[t] = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    double cz;
+    const double *pv;
+    if (a.hftyp[i] == PGK_HFTYP_MOLE) {
+      const int m = a.molid[i] - 1;
+      cz = a.molecom[3 * m + 2];
+      pv = a.molevel + 3 * m;
+    } else {
+      cz = a.pos[3 * i + 2];
+      pv = a.vel + 3 * i;
+    }
+    if (cz < 0.0) cz = cz + a.L[2];
+    else if (cz >= a.L[2]) cz = cz - a.L[2];
+    if (!(z1 <= cz && cz <= z2)) continue;
+    const double *v = a.vel + 3 * i;
+    const double ke = 0.5 * a.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double vc[3] = {a.bulk ? pv[0] : 0.0, a.bulk ? pv[1] : 0.0, pv[2]};
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += ke * vc[c];
+#pragma unroll
+    for (int k = 0; k < PGK_NHF; k++) {
+      const double e = acc_load(&pe[(size_t)k * a.natom + i]);
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[3 * (1 + k) + c] += e * vc[c];
+    }
+  }
+  block_record<kNterm * 3, 4>(acc, part + ((size_t)r * a.nsplit + split) * (kNterm * 3));
+}
+
+// collision part: block (chunk, class-region row); record [3]
+template <class AH>
+__global__ __launch_bounds__(256) void k_heatf_collision(HeatfArgs a, double *__restrict__ part) {
+  const int row = blockIdx.y, split = blockIdx.x;
+  const int i0 = split * a.chunk;
+  const int i1 = min(a.natom, i0 + a.chunk);
+  const AH *W = reinterpret_cast<const AH *>(a.w) + (size_t)row * a.natom * a.nw;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    const double v0 = a.vel[3 * i], v1 = a.vel[3 * i + 1], v2 = a.vel[3 * i + 2];
+    const AH *w = W + (size_t)i * a.nw;
+    if (a.nw == 9) {
+#pragma unroll
+      for (int n = 0; n < 3; n++)
+        acc[n] += acc_load(&w[3 * n]) * v0 + acc_load(&w[3 * n + 1]) * v1 +
+                  acc_load(&w[3 * n + 2]) * v2;
+    } else {
+      acc[2] += acc_load(&w[0]) * v0 + acc_load(&w[1]) * v1 + acc_load(&w[2]) * v2;
+    }
+  }
+  block_record<3, 4>(acc, part + ((size_t)row * a.nsplit + split) * 3);
+}
+
+// partials in chunk order, the volume factor, the total (calheatf.F90:1334-1436)
+__global__ __launch_bounds__(64) void k_heatf_final(HeatfArgs a, const double *__restrict__ ptr,
+                                                    const double *__restrict__ pcol,
+                                                    double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nreg * 3) return;
+  const int r = idx / 3, c = idx % 3;
+  const double vol_inv = ((1.0 / a.L[0]) * (1.0 / a.L[1])) / (a.z2[r] - a.z1[r]);
+  double *o = out + (size_t)r * PGK_NHFJ * 3;
+  double tot = 0.0;
+  for (int t = 0; t < PGK_HFJ_TOT; t++) {
+    double s = 0.0;
+    if (t < kNterm) {
+      for (int k = 0; k < a.nsplit; k++)
+        s += ptr[((size_t)r * a.nsplit + k) * (kNterm * 3) + 3 * t + c];
+    } else {
+      const size_t row = (size_t)(t - kNterm) * a.nreg + r;
+      for (int k = 0; k < a.nsplit; k++) s += pcol[(row * a.nsplit + k) * 3 + c];
+    }
+    s = s * vol_inv;
+    o[3 * t + c] = s;
+    tot = (t == 0) ? s : tot + s;
+  }
+  o[3 * PGK_HFJ_TOT + c] = tot;
+}
+
+template <int PREC>
+int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
+  using AH = typename HfAcc<PREC>::type;
+  hipStream_t st = c->stream;
+  k_heatf_transport<AH><<<dim3(a.nsplit, a.nreg), 256, 0, st>>>(a, ptr);
+  HIP_CHECK(hipGetLastError());
+  k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+}  // namespace
+
+// Queue calheatf over the context's heat-flux arrays; velocities and masses
+// are already in d_vel / d_mass (device, natom entries), the result goes to
+// c->d_hj_out.
+int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
+  const HfDev &h = c->hfdev;
+  const int n = c->natom;
+  HeatfArgs a;
+  a.natom = n;
+  a.nreg = h.nreg;
+  a.nw = h.nw;
+  a.bulk = h.mode == PGK_HF_BULK;
+  // chunks: about 2048 collision blocks, at least 256 atoms each
+  const int rows = PGK_NHF * h.nreg;
+  int nsplit = (2048 + rows - 1) / rows;
+  nsplit = std::max(1, std::min(nsplit, (n + 255) / 256));
+  a.nsplit = nsplit;
+  a.chunk = (n + nsplit - 1) / nsplit;
+  a.z1 = h.z1;
+  a.z2 = h.z2;
+  a.hftyp = h.hftyp;
+  a.molid = h.molid;
+  a.molecom = h.molecom;
+  a.pos = h.pos;
+  a.vel = d_vel;
+  a.mass = d_mass;
+  a.molevel = c->d_hj_molevel;
+  a.pe = h.pe;
+  a.w = h.w;
+  for (int d = 0; d < 3; d++) a.L[d] = h.L[d];
+  const size_t ntr = (size_t)h.nreg * nsplit * kNterm * 3;
+  const size_t ncol = (size_t)rows * nsplit * 3;
+  int rc;
+  if ((rc = grow_dev((void **)&c->d_hj_part, &c->hj_part_cap, sizeof(double) * (ntr + ncol))) ||
+      (rc = grow_dev((void **)&c->d_hj_out, &c->hj_out_cap,
+                     sizeof(double) * (size_t)h.nreg * PGK_NHFJ * 3)))
+    return rc;
+  if (h.molecom) {
+    k_molevel<<<pgk_grid_blocks(c->hj_nmol, 256), 256, 0, c->stream>>>(
+        c->hj_nmol, c->d_hj_mol_start, c->d_hj_mol_atoms, d_mass, d_vel, c->d_hj_molevel);
+    HIP_CHECK(hipGetLastError());
+  }
+  double *ptr = c->d_hj_part, *pcol = c->d_hj_part + ntr;
+  switch (c->precision) {
+    case PGK_PREC_MIXED: rc = launch_heatf<PGK_PREC_MIXED>(c, a, ptr, pcol); break;
+    case PGK_PREC_FP64: rc = launch_heatf<PGK_PREC_FP64>(c, a, ptr, pcol); break;
+    default: rc = launch_heatf<PGK_PREC_FP32>(c, a, ptr, pcol); break;
+  }
+  if (rc) return rc;
+  k_heatf_final<<<pgk_grid_blocks(h.nreg * 3, 64), 64, 0, c->stream>>>(a, ptr, pcol,
+                                                                        c->d_hj_out);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}

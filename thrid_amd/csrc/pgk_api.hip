@@ -357,7 +357,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
                                hipMemcpyDeviceToHost, st));
   }
-  if (hf && (rc = pgk_hf_export(c))) return rc;
+  if (hf && c->hf_export && (rc = pgk_hf_export(c))) return rc;
   c->last_nseg = nseg;
   for (int k = 0; k < nseg; k++) c->last_seg_kind[k] = seg_kind[k];
   c->last_mts_flag = mts_flag;
@@ -870,6 +870,7 @@ int pgk_set_positions(pgk_ctx *c, const double *pos) {
                            c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->have_pos = true;
+  c->hf_ready = false;   // calheatf pairs the arrays with the positions of their call
   return PGK_OK;
 }
 
@@ -883,6 +884,7 @@ int pgk_set_positions_device(pgk_ctx *c, const double *d_pos) {
   HIP_CHECK(hipMemcpyAsync(c->d_pos, d_pos, sizeof(double) * 3 * c->natom,
                            hipMemcpyDeviceToDevice, c->stream));
   c->have_pos = true;
+  c->hf_ready = false;   // calheatf pairs the arrays with the positions of their call
   return PGK_OK;
 }
 
@@ -992,7 +994,7 @@ int pgk_calforce_device(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, int32_
 }
 
 // grow a device buffer to at least `need` bytes (contents not kept)
-static int grow_dev(void **p, size_t *cap, size_t need) {
+int grow_dev(void **p, size_t *cap, size_t need) {
   if (need <= *cap && *p) return PGK_OK;
   if (*p) (void)hipFree(*p);
   *p = nullptr;
@@ -1024,11 +1026,12 @@ int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
 
 int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
                     pgk_force_out *out, pgk_hf_out *hfo) {
-  if (!c || !mts || !hf || !out || !out->force || !out->ene || !hfo || !hfo->pot_atm ||
-      !hfo->viri_atm) {
+  if (!c || !mts || !hf || !out || !out->force || !out->ene) {
     pgk_set_error("pgk_calforce_hf: null argument");
     return PGK_EINVAL;
   }
+  // per-atom arrays the caller does not take stay on the device only
+  const bool want_pe = hfo && hfo->pot_atm, want_w = hfo && hfo->viri_atm;
   if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
       !hf->z2 || !hf->hftyp) {
     pgk_set_error("pgk_calforce_hf: invalid heat-flux description");
@@ -1096,17 +1099,77 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
   h.pe = c->d_hf_pe;
   h.w = c->d_hf_w;
   const bool split = out->for_viri_coul && out->for_viri_lj;
+  c->hf_ready = false;
+  c->hf_export = want_pe || want_w;
   rc = run_calforce(c, mts_flag, mts, 1, split, c->d_force, c->d_fvc, c->d_fvl, out->ene,
                     out->virt, true);
   if (rc) return rc;
+  c->hf_ready = true;
   const size_t nb = sizeof(double) * 3 * n;
   HIP_CHECK(hipMemcpy(out->force, c->d_force, nb, hipMemcpyDeviceToHost));
   if (split) {
     HIP_CHECK(hipMemcpy(out->for_viri_coul, c->d_fvc, nb, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(out->for_viri_lj, c->d_fvl, nb, hipMemcpyDeviceToHost));
   }
-  HIP_CHECK(hipMemcpy(hfo->pot_atm, c->d_hf_pe_out, pe_b, hipMemcpyDeviceToHost));
-  HIP_CHECK(hipMemcpy(hfo->viri_atm, c->d_hf_w_out, wout_b, hipMemcpyDeviceToHost));
+  if (want_pe) HIP_CHECK(hipMemcpy(hfo->pot_atm, c->d_hf_pe_out, pe_b, hipMemcpyDeviceToHost));
+  if (want_w) HIP_CHECK(hipMemcpy(hfo->viri_atm, c->d_hf_w_out, wout_b, hipMemcpyDeviceToHost));
+  return PGK_OK;
+}
+
+// calheatf over the arrays of the last pgk_calforce_hf call
+int pgk_calheatf(pgk_ctx *c, const double *vel, const double *mass, double *hf) {
+  if (!c || !vel || !hf) {
+    pgk_set_error("pgk_calheatf: null argument");
+    return PGK_EINVAL;
+  }
+  if (!c->hf_ready) {
+    pgk_set_error("pgk_calheatf: no pgk_calforce_hf results on the device");
+    return PGK_ESTATE;
+  }
+  if (c->hfdev.mode == PGK_HF_SURFACE) {
+    pgk_set_error("pgk_calheatf: the surface-based form is not offered");
+    return PGK_EINVAL;
+  }
+  if (!mass && !c->hj_have_mass) {
+    pgk_set_error("pgk_calheatf: no masses given yet");
+    return PGK_EINVAL;
+  }
+  const int n = c->natom, nreg = c->hfdev.nreg;
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if (!c->d_hj_mol_start) {   // the atoms of each molecule, in index order
+    int nmol = 0;
+    for (int i = 0; i < n; i++) nmol = std::max(nmol, c->molid[i]);
+    std::vector<int32_t> start(nmol + 1, 0), atoms(std::max(n, 1));
+    for (int i = 0; i < n; i++) start[c->molid[i]]++;
+    for (int m = 0; m < nmol; m++) start[m + 1] += start[m];
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; i++) atoms[fill[c->molid[i] - 1]++] = i;
+    if ((rc = upload(&c->d_hj_mol_start, start)) || (rc = upload(&c->d_hj_mol_atoms, atoms)))
+      return rc;
+    std::vector<double> mv((size_t)3 * std::max(nmol, 1), 0.0);
+    if ((rc = upload(&c->d_hj_molevel, mv))) return rc;
+    c->hj_nmol = nmol;
+  }
+  if (!c->d_hj_vel) {
+    std::vector<double> z((size_t)3 * n, 0.0);
+    if ((rc = upload(&c->d_hj_vel, z))) return rc;
+  }
+  HIP_CHECK(hipMemcpyAsync(c->d_hj_vel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  if (mass) {
+    if (!c->d_hj_mass) {
+      std::vector<double> z(n, 0.0);
+      if ((rc = upload(&c->d_hj_mass, z))) return rc;
+    }
+    HIP_CHECK(hipMemcpyAsync(c->d_hj_mass, mass, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream));
+    c->hj_have_mass = true;
+  }
+  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass))) return rc;
+  HIP_CHECK(hipMemcpyAsync(hf, c->d_hj_out, sizeof(double) * (size_t)nreg * PGK_NHFJ * 3,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return PGK_OK;
 }
 
@@ -1392,6 +1455,7 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   }
   int rc = set_device_ctx(c);
   if (rc) return rc;
+  c->hf_ready = false;
   MdRunner run(c, c->md_graphs);
   if ((rc = run.begin(rcut_book)) || (rc = md_sample(c, samples))) return rc;
   int row = 1;
@@ -1441,7 +1505,10 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
   // (With graph replay on, one thread: a capture on one replica's stream
   // must not overlap another thread's synchronous copies.)
   bool any_graphs = false;
-  for (int r = 0; r < n; r++) any_graphs = any_graphs || ctxs[r]->md_batch_graphs;
+  for (int r = 0; r < n; r++) {
+    any_graphs = any_graphs || ctxs[r]->md_batch_graphs;
+    ctxs[r]->hf_ready = false;
+  }
   const int nth = any_graphs ? 1 : std::min(n, 8);
   std::vector<std::vector<int>> part(nth);
   {

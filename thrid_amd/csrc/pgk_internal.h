@@ -199,6 +199,15 @@ struct pgk_ctx {
   void *d_hf_pe = nullptr, *d_hf_w = nullptr;
   double *d_hf_pe_out = nullptr, *d_hf_w_out = nullptr;
   size_t hf_z_cap = 0, hf_com_cap = 0, hf_w_cap = 0, hf_wout_cap = 0;
+  bool hf_ready = false;           // a pgk_calforce_hf call left its arrays on the device
+  bool hf_export = true;           // run_calforce writes the caller-layout HF arrays
+  // calheatf (pgk_calheatf): velocities, masses, molecule lists, partial records
+  double *d_hj_vel = nullptr, *d_hj_mass = nullptr, *d_hj_molevel = nullptr;
+  double *d_hj_part = nullptr, *d_hj_out = nullptr;
+  int32_t *d_hj_mol_start = nullptr, *d_hj_mol_atoms = nullptr;   // atoms of each molecule
+  int hj_nmol = 0;
+  bool hj_have_mass = false;
+  size_t hj_part_cap = 0, hj_out_cap = 0;
 
   // device-resident MD step (velocity Verlet + RATTLE)
   bool md_ready = false;
@@ -266,6 +275,9 @@ int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off,
                        int *seg_kind, int *nseg, bool hf);
 int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_export(pgk_ctx *c);
+int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
+// grow a device buffer to at least `need` bytes (contents not kept)
+extern "C" int grow_dev(void **p, size_t *cap, size_t need);
 int pgk_md_kick_drift(pgk_ctx *c);
 int pgk_md_kick(pgk_ctx *c);
 int pgk_md_rattle_v(pgk_ctx *c);

@@ -26,7 +26,7 @@ module pgk_force_mod
   integer(c_int32_t), parameter :: PGK_MTS_LONG = 1, PGK_MTS_MED = 2, PGK_MTS_SHORT = 3
   integer(c_int32_t), parameter :: PGK_HF_VOLUME = 0, PGK_HF_SURFACE = 1, PGK_HF_BULK = 2
   integer(c_int32_t), parameter :: PGK_HFTYP_ATOM = 1, PGK_HFTYP_MOLE = 2
-  integer, parameter :: PGK_NENE = 19, PGK_NTENS = 6, PGK_NHF = 8
+  integer, parameter :: PGK_NENE = 19, PGK_NTENS = 6, PGK_NHF = 8, PGK_NHFJ = 18
   ! energy record (ene(1:PGK_NENE), 1-based here; C index + 1)
   integer, parameter :: PGK_E_TOT = 1, PGK_E_NONBON = 2, PGK_E_VDW = 3, PGK_E_ELC = 4, &
        &                PGK_E_EWK = 5, PGK_E_EWC = 6, PGK_E_EWNQ = 7, PGK_E_VDW14 = 8, &
@@ -161,6 +161,17 @@ module pgk_force_mod
        type(pgk_hf_out), intent(in) :: hfo
        integer(c_int) :: rc
      end function pgk_calforce_hf
+
+     ! calheatf over the last pgk_calforce_hf call's arrays (which that call
+     ! may leave on the device: hfo%pot_atm = hfo%viri_atm = c_null_ptr);
+     ! hf(3, PGK_NHFJ, nhfregion) = hfkin, hfpot*, hfviri*, hftot
+     function pgk_calheatf(ctx, vel, mass, hf) bind(C, name='pgk_calheatf') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: vel(*), mass(*)
+       real(c_double), intent(out) :: hf(*)
+       integer(c_int) :: rc
+     end function pgk_calheatf
 
      function pgk_calforce_async(ctx, mts_flag, mts, ifcalpreatom, d_force) &
           & bind(C, name='pgk_calforce_async') result(rc)

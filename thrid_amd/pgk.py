@@ -57,6 +57,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_get_state.argtypes = [vp, vp, vp]
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
+    L.pgk_calheatf.argtypes = [vp, vp, vp, vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
     L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
@@ -69,7 +70,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
-              "pgk_md_nhc", "pgk_md_nhc_state"):
+              "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -181,7 +182,9 @@ class ForceContext:
         _check(self.L.pgk_energies(self.h, e.ctypes.data, v.ctypes.data), "pgk_energies")
         return e, v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy()
 
-    def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None):
+    def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None, per_atom=True):
+        """calforce_hf; per_atom=False leaves the per-atom arrays on the
+        device (for calheatf) and returns None in their place."""
         mts = mts or abi.mts_all(mts_flag)
         keep = {}
         h = abi.HfDesc()
@@ -199,14 +202,30 @@ class ForceContext:
         f = np.zeros((self.n, 3))
         e = np.zeros(abi.NENE)
         v = np.zeros((abi.NTENS, 9))
-        pe = np.zeros((abi.NHF, self.n))
-        wv = np.zeros((abi.NHF, h.nregion, self.n, 9))
         o = abi.ForceOut(abi.dptr(f), abi.dptr(e), abi.dptr(v), None, None)
-        ho = abi.HfOut(abi.dptr(pe), abi.dptr(wv))
+        if per_atom:
+            pe = np.zeros((abi.NHF, self.n))
+            wv = np.zeros((abi.NHF, h.nregion, self.n, 9))
+            ho = C.byref(abi.HfOut(abi.dptr(pe), abi.dptr(wv)))
+        else:
+            pe = wv = ho = None
         _check(self.L.pgk_calforce_hf(self.h, mts_flag, C.byref(mts), C.byref(h), C.byref(o),
-                                      C.byref(ho)), "pgk_calforce_hf")
+                                      ho), "pgk_calforce_hf")
+        self._hf_nreg = h.nregion
         return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
-                    pot_atm=pe, viri_atm=wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+                    pot_atm=pe,
+                    viri_atm=None if wv is None else wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+
+    def calheatf(self, vel, mass=None):
+        """calheatf on the device over the last calforce_hf call's arrays:
+        returns hf[nregion, PGK_NHFJ, 3] (abi.HFJ_NAMES).  mass=None reuses
+        the last masses given."""
+        vv = np.ascontiguousarray(vel, np.float64)
+        mm = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        out = np.zeros((getattr(self, "_hf_nreg", 1), abi.NHFJ, 3))
+        _check(self.L.pgk_calheatf(self.h, vv.ctypes.data, None if mm is None else mm.ctypes.data,
+                                   out.ctypes.data), "pgk_calheatf")
+        return out
 
     # ---- device-resident NVE step ------------------------------------------
     def md_setup(self, mass, cons_ij, cons_d, group_start, dt, tol=1e-10):
