@@ -135,6 +135,52 @@ def run_force_only(sysd, precision, steps):
     return (t1 - t0) / steps, force
 
 
+def run_heatf(sysd, state, precision, reps=5):
+    """A heat-flux step of the 100k box: calforce_hf with the per-atom
+    arrays left in HBM, then calheatf on the device, for one bulk region
+    (the Green-Kubo case) and for 10 volume slabs; beside it the same
+    calforce_hf copying the per-atom arrays to the host, which a host-side
+    calheatf needs, and the fused calforce_heatf.  Wall-clock ms per call
+    (each call waits for its results)."""
+    from thrid_amd import abi, md
+    from thrid_amd.pgk import ForceContext
+    pos, vel, _cons = state
+    s2 = dict(sysd)
+    s2["pos"] = pos
+    ctx = ForceContext(s2, precision)
+    ctx.mklist(sysd["rcut_book"])
+    mass = np.asarray(sysd["mass"], np.float64) / md.MD_CONV    # m v^2 / 2 in eref
+    lz = sysd["box"][2]
+    edges = np.linspace(0.0, lz, 11)
+    cases = {"bulk_1": dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([lz])),
+             "volume_10": dict(mode=abi.HF_VOLUME, z1=edges[:-1], z2=edges[1:])}
+    out = {}
+    for name, hf in cases.items():
+        ctx.calforce_hf(hf, per_atom=False)
+        ctx.calheatf(vel, mass)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.calforce_hf(hf, per_atom=False)
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            ctx.calheatf(vel)
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            ctx.calforce_hf(hf, per_atom=True)
+        t3 = time.perf_counter()
+        ctx.calforce_heatf(hf, vel)
+        t4 = time.perf_counter()
+        for _ in range(reps):
+            ctx.calforce_heatf(hf, vel)
+        t5 = time.perf_counter()
+        out[name] = {"calforce_hf_ms": (t1 - t0) / reps * 1e3,
+                     "calheatf_ms": (t2 - t1) / reps * 1e3,
+                     "calforce_hf_with_host_arrays_ms": (t3 - t2) / reps * 1e3,
+                     "fused_calforce_heatf_ms": (t5 - t4) / reps * 1e3}
+    ctx.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (scale for the HBM byte counters)."""
     import torch
@@ -272,6 +318,7 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64, metavar="NREP",
                     help="replicas in the batched-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-heatf", action="store_true", help="skip the heat-flux leg")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg:   # child process of the CPU baseline leg (no GPU use)
@@ -353,6 +400,14 @@ def main():
         out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
                                                  / PEAK["fp64"])
         out["speedup_mixed_vs_fp64"] = f64["sec"] / res["mixed"]["sec"]
+    if not args.no_heatf:
+        out["heatf"] = {p: run_heatf(sysd, state, p) for p in precs}
+        for p, h in out["heatf"].items():
+            print(f"# heatf {p}: " + "; ".join(
+                f"{k}: calforce_hf {v['calforce_hf_ms']:.2f} ms + calheatf {v['calheatf_ms']:.3f} "
+                f"ms (with host arrays {v['calforce_hf_with_host_arrays_ms']:.2f} ms; fused "
+                f"{v['fused_calforce_heatf_ms']:.2f} ms)"
+                for k, v in h.items()), file=sys.stderr, flush=True)
     if reps:
         out["ensemble"] = run_ensemble(reps, args.ensemble_steps, 10)
         e = out["ensemble"]

@@ -253,6 +253,18 @@ int pgk_calforce_hf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
  * surface mode keeps state across steps and is not offered.                 */
 int pgk_calheatf(pgk_ctx *ctx, const double *vel, const double *mass, double *hf);
 
+/* calforce_hf followed by calheatf in one pass (no reference counterpart as
+ * one call: the fused z-binned heat-flux reduction for hosts that only need
+ * the region fluxes).  The heat-flux pass sums each interaction's share of
+ * sum_a W(a,n,i,r) v_i(a) directly, fi.v_i - fj.v_j times the region weight
+ * of calvirihf, so the (3,3,natom,nregion) arrays are never formed.  Forces,
+ * energies and virials as pgk_calforce_hf; hf as pgk_calheatf.  Volume form
+ * up to 256 regions, or the bulk form; the order of the FP64 additions is
+ * not fixed (hf agrees with pgk_calforce_hf + pgk_calheatf to rounding).    */
+int pgk_calforce_heatf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
+                       const pgk_hf_desc *hf, const double *vel, const double *mass,
+                       pgk_force_out *out, double *hf_out);
+
 /* Device-resident variant used by GPU-resident drivers: same as pgk_calforce
  * but the force stays on the device (d_force, [3*natom] doubles, may be NULL)
  * and only the energy/virial records are copied to the host.                */

@@ -184,3 +184,40 @@ def test_calheatf_order_of_calls():
     c.set_positions(s["pos"])
     with pytest.raises(PgkError):
         c.calheatf(vel)                       # positions changed since the call
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", [abi.HF_VOLUME, abi.HF_BULK])
+def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode):
+    """The fused pass (no W arrays: each interaction's fi.v_i - fj.v_j times
+    its region weights) against calforce_hf + calheatf on the same context,
+    and end to end against the oracle; forces and energies are calforce_hf's."""
+    from thrid_amd.pgk import PgkError
+    s = small_slab("rb")
+    lz = s["box"][2]
+    hf = dict(mode=mode, z1=np.array([0.0, 0.3 * lz, 0.45 * lz, 0.2 * lz]),
+              z2=np.array([0.5 * lz, 0.6 * lz, lz, 0.7 * lz]), hftyp=_hf_centres(s),
+              molecom=molecom(s))
+    ref, got, c = _run(oracle_mod, s, hf, prec)
+    rng = np.random.default_rng(23)
+    vel = rng.normal(scale=0.02, size=(s["natom"], 3))
+    mass = s["mass"] * 0.37
+    two = c.calheatf(vel, mass)
+    fused = c.calforce_heatf(hf, vel)           # masses kept from calheatf
+    scale = np.max(np.abs(two))
+    # the mixed form's W arrays are 64-bit fixed point; the fused sums are FP64
+    assert np.max(np.abs(fused["hf"] - two)) <= (1e-11 if prec == "fp64" else 1e-8) * scale
+    np.testing.assert_array_equal(fused["force"], got["force"]) if prec == "mixed" else \
+        np.testing.assert_allclose(fused["force"], got["force"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(fused["ene"], got["ene"], rtol=1e-12, atol=1e-12)
+    o = oracle_mod.Oracle(s)
+    full = o.calheatf(hf, vel, mass, ref["pot_atm"], ref["viri_atm"].reshape(abi.NHF, 4, -1, 9))
+    tol = 1e-10 if prec == "fp64" else 1e-4
+    for t in range(abi.NHFJ):
+        ts = np.max(np.abs(full[:, t]))
+        assert np.max(np.abs(fused["hf"][:, t] - full[:, t])) <= tol * max(ts, 1e-3 * scale), \
+            abi.HFJ_NAMES[t]
+    if mode == abi.HF_VOLUME:
+        z = np.linspace(0.0, lz, 258)
+        with pytest.raises(PgkError):
+            c.calforce_heatf(dict(mode=mode, z1=z[:-1], z2=z[1:]), vel)

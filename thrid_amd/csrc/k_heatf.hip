@@ -25,6 +25,7 @@ constexpr int kNterm = 1 + PGK_NHF;   // kinetic + potential classes
 
 struct HeatfArgs {
   int natom, nreg, nw, bulk, nsplit, chunk;
+  int nsplit_col;   // partial records per collision row
   const double *z1, *z2;
   const int32_t *hftyp, *molid;
   const double *molecom, *pos, *vel, *mass, *molevel;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(64) void k_heatf_final(HeatfArgs a, const double *_
         s += ptr[((size_t)r * a.nsplit + k) * (kNterm * 3) + 3 * t + c];
     } else {
       const size_t row = (size_t)(t - kNterm) * a.nreg + r;
-      for (int k = 0; k < a.nsplit; k++) s += pcol[(row * a.nsplit + k) * 3 + c];
+      for (int k = 0; k < a.nsplit_col; k++) s += pcol[(row * a.nsplit_col + k) * 3 + c];
     }
     s = s * vol_inv;
     o[3 * t + c] = s;
@@ -146,13 +147,40 @@ __global__ __launch_bounds__(64) void k_heatf_final(HeatfArgs a, const double *_
   o[3 * PGK_HFJ_TOT + c] = tot;
 }
 
+// flux form: collision row (class, region) = the bonded terms' atomics in
+// jcol plus, for the pair classes, the pair pass's per-block records in
+// block order (block (row, component) per launch block, fixed-order tree)
+__global__ __launch_bounds__(256) void k_heatf_collect(int nreg, int bulk, int nblk,
+                                                       const double *__restrict__ jcol,
+                                                       const double *__restrict__ jpart,
+                                                       double *__restrict__ pcol) {
+  const int row = blockIdx.x / 3, c = blockIdx.x % 3;   // row = class * nreg + region
+  const int k = row / nreg, r = row % nreg;
+  const int q = k - PGK_HF_ELIN;
+  double v[1] = {0.0};
+  if (q >= 0 && q < 4 && (bulk || c == 2)) {
+    const int stride = bulk ? 12 : 4 * nreg;
+    const int off = bulk ? 3 * q + c : q * nreg + r;
+    for (int b = threadIdx.x; b < nblk; b += 256) v[0] += jpart[(size_t)b * stride + off];
+  }
+  __shared__ double rec[PGK_REC];
+  block_record<1, 4>(v, rec);
+  __syncthreads();
+  if (threadIdx.x == 0) pcol[(size_t)row * 3 + c] = jcol[(size_t)row * 3 + c] + rec[0];
+}
+
 template <int PREC>
 int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   using AH = typename HfAcc<PREC>::type;
   hipStream_t st = c->stream;
   k_heatf_transport<AH><<<dim3(a.nsplit, a.nreg), 256, 0, st>>>(a, ptr);
   HIP_CHECK(hipGetLastError());
-  k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
+  if (c->hfdev.flux) {
+    k_heatf_collect<<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
+        a.nreg, a.bulk, pgk_grid_blocks(c->ncl, 4), c->hfdev.jcol, c->hfdev.jpart, pcol);
+  } else {
+    k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
+  }
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
@@ -176,6 +204,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
   nsplit = std::max(1, std::min(nsplit, (n + 255) / 256));
   a.nsplit = nsplit;
   a.chunk = (n + nsplit - 1) / nsplit;
+  a.nsplit_col = h.flux ? 1 : nsplit;
   a.z1 = h.z1;
   a.z2 = h.z2;
   a.hftyp = h.hftyp;
@@ -189,7 +218,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
   a.w = h.w;
   for (int d = 0; d < 3; d++) a.L[d] = h.L[d];
   const size_t ntr = (size_t)h.nreg * nsplit * kNterm * 3;
-  const size_t ncol = (size_t)rows * nsplit * 3;
+  const size_t ncol = (size_t)rows * a.nsplit_col * 3;
   int rc;
   if ((rc = grow_dev((void **)&c->d_hj_part, &c->hj_part_cap, sizeof(double) * (ntr + ncol))) ||
       (rc = grow_dev((void **)&c->d_hj_out, &c->hj_out_cap,

@@ -90,6 +90,147 @@ __global__ __launch_bounds__(256) void k_hf_pairs(HfPairArgs p, HfDev h) {
   }
 }
 
+// Flux form of the pass (pgk_calforce_heatf): the same pairs, energies and
+// per-atom energy sums, but the collision part of calheatf is summed on the
+// spot, s = fi . (v_i + v_j) times the region weights, instead of scattering
+// W arrays.  Bulk: 4 pair classes x 3 components per lane in registers.
+// Volume: a [4][nreg] slice of LDS per wave, added to with LDS atomics.  Each
+// block then adds its waves in order into jpart[block].
+constexpr int kPairCls0 = PGK_HF_ELIN;   // ELIN, LJIN, ELC, VDW are 4..7
+
+template <class AH, bool BULK>
+__global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + wave;
+  const int nreg = h.nreg;
+  double acc[4][3] = {};
+  double *sl = lds + (size_t)wave * 4 * nreg;   // volume: this wave's slice
+  if (!BULK) {
+    for (int k = lane; k < 4 * nreg; k += 64) sl[k] = 0.0;
+    __syncthreads();
+  }
+  if (ci < p.ncl) {
+    const int ii = lane & 7, jq = (lane >> 3) & 3, hs = lane >> 5;
+    const int ai = p.cl_atom[ci * PGK_CL + ii];
+    // padding slots (ai = -1) have no mask bits and read nothing
+    double cai[3] = {0.0, 0.0, 0.0}, vi[3] = {0.0, 0.0, 0.0};
+    int fla = 0;
+    if (ai >= 0) {
+      fla = hf_centre(h, ai, BULK, cai);
+      for (int d = 0; d < 3; d++) vi[d] = h.vel[3 * ai + d];
+    }
+    const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
+    for (int e2 = e0; e2 < e1; e2 += 2) {
+      const int e = e2 + hs;
+      if (e >= e1) continue;
+      const uint32_t m = p.mask[e];
+      if (!((m >> (lane & 31)) & 1u)) continue;
+      const int aj = p.cl_atom[p.cj[e] * 4 + jq];
+      double r[3];
+      for (int d = 0; d < 3; d++) {
+        r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
+        r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
+      }
+      const double r2 = r2_exact(r);
+      if (r2 > p.sqcut) continue;
+      const double rinv2 = 1.0 / r2;
+      const double ra = sqrt(r2);
+      const double rinv1 = 1.0 / ra;
+      const bool same = p.molid[ai] == p.molid[aj];
+      // the centres' separation and, in the volume form, whether it counts
+      double caj[3];
+      const int fl = fla + hf_centre(h, aj, BULK, caj);
+      bool seg = !(fl == 0 && same);
+      double rc[3], rz = 0.0;
+      for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
+      if (BULK) {
+        for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
+      } else {
+        rz = fabs(rc[2]);
+        seg = seg && rz * h.Li[2] < 0.5;
+      }
+      // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
+      const double* vjp = h.vel + 3 * aj;
+      const double w[3] = {vi[0] + vjp[0], vi[1] + vjp[1], vi[2] + vjp[2]};
+      const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
+      if (ra <= p.rrcut) {
+        const double qq = p.q[ai] * p.q[aj];
+        const double ph = qq * rinv1;
+        const double fh = ph * rinv2;
+        const int cls = same ? PGK_HF_ELIN : PGK_HF_ELC;
+        hf_pe_add<AH>(h, cls, ai, 0.5 * ph);
+        hf_pe_add<AH>(h, cls, aj, 0.5 * ph);
+        if (seg) {
+          const double sv = fh * rw * 0.5;
+          const int q = cls - kPairCls0;
+          if (BULK) {   // constant indices keep acc in registers
+            for (int n = 0; n < 3; n++) {
+              if (same) acc[PGK_HF_ELIN - kPairCls0][n] += rc[n] * sv;
+              else acc[PGK_HF_ELC - kPairCls0][n] += rc[n] * sv;
+            }
+          } else {
+            for (int ir = 0; ir < nreg; ir++) {
+              const double ff = hf_volume_weight(h, ir, cai[2], caj[2], rc[2], rz, 1.0);
+              if (ff != 0.0) atomicAdd(&sl[q * nreg + ir], ff * sv);
+            }
+          }
+        }
+      }
+      if (r2 <= p.sqcut2) {
+        const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
+        if (p.inttyp[tij] != 1) continue;
+        const double rad = p.rmin[tij];
+        const double t2 = rinv2 * (rad * rad);
+        const double t6 = t2 * t2 * t2;
+        const double t12 = t6 * t6;
+        const double ep = p.eps[tij];
+        const double pot = ep * (t12 - 2.0 * t6);
+        const double ft = 12.0 * ep * (t12 - t6) * rinv2;
+        const int cls = same ? PGK_HF_LJIN : PGK_HF_VDW;
+        hf_pe_add<AH>(h, cls, ai, 0.5 * pot);
+        hf_pe_add<AH>(h, cls, aj, 0.5 * pot);
+        if (seg) {
+          const double sv = ft * rw * 0.5;
+          const int q = cls - kPairCls0;
+          if (BULK) {
+            for (int n = 0; n < 3; n++) {
+              if (same) acc[PGK_HF_LJIN - kPairCls0][n] += rc[n] * sv;
+              else acc[PGK_HF_VDW - kPairCls0][n] += rc[n] * sv;
+            }
+          } else {
+            for (int ir = 0; ir < nreg; ir++) {
+              const double ff = hf_volume_weight(h, ir, cai[2], caj[2], rc[2], rz, 1.0);
+              if (ff != 0.0) atomicAdd(&sl[q * nreg + ir], ff * sv);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (BULK) {
+    __shared__ double wsum[4][12];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        const double t = wave_sum(acc[q][n]);
+        if (lane == 0) wsum[wave][3 * q + n] = t;
+      }
+    __syncthreads();
+    if (threadIdx.x < 12)
+      h.jpart[(size_t)blockIdx.x * 12 + threadIdx.x] =
+          ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) +
+          wsum[3][threadIdx.x];
+  } else {
+    __syncthreads();
+    const int nv = 4 * nreg;
+    for (int k = threadIdx.x; k < nv; k += 256)
+      h.jpart[(size_t)blockIdx.x * nv + k] =
+          ((lds[k] + lds[nv + k]) + lds[2 * nv + k]) + lds[3 * nv + k];
+  }
+}
+
 // accumulators -> doubles in the caller's layout: pot_atm[class][atom] and the
 // Fortran (3,3,natom,nregion) array per class
 template <class AH>
@@ -138,7 +279,15 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.sqcut = sq2 > sq ? sq2 : sq;
     p.sqcut2 = sq2;
     p.rrcut = c->rrcut;
-    k_hf_pairs<AH><<<pgk_grid_blocks(c->ncl, 4), 256, 0, st>>>(p, h);
+    const int nblk = pgk_grid_blocks(c->ncl, 4);
+    if (!h.flux) {
+      k_hf_pairs<AH><<<nblk, 256, 0, st>>>(p, h);
+    } else if (h.mode == PGK_HF_BULK) {
+      k_hf_pairs_flux<AH, true><<<nblk, 256, 0, st>>>(p, h);
+    } else {
+      const size_t lds = sizeof(double) * 16 * (size_t)h.nreg;
+      k_hf_pairs_flux<AH, false><<<nblk, 256, lds, st>>>(p, h);
+    }
     HIP_CHECK(hipGetLastError());
   }
   return PGK_OK;

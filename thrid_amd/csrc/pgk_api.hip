@@ -1024,24 +1024,21 @@ int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
   return finalize_energies(c, ene, virt);
 }
 
-int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
-                    pgk_force_out *out, pgk_hf_out *hfo) {
-  if (!c || !mts || !hf || !out || !out->force || !out->ene) {
-    pgk_set_error("pgk_calforce_hf: null argument");
-    return PGK_EINVAL;
-  }
-  // per-atom arrays the caller does not take stay on the device only
-  const bool want_pe = hfo && hfo->pot_atm, want_w = hfo && hfo->viri_atm;
+// Shared set-up of the heat-flux calls: checks the description, uploads the
+// regions, hftyp and molecule centres, clears the per-atom sums and points
+// hfdev at them.  flux: no W arrays; the collision part is summed into
+// d_hj_jcol / d_hj_jpart with the velocities already in d_hj_vel.
+static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *who) {
   if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
       !hf->z2 || !hf->hftyp) {
-    pgk_set_error("pgk_calforce_hf: invalid heat-flux description");
+    pgk_set_error(std::string(who) + ": invalid heat-flux description");
     return PGK_EINVAL;
   }
   const int n = c->natom, nreg = hf->nregion;
   bool need_com = false;
   for (int i = 0; i < n; i++) {
     if (hf->hftyp[i] != PGK_HFTYP_ATOM && hf->hftyp[i] != PGK_HFTYP_MOLE) {
-      pgk_set_error("pgk_calforce_hf: hftyp must be ATOM (1) or MOLE (2)");
+      pgk_set_error(std::string(who) + ": hftyp must be ATOM (1) or MOLE (2)");
       return PGK_EINVAL;
     }
     if (hf->hftyp[i] == PGK_HFTYP_MOLE) need_com = true;
@@ -1049,7 +1046,7 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
   int maxmol = 0;
   for (int i = 0; i < n; i++) maxmol = std::max(maxmol, c->molid[i]);
   if (need_com && (!hf->molecom || hf->nmol < maxmol)) {
-    pgk_set_error("pgk_calforce_hf: MOLE atoms need molecom for every molecule");
+    pgk_set_error(std::string(who) + ": MOLE atoms need molecom for every molecule");
     return PGK_EINVAL;
   }
   int rc = set_device_ctx(c);
@@ -1063,10 +1060,11 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
   size_t pe_cap = c->d_hf_pe ? pe_b : 0, peo_cap = c->d_hf_pe_out ? pe_b : 0;
   if ((rc = grow_dev((void **)&c->d_hf_z, &c->hf_z_cap, sizeof(double) * 2 * nreg)) ||
       (rc = grow_dev((void **)&c->d_hf_typ, &typ_cap, sizeof(int32_t) * n)) ||
-      (rc = grow_dev(&c->d_hf_pe, &pe_cap, pe_b)) ||
-      (rc = grow_dev(&c->d_hf_w, &c->hf_w_cap, w_b)) ||
-      (rc = grow_dev((void **)&c->d_hf_pe_out, &peo_cap, pe_b)) ||
-      (rc = grow_dev((void **)&c->d_hf_w_out, &c->hf_wout_cap, wout_b)))
+      (rc = grow_dev(&c->d_hf_pe, &pe_cap, pe_b)))
+    return rc;
+  if (!flux && ((rc = grow_dev(&c->d_hf_w, &c->hf_w_cap, w_b)) ||
+                (rc = grow_dev((void **)&c->d_hf_pe_out, &peo_cap, pe_b)) ||
+                (rc = grow_dev((void **)&c->d_hf_w_out, &c->hf_wout_cap, wout_b))))
     return rc;
   if (need_com &&
       (rc = grow_dev((void **)&c->d_hf_com, &c->hf_com_cap, sizeof(double) * 3 * hf->nmol)))
@@ -1080,7 +1078,7 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
     HIP_CHECK(hipMemcpyAsync(c->d_hf_com, hf->molecom, sizeof(double) * 3 * hf->nmol,
                              hipMemcpyHostToDevice, st));
   HIP_CHECK(hipMemsetAsync(c->d_hf_pe, 0, pe_b, st));
-  HIP_CHECK(hipMemsetAsync(c->d_hf_w, 0, w_b, st));
+  if (!flux) HIP_CHECK(hipMemsetAsync(c->d_hf_w, 0, w_b, st));
   HfDev &h = c->hfdev;
   h.mode = hf->mode;
   h.nreg = nreg;
@@ -1097,7 +1095,50 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
     h.Li[d] = 1.0 / c->box[d];
   }
   h.pe = c->d_hf_pe;
-  h.w = c->d_hf_w;
+  h.w = flux ? nullptr : c->d_hf_w;
+  h.flux = flux ? 1 : 0;
+  h.vel = flux ? c->d_hj_vel : nullptr;
+  h.jcol = nullptr;
+  h.jpart = nullptr;
+  if (flux) {
+    const size_t jcol_b = sizeof(double) * PGK_NHF * (size_t)nreg * 3;
+    const size_t nblk = (size_t)pgk_grid_blocks(std::max(c->ncl, 1), 4);
+    const size_t jpart_b =
+        sizeof(double) * nblk * (hf->mode == PGK_HF_BULK ? 12 : 4 * (size_t)nreg);
+    if ((rc = grow_dev((void **)&c->d_hj_jcol, &c->hj_jcol_cap, jcol_b)) ||
+        (rc = grow_dev((void **)&c->d_hj_jpart, &c->hj_jpart_cap, jpart_b)))
+      return rc;
+    HIP_CHECK(hipMemsetAsync(c->d_hj_jcol, 0, jcol_b, st));
+    HIP_CHECK(hipMemsetAsync(c->d_hj_jpart, 0, jpart_b, st));
+    h.jcol = c->d_hj_jcol;
+    h.jpart = c->d_hj_jpart;
+  }
+  return PGK_OK;
+}
+
+static int copy_force_out(pgk_ctx *c, pgk_force_out *out) {
+  const size_t nb = sizeof(double) * 3 * c->natom;
+  HIP_CHECK(hipMemcpy(out->force, c->d_force, nb, hipMemcpyDeviceToHost));
+  if (out->for_viri_coul && out->for_viri_lj) {
+    HIP_CHECK(hipMemcpy(out->for_viri_coul, c->d_fvc, nb, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(out->for_viri_lj, c->d_fvl, nb, hipMemcpyDeviceToHost));
+  }
+  return PGK_OK;
+}
+
+int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
+                    pgk_force_out *out, pgk_hf_out *hfo) {
+  if (!c || !mts || !hf || !out || !out->force || !out->ene) {
+    pgk_set_error("pgk_calforce_hf: null argument");
+    return PGK_EINVAL;
+  }
+  // per-atom arrays the caller does not take stay on the device only
+  const bool want_pe = hfo && hfo->pot_atm, want_w = hfo && hfo->viri_atm;
+  int rc = hf_setup(c, hf, false, "pgk_calforce_hf");
+  if (rc) return rc;
+  const size_t n = (size_t)c->natom, nreg = (size_t)hf->nregion;
+  const size_t pe_b = sizeof(double) * PGK_NHF * n;
+  const size_t wout_b = sizeof(double) * PGK_NHF * nreg * n * 9;
   const bool split = out->for_viri_coul && out->for_viri_lj;
   c->hf_ready = false;
   c->hf_export = want_pe || want_w;
@@ -1105,14 +1146,89 @@ int pgk_calforce_hf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_
                     out->virt, true);
   if (rc) return rc;
   c->hf_ready = true;
-  const size_t nb = sizeof(double) * 3 * n;
-  HIP_CHECK(hipMemcpy(out->force, c->d_force, nb, hipMemcpyDeviceToHost));
-  if (split) {
-    HIP_CHECK(hipMemcpy(out->for_viri_coul, c->d_fvc, nb, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(out->for_viri_lj, c->d_fvl, nb, hipMemcpyDeviceToHost));
-  }
+  if ((rc = copy_force_out(c, out))) return rc;
   if (want_pe) HIP_CHECK(hipMemcpy(hfo->pot_atm, c->d_hf_pe_out, pe_b, hipMemcpyDeviceToHost));
   if (want_w) HIP_CHECK(hipMemcpy(hfo->viri_atm, c->d_hf_w_out, wout_b, hipMemcpyDeviceToHost));
+  return PGK_OK;
+}
+
+// host velocities / masses -> d_hj_vel / d_hj_mass (mass NULL keeps the last)
+static int hj_upload(pgk_ctx *c, const double *vel, const double *mass, const char *who) {
+  if (!mass && !c->hj_have_mass) {
+    pgk_set_error(std::string(who) + ": no masses given yet");
+    return PGK_EINVAL;
+  }
+  const int n = c->natom;
+  int rc;
+  if (!c->d_hj_vel) {
+    std::vector<double> z((size_t)3 * n, 0.0);
+    if ((rc = upload(&c->d_hj_vel, z))) return rc;
+  }
+  HIP_CHECK(hipMemcpyAsync(c->d_hj_vel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  if (mass) {
+    if (!c->d_hj_mass) {
+      std::vector<double> z(n, 0.0);
+      if ((rc = upload(&c->d_hj_mass, z))) return rc;
+    }
+    HIP_CHECK(hipMemcpyAsync(c->d_hj_mass, mass, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream));
+    c->hj_have_mass = true;
+  }
+  return PGK_OK;
+}
+
+// the atoms of each molecule, in index order (for centre-of-mass velocities)
+static int hj_molecules(pgk_ctx *c) {
+  if (c->d_hj_mol_start) return PGK_OK;
+  const int n = c->natom;
+  int nmol = 0, rc;
+  for (int i = 0; i < n; i++) nmol = std::max(nmol, c->molid[i]);
+  std::vector<int32_t> start(nmol + 1, 0), atoms(std::max(n, 1));
+  for (int i = 0; i < n; i++) start[c->molid[i]]++;
+  for (int m = 0; m < nmol; m++) start[m + 1] += start[m];
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int i = 0; i < n; i++) atoms[fill[c->molid[i] - 1]++] = i;
+  if ((rc = upload(&c->d_hj_mol_start, start)) || (rc = upload(&c->d_hj_mol_atoms, atoms)))
+    return rc;
+  std::vector<double> mv((size_t)3 * std::max(nmol, 1), 0.0);
+  if ((rc = upload(&c->d_hj_molevel, mv))) return rc;
+  c->hj_nmol = nmol;
+  return PGK_OK;
+}
+
+// calforce_hf and calheatf in one pass, without the per-atom W arrays
+int pgk_calforce_heatf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
+                       const double *vel, const double *mass, pgk_force_out *out, double *hfj) {
+  if (!c || !mts || !hf || !out || !out->force || !out->ene || !vel || !hfj) {
+    pgk_set_error("pgk_calforce_heatf: null argument");
+    return PGK_EINVAL;
+  }
+  if (hf->mode == PGK_HF_SURFACE) {
+    pgk_set_error("pgk_calforce_heatf: the surface-based form is not offered");
+    return PGK_EINVAL;
+  }
+  if (hf->mode == PGK_HF_VOLUME && hf->nregion > PGK_HEATF_MAXREG) {
+    pgk_set_error("pgk_calforce_heatf: more than " + std::to_string(PGK_HEATF_MAXREG) +
+                  " volume regions (use pgk_calforce_hf + pgk_calheatf)");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if ((rc = hj_upload(c, vel, mass, "pgk_calforce_heatf")) || (rc = hj_molecules(c)) ||
+      (rc = hf_setup(c, hf, true, "pgk_calforce_heatf")))
+    return rc;
+  c->hf_ready = false;
+  c->hf_export = false;
+  rc = run_calforce(c, mts_flag, mts, 1, out->for_viri_coul && out->for_viri_lj, c->d_force,
+                    c->d_fvc, c->d_fvl, out->ene, out->virt, true);
+  if (rc) return rc;
+  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass))) return rc;
+  if ((rc = copy_force_out(c, out))) return rc;
+  HIP_CHECK(hipMemcpyAsync(hfj, c->d_hj_out, sizeof(double) * (size_t)hf->nregion * PGK_NHFJ * 3,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->hfdev.flux = 0;
   return PGK_OK;
 }
 
@@ -1130,44 +1246,12 @@ int pgk_calheatf(pgk_ctx *c, const double *vel, const double *mass, double *hf) 
     pgk_set_error("pgk_calheatf: the surface-based form is not offered");
     return PGK_EINVAL;
   }
-  if (!mass && !c->hj_have_mass) {
-    pgk_set_error("pgk_calheatf: no masses given yet");
-    return PGK_EINVAL;
-  }
-  const int n = c->natom, nreg = c->hfdev.nreg;
   int rc = set_device_ctx(c);
   if (rc) return rc;
-  if (!c->d_hj_mol_start) {   // the atoms of each molecule, in index order
-    int nmol = 0;
-    for (int i = 0; i < n; i++) nmol = std::max(nmol, c->molid[i]);
-    std::vector<int32_t> start(nmol + 1, 0), atoms(std::max(n, 1));
-    for (int i = 0; i < n; i++) start[c->molid[i]]++;
-    for (int m = 0; m < nmol; m++) start[m + 1] += start[m];
-    std::vector<int32_t> fill(start.begin(), start.end() - 1);
-    for (int i = 0; i < n; i++) atoms[fill[c->molid[i] - 1]++] = i;
-    if ((rc = upload(&c->d_hj_mol_start, start)) || (rc = upload(&c->d_hj_mol_atoms, atoms)))
-      return rc;
-    std::vector<double> mv((size_t)3 * std::max(nmol, 1), 0.0);
-    if ((rc = upload(&c->d_hj_molevel, mv))) return rc;
-    c->hj_nmol = nmol;
-  }
-  if (!c->d_hj_vel) {
-    std::vector<double> z((size_t)3 * n, 0.0);
-    if ((rc = upload(&c->d_hj_vel, z))) return rc;
-  }
-  HIP_CHECK(hipMemcpyAsync(c->d_hj_vel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-                           c->stream));
-  if (mass) {
-    if (!c->d_hj_mass) {
-      std::vector<double> z(n, 0.0);
-      if ((rc = upload(&c->d_hj_mass, z))) return rc;
-    }
-    HIP_CHECK(hipMemcpyAsync(c->d_hj_mass, mass, sizeof(double) * n, hipMemcpyHostToDevice,
-                             c->stream));
-    c->hj_have_mass = true;
-  }
+  if ((rc = hj_upload(c, vel, mass, "pgk_calheatf")) || (rc = hj_molecules(c))) return rc;
   if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass))) return rc;
-  HIP_CHECK(hipMemcpyAsync(hf, c->d_hj_out, sizeof(double) * (size_t)nreg * PGK_NHFJ * 3,
+  HIP_CHECK(hipMemcpyAsync(hf, c->d_hj_out,
+                           sizeof(double) * (size_t)c->hfdev.nreg * PGK_NHFJ * 3,
                            hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   return PGK_OK;

@@ -47,6 +47,64 @@ __device__ __forceinline__ int hf_centre(const HfDev &h, int i, bool bulk, doubl
   return 1;
 }
 
+// The weight of region ir for a segment between centres with z coordinates
+// ciz, cjz (r2 = ciz - cjz, rz = |r2|) in the volume form: the signed length
+// of the part of the segment inside [z1, z2], times nbc (calvirihf.F90:201-357).
+// Returns 0 when the segment misses the region.
+__device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, double ciz,
+                                                   double cjz, double r2, double rz,
+                                                   double nbc) {
+  const double z1 = h.z1[ir], z2 = h.z2[ir];
+  const double ri1 = ciz - z1, rj1 = cjz - z1, ri2 = ciz - z2, rj2 = cjz - z2;
+  int np = 0;
+  bool p1 = false;
+  if (ri1 * rj1 < 0.0) {
+    p1 = true;
+    np++;
+  }
+  if (ri2 * rj2 < 0.0) np++;
+  double ff;
+  if (np == 0) {
+    if (ri1 * ri2 > 0.0) return 0.0;
+    ff = rz;
+  } else if (np == 1) {
+    if (p1) ff = (ri1 > 0.0) ? ri1 : rj1;
+    else ff = (ri2 < 0.0) ? -ri2 : -rj2;
+  } else {
+    ff = z2 - z1;
+  }
+  return (r2 >= 0.0) ? ff * nbc : -ff * nbc;
+}
+
+// flux form of calvirihf: the interaction's share of sum_a W(a,n,.,r) v(a),
+// i.e. the region weight times s = fi . v_i - fj . v_j, added to
+// jcol[cls][r][n] (global FP64 atomics; the bonded terms use this, the pair
+// pass keeps its own per-block sums)
+__device__ void calvirihf_flux(const HfDev &h, int i, int j, const double *fi, const double *fj,
+                               double nbc, int cls, const double *ci, const double *cj) {
+  const bool bulk = (h.mode == PGK_HF_BULK);
+  const double *vi = h.vel + 3 * i, *vj = h.vel + 3 * j;
+  const double s = (fi[0] * vi[0] + fi[1] * vi[1] + fi[2] * vi[2]) -
+                   (fj[0] * vj[0] + fj[1] * vj[1] + fj[2] * vj[2]);
+  double *J = h.jcol + (size_t)cls * h.nreg * 3;
+  if (bulk) {
+    double r[3];
+    for (int d = 0; d < 3; d++) {
+      r[d] = ci[d] - cj[d];
+      r[d] = r[d] - h.L[d] * round(r[d] * h.Li[d]);
+    }
+    for (int ir = 0; ir < h.nreg; ir++)
+      for (int n = 0; n < 3; n++) atomicAdd(&J[3 * ir + n], r[n] * nbc * s);
+    return;
+  }
+  const double r2 = ci[2] - cj[2], rz = fabs(r2);
+  if (rz * h.Li[2] >= 0.5) return;
+  for (int ir = 0; ir < h.nreg; ir++) {
+    const double ff = hf_volume_weight(h, ir, ci[2], cj[2], r2, rz, nbc);
+    if (ff != 0.0) atomicAdd(&J[3 * ir + 2], ff * s);
+  }
+}
+
 template <class AH>
 __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, const double *fj,
                               double nbc, int cls) {
@@ -54,6 +112,10 @@ __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, co
   double ci[3], cj[3];
   const int atomflag = hf_centre(h, i, bulk, ci) + hf_centre(h, j, bulk, cj);
   if (atomflag == 0 && h.molid[i] == h.molid[j]) return;
+  if (h.flux) {
+    calvirihf_flux(h, i, j, fi, fj, nbc, cls, ci, cj);
+    return;
+  }
   double r[3] = {0.0, 0.0, 0.0}, rz = 0.0;
   if (bulk) {
     r[0] = ci[0] - cj[0];
@@ -95,26 +157,8 @@ __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, co
     return;
   }
   for (int ir = 0; ir < h.nreg; ir++) {
-    const double z1 = h.z1[ir], z2 = h.z2[ir];
-    const double ri1 = ci[2] - z1, rj1 = cj[2] - z1, ri2 = ci[2] - z2, rj2 = cj[2] - z2;
-    int np = 0;
-    bool p1 = false;
-    if (ri1 * rj1 < 0.0) {
-      p1 = true;
-      np++;
-    }
-    if (ri2 * rj2 < 0.0) np++;
-    double ff;
-    if (np == 0) {
-      if (ri1 * ri2 > 0.0) continue;
-      ff = rz;
-    } else if (np == 1) {
-      if (p1) ff = (ri1 > 0.0) ? ri1 : rj1;
-      else ff = (ri2 < 0.0) ? -ri2 : -rj2;
-    } else {
-      ff = z2 - z1;
-    }
-    ff = (r[2] >= 0.0) ? ff * nbc : -ff * nbc;
+    const double ff = hf_volume_weight(h, ir, ci[2], cj[2], r[2], rz, nbc);
+    if (ff == 0.0) continue;
     AH *wi = W + (ir * N + i) * 3, *wj = W + (ir * N + j) * 3;
     for (int a = 0; a < 3; a++) {
       acc_add(&wi[a], fi[a] * ff);

@@ -60,6 +60,10 @@ void pgk_set_error(const std::string &msg);
 // device view of one calforce_hf call's heat-flux set-up and accumulators
 // (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
 // else 3 = the z column only)
+// volume regions of the flux form: the pair pass keeps [4][nreg] doubles of
+// LDS per wave
+#define PGK_HEATF_MAXREG 256
+
 struct HfDev {
   int mode = 0, nreg = 0, natom = 0, nw = 3;
   const double *z1 = nullptr, *z2 = nullptr;   // [nreg]
@@ -70,6 +74,12 @@ struct HfDev {
   double L[3] = {0, 0, 0}, Li[3] = {0, 0, 0};
   void *pe = nullptr;
   void *w = nullptr;
+  // flux form (pgk_calforce_heatf): instead of the W arrays, the collision
+  // part of calheatf, sum_a W(a,n,i,r) v_i(a), is accumulated directly
+  int flux = 0;
+  const double *vel = nullptr;                 // [3*natom] atom velocities
+  double *jcol = nullptr;                      // [PGK_NHF][nreg][3] FP64 sums
+  double *jpart = nullptr;                     // pair pass: per-block records
 };
 
 struct DevBuf {
@@ -205,6 +215,8 @@ struct pgk_ctx {
   double *d_hj_vel = nullptr, *d_hj_mass = nullptr, *d_hj_molevel = nullptr;
   double *d_hj_part = nullptr, *d_hj_out = nullptr;
   int32_t *d_hj_mol_start = nullptr, *d_hj_mol_atoms = nullptr;   // atoms of each molecule
+  double *d_hj_jcol = nullptr, *d_hj_jpart = nullptr;   // flux form sums
+  size_t hj_jcol_cap = 0, hj_jpart_cap = 0;
   int hj_nmol = 0;
   bool hj_have_mass = false;
   size_t hj_part_cap = 0, hj_out_cap = 0;

@@ -58,6 +58,8 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
     L.pgk_calheatf.argtypes = [vp, vp, vp, vp]
+    L.pgk_calforce_heatf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
+                                     vp, vp, C.POINTER(abi.ForceOut), vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
     L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
@@ -70,7 +72,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
-              "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf"):
+              "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -182,11 +184,7 @@ class ForceContext:
         _check(self.L.pgk_energies(self.h, e.ctypes.data, v.ctypes.data), "pgk_energies")
         return e, v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy()
 
-    def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None, per_atom=True):
-        """calforce_hf; per_atom=False leaves the per-atom arrays on the
-        device (for calheatf) and returns None in their place."""
-        mts = mts or abi.mts_all(mts_flag)
-        keep = {}
+    def _hf_desc(self, hf, keep):
         h = abi.HfDesc()
         h.mode = int(hf.get("mode", abi.HF_VOLUME))
         for k in ("z1", "z2"):
@@ -199,6 +197,14 @@ class ForceContext:
                                           np.float64)
         h.nmol = keep["mc"].shape[0]
         h.molecom = abi.dptr(keep["mc"])
+        return h
+
+    def calforce_hf(self, hf, mts_flag=abi.MTS_LONG, mts=None, per_atom=True):
+        """calforce_hf; per_atom=False leaves the per-atom arrays on the
+        device (for calheatf) and returns None in their place."""
+        mts = mts or abi.mts_all(mts_flag)
+        keep = {}
+        h = self._hf_desc(hf, keep)
         f = np.zeros((self.n, 3))
         e = np.zeros(abi.NENE)
         v = np.zeros((abi.NTENS, 9))
@@ -215,6 +221,26 @@ class ForceContext:
         return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
                     pot_atm=pe,
                     viri_atm=None if wv is None else wv.reshape(abi.NHF, h.nregion, self.n, 3, 3))
+
+    def calforce_heatf(self, hf, vel, mass=None, mts_flag=abi.MTS_LONG, mts=None):
+        """calforce_hf + calheatf in one pass without the per-atom arrays:
+        returns the calforce record (force, ene, virt) and hf[nregion,
+        PGK_NHFJ, 3]."""
+        mts = mts or abi.mts_all(mts_flag)
+        keep = {}
+        h = self._hf_desc(hf, keep)
+        f = np.zeros((self.n, 3))
+        e = np.zeros(abi.NENE)
+        v = np.zeros((abi.NTENS, 9))
+        o = abi.ForceOut(abi.dptr(f), abi.dptr(e), abi.dptr(v), None, None)
+        vv = np.ascontiguousarray(vel, np.float64)
+        mm = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        out = np.zeros((h.nregion, abi.NHFJ, 3))
+        _check(self.L.pgk_calforce_heatf(self.h, mts_flag, C.byref(mts), C.byref(h),
+                                         vv.ctypes.data, None if mm is None else mm.ctypes.data,
+                                         C.byref(o), out.ctypes.data), "pgk_calforce_heatf")
+        return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
+                    hf=out)
 
     def calheatf(self, vel, mass=None):
         """calheatf on the device over the last calforce_hf call's arrays:
