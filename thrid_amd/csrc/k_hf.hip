@@ -98,6 +98,36 @@ __global__ __launch_bounds__(256) void k_hf_pairs(HfPairArgs p, HfDev h) {
 // block then adds its waves in order into jpart[block].
 constexpr int kPairCls0 = PGK_HF_ELIN;   // ELIN, LJIN, ELC, VDW are 4..7
 
+// what an interaction needs besides its class, energy and force coefficient
+struct FluxPair {
+  bool seg;          // the centres' segment counts (not two MOLE atoms of one molecule...)
+  double rw;         // r . (v_i + v_j)
+  double rc[3];      // centre separation (minimum image in the bulk form)
+  double ciz, cjz, rz;
+};
+
+// one interaction of pair class Q (0 ELIN, 1 LJIN, 2 ELC, 3 VDW) with energy
+// e and force coefficient f (force on i = f r): half the energy to each
+// atom, and f (r . (v_i + v_j)) / 2 times the region weights to the flux
+template <int Q, bool BULK>
+__device__ __forceinline__ void flux_add(const HfDev &h, const FluxPair &fp, double e, double f,
+                                         double (&ei)[4], double (&ej)[4],
+                                         double (&acc)[4][3], double *sl) {
+  ei[Q] += 0.5 * e;
+  ej[Q] += 0.5 * e;
+  if (!fp.seg) return;
+  const double sv = f * fp.rw * 0.5;
+  if (BULK) {
+#pragma unroll
+    for (int n = 0; n < 3; n++) acc[Q][n] += fp.rc[n] * sv;
+  } else {
+    for (int ir = 0; ir < h.nreg; ir++) {
+      const double ff = hf_volume_weight(h, ir, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
+      if (ff != 0.0) atomicAdd(&sl[Q * h.nreg + ir], ff * sv);
+    }
+  }
+}
+
 template <class AH, bool BULK>
 __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
   extern __shared__ double lds[];
@@ -120,92 +150,84 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
       fla = hf_centre(h, ai, BULK, cai);
       for (int d = 0; d < 3; d++) vi[d] = h.vel[3 * ai + d];
     }
+    // per-atom energies: the i atom's shares stay in registers for the whole
+    // list; the j atom's are summed over the 8 lanes that share it (lanes
+    // ii = 0..7 of one (jq, hs) group) and added once per entry
+    double ei[4] = {0.0, 0.0, 0.0, 0.0};   // ELIN, LJIN, ELC, VDW
     const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
     for (int e2 = e0; e2 < e1; e2 += 2) {
       const int e = e2 + hs;
-      if (e >= e1) continue;
-      const uint32_t m = p.mask[e];
-      if (!((m >> (lane & 31)) & 1u)) continue;
-      const int aj = p.cl_atom[p.cj[e] * 4 + jq];
-      double r[3];
-      for (int d = 0; d < 3; d++) {
-        r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
-        r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
-      }
-      const double r2 = r2_exact(r);
-      if (r2 > p.sqcut) continue;
-      const double rinv2 = 1.0 / r2;
-      const double ra = sqrt(r2);
-      const double rinv1 = 1.0 / ra;
-      const bool same = p.molid[ai] == p.molid[aj];
-      // the centres' separation and, in the volume form, whether it counts
-      double caj[3];
-      const int fl = fla + hf_centre(h, aj, BULK, caj);
-      bool seg = !(fl == 0 && same);
-      double rc[3], rz = 0.0;
-      for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
-      if (BULK) {
-        for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
-      } else {
-        rz = fabs(rc[2]);
-        seg = seg && rz * h.Li[2] < 0.5;
-      }
-      // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
-      const double* vjp = h.vel + 3 * aj;
-      const double w[3] = {vi[0] + vjp[0], vi[1] + vjp[1], vi[2] + vjp[2]};
-      const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
-      if (ra <= p.rrcut) {
-        const double qq = p.q[ai] * p.q[aj];
-        const double ph = qq * rinv1;
-        const double fh = ph * rinv2;
-        const int cls = same ? PGK_HF_ELIN : PGK_HF_ELC;
-        hf_pe_add<AH>(h, cls, ai, 0.5 * ph);
-        hf_pe_add<AH>(h, cls, aj, 0.5 * ph);
-        if (seg) {
-          const double sv = fh * rw * 0.5;
-          const int q = cls - kPairCls0;
-          if (BULK) {   // constant indices keep acc in registers
-            for (int n = 0; n < 3; n++) {
-              if (same) acc[PGK_HF_ELIN - kPairCls0][n] += rc[n] * sv;
-              else acc[PGK_HF_ELC - kPairCls0][n] += rc[n] * sv;
-            }
-          } else {
-            for (int ir = 0; ir < nreg; ir++) {
-              const double ff = hf_volume_weight(h, ir, cai[2], caj[2], rc[2], rz, 1.0);
-              if (ff != 0.0) atomicAdd(&sl[q * nreg + ir], ff * sv);
-            }
-          }
+      const bool ein = e < e1;
+      const uint32_t m = ein ? p.mask[e] : 0u;
+      const int aj = ein ? p.cl_atom[p.cj[e] * 4 + jq] : -1;
+      double ej[4] = {0.0, 0.0, 0.0, 0.0};
+      if ((m >> (lane & 31)) & 1u) {
+        double r[3];
+        for (int d = 0; d < 3; d++) {
+          r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
+          r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
         }
-      }
-      if (r2 <= p.sqcut2) {
-        const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
-        if (p.inttyp[tij] != 1) continue;
-        const double rad = p.rmin[tij];
-        const double t2 = rinv2 * (rad * rad);
-        const double t6 = t2 * t2 * t2;
-        const double t12 = t6 * t6;
-        const double ep = p.eps[tij];
-        const double pot = ep * (t12 - 2.0 * t6);
-        const double ft = 12.0 * ep * (t12 - t6) * rinv2;
-        const int cls = same ? PGK_HF_LJIN : PGK_HF_VDW;
-        hf_pe_add<AH>(h, cls, ai, 0.5 * pot);
-        hf_pe_add<AH>(h, cls, aj, 0.5 * pot);
-        if (seg) {
-          const double sv = ft * rw * 0.5;
-          const int q = cls - kPairCls0;
+        const double r2 = r2_exact(r);
+        if (r2 <= p.sqcut) {
+          const double rinv2 = 1.0 / r2;
+          const double ra = sqrt(r2);
+          const double rinv1 = 1.0 / ra;
+          const bool same = p.molid[ai] == p.molid[aj];
+          // the centres' separation and, in the volume form, whether it counts
+          double caj[3];
+          const int fl = fla + hf_centre(h, aj, BULK, caj);
+          bool seg = !(fl == 0 && same);
+          double rc[3], rz = 0.0;
+          for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
           if (BULK) {
-            for (int n = 0; n < 3; n++) {
-              if (same) acc[PGK_HF_LJIN - kPairCls0][n] += rc[n] * sv;
-              else acc[PGK_HF_VDW - kPairCls0][n] += rc[n] * sv;
-            }
+            for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
           } else {
-            for (int ir = 0; ir < nreg; ir++) {
-              const double ff = hf_volume_weight(h, ir, cai[2], caj[2], rc[2], rz, 1.0);
-              if (ff != 0.0) atomicAdd(&sl[q * nreg + ir], ff * sv);
+            rz = fabs(rc[2]);
+            seg = seg && rz * h.Li[2] < 0.5;
+          }
+          // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
+          const double *vjp = h.vel + 3 * aj;
+          const double w[3] = {vi[0] + vjp[0], vi[1] + vjp[1], vi[2] + vjp[2]};
+          const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
+          const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cai[2], caj[2], rz};
+          if (ra <= p.rrcut) {
+            const double qq = p.q[ai] * p.q[aj];
+            const double ph = qq * rinv1;
+            if (same) flux_add<0, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
+            else flux_add<2, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
+          }
+          if (r2 <= p.sqcut2) {
+            const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
+            if (p.inttyp[tij] == 1) {
+              const double rad = p.rmin[tij];
+              const double t2 = rinv2 * (rad * rad);
+              const double t6 = t2 * t2 * t2;
+              const double t12 = t6 * t6;
+              const double ep = p.eps[tij];
+              const double pot = ep * (t12 - 2.0 * t6);
+              const double ft = 12.0 * ep * (t12 - t6) * rinv2;
+              if (same) flux_add<1, BULK>(h, fp, pot, ft, ei, ej, acc, sl);
+              else flux_add<3, BULK>(h, fp, pot, ft, ei, ej, acc, sl);
             }
           }
         }
       }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        ej[q] += __shfl_xor(ej[q], 1, 64);
+        ej[q] += __shfl_xor(ej[q], 2, 64);
+        ej[q] += __shfl_xor(ej[q], 4, 64);
+      }
+      if (ii == 0 && aj >= 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (ej[q] != 0.0) hf_pe_add<AH>(h, kPairCls0 + q, aj, ej[q]);
+      }
+    }
+    if (ai >= 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (ei[q] != 0.0) hf_pe_add<AH>(h, kPairCls0 + q, ai, ei[q]);
     }
   }
   if (BULK) {
