@@ -332,6 +332,21 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
  * launches it replaced, for the 100k box and for the 64-replica batch.      */
 int pgk_md_use_graphs(pgk_ctx *ctx, int32_t enable);
 
+/* Heat-flux time series of pgk_md_run / pgk_md_run_batch (no reference
+ * counterpart as a call: the device form of the heat-flux outputs that the
+ * reference's moldyn_hf writes every heatfinterval steps, output_hf.F90:8-686,
+ * from calforce_hf + calheatf).  At step 0 and every `interval` steps the
+ * fused pass of pgk_calforce_heatf runs at the step's end state (positions
+ * and velocities after the second half kick and RATTLE), every atom its own
+ * centre (hftyp NULL or all ATOM), with the MD units: velocities in A/fs,
+ * m v^2 / 2 in eref, so the flux is in eref A fs^-1 / A^3.  Rows
+ * [nregion*PGK_NHFJ*3] stay on the device until pgk_md_heatf_series.  hf
+ * NULL or interval <= 0 turns it off.                                       */
+int pgk_md_heatf(pgk_ctx *ctx, const pgk_hf_desc *hf, int32_t interval);
+/* Rows of the last run (or batch): nrows receives their number; up to
+ * max_rows of them are copied to out.                                       */
+int pgk_md_heatf_series(pgk_ctx *ctx, double *out, int32_t max_rows, int32_t *nrows);
+
 /* Nose-Hoover chain thermostat for pgk_md_run / pgk_md_run_batch: the NVT
  * step of moldyn_nhc.F90:1238-1690 with every term on the long step, i.e.
  * nhcint (nhcint.F90:8-140), the thermostatted velocity update

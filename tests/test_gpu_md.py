@@ -204,3 +204,38 @@ def test_md_nhc_thermalises_and_conserves(prec):
     r = np.linalg.norm(p[ij[:, 0] - 1] - p[ij[:, 1] - 1], axis=1)
     assert np.max(np.abs(r - d)) < 1e-8
 
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", ["bulk", "volume"])
+def test_md_heatf_series_matches_fused_call(prec, mode):
+    """The heat-flux rows of the MD loop (steps 0, 5, 10) against the fused
+    calforce_heatf call on the same state: the start, and the end of the run."""
+    from thrid_amd import abi
+    s = water648()
+    c, _cons = _ctx(s, prec, 1.0)
+    c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=12))
+    p0, v0 = c.md_state()
+    lz = s["box"][2]
+    if mode == "bulk":
+        hf = dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([lz]))
+    else:
+        z = np.linspace(0.0, lz, 5)
+        hf = dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:])
+    c.md_heatf(hf, 5)
+    c.md_run(10, 20, s["rcut_book"], 10)
+    series = c.md_heatf_series()
+    assert series.shape == (3, len(hf["z1"]), abi.NHFJ, 3)
+    p1, v1 = c.md_state()
+    m = s["mass"] / md.MD_CONV          # m v^2 / 2 in eref, as the MD loop uses
+    end = c.calforce_heatf(hf, v1, m)["hf"]
+    c.set_positions(p0)
+    start = c.calforce_heatf(hf, v0)["hf"]
+    for got, ref in ((series[2], end), (series[0], start)):
+        scale = np.max(np.abs(ref))
+        assert scale > 0
+        assert np.max(np.abs(got - ref)) <= 1e-11 * scale
+    assert np.max(np.abs(series[2] - series[0])) > 1e-3 * np.max(np.abs(series[0]))
+    c.md_heatf(None, 0)
+    c.md_run(2, 20, s["rcut_book"], 1)
+    assert c.md_heatf_series().shape[0] == 0

@@ -18,7 +18,6 @@
 // iteration are multiplications by reciprocals taken once.
 #include "pgk_device.h"
 
-#define PGK_MD_CONV (6.9511042e-21 / 1.66053907e-17)
 
 namespace {
 

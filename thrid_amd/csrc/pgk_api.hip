@@ -1028,7 +1028,8 @@ int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
 // regions, hftyp and molecule centres, clears the per-atom sums and points
 // hfdev at them.  flux: no W arrays; the collision part is summed into
 // d_hj_jcol / d_hj_jpart with the velocities already in d_hj_vel.
-static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *who) {
+static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *who,
+                    bool upload = true) {
   if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
       !hf->z2 || !hf->hftyp) {
     pgk_set_error(std::string(who) + ": invalid heat-flux description");
@@ -1069,11 +1070,14 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   if (need_com &&
       (rc = grow_dev((void **)&c->d_hf_com, &c->hf_com_cap, sizeof(double) * 3 * hf->nmol)))
     return rc;
-  HIP_CHECK(hipMemcpyAsync(c->d_hf_z, hf->z1, sizeof(double) * nreg, hipMemcpyHostToDevice, st));
-  HIP_CHECK(hipMemcpyAsync(c->d_hf_z + nreg, hf->z2, sizeof(double) * nreg,
-                           hipMemcpyHostToDevice, st));
-  HIP_CHECK(hipMemcpyAsync(c->d_hf_typ, hf->hftyp, sizeof(int32_t) * n, hipMemcpyHostToDevice,
-                           st));
+  if (upload) {
+    HIP_CHECK(hipMemcpyAsync(c->d_hf_z, hf->z1, sizeof(double) * nreg, hipMemcpyHostToDevice,
+                             st));
+    HIP_CHECK(hipMemcpyAsync(c->d_hf_z + nreg, hf->z2, sizeof(double) * nreg,
+                             hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(c->d_hf_typ, hf->hftyp, sizeof(int32_t) * n,
+                             hipMemcpyHostToDevice, st));
+  }
   if (need_com)
     HIP_CHECK(hipMemcpyAsync(c->d_hf_com, hf->molecom, sizeof(double) * 3 * hf->nmol,
                              hipMemcpyHostToDevice, st));
@@ -1364,6 +1368,7 @@ int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
   c->md_ncons = md->ncons;
   c->md_dt = md->dt;
   c->md_tol = md->tol > 0 ? md->tol : 1e-10;
+  c->md_mass.assign(md->mass, md->mass + n);
   c->md_ready = true;
   return PGK_OK;
 }
@@ -1427,6 +1432,53 @@ static int md_sample(pgk_ctx *c, double *row) {
   row[0] = ke;
   row[1] = ene[PGK_E_TOT];
   row[2] = ke + ene[PGK_E_TOT] + eth;
+  return PGK_OK;
+}
+
+// ---- heat-flux time series of the MD loop ----------------------------------
+const pgk_mts kMdHfMts = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
+                          PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+
+// size the series for nstep steps and upload the regions once
+static int md_hf_begin(pgk_ctx *c, int nstep) {
+  c->md_hf_rows = 0;
+  if (c->md_hf_interval <= 0) return PGK_OK;
+  const int nreg = (int)c->md_hf_z1.size();
+  const size_t rows = (size_t)nstep / c->md_hf_interval + 1;
+  int rc;
+  if ((rc = grow_dev((void **)&c->d_md_hfseries, &c->md_hfseries_cap,
+                     sizeof(double) * rows * nreg * PGK_NHFJ * 3)))
+    return rc;
+  if (!c->d_hj_fscratch) {
+    std::vector<double> z((size_t)3 * c->natom, 0.0);
+    if ((rc = upload(&c->d_hj_fscratch, z))) return rc;
+  }
+  const pgk_hf_desc d = {c->md_hf_mode, nreg, c->md_hf_z1.data(), c->md_hf_z2.data(),
+                         c->md_hf_typ.data(), 0, nullptr};
+  return hf_setup(c, &d, true, "pgk_md_run");
+}
+
+// at step s (a multiple of the interval): the fused heat-flux pass at the
+// step's end state (positions x(t+dt), velocities v(t+dt)); its forces go to
+// a scratch buffer, the row to the series
+static int md_hf_sample(pgk_ctx *c, int s) {
+  if (c->md_hf_interval <= 0 || s % c->md_hf_interval != 0) return PGK_OK;
+  const int nreg = (int)c->md_hf_z1.size();
+  const pgk_hf_desc d = {c->md_hf_mode, nreg, c->md_hf_z1.data(), c->md_hf_z2.data(),
+                         c->md_hf_typ.data(), 0, nullptr};
+  int rc;
+  if ((rc = hf_setup(c, &d, true, "pgk_md_run", false))) return rc;
+  c->hfdev.vel = c->d_md_vel;
+  c->hf_export = false;
+  if ((rc = run_calforce(c, PGK_MTS_LONG, &kMdHfMts, 0, false, c->d_hj_fscratch, nullptr,
+                         nullptr, nullptr, nullptr, true, false, false)))
+    return rc;
+  if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass))) return rc;
+  c->hfdev.flux = 0;
+  const size_t row = (size_t)nreg * PGK_NHFJ * 3;
+  HIP_CHECK(hipMemcpyAsync(c->d_md_hfseries + (size_t)c->md_hf_rows * row, c->d_hj_out,
+                           sizeof(double) * row, hipMemcpyDeviceToDevice, c->stream));
+  c->md_hf_rows++;
   return PGK_OK;
 }
 
@@ -1541,7 +1593,9 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   if (rc) return rc;
   c->hf_ready = false;
   MdRunner run(c, c->md_graphs);
-  if ((rc = run.begin(rcut_book)) || (rc = md_sample(c, samples))) return rc;
+  if ((rc = md_hf_begin(c, nstep)) || (rc = run.begin(rcut_book)) ||
+      (rc = md_sample(c, samples)) || (rc = md_hf_sample(c, 0)))
+    return rc;
   int row = 1;
   for (int s = 1; s <= nstep; s++) {
     // energies only where a sample (or the caller, after the last step) reads them
@@ -1551,6 +1605,7 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
       if ((rc = md_sample(c, samples + 3 * row))) return rc;
       row++;
     }
+    if ((rc = md_hf_sample(c, s))) return rc;
   }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   return PGK_OK;
@@ -1620,8 +1675,10 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
     int rc;
     for (size_t k = 0; k < mine.size(); k++) {
       const int r = mine[k];
-      if ((rc = set_device_ctx(ctxs[r])) || (rc = run[k]->begin(rcut_book[r])) ||
-          (rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r)))
+      if ((rc = set_device_ctx(ctxs[r])) || (rc = md_hf_begin(ctxs[r], nstep)) ||
+          (rc = run[k]->begin(rcut_book[r])) ||
+          (rc = md_sample(ctxs[r], samples + (size_t)3 * nrow * r)) ||
+          (rc = md_hf_sample(ctxs[r], 0)))
         return fail(rc);
     }
     int row = 1;
@@ -1639,6 +1696,8 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
             return fail(rc);
         row++;
       }
+      for (int r : mine)
+        if ((rc = set_device_ctx(ctxs[r])) || (rc = md_hf_sample(ctxs[r], s))) return fail(rc);
     }
     for (int r : mine) {
       const hipError_t e = hipStreamSynchronize(ctxs[r]->stream);
@@ -1720,6 +1779,59 @@ int pgk_md_nhc_state(pgk_ctx *c, double *xlogs, double *vlogs) {
   for (int i = 0; i < m; i++) {
     if (xlogs) xlogs[i] = st[i];
     if (vlogs) vlogs[i] = st[m + i];
+  }
+  return PGK_OK;
+}
+
+int pgk_md_heatf(pgk_ctx *c, const pgk_hf_desc *hf, int32_t interval) {
+  if (!c || !c->md_ready) {
+    pgk_set_error("pgk_md_heatf: no MD set-up");
+    return PGK_EINVAL;
+  }
+  if (!hf || interval <= 0) {
+    c->md_hf_interval = 0;
+    return PGK_OK;
+  }
+  const int n = c->natom;
+  if (hf->mode == PGK_HF_SURFACE || hf->nregion < 1 || !hf->z1 || !hf->z2 ||
+      (hf->mode == PGK_HF_VOLUME && hf->nregion > PGK_HEATF_MAXREG)) {
+    pgk_set_error("pgk_md_heatf: volume (up to " + std::to_string(PGK_HEATF_MAXREG) +
+                  " regions) or bulk form needed");
+    return PGK_EINVAL;
+  }
+  for (int i = 0; hf->hftyp && i < n; i++)
+    if (hf->hftyp[i] != PGK_HFTYP_ATOM) {
+      pgk_set_error("pgk_md_heatf: every atom must be its own centre (hftyp ATOM)");
+      return PGK_EINVAL;
+    }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  std::vector<double> m(n);
+  for (int i = 0; i < n; i++) m[i] = c->md_mass[i] / PGK_MD_CONV;
+  if ((rc = upload(&c->d_md_hfmass, m))) return rc;
+  c->md_hf_z1.assign(hf->z1, hf->z1 + hf->nregion);
+  c->md_hf_z2.assign(hf->z2, hf->z2 + hf->nregion);
+  c->md_hf_typ.assign(n, PGK_HFTYP_ATOM);
+  c->md_hf_mode = hf->mode;
+  c->md_hf_interval = interval;
+  c->md_hf_rows = 0;
+  return PGK_OK;
+}
+
+int pgk_md_heatf_series(pgk_ctx *c, double *out, int32_t max_rows, int32_t *nrows) {
+  if (!c || !nrows || (max_rows > 0 && !out)) {
+    pgk_set_error("pgk_md_heatf_series: null argument");
+    return PGK_EINVAL;
+  }
+  const int rows = std::min<int>(c->md_hf_rows, std::max(max_rows, 0));
+  *nrows = c->md_hf_rows;
+  if (rows > 0) {
+    int rc = set_device_ctx(c);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipMemcpy(out, c->d_md_hfseries,
+                        sizeof(double) * rows * c->md_hf_z1.size() * PGK_NHFJ * 3,
+                        hipMemcpyDeviceToHost));
   }
   return PGK_OK;
 }

@@ -60,6 +60,9 @@ void pgk_set_error(const std::string &msg);
 // device view of one calforce_hf call's heat-flux set-up and accumulators
 // (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
 // else 3 = the z column only)
+// eref / (amu A^2 fs^-2): a = F/m * PGK_MD_CONV, KE = m v^2 / 2 / PGK_MD_CONV
+#define PGK_MD_CONV (6.9511042e-21 / 1.66053907e-17)
+
 // volume regions of the flux form: the pair pass keeps [4][nreg] doubles of
 // LDS per wave
 #define PGK_HEATF_MAXREG 256
@@ -216,6 +219,14 @@ struct pgk_ctx {
   double *d_hj_part = nullptr, *d_hj_out = nullptr;
   int32_t *d_hj_mol_start = nullptr, *d_hj_mol_atoms = nullptr;   // atoms of each molecule
   double *d_hj_jcol = nullptr, *d_hj_jpart = nullptr;   // flux form sums
+  // heat-flux time series of the MD loop (pgk_md_heatf)
+  int md_hf_interval = 0, md_hf_rows = 0;
+  std::vector<double> md_hf_z1, md_hf_z2;
+  std::vector<int32_t> md_hf_typ;
+  int md_hf_mode = 0;
+  double *d_md_hfseries = nullptr, *d_md_hfmass = nullptr, *d_hj_fscratch = nullptr;
+  size_t md_hfseries_cap = 0;
+  std::vector<double> md_mass;     // masses (amu) given to pgk_md_setup
   size_t hj_jcol_cap = 0, hj_jpart_cap = 0;
   int hj_nmol = 0;
   bool hj_have_mass = false;

@@ -61,6 +61,8 @@ def load_library(path: str = LIB_PATH):
     L.pgk_calforce_heatf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
                                      vp, vp, C.POINTER(abi.ForceOut), vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
+    L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
+    L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
     L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
@@ -72,7 +74,8 @@ def load_library(path: str = LIB_PATH):
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
-              "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf"):
+              "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
+              "pgk_md_heatf", "pgk_md_heatf_series"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -312,6 +315,27 @@ class ForceContext:
                         int(next), abi.dptr(k["x"]), abi.dptr(k["v"]))
         _check(self.L.pgk_md_nhc(self.h, C.byref(d)), "pgk_md_nhc")
         self._nhc_m = int(mchain)
+
+    def md_heatf(self, hf, interval):
+        """Heat-flux rows every `interval` MD steps (and at step 0): hf as
+        for calforce_hf (every atom its own centre); hf=None turns it off."""
+        if hf is None:
+            _check(self.L.pgk_md_heatf(self.h, None, 0), "pgk_md_heatf")
+            return
+        keep = self._md_hf_keep = {}
+        h = self._hf_desc(hf, keep)
+        _check(self.L.pgk_md_heatf(self.h, C.byref(h), int(interval)), "pgk_md_heatf")
+        self._md_hf_nreg = h.nregion
+
+    def md_heatf_series(self):
+        """Rows of the last md_run: (nrows, nregion, PGK_NHFJ, 3)."""
+        nreg = getattr(self, "_md_hf_nreg", 1)
+        n = C.c_int32(0)
+        _check(self.L.pgk_md_heatf_series(self.h, None, 0, C.byref(n)), "pgk_md_heatf_series")
+        out = np.zeros((max(n.value, 1), nreg, abi.NHFJ, 3))
+        _check(self.L.pgk_md_heatf_series(self.h, out.ctypes.data, n.value, C.byref(n)),
+               "pgk_md_heatf_series")
+        return out[:n.value]
 
     def md_nhc_state(self):
         """(xlogs, vlogs) of the chain, vlogs in 1/fs."""
