@@ -257,12 +257,14 @@ def force_error(f, ref):
             "max_rel": float(d.max() / rms_ref)}
 
 
-def run_ensemble(reps, steps, warmup):
+def run_ensemble(reps, steps, warmup, hf_every):
     """The batched Green-Kubo ensemble leg: every replica relaxed from its
     generated start, then `steps` mixed-precision MD steps of all replicas run
-    one replica after another and then stepped together (pgk_md_run_batch).
-    Returns the two throughputs in atom-steps/s."""
-    from thrid_amd import md
+    one replica after another and then stepped together (pgk_md_run_batch),
+    with the box's heat-flux vector (bulk form, one region) sampled every
+    `hf_every` steps on the device (0: none).  Returns the two throughputs in
+    atom-steps/s."""
+    from thrid_amd import abi, md
     from thrid_amd.pgk import ForceContext, md_run_batch
     ctxs, cons = [], []
     t0 = time.perf_counter()
@@ -272,6 +274,9 @@ def run_ensemble(reps, steps, warmup):
         md.relax(c, s, cn)
         c.md_setup(s["mass"], *cn, DT_FS)
         c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=100 + k))
+        if hf_every > 0:
+            c.md_heatf(dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([s["box"][2]])),
+                       hf_every)
         ctxs.append(c)
         cons.append(cn)
     t_setup = time.perf_counter() - t0
@@ -291,11 +296,14 @@ def run_ensemble(reps, steps, warmup):
     t0 = time.perf_counter()
     md_run_batch(ctxs, steps, NSTEP_BOOK, books, steps)
     t_batch = time.perf_counter() - t0
+    nhf = [c.md_heatf_series().shape[0] for c in ctxs] if hf_every > 0 else [0]
     for c in ctxs:
         c.close()
     return {"replicas": len(reps), "atoms": natoms,
             "sizes": [int(min(s["natom"] for s in reps)), int(max(s["natom"] for s in reps))],
             "steps": steps, "precision": "mixed",
+            "heat_flux_every": hf_every,
+            "heat_flux_rows_per_replica": int(min(nhf)),
             "sequential_atom_steps_per_s": natoms * steps / t_seq,
             "batched_atom_steps_per_s": natoms * steps / t_batch,
             "batched_over_sequential": t_seq / t_batch,
@@ -318,6 +326,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64, metavar="NREP",
                     help="replicas in the batched-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--ensemble-hf", type=int, default=1, metavar="K",
+                    help="heat-flux sample every K ensemble steps (0: none)")
     ap.add_argument("--no-heatf", action="store_true", help="skip the heat-flux leg")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -409,9 +419,10 @@ def main():
                 f"{v['fused_calforce_heatf_ms']:.2f} ms)"
                 for k, v in h.items()), file=sys.stderr, flush=True)
     if reps:
-        out["ensemble"] = run_ensemble(reps, args.ensemble_steps, 10)
+        out["ensemble"] = run_ensemble(reps, args.ensemble_steps, 10, args.ensemble_hf)
         e = out["ensemble"]
-        print(f"# ensemble: {e['replicas']} replicas, {e['atoms']} atoms: sequential "
+        print(f"# ensemble: {e['replicas']} replicas, {e['atoms']} atoms, heat flux every "
+              f"{e['heat_flux_every']} steps: sequential "
               f"{e['sequential_atom_steps_per_s']:.3e}, batched {e['batched_atom_steps_per_s']:.3e} "
               f"atom-steps/s ({e['batched_over_sequential']:.2f}x)", file=sys.stderr, flush=True)
     if not args.no_cpu:

@@ -223,10 +223,21 @@ def test_md_heatf_series_matches_fused_call(prec, mode):
         z = np.linspace(0.0, lz, 5)
         hf = dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:])
     c.md_heatf(hf, 5)
-    c.md_run(10, 20, s["rcut_book"], 10)
+    rows = c.md_run(10, 20, s["rcut_book"], 5)
     series = c.md_heatf_series()
     assert series.shape == (3, len(hf["z1"]), abi.NHFJ, 3)
     p1, v1 = c.md_state()
+    ene = c.energies()
+    # sampling the heat flux leaves the trajectory and the energy records alone
+    c2, _ = _ctx(s, prec, 1.0)
+    c2.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=12))
+    rows2 = c2.md_run(10, 20, s["rcut_book"], 5)
+    p2, v2 = c2.md_state()
+    ene2 = c2.energies()
+    c2.close()
+    tol = 0.0 if prec == "mixed" else 1e-12
+    for a, b in ((rows, rows2), (p1, p2), (v1, v2), (ene[0], ene2[0])):
+        assert np.max(np.abs(a - b)) <= tol * max(1.0, np.max(np.abs(b)))
     m = s["mass"] / md.MD_CONV          # m v^2 / 2 in eref, as the MD loop uses
     end = c.calforce_heatf(hf, v1, m)["hf"]
     c.set_positions(p0)

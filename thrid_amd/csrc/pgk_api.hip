@@ -246,7 +246,10 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     return PGK_ESTATE;
   }
   const bool do_pair = (m->vdw == mts_flag);
-  if (do_pair && !c->have_list) {
+  // the heat-flux pass over the listed pairs: with the pair forces, or on its
+  // own (hf_pairs_only: the MD loop's heat-flux samples, forces already known)
+  const bool hf_pairs = hf && (do_pair || c->hf_pairs_only);
+  if ((do_pair || hf_pairs) && !c->have_list) {
     pgk_set_error("pgk_calforce: neighbour list not built");
     return PGK_ESTATE;
   }
@@ -316,7 +319,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   pgk_prof_begin(c, PGK_PROF_BONDED);
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
-  if (hf && do_pair && (rc = pgk_hf_launch_pairs(c, true))) return rc;
+  if (hf_pairs && (rc = pgk_hf_launch_pairs(c, true))) return rc;
   // forces: accumulators + excluded-pair correction -> FP64 output
   double *fout = d_force_out ? d_force_out : c->d_force;
   {
@@ -336,20 +339,23 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   // (a dynamics step that samples nothing sums them on the device but skips
   // the copy)
   if (nseg > 0) {
-    int *dseg = reinterpret_cast<int *>(c->d_sum + 512);
-    int *hseg = reinterpret_cast<int *>(c->h_sum + 512);   // pinned staging
+    // two table slots: the MD loop's heat-flux pass alternates with the
+    // step's own call without re-uploading either table
+    const int slot = c->hf_pairs_only ? 1 : 0;
+    int *dseg = reinterpret_cast<int *>(c->d_sum + 512 + 64 * slot);
+    int *hseg = reinterpret_cast<int *>(c->h_sum + 512 + 64 * slot);   // pinned staging
     int tab[32] = {};
     for (int k = 0; k < nseg; k++) {
       tab[k] = seg_off[k];
       tab[16 + k] = seg_n[k];
     }
-    if (c->seg_up_n != nseg || memcmp(tab, c->seg_up, sizeof(tab)) != 0) {
+    if (c->seg_up_n[slot] != nseg || memcmp(tab, c->seg_up[slot], sizeof(tab)) != 0) {
       // an earlier queued upload may still read the staging area
       HIP_CHECK(hipStreamSynchronize(st));
       memcpy(hseg, tab, sizeof(tab));
       HIP_CHECK(hipMemcpyAsync(dseg, hseg, 32 * sizeof(int), hipMemcpyHostToDevice, st));
-      memcpy(c->seg_up, tab, sizeof(tab));
-      c->seg_up_n = nseg;
+      memcpy(c->seg_up[slot], tab, sizeof(tab));
+      c->seg_up_n[slot] = nseg;
     }
     k_reduce_segments<<<nseg * PGK_REC, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
     HIP_CHECK(hipGetLastError());
@@ -1436,8 +1442,10 @@ static int md_sample(pgk_ctx *c, double *row) {
 }
 
 // ---- heat-flux time series of the MD loop ----------------------------------
+// the bonded terms (their heat-flux parts) and the pair heat-flux pass; the
+// pair and reciprocal forces of the step are already known
 const pgk_mts kMdHfMts = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
-                          PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+                          0, 0, PGK_MTS_LONG};
 
 // size the series for nstep steps and upload the regions once
 static int md_hf_begin(pgk_ctx *c, int nstep) {
@@ -1470,9 +1478,22 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   if ((rc = hf_setup(c, &d, true, "pgk_md_run", false))) return rc;
   c->hfdev.vel = c->d_md_vel;
   c->hf_export = false;
-  if ((rc = run_calforce(c, PGK_MTS_LONG, &kMdHfMts, 0, false, c->d_hj_fscratch, nullptr,
-                         nullptr, nullptr, nullptr, true, false, false)))
-    return rc;
+  c->hf_pairs_only = true;
+  // the energy records of the step stay the ones pgk_energies reports
+  const int last_nseg = c->last_nseg, last_flag = c->last_mts_flag;
+  int last_kind[16];
+  memcpy(last_kind, c->last_seg_kind, sizeof(last_kind));
+  const pgk_mts last_mts = c->last_mts;
+  const bool last_pending = c->last_pending;
+  rc = run_calforce(c, PGK_MTS_LONG, &kMdHfMts, 0, false, c->d_hj_fscratch, nullptr, nullptr,
+                    nullptr, nullptr, true, false, false);
+  c->hf_pairs_only = false;
+  c->last_nseg = last_nseg;
+  c->last_mts_flag = last_flag;
+  memcpy(c->last_seg_kind, last_kind, sizeof(last_kind));
+  c->last_mts = last_mts;
+  c->last_pending = last_pending;
+  if (rc) return rc;
   if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass))) return rc;
   c->hfdev.flux = 0;
   const size_t row = (size_t)nreg * PGK_NHFJ * 3;

@@ -57,9 +57,6 @@ enum {
 
 void pgk_set_error(const std::string &msg);
 
-// device view of one calforce_hf call's heat-flux set-up and accumulators
-// (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
-// else 3 = the z column only)
 // eref / (amu A^2 fs^-2): a = F/m * PGK_MD_CONV, KE = m v^2 / 2 / PGK_MD_CONV
 #define PGK_MD_CONV (6.9511042e-21 / 1.66053907e-17)
 
@@ -67,6 +64,9 @@ void pgk_set_error(const std::string &msg);
 // LDS per wave
 #define PGK_HEATF_MAXREG 256
 
+// device view of one calforce_hf call's heat-flux set-up and accumulators
+// (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
+// else 3 = the z column only)
 struct HfDev {
   int mode = 0, nreg = 0, natom = 0, nw = 3;
   const double *z1 = nullptr, *z2 = nullptr;   // [nreg]
@@ -214,6 +214,7 @@ struct pgk_ctx {
   size_t hf_z_cap = 0, hf_com_cap = 0, hf_w_cap = 0, hf_wout_cap = 0;
   bool hf_ready = false;           // a pgk_calforce_hf call left its arrays on the device
   bool hf_export = true;           // run_calforce writes the caller-layout HF arrays
+  bool hf_pairs_only = false;      // run_calforce: pair heat-flux pass without pair forces
   // calheatf (pgk_calheatf): velocities, masses, molecule lists, partial records
   double *d_hj_vel = nullptr, *d_hj_mass = nullptr, *d_hj_molevel = nullptr;
   double *d_hj_part = nullptr, *d_hj_out = nullptr;
@@ -272,8 +273,8 @@ struct pgk_ctx {
   // call waits): its reduction segments and the inputs of the energy record
   int last_nseg = 0;
   int last_seg_kind[16] = {};
-  int seg_up[32] = {};             // segment table currently on the device
-  int seg_up_n = -1;
+  int seg_up[2][32] = {};          // segment tables currently on the device (2 slots)
+  int seg_up_n[2] = {-1, -1};
   int32_t last_mts_flag = 0;
   pgk_mts last_mts = {};
   bool last_pending = false;
