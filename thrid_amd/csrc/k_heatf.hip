@@ -120,31 +120,45 @@ __global__ __launch_bounds__(256) void k_heatf_collision(HeatfArgs a, double *__
   block_record<3, 4>(acc, part + ((size_t)row * a.nsplit + split) * 3);
 }
 
-// partials in chunk order, the volume factor, the total (calheatf.F90:1334-1436)
-__global__ __launch_bounds__(64) void k_heatf_final(HeatfArgs a, const double *__restrict__ ptr,
-                                                    const double *__restrict__ pcol,
-                                                    double *__restrict__ out) {
+// partials in chunk order, the volume factor, the total (calheatf.F90:1334-1436).
+// One block per region; lane l < 51 of each wave owns (term l / 3, component
+// l % 3), wave w sums the chunks k = w, w + 4, ... and wave 0 adds the four
+// partial sums in order, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_heatf_final(HeatfArgs a, const double *__restrict__ ptr,
+                                                     const double *__restrict__ pcol,
+                                                     double *__restrict__ out) {
 #pragma clang fp contract(off)
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.nreg * 3) return;
-  const int r = idx / 3, c = idx % 3;
-  const double vol_inv = ((1.0 / a.L[0]) * (1.0 / a.L[1])) / (a.z2[r] - a.z1[r]);
-  double *o = out + (size_t)r * PGK_NHFJ * 3;
-  double tot = 0.0;
-  for (int t = 0; t < PGK_HFJ_TOT; t++) {
+  constexpr int kTc = PGK_HFJ_TOT * 3;   // 51 (term, component) pairs
+  __shared__ double part[4][kTc];
+  __shared__ double fin[kTc];
+  const int r = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane < kTc) {
+    const int t = lane / 3, c = lane % 3;
     double s = 0.0;
     if (t < kNterm) {
-      for (int k = 0; k < a.nsplit; k++)
+      for (int k = wave; k < a.nsplit; k += 4)
         s += ptr[((size_t)r * a.nsplit + k) * (kNterm * 3) + 3 * t + c];
     } else {
       const size_t row = (size_t)(t - kNterm) * a.nreg + r;
-      for (int k = 0; k < a.nsplit_col; k++) s += pcol[(row * a.nsplit_col + k) * 3 + c];
+      for (int k = wave; k < a.nsplit_col; k += 4) s += pcol[(row * a.nsplit_col + k) * 3 + c];
     }
-    s = s * vol_inv;
-    o[3 * t + c] = s;
-    tot = (t == 0) ? s : tot + s;
+    part[wave][lane] = s;
   }
-  o[3 * PGK_HFJ_TOT + c] = tot;
+  __syncthreads();
+  const double vol_inv = ((1.0 / a.L[0]) * (1.0 / a.L[1])) / (a.z2[r] - a.z1[r]);
+  double *o = out + (size_t)r * PGK_NHFJ * 3;
+  if (threadIdx.x < kTc) {
+    const double s = (((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                      part[3][threadIdx.x]) * vol_inv;
+    fin[threadIdx.x] = s;
+    o[threadIdx.x] = s;   // term t, component c at 3 t + c
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double tot = fin[threadIdx.x];
+    for (int t = 1; t < PGK_HFJ_TOT; t++) tot = tot + fin[3 * t + threadIdx.x];
+    o[3 * PGK_HFJ_TOT + threadIdx.x] = tot;
+  }
 }
 
 // flux form: collision row (class, region) = the bonded terms' atomics in
@@ -236,8 +250,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
     default: rc = launch_heatf<PGK_PREC_FP32>(c, a, ptr, pcol); break;
   }
   if (rc) return rc;
-  k_heatf_final<<<pgk_grid_blocks(h.nreg * 3, 64), 64, 0, c->stream>>>(a, ptr, pcol,
-                                                                        c->d_hj_out);
+  k_heatf_final<<<h.nreg, 256, 0, c->stream>>>(a, ptr, pcol, c->d_hj_out);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

@@ -144,11 +144,18 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
     const int ii = lane & 7, jq = (lane >> 3) & 3, hs = lane >> 5;
     const int ai = p.cl_atom[ci * PGK_CL + ii];
     // padding slots (ai = -1) have no mask bits and read nothing
-    double cai[3] = {0.0, 0.0, 0.0}, vi[3] = {0.0, 0.0, 0.0};
-    int fla = 0;
-    if (ai >= 0) {
+    double cai[3] = {0.0, 0.0, 0.0}, vi[3] = {0.0, 0.0, 0.0}, pi[3] = {0.0, 0.0, 0.0};
+    double qi = 0.0;
+    int fla = 0, moli = 0, typi = 1;
+    if (ai >= 0) {   // the i atom's data, loaded once for the whole list
       fla = hf_centre(h, ai, BULK, cai);
-      for (int d = 0; d < 3; d++) vi[d] = h.vel[3 * ai + d];
+      for (int d = 0; d < 3; d++) {
+        vi[d] = h.vel[3 * ai + d];
+        pi[d] = h.pos[3 * ai + d];
+      }
+      qi = p.q[ai];
+      moli = p.molid[ai];
+      typi = p.type[ai];
     }
     // per-atom energies: the i atom's shares stay in registers for the whole
     // list; the j atom's are summed over the 8 lanes that share it (lanes
@@ -164,15 +171,15 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
       if ((m >> (lane & 31)) & 1u) {
         double r[3];
         for (int d = 0; d < 3; d++) {
-          r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
+          r[d] = pi[d] - h.pos[3 * aj + d];
           r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
         }
         const double r2 = r2_exact(r);
         if (r2 <= p.sqcut) {
-          const double rinv2 = 1.0 / r2;
           const double ra = sqrt(r2);
           const double rinv1 = 1.0 / ra;
-          const bool same = p.molid[ai] == p.molid[aj];
+          const double rinv2 = rinv1 * rinv1;   // one division instead of two
+          const bool same = moli == p.molid[aj];
           // the centres' separation and, in the volume form, whether it counts
           double caj[3];
           const int fl = fla + hf_centre(h, aj, BULK, caj);
@@ -191,13 +198,13 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
           const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cai[2], caj[2], rz};
           if (ra <= p.rrcut) {
-            const double qq = p.q[ai] * p.q[aj];
+            const double qq = qi * p.q[aj];
             const double ph = qq * rinv1;
             if (same) flux_add<0, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
             else flux_add<2, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
           }
           if (r2 <= p.sqcut2) {
-            const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
+            const size_t tij = (size_t)(typi - 1) + (size_t)p.ntype * (p.type[aj] - 1);
             if (p.inttyp[tij] == 1) {
               const double rad = p.rmin[tij];
               const double t2 = rinv2 * (rad * rad);
