@@ -110,9 +110,9 @@ struct FluxPair {
 // e and force coefficient f (force on i = f r): half the energy to each
 // atom, and f (r . (v_i + v_j)) / 2 times the region weights to the flux
 template <int Q, bool BULK>
-__device__ __forceinline__ void flux_add(const HfDev &h, const FluxPair &fp, double e, double f,
+__device__ __forceinline__ void flux_add(const FluxPair &fp, double e, double f,
                                          double (&ei)[4], double (&ej)[4],
-                                         double (&acc)[4][3], double *sl) {
+                                         double (&acc)[4][3], double &sv_out) {
   ei[Q] += 0.5 * e;
   ej[Q] += 0.5 * e;
   if (!fp.seg) return;
@@ -121,10 +121,22 @@ __device__ __forceinline__ void flux_add(const HfDev &h, const FluxPair &fp, dou
 #pragma unroll
     for (int n = 0; n < 3; n++) acc[Q][n] += fp.rc[n] * sv;
   } else {
-    for (int ir = 0; ir < h.nreg; ir++) {
-      const double ff = hf_volume_weight(h, ir, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
-      if (ff != 0.0) atomicAdd(&sl[Q * h.nreg + ir], ff * sv);
-    }
+    sv_out = sv;   // the region walk adds it once per pair, for both classes
+  }
+}
+
+// volume form: the pair's Coulomb (class qc) and LJ (class ql) shares times
+// the weight of every region its centres' segment crosses
+__device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp, int qc,
+                                             double svc, int ql, double svl, double *sl) {
+  if (!fp.seg || (svc == 0.0 && svl == 0.0)) return;
+  const double lo = fmin(fp.ciz, fp.cjz), hi = fmax(fp.ciz, fp.cjz);
+  for (int ir = 0; ir < h.nreg; ir++) {
+    if (hi < h.z1[ir] || lo > h.z2[ir]) continue;   // the segment misses the region
+    const double ff = hf_volume_weight(h, ir, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
+    if (ff == 0.0) continue;
+    if (svc != 0.0) atomicAdd(&sl[qc * h.nreg + ir], ff * svc);
+    if (svl != 0.0) atomicAdd(&sl[ql * h.nreg + ir], ff * svl);
   }
 }
 
@@ -197,11 +209,12 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           const double w[3] = {vi[0] + vjp[0], vi[1] + vjp[1], vi[2] + vjp[2]};
           const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
           const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cai[2], caj[2], rz};
+          double svc = 0.0, svl = 0.0;
           if (ra <= p.rrcut) {
             const double qq = qi * p.q[aj];
             const double ph = qq * rinv1;
-            if (same) flux_add<0, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
-            else flux_add<2, BULK>(h, fp, ph, ph * rinv2, ei, ej, acc, sl);
+            if (same) flux_add<0, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
+            else flux_add<2, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
           }
           if (r2 <= p.sqcut2) {
             const size_t tij = (size_t)(typi - 1) + (size_t)p.ntype * (p.type[aj] - 1);
@@ -213,10 +226,11 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
               const double ep = p.eps[tij];
               const double pot = ep * (t12 - 2.0 * t6);
               const double ft = 12.0 * ep * (t12 - t6) * rinv2;
-              if (same) flux_add<1, BULK>(h, fp, pot, ft, ei, ej, acc, sl);
-              else flux_add<3, BULK>(h, fp, pot, ft, ei, ej, acc, sl);
+              if (same) flux_add<1, BULK>(fp, pot, ft, ei, ej, acc, svl);
+              else flux_add<3, BULK>(fp, pot, ft, ei, ej, acc, svl);
             }
           }
+          if (!BULK) flux_regions(h, fp, same ? 0 : 2, svc, same ? 1 : 3, svl, sl);
         }
       }
 #pragma unroll
