@@ -371,6 +371,34 @@ int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
 /* Current chain state: xlogs[mchain], vlogs[mchain] (1/fs); NULL skips.      */
 int pgk_md_nhc_state(pgk_ctx *ctx, double *xlogs, double *vlogs);
 
+/* ---- on-disk formats (host only; no device needed) ------------------------
+ * The state file of wrsta / rdstarec (wrsta.F90:8-180, rdstarec.F90:8-393):
+ * counts, box (m), step, one line per atom (x y z in m, vx vy vz in m/s,
+ * atom type) in three sections (poly, water, monatomic molecules), the
+ * Nose-Hoover chain (xlogs, vlogs in 1/s), the Andersen barostat (vlogv in
+ * 1/s, pint in Pa) and its anisotropic form (vboxg(3) in 1/s, pintt(3,3) in
+ * Pa).  Written with the reference's edit descriptors, so a file read and
+ * written back is byte-identical.                                            */
+typedef struct pgk_sta_header {
+  int32_t npoly, nwater, nmatom, natom, mchain;
+  int32_t nat_poly, nat_water, nat_matom;   /* atom lines in each section    */
+  double box[3];                            /* xcel, ycel, zcel (m)          */
+  int64_t nstep;
+} pgk_sta_header;
+int pgk_sta_read_header(const char *path, pgk_sta_header *h);
+/* pos, vel [3*natom]; types [4*natom] (each up to 3 characters, NUL-padded);
+ * nhc [2*mchain] (xlogs, vlogs pairs); andersen [2]; aniso [12].            */
+int pgk_sta_read(const char *path, const pgk_sta_header *h, double *pos, double *vel,
+                 char *types, double *nhc, double *andersen, double *aniso);
+int pgk_sta_write(const char *path, const pgk_sta_header *h, const double *pos,
+                  const double *vel, const char *types, const double *nhc,
+                  const double *andersen, const double *aniso);
+/* One data row of out_ene.dat as outene writes it (output_data.F90:9-342):
+ * (I10, nvals(1X,E15.8)), or E23.16 with high != 0 (-D_OUTENE_HIGH), with a
+ * newline, into buf (cap bytes).                                             */
+int pgk_out_ene_row(int32_t step, const double *vals, int32_t nvals, int32_t high, char *buf,
+                    int32_t cap);
+
 /* Stage timing (no reference counterpart: instrumentation for measurement).
  * enable != 0 turns HIP-event timing of each stage on (and clears the
  * totals); ms[PGK_NPROF_STAGES] and calls[PGK_NPROF_STAGES] receive the

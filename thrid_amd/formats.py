@@ -1,0 +1,82 @@
+"""The reference's on-disk formats through libpgkforce.so (host code, no GPU):
+the wrsta / rdstarec state file and outene's energy rows.  Values stay in
+the file's SI units (m, m/s, 1/s, Pa)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import pgk
+
+
+class StaHeader(C.Structure):
+    _fields_ = [("npoly", C.c_int32), ("nwater", C.c_int32), ("nmatom", C.c_int32),
+                ("natom", C.c_int32), ("mchain", C.c_int32), ("nat_poly", C.c_int32),
+                ("nat_water", C.c_int32), ("nat_matom", C.c_int32),
+                ("box", C.c_double * 3), ("nstep", C.c_int64)]
+
+
+def _lib():
+    L = pgk.load_library()
+    vp = C.c_void_p
+    L.pgk_sta_read_header.argtypes = [C.c_char_p, C.POINTER(StaHeader)]
+    L.pgk_sta_read.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
+    L.pgk_sta_write.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
+    L.pgk_out_ene_row.argtypes = [C.c_int32, vp, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    for f in ("pgk_sta_read_header", "pgk_sta_read", "pgk_sta_write", "pgk_out_ene_row"):
+        getattr(L, f).restype = C.c_int
+    return L
+
+
+def read_sta(path):
+    """State file -> dict (header fields, pos/vel (natom, 3), types, nhc
+    (mchain, 2) = (xlogs, vlogs), andersen (2,), aniso (12,))."""
+    L = _lib()
+    h = StaHeader()
+    pgk._check(L.pgk_sta_read_header(str(path).encode(), C.byref(h)), "pgk_sta_read_header")
+    n, m = h.natom, h.mchain
+    pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+    types = C.create_string_buffer(4 * max(n, 1))
+    nhc, ander, aniso = np.zeros((max(m, 1), 2)), np.zeros(2), np.zeros(12)
+    pgk._check(L.pgk_sta_read(str(path).encode(), C.byref(h), pos.ctypes.data, vel.ctypes.data,
+                              C.addressof(types), nhc.ctypes.data, ander.ctypes.data,
+                              aniso.ctypes.data), "pgk_sta_read")
+    raw = types.raw
+    return dict(npoly=h.npoly, nwater=h.nwater, nmatom=h.nmatom, natom=n, mchain=m,
+                sections=(h.nat_poly, h.nat_water, h.nat_matom), box=np.array(h.box[:]),
+                nstep=h.nstep, pos=pos, vel=vel,
+                types=[raw[4 * i:4 * i + 4].rstrip(b"\0").decode() for i in range(n)],
+                nhc=nhc[:m], andersen=ander, aniso=aniso)
+
+
+def write_sta(path, d):
+    L = _lib()
+    h = StaHeader()
+    h.npoly, h.nwater, h.nmatom = d["npoly"], d["nwater"], d["nmatom"]
+    h.natom, h.mchain = d["natom"], d["mchain"]
+    h.nat_poly, h.nat_water, h.nat_matom = d["sections"]
+    h.box[:] = list(d["box"])
+    h.nstep = int(d["nstep"])
+    pos = np.ascontiguousarray(d["pos"], np.float64)
+    vel = np.ascontiguousarray(d["vel"], np.float64)
+    types = b"".join(t.encode()[:3].ljust(4, b"\0") for t in d["types"])
+    tb = C.create_string_buffer(types, max(len(types), 4))
+    nhc = np.ascontiguousarray(d["nhc"], np.float64).reshape(-1, 2)
+    if nhc.shape[0] == 0:
+        nhc = np.zeros((1, 2))
+    ander = np.ascontiguousarray(d["andersen"], np.float64)
+    aniso = np.ascontiguousarray(d["aniso"], np.float64)
+    pgk._check(L.pgk_sta_write(str(path).encode(), C.byref(h), pos.ctypes.data, vel.ctypes.data,
+                               C.addressof(tb), nhc.ctypes.data, ander.ctypes.data,
+                               aniso.ctypes.data), "pgk_sta_write")
+
+
+def out_ene_row(step, vals, high=False):
+    """One out_ene.dat data row (with its newline)."""
+    L = _lib()
+    v = np.ascontiguousarray(vals, np.float64)
+    buf = C.create_string_buffer(32 * (len(v) + 2))
+    pgk._check(L.pgk_out_ene_row(int(step), v.ctypes.data, len(v), int(bool(high)), buf,
+                                 len(buf)), "pgk_out_ene_row")
+    return buf.value.decode()
