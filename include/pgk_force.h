@@ -399,6 +399,24 @@ int pgk_sta_write(const char *path, const pgk_sta_header *h, const double *pos,
 int pgk_out_ene_row(int32_t step, const double *vals, int32_t nvals, int32_t high, char *buf,
                     int32_t cap);
 
+/* The .ini control script (rdscript.F90 with rdfree.F90's word splitting):
+ * each line's words split at spaces, commas and tabs, the first word the key,
+ * reading stops at a line starting with END.  Keys may repeat (occurrence
+ * occ counts from 0); value idx 0 is the word after the key.  Reals accept
+ * Fortran d/D exponents, logicals [.]T... / [.]F... as list-directed input. */
+typedef struct pgk_ini pgk_ini;
+int pgk_ini_read(const char *path, pgk_ini **ini);
+void pgk_ini_free(pgk_ini *ini);
+int32_t pgk_ini_count(const pgk_ini *ini, const char *key);
+int32_t pgk_ini_nvalues(const pgk_ini *ini, const char *key, int32_t occ);  /* -1: absent */
+int pgk_ini_get_str(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx, char *buf,
+                    int32_t cap);
+int pgk_ini_get_double(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                       double *v);
+int pgk_ini_get_int(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx, int64_t *v);
+int pgk_ini_get_bool(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                     int32_t *v);
+
 /* Stage timing (no reference counterpart: instrumentation for measurement).
  * enable != 0 turns HIP-event timing of each stage on (and clears the
  * totals); ms[PGK_NPROF_STAGES] and calls[PGK_NPROF_STAGES] receive the

@@ -64,3 +64,28 @@ def test_fortran_e_edge_cases(v, w, tmp_path):
     formats.write_sta(p, d)
     line = [l for l in p.read_text().splitlines() if l.startswith("#xcel")][0]
     assert line == "#xcel" + w
+
+
+def test_ini_words_of_the_reference_script():
+    """peachgk.ini as rdscript reads it: keys, repeated keys, Fortran reals
+    with d exponents, logicals, trailing comments, END."""
+    from thrid_amd.pgk import PgkError
+    ini = formats.Ini(os.path.join(GOLD, "peachgk.ini"))
+    assert ini.float("xcel") == 38.595000000e-10
+    assert ini.float("dt_long_cal") == 1.0e-15
+    assert ini.int("nstep_short") == 5
+    assert ini.str("mts_vdw") == "long" and ini.str("mts_bond") == "short"
+    assert ini.str("ouenename") == "out_ene.dat" and ini.bool("ouenename", 1) is True
+    assert ini.bool("ouforname", 1) is False
+    assert ini.bool("ifstarec") is False and ini.bool("ifcreatecor") is True
+    assert ini.count("npolymoletyp") == 2
+    assert [ini.int("npolymoletyp", k, occ=1) for k in range(3)] == [2, 190, 14]
+    assert ini.str("npolymoletyp", 4, occ=1) == "C4H10_top.dat"
+    assert ini.float("r_expand") == 1.1111111     # the "! expansion ratio ..." comment follows
+    assert ini.count("nmatomtyp") == 6 and ini.str("nmatomtyp", 4, occ=1) == "localheat" and ini.float("nmatomtyp", 5, occ=1) == 146.0
+    assert ini.count("no_such_key") == 0 and ini.nvalues("no_such_key") == -1
+    with pytest.raises(PgkError):
+        ini.float("mts_vdw")
+    with pytest.raises(PgkError):
+        ini.int("npolytyp", 3)
+    ini.close()

@@ -1,7 +1,8 @@
 // pgk_io.hip - the reference's on-disk formats on the host, for drivers that
 // keep the MD loop on the device: the state file of wrsta / rdstarec
-// (wrsta.F90:8-180, rdstarec.F90:8-393) and the energy rows of outene
-// (output_data.F90:9-342).  Host code only; the files are written with the
+// (wrsta.F90:8-180, rdstarec.F90:8-393), the energy rows of outene
+// (output_data.F90:9-342) and the words of the .ini control script
+// (rdscript.F90, rdfree.F90).  Host code only; the files are written with the
 // reference's Fortran edit descriptors (Iw, Aw right-justified, Ew.d as
 // 0.d...dE+ee), so a state file read and written back is byte-identical.
 #include <cerrno>
@@ -305,6 +306,144 @@ int pgk_out_ene_row(int32_t step, const double *vals, int32_t nvals, int32_t hig
     return PGK_EINVAL;
   }
   memcpy(buf, l.c_str(), l.size() + 1);
+  return PGK_OK;
+}
+
+}  // extern "C"
+
+// ---- the .ini control script (rdscript.F90, words split as rdfree.F90) -----
+// Every line becomes its words, split at spaces, commas and tabs; the first
+// word is the key (lines starting with '#' never match a key the caller asks
+// for); reading stops at a line whose first word begins with END.  Values
+// are read as Fortran list-directed input reads them: reals with d/D
+// exponents, logicals as [.]T... / [.]F....
+struct pgk_ini {
+  std::vector<std::vector<std::string>> lines;   // words of each line, key first
+};
+
+namespace {
+
+const std::vector<std::string> *ini_word_line(const pgk_ini *ini, const char *key, int32_t occ) {
+  if (!ini || !key || occ < 0) return nullptr;
+  int32_t seen = 0;
+  for (const auto &l : ini->lines)
+    if (l[0] == key && seen++ == occ) return &l;
+  return nullptr;
+}
+
+const std::string *ini_value(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                             const char *who) {
+  const std::vector<std::string> *l = ini_word_line(ini, key, occ);
+  if (!l || idx < 0 || (size_t)idx + 1 >= l->size()) {
+    pgk_set_error(std::string(who) + ": no value " + std::to_string(idx) + " of occurrence " +
+                  std::to_string(occ) + " of '" + (key ? key : "") + "'");
+    return nullptr;
+  }
+  return &(*l)[idx + 1];
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgk_ini_read(const char *path, pgk_ini **out) {
+  if (!path || !out) {
+    pgk_set_error("pgk_ini_read: null argument");
+    return PGK_EINVAL;
+  }
+  std::vector<std::string> lines;
+  if (!read_lines(path, lines)) {
+    pgk_set_error(std::string("pgk_ini_read: cannot open ") + path);
+    return PGK_EINVAL;
+  }
+  pgk_ini *ini = new pgk_ini;
+  for (const std::string &s : lines) {
+    std::vector<std::string> w;
+    std::string cur;
+    for (char ch : s) {
+      if (ch == ' ' || ch == ',' || ch == '\t' || ch == '\r') {
+        if (!cur.empty()) w.push_back(cur);
+        cur.clear();
+      } else {
+        cur.push_back(ch);
+      }
+    }
+    if (!cur.empty()) w.push_back(cur);
+    if (w.empty()) continue;
+    if (w[0].compare(0, 3, "END") == 0) break;
+    ini->lines.push_back(w);
+  }
+  *out = ini;
+  return PGK_OK;
+}
+
+void pgk_ini_free(pgk_ini *ini) { delete ini; }
+
+int32_t pgk_ini_count(const pgk_ini *ini, const char *key) {
+  int32_t n = 0;
+  if (ini && key)
+    for (const auto &l : ini->lines) n += (l[0] == key) ? 1 : 0;
+  return n;
+}
+
+int32_t pgk_ini_nvalues(const pgk_ini *ini, const char *key, int32_t occ) {
+  const std::vector<std::string> *l = ini_word_line(ini, key, occ);
+  return l ? (int32_t)l->size() - 1 : -1;
+}
+
+int pgk_ini_get_str(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx, char *buf,
+                    int32_t cap) {
+  const std::string *v = ini_value(ini, key, occ, idx, "pgk_ini_get_str");
+  if (!v) return PGK_EINVAL;
+  if (!buf || (int64_t)v->size() + 1 > cap) {
+    pgk_set_error("pgk_ini_get_str: buffer too small");
+    return PGK_EINVAL;
+  }
+  memcpy(buf, v->c_str(), v->size() + 1);
+  return PGK_OK;
+}
+
+int pgk_ini_get_double(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                       double *out) {
+  const std::string *v = ini_value(ini, key, occ, idx, "pgk_ini_get_double");
+  if (!v || !out) return PGK_EINVAL;
+  std::string s = *v;
+  for (char &ch : s)
+    if (ch == 'd' || ch == 'D') ch = 'e';
+  if (!to_double(s, out)) {
+    pgk_set_error("pgk_ini_get_double: '" + *v + "' is not a real");
+    return PGK_EINVAL;
+  }
+  return PGK_OK;
+}
+
+int pgk_ini_get_int(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                    int64_t *out) {
+  const std::string *v = ini_value(ini, key, occ, idx, "pgk_ini_get_int");
+  if (!v || !out) return PGK_EINVAL;
+  char *end = nullptr;
+  errno = 0;
+  const long long x = strtoll(v->c_str(), &end, 10);
+  if (!end || *end != '\0' || errno == ERANGE) {
+    pgk_set_error("pgk_ini_get_int: '" + *v + "' is not an integer");
+    return PGK_EINVAL;
+  }
+  *out = x;
+  return PGK_OK;
+}
+
+int pgk_ini_get_bool(const pgk_ini *ini, const char *key, int32_t occ, int32_t idx,
+                     int32_t *out) {
+  const std::string *v = ini_value(ini, key, occ, idx, "pgk_ini_get_bool");
+  if (!v || !out) return PGK_EINVAL;
+  size_t k = (!v->empty() && (*v)[0] == '.') ? 1 : 0;
+  const char ch = k < v->size() ? (*v)[k] : ' ';
+  if (ch == 'T' || ch == 't') *out = 1;
+  else if (ch == 'F' || ch == 'f') *out = 0;
+  else {
+    pgk_set_error("pgk_ini_get_bool: '" + *v + "' is not a logical");
+    return PGK_EINVAL;
+  }
   return PGK_OK;
 }
 

@@ -1,6 +1,6 @@
 """The reference's on-disk formats through libpgkforce.so (host code, no GPU):
-the wrsta / rdstarec state file and outene's energy rows.  Values stay in
-the file's SI units (m, m/s, 1/s, Pa)."""
+the wrsta / rdstarec state file, outene's energy rows and the .ini control
+script.  State-file values stay in the file's SI units (m, m/s, 1/s, Pa)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -24,7 +24,20 @@ def _lib():
     L.pgk_sta_read.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
     L.pgk_sta_write.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
     L.pgk_out_ene_row.argtypes = [C.c_int32, vp, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
-    for f in ("pgk_sta_read_header", "pgk_sta_read", "pgk_sta_write", "pgk_out_ene_row"):
+    L.pgk_ini_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pgk_ini_free.argtypes = [vp]
+    L.pgk_ini_free.restype = None
+    L.pgk_ini_count.argtypes = [vp, C.c_char_p]
+    L.pgk_ini_count.restype = C.c_int32
+    L.pgk_ini_nvalues.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.pgk_ini_nvalues.restype = C.c_int32
+    L.pgk_ini_get_str.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.pgk_ini_get_double.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, vp]
+    L.pgk_ini_get_int.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, vp]
+    L.pgk_ini_get_bool.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, vp]
+    for f in ("pgk_sta_read_header", "pgk_sta_read", "pgk_sta_write", "pgk_out_ene_row",
+              "pgk_ini_read", "pgk_ini_get_str", "pgk_ini_get_double", "pgk_ini_get_int",
+              "pgk_ini_get_bool"):
         getattr(L, f).restype = C.c_int
     return L
 
@@ -80,3 +93,50 @@ def out_ene_row(step, vals, high=False):
     pgk._check(L.pgk_out_ene_row(int(step), v.ctypes.data, len(v), int(bool(high)), buf,
                                  len(buf)), "pgk_out_ene_row")
     return buf.value.decode()
+
+
+class Ini:
+    """The words of a .ini control script (rdscript.F90 / rdfree.F90)."""
+
+    def __init__(self, path):
+        self.L = _lib()
+        self.h = C.c_void_p()
+        pgk._check(self.L.pgk_ini_read(str(path).encode(), C.byref(self.h)), "pgk_ini_read")
+
+    def close(self):
+        if self.h:
+            self.L.pgk_ini_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def count(self, key):
+        return self.L.pgk_ini_count(self.h, key.encode())
+
+    def nvalues(self, key, occ=0):
+        return self.L.pgk_ini_nvalues(self.h, key.encode(), occ)
+
+    def str(self, key, idx=0, occ=0):
+        buf = C.create_string_buffer(256)
+        pgk._check(self.L.pgk_ini_get_str(self.h, key.encode(), occ, idx, buf, 256),
+                   "pgk_ini_get_str")
+        return buf.value.decode()
+
+    def float(self, key, idx=0, occ=0):
+        v = C.c_double()
+        pgk._check(self.L.pgk_ini_get_double(self.h, key.encode(), occ, idx, C.byref(v)),
+                   "pgk_ini_get_double")
+        return v.value
+
+    def int(self, key, idx=0, occ=0):
+        v = C.c_int64()
+        pgk._check(self.L.pgk_ini_get_int(self.h, key.encode(), occ, idx, C.byref(v)),
+                   "pgk_ini_get_int")
+        return v.value
+
+    def bool(self, key, idx=0, occ=0):
+        v = C.c_int32()
+        pgk._check(self.L.pgk_ini_get_bool(self.h, key.encode(), occ, idx, C.byref(v)),
+                   "pgk_ini_get_bool")
+        return bool(v.value)
