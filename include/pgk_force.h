@@ -332,6 +332,18 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
  * launches it replaced, for the 100k box and for the 64-replica batch.      */
 int pgk_md_use_graphs(pgk_ctx *ctx, int32_t enable);
 
+/* Standard Ewald reciprocal sum with per-atom energy and virial: prepewk's
+ * wave vectors (prepewk.F90:9-158, |n| <= kmax with the y/z ratios) and
+ * ewaldkp_hf (ewaldk_hf.F90:8-304), the reciprocal part of calforce_hf in the
+ * reference's _HF_BULK_EWK build, for small systems (natom x nwave work).
+ * Uses the context's positions, charges, box and alpha.  As the reference:
+ * adds into force [3*natom], pot_viri_coul, virt (pot_virit_coul, Fortran
+ * 3x3), pot_atm (pot_elc_atm [natom], with the self term -alpha/sqrt(pi)
+ * q_i^2) and viri_atm (virielct_atm(:,:,:,1) [9*natom]); sets pot_ewk.      */
+int pgk_ewaldk_hf(pgk_ctx *ctx, int32_t kmax, double yratio, double zratio,
+                  int32_t ifnetqcorrp, double *force, double *pot_ewk, double *pot_viri_coul,
+                  double *virt, double *pot_atm, double *viri_atm);
+
 /* Heat-flux time series of pgk_md_run / pgk_md_run_batch (no reference
  * counterpart as a call: the device form of the heat-flux outputs that the
  * reference's moldyn_hf writes every heatfinterval steps, output_hf.F90:8-686,

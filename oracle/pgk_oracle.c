@@ -1413,3 +1413,126 @@ int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
   free(mm);
   return PGK_OK;
 }
+
+/* ---- standard Ewald with per-atom terms: prepewk + ewaldkp_hf -------------
+ * Wave vectors as prepewk.F90:80-150: n = (kx, ky, kz), kx in [-kmax, kmax],
+ * ky in [-kmax*ry, kmax*ry], kz likewise with rz = anint(y/z ratios), kept
+ * when kx^2 + ky^2/ry^2 + kz^2/rz^2 <= kmax^2 (entry 0 is n = 0 with A = 0);
+ * K = n / L, A(K) = exp(-(pi/alpha)^2 |K|^2) / |K|^2.  The sum
+ * (ewaldk_hf.F90:60-300): per K the structure factors S_s, S_c over atoms in
+ * order; E_K = A (S_s^2 + S_c^2) / (2 pi V); per atom
+ * e_i += A q_i (sin_i S_s + cos_i S_c) / (2 pi V), f_i += A (S_c sin_i -
+ * S_s cos_i) K (times 2 q_i / V at the end), and for K != 0 the virial
+ * tensors with k_fact(a,n) = delta_an - (2/|K|^2 + 2 pi^2/alpha^2) K_a K_n,
+ * net-charge corrected when ifnetq.  Self term -alpha/sqrt(pi) q_i^2 per atom. */
+int32_t pgko_ewald_waves(const double *box, int32_t kmax, double yratio, double zratio,
+                         int32_t cap, double *kw, double *aw, double alpha) {
+  const int ry = (int)round(yratio), rz = (int)round(zratio);
+  const int ykmax = kmax * ry, zkmax = kmax * rz;
+  const double pi = acos(-1.0);
+  const int kmax2 = kmax * kmax;
+  int nw = 1;
+  if (cap < 1) return -1;
+  kw[0] = kw[1] = kw[2] = 0.0;
+  aw[0] = 0.0;
+  for (int kx = -kmax; kx <= kmax; kx++)
+    for (int ky = -ykmax; ky <= ykmax; ky++)
+      for (int kz = -zkmax; kz <= zkmax; kz++) {
+        if (kx == 0 && ky == 0 && kz == 0) continue;
+        const double k2 = (double)(kx * kx) + (double)(ky * ky) / (double)(ry * ry) +
+                          (double)(kz * kz) / (double)(rz * rz);
+        if (!(k2 <= (double)kmax2)) continue;
+        if (nw >= cap) return -1;
+        const double n[3] = {(double)kx, (double)ky, (double)kz};
+        const double kk = (n[0] / box[0]) * (n[0] / box[0]) + (n[1] / box[1]) * (n[1] / box[1]) +
+                          (n[2] / box[2]) * (n[2] / box[2]);
+        const double coeff = (pi / alpha) * (pi / alpha);
+        aw[nw] = exp(-coeff * kk) / kk;
+        for (int d = 0; d < 3; d++) kw[3 * nw + d] = n[d] / box[d];
+        nw++;
+      }
+  return nw;
+}
+
+int pgko_ewaldk_hf(const pgko_sys *s, int32_t kmax, double yratio, double zratio,
+                   int32_t ifnetq, double *force, double *pot_ewk, double *pot_viri_coul,
+                   double *virt, double *pot_atm, double *viri_atm) {
+  const int n = s->natom;
+  const int cap = 8 * (2 * kmax + 1) * (2 * kmax + 1) * (2 * kmax + 1) *
+                  (int)(round(yratio) * round(zratio)) + 8;
+  double *kw = (double *)malloc(sizeof(double) * 3 * cap);
+  double *aw = (double *)malloc(sizeof(double) * cap);
+  const int nw = pgko_ewald_waves(s->box, kmax, yratio, zratio, cap, kw, aw, s->alpha);
+  if (nw < 0) {
+    free(kw);
+    free(aw);
+    return PGK_EINVAL;
+  }
+  const double pi = acos(-1.0);
+  const double vol = s->box[0] * s->box[1] * s->box[2];
+  const double pot_fact = 0.5 / pi / vol, pi2 = 2.0 * pi, inv_vol = 1.0 / vol;
+  const double fac_vir = 2.0 * pi * pi * (1.0 / (s->alpha * s->alpha));
+  const double alpha_sqrtpi = s->alpha / sqrt(pi);
+  double *sn = (double *)malloc(sizeof(double) * n), *cs = (double *)malloc(sizeof(double) * n);
+  double *fi = (double *)calloc((size_t)3 * n, sizeof(double));
+  double ewk = 0.0, ewk_cc = 0.0;
+  for (int k = 0; k < nw; k++) {
+    const double *K = kw + 3 * k;
+    double bsk = 0.0, bck = 0.0;
+    for (int i = 0; i < n; i++) {
+      const double *r = s->pos + 3 * i;
+      double th = K[0] * r[0] + K[1] * r[1] + K[2] * r[2];
+      th = pi2 * th;
+      sn[i] = sin(th);
+      cs[i] = cos(th);
+      bsk = bsk + s->chrg[i] * sn[i];
+      bck = bck + s->chrg[i] * cs[i];
+    }
+    const double pfa = pot_fact * aw[k];
+    const double poti = pfa * (bsk * bsk + bck * bck);
+    ewk = ewk + poti;
+    for (int i = 0; i < n; i++)
+      pot_atm[i] = pot_atm[i] + pfa * s->chrg[i] * (sn[i] * bsk + cs[i] * bck);
+    double csk[3], cck[3];
+    for (int d = 0; d < 3; d++) {
+      csk[d] = aw[k] * bsk * K[d];
+      cck[d] = aw[k] * bck * K[d];
+    }
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) fi[3 * i + d] = fi[3 * i + d] + cck[d] * sn[i] - csk[d] * cs[i];
+    if (k == 0) continue;
+    const double k2 = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
+    double kf[9];   /* (a, n) at a + 3n, before the per-call factor */
+    for (int nn = 0; nn < 3; nn++)
+      for (int a = 0; a < 3; a++) {
+        kf[a + 3 * nn] = -1.0 * (2.0 / k2 + fac_vir) * K[a] * K[nn];
+        if (a == nn) kf[a + 3 * nn] = kf[a + 3 * nn] + 1.0;
+      }
+    double fv3 = 0.0;
+    if (ifnetq) {
+      fv3 = pfa * s->netchrgsq;
+      ewk_cc = ewk_cc + fv3 * (-1.0 + fac_vir * k2);
+    }
+    for (int q = 0; q < 9; q++) virt[q] = virt[q] + kf[q] * (poti - fv3);
+    for (int i = 0; i < n; i++) {
+      const double fv2 = pfa * s->chrg[i] * (sn[i] * bsk + cs[i] * bck);
+      for (int q = 0; q < 9; q++)
+        viri_atm[(size_t)9 * i + q] = viri_atm[(size_t)9 * i + q] + kf[q] * (fv2 - fv3);
+    }
+  }
+  *pot_ewk = ewk;
+  *pot_viri_coul = *pot_viri_coul + ewk + ewk_cc;
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      const double f = 2.0 * s->chrg[i] * inv_vol * fi[3 * i + d];
+      force[3 * i + d] = force[3 * i + d] + f;
+    }
+    pot_atm[i] = pot_atm[i] - alpha_sqrtpi * s->chrg[i] * s->chrg[i];
+  }
+  free(sn);
+  free(cs);
+  free(fi);
+  free(kw);
+  free(aw);
+  return PGK_OK;
+}

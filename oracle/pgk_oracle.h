@@ -46,6 +46,18 @@ int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
                   const double *mass, const double *pot_atm,
                   const double *viri_atm, double *hfout);
 
+/* prepewk + ewaldkp_hf (prepewk.F90:9-158, ewaldk_hf.F90:8-304): standard
+ * Ewald reciprocal sum with per-atom energy and virial.  Adds into force
+ * [3N], pot_viri_coul, virt [9] (Fortran (a,n)), pot_atm [N] (with the self
+ * term) and viri_atm [9N] (Fortran (3,3,N)); sets pot_ewk. */
+int pgko_ewaldk_hf(const pgko_sys *s, int32_t kmax, double yratio, double zratio,
+                   int32_t ifnetq, double *force, double *pot_ewk, double *pot_viri_coul,
+                   double *virt, double *pot_atm, double *viri_atm);
+/* prepewk's wave vectors K (3 per entry) and A(K); returns their number
+ * (entry 0 is K = 0) or -1 when cap is too small */
+int32_t pgko_ewald_waves(const double *box, int32_t kmax, double yratio, double zratio,
+                         int32_t cap, double *kw, double *aw, double alpha);
+
 /* spline tables [16*nspltbl] (same column order as the reference harness),
  * B-spline moduli [n1+n2+n3], rkcut */
 void pgko_get_tables(const pgko_sys *s, double *tbl, double *bsp,

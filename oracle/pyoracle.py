@@ -65,6 +65,9 @@ def _oracle_lib():
                                        C.POINTER(abi.HfDesc), C.POINTER(abi.ForceOut),
                                        C.POINTER(abi.HfOut)]
         L.pgko_get_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pgko_ewaldk_hf.restype = C.c_int
+        L.pgko_ewaldk_hf.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double,
+                                     C.c_int32] + [C.c_void_p] * 6
         L.pgko_calheatf.restype = C.c_int
         L.pgko_calheatf.argtypes = [C.c_void_p, C.POINTER(abi.HfDesc)] + [C.c_void_p] * 5
         L.pgko_fft3d.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
@@ -213,6 +216,21 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle calforce_hf failed: {rc}")
         return o.result(hf=True)
+
+    def ewaldk_hf(self, kmax, yratio=1.0, zratio=1.0, ifnetq=0):
+        """prepewk + ewaldkp_hf from zeroed outputs: force, pot_ewk,
+        pot_viri_coul, virt (force comp, r comp), pot_atm, viri_atm (N, n, a)."""
+        n = self.n
+        f, pe, wv = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 9))
+        ewk, pvc, vt = np.zeros(1), np.zeros(1), np.zeros(9)
+        rc = self.L.pgko_ewaldk_hf(self.h, int(kmax), float(yratio), float(zratio), int(ifnetq),
+                                   f.ctypes.data, ewk.ctypes.data, pvc.ctypes.data,
+                                   vt.ctypes.data, pe.ctypes.data, wv.ctypes.data)
+        if rc:
+            raise RuntimeError(f"oracle ewaldk_hf failed: {rc}")
+        return dict(force=f, pot_ewk=ewk[0], pot_viri_coul=pvc[0],
+                    virt=vt.reshape(3, 3).T.copy(), pot_atm=pe,
+                    viri_atm=wv.reshape(n, 3, 3))
 
     def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
         """calheatf over calforce_hf's per-atom arrays (as returned by
@@ -433,6 +451,21 @@ class Reference:
                         C.c_void_p(ene.ctypes.data), C.c_void_p(vir.ctypes.data), C.c_void_p(pe.ctypes.data), C.c_void_p(wv.ctypes.data))
         return dict(force=f, ene=ene, virt=vir.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1),
                     pot_atm=pe, viri_atm=wv.reshape(abi.NHF, nreg, self.n, 3, 3))
+
+    def ewaldk_hf(self, kmax, yratio=1.0, zratio=1.0, ifnetq=0):
+        """The reference's prepewk + ewaldkp_hf from zeroed outputs (same
+        layout as Oracle.ewaldk_hf)."""
+        n = self.n
+        f, pe, wv = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 9))
+        ewk, pvc, vt = np.zeros(1), np.zeros(1), np.zeros(9)
+        _big_stack_call(self.L.pgkref_ewaldk_hf, C.c_int32(kmax), C.c_double(yratio),
+                        C.c_double(zratio), C.c_int32(ifnetq), C.c_void_p(f.ctypes.data),
+                        C.c_void_p(ewk.ctypes.data), C.c_void_p(pvc.ctypes.data),
+                        C.c_void_p(vt.ctypes.data), C.c_void_p(pe.ctypes.data),
+                        C.c_void_p(wv.ctypes.data))
+        return dict(force=f, pot_ewk=ewk[0], pot_viri_coul=pvc[0],
+                    virt=vt.reshape(3, 3).T.copy(), pot_atm=pe,
+                    viri_atm=wv.reshape(n, 3, 3))
 
     def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
         """The reference's calheatf (volume-based) on the given per-atom

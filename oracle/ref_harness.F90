@@ -28,7 +28,8 @@ module pgkref_harness
   use interface_mdtech, only: calforce, calforce_hf, mklist2a, mklist_cell, &
        &                      erf_corr_cutoff, calheatf
   use interface_preproc, only: MPI_loopinit, malloc_larray, mkexcl, &
-       &                       spline_interp_init, fft_pme_init
+       &                       spline_interp_init, fft_pme_init, prepewk
+  use interface_interact, only: ewaldkp_hf
   implicit none
   private
 
@@ -502,5 +503,35 @@ contains
     end do
     deallocate(z, zx, w0, wx, hp, hv, hz, hx, hkx, hpx, hvx, tot, cor_hf, com_hf, typ_save)
   end subroutine pgkref_calheatf
+
+  ! prepewk (prepewk.F90:9-158) then ewaldkp_hf (ewaldk_hf.F90:8-304): the
+  ! standard Ewald reciprocal sum with per-atom terms, for the box and alpha
+  ! of pgkref_init_ewald.  Adds into force, pvc, virt, pe and wv (the
+  ! per-atom virial of region 1); sets pot_ewk.
+  subroutine pgkref_ewaldk_hf(kmax, yratio, zratio, ifnetq, force, pot_ewk, pvc, virt, &
+       &                      pe, wv) bind(C, name='pgkref_ewaldk_hf')
+    integer(c_int), value :: kmax, ifnetq
+    real(c_double), value :: yratio, zratio
+    real(c_double), intent(inout) :: force(3,natom), pvc, virt(3,3), pe(natom)
+    real(c_double), intent(inout) :: wv(3,3,natom)
+    real(c_double), intent(out) :: pot_ewk
+    real(8), allocatable :: fvc(:,:), w4(:,:,:,:), mc(:,:)
+    integer, allocatable :: ht(:)
+    real(8) :: ewc, z1(1), z2(1)
+
+    allocate(fvc(3,natom), w4(3,3,natom,1), mc(3,1), ht(natom))
+    fvc = 0.0d0
+    w4(:,:,:,1) = wv(:,:,:)
+    mc = 0.0d0
+    ht = 1
+    z1 = 0.0d0
+    z2 = box_s(3)
+    call prepewk(ewc, alpha_s, kmax, box_s(1), box_s(2), box_s(3), yratio, zratio)
+    call ewaldkp_hf(box_s(1), box_s(2), box_s(3), alpha_s, force, pot_ewk, &
+         &          (ifnetq /= 0), netchrgsq_s, fvc, pvc, virt, pe, w4, &
+         &          .true., 1, z1, z2, ht, mc)
+    wv(:,:,:) = w4(:,:,:,1)
+    deallocate(fvc, w4, mc, ht)
+  end subroutine pgkref_ewaldk_hf
 
 end module pgkref_harness

@@ -58,6 +58,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
     L.pgk_calheatf.argtypes = [vp, vp, vp, vp]
+    L.pgk_ewaldk_hf.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_int32] + [vp] * 6
     L.pgk_calforce_heatf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
                                      vp, vp, C.POINTER(abi.ForceOut), vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
@@ -75,7 +76,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
-              "pgk_md_heatf", "pgk_md_heatf_series"):
+              "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -244,6 +245,20 @@ class ForceContext:
                                          C.byref(o), out.ctypes.data), "pgk_calforce_heatf")
         return dict(force=f, ene=e, virt=v.reshape(abi.NTENS, 3, 3).transpose(0, 2, 1).copy(),
                     hf=out)
+
+    def ewaldk_hf(self, kmax, yratio=1.0, zratio=1.0, ifnetq=0):
+        """Standard Ewald reciprocal sum with per-atom terms (prepewk +
+        ewaldkp_hf) from zeroed outputs: force, pot_ewk, pot_viri_coul, virt
+        (force comp, r comp), pot_atm, viri_atm (natom, r comp n, force comp a)."""
+        n = self.n
+        f, pe, wv = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 9))
+        ewk, pvc, vt = np.zeros(1), np.zeros(1), np.zeros(9)
+        _check(self.L.pgk_ewaldk_hf(self.h, int(kmax), float(yratio), float(zratio), int(ifnetq),
+                                    f.ctypes.data, ewk.ctypes.data, pvc.ctypes.data,
+                                    vt.ctypes.data, pe.ctypes.data, wv.ctypes.data),
+               "pgk_ewaldk_hf")
+        return dict(force=f, pot_ewk=ewk[0], pot_viri_coul=pvc[0], virt=vt.reshape(3, 3).T.copy(),
+                    pot_atm=pe, viri_atm=wv.reshape(n, 3, 3))
 
     def calheatf(self, vel, mass=None):
         """calheatf on the device over the last calforce_hf call's arrays:
