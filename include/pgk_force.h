@@ -359,6 +359,17 @@ int pgk_md_heatf(pgk_ctx *ctx, const pgk_hf_desc *hf, int32_t interval);
  * max_rows of them are copied to out.                                       */
 int pgk_md_heatf_series(pgk_ctx *ctx, double *out, int32_t max_rows, int32_t *nrows);
 
+/* RESPA multiple time stepping for pgk_md_run / pgk_md_run_batch
+ * (moldyn.F90:1465-1771): nstep_short inner steps of dt/nstep_short per
+ * outer step; the long-level forces' half kick opens the first inner step and
+ * closes the last, they are evaluated once per outer step after the last
+ * drift; the short-level forces every inner step; rattle_c at the inner step,
+ * rattlep_v after every inner step.  mts gives each term's level
+ * (PGK_MTS_LONG, PGK_MTS_SHORT or 0; NULL = all long); nstep_short 1 turns
+ * it off.  The sampled potential energy is the short call's plus the long
+ * call's.  Not combined with the Nose-Hoover chain.                          */
+int pgk_md_respa(pgk_ctx *ctx, int32_t nstep_short, const pgk_mts *mts);
+
 /* Nose-Hoover chain thermostat for pgk_md_run / pgk_md_run_batch: the NVT
  * step of moldyn_nhc.F90:1238-1690 with every term on the long step, i.e.
  * nhcint (nhcint.F90:8-140), the thermostatted velocity update

@@ -236,3 +236,56 @@ def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, rec
     return nhc_steps(force, pos, vel, minv, dt, nstep, nhc,
                      lambda p, v, o: shake(p, v, o, minv, cons_ij, d2, groups, dt),
                      lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record)
+
+
+def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, nstep, mts):
+    """RESPA steps as moldyn.F90:1465-1771: nshort inner steps of
+    dt_long / nshort; the long half kick before the first inner kick and
+    after the last; long forces after the last inner drift; rattle_c at the
+    inner step, rattlep_v after every inner step.  Returns positions,
+    velocities and [KE, PE, E] per outer step (PE: every term once)."""
+    mass = np.asarray(sysd["mass"], np.float64)
+    minv = 1.0 / mass
+    d2 = np.asarray(cons_d) ** 2
+    groups = _groups(cons_ij, group_start)
+    pos = np.array(sysd["pos"], np.float64)
+    vel = np.array(vel, np.float64)
+    rattle_v(pos, vel, minv, cons_ij, d2, groups)
+    oracle.set_positions(pos)
+    oracle.mklist(sysd["rcut_book"], 0)
+    itot = abi.ENE_NAMES.index("pot_tot")
+    rs = oracle.calforce(abi.MTS_SHORT, mts)
+    rl = oracle.calforce(abi.MTS_LONG, mts)
+    fs, fl = rs["force"], rl["force"]
+    dts = dt_long / nshort
+
+    iewc = abi.ENE_NAMES.index("pot_ewc")
+
+    def row():
+        # each call's pot_tot holds the terms it evaluated plus the Ewald self
+        # energy (a constant), which the step counts once
+        ke = 0.5 * (mass[:, None] * vel ** 2).sum() / MD_CONV
+        pe = rl["ene"][itot] + (rs["ene"][itot] - rs["ene"][iewc])
+        return [ke, pe, ke + pe]
+
+    rows = [row()]
+    for _ in range(nstep):
+        for s in range(1, nshort + 1):
+            if s == 1:
+                vel += 0.5 * dt_long * MD_CONV * minv[:, None] * fl
+            vel += 0.5 * dts * MD_CONV * minv[:, None] * fs
+            old = pos.copy()
+            pos += dts * vel
+            shake(pos, vel, old, minv, cons_ij, d2, groups, dts)
+            oracle.set_positions(pos)
+            rs = oracle.calforce(abi.MTS_SHORT, mts)
+            fs = rs["force"]
+            if s == nshort:
+                rl = oracle.calforce(abi.MTS_LONG, mts)
+                fl = rl["force"]
+            vel += 0.5 * dts * MD_CONV * minv[:, None] * fs
+            if s == nshort:
+                vel += 0.5 * dt_long * MD_CONV * minv[:, None] * fl
+            rattle_v(pos, vel, minv, cons_ij, d2, groups)
+        rows.append(row())
+    return pos, vel, np.array(rows)

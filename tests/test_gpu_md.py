@@ -250,3 +250,60 @@ def test_md_heatf_series_matches_fused_call(prec, mode):
     c.md_heatf(None, 0)
     c.md_run(2, 20, s["rcut_book"], 1)
     assert c.md_heatf_series().shape[0] == 0
+
+
+def _respa_system():
+    from helpers import small_slab
+    from thrid_amd import abi
+    s = small_slab("rb")
+    # bonded terms on the short level, non-bonded and reciprocal on the long
+    mts = abi.Mts(abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT,
+                  abi.MTS_LONG, abi.MTS_LONG, abi.MTS_LONG)
+    return s, mts
+
+
+def test_md_respa_matches_reference(oracle_mod):
+    """3 outer steps of RESPA (3 inner steps, bonded terms short) on the
+    water/propanol slab against the NumPy RESPA on the oracle's forces."""
+    from thrid_amd.pgk import ForceContext
+    s, mts = _respa_system()
+    ij, d, gs = md.rigid_constraints(s)
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=14)
+    c = ForceContext(s, "fp64")
+    c.md_setup(s["mass"], ij, d, gs, 1.5)
+    c.md_respa(3, mts)
+    c.md_set_velocities(v0)
+    rows = c.md_run(3, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    c.close()
+    rp, rv, rrows = md_reference.run_respa(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.5, 3, 3, mts)
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    np.testing.assert_allclose(rows, rrows, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_md_respa_conserves_energy(prec):
+    """300 outer steps of 2 fs with 4 inner steps of 0.5 fs: the total energy
+    stays put and the rigid waters stay rigid."""
+    from thrid_amd.pgk import ForceContext
+    s, mts = _respa_system()
+    cons = md.rigid_constraints(s)
+    c = ForceContext(s, "fp64")
+    s = dict(s)
+    s["pos"] = md.relax(c, s, cons)
+    c.close()
+    c = ForceContext(s, prec)
+    c.md_setup(s["mass"], *cons, 2.0)
+    c.md_respa(4, mts)
+    c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=15))
+    rows = c.md_run(300, 10, s["rcut_book"], 5)
+    e, ke = rows[:, 2], rows[:, 0]
+    print(prec, "RESPA E fluctuation / KE", np.std(e) / np.mean(ke),
+          "end - start", (e[-1] - e[0]) / np.mean(ke))
+    assert np.std(e) < 1e-2 * np.mean(ke)
+    assert abs(e[-1] - e[0]) < 2e-2 * np.mean(ke)
+    p, _v = c.md_state()
+    ij, d, _gs = cons
+    r = np.linalg.norm(p[ij[:, 0] - 1] - p[ij[:, 1] - 1], axis=1)
+    assert np.max(np.abs(r - d)) < 1e-8

@@ -98,3 +98,19 @@ def test_nhc_conserves_extended_energy_and_thermalises():
     assert abs(h[-500:].mean() - h[:500].mean()) < 1e-3 * np.mean(ke)
     assert temp[:20].mean() > 450.0
     assert abs(temp[3000:].mean() - target) < 0.08 * target
+
+
+def test_respa_with_one_inner_step_is_velocity_verlet(oracle_mod):
+    """With every term on the long level and one inner step, the RESPA
+    restatement performs the same operations as the plain step (the short
+    forces are zero, and adding them changes nothing)."""
+    from thrid_amd import abi
+    s = water648()
+    ij, d, gs = md.rigid_constraints(s)
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=2)
+    mts = abi.mts_all(abi.MTS_LONG)
+    p1, v1, r1 = mr.run(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.0, 3)
+    p2, v2, r2 = mr.run_respa(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.0, 1, 3, mts)
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(r1, r2)

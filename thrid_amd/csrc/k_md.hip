@@ -54,17 +54,24 @@ __device__ __forceinline__ double kickv(double v, double f, double minv, double 
   return k.nhc ? kick_nhc(v, f, minv, k) : kick1(v, f, minv, dt);
 }
 
+// RESPA (moldyn.F90:1465-1771): the long force's half kick (dt_l) comes
+// before the short one on the first inner step and after it on the last;
+// force_l is null on the other inner steps and without RESPA
 __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict__ pos,
                                                        double *__restrict__ pos_old,
                                                        double *__restrict__ vel,
                                                        const double *__restrict__ force,
                                                        const double *__restrict__ minv, double dt,
-                                                       const double *__restrict__ nhc) {
+                                                       const double *__restrict__ nhc,
+                                                       const double *__restrict__ force_l,
+                                                       double dt_l) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Kick k = kick_of(nhc);
   for (int d = 0; d < 3; d++) {
-    const double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
+    double v0 = vel[3 * i + d];
+    if (force_l) v0 = kick1(v0, force_l[3 * i + d], minv[i], dt_l);
+    const double v = kickv(v0, force[3 * i + d], minv[i], dt, k);
     vel[3 * i + d] = v;
     pos_old[3 * i + d] = pos[3 * i + d];
     pos[3 * i + d] = drift1(pos[3 * i + d], v, dt);
@@ -74,12 +81,16 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict
 __global__ __launch_bounds__(256) void k_md_kick(int n, double *__restrict__ vel,
                                                  const double *__restrict__ force,
                                                  const double *__restrict__ minv, double dt,
-                                                 const double *__restrict__ nhc) {
+                                                 const double *__restrict__ nhc,
+                                                 const double *__restrict__ force_l, double dt_l) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Kick k = kick_of(nhc);
-  for (int d = 0; d < 3; d++)
-    vel[3 * i + d] = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
+  for (int d = 0; d < 3; d++) {
+    double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
+    if (force_l) v = kick1(v, force_l[3 * i + d], minv[i], dt_l);
+    vel[3 * i + d] = v;
+  }
 }
 
 struct ConsArgs {
@@ -95,6 +106,8 @@ struct ConsArgs {
   int maxit;
   int32_t *fail;           // set when a group does not converge
   const double *nhc;       // thermostat factors of the half kicks, or null
+  const double *force_l;   // RESPA: long force of this inner step's extra kick, or null
+  double dt_l;             // RESPA: its time step (the outer one)
 };
 
 // Triangle groups (three atoms held by three constraints: the rigid 3-site
@@ -211,9 +224,15 @@ __device__ __forceinline__ bool rattle_triangle(const ConsArgs &a, int c0, int i
 // kick and drift of the three atoms of a triangle group, in registers
 __device__ __forceinline__ void kick_drift3(const double *force, const double *minv, int i,
                                             double dt, const Kick &k, double3 &p, double3 &v,
-                                            double3 &o) {
+                                            double3 &o, const double *force_l = nullptr,
+                                            double dt_l = 0.0) {
   const double m = minv[i];
   o = p;
+  if (force_l) {
+    v.x = kick1(v.x, force_l[3 * i], m, dt_l);
+    v.y = kick1(v.y, force_l[3 * i + 1], m, dt_l);
+    v.z = kick1(v.z, force_l[3 * i + 2], m, dt_l);
+  }
   v.x = kickv(v.x, force[3 * i], m, dt, k);
   v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
   v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
@@ -222,11 +241,17 @@ __device__ __forceinline__ void kick_drift3(const double *force, const double *m
   p.z = drift1(p.z, v.z, dt);
 }
 __device__ __forceinline__ void kick3(const double *force, const double *minv, int i, double dt,
-                                      const Kick &k, double3 &v) {
+                                      const Kick &k, double3 &v,
+                                      const double *force_l = nullptr, double dt_l = 0.0) {
   const double m = minv[i];
   v.x = kickv(v.x, force[3 * i], m, dt, k);
   v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
   v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
+  if (force_l) {
+    v.x = kick1(v.x, force_l[3 * i], m, dt_l);
+    v.y = kick1(v.y, force_l[3 * i + 1], m, dt_l);
+    v.z = kick1(v.z, force_l[3 * i + 2], m, dt_l);
+  }
 }
 
 // When every group is a triangle: the first half step (kick, drift, position
@@ -243,9 +268,9 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     double3 o0, o1, o2;
-    kick_drift3(force, a.minv, i0, a.dt, k, p0, v0, o0);
-    kick_drift3(force, a.minv, i1, a.dt, k, p1, v1, o1);
-    kick_drift3(force, a.minv, i2, a.dt, k, p2, v2, o2);
+    kick_drift3(force, a.minv, i0, a.dt, k, p0, v0, o0, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i1, a.dt, k, p1, v1, o1, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i2, a.dt, k, p2, v2, o2, a.force_l, a.dt_l);
     const bool done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
@@ -257,7 +282,7 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 p = ld3(a.pos, i), v = ld3(a.vel, i), o;
-    kick_drift3(force, a.minv, i, a.dt, k, p, v, o);
+    kick_drift3(force, a.minv, i, a.dt, k, p, v, o, a.force_l, a.dt_l);
     st3(a.pos, i, p);
     st3(a.vel, i, v);
   }
@@ -275,9 +300,9 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
-    kick3(force, a.minv, i0, a.dt, k, v0);
-    kick3(force, a.minv, i1, a.dt, k, v1);
-    kick3(force, a.minv, i2, a.dt, k, v2);
+    kick3(force, a.minv, i0, a.dt, k, v0, a.force_l, a.dt_l);
+    kick3(force, a.minv, i1, a.dt, k, v1, a.force_l, a.dt_l);
+    kick3(force, a.minv, i2, a.dt, k, v2, a.force_l, a.dt_l);
     const bool done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
@@ -286,7 +311,7 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 v = ld3(a.vel, i);
-    kick3(force, a.minv, i, a.dt, k, v);
+    kick3(force, a.minv, i, a.dt, k, v, a.force_l, a.dt_l);
     st3(a.vel, i, v);
   }
 }
@@ -484,44 +509,56 @@ ConsArgs cons_args(pgk_ctx *c) {
   a.maxit = 500;
   a.fail = c->d_md_fail;
   a.nhc = c->md_nhc_m ? c->d_md_nhc + 3 * c->md_nhc_m : nullptr;
+  a.force_l = nullptr;
+  a.dt_l = 0.0;
   return a;
 }
 
 }  // namespace
 
 // ---- launchers used by the C ABI (pgk_api.hip) ----------------------------
-int pgk_md_kick_drift(pgk_ctx *c) {
+// kick (force f over dt, after the long force fl over dtl when given), drift
+// over dt and the position constraints at dt
+int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl) {
   const int n = c->natom;
+  ConsArgs a = cons_args(c);
+  a.dt = dt;
+  a.force_l = fl;
+  a.dt_l = dtl;
   if (c->md_all_tri) {
     const int nt = c->md_ngroup + c->md_nfree;
-    k_md_first_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(cons_args(c), c->d_force,
-                                                                   c->d_md_free, c->md_nfree);
+    k_md_first_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(a, f, c->d_md_free,
+                                                                   c->md_nfree);
     HIP_CHECK(hipGetLastError());
     return PGK_OK;
   }
   k_md_kick_drift<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(
-      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, c->d_force, c->d_md_minv, c->md_dt,
-      cons_args(c).nhc);
+      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, f, c->d_md_minv, dt, a.nhc, fl, dtl);
   HIP_CHECK(hipGetLastError());
   if (c->md_ngroup > 0) {
-    k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(cons_args(c));
+    k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(a);
     HIP_CHECK(hipGetLastError());
   }
   return PGK_OK;
 }
 
-int pgk_md_kick(pgk_ctx *c) {
+// kick (force f over dt, then the long force fl over dtl when given) and the
+// velocity constraints
+int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl) {
   const int n = c->natom;
+  ConsArgs a = cons_args(c);
+  a.dt = dt;
+  a.force_l = fl;
+  a.dt_l = dtl;
   if (c->md_all_tri) {
     const int nt = c->md_ngroup + c->md_nfree;
-    k_md_second_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(cons_args(c), c->d_force,
-                                                                    c->d_md_free, c->md_nfree);
+    k_md_second_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(a, f, c->d_md_free,
+                                                                    c->md_nfree);
     HIP_CHECK(hipGetLastError());
     return PGK_OK;
   }
-  k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_force,
-                                                            c->d_md_minv, c->md_dt,
-                                                            cons_args(c).nhc);
+  k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, f, c->d_md_minv,
+                                                            dt, a.nhc, fl, dtl);
   HIP_CHECK(hipGetLastError());
   return pgk_md_rattle_v(c);
 }

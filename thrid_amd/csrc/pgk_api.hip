@@ -339,9 +339,9 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   // (a dynamics step that samples nothing sums them on the device but skips
   // the copy)
   if (nseg > 0) {
-    // two table slots: the MD loop's heat-flux pass alternates with the
-    // step's own call without re-uploading either table
-    const int slot = c->hf_pairs_only ? 1 : 0;
+    // table slots: the MD loop's heat-flux pass and RESPA's short-level calls
+    // alternate with the step's own call without re-uploading any table
+    const int slot = c->hf_pairs_only ? 1 : (mts_flag == PGK_MTS_SHORT ? 2 : 0);
     int *dseg = reinterpret_cast<int *>(c->d_sum + 512 + 64 * slot);
     int *hseg = reinterpret_cast<int *>(c->h_sum + 512 + 64 * slot);   // pinned staging
     int tab[32] = {};
@@ -1410,7 +1410,11 @@ int pgk_md_get_state(pgk_ctx *c, double *pos, double *vel) {
 static int md_sample(pgk_ctx *c, double *row) {
   int rc = pgk_md_ke(c, c->d_md_ke);
   if (rc) return rc;
+  // RESPA: the long call's record over the short call's, as calforce leaves
+  // the terms it does not evaluate alone and sums every term into pot_tot
   double ene[PGK_NENE] = {};
+  if (c->md_nshort > 1)
+    for (int k = 0; k < PGK_NENE; k++) ene[k] = c->md_e_short[k];
   if ((rc = finalize_energies(c, ene, nullptr))) return rc;   // waits for the stream
   const int nb = pgk_grid_blocks(c->natom, 256);
   std::vector<double> rec((size_t)nb * PGK_REC);
@@ -1435,9 +1439,10 @@ static int md_sample(pgk_ctx *c, double *row) {
     eth += c->md_nhc_gnkt * st[0];
     for (int i = 1; i < m; i++) eth += c->md_nhc_gkt * st[i];
   }
+  const double pe = ene[PGK_E_TOT];
   row[0] = ke;
-  row[1] = ene[PGK_E_TOT];
-  row[2] = ke + ene[PGK_E_TOT] + eth;
+  row[1] = pe;
+  row[2] = ke + pe + eth;
   return PGK_OK;
 }
 
@@ -1540,23 +1545,67 @@ struct MdRunner {
   }
   int eager(bool ene) {
     int r;
-    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c))) return r;
+    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c, c->d_force, c->md_dt, nullptr, 0.0)))
+      return r;
     if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                           nullptr, nullptr, false, false, ene)))
       return r;
-    if ((r = pgk_md_kick(c))) return r;
+    if ((r = pgk_md_kick(c, c->d_force, c->md_dt, nullptr, 0.0))) return r;
     return nhc_close();
+  }
+  bool respa() const { return c->md_nshort > 1; }
+  // the short-level forces into d_md_fshort; at a sample their energy record
+  // is finalised into md_e_short (the long call's record is the step's own)
+  int short_forces(bool ene) {
+    if (!ene)
+      return run_calforce(c, PGK_MTS_SHORT, &c->md_mts, 0, false, c->d_md_fshort, nullptr,
+                          nullptr, nullptr, nullptr, false, false, false);
+    for (double &e : c->md_e_short) e = 0.0;
+    return run_calforce(c, PGK_MTS_SHORT, &c->md_mts, 0, false, c->d_md_fshort, nullptr,
+                        nullptr, c->md_e_short, nullptr, false, true, true);
+  }
+  // One RESPA step (moldyn.F90:1465-1771): nshort inner steps of dt/nshort;
+  // the long half kick opens the first and closes the last, the long forces
+  // are evaluated once, after the last drift.  A list rebuild happens at the
+  // positions of the first short force call.
+  int respa_step(bool rebuild, bool ene, double rcut_book) {
+    int r;
+    const int ns = c->md_nshort;
+    const double dts = c->md_dt / ns;
+    for (int s = 1; s <= ns; s++) {
+      const bool last = s == ns;
+      if ((r = pgk_md_kick_drift(c, c->d_md_fshort, dts, s == 1 ? c->d_force : nullptr,
+                                 c->md_dt)))
+        return r;
+      if (s == 1 && rebuild) {
+        drop();
+        if ((r = pgk_build_list(c, rcut_book))) return r;
+      }
+      if ((r = short_forces(ene && last))) return r;
+      if (last && (r = run_calforce(c, PGK_MTS_LONG, &c->md_mts, 0, false, c->d_force, nullptr,
+                                    nullptr, nullptr, nullptr, false, false, ene)))
+        return r;
+      if ((r = pgk_md_kick(c, c->d_md_fshort, dts, last ? c->d_force : nullptr, c->md_dt)))
+        return r;
+    }
+    return PGK_OK;
   }
   // step 0: list and forces at the starting positions
   int begin(double rcut_book) {
     int r;
     if ((r = pgk_build_list(c, rcut_book))) return r;
     if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_ke))) return r;
+    if (respa()) {
+      if ((r = short_forces(true))) return r;
+      return run_calforce(c, PGK_MTS_LONG, &c->md_mts, 0, false, c->d_force, nullptr, nullptr,
+                          nullptr, nullptr, false, false);
+    }
     return run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                         nullptr, nullptr, false, false);
   }
   int step(bool rebuild, bool ene, double rcut_book) {
     int r;
+    if (respa()) return respa_step(rebuild, ene, rcut_book);
     const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && graphs;
     if (plain && gexec) {
       HIP_CHECK(hipGraphLaunch(gexec, c->stream));
@@ -1584,7 +1633,8 @@ struct MdRunner {
       HIP_CHECK(hipGraphLaunch(gexec, c->stream));
       return PGK_OK;
     }
-    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c))) return r;
+    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c, c->d_force, c->md_dt, nullptr, 0.0)))
+      return r;
     if (rebuild) {
       drop();
       if ((r = pgk_build_list(c, rcut_book))) return r;
@@ -1592,7 +1642,7 @@ struct MdRunner {
     if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                           nullptr, nullptr, false, false, ene)))
       return r;
-    if ((r = pgk_md_kick(c)) || (r = nhc_close())) return r;
+    if ((r = pgk_md_kick(c, c->d_force, c->md_dt, nullptr, 0.0)) || (r = nhc_close())) return r;
     if (plain) primed = true;
     return PGK_OK;
   }
@@ -1609,6 +1659,10 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   if (!c->have_pos) {
     pgk_set_error("pgk_md_run: positions not set");
     return PGK_ESTATE;
+  }
+  if (c->md_nshort > 1 && c->md_nhc_m) {
+    pgk_set_error("pgk_md_run: RESPA with the Nose-Hoover chain is not offered");
+    return PGK_EINVAL;
   }
   int rc = set_device_ctx(c);
   if (rc) return rc;
@@ -1647,6 +1701,10 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
   }
   for (int r = 0; r < n; r++) {
     pgk_ctx *c = ctxs[r];
+    if (c && c->md_nshort > 1 && c->md_nhc_m) {
+      pgk_set_error("pgk_md_run_batch: RESPA with the Nose-Hoover chain is not offered");
+      return PGK_EINVAL;
+    }
     if (!c || !c->md_ready || !c->have_pos) {
       pgk_set_error("pgk_md_run_batch: replica " + std::to_string(r) +
                     " has no MD set-up or no positions");
@@ -1801,6 +1859,31 @@ int pgk_md_nhc_state(pgk_ctx *c, double *xlogs, double *vlogs) {
     if (xlogs) xlogs[i] = st[i];
     if (vlogs) vlogs[i] = st[m + i];
   }
+  return PGK_OK;
+}
+
+int pgk_md_respa(pgk_ctx *c, int32_t nstep_short, const pgk_mts *mts) {
+  if (!c || !c->md_ready || nstep_short < 1) {
+    pgk_set_error("pgk_md_respa: no MD set-up or nstep_short < 1");
+    return PGK_EINVAL;
+  }
+  pgk_mts m = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
+               PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+  if (mts) m = *mts;
+  const int32_t lv[7] = {m.bond, m.angl, m.tors, m.torsrb, m.vdw, m.ewk, m.nonb14};
+  for (int k = 0; k < 7; k++)
+    if (lv[k] != PGK_MTS_LONG && lv[k] != PGK_MTS_SHORT && lv[k] != 0) {
+      pgk_set_error("pgk_md_respa: each term's level must be MTS_LONG, MTS_SHORT or 0");
+      return PGK_EINVAL;
+    }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if (nstep_short > 1 && !c->d_md_fshort) {
+    std::vector<double> z((size_t)3 * c->natom, 0.0);
+    if ((rc = upload(&c->d_md_fshort, z))) return rc;
+  }
+  c->md_nshort = nstep_short;
+  c->md_mts = m;
   return PGK_OK;
 }
 

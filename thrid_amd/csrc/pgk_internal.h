@@ -228,6 +228,13 @@ struct pgk_ctx {
   double *d_md_hfseries = nullptr, *d_md_hfmass = nullptr, *d_hj_fscratch = nullptr;
   size_t md_hfseries_cap = 0;
   std::vector<double> md_mass;     // masses (amu) given to pgk_md_setup
+  // RESPA (pgk_md_respa): inner steps per outer step and the MTS level of
+  // each term; the short force lives in d_md_fshort
+  int md_nshort = 1;
+  pgk_mts md_mts = {PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG,
+                    PGK_MTS_LONG, PGK_MTS_LONG, PGK_MTS_LONG};
+  double *d_md_fshort = nullptr;
+  double md_e_short[PGK_NENE] = {};   // the short call's energy record at a sample
   size_t hj_jcol_cap = 0, hj_jpart_cap = 0;
   int hj_nmol = 0;
   bool hj_have_mass = false;
@@ -273,8 +280,8 @@ struct pgk_ctx {
   // call waits): its reduction segments and the inputs of the energy record
   int last_nseg = 0;
   int last_seg_kind[16] = {};
-  int seg_up[2][32] = {};          // segment tables currently on the device (2 slots)
-  int seg_up_n[2] = {-1, -1};
+  int seg_up[3][32] = {};          // segment tables currently on the device (3 slots)
+  int seg_up_n[3] = {-1, -1, -1};
   int32_t last_mts_flag = 0;
   pgk_mts last_mts = {};
   bool last_pending = false;
@@ -302,8 +309,8 @@ int pgk_hf_export(pgk_ctx *c);
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
 // grow a device buffer to at least `need` bytes (contents not kept)
 extern "C" int grow_dev(void **p, size_t *cap, size_t need);
-int pgk_md_kick_drift(pgk_ctx *c);
-int pgk_md_kick(pgk_ctx *c);
+int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl);
+int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl);
 int pgk_md_rattle_v(pgk_ctx *c);
 int pgk_md_ke(pgk_ctx *c, double *rec);
 int pgk_md_nhc_half(pgk_ctx *c);

@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_calforce_heatf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
                                      vp, vp, C.POINTER(abi.ForceOut), vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
+    L.pgk_md_respa.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts)]
     L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
     L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
@@ -76,7 +77,8 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
-              "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf"):
+              "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
+              "pgk_md_respa"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -330,6 +332,12 @@ class ForceContext:
                         int(next), abi.dptr(k["x"]), abi.dptr(k["v"]))
         _check(self.L.pgk_md_nhc(self.h, C.byref(d)), "pgk_md_nhc")
         self._nhc_m = int(mchain)
+
+    def md_respa(self, nstep_short, mts=None):
+        """RESPA for md_run / md_run_batch: nstep_short inner steps per
+        outer step, mts = abi.Mts with each term's level (None: all long)."""
+        _check(self.L.pgk_md_respa(self.h, int(nstep_short),
+                                   None if mts is None else C.byref(mts)), "pgk_md_respa")
 
     def md_heatf(self, hf, interval):
         """Heat-flux rows every `interval` MD steps (and at step 0): hf as
