@@ -344,6 +344,14 @@ int pgk_ewaldk_hf(pgk_ctx *ctx, int32_t kmax, double yratio, double zratio,
                   int32_t ifnetqcorrp, double *force, double *pot_ewk, double *pot_viri_coul,
                   double *virt, double *pot_atm, double *viri_atm);
 
+/* Per-atom reciprocal (SPME) energy of the last call that evaluated the
+ * reciprocal term at the current positions (no reference counterpart: the
+ * SPME form of ewaldkp_hf's per-atom energy): e_i = q_i phi(r_i) / 2 with
+ * phi the reciprocal potential interpolated from the same grid as the forces
+ * (fft_pme.F90:872-937 with B-splines in place of their derivatives), so
+ * sum_i e_i = pot_ewk.  e_atom [natom], eref; the self term is not included. */
+int pgk_recip_energy_atom(pgk_ctx *ctx, double *e_atom);
+
 /* Heat-flux time series of pgk_md_run / pgk_md_run_batch (no reference
  * counterpart as a call: the device form of the heat-flux outputs that the
  * reference's moldyn_hf writes every heatfinterval steps, output_hf.F90:8-686,

@@ -35,3 +35,32 @@ def test_ewaldk_hf_rejects_bad_input():
     with pytest.raises(PgkError):
         c.ewaldk_hf(4, yratio=0.2)
     c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_spme_atom_energies(oracle_mod, prec):
+    """Per-atom SPME reciprocal energies: they add up to pot_ewk of the same
+    call, and with a fine grid they approach the standard Ewald per-atom
+    energies (minus the self term) computed by the independent sum."""
+    from thrid_amd import abi
+    from thrid_amd.pgk import ForceContext, PgkError
+    s = dict(water648())
+    s["nfft"] = np.array([48, 48, 48], np.int32)
+    s["pme_order"] = 8
+    c = ForceContext(s, prec)
+    with pytest.raises(PgkError):
+        c.recip_energy_atom()                 # nothing evaluated yet
+    c.mklist(s["rcut_book"])
+    r = c.calforce(abi.MTS_LONG, abi.Mts(0, 0, 0, 0, 0, abi.MTS_LONG, 0))
+    e = c.recip_energy_atom()
+    ewk = r["ene"][abi.ENE_NAMES.index("pot_ewk")]
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert abs(e.sum() - ewk) <= tol * abs(ewk)
+    ew = oracle_mod.Oracle(s).ewaldk_hf(10)
+    q = np.asarray(s["charge"])
+    ref = ew["pot_atm"] + s["alpha"] / np.sqrt(np.pi) * q * q
+    assert np.max(np.abs(e - ref)) <= 1e-4 * np.max(np.abs(ref))
+    c.set_positions(s["pos"])
+    with pytest.raises(PgkError):
+        c.recip_energy_atom()                 # the grid belongs to the old call
+    c.close()

@@ -491,6 +491,44 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
   }
 }
 
+// Per-atom reciprocal energy from the potential grid the last call left in
+// d_pgrid: e_i = q_i phi(r_i) / 2 with phi(r_i) = sum theta1 theta2 theta3
+// grid, the same interpolation as k_gather_t's with the B-splines in place
+// of their derivatives (F_i = -q_i grad phi(r_i)), so sum_i e_i is the
+// reciprocal energy.  One thread per atom, atom order.
+template <int PREC, int P>
+__global__ __launch_bounds__(256) void k_phi_atoms(int n, const double *__restrict__ pos,
+                                                   const double *__restrict__ q, PmeGeom g,
+                                                   const void *gridv, double *__restrict__ e) {
+  using R = typename Prec<PREC>::real;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R *grid = reinterpret_cast<const R *>(gridv);
+  R th[3][P + 1], dth[3][P + 1];
+  int i0[3];
+  atom_frac_t<R, P>(pos, i, g, i0, th, dth);
+  R phi = 0;
+#pragma unroll
+  for (int a3 = 1; a3 <= P; a3++) {
+    int kk = i0[2] + a3;
+    if (kk < 0) kk += g.K[2];
+#pragma unroll
+    for (int a2 = 1; a2 <= P; a2++) {
+      int jj = i0[1] + a2;
+      if (jj < 0) jj += g.K[1];
+      const R *row = grid + ((size_t)kk * g.K[1] + jj) * g.K[0];
+      const R t23 = th[1][a2] * th[2][a3];
+#pragma unroll
+      for (int a1 = 1; a1 <= P; a1++) {
+        int ii = i0[0] + a1;
+        if (ii < 0) ii += g.K[0];
+        phi += row[ii] * th[0][a1] * t23;
+      }
+    }
+  }
+  e[i] = 0.5 * q[i] * (double)phi;
+}
+
 // ---- charge spreading (fill_charge_grid, fft_pme.F90:359-422) -------------
 // Owner-computes: the grid is cut into equal tiles of T0 x T1 x T2 points
 // (T = largest divisor of K that is <= 8).  Atoms are binned by the tile of
@@ -1492,5 +1530,41 @@ int pgk_pme_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
     case PGK_PREC_FP64: return launch_pme<PGK_PREC_FP64>(c, split, off, nrec);
     case PGK_PREC_MIXED: return launch_pme<PGK_PREC_MIXED>(c, split, off, nrec);
     default: return launch_pme<PGK_PREC_FP32>(c, split, off, nrec);
+  }
+}
+
+// per-atom reciprocal energies of the last call's potential grid into d_e
+template <int PREC>
+static int launch_phi_atoms(pgk_ctx *c, double *d_e) {
+  PmeGeom g;
+  for (int d = 0; d < 3; d++) {
+    g.K[d] = c->nfft[d];
+    g.recip[d] = 1.0 / c->box[d];
+  }
+  g.H = c->nfft[0] / 2 + 1;
+  g.order = c->order;
+  const int n = c->natom, nb = pgk_grid_blocks(n, 256);
+#define PGK_PHI(P) k_phi_atoms<PREC, P><<<nb, 256, 0, c->stream>>>(n, c->d_pos, c->d_q, g, \
+                                                                  c->d_pgrid, d_e)
+  switch (c->order) {
+    case 3: PGK_PHI(3); break;
+    case 4: PGK_PHI(4); break;
+    case 5: PGK_PHI(5); break;
+    case 6: PGK_PHI(6); break;
+    case 7: PGK_PHI(7); break;
+    case 8: PGK_PHI(8); break;
+    case 9: PGK_PHI(9); break;
+    default: PGK_PHI(10); break;
+  }
+#undef PGK_PHI
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+int pgk_pme_atom_energy(pgk_ctx *c, double *d_e) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_phi_atoms<PGK_PREC_FP64>(c, d_e);
+    case PGK_PREC_MIXED: return launch_phi_atoms<PGK_PREC_MIXED>(c, d_e);
+    default: return launch_phi_atoms<PGK_PREC_FP32>(c, d_e);
   }
 }

@@ -315,6 +315,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   if (do_ewk) {
     if ((rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
     seg_kind[nseg++] = SEG_PME;
+    c->pme_grid_ready = true;
   }
   pgk_prof_begin(c, PGK_PROF_BONDED);
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
@@ -862,6 +863,7 @@ int pgk_set_box(pgk_ctx *c, const double *box) {
     return PGK_EINVAL;
   }
   for (int d = 0; d < 3; d++) c->box[d] = box[d];
+  c->pme_grid_ready = false;
   return PGK_OK;
 }
 
@@ -877,6 +879,7 @@ int pgk_set_positions(pgk_ctx *c, const double *pos) {
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->have_pos = true;
   c->hf_ready = false;   // calheatf pairs the arrays with the positions of their call
+  c->pme_grid_ready = false;
   return PGK_OK;
 }
 
@@ -891,6 +894,7 @@ int pgk_set_positions_device(pgk_ctx *c, const double *d_pos) {
                            hipMemcpyDeviceToDevice, c->stream));
   c->have_pos = true;
   c->hf_ready = false;   // calheatf pairs the arrays with the positions of their call
+  c->pme_grid_ready = false;
   return PGK_OK;
 }
 
@@ -1859,6 +1863,29 @@ int pgk_md_nhc_state(pgk_ctx *c, double *xlogs, double *vlogs) {
     if (xlogs) xlogs[i] = st[i];
     if (vlogs) vlogs[i] = st[m + i];
   }
+  return PGK_OK;
+}
+
+int pgk_recip_energy_atom(pgk_ctx *c, double *e_atom) {
+  if (!c || !e_atom) {
+    pgk_set_error("pgk_recip_energy_atom: null argument");
+    return PGK_EINVAL;
+  }
+  if (!c->pme_grid_ready) {
+    pgk_set_error("pgk_recip_energy_atom: no reciprocal-space evaluation at the current "
+                  "positions");
+    return PGK_ESTATE;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if (!c->d_hj_fscratch) {   // natom doubles are enough; the buffer holds 3 natom
+    std::vector<double> z((size_t)3 * c->natom, 0.0);
+    if ((rc = upload(&c->d_hj_fscratch, z))) return rc;
+  }
+  if ((rc = pgk_pme_atom_energy(c, c->d_hj_fscratch))) return rc;
+  HIP_CHECK(hipMemcpyAsync(e_atom, c->d_hj_fscratch, sizeof(double) * c->natom,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return PGK_OK;
 }
 

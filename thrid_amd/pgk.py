@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
                                      vp, vp, C.POINTER(abi.ForceOut), vp]
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
     L.pgk_md_respa.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts)]
+    L.pgk_recip_energy_atom.argtypes = [vp, vp]
     L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
     L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
@@ -78,7 +79,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
               "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
-              "pgk_md_respa"):
+              "pgk_md_respa", "pgk_recip_energy_atom"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -332,6 +333,13 @@ class ForceContext:
                         int(next), abi.dptr(k["x"]), abi.dptr(k["v"]))
         _check(self.L.pgk_md_nhc(self.h, C.byref(d)), "pgk_md_nhc")
         self._nhc_m = int(mchain)
+
+    def recip_energy_atom(self):
+        """Per-atom SPME reciprocal energies of the last reciprocal-space
+        evaluation at the current positions (sum = pot_ewk)."""
+        e = np.zeros(self.n)
+        _check(self.L.pgk_recip_energy_atom(self.h, e.ctypes.data), "pgk_recip_energy_atom")
+        return e
 
     def md_respa(self, nstep_short, mts=None):
         """RESPA for md_run / md_run_batch: nstep_short inner steps per
