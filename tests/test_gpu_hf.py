@@ -205,8 +205,9 @@ def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode):
     two = c.calheatf(vel, mass)
     fused = c.calforce_heatf(hf, vel)           # masses kept from calheatf
     scale = np.max(np.abs(two))
-    # the mixed form's W arrays are 64-bit fixed point; the fused sums are FP64
-    assert np.max(np.abs(fused["hf"] - two)) <= (1e-11 if prec == "fp64" else 1e-8) * scale
+    # the mixed form: the fused pass computes the pair terms in FP32, the
+    # two-step pass in FP64 into 64-bit fixed-point W arrays
+    assert np.max(np.abs(fused["hf"] - two)) <= (1e-11 if prec == "fp64" else 1e-5) * scale
     np.testing.assert_array_equal(fused["force"], got["force"]) if prec == "mixed" else \
         np.testing.assert_allclose(fused["force"], got["force"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(fused["ene"], got["ene"], rtol=1e-12, atol=1e-12)

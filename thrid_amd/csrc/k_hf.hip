@@ -14,6 +14,8 @@
 // reference's: no contraction, r = sqrt(r^2), r <= rrcut).  The pair terms
 // are symmetric under i <-> j, so the cluster list's orientation of a pair
 // does not matter.  One 64-lane wave per i-cluster, lane = ii + 8 jj.
+#include <type_traits>
+
 #include "pgk_hf.h"
 
 namespace {
@@ -26,6 +28,7 @@ struct HfPairArgs {
   const double *eps, *rmin;
   int ncl, ntype;
   double sqcut, sqcut2, rrcut;
+  double rrsq;   // largest r^2 with sqrt(r^2) <= rrcut: r2 <= rrsq decides r <= rrcut exactly
 };
 
 __device__ __forceinline__ double r2_exact(const double *r) {
@@ -140,8 +143,19 @@ __device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp,
   }
 }
 
+// 1/sqrt(r^2) in the compute type: FP64 divide of the root, or FP32's
+// hardware reciprocal root (about one ulp)
+template <class R> __device__ __forceinline__ R hf_rinv(double r2);
+template <> __device__ __forceinline__ double hf_rinv<double>(double r2) { return 1.0 / sqrt(r2); }
+template <> __device__ __forceinline__ float hf_rinv<float>(double r2) {
+  return __builtin_amdgcn_rsqf((float)r2);
+}
+
 template <class AH, bool BULK>
 __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
+  // the mixed form (fixed-point accumulators) computes the pair terms in FP32
+  using R = typename std::conditional<std::is_same<AH, unsigned long long>::value, float,
+                                      double>::type;
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ci = blockIdx.x * 4 + wave;
@@ -188,9 +202,10 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
         }
         const double r2 = r2_exact(r);
         if (r2 <= p.sqcut) {
-          const double ra = sqrt(r2);
-          const double rinv1 = 1.0 / ra;
-          const double rinv2 = rinv1 * rinv1;   // one division instead of two
+          // the cutoff tests stay exact FP64 decisions; the pair terms are in
+          // the compute type R (FP32 in the mixed form: one hardware rsqrt)
+          const R rinv1 = hf_rinv<R>(r2);
+          const R rinv2 = rinv1 * rinv1;
           const bool same = moli == p.molid[aj];
           // the centres' separation and, in the volume form, whether it counts
           double caj[3];
@@ -210,22 +225,22 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
           const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cai[2], caj[2], rz};
           double svc = 0.0, svl = 0.0;
-          if (ra <= p.rrcut) {
-            const double qq = qi * p.q[aj];
-            const double ph = qq * rinv1;
+          if (r2 <= p.rrsq) {
+            const R qq = (R)qi * (R)p.q[aj];
+            const R ph = qq * rinv1;
             if (same) flux_add<0, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
             else flux_add<2, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
           }
           if (r2 <= p.sqcut2) {
             const size_t tij = (size_t)(typi - 1) + (size_t)p.ntype * (p.type[aj] - 1);
             if (p.inttyp[tij] == 1) {
-              const double rad = p.rmin[tij];
-              const double t2 = rinv2 * (rad * rad);
-              const double t6 = t2 * t2 * t2;
-              const double t12 = t6 * t6;
-              const double ep = p.eps[tij];
-              const double pot = ep * (t12 - 2.0 * t6);
-              const double ft = 12.0 * ep * (t12 - t6) * rinv2;
+              const R rad = (R)p.rmin[tij];
+              const R t2 = rinv2 * (rad * rad);
+              const R t6 = t2 * t2 * t2;
+              const R t12 = t6 * t6;
+              const R ep = (R)p.eps[tij];
+              const R pot = ep * (t12 - (R)2 * t6);
+              const R ft = (R)12 * ep * (t12 - t6) * rinv2;
               if (same) flux_add<1, BULK>(fp, pot, ft, ei, ej, acc, svl);
               else flux_add<3, BULK>(fp, pot, ft, ei, ej, acc, svl);
             }
@@ -322,6 +337,7 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.sqcut = sq2 > sq ? sq2 : sq;
     p.sqcut2 = sq2;
     p.rrcut = c->rrcut;
+    p.rrsq = c->rrsq_cut;
     const int nblk = pgk_grid_blocks(c->ncl, 4);
     if (!h.flux) {
       k_hf_pairs<AH><<<nblk, 256, 0, st>>>(p, h);

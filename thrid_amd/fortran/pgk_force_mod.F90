@@ -173,6 +173,33 @@ module pgk_force_mod
        integer(c_int) :: rc
      end function pgk_calheatf
 
+     ! calforce_hf and calheatf in one pass, without the per-atom arrays
+     function pgk_calforce_heatf(ctx, mts_flag, mts, hf, vel, mass, out, hfj) &
+          & bind(C, name='pgk_calforce_heatf') result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_double, pgk_mts, pgk_hf_desc, pgk_force_out
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mts_flag
+       type(pgk_mts), intent(in) :: mts
+       type(pgk_hf_desc), intent(in) :: hf
+       real(c_double), intent(in) :: vel(*), mass(*)
+       type(pgk_force_out), intent(in) :: out
+       real(c_double), intent(out) :: hfj(*)
+       integer(c_int) :: rc
+     end function pgk_calforce_heatf
+
+     ! ewaldkp_hf with prepewk's wave vectors (the _HF_BULK_EWK build)
+     function pgk_ewaldk_hf(ctx, kmax, yratio, zratio, ifnetqcorrp, force, pot_ewk, &
+          &                 pot_viri_coul, virt, pot_atm, viri_atm) &
+          & bind(C, name='pgk_ewaldk_hf') result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: kmax, ifnetqcorrp
+       real(c_double), value :: yratio, zratio
+       real(c_double), intent(inout) :: force(*), pot_viri_coul, virt(*), pot_atm(*), viri_atm(*)
+       real(c_double), intent(out) :: pot_ewk
+       integer(c_int) :: rc
+     end function pgk_ewaldk_hf
+
      function pgk_calforce_async(ctx, mts_flag, mts, ifcalpreatom, d_force) &
           & bind(C, name='pgk_calforce_async') result(rc)
        import :: c_ptr, c_int, c_int32_t, pgk_mts
