@@ -399,6 +399,45 @@ typedef struct pgk_nhc_desc {
 } pgk_nhc_desc;
 /* nhc NULL switches the thermostat off (plain NVE).                          */
 int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
+
+/* Region Langevin thermostat for pgk_md_run / pgk_md_run_batch (the
+ * ifreglange option of the reference's NVE and RESPA drivers,
+ * moldyn.F90 / moldyn_hf.F90:1667-1678, 2348-2372).  At the end of step 0
+ * and of every step the atoms are assigned to the first region whose box
+ * [x1,x2) x [y1,y2) x [z1,z2) holds them (langevin_regist.F90:8-364): a
+ * molecule flagged in mol_com by its centre of mass, moving as a whole,
+ * other atoms one by one; coordinates are wrapped once into [0, L) and a
+ * box may reach past the cell by one period.  At the start of each step
+ * every registered atom draws a random acceleration by the Box-Muller method
+ * with variance 2 damp kT / (m dt) per component and gets
+ * v = (v + dt/2 a) exp(-dt damp/2); at the end of the step, after RATTLE,
+ * v = v exp(-dt damp/2) + dt/2 a with the same a (zero for atoms that joined
+ * a region during the step) (langevin_region.F90:8-203, streaming velocity
+ * zero).  The uniform numbers come from SplitMix64 at counter positions
+ * seed + (4 (draw natom + i) + k + 1) 0x9E3779B97F4A7C15, mapped to
+ * ((z >> 11) + 1) 2^-53 in (0, 1], in place of the reference's SFMT stream;
+ * draw counts the steps run since pgk_md_langevin.  Not combined with the
+ * Nose-Hoover chain (as in the reference).  lt NULL switches it off.       */
+#define PGK_LT_MAXREG 16
+typedef struct pgk_langevin_desc {
+  int32_t nregion;             /* 1..PGK_LT_MAXREG (nlangeregion)             */
+  const double *x1, *x2;       /* [nregion] ltxpos1 / ltxpos2 (A)             */
+  const double *y1, *y2;       /* [nregion] ltypos1 / ltypos2 (A)             */
+  const double *z1, *z2;       /* [nregion] ltzpos1 / ltzpos2 (A)             */
+  const double *temperature;   /* [nregion] r_ltemp (K)                       */
+  const double *damp;          /* [nregion] r_ltdamp (1/fs)                   */
+  const int32_t *mol_com;      /* [nmol] 1: molecule registered by its centre *
+                                * of mass (water; polymers unless iftcratom), *
+                                * 0: atom by atom; NULL: every molecule by    *
+                                * its centre of mass.  Molecules are molid's. */
+  uint64_t seed;
+} pgk_langevin_desc;
+int pgk_md_langevin(pgk_ctx *ctx, const pgk_langevin_desc *lt);
+/* Thermostat record: det_ene_kin[nregion] the kinetic energy (eref) the
+ * thermostat gave each region over the last run (langevin_region.F90's
+ * det_ene_kin summed over the run's steps); natom_lt[nregion] the atoms in
+ * each region at the last registration.  Either pointer may be NULL.       */
+int pgk_md_langevin_state(pgk_ctx *ctx, double *det_ene_kin, int32_t *natom_lt);
 /* Current chain state: xlogs[mchain], vlogs[mchain] (1/fs); NULL skips.      */
 int pgk_md_nhc_state(pgk_ctx *ctx, double *xlogs, double *vlogs);
 

@@ -67,9 +67,10 @@ def rattle_v(pos, vel, minv, cons_ij, d2, groups, eps=1e-10, maxit=500):
             raise RuntimeError("rattlep_v did not converge")
 
 
-def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep):
+def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, lang=None):
     """nstep steps with the list built once at the start (nstep < nstep_book);
-    returns positions, velocities and [KE, PE, E] per step."""
+    returns positions, velocities and [KE, PE, E] per step.  lang (a
+    Langevin) adds the region thermostat's halves around each step."""
     mass = np.asarray(sysd["mass"], np.float64)
     minv = 1.0 / mass
     d2 = np.asarray(cons_d) ** 2
@@ -89,7 +90,11 @@ def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep):
         return [ke, pe, ke + pe]
 
     rows.append(energies(r))
+    if lang:
+        lang.register(pos)
     for _ in range(nstep):
+        if lang:
+            lang.half(vel, True)
         vel += 0.5 * dt * MD_CONV * minv[:, None] * f
         old = pos.copy()
         pos += dt * vel
@@ -99,6 +104,9 @@ def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep):
         f = r["force"]
         vel += 0.5 * dt * MD_CONV * minv[:, None] * f
         rattle_v(pos, vel, minv, cons_ij, d2, groups)
+        if lang:
+            lang.register(pos)
+            lang.half(vel, False)
         rows.append(energies(r))
     return pos, vel, np.array(rows)
 
@@ -238,7 +246,8 @@ def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, rec
                      lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record)
 
 
-def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, nstep, mts):
+def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, nstep, mts,
+              lang=None):
     """RESPA steps as moldyn.F90:1465-1771: nshort inner steps of
     dt_long / nshort; the long half kick before the first inner kick and
     after the last; long forces after the last inner drift; rattle_c at the
@@ -269,7 +278,11 @@ def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, 
         return [ke, pe, ke + pe]
 
     rows = [row()]
+    if lang:
+        lang.register(pos)
     for _ in range(nstep):
+        if lang:
+            lang.half(vel, True)
         for s in range(1, nshort + 1):
             if s == 1:
                 vel += 0.5 * dt_long * MD_CONV * minv[:, None] * fl
@@ -287,5 +300,147 @@ def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, 
             if s == nshort:
                 vel += 0.5 * dt_long * MD_CONV * minv[:, None] * fl
             rattle_v(pos, vel, minv, cons_ij, d2, groups)
+        if lang:
+            lang.register(pos)
+            lang.half(vel, False)
         rows.append(row())
     return pos, vel, np.array(rows)
+
+
+# ---- region Langevin thermostat (langevin_regist.F90, langevin_region.F90) ----
+
+_SM_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, ctr):
+    """SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) output at counter
+    positions ctr: the (ctr+1)-th output of the generator seeded with seed."""
+    z = np.uint64(seed) + (np.asarray(ctr, np.uint64) + np.uint64(1)) * _SM_GAMMA
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, ctr):
+    """The top 53 bits of splitmix64 mapped to (0, 1]."""
+    return ((splitmix64(seed, ctr) >> np.uint64(11)) + np.uint64(1)).astype(np.float64) \
+        * 2.0 ** -53
+
+
+class Langevin:
+    """The region Langevin thermostat in this package's units (A, fs, amu):
+    registration as langevin_regist.F90:8-364 (first region holding the
+    wrapped point; a molecule flagged in mol_com by its centre of mass, the
+    others atom by atom) and the two halves of langevin_region.F90:8-203
+    (streaming velocity zero), with the counter-based uniforms of
+    splitmix64 in place of SFMT: atom i's k-th uniform in draw d sits at
+    counter (d natom + i) 4 + k."""
+
+    def __init__(self, regions, temperature, damp, molid, mass, box, dt, mol_com=None, seed=0):
+        import math
+        self.reg_box = np.atleast_2d(np.asarray(regions, np.float64))
+        nr = self.reg_box.shape[0]
+        self.temp = np.broadcast_to(np.asarray(temperature, np.float64), (nr,))
+        self.damp = np.broadcast_to(np.asarray(damp, np.float64), (nr,))
+        self.mass = np.asarray(mass, np.float64)
+        self.n = len(self.mass)
+        self.box = [float(b) for b in box]
+        self.dt = dt
+        self.seed = seed
+        self.draw = 0
+        molid = np.asarray(molid)
+        nmol = int(molid.max())
+        self.units = []          # (com?, [atoms])
+        for m in range(nmol):
+            atoms = [int(i) for i in np.nonzero(molid == m + 1)[0]]
+            if not atoms:
+                continue
+            if mol_com is None or mol_com[m]:
+                self.units.append((True, atoms))
+            else:
+                self.units += [(False, [i]) for i in atoms]
+        kt = [1.380649e-23 / 6.9511042e-21 * float(t) * MD_CONV for t in self.temp]
+        self.vtmp = [2.0 * float(g) * k / dt for g, k in zip(self.damp, kt)]
+        self.efac = [math.exp(-0.5 * dt * float(g)) for g in self.damp]
+        self.reg = np.full(self.n, -1, np.int64)
+        self.frand = np.zeros((self.n, 3))
+        self.det = np.zeros(nr)
+
+    def region_of(self, g):
+        """First region holding point g (already wrapped), or -1
+        (langevin_regist.F90:112-140: each range may reach one period past
+        the cell)."""
+        for k, (x1, x2, y1, y2, z1, z2) in enumerate(self.reg_box):
+            ok = True
+            for d, (lo, hi) in enumerate(((x1, x2), (y1, y2), (z1, z2))):
+                l = self.box[d]
+                if not ((lo <= g[d] < hi) or (lo <= g[d] - l < hi)):
+                    ok = False
+                    break
+            if ok:
+                return k
+        return -1
+
+    def register(self, pos):
+        for com, atoms in self.units:
+            if com:
+                mm, s = 0.0, [0.0, 0.0, 0.0]
+                for i in atoms:
+                    m = float(self.mass[i])
+                    mm = mm + m
+                    for d in range(3):
+                        s[d] = s[d] + m * float(pos[i, d])
+                g = [s[d] / mm for d in range(3)]
+            else:
+                g = [float(pos[atoms[0], d]) for d in range(3)]
+            for d in range(3):
+                if g[d] < 0.0:
+                    g[d] = g[d] + self.box[d]
+                elif g[d] >= self.box[d]:
+                    g[d] = g[d] - self.box[d]
+            r = self.region_of(g)
+            for i in atoms:
+                self.reg[i] = r
+        return self.reg
+
+    def half(self, vel, first):
+        """One half of langevin_region.F90 on vel (in place)."""
+        on = self.reg >= 0
+        idx = np.nonzero(on)[0]
+        r = self.reg[idx]
+        h = 0.5 * self.dt
+        if first:
+            c = (np.uint64(self.draw) * np.uint64(self.n) + idx.astype(np.uint64)) * np.uint64(4)
+            m = self.mass[idx]
+            var = np.array([self.vtmp[k] for k in r]) * m
+            twopi = 2.0 * np.pi
+            u = uniform(self.seed, c)
+            fx = np.sqrt(-2.0 * np.log(u) * var)
+            fy = np.sqrt(-2.0 * np.log(u) * var)
+            u = uniform(self.seed, c + np.uint64(1))
+            fx = fx * np.cos(twopi * u)
+            fy = fy * np.sin(twopi * u)
+            u = uniform(self.seed, c + np.uint64(2))
+            fz = np.sqrt(-2.0 * np.log(u) * var)
+            u = uniform(self.seed, c + np.uint64(3))
+            fz = fz * np.cos(twopi * u)
+            self.frand[:] = 0.0
+            self.frand[idx] = np.stack([fx / m, fy / m, fz / m], axis=1)
+            self.draw += 1
+        fr = self.frand[idx]
+        ef = np.array([self.efac[k] for k in r])[:, None]
+        v0 = vel[idx]
+        v = (v0 + h * fr) * ef if first else v0 * ef + h * fr
+        vel[idx] = v
+        de = 0.5 * self.mass[idx] * ((v * v).sum(axis=1) - (v0 * v0).sum(axis=1)) / MD_CONV
+        np.add.at(self.det, r, de)
+        return vel
+
+
+def langevin_variance(kt_over_m, damp, dt):
+    """Stationary per-component velocity variance of a free particle under
+    the two halves of langevin_region.F90 with no force in between:
+    v' = v c + (dt/2) a (1 + c), c = exp(-damp dt), a ~ N(0, 2 damp kT /
+    (m dt)), so var = (kT/m) (damp dt / 2) coth(damp dt / 2)."""
+    x = 0.5 * damp * dt
+    return kt_over_m * x / np.tanh(x)

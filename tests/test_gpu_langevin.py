@@ -1,0 +1,136 @@
+"""The device region Langevin thermostat against the NumPy restatement
+(trajectories, energy samples, the thermostat's energy record and region
+counts) and against the closed-form stationary variance of free particles."""
+import numpy as np
+import pytest
+
+from thrid_amd import md
+from helpers import water648
+import md_reference
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(s, prec, dt, cons=True):
+    from thrid_amd.pgk import ForceContext
+    c = ForceContext(s, prec)
+    ij, d, gs = md.rigid_constraints(s)
+    if cons:
+        c.md_setup(s["mass"], ij, d, gs, dt)
+    else:
+        c.md_setup(s["mass"], np.zeros((0, 2), np.int32), np.zeros(0), np.zeros(1, np.int32), dt)
+    return c, (ij, d, gs)
+
+
+def _regions(s):
+    """Two slabs in z with different temperatures and damping, the second
+    given in negative coordinates so that it reaches the top of the cell
+    through the period, plus an x slab that overlaps both (it takes only the
+    atoms the slabs leave)."""
+    lx, ly, lz = (float(b) for b in s["box"])
+    return np.array([[0.0, lx, 0.0, ly, 0.2 * lz, 0.5 * lz],
+                     [0.0, lx, 0.0, ly, -0.2 * lz, 0.1 * lz],
+                     [0.0, 0.5 * lx, 0.0, ly, 0.0, lz]])
+
+
+@pytest.mark.parametrize("by_com", [True, False])
+def test_langevin_matches_reference(oracle_mod, by_com):
+    s = water648()
+    dt = 1.0
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=12)
+    c, (ij, d, gs) = _ctx(s, "fp64", dt)
+    c.md_set_velocities(v0)
+    reg, temp, damp = _regions(s), [250.0, 400.0, 320.0], [0.02, 0.05, 0.01]
+    mol_com = None if by_com else np.zeros(int(s["molid"].max()), np.int32)
+    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=2024)
+    rows = c.md_run(5, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    det, cnt = c.md_langevin_state()
+    lt = md_reference.Langevin(reg, temp, damp, s["molid"], s["mass"], s["box"], dt,
+                               mol_com=mol_com, seed=2024)
+    rp, rv, rrows = md_reference.run(oracle_mod.Oracle(s), s, v0, ij, d, gs, dt, 5, lang=lt)
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    np.testing.assert_allclose(rows, rrows, rtol=1e-9, atol=1e-9)
+    assert list(cnt) == [int(np.sum(lt.reg == k)) for k in range(3)]
+    assert min(cnt) > 0
+    np.testing.assert_allclose(det, lt.det, rtol=1e-8, atol=1e-9 * np.abs(lt.det).max())
+    # the thermostat did act: the run differs from plain NVE
+    c2, _ = _ctx(s, "fp64", dt)
+    c2.md_set_velocities(v0)
+    c2.md_run(5, 20, s["rcut_book"], 1)
+    assert np.max(np.abs(c2.md_state()[1] - v)) > 1e-4
+
+
+def test_langevin_respa_matches_reference(oracle_mod):
+    """RESPA (3 inner steps, bonded terms short) on the water/propanol slab
+    with the thermostat, waters registered by centre of mass and propanol
+    atom by atom (the reference's iftcratom split)."""
+    from helpers import small_slab
+    from thrid_amd import abi
+    from thrid_amd.pgk import ForceContext
+    s = small_slab("rb")
+    mts = abi.Mts(abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT,
+                  abi.MTS_LONG, abi.MTS_LONG, abi.MTS_LONG)
+    ij, d, gs = md.rigid_constraints(s)
+    molid = np.asarray(s["molid"])
+    mol_com = (np.bincount(molid)[1:] == 3).astype(np.int32)
+    assert 0 < mol_com.sum() < len(mol_com)
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=13)
+    c = ForceContext(s, "fp64")
+    c.md_setup(s["mass"], ij, d, gs, 1.5)
+    c.md_respa(3, mts)
+    c.md_set_velocities(v0)
+    reg, temp, damp = _regions(s), [250.0, 400.0, 320.0], [0.02, 0.05, 0.01]
+    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=77)
+    rows = c.md_run(3, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    det, cnt = c.md_langevin_state()
+    lt = md_reference.Langevin(reg, temp, damp, molid, s["mass"], s["box"], 1.5,
+                               mol_com=mol_com, seed=77)
+    rp, rv, rrows = md_reference.run_respa(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.5, 3, 3,
+                                           mts, lang=lt)
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    np.testing.assert_allclose(rows, rrows, rtol=1e-9, atol=1e-9)
+    assert list(cnt) == [int(np.sum(lt.reg == k)) for k in range(3)]
+    np.testing.assert_allclose(det, lt.det, rtol=1e-8, atol=1e-9 * np.abs(lt.det).max())
+
+
+def test_langevin_free_particles_closed_form():
+    """Uncharged atoms with no LJ, bond or angle force and no constraints,
+    one region over the whole cell: after relaxing, m v^2 / kT per component
+    averages x coth x (x = damp dt / 2, 1.082 here), the kinetic energy the
+    samples record changes by exactly what the thermostat reports, and the
+    random stream continues across runs (a second run does not repeat the
+    first)."""
+    s = dict(water648())
+    s["charge"] = np.zeros_like(s["charge"])
+    s["lj_eps"] = np.zeros_like(s["lj_eps"])
+    s["bond_k"] = np.zeros_like(s["bond_k"])
+    s["angl_k"] = np.zeros_like(s["angl_k"])
+    dt, damp, temp = 1.0, 1.0, 300.0
+    c, _ = _ctx(s, "fp64", dt, cons=False)
+    c.md_set_velocities(np.zeros((s["natom"], 3)))
+    lx, ly, lz = (float(b) for b in s["box"])
+    c.md_langevin([[0.0, lx, 0.0, ly, 0.0, lz]], temp, damp, seed=99)
+    rows = c.md_run(20, 1000, s["rcut_book"], 20)
+    det, cnt = c.md_langevin_state()
+    assert cnt[0] == s["natom"]
+    assert abs((rows[-1, 0] - rows[0, 0]) - det[0]) < 1e-9 * rows[-1, 0]
+    assert abs(rows[-1, 1] - rows[0, 1]) < 1e-12 * max(1.0, abs(rows[0, 1]))   # no force
+    mass = np.asarray(s["mass"], np.float64)
+    kt = md.KB_EREF * temp * md.MD_CONV
+    z, last = [], None
+    for _ in range(20):
+        c.md_run(10, 1000, s["rcut_book"], 10)
+        v = c.md_state()[1]
+        if last is not None:
+            assert np.max(np.abs(v - last)) > 0.0
+        last = v
+        z.append(mass[:, None] * v * v / kt)
+    want = md_reference.langevin_variance(1.0, damp, dt)
+    got = np.mean(z)
+    print("m v^2 / kT", got, "closed form", want)
+    assert abs(got / want - 1.0) < 0.025
+    assert abs(got - 1.0) > 0.05       # the closed form, not kT/m

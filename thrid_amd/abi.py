@@ -94,6 +94,15 @@ class NhcDesc(C.Structure):
                 ("xlogs", _dp), ("vlogs", _dp)]
 
 
+LT_MAXREG = 16
+
+
+class LangevinDesc(C.Structure):
+    _fields_ = [("nregion", C.c_int32), ("x1", _dp), ("x2", _dp), ("y1", _dp), ("y2", _dp),
+                ("z1", _dp), ("z2", _dp), ("temperature", _dp), ("damp", _dp),
+                ("mol_com", _ip), ("seed", C.c_uint64)]
+
+
 def dptr(a):
     return None if a is None else a.ctypes.data_as(_dp)
 

@@ -845,7 +845,8 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_hf_pe_out, c->d_hf_w_out, c->d_md_gstart, c->d_md_cij, c->d_md_fail,
                   c->d_md_small,
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
-                  c->d_md_nhc};
+                  c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
+                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -1557,6 +1558,18 @@ struct MdRunner {
     if ((r = pgk_md_kick(c, c->d_force, c->md_dt, nullptr, 0.0))) return r;
     return nhc_close();
   }
+  // Region Langevin thermostat: its first half opens the step, before the
+  // first half kick; at the close the atoms are registered at the new
+  // positions and the second half follows RATTLE (moldyn_hf.F90:1667-1678,
+  // 2348-2372).  Steps with it are never replayed from a graph: every step
+  // draws new random numbers.
+  int lt_open() { return c->md_lt_nreg ? pgk_md_lt_half(c, true) : PGK_OK; }
+  int lt_close() {
+    int r;
+    if (!c->md_lt_nreg) return PGK_OK;
+    if ((r = pgk_md_lt_register(c))) return r;
+    return pgk_md_lt_half(c, false);
+  }
   bool respa() const { return c->md_nshort > 1; }
   // the short-level forces into d_md_fshort; at a sample their energy record
   // is finalised into md_e_short (the long call's record is the step's own)
@@ -1576,6 +1589,7 @@ struct MdRunner {
     int r;
     const int ns = c->md_nshort;
     const double dts = c->md_dt / ns;
+    if ((r = lt_open())) return r;
     for (int s = 1; s <= ns; s++) {
       const bool last = s == ns;
       if ((r = pgk_md_kick_drift(c, c->d_md_fshort, dts, s == 1 ? c->d_force : nullptr,
@@ -1592,12 +1606,16 @@ struct MdRunner {
       if ((r = pgk_md_kick(c, c->d_md_fshort, dts, last ? c->d_force : nullptr, c->md_dt)))
         return r;
     }
-    return PGK_OK;
+    return lt_close();
   }
   // step 0: list and forces at the starting positions
   int begin(double rcut_book) {
     int r;
     if ((r = pgk_build_list(c, rcut_book))) return r;
+    if (c->md_lt_nreg) {
+      HIP_CHECK(hipMemsetAsync(c->d_lt_det, 0, sizeof(double) * PGK_LT_MAXREG, c->stream));
+      if ((r = pgk_md_lt_register(c))) return r;
+    }
     if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_ke))) return r;
     if (respa()) {
       if ((r = short_forces(true))) return r;
@@ -1610,7 +1628,7 @@ struct MdRunner {
   int step(bool rebuild, bool ene, double rcut_book) {
     int r;
     if (respa()) return respa_step(rebuild, ene, rcut_book);
-    const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && graphs;
+    const bool plain = !rebuild && !ene && !c->prof && !c->acc_dirty && graphs && !c->md_lt_nreg;
     if (plain && gexec) {
       HIP_CHECK(hipGraphLaunch(gexec, c->stream));
       return PGK_OK;
@@ -1637,7 +1655,8 @@ struct MdRunner {
       HIP_CHECK(hipGraphLaunch(gexec, c->stream));
       return PGK_OK;
     }
-    if ((r = nhc_open()) || (r = pgk_md_kick_drift(c, c->d_force, c->md_dt, nullptr, 0.0)))
+    if ((r = lt_open()) || (r = nhc_open()) ||
+        (r = pgk_md_kick_drift(c, c->d_force, c->md_dt, nullptr, 0.0)))
       return r;
     if (rebuild) {
       drop();
@@ -1646,7 +1665,9 @@ struct MdRunner {
     if ((r = run_calforce(c, PGK_MTS_LONG, &kMdMts, 0, false, c->d_force, nullptr, nullptr,
                           nullptr, nullptr, false, false, ene)))
       return r;
-    if ((r = pgk_md_kick(c, c->d_force, c->md_dt, nullptr, 0.0)) || (r = nhc_close())) return r;
+    if ((r = pgk_md_kick(c, c->d_force, c->md_dt, nullptr, 0.0)) || (r = nhc_close()) ||
+        (r = lt_close()))
+      return r;
     if (plain) primed = true;
     return PGK_OK;
   }
@@ -1666,6 +1687,10 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
   }
   if (c->md_nshort > 1 && c->md_nhc_m) {
     pgk_set_error("pgk_md_run: RESPA with the Nose-Hoover chain is not offered");
+    return PGK_EINVAL;
+  }
+  if (c->md_lt_nreg && c->md_nhc_m) {
+    pgk_set_error("pgk_md_run: the Langevin thermostat with the Nose-Hoover chain is not offered");
     return PGK_EINVAL;
   }
   int rc = set_device_ctx(c);
@@ -1707,6 +1732,11 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
     pgk_ctx *c = ctxs[r];
     if (c && c->md_nshort > 1 && c->md_nhc_m) {
       pgk_set_error("pgk_md_run_batch: RESPA with the Nose-Hoover chain is not offered");
+      return PGK_EINVAL;
+    }
+    if (c && c->md_lt_nreg && c->md_nhc_m) {
+      pgk_set_error("pgk_md_run_batch: the Langevin thermostat with the Nose-Hoover chain is "
+                    "not offered");
       return PGK_EINVAL;
     }
     if (!c || !c->md_ready || !c->have_pos) {
@@ -1911,6 +1941,91 @@ int pgk_md_respa(pgk_ctx *c, int32_t nstep_short, const pgk_mts *mts) {
   }
   c->md_nshort = nstep_short;
   c->md_mts = m;
+  return PGK_OK;
+}
+
+int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
+  if (!c || !c->md_ready) {
+    pgk_set_error("pgk_md_langevin: no MD set-up");
+    return PGK_EINVAL;
+  }
+  if (!lt) {
+    c->md_lt_nreg = 0;
+    return PGK_OK;
+  }
+  const int nr = lt->nregion;
+  if (nr < 1 || nr > PGK_LT_MAXREG || !lt->x1 || !lt->x2 || !lt->y1 || !lt->y2 || !lt->z1 ||
+      !lt->z2 || !lt->temperature || !lt->damp) {
+    pgk_set_error("pgk_md_langevin: need 1 <= nregion <= PGK_LT_MAXREG and every region array");
+    return PGK_EINVAL;
+  }
+  for (int k = 0; k < nr; k++)
+    if (!(lt->temperature[k] >= 0) || !(lt->damp[k] >= 0)) {
+      pgk_set_error("pgk_md_langevin: temperatures and damping factors must be >= 0");
+      return PGK_EINVAL;
+    }
+  const int n = c->natom;
+  int nmol = 0, rc;
+  for (int i = 0; i < n; i++) nmol = std::max(nmol, c->molid[i]);
+  // registration units: molecules by centre of mass, or single atoms
+  std::vector<int32_t> ustart(1, 0), uatom, ucom;
+  std::vector<std::vector<int32_t>> mol(nmol);
+  for (int i = 0; i < n; i++) mol[c->molid[i] - 1].push_back(i);
+  for (int m = 0; m < nmol; m++) {
+    if (mol[m].empty()) continue;
+    if (!lt->mol_com || lt->mol_com[m]) {
+      uatom.insert(uatom.end(), mol[m].begin(), mol[m].end());
+      ustart.push_back((int32_t)uatom.size());
+      ucom.push_back(1);
+    } else {
+      for (int32_t i : mol[m]) {
+        uatom.push_back(i);
+        ustart.push_back((int32_t)uatom.size());
+        ucom.push_back(0);
+      }
+    }
+  }
+  if ((rc = set_device_ctx(c))) return rc;
+  std::vector<double> zero((size_t)3 * n, 0.0);
+  std::vector<int32_t> none(n, -1);
+  if ((rc = upload(&c->d_lt_ustart, ustart)) || (rc = upload(&c->d_lt_uatom, uatom)) ||
+      (rc = upload(&c->d_lt_ucom, ucom)) || (rc = upload(&c->d_lt_mass, c->md_mass)) ||
+      (rc = upload(&c->d_lt_frand, zero)) || (rc = upload(&c->d_lt_reg, none)) ||
+      (rc = upload(&c->d_lt_det, std::vector<double>(PGK_LT_MAXREG, 0.0))) ||
+      (rc = upload(&c->d_lt_cnt, std::vector<int32_t>(PGK_LT_MAXREG, 0))))
+    return rc;
+  for (int k = 0; k < PGK_LT_MAXREG; k++) {
+    const bool on = k < nr;
+    c->md_lt_lo[0][k] = on ? lt->x1[k] : 0.0;
+    c->md_lt_hi[0][k] = on ? lt->x2[k] : 0.0;
+    c->md_lt_lo[1][k] = on ? lt->y1[k] : 0.0;
+    c->md_lt_hi[1][k] = on ? lt->y2[k] : 0.0;
+    c->md_lt_lo[2][k] = on ? lt->z1[k] : 0.0;
+    c->md_lt_hi[2][k] = on ? lt->z2[k] : 0.0;
+    c->md_lt_temp[k] = on ? lt->temperature[k] : 0.0;
+    c->md_lt_damp[k] = on ? lt->damp[k] : 0.0;
+  }
+  c->md_lt_nunit = (int)ucom.size();
+  c->md_lt_seed = lt->seed;
+  c->md_lt_draw = 0;
+  c->md_lt_nreg = nr;
+  return PGK_OK;
+}
+
+int pgk_md_langevin_state(pgk_ctx *c, double *det_ene_kin, int32_t *natom_lt) {
+  if (!c || !c->md_lt_nreg) {
+    pgk_set_error("pgk_md_langevin_state: no Langevin thermostat set up");
+    return PGK_EINVAL;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if ((rc = pgk_md_lt_count(c))) return rc;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int nr = c->md_lt_nreg;
+  if (det_ene_kin)
+    HIP_CHECK(hipMemcpy(det_ene_kin, c->d_lt_det, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  if (natom_lt)
+    HIP_CHECK(hipMemcpy(natom_lt, c->d_lt_cnt, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
   return PGK_OK;
 }
 

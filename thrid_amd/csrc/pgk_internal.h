@@ -259,6 +259,16 @@ struct pgk_ctx {
   double md_nhc_gkt = 0.0, md_nhc_gnkt = 0.0;
   double md_nhc_ys[5] = {};
   double *d_md_nhc = nullptr;      // [3m+2]: xlogs, vlogs, qmass, exp(-dt vxi1/2), bbv
+  // region Langevin thermostat (pgk_md_langevin); off when md_lt_nreg == 0.
+  // Registration units: a molecule by its centre of mass (ucom 1) or one
+  // atom (ucom 0), CSR ustart over uatom.
+  int md_lt_nreg = 0, md_lt_nunit = 0;
+  double md_lt_lo[3][PGK_LT_MAXREG] = {}, md_lt_hi[3][PGK_LT_MAXREG] = {};
+  double md_lt_temp[PGK_LT_MAXREG] = {}, md_lt_damp[PGK_LT_MAXREG] = {};
+  uint64_t md_lt_seed = 0, md_lt_draw = 0;   // draw: steps run since set-up
+  int32_t *d_lt_ustart = nullptr, *d_lt_uatom = nullptr, *d_lt_ucom = nullptr;
+  int32_t *d_lt_reg = nullptr, *d_lt_cnt = nullptr;   // region of each atom (-1 none)
+  double *d_lt_mass = nullptr, *d_lt_frand = nullptr, *d_lt_det = nullptr;
 
   // reduction segment bookkeeping for the current call
   int red_used = 0;
@@ -316,6 +326,9 @@ int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double
 int pgk_md_rattle_v(pgk_ctx *c);
 int pgk_md_ke(pgk_ctx *c, double *rec);
 int pgk_md_nhc_half(pgk_ctx *c);
+int pgk_md_lt_register(pgk_ctx *c);      // assign atoms to the Langevin regions
+int pgk_md_lt_half(pgk_ctx *c, bool first);
+int pgk_md_lt_count(pgk_ctx *c);         // natom_lt into d_lt_cnt
 
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);

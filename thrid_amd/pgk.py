@@ -67,6 +67,8 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
     L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
+    L.pgk_md_langevin.argtypes = [vp, C.POINTER(abi.LangevinDesc)]
+    L.pgk_md_langevin_state.argtypes = [vp, vp, vp]
     L.pgk_md_run_batch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp]
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
@@ -79,7 +81,8 @@ def load_library(path: str = LIB_PATH):
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
               "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
-              "pgk_md_respa", "pgk_recip_energy_atom"):
+              "pgk_md_respa", "pgk_recip_energy_atom", "pgk_md_langevin",
+              "pgk_md_langevin_state"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
         raise PgkError("libpgkforce.so ABI version mismatch")
@@ -367,6 +370,41 @@ class ForceContext:
         _check(self.L.pgk_md_heatf_series(self.h, out.ctypes.data, n.value, C.byref(n)),
                "pgk_md_heatf_series")
         return out[:n.value]
+
+    def md_langevin(self, regions, temperature=None, damp=None, mol_com=None, seed=0):
+        """Region Langevin thermostat for md_run / md_run_batch
+        (langevin_regist.F90 + langevin_region.F90): regions [nregion, 6] of
+        (x1, x2, y1, y2, z1, z2) in A, temperature [nregion] in K, damp
+        [nregion] in 1/fs, mol_com [nmol] 1 where a molecule is registered
+        by its centre of mass (None: every molecule); regions=None switches
+        it off."""
+        if regions is None:
+            _check(self.L.pgk_md_langevin(self.h, None), "pgk_md_langevin")
+            self._lt_nreg = 0
+            return
+        reg = np.ascontiguousarray(np.atleast_2d(regions), np.float64)
+        nr = reg.shape[0]
+        if reg.shape[1] != 6 or temperature is None or damp is None:
+            raise ValueError("md_langevin needs regions [nregion, 6], temperature and damp")
+        cols = [np.ascontiguousarray(reg[:, k]) for k in range(6)]
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperature, np.float64), (nr,)))
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(damp, np.float64), (nr,)))
+        mc = None if mol_com is None else np.ascontiguousarray(mol_com, np.int32)
+        self._lt_keep = (cols, t, g, mc)
+        d = abi.LangevinDesc(nr, *[abi.dptr(a) for a in cols], abi.dptr(t), abi.dptr(g),
+                             abi.iptr(mc), int(seed) & 0xFFFFFFFFFFFFFFFF)
+        _check(self.L.pgk_md_langevin(self.h, C.byref(d)), "pgk_md_langevin")
+        self._lt_nreg = nr
+
+    def md_langevin_state(self):
+        """(det_ene_kin [nregion] in eref over the last run, natom_lt
+        [nregion] at the last registration)."""
+        nr = getattr(self, "_lt_nreg", 0)
+        det = np.zeros(max(nr, 1))
+        cnt = np.zeros(max(nr, 1), np.int32)
+        _check(self.L.pgk_md_langevin_state(self.h, det.ctypes.data, cnt.ctypes.data),
+               "pgk_md_langevin_state")
+        return det[:nr], cnt[:nr]
 
     def md_nhc_state(self):
         """(xlogs, vlogs) of the chain, vlogs in 1/fs."""
