@@ -468,6 +468,28 @@ int pgk_sta_write(const char *path, const pgk_sta_header *h, const double *pos,
  * newline, into buf (cap bytes).                                             */
 int pgk_out_ene_row(int32_t step, const double *vals, int32_t nvals, int32_t high, char *buf,
                     int32_t cap);
+/* out_pre.dat as outpre writes it (output_data.F90:549-818), for a build
+ * without custom non-bonded terms (ncstmnbex = 0): the header line of its
+ * first call, (A9,1X,81(1X,A23)) with the field names right-justified, and
+ * one data row (I10, nvals(1X,E23.8)), E23.16 with high != 0
+ * (-D_OUTENE_HIGH).  The caller supplies the 81 values in the file's order,
+ * already in Pa / J.                                                        */
+int pgk_out_pre_header(char *buf, int32_t cap);
+int pgk_out_pre_row(int32_t step, const double *vals, int32_t nvals, int32_t high, char *buf,
+                    int32_t cap);
+/* out_htf.dat as outhtf writes it (output_hf.F90:8-686), for a build
+ * without custom non-bonded terms: 36 fields per region (total, kinetic,
+ * the 13 potential and 13 virial classes, the 8 argon-pair splits), each
+ * with x, y, z of every region when all_dir != 0 (_HF_ALL_DIR / _HF_BULK
+ * builds) or z only.  The header names are name_<c>_<region> written
+ * left-justified in A23; a row takes hf as pgk_calheatf returns it
+ * ([nregion][PGK_NHFJ][3]) times jref, with the classes this library does
+ * not evaluate (Morse, SH, RFH, DOU, custom, argon splits) as zero,
+ * (I10, n(1X,E23.8)) or E23.16 with high != 0 (-D_OUTHF_HIGH).  As the
+ * reference, a surface-mode driver writes no row at its first step.       */
+int pgk_out_htf_header(int32_t nregion, int32_t all_dir, char *buf, int32_t cap);
+int pgk_out_htf_row(int32_t step, const double *hf, int32_t nregion, int32_t all_dir,
+                    int32_t high, double jref, char *buf, int32_t cap);
 
 /* The .ini control script (rdscript.F90 with rdfree.F90's word splitting):
  * each line's words split at spaces, commas and tabs, the first word the key,

@@ -89,3 +89,55 @@ def test_ini_words_of_the_reference_script():
     with pytest.raises(PgkError):
         ini.int("npolytyp", 3)
     ini.close()
+
+
+def test_out_pre_header_and_row_match_the_reference_file():
+    lines = open(os.path.join(GOLD, "out_pre.dat")).read().splitlines()
+    assert formats.out_pre_header() == lines[0] + "\n"
+    f = lines[1].split()
+    assert len(f) == 82                      # nstep + nprefield = 81 (output_data.F90:700)
+    assert formats.out_pre_row(int(f[0]), [float(x) for x in f[1:]]) == lines[1] + "\n"
+    hi = formats.out_pre_row(int(f[0]), [float(x) for x in f[1:]], high=True)
+    assert len(hi) == 10 + 81 * 24 + 1
+
+
+@pytest.mark.parametrize("all_dir", [True, False])
+def test_out_htf_layout(all_dir):
+    """outhtf's layout (output_hf.F90:164-167, 485-686): 36 fields per
+    region; for each field the regions in order, x y z of each with all_dir;
+    header names name_<c>_<region> left-justified in A23; values from
+    calheatf's [region, term, component] times jref, classes this library
+    does not evaluate as zero."""
+    from thrid_amd import abi
+    nreg = 3
+    nc = 3 if all_dir else 1
+    head = formats.out_htf_header(nreg, all_dir)
+    names = head.split()
+    assert names[0] == "nstep" and len(names) == 1 + 36 * nreg * nc
+    assert len(head) == 9 + 1 + 24 * 36 * nreg * nc + 1
+    comps = "xyz" if all_dir else "z"
+    assert names[1:1 + nreg * nc] == [f"hftot_atm_{c}_{r}" for r in range(1, nreg + 1)
+                                      for c in comps]
+    assert names[-1] == f"hfvirilj_ar2pt_z_{nreg}"
+    assert head.index("hfkin_atm_") == 9 + 1 + 24 * nreg * nc + 1   # left-justified
+    hf = np.zeros((nreg, abi.NHFJ, 3))
+    for r in range(nreg):
+        for t in range(abi.NHFJ):
+            for c in range(3):
+                hf[r, t, c] = (r + 1) * 1000 + t * 10 + c + 1
+    row = formats.out_htf_row(7, hf, jref=0.5, all_dir=all_dir)
+    vals = row.split()
+    assert int(vals[0]) == 7 and len(vals) == len(names)
+    assert len(row) == 10 + 24 * 36 * nreg * nc + 1
+    got = dict(zip(names[1:], (float(v) for v in vals[1:])))
+    for r in range(nreg):
+        for ci, c in enumerate("xyz"):
+            if c not in comps:
+                continue
+            k = f"_{c}_{r + 1}"
+            assert got["hftot_atm" + k] == 0.5 * hf[r, abi.HFJ_TOT, ci]
+            assert got["hfkin_atm" + k] == 0.5 * hf[r, abi.HFJ_KIN, ci]
+            assert got["hfpotlj_atm" + k] == 0.5 * hf[r, abi.HFJ_POT + 7, ci]
+            assert got["hfvirielc_atm" + k] == 0.5 * hf[r, abi.HFJ_VIRI + 6, ci]
+            assert got["hfpot14_atm" + k] == 0.5 * hf[r, abi.HFJ_POT + 3, ci]
+            assert got["hfpotmor_atm" + k] == 0.0 and got["hfvirilj_ar12" + k] == 0.0

@@ -309,6 +309,163 @@ int pgk_out_ene_row(int32_t step, const double *vals, int32_t nvals, int32_t hig
   return PGK_OK;
 }
 
+// ---- out_pre.dat (outpre, output_data.F90:549-818) --------------------------
+// The 81 fields of a build without custom non-bonded terms: 21 scalars, six
+// 3x3 tensors listed row by row (11, 12, 13, 21, ...), six traces.
+int pgk_out_pre_header(char *buf, int32_t cap) {
+  if (!buf) {
+    pgk_set_error("pgk_out_pre_header: null argument");
+    return PGK_EINVAL;
+  }
+  static const char *const scalars[] = {
+      "pressmol_tot",   "pressmol_ktot",  "pressmol_vtot",   "pressatm_tot",
+      "pressatm_ktot",  "pressatm_vtot",  "pot_viric_all",   "pot_virilj_all",
+      "pot_virimor_all", "pot_virish_all", "pot_virirfh_all", "pot_viridou_all",
+      "pot_viricstmnb", "pot_viri_all",   "viri_fdotd",      "atm_viribo_all",
+      "atm_virian_all", "atm_virito_all", "atm_viri14_all",  "atm_viri_const",
+      "atm_viri_corr"};
+  static const char *const tensors[] = {"premolt_tot", "premolt_ktot", "premolt_vtot",
+                                        "preatmt_tot", "preatmt_ktot", "preatmt_vtot"};
+  static const char *const traces[] = {"premol_trPdiv3", "premol_trK", "premol_trV",
+                                       "preatm_trPdiv3", "preatm_trK", "preatm_trV"};
+  // (A9, 1X, n(1X, A23)): literal names, right-justified
+  std::string l = fort_a("nstep", 9) + " ";
+  for (const char *n : scalars) l += " " + fort_a(n, 23);
+  for (const char *t : tensors)
+    for (int i = 1; i <= 3; i++)
+      for (int j = 1; j <= 3; j++)
+        l += " " + fort_a(std::string(t) + std::to_string(i) + std::to_string(j), 23);
+  for (const char *n : traces) l += " " + fort_a(n, 23);
+  l += '\n';
+  if ((int64_t)l.size() + 1 > cap) {
+    pgk_set_error("pgk_out_pre_header: buffer too small");
+    return PGK_EINVAL;
+  }
+  memcpy(buf, l.c_str(), l.size() + 1);
+  return PGK_OK;
+}
+
+int pgk_out_pre_row(int32_t step, const double *vals, int32_t nvals, int32_t high, char *buf,
+                    int32_t cap) {
+  if (!buf || (nvals > 0 && !vals) || nvals < 0) {
+    pgk_set_error("pgk_out_pre_row: null argument");
+    return PGK_EINVAL;
+  }
+  std::string l = fort_i(step, 10);
+  for (int i = 0; i < nvals; i++) l += " " + fort_e(vals[i], 23, high ? 16 : 8);
+  l += '\n';
+  if ((int64_t)l.size() + 1 > cap) {
+    pgk_set_error("pgk_out_pre_row: buffer too small");
+    return PGK_EINVAL;
+  }
+  memcpy(buf, l.c_str(), l.size() + 1);
+  return PGK_OK;
+}
+
+}  // extern "C"
+
+// ---- out_htf.dat (outhtf, output_hf.F90:8-686) ------------------------------
+// The 36 fields per region of a build without custom non-bonded terms, in
+// the file's order, with the heat-flux term each one takes (-1: a
+// potential this library does not evaluate, written as zero).
+namespace {
+struct HtfField {
+  const char *name;
+  int hfj;
+};
+const HtfField kHtf[36] = {
+    {"hftot_atm", PGK_HFJ_TOT},
+    {"hfkin_atm", PGK_HFJ_KIN},
+    {"hfpotbo_atm", PGK_HFJ_POT + PGK_HF_BO},
+    {"hfpotan_atm", PGK_HFJ_POT + PGK_HF_AN},
+    {"hfpotto_atm", PGK_HFJ_POT + PGK_HF_TO},
+    {"hfpot14_atm", PGK_HFJ_POT + PGK_HF_14},
+    {"hfpotelin_atm", PGK_HFJ_POT + PGK_HF_ELIN},
+    {"hfpotljin_atm", PGK_HFJ_POT + PGK_HF_LJIN},
+    {"hfpotelc_atm", PGK_HFJ_POT + PGK_HF_ELC},
+    {"hfpotlj_atm", PGK_HFJ_POT + PGK_HF_VDW},
+    {"hfpotmor_atm", -1},
+    {"hfpotsh_atm", -1},
+    {"hfpotrfh_atm", -1},
+    {"hfpotdou_atm", -1},
+    {"hfpotcstmnb_atm", -1},
+    {"hfviribo_atm", PGK_HFJ_VIRI + PGK_HF_BO},
+    {"hfvirian_atm", PGK_HFJ_VIRI + PGK_HF_AN},
+    {"hfvirito_atm", PGK_HFJ_VIRI + PGK_HF_TO},
+    {"hfviri14_atm", PGK_HFJ_VIRI + PGK_HF_14},
+    {"hfvirielin_atm", PGK_HFJ_VIRI + PGK_HF_ELIN},
+    {"hfviriljin_atm", PGK_HFJ_VIRI + PGK_HF_LJIN},
+    {"hfvirielc_atm", PGK_HFJ_VIRI + PGK_HF_ELC},
+    {"hfvirilj_atm", PGK_HFJ_VIRI + PGK_HF_VDW},
+    {"hfvirimor_atm", -1},
+    {"hfvirish_atm", -1},
+    {"hfvirirfh_atm", -1},
+    {"hfviridou_atm", -1},
+    {"hfviricstmnb_atm", -1},
+    {"hfpotlj_ar11", -1},
+    {"hfpotlj_ar22", -1},
+    {"hfpotlj_ar12", -1},
+    {"hfvirilj_ar11", -1},
+    {"hfvirilj_ar22", -1},
+    {"hfvirilj_ar12", -1},
+    {"hfvirilj_ar1pt", -1},
+    {"hfvirilj_ar2pt", -1},
+};
+
+int put_line(const std::string &l, char *buf, int32_t cap, const char *who) {
+  if ((int64_t)l.size() + 1 > cap) {
+    pgk_set_error(std::string(who) + ": buffer too small");
+    return PGK_EINVAL;
+  }
+  memcpy(buf, l.c_str(), l.size() + 1);
+  return PGK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// Header: names built as name_<x|y|z>_<region> with the blanks removed
+// (excl_sp) and held in 80-character variables, so (1X, A23) writes them
+// left-justified; all_dir selects the _HF_ALL_DIR / _HF_BULK layout
+// (x, y, z of every region), otherwise the z component only.
+int pgk_out_htf_header(int32_t nregion, int32_t all_dir, char *buf, int32_t cap) {
+  if (!buf || nregion < 1) {
+    pgk_set_error("pgk_out_htf_header: null buffer or nregion < 1");
+    return PGK_EINVAL;
+  }
+  static const char comp[3] = {'x', 'y', 'z'};
+  std::string l = fort_a("nstep", 9) + " ";
+  for (const HtfField &f : kHtf)
+    for (int r = 1; r <= nregion; r++)
+      for (int n = all_dir ? 0 : 2; n < 3; n++) {
+        std::string nm = std::string(f.name) + "_" + comp[n] + "_" + std::to_string(r);
+        nm.resize(80, ' ');
+        l += " " + nm.substr(0, 23);
+      }
+  l += '\n';
+  return put_line(l, buf, cap, "pgk_out_htf_header");
+}
+
+// Row: hf as pgk_calheatf writes it ([nregion][PGK_NHFJ][3]), every value
+// times jref (the caller's unit factor, eref / (xref^2 timeref) in the
+// reference), as (I10, n(1X, E23.8)) or E23.16 with high != 0.
+int pgk_out_htf_row(int32_t step, const double *hf, int32_t nregion, int32_t all_dir,
+                    int32_t high, double jref, char *buf, int32_t cap) {
+  if (!buf || !hf || nregion < 1) {
+    pgk_set_error("pgk_out_htf_row: null argument or nregion < 1");
+    return PGK_EINVAL;
+  }
+  std::string l = fort_i(step, 10);
+  for (const HtfField &f : kHtf)
+    for (int r = 0; r < nregion; r++)
+      for (int n = all_dir ? 0 : 2; n < 3; n++) {
+        const double v = f.hfj < 0 ? 0.0 : hf[((size_t)r * PGK_NHFJ + f.hfj) * 3 + n] * jref;
+        l += " " + fort_e(v, 23, high ? 16 : 8);
+      }
+  l += '\n';
+  return put_line(l, buf, cap, "pgk_out_htf_row");
+}
+
 }  // extern "C"
 
 // ---- the .ini control script (rdscript.F90, words split as rdfree.F90) -----

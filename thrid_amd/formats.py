@@ -24,6 +24,11 @@ def _lib():
     L.pgk_sta_read.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
     L.pgk_sta_write.argtypes = [C.c_char_p, C.POINTER(StaHeader)] + [vp] * 6
     L.pgk_out_ene_row.argtypes = [C.c_int32, vp, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.pgk_out_pre_row.argtypes = [C.c_int32, vp, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.pgk_out_pre_header.argtypes = [C.c_char_p, C.c_int32]
+    L.pgk_out_htf_header.argtypes = [C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.pgk_out_htf_row.argtypes = [C.c_int32, vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                  C.c_char_p, C.c_int32]
     L.pgk_ini_read.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.pgk_ini_free.argtypes = [vp]
     L.pgk_ini_free.restype = None
@@ -36,6 +41,7 @@ def _lib():
     L.pgk_ini_get_int.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, vp]
     L.pgk_ini_get_bool.argtypes = [vp, C.c_char_p, C.c_int32, C.c_int32, vp]
     for f in ("pgk_sta_read_header", "pgk_sta_read", "pgk_sta_write", "pgk_out_ene_row",
+              "pgk_out_pre_row", "pgk_out_pre_header", "pgk_out_htf_header", "pgk_out_htf_row",
               "pgk_ini_read", "pgk_ini_get_str", "pgk_ini_get_double", "pgk_ini_get_int",
               "pgk_ini_get_bool"):
         getattr(L, f).restype = C.c_int
@@ -92,6 +98,46 @@ def out_ene_row(step, vals, high=False):
     buf = C.create_string_buffer(32 * (len(v) + 2))
     pgk._check(L.pgk_out_ene_row(int(step), v.ctypes.data, len(v), int(bool(high)), buf,
                                  len(buf)), "pgk_out_ene_row")
+    return buf.value.decode()
+
+
+def out_pre_header():
+    """The header line of out_pre.dat (with its newline)."""
+    L = _lib()
+    buf = C.create_string_buffer(32 * 84)
+    pgk._check(L.pgk_out_pre_header(buf, len(buf)), "pgk_out_pre_header")
+    return buf.value.decode()
+
+
+def out_pre_row(step, vals, high=False):
+    """One out_pre.dat data row (with its newline): the 81 values in the
+    file's order."""
+    L = _lib()
+    v = np.ascontiguousarray(vals, np.float64)
+    buf = C.create_string_buffer(32 * (len(v) + 2))
+    pgk._check(L.pgk_out_pre_row(int(step), v.ctypes.data, len(v), int(bool(high)), buf,
+                                 len(buf)), "pgk_out_pre_row")
+    return buf.value.decode()
+
+
+def out_htf_header(nregion, all_dir=True):
+    """The header line of out_htf.dat for nregion regions."""
+    L = _lib()
+    buf = C.create_string_buffer(32 * (36 * 3 * int(nregion) + 2))
+    pgk._check(L.pgk_out_htf_header(int(nregion), int(bool(all_dir)), buf, len(buf)),
+               "pgk_out_htf_header")
+    return buf.value.decode()
+
+
+def out_htf_row(step, hf, jref=1.0, all_dir=True, high=False):
+    """One out_htf.dat data row from hf [nregion, NHFJ, 3] (calheatf's
+    layout) times jref."""
+    L = _lib()
+    h = np.ascontiguousarray(hf, np.float64)
+    nreg = h.shape[0]
+    buf = C.create_string_buffer(32 * (36 * 3 * nreg + 2))
+    pgk._check(L.pgk_out_htf_row(int(step), h.ctypes.data, nreg, int(bool(all_dir)),
+                                 int(bool(high)), float(jref), buf, len(buf)), "pgk_out_htf_row")
     return buf.value.decode()
 
 
