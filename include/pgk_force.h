@@ -351,6 +351,16 @@ int pgk_ewaldk_hf(pgk_ctx *ctx, int32_t kmax, double yratio, double zratio,
  * (fft_pme.F90:872-937 with B-splines in place of their derivatives), so
  * sum_i e_i = pot_ewk.  e_atom [natom], eref; the self term is not included. */
 int pgk_recip_energy_atom(pgk_ctx *ctx, double *e_atom);
+/* Per-atom reciprocal (SPME) virial tensors of the same evaluation (no
+ * reference counterpart: the SPME form of ewaldkp_hf's virielct_atm):
+ * W_i^ab = q_i phi_ab(r_i) / 2, with phi_ab the charge grid convolved with
+ * scalar_sum's virial kernel eterm (delta_ab - vterm m_a m_b)
+ * (fft_pme.F90:544-690; the sign of the Coulomb virial tensor and of
+ * virielct_atm), one transform pair per tensor component.  viri_atm
+ * [9*natom], Fortran (3,3,natom), eref; sum_i W_i is the reciprocal virial
+ * tensor of the evaluation.  The net-charge correction (ifnetqcorrp) is not
+ * distributed over the atoms.                                              */
+int pgk_recip_virial_atom(pgk_ctx *ctx, double *viri_atm);
 
 /* Heat-flux time series of pgk_md_run / pgk_md_run_batch (no reference
  * counterpart as a call: the device form of the heat-flux outputs that the

@@ -64,3 +64,42 @@ def test_spme_atom_energies(oracle_mod, prec):
     with pytest.raises(PgkError):
         c.recip_energy_atom()                 # the grid belongs to the old call
     c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_spme_atom_virials(oracle_mod, prec):
+    """Per-atom SPME reciprocal virial tensors: symmetric, they add up to the
+    reciprocal virial tensor of the same call (bonds and exclusions removed,
+    real-space terms off, so the Coulomb tensor is the reciprocal one alone),
+    and with a fine grid they approach the standard Ewald per-atom tensors of
+    the independent k-space sum (ewaldkp_hf's virielct_atm)."""
+    from thrid_amd import abi
+    from thrid_amd.pgk import ForceContext, PgkError
+    s = dict(water648())
+    for k in ("bond_ij", "bond_type", "angl_ijk", "angl_type"):
+        s[k] = np.asarray(s[k])[:0]
+    s["nfft"] = np.array([48, 48, 48], np.int32)
+    s["pme_order"] = 8
+    c = ForceContext(s, prec)
+    with pytest.raises(PgkError):
+        c.recip_virial_atom()                 # nothing evaluated yet
+    c.mklist(s["rcut_book"])
+    r = c.calforce(abi.MTS_LONG, abi.Mts(0, 0, 0, 0, 0, abi.MTS_LONG, 0))
+    w = c.recip_virial_atom()
+    vt = r["virt"][abi.TENS_NAMES.index("pot_virit_coul")]
+    assert np.max(np.abs(w - np.transpose(w, (0, 2, 1)))) == 0.0
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    wsum = w.sum(axis=0)
+    assert np.max(np.abs(wsum - vt)) <= tol * np.max(np.abs(vt)), (wsum, vt)
+    ew = oracle_mod.Oracle(s).ewaldk_hf(10)
+    ref = np.asarray(ew["viri_atm"]).reshape(-1, 3, 3)
+    err = np.max(np.abs(w - ref)) / np.max(np.abs(ref))
+    print(prec, "per-atom virial vs standard Ewald", err)
+    assert err <= (1e-7 if prec == "fp64" else 1e-5)
+    # the energies of the same evaluation still come from the potential grid
+    e = c.recip_energy_atom()
+    assert abs(e.sum() - r["ene"][abi.ENE_NAMES.index("pot_ewk")]) <= 1e-5 * abs(e.sum())
+    c.set_positions(s["pos"])
+    with pytest.raises(PgkError):
+        c.recip_virial_atom()
+    c.close()

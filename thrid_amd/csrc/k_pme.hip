@@ -1199,6 +1199,111 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(void *cgrid, int TW, FftPlan
   block_record<8, 4>(ev, red + (size_t)blockIdx.x * PGK_REC);
 }
 
+// Per-atom reciprocal virial, component comp (0..5: xx yy zz xy xz yz) of
+// the tensor kernel G_ab(m) = eterm(m) (delta_ab - vterm m_a m_b), the
+// virial of scalar_sum (fft_pme.F90:544-690) in the sign of the library's
+// Coulomb virial tensor and of ewaldkp_hf's virielct_atm: the z pass of the transformed
+// charge grid forward, the product with G_ab, and back, as k_fft_z_conv
+// does with eterm.  Off-diagonal terms on a Nyquist plane that the
+// reference's half-spectrum weights drop (w12, w13, w23 = 0) are zeroed, so
+// the per-atom tensors add up to the reference's virial.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_z_vir(void *cgrid, int TW, FftPlan p,
+                                                    const void *twv, ConvArgs ca, int comp) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n;   // == K3
+  const int ls = n + 1;
+  const int H = ca.H, K2 = ca.K[1];
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + TW * ls;
+  C *tw = pb + TW * ls;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = reinterpret_cast<const C *>(twv)[t];
+  C *cg = reinterpret_cast<C *>(cgrid);
+  const int ntile = (H + TW - 1) / TW;
+  const int k2 = blockIdx.x / ntile;
+  const int h0 = (blockIdx.x - k2 * ntile) * TW;
+  const size_t zstride = (size_t)K2 * H;
+  const int lt = __ffs(TW) - 1;   // TW is a power of two
+  double *sb1 = reinterpret_cast<double *>(tw + n);
+  double *sb3 = sb1 + TW;
+  for (int t = threadIdx.x; t < TW + n; t += blockDim.x) {
+    if (t < TW) sb1[t] = (h0 + t < H) ? ca.bsp[h0 + t] : 1.0;
+    else sb3[t - TW] = ca.bsp[ca.K[0] + K2 + (t - TW)];
+  }
+  stage_in<C>(
+      TW * n,
+      [&](int t) {
+        const int k3 = t >> lt, l = t & (TW - 1);
+        C z;
+        z.x = z.y = R(0);
+        if (h0 + l < H) z = cg[(size_t)k3 * zstride + (size_t)(h0 + l) * K2 + k2];
+        return z;
+      },
+      [&](int t, C z) { pa[(t & (TW - 1)) * ls + (t >> lt)] = z; });
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, ls, p, tw, -1);
+  const int K1 = ca.K[0], K3 = ca.K[2];
+  const int nf1 = (K1 + 1) / 2, nf2 = (K2 + 1) / 2, nf3 = (K3 + 1) / 2;
+  const int m2 = (k2 < nf2) ? k2 : k2 - K2;
+  const double mh2 = ca.recip[1] * m2;
+  const double s2 = ((K2 & 1) == 0 && k2 == K2 / 2) ? 1.0 : -1.0;
+  const double bsp2 = ca.bsp[K1 + k2];
+  const int a = comp < 3 ? comp : (comp == 5 ? 1 : 0);
+  const int b = comp < 3 ? comp : (comp == 3 ? 1 : 2);
+  const float rn = 1.0f / (float)n;
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int l = fdiv(t, n, rn), k3 = t - l * n;
+    int h = h0 + l;
+    if (h >= H) continue;
+    double g = 0.0;
+    if (h != 0 || k2 != 0 || k3 != 0) {
+      double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const double eterm = conv_mode(ca, h, k3, mh2, s2, sb1[l], bsp2, sb3[k3], 0.0, 0.0, ev);
+      if (eterm != 0.0) {
+        const int m1 = (h < nf1) ? h : h - K1;
+        const int m3 = (k3 < nf3) ? k3 : k3 - K3;
+        const double mh1 = ca.recip[0] * m1, mh3 = ca.recip[2] * m3;
+        const double msq = msq_nc(mh1, mh2, mh3);
+        const double vterm = 2.0 * (ca.fac * msq + 1.0) / msq;
+        const double ma = a == 0 ? mh1 : (a == 1 ? mh2 : mh3);
+        const double mb = b == 0 ? mh1 : (b == 1 ? mh2 : mh3);
+        g = eterm * ((a == b ? 1.0 : 0.0) - vterm * ma * mb);
+        const bool single = (h == 0) || ((K1 & 1) == 0 && h == K1 / 2);
+        const double s3 = ((K3 & 1) == 0 && k3 == K3 / 2) ? 1.0 : -1.0;
+        if (!single && ((comp == 3 && s2 > 0.0) || (comp == 4 && s3 > 0.0) ||
+                        (comp == 5 && s2 * s3 < 0.0)))
+          g = 0.0;
+      }
+    }
+    C z = pa[l * ls + k3];
+    z.x = (R)(g * (double)z.x);
+    z.y = (R)(g * (double)z.y);
+    pa[l * ls + k3] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, TW, ls, p, tw, +1);
+  for (int t = threadIdx.x; t < TW * n; t += blockDim.x) {
+    int k3 = t >> lt, l = t & (TW - 1);
+    if (h0 + l < H) cg[(size_t)k3 * zstride + (size_t)(h0 + l) * K2 + k2] = pa[l * ls + k3];
+  }
+}
+
+// the six per-atom components into the Fortran (3,3,natom) layout
+__global__ __launch_bounds__(256) void k_vir_assemble(int n, const double *__restrict__ v6,
+                                                      double *__restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xx = v6[i], yy = v6[n + i], zz = v6[2 * (size_t)n + i];
+  const double xy = v6[3 * (size_t)n + i], xz = v6[4 * (size_t)n + i],
+               yz = v6[5 * (size_t)n + i];
+  double *o = w + 9 * (size_t)i;
+  o[0] = xx; o[1] = xy; o[2] = xz;
+  o[3] = xy; o[4] = yy; o[5] = yz;
+  o[6] = xz; o[7] = yz; o[8] = zz;
+}
+
 FftPlan make_plan(int n) {
   FftPlan p;
   p.n = n;
@@ -1566,5 +1671,118 @@ int pgk_pme_atom_energy(pgk_ctx *c, double *d_e) {
     case PGK_PREC_FP64: return launch_phi_atoms<PGK_PREC_FP64>(c, d_e);
     case PGK_PREC_MIXED: return launch_phi_atoms<PGK_PREC_MIXED>(c, d_e);
     default: return launch_phi_atoms<PGK_PREC_FP32>(c, d_e);
+  }
+}
+
+// Per-atom reciprocal virial tensors of the charge grid of the last
+// reciprocal evaluation (still in d_grid): for each of the six components
+// the grid is transformed, multiplied by G_ab, transformed back into
+// d_vgrid and interpolated at the atoms as the energies are,
+// W_i^ab = q_i / 2 sum theta_i phi_ab; d_w receives (3,3,natom).
+template <int PREC>
+static int launch_atom_virial(pgk_ctx *c, double *d_w) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  hipStream_t st = c->stream;
+  const int K1 = c->nfft[0], K2 = c->nfft[1], K3 = c->nfft[2];
+  const int H = K1 / 2 + 1, n = c->natom;
+  const size_t G = (size_t)K1 * K2 * K3;
+  int rc;
+  if ((rc = grow_dev(&c->d_vgrid, &c->vgrid_bytes, G * sizeof(R))) ||
+      (rc = grow_dev((void **)&c->d_vir6, &c->vir6_bytes, sizeof(double) * 6 * (size_t)n)))
+    return rc;
+  const void *tw = (PREC == PGK_PREC_FP64) ? (const void *)c->d_tw : (const void *)c->d_twf;
+  const size_t csz = sizeof(C);
+  const C *twc = reinterpret_cast<const C *>(tw);
+  FftPlan p1 = make_plan(K1), p2 = make_plan(K2), p3 = make_plan(K3);
+  const int nrow = K2 * K3;
+  ConvArgs ca;
+  ca.bsp = c->d_bsp;
+  for (int d = 0; d < 3; d++) {
+    ca.recip[d] = 1.0 / c->box[d];
+    ca.K[d] = c->nfft[d];
+  }
+  const double pi = 3.14159265358979323846;
+  ca.fac = pi * pi / (c->alpha * c->alpha);
+  ca.rk2 = c->rkcut * c->rkcut;
+  ca.pivol = pi * c->box[0] * c->box[1] * c->box[2];
+  ca.rpivol = 1.0 / ca.pivol;
+  ca.netq = c->netchrgsq;
+  ca.H = H;
+  ca.ifnetq = 0;
+  const void *tw1 = twc + c->tw_off[0], *tw2 = twc + c->tw_off[1], *tw3 = twc + c->tw_off[2];
+  const int nlx = lines_for(K1, csz, 16);
+  const int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
+  const int twy = lines_for(K2, csz, 16);
+  const int by = K3 * ((H + twy - 1) / twy);
+  const int twz = lines_for(K3, csz, 16);
+  const int bz = K2 * ((H + twz - 1) / twz);
+  PmeGeom g;
+  for (int d = 0; d < 3; d++) {
+    g.K[d] = c->nfft[d];
+    g.recip[d] = 1.0 / c->box[d];
+  }
+  g.H = H;
+  g.order = c->order;
+  for (int comp = 0; comp < 6; comp++) {
+    const bool rx = with_reg_fft(K1, [&](auto f) {
+      using F = decltype(f);
+      const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+      k_fft_x_fwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_grid, c->d_cgrid, nrow, tw1, H, K2);
+    });
+    if (!rx)
+      k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow,
+                                                                   nlx, p1, tw1, H, K2);
+    auto fft_y = [&](auto sgn) {
+      constexpr int SGN = decltype(sgn)::value;
+      const bool ry = with_reg_fft(K2, [&](auto f) {
+        using F = decltype(f);
+        const int nb = (H * K3 + F::LPW - 1) / F::LPW;
+        k_fft_y_reg<PREC, F, SGN><<<nb, 64, 0, st>>>(c->d_cgrid, H, K3, tw2);
+      });
+      if (!ry)
+        k_fft_y<PREC><<<by, 256, fft_lds_bytes(twy, K2, csz), st>>>(c->d_cgrid, H, K2, twy, p2,
+                                                                    tw2, SGN);
+    };
+    fft_y(std::integral_constant<int, -1>{});
+    k_fft_z_vir<PREC><<<bz, 256, fft_lds_bytes(twz, K3, csz) + (twz + K3) * sizeof(double),
+                        st>>>(c->d_cgrid, twz, p3, tw3, ca, comp);
+    fft_y(std::integral_constant<int, +1>{});
+    if (!rx)
+      k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_vgrid, nrow,
+                                                                   nlx, p1, tw1, H, K2);
+    else
+      with_reg_fft(K1, [&](auto f) {
+        using F = decltype(f);
+        const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+        k_fft_x_bwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_cgrid, c->d_vgrid, nrow, tw1, H, K2);
+      });
+    HIP_CHECK(hipGetLastError());
+    double *out = c->d_vir6 + (size_t)comp * n;
+    const int nb = pgk_grid_blocks(n, 256);
+#define PGK_PHI(P) k_phi_atoms<PREC, P><<<nb, 256, 0, st>>>(n, c->d_pos, c->d_q, g, c->d_vgrid, out)
+    switch (c->order) {
+      case 3: PGK_PHI(3); break;
+      case 4: PGK_PHI(4); break;
+      case 5: PGK_PHI(5); break;
+      case 6: PGK_PHI(6); break;
+      case 7: PGK_PHI(7); break;
+      case 8: PGK_PHI(8); break;
+      case 9: PGK_PHI(9); break;
+      default: PGK_PHI(10); break;
+    }
+#undef PGK_PHI
+    HIP_CHECK(hipGetLastError());
+  }
+  k_vir_assemble<<<pgk_grid_blocks(n, 256), 256, 0, st>>>(n, c->d_vir6, d_w);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+int pgk_pme_atom_virial(pgk_ctx *c, double *d_w) {
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_atom_virial<PGK_PREC_FP64>(c, d_w);
+    case PGK_PREC_MIXED: return launch_atom_virial<PGK_PREC_MIXED>(c, d_w);
+    default: return launch_atom_virial<PGK_PREC_FP32>(c, d_w);
   }
 }

@@ -846,7 +846,8 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_md_small,
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
-                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det};
+                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_vgrid,
+                  c->d_vir6};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);
@@ -1916,6 +1917,34 @@ int pgk_recip_energy_atom(pgk_ctx *c, double *e_atom) {
   HIP_CHECK(hipMemcpyAsync(e_atom, c->d_hj_fscratch, sizeof(double) * c->natom,
                            hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  return PGK_OK;
+}
+
+int pgk_recip_virial_atom(pgk_ctx *c, double *viri_atm) {
+  if (!c || !viri_atm) {
+    pgk_set_error("pgk_recip_virial_atom: null argument");
+    return PGK_EINVAL;
+  }
+  if (!c->pme_grid_ready) {
+    pgk_set_error("pgk_recip_virial_atom: no reciprocal-space evaluation at the current "
+                  "positions");
+    return PGK_ESTATE;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  std::vector<double> z((size_t)9 * c->natom, 0.0);
+  double *d_w = nullptr;
+  if ((rc = upload(&d_w, z))) return rc;
+  if ((rc = pgk_pme_atom_virial(c, d_w))) {
+    (void)hipFree(d_w);
+    return rc;
+  }
+  const hipError_t e1 = hipMemcpyAsync(viri_atm, d_w, sizeof(double) * z.size(),
+                                       hipMemcpyDeviceToHost, c->stream);
+  const hipError_t e2 = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_w);
+  HIP_CHECK(e1);
+  HIP_CHECK(e2);
   return PGK_OK;
 }
 

@@ -216,6 +216,12 @@ struct pgk_ctx {
   bool hf_export = true;           // run_calforce writes the caller-layout HF arrays
   bool hf_pairs_only = false;      // run_calforce: pair heat-flux pass without pair forces
   bool pme_grid_ready = false;     // d_pgrid holds the potential of the current positions
+  // per-atom reciprocal virial (pgk_recip_virial_atom): one component's
+  // potential grid at a time, and the six per-atom components
+  void *d_vgrid = nullptr;
+  size_t vgrid_bytes = 0;
+  double *d_vir6 = nullptr;
+  size_t vir6_bytes = 0;
   // calheatf (pgk_calheatf): velocities, masses, molecule lists, partial records
   double *d_hj_vel = nullptr, *d_hj_mass = nullptr, *d_hj_molevel = nullptr;
   double *d_hj_part = nullptr, *d_hj_out = nullptr;
@@ -319,6 +325,7 @@ int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_export(pgk_ctx *c);
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
 int pgk_pme_atom_energy(pgk_ctx *c, double *d_e);
+int pgk_pme_atom_virial(pgk_ctx *c, double *d_w);   // (3,3,natom) into d_w
 // grow a device buffer to at least `need` bytes (contents not kept)
 extern "C" int grow_dev(void **p, size_t *cap, size_t need);
 int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl);

@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
     L.pgk_md_respa.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts)]
     L.pgk_recip_energy_atom.argtypes = [vp, vp]
+    L.pgk_recip_virial_atom.argtypes = [vp, vp]
     L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
     L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
     L.pgk_md_nhc_state.argtypes = [vp, vp, vp]
@@ -81,7 +82,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
               "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
-              "pgk_md_respa", "pgk_recip_energy_atom", "pgk_md_langevin",
+              "pgk_md_respa", "pgk_recip_energy_atom", "pgk_recip_virial_atom", "pgk_md_langevin",
               "pgk_md_langevin_state"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
@@ -343,6 +344,14 @@ class ForceContext:
         e = np.zeros(self.n)
         _check(self.L.pgk_recip_energy_atom(self.h, e.ctypes.data), "pgk_recip_energy_atom")
         return e
+
+    def recip_virial_atom(self):
+        """Per-atom SPME reciprocal virial tensors (natom, 3, 3) of the last
+        reciprocal-space evaluation at the current positions (their sum is
+        the reciprocal virial tensor)."""
+        w = np.zeros((self.n, 3, 3))
+        _check(self.L.pgk_recip_virial_atom(self.h, w.ctypes.data), "pgk_recip_virial_atom")
+        return w
 
     def md_respa(self, nstep_short, mts=None):
         """RESPA for md_run / md_run_batch: nstep_short inner steps per
