@@ -134,3 +134,45 @@ def test_langevin_free_particles_closed_form():
     print("m v^2 / kT", got, "closed form", want)
     assert abs(got / want - 1.0) < 0.025
     assert abs(got - 1.0) > 0.05       # the closed form, not kT/m
+
+
+def test_langevin_batch_matches_single_runs():
+    """Replicas under the region thermostat (different seeds, one of them
+    with RESPA) stepped together by pgk_md_run_batch give each replica the
+    trajectory and thermostat record it gets alone, bit for bit (mixed form),
+    with per-replica graph replay asked for (Langevin steps never replay)."""
+    from thrid_amd import abi
+    from thrid_amd.pgk import ForceContext, md_run_batch
+    s = water648()
+    reg, temp, damp = _regions(s), [250.0, 400.0, 320.0], [0.02, 0.05, 0.01]
+
+    def make(k):
+        c = ForceContext(s, "mixed")
+        ij, d, gs = md.rigid_constraints(s)
+        c.md_setup(s["mass"], ij, d, gs, 1.0)
+        c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=30 + k))
+        if k == 1:
+            c.md_respa(2, abi.Mts(abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT, abi.MTS_SHORT,
+                                  abi.MTS_LONG, abi.MTS_LONG, abi.MTS_LONG))
+        c.md_langevin(reg, temp, damp, seed=1000 + k)
+        return c
+
+    single = []
+    for k in range(3):
+        c = make(k)
+        rows = c.md_run(12, 6, s["rcut_book"], 3)
+        single.append((rows, c.md_state(), c.md_langevin_state()))
+        c.close()
+    ctxs = [make(k) for k in range(3)]
+    for c in ctxs:
+        c.md_use_graphs(single=False, batch=True)
+    rows = md_run_batch(ctxs, 12, 6, [s["rcut_book"]] * 3, 3)
+    for k, c in enumerate(ctxs):
+        assert np.array_equal(rows[k], single[k][0])
+        p, v = c.md_state()
+        assert np.array_equal(p, single[k][1][0]) and np.array_equal(v, single[k][1][1])
+        det, cnt = c.md_langevin_state()
+        np.testing.assert_allclose(det, single[k][2][0], rtol=1e-12, atol=1e-12)
+        assert np.array_equal(cnt, single[k][2][1])
+        c.close()
+    assert not np.array_equal(single[0][1][1], single[2][1][1])
