@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__r
 // the registered atoms (zero for the rest) and applies the random term then
 // the friction; the second half applies friction then the same random term
 // to the atoms registered now.  The kinetic energy each region gains,
-// sum m (v'^2 - v^2) / 2, is added to det[region] (wave sums, one atomic per
-// wave and region).
+// sum m (v'^2 - v^2) / 2, is added to the block's own record
+// det[block][region] (wave sums combined in LDS in wave order): no atomics,
+// so the records, summed over blocks when read, do not depend on timing.
 template <bool FIRST>
 __global__ __launch_bounds__(256) void k_lt_half(LtArgs a, double *__restrict__ vel) {
 #pragma clang fp contract(off)
@@ -145,9 +146,16 @@ __global__ __launch_bounds__(256) void k_lt_half(LtArgs a, double *__restrict__ 
     }
     de = 0.5 * a.mass[i] * (v2n - v2o) / PGK_MD_CONV;
   }
+  __shared__ double sred[4][PGK_LT_MAXREG];
   for (int k = 0; k < a.nreg; k++) {
     const double t = wave_sum(r == k ? de : 0.0);
-    if ((threadIdx.x & 63) == 0 && t != 0.0) atomicAdd(&a.det[k], t);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6][k] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < a.nreg) {
+    const int k = threadIdx.x;
+    a.det[(size_t)blockIdx.x * PGK_LT_MAXREG + k] += sred[0][k] + sred[1][k] + sred[2][k] +
+                                                     sred[3][k];
   }
 }
 

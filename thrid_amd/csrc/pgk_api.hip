@@ -1614,7 +1614,9 @@ struct MdRunner {
     int r;
     if ((r = pgk_build_list(c, rcut_book))) return r;
     if (c->md_lt_nreg) {
-      HIP_CHECK(hipMemsetAsync(c->d_lt_det, 0, sizeof(double) * PGK_LT_MAXREG, c->stream));
+      HIP_CHECK(hipMemsetAsync(
+          c->d_lt_det, 0,
+          sizeof(double) * PGK_LT_MAXREG * (size_t)pgk_grid_blocks(c->natom, 256), c->stream));
       if ((r = pgk_md_lt_register(c))) return r;
     }
     if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_ke))) return r;
@@ -2020,7 +2022,8 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
   if ((rc = upload(&c->d_lt_ustart, ustart)) || (rc = upload(&c->d_lt_uatom, uatom)) ||
       (rc = upload(&c->d_lt_ucom, ucom)) || (rc = upload(&c->d_lt_mass, c->md_mass)) ||
       (rc = upload(&c->d_lt_frand, zero)) || (rc = upload(&c->d_lt_reg, none)) ||
-      (rc = upload(&c->d_lt_det, std::vector<double>(PGK_LT_MAXREG, 0.0))) ||
+      (rc = upload(&c->d_lt_det, std::vector<double>(
+                                     (size_t)pgk_grid_blocks(n, 256) * PGK_LT_MAXREG, 0.0))) ||
       (rc = upload(&c->d_lt_cnt, std::vector<int32_t>(PGK_LT_MAXREG, 0))))
     return rc;
   for (int k = 0; k < PGK_LT_MAXREG; k++) {
@@ -2051,8 +2054,18 @@ int pgk_md_langevin_state(pgk_ctx *c, double *det_ene_kin, int32_t *natom_lt) {
   if ((rc = pgk_md_lt_count(c))) return rc;
   HIP_CHECK(hipStreamSynchronize(c->stream));
   const int nr = c->md_lt_nreg;
-  if (det_ene_kin)
-    HIP_CHECK(hipMemcpy(det_ene_kin, c->d_lt_det, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  if (det_ene_kin) {
+    // per-block records, summed in block order
+    const int nb = pgk_grid_blocks(c->natom, 256);
+    std::vector<double> part((size_t)nb * PGK_LT_MAXREG);
+    HIP_CHECK(hipMemcpy(part.data(), c->d_lt_det, sizeof(double) * part.size(),
+                        hipMemcpyDeviceToHost));
+    for (int k = 0; k < nr; k++) {
+      double t = 0.0;
+      for (int b = 0; b < nb; b++) t += part[(size_t)b * PGK_LT_MAXREG + k];
+      det_ene_kin[k] = t;
+    }
+  }
   if (natom_lt)
     HIP_CHECK(hipMemcpy(natom_lt, c->d_lt_cnt, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
   return PGK_OK;

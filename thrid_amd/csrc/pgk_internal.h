@@ -274,7 +274,8 @@ struct pgk_ctx {
   uint64_t md_lt_seed = 0, md_lt_draw = 0;   // draw: steps run since set-up
   int32_t *d_lt_ustart = nullptr, *d_lt_uatom = nullptr, *d_lt_ucom = nullptr;
   int32_t *d_lt_reg = nullptr, *d_lt_cnt = nullptr;   // region of each atom (-1 none)
-  double *d_lt_mass = nullptr, *d_lt_frand = nullptr, *d_lt_det = nullptr;
+  double *d_lt_mass = nullptr, *d_lt_frand = nullptr;
+  double *d_lt_det = nullptr;      // [block][PGK_LT_MAXREG] energy records
 
   // reduction segment bookkeeping for the current call
   int red_used = 0;
