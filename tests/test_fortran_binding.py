@@ -85,3 +85,64 @@ def test_fortran_host_calforce_matches_oracle(tmp_path, oracle_mod):
     ref = o.calforce(abi.MTS_LONG)
     assert rms_rel(force, ref["force"]) < 1e-9
     np.testing.assert_allclose(ene, ref["ene"], rtol=1e-9, atol=1e-9)
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+MODDIR = os.path.join(ROOT, "thrid_amd", "_build", "fortran")
+
+
+@pytest.mark.skipif(not (os.path.exists(FLANG) and
+                         os.path.exists(os.path.join(MODDIR, "pgk_force_mod.mod"))),
+                    reason="flang or the built module missing")
+def test_fortran_host_writes_out_htf_lines(tmp_path):
+    """A Fortran host writes out_htf.dat lines through the module's
+    interfaces (host-only calls, no device): the same text as the C ABI
+    gives Python, with hf in the reference's (3, PGK_NHFJ, nregion) order."""
+    from thrid_amd import formats
+    src = tmp_path / "htf.F90"
+    src.write_text("""
+program htf
+  use iso_c_binding
+  use pgk_force_mod
+  implicit none
+  integer, parameter :: nreg = 2
+  real(c_double) :: hf(3, PGK_NHFJ, nreg)
+  character(kind=c_char) :: buf(20000)
+  integer :: i, n, t, r
+  do r = 1, nreg
+    do t = 1, PGK_NHFJ
+      do n = 1, 3
+        hf(n, t, r) = r * 1000.0d0 + t * 10.0d0 + n + 0.125d0
+      end do
+    end do
+  end do
+  if (pgk_out_htf_header(nreg, 1, buf, 20000) /= 0) stop 1
+  call put(buf)
+  if (pgk_out_htf_row(5, hf, nreg, 1, 0, 0.5d0, buf, 20000) /= 0) stop 2
+  call put(buf)
+contains
+  subroutine put(b)
+    character(kind=c_char), intent(in) :: b(*)
+    integer :: k
+    k = 1
+    do while (b(k) /= c_null_char)
+      k = k + 1
+    end do
+    write(*, '(*(A))', advance='no') (b(i), i = 1, k - 1)
+  end subroutine put
+end program htf
+""")
+    exe = tmp_path / "htf"
+    # the program calls only bind(C) interfaces of the module: no module object to link
+    subprocess.run([FLANG, "-O1", "-I", MODDIR, "-o", str(exe), str(src),
+                    "-L" + os.path.join(ROOT, "thrid_amd"), "-lpgkforce",
+                    "-Wl,-rpath," + os.path.join(ROOT, "thrid_amd")],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    hf = np.zeros((2, abi.NHFJ, 3))
+    for r in range(2):
+        for t in range(abi.NHFJ):
+            for n in range(3):
+                hf[r, t, n] = (r + 1) * 1000.0 + (t + 1) * 10.0 + (n + 1) + 0.125
+    want = formats.out_htf_header(2, True) + formats.out_htf_row(5, hf, jref=0.5, all_dir=True)
+    assert out == want

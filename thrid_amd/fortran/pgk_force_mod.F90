@@ -200,6 +200,44 @@ module pgk_force_mod
        integer(c_int) :: rc
      end function pgk_ewaldk_hf
 
+     ! per-atom SPME reciprocal energies (natom) and virial tensors
+     ! (3,3,natom, virielct_atm's layout) of the last reciprocal evaluation
+     function pgk_recip_energy_atom(ctx, e_atom) &
+          & bind(C, name='pgk_recip_energy_atom') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: e_atom(*)
+       integer(c_int) :: rc
+     end function pgk_recip_energy_atom
+
+     function pgk_recip_virial_atom(ctx, viri_atm) &
+          & bind(C, name='pgk_recip_virial_atom') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: viri_atm(*)
+       integer(c_int) :: rc
+     end function pgk_recip_virial_atom
+
+     ! one line of out_htf.dat (outhtf) from hf(3, PGK_NHFJ, nhfregion) times
+     ! jref, and its header; buf receives a NUL-terminated line
+     function pgk_out_htf_header(nregion, all_dir, buf, cap) &
+          & bind(C, name='pgk_out_htf_header') result(rc)
+       import :: c_int, c_int32_t, c_char
+       integer(c_int32_t), value :: nregion, all_dir, cap
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int) :: rc
+     end function pgk_out_htf_header
+
+     function pgk_out_htf_row(step, hf, nregion, all_dir, high, jref, buf, cap) &
+          & bind(C, name='pgk_out_htf_row') result(rc)
+       import :: c_int, c_int32_t, c_double, c_char
+       integer(c_int32_t), value :: step, nregion, all_dir, high, cap
+       real(c_double), intent(in) :: hf(*)
+       real(c_double), value :: jref
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int) :: rc
+     end function pgk_out_htf_row
+
      function pgk_calforce_async(ctx, mts_flag, mts, ifcalpreatom, d_force) &
           & bind(C, name='pgk_calforce_async') result(rc)
        import :: c_ptr, c_int, c_int32_t, pgk_mts
