@@ -176,3 +176,25 @@ def test_langevin_batch_matches_single_runs():
         assert np.array_equal(cnt, single[k][2][1])
         c.close()
     assert not np.array_equal(single[0][1][1], single[2][1][1])
+
+
+def test_langevin_heatf_series_sees_thermostatted_state():
+    """With the thermostat on, the heat-flux row of a sampled step is taken
+    after the thermostat's second half (calheatf follows langevin_region in
+    moldyn_hf.F90:2363-2435): it equals the fused call on the final state."""
+    from thrid_amd import abi
+    s = water648()
+    c, _ = _ctx(s, "fp64", 1.0)
+    c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=21))
+    c.md_langevin(_regions(s), [250.0, 400.0, 320.0], [0.05, 0.05, 0.05], seed=5)
+    lz = s["box"][2]
+    z = np.linspace(0.0, lz, 5)
+    hf = dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:])
+    c.md_heatf(hf, 4)
+    c.md_run(8, 20, s["rcut_book"], 4)
+    series = c.md_heatf_series()
+    p1, v1 = c.md_state()
+    end = c.calforce_heatf(hf, v1, s["mass"] / md.MD_CONV)["hf"]
+    scale = np.max(np.abs(end))
+    assert np.max(np.abs(series[-1] - end)) <= 1e-11 * scale
+    c.close()
