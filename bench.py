@@ -181,6 +181,45 @@ def run_heatf(sysd, state, precision, reps=5):
     return out
 
 
+def run_extras(sysd, state, precision, steps=200):
+    """Cost of the region Langevin thermostat in the MD step (two slabs
+    holding a fifth of the atoms; ms per step with and without it) and of
+    one per-atom SPME reciprocal virial call (six transform pairs, with the
+    copy of the 9N doubles to the host)."""
+    from thrid_amd import abi
+    from thrid_amd.pgk import ForceContext
+    pos, vel, (ij, d, gs) = state
+    s2 = dict(sysd)
+    s2["pos"] = pos
+    lx, ly, lz = (float(b) for b in sysd["box"])
+    regions = [[0, lx, 0, ly, 0, 0.1 * lz], [0, lx, 0, ly, 0.5 * lz, 0.6 * lz]]
+    out = {}
+    for lang in (False, True):
+        ctx = ForceContext(s2, precision)
+        ctx.md_setup(sysd["mass"], ij, d, gs, DT_FS)
+        ctx.md_set_velocities(vel)
+        if lang:
+            ctx.md_langevin(regions, [350.0, 250.0], [0.01, 0.01], seed=1)
+        ctx.md_run(20, NSTEP_BOOK, sysd["rcut_book"], 20)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.md_run(steps, NSTEP_BOOK, sysd["rcut_book"], steps)
+        ctx.synchronize()
+        out["langevin_ms_per_step" if lang else "nve_ms_per_step"] = \
+            (time.perf_counter() - t0) / steps * 1e3
+        ctx.close()
+    ctx = ForceContext(s2, precision)
+    ctx.mklist(sysd["rcut_book"])
+    ctx.calforce(abi.MTS_LONG)
+    ctx.recip_virial_atom()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ctx.recip_virial_atom()
+    out["recip_virial_atom_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+    ctx.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (scale for the HBM byte counters)."""
     import torch
@@ -329,6 +368,8 @@ def main():
     ap.add_argument("--ensemble-hf", type=int, default=1, metavar="K",
                     help="heat-flux sample every K ensemble steps (0: none)")
     ap.add_argument("--no-heatf", action="store_true", help="skip the heat-flux leg")
+    ap.add_argument("--extras", action="store_true",
+                    help="add the region Langevin and per-atom virial cost leg")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg:   # child process of the CPU baseline leg (no GPU use)
@@ -410,6 +451,12 @@ def main():
         out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
                                                  / PEAK["fp64"])
         out["speedup_mixed_vs_fp64"] = f64["sec"] / res["mixed"]["sec"]
+    if args.extras:
+        out["extras"] = {p: run_extras(sysd, state, p) for p in precs}
+        for p, e in out["extras"].items():
+            print(f"# extras {p}: MD step {e['nve_ms_per_step']:.4f} ms, with region Langevin "
+                  f"{e['langevin_ms_per_step']:.4f} ms; per-atom reciprocal virial "
+                  f"{e['recip_virial_atom_ms']:.3f} ms", file=sys.stderr, flush=True)
     if not args.no_heatf:
         out["heatf"] = {p: run_heatf(sysd, state, p) for p in precs}
         for p, h in out["heatf"].items():
