@@ -422,8 +422,7 @@ int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
  * with variance 2 damp kT / (m dt) per component and gets
  * v = (v + dt/2 a) exp(-dt damp/2); at the end of the step, after RATTLE,
  * v = v exp(-dt damp/2) + dt/2 a with the same a (zero for atoms that joined
- * a region during the step) (langevin_region.F90:8-203, streaming velocity
- * zero).  The uniform numbers come from SplitMix64 at counter positions
+ * a region during the step) (langevin_region.F90:8-203).  The uniform numbers come from SplitMix64 at counter positions
  * seed + (4 (draw natom + i) + k + 1) 0x9E3779B97F4A7C15, mapped to
  * ((z >> 11) + 1) 2^-53 in (0, 1], in place of the reference's SFMT stream;
  * draw counts the steps run since pgk_md_langevin.  Not combined with the
@@ -441,6 +440,14 @@ typedef struct pgk_langevin_desc {
                                 * 0: atom by atom; NULL: every molecule by    *
                                 * its centre of mass.  Molecules are molid's. */
   uint64_t seed;
+  /* streaming velocity (ass_strmvel.F90:8-295, ifstrmvel): nstream slabs
+   * [z1,z2) in the wrapped z of the same registration units; an atom whose
+   * unit lies in slab s has streaming velocity strm_vel[3s..3s+2] (A/fs),
+   * and the thermostat acts on v minus it (langevin_region.F90:81-82, 190).
+   * nstream 0 (or NULL arrays): none.                                       */
+  int32_t nstream;             /* 0..PGK_LT_MAXREG                            */
+  const double *strm_z1, *strm_z2;   /* [nstream] strmzpos1 / strmzpos2 (A)   */
+  const double *strm_vel;      /* [3*nstream] strmvel (A/fs)                  */
 } pgk_langevin_desc;
 int pgk_md_langevin(pgk_ctx *ctx, const pgk_langevin_desc *lt);
 /* Thermostat record: det_ene_kin[nregion] the kinetic energy (eref) the

@@ -336,7 +336,8 @@ class Langevin:
     splitmix64 in place of SFMT: atom i's k-th uniform in draw d sits at
     counter (d natom + i) 4 + k."""
 
-    def __init__(self, regions, temperature, damp, molid, mass, box, dt, mol_com=None, seed=0):
+    def __init__(self, regions, temperature, damp, molid, mass, box, dt, mol_com=None, seed=0,
+                 stream=None):
         import math
         self.reg_box = np.atleast_2d(np.asarray(regions, np.float64))
         nr = self.reg_box.shape[0]
@@ -363,6 +364,11 @@ class Langevin:
         self.vtmp = [2.0 * float(g) * k / dt for g, k in zip(self.damp, kt)]
         self.efac = [math.exp(-0.5 * dt * float(g)) for g in self.damp]
         self.reg = np.full(self.n, -1, np.int64)
+        # streaming-velocity slabs (ass_strmvel.F90): z ranges and velocities
+        self.stream = None if stream is None else (
+            [float(z) for z in stream[0]], [float(z) for z in stream[1]],
+            np.asarray(stream[2], np.float64).reshape(-1, 3))
+        self.vstrm = np.zeros((self.n, 3))
         self.frand = np.zeros((self.n, 3))
         self.det = np.zeros(nr)
 
@@ -399,12 +405,21 @@ class Langevin:
                 elif g[d] >= self.box[d]:
                     g[d] = g[d] - self.box[d]
             r = self.region_of(g)
+            vs = np.zeros(3)
+            if self.stream is not None:
+                for k, (z1, z2) in enumerate(zip(self.stream[0], self.stream[1])):
+                    if z1 <= g[2] < z2:
+                        vs = self.stream[2][k]
+                        break
             for i in atoms:
                 self.reg[i] = r
+                self.vstrm[i] = vs
         return self.reg
 
     def half(self, vel, first):
-        """One half of langevin_region.F90 on vel (in place)."""
+        """One half of langevin_region.F90 on vel (in place): every atom
+        loses its streaming velocity first and regains it last."""
+        vel -= self.vstrm
         on = self.reg >= 0
         idx = np.nonzero(on)[0]
         r = self.reg[idx]
@@ -434,6 +449,7 @@ class Langevin:
         vel[idx] = v
         de = 0.5 * self.mass[idx] * ((v * v).sum(axis=1) - (v0 * v0).sum(axis=1)) / MD_CONV
         np.add.at(self.det, r, de)
+        vel += self.vstrm
         return vel
 
 

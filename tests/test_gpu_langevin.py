@@ -33,8 +33,8 @@ def _regions(s):
                      [0.0, 0.5 * lx, 0.0, ly, 0.0, lz]])
 
 
-@pytest.mark.parametrize("by_com", [True, False])
-def test_langevin_matches_reference(oracle_mod, by_com):
+@pytest.mark.parametrize("by_com,streaming", [(True, False), (False, False), (True, True)])
+def test_langevin_matches_reference(oracle_mod, by_com, streaming):
     s = water648()
     dt = 1.0
     v0 = md.maxwell_boltzmann(s, 300.0, seed=12)
@@ -42,12 +42,15 @@ def test_langevin_matches_reference(oracle_mod, by_com):
     c.md_set_velocities(v0)
     reg, temp, damp = _regions(s), [250.0, 400.0, 320.0], [0.02, 0.05, 0.01]
     mol_com = None if by_com else np.zeros(int(s["molid"].max()), np.int32)
-    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=2024)
+    lz = float(s["box"][2])
+    stream = ([0.0, 0.3 * lz], [0.3 * lz, 0.7 * lz],
+              [[0.002, 0.0, 0.0], [-0.001, 0.003, 0.0]]) if streaming else None
+    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=2024, stream=stream)
     rows = c.md_run(5, 20, s["rcut_book"], 1)
     p, v = c.md_state()
     det, cnt = c.md_langevin_state()
     lt = md_reference.Langevin(reg, temp, damp, s["molid"], s["mass"], s["box"], dt,
-                               mol_com=mol_com, seed=2024)
+                               mol_com=mol_com, seed=2024, stream=stream)
     rp, rv, rrows = md_reference.run(oracle_mod.Oracle(s), s, v0, ij, d, gs, dt, 5, lang=lt)
     assert np.max(np.abs(p - rp)) < 1e-9
     assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))

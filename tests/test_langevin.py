@@ -136,3 +136,34 @@ def test_outside_atoms_untouched_and_joiners_get_friction_only():
     lt.register(pos)
     lt.half(v, False)
     assert np.all(v[1] == np.array([0.0, 0.02, 0.0]) * math.exp(-0.05))
+
+
+def test_streaming_velocity():
+    """With streaming slabs (ass_strmvel.F90: wrapped z of the unit, first
+    slab, no reach through the period) the thermostat acts on v - v_s: at
+    T = 0 the peculiar velocity decays by exp(-dt damp / 2) per half while
+    v_s is kept; atoms in no thermostat region keep their velocity."""
+    box = (10.0, 10.0, 10.0)
+    regions = [[0.0, 10.0, 0.0, 10.0, 0.0, 5.0]]
+    stream = ([0.0, 2.0], [2.0, 8.0], [[0.01, 0.0, 0.0], [0.0, -0.02, 0.0]])
+    pos = np.array([[1.0, 1.0, 1.0],    # region 0, slab 0
+                    [1.0, 1.0, 3.0],    # region 0, slab 1
+                    [1.0, 1.0, 7.0],    # no region, slab 1
+                    [1.0, 1.0, -1.5],   # z wraps to 8.5: no region, no slab
+                    [1.0, 1.0, 4.0]])   # region 0, slab 1
+    n = len(pos)
+    lt = mr.Langevin(regions, 0.0, 0.2, np.arange(1, n + 1), np.full(n, 16.0), box, 1.0,
+                     stream=stream)
+    lt.register(pos)
+    assert list(lt.reg) == [0, 0, -1, -1, 0]
+    np.testing.assert_array_equal(lt.vstrm[:, 0], [0.01, 0.0, 0.0, 0.0, 0.0])
+    np.testing.assert_array_equal(lt.vstrm[:, 1], [0.0, -0.02, -0.02, 0.0, -0.02])
+    v0 = np.array([[0.03, 0.0, 0.0], [0.0, 0.01, 0.0], [0.0, 0.05, 0.0], [0.0, 0.0, 0.04],
+                   [0.02, -0.02, 0.0]])
+    v = v0.copy()
+    lt.half(v, True)
+    f = math.exp(-0.1)
+    want = lt.vstrm + (v0 - lt.vstrm) * f
+    want[2:4] = v0[2:4]
+    np.testing.assert_allclose(v, want, rtol=0, atol=1e-17)
+    np.testing.assert_array_equal(v[4], [0.02 * f, -0.02, 0.0])   # at its stream in y

@@ -100,7 +100,8 @@ LT_MAXREG = 16
 class LangevinDesc(C.Structure):
     _fields_ = [("nregion", C.c_int32), ("x1", _dp), ("x2", _dp), ("y1", _dp), ("y2", _dp),
                 ("z1", _dp), ("z2", _dp), ("temperature", _dp), ("damp", _dp),
-                ("mol_com", _ip), ("seed", C.c_uint64)]
+                ("mol_com", _ip), ("seed", C.c_uint64), ("nstream", C.c_int32),
+                ("strm_z1", _dp), ("strm_z2", _dp), ("strm_vel", _dp)]
 
 
 def dptr(a):

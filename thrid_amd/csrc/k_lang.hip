@@ -30,9 +30,11 @@ struct LtArgs {
   double efac[PGK_LT_MAXREG];   // exp(-dt damp / 2)
   double h;                     // dt / 2
   uint64_t seed, ctr0;          // ctr0 = 4 natom draw
+  int nstrm;                    // streaming-velocity slabs [sz0, sz1) in z
+  double sz[2][PGK_LT_MAXREG], sv[3][PGK_LT_MAXREG];
   const int32_t *ustart, *uatom, *ucom;
   const double *mass;
-  int32_t *reg;
+  int32_t *reg, *sreg;
   double *frand, *det;
 };
 
@@ -94,7 +96,18 @@ __global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__r
       break;
     }
   }
-  for (int j = s; j < e; j++) a.reg[a.uatom[j]] = r;
+  // streaming slab of the same point (ass_strmvel.F90: z only, first match)
+  int sr = -1;
+  for (int k = 0; k < a.nstrm; k++) {
+    if (a.sz[0][k] <= g[2] && g[2] < a.sz[1][k]) {
+      sr = k;
+      break;
+    }
+  }
+  for (int j = s; j < e; j++) {
+    a.reg[a.uatom[j]] = r;
+    a.sreg[a.uatom[j]] = sr;
+  }
 }
 
 // One half of langevin_region.F90: FIRST draws the random accelerations of
@@ -132,19 +145,26 @@ __global__ __launch_bounds__(256) void k_lt_half(LtArgs a, double *__restrict__ 
   if (FIRST && live) {
     for (int d = 0; d < 3; d++) a.frand[3 * i + d] = fr[d];
   }
-  if (live && r >= 0) {
-    if (!FIRST)
+  // the thermostat acts on v minus the streaming velocity, which every atom
+  // loses and regains (langevin_region.F90:81-82, 190)
+  const int sr = live ? a.sreg[i] : -1;
+  if (live && (r >= 0 || sr >= 0)) {
+    double vs[3] = {0.0, 0.0, 0.0};
+    if (sr >= 0)
+      for (int d = 0; d < 3; d++) vs[d] = a.sv[d][sr];
+    if (!FIRST && r >= 0)
       for (int d = 0; d < 3; d++) fr[d] = a.frand[3 * i + d];
-    const double ef = a.efac[r];
+    const double ef = r >= 0 ? a.efac[r] : 1.0;
     double v2o = 0.0, v2n = 0.0;
     for (int d = 0; d < 3; d++) {
-      const double v0 = vel[3 * i + d];
-      const double v = FIRST ? (v0 + a.h * fr[d]) * ef : v0 * ef + a.h * fr[d];
-      vel[3 * i + d] = v;
+      const double v0 = vel[3 * i + d] - vs[d];
+      double v = v0;
+      if (r >= 0) v = FIRST ? (v0 + a.h * fr[d]) * ef : v0 * ef + a.h * fr[d];
+      vel[3 * i + d] = v + vs[d];
       v2o = v2o + v0 * v0;
       v2n = v2n + v * v;
     }
-    de = 0.5 * a.mass[i] * (v2n - v2o) / PGK_MD_CONV;
+    if (r >= 0) de = 0.5 * a.mass[i] * (v2n - v2o) / PGK_MD_CONV;
   }
   __shared__ double sred[4][PGK_LT_MAXREG];
   for (int k = 0; k < a.nreg; k++) {
@@ -184,6 +204,13 @@ LtArgs lt_args(pgk_ctx *c) {
     a.efac[k] = exp(-0.5 * dt * c->md_lt_damp[k]);
   }
   a.h = 0.5 * dt;
+  a.nstrm = c->md_lt_nstrm;
+  for (int k = 0; k < PGK_LT_MAXREG; k++) {
+    a.sz[0][k] = c->md_lt_sz[0][k];
+    a.sz[1][k] = c->md_lt_sz[1][k];
+    for (int d = 0; d < 3; d++) a.sv[d][k] = c->md_lt_sv[d][k];
+  }
+  a.sreg = c->d_lt_sreg;
   a.seed = c->md_lt_seed;
   a.ctr0 = 4ull * (uint64_t)c->natom * c->md_lt_draw;
   a.ustart = c->d_lt_ustart;

@@ -846,7 +846,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_md_small,
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
-                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_vgrid,
+                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_vgrid,
                   c->d_vir6};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1990,6 +1990,11 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
     pgk_set_error("pgk_md_langevin: need 1 <= nregion <= PGK_LT_MAXREG and every region array");
     return PGK_EINVAL;
   }
+  const int ns = (lt->strm_z1 && lt->strm_z2 && lt->strm_vel) ? lt->nstream : 0;
+  if (ns < 0 || ns > PGK_LT_MAXREG) {
+    pgk_set_error("pgk_md_langevin: need 0 <= nstream <= PGK_LT_MAXREG");
+    return PGK_EINVAL;
+  }
   for (int k = 0; k < nr; k++)
     if (!(lt->temperature[k] >= 0) || !(lt->damp[k] >= 0)) {
       pgk_set_error("pgk_md_langevin: temperatures and damping factors must be >= 0");
@@ -2022,6 +2027,7 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
   if ((rc = upload(&c->d_lt_ustart, ustart)) || (rc = upload(&c->d_lt_uatom, uatom)) ||
       (rc = upload(&c->d_lt_ucom, ucom)) || (rc = upload(&c->d_lt_mass, c->md_mass)) ||
       (rc = upload(&c->d_lt_frand, zero)) || (rc = upload(&c->d_lt_reg, none)) ||
+      (rc = upload(&c->d_lt_sreg, none)) ||
       (rc = upload(&c->d_lt_det, std::vector<double>(
                                      (size_t)pgk_grid_blocks(n, 256) * PGK_LT_MAXREG, 0.0))) ||
       (rc = upload(&c->d_lt_cnt, std::vector<int32_t>(PGK_LT_MAXREG, 0))))
@@ -2037,6 +2043,13 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
     c->md_lt_temp[k] = on ? lt->temperature[k] : 0.0;
     c->md_lt_damp[k] = on ? lt->damp[k] : 0.0;
   }
+  for (int k = 0; k < PGK_LT_MAXREG; k++) {
+    const bool on = k < ns;
+    c->md_lt_sz[0][k] = on ? lt->strm_z1[k] : 0.0;
+    c->md_lt_sz[1][k] = on ? lt->strm_z2[k] : 0.0;
+    for (int d = 0; d < 3; d++) c->md_lt_sv[d][k] = on ? lt->strm_vel[3 * k + d] : 0.0;
+  }
+  c->md_lt_nstrm = ns;
   c->md_lt_nunit = (int)ucom.size();
   c->md_lt_seed = lt->seed;
   c->md_lt_draw = 0;

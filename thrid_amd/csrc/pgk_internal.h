@@ -272,6 +272,9 @@ struct pgk_ctx {
   double md_lt_lo[3][PGK_LT_MAXREG] = {}, md_lt_hi[3][PGK_LT_MAXREG] = {};
   double md_lt_temp[PGK_LT_MAXREG] = {}, md_lt_damp[PGK_LT_MAXREG] = {};
   uint64_t md_lt_seed = 0, md_lt_draw = 0;   // draw: steps run since set-up
+  int md_lt_nstrm = 0;             // streaming-velocity slabs
+  double md_lt_sz[2][PGK_LT_MAXREG] = {}, md_lt_sv[3][PGK_LT_MAXREG] = {};
+  int32_t *d_lt_sreg = nullptr;    // streaming slab of each atom (-1 none)
   int32_t *d_lt_ustart = nullptr, *d_lt_uatom = nullptr, *d_lt_ucom = nullptr;
   int32_t *d_lt_reg = nullptr, *d_lt_cnt = nullptr;   // region of each atom (-1 none)
   double *d_lt_mass = nullptr, *d_lt_frand = nullptr;

@@ -380,13 +380,15 @@ class ForceContext:
                "pgk_md_heatf_series")
         return out[:n.value]
 
-    def md_langevin(self, regions, temperature=None, damp=None, mol_com=None, seed=0):
+    def md_langevin(self, regions, temperature=None, damp=None, mol_com=None, seed=0,
+                    stream=None):
         """Region Langevin thermostat for md_run / md_run_batch
         (langevin_regist.F90 + langevin_region.F90): regions [nregion, 6] of
         (x1, x2, y1, y2, z1, z2) in A, temperature [nregion] in K, damp
         [nregion] in 1/fs, mol_com [nmol] 1 where a molecule is registered
-        by its centre of mass (None: every molecule); regions=None switches
-        it off."""
+        by its centre of mass (None: every molecule); stream = (z1, z2,
+        vel [nslab, 3] in A/fs) streaming-velocity slabs (ass_strmvel) or
+        None; regions=None switches it off."""
         if regions is None:
             _check(self.L.pgk_md_langevin(self.h, None), "pgk_md_langevin")
             self._lt_nreg = 0
@@ -399,9 +401,18 @@ class ForceContext:
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperature, np.float64), (nr,)))
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(damp, np.float64), (nr,)))
         mc = None if mol_com is None else np.ascontiguousarray(mol_com, np.int32)
-        self._lt_keep = (cols, t, g, mc)
+        sz1 = sz2 = sv = None
+        if stream is not None:
+            sz1 = np.ascontiguousarray(stream[0], np.float64)
+            sz2 = np.ascontiguousarray(stream[1], np.float64)
+            sv = np.ascontiguousarray(stream[2], np.float64).reshape(-1)
+            if not (len(sz1) == len(sz2) and sv.size == 3 * len(sz1)):
+                raise ValueError("stream needs z1, z2 [nslab] and vel [nslab, 3]")
+        self._lt_keep = (cols, t, g, mc, sz1, sz2, sv)
         d = abi.LangevinDesc(nr, *[abi.dptr(a) for a in cols], abi.dptr(t), abi.dptr(g),
-                             abi.iptr(mc), int(seed) & 0xFFFFFFFFFFFFFFFF)
+                             abi.iptr(mc), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                             0 if sz1 is None else len(sz1), abi.dptr(sz1), abi.dptr(sz2),
+                             abi.dptr(sv))
         _check(self.L.pgk_md_langevin(self.h, C.byref(d)), "pgk_md_langevin")
         self._lt_nreg = nr
 
