@@ -422,7 +422,8 @@ int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
  * with variance 2 damp kT / (m dt) per component and gets
  * v = (v + dt/2 a) exp(-dt damp/2); at the end of the step, after RATTLE,
  * v = v exp(-dt damp/2) + dt/2 a with the same a (zero for atoms that joined
- * a region during the step) (langevin_region.F90:8-203).  The uniform numbers come from SplitMix64 at counter positions
+ * a region during the step) (langevin_region.F90:8-203).  The uniform
+ * numbers come from SplitMix64 at counter positions
  * seed + (4 (draw natom + i) + k + 1) 0x9E3779B97F4A7C15, mapped to
  * ((z >> 11) + 1) 2^-53 in (0, 1], in place of the reference's SFMT stream;
  * draw counts the steps run since pgk_md_langevin.  Not combined with the
