@@ -259,6 +259,31 @@ def mkexcl(sysd):
     return idx, lst[:nl]
 
 
+def mkexcl_topology(topo):
+    """Oracle mkexcl from a bare topology: natom and the 1-based bond_ij,
+    angl_ijk, tors_ijkl and torsrb_ijkl index arrays (the only inputs
+    mkexcl.F90 reads)."""
+    d = abi.SystemDesc()
+    n = int(topo["natom"])
+    d.natom = n
+    keep = []
+
+    def ia(key, width):
+        a = np.ascontiguousarray(np.asarray(topo[key], np.int32).reshape(-1, width))
+        keep.append(a)
+        return len(a), abi.iptr(a.reshape(-1))
+
+    d.nbond, d.bond_ij = ia("bond_ij", 2)
+    d.nangl, d.angl_ijk = ia("angl_ijk", 3)
+    d.ntors, d.tors_ijkl = ia("tors_ijkl", 4)
+    d.ntorsrb, d.torsrb_ijkl = ia("torsrb_ijkl", 4)
+    cap = 64 * n + 16
+    idx = np.zeros(n + 1, np.int32)
+    lst = np.zeros(cap, np.int32)
+    nl = _oracle_lib().pgko_mkexcl(C.byref(d), idx.ctypes.data, lst.ctypes.data, cap)
+    return idx, lst[:nl]
+
+
 # ---------------------------------------------------------------------------
 # the compiled reference
 # ---------------------------------------------------------------------------

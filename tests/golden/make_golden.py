@@ -38,13 +38,36 @@ def tiny_hf_slab():
     return s
 
 
+def write_mkexcl():
+    """mkexcl.npz: per case the topology mkexcl.F90 reads (natom, bond, angle,
+    torsion and RB-torsion index arrays) and the reference's own exclusion CSR."""
+    from test_mkexcl import CASES
+    out = {"names": np.array(list(CASES))}
+    for name, make in CASES.items():
+        s = make()
+        r = pyoracle.Reference(s, use_mkexcl=True)
+        idx, lst = r.excl()
+        out[f"{name}_natom"] = np.array(int(s["natom"]))
+        for k in ("bond_ij", "angl_ijk", "tors_ijkl", "torsrb_ijkl"):
+            out[f"{name}_{k}"] = np.asarray(s[k], np.int32)
+        out[f"{name}_index"] = idx
+        out[f"{name}_list"] = lst
+        print("mkexcl", name, s["natom"], len(lst))
+    np.savez_compressed(os.path.join(HERE, "mkexcl.npz"), **out)
+
+
 def main():
     if not pyoracle.reference_available():
         raise SystemExit("build the reference first: oracle/build_ref.sh")
+    if sys.argv[1:] == ["mkexcl"]:
+        write_mkexcl()
+        return
     cases = {"water648": (water648(), 0), "slab_rb": (small_slab("rb"), 1),
              "slab_periodic": (small_slab("periodic"), 1)}
     for name, (s, mode) in cases.items():
-        r = pyoracle.Reference(s)
+        # the reference builds its own exclusion list (mkexcl.F90)
+        r = pyoracle.Reference(s, use_mkexcl=True)
+        assert all(np.array_equal(a, b) for a, b in zip(r.excl(), systems.exclusions(s)))
         idx, lst = r.mklist(s["rcut_book"], mode)
         out = r.calforce(abi.MTS_LONG)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **system_arrays(s),
