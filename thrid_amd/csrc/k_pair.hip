@@ -18,6 +18,8 @@
 // cluster-pair mask bit is exactly the lane's predicate.  j-forces are summed
 // over the 8 i-lanes with cross-lane shuffles and added once per j-atom; the
 // i-forces stay in registers over the whole j list.
+#include <type_traits>
+
 #include "pgk_device.h"
 
 template <int PREC>
@@ -153,10 +155,16 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
 // bit `lane` of a wave-uniform 64-bit mask, used directly as the lane predicate
 __device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
-template <int PREC, bool SPLIT, bool LJLDS, int NW>
+// ENE: energies and virial tensors (a call whose energies nobody reads, such
+// as an MD step that samples nothing, skips them and writes a zero record).
+template <int PREC, bool SPLIT, bool LJLDS, int NW, bool ENE>
 __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
+  // per-lane i-force partials: FP64 in the FP64 form; FP32 otherwise (each
+  // lane adds its own ~40 pair forces in a fixed order, then the 8 partials
+  // of an atom are summed in FP64)
+  using IA = typename std::conditional<PREC == PGK_PREC_FP64, double, float>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   R *s_spl = reinterpret_cast<R *>(smem);
   R *s_lj = s_spl + 8 * a.nspl_lds;
@@ -177,10 +185,10 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
     // lane = ii + 8 jq + 32 hs: i-slot ii against j-slot jq of the pass's
     // first (hs = 0) or second (hs = 1) half-cluster entry
     const int ii = lane & 7, jj = lane >> 3, jq = jj & 3, hs = lane >> 5;
-    const R4 xi = a.xq[ci * PGK_CL + ii];
-    const int ti = a.ct[ci * PGK_CL + ii];   // 0-based type (0 for padding)
+    const R4 xi = a.xq[(unsigned)(ci * PGK_CL + ii)];
+    const int ti = a.ct[(unsigned)(ci * PGK_CL + ii)];   // 0-based type (0 for padding)
     const int nlast = a.nspl_lds - 1;
-    double fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
+    IA fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
     const int e0 = uni32(a.ci_start[ci]), e1 = uni32(a.ci_start[ci + 1]);
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       const R se = t[4] + d * (t[5] + d * (t[6] + d * t[7]));
       const R qq = coul ? xi.w * xj.w : R(0);
       const R ftc = sf * qq;
-      e_elc += se * qq;
+      if (ENE) e_elc += se * qq;
       // Lennard-Jones (eps is 0 unless the type pair interacts by VDW)
       const bool ljon = in && (r2 <= a.sqcut2);
       const int tp = ti + __mul24(a.ntype, tj);
@@ -218,23 +226,25 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       const R t2 = rinv2 * rm2;
       const R t6 = t2 * t2 * t2;
       const R t12 = t6 * t6;
-      e_vdw += eps * (t12 - R(2) * t6);
       const R ftl = R(12) * eps * (t12 - t6) * rinv2;
-      const R xx = rx * rx, yy = ry * ry, zz = rz * rz;
-      const R xy = rx * ry, xz = rx * rz, yz = ry * rz;
-      tc[0] += ftc * xx; tc[1] += ftc * yy; tc[2] += ftc * zz;
-      tc[3] += ftc * xy; tc[4] += ftc * xz; tc[5] += ftc * yz;
-      tl[0] += ftl * xx; tl[1] += ftl * yy; tl[2] += ftl * zz;
-      tl[3] += ftl * xy; tl[4] += ftl * xz; tl[5] += ftl * yz;
+      if (ENE) {
+        e_vdw += eps * (t12 - R(2) * t6);
+        const R xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        const R xy = rx * ry, xz = rx * rz, yz = ry * rz;
+        tc[0] += ftc * xx; tc[1] += ftc * yy; tc[2] += ftc * zz;
+        tc[3] += ftc * xy; tc[4] += ftc * xz; tc[5] += ftc * yz;
+        tl[0] += ftl * xx; tl[1] += ftl * yy; tl[2] += ftl * zz;
+        tl[3] += ftl * xy; tl[4] += ftl * xz; tl[5] += ftl * yz;
+      }
       const R ft = ftc + ftl;
       const R fx = ft * rx, fy = ft * ry, fz = ft * rz;
-      fix += (double)fx;
-      fiy += (double)fy;
-      fiz += (double)fz;
+      fix += (IA)fx;
+      fiy += (IA)fy;
+      fiz += (IA)fz;
       if (SPLIT) {
-        cix += (double)(ftc * rx);
-        ciy += (double)(ftc * ry);
-        ciz += (double)(ftc * rz);
+        cix += (IA)(ftc * rx);
+        ciy += (IA)(ftc * ry);
+        ciz += (IA)(ftc * rz);
       }
       // j side: x, y, z of each j-slot summed over its 8 i-lanes; lanes
       // ii = 0, 2, 4 add -sum to the x, y, z accumulator of the j atom
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       mb |= mb >> 2;
       mb |= mb >> 1;
       if (lane_bit((mb & 0x0101010101010101ull) * 0x15ull)) {
-        const size_t s = ((size_t)hc * 4 + jq) * 3 + (ii >> 1);
+        const unsigned s = ((unsigned)hc * 4u + (unsigned)jq) * 3u + (unsigned)(ii >> 1);
         acc_add_j(&a.fcl[s], -g);
         if (SPLIT) acc_add_j(&a.fcl_c[s], -gc);
       }
@@ -280,7 +290,7 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
             (k + 1 < nin) ? (uint32_t)__builtin_amdgcn_readlane((int)c_m, kk + 1) : 0u;
         d.m = ((uint64_t)mb << 32) | ma;
         d.hc = hs ? hb : ha;
-        const int slot = d.hc * 4 + jq;
+        const unsigned slot = (unsigned)d.hc * 4u + (unsigned)jq;
         d.xj = a.xq[slot];
         d.tj = a.ct[slot];
       };
@@ -302,36 +312,42 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
       n_hc = (cb + 128 + lane < nent) ? a.cj[e0 + cb + 128 + lane] : 0;
       n_m = (cb + 128 + lane < nent) ? a.mask[e0 + cb + 128 + lane] : 0u;
     }
-    // i side: sum over the 8 j-lanes that share this i-slot
+    // i side: sum over the 8 j-lanes that share this i-slot (FP64)
+    double gx = fix, gy = fiy, gz = fiz, hx = cix, hy = ciy, hz = ciz;
 #pragma unroll
     for (int o = 8; o < 64; o <<= 1) {
-      fix += __shfl_xor(fix, o, 64);
-      fiy += __shfl_xor(fiy, o, 64);
-      fiz += __shfl_xor(fiz, o, 64);
+      gx += __shfl_xor(gx, o, 64);
+      gy += __shfl_xor(gy, o, 64);
+      gz += __shfl_xor(gz, o, 64);
       if (SPLIT) {
-        cix += __shfl_xor(cix, o, 64);
-        ciy += __shfl_xor(ciy, o, 64);
-        ciz += __shfl_xor(ciz, o, 64);
+        hx += __shfl_xor(hx, o, 64);
+        hy += __shfl_xor(hy, o, 64);
+        hz += __shfl_xor(hz, o, 64);
       }
     }
     if (jj < 3) {   // lanes jj = 0, 1, 2 hold x, y, z of slot ii
-      const size_t s = (size_t)(ci * PGK_CL + ii) * 3;
-      const double v = (jj == 0) ? fix : ((jj == 1) ? fiy : fiz);
+      const unsigned s = (unsigned)(ci * PGK_CL + ii) * 3u;
+      const double v = (jj == 0) ? gx : ((jj == 1) ? gy : gz);
       acc_add(&a.fcl[s + jj], v);
       if (SPLIT) {
-        const double w = (jj == 0) ? cix : ((jj == 1) ? ciy : ciz);
+        const double w = (jj == 0) ? hx : ((jj == 1) ? hy : hz);
         acc_add(&a.fcl_c[s + jj], w);
       }
     }
-    ev[0] = e_elc;
-    ev[1] = e_vdw;
+    if (ENE) {
+      ev[0] = e_elc;
+      ev[1] = e_vdw;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      ev[2 + k] = tc[k];
-      ev[8 + k] = tl[k];
+      for (int k = 0; k < 6; k++) {
+        ev[2 + k] = tc[k];
+        ev[8 + k] = tl[k];
+      }
     }
   }
-  block_record<14, NW>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (ENE)
+    block_record<14, NW>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  else if (threadIdx.x < 14)
+    a.red[(size_t)blockIdx.x * PGK_REC + threadIdx.x] = 0.0;
 }
 
 // ---- force combination + excluded pairs ------------------------------------
@@ -419,7 +435,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs a, A *__restrict__ 
 }
 
 template <int PREC>
-static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
+static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
   using A = typename Prec<PREC>::acc;
@@ -464,28 +480,30 @@ static int launch_pairs(pgk_ctx *c, bool split, int *off, int *nrec) {
   *nrec = nblk;
   a.red = c->d_red + (size_t)c->red_used * PGK_REC;
   c->red_used += nblk;
-#define PGK_LAUNCH_PAIRS(SP, LL)                                                        \
+#define PGK_LAUNCH_PAIRS(SP, LL, EN)                                                    \
   {                                                                                      \
-    auto kf = k_pairs<PREC, SP, LL, NW>;                                                 \
+    auto kf = k_pairs<PREC, SP, LL, NW, EN>;                                             \
     HIP_CHECK(hipFuncSetAttribute((const void *)kf,                                      \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     kf<<<nblk, 64 * NW, lds, c->stream>>>(a);                                            \
   }
-  if (split) {
-    if (a.lj_lds) PGK_LAUNCH_PAIRS(true, true) else PGK_LAUNCH_PAIRS(true, false)
+  if (split) {   // the per-class forces come with their energies
+    if (a.lj_lds) PGK_LAUNCH_PAIRS(true, true, true) else PGK_LAUNCH_PAIRS(true, false, true)
+  } else if (ene) {
+    if (a.lj_lds) PGK_LAUNCH_PAIRS(false, true, true) else PGK_LAUNCH_PAIRS(false, false, true)
   } else {
-    if (a.lj_lds) PGK_LAUNCH_PAIRS(false, true) else PGK_LAUNCH_PAIRS(false, false)
+    if (a.lj_lds) PGK_LAUNCH_PAIRS(false, true, false) else PGK_LAUNCH_PAIRS(false, false, false)
   }
 #undef PGK_LAUNCH_PAIRS
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
 
-int pgk_pairs_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+int pgk_pairs_launch(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   switch (c->precision) {
-    case PGK_PREC_FP64: return launch_pairs<PGK_PREC_FP64>(c, split, off, nrec);
-    case PGK_PREC_MIXED: return launch_pairs<PGK_PREC_MIXED>(c, split, off, nrec);
-    default: return launch_pairs<PGK_PREC_FP32>(c, split, off, nrec);
+    case PGK_PREC_FP64: return launch_pairs<PGK_PREC_FP64>(c, split, ene, off, nrec);
+    case PGK_PREC_MIXED: return launch_pairs<PGK_PREC_MIXED>(c, split, ene, off, nrec);
+    default: return launch_pairs<PGK_PREC_FP32>(c, split, ene, off, nrec);
   }
 }
 

@@ -308,7 +308,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipGetLastError());
     }
     pgk_prof_begin(c, PGK_PROF_PAIRS);
-    if ((rc = pgk_pairs_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+    if ((rc = pgk_pairs_launch(c, split, energies, &seg_off[nseg], &seg_n[nseg]))) return rc;
     pgk_prof_end(c, PGK_PROF_PAIRS);
     seg_kind[nseg++] = SEG_PAIR;
   }

@@ -472,29 +472,75 @@ def mixture_box(n_target, composition, *, seed=4, density_gcc=0.997, rcut=10.0,
     vol = mass / 6.0221367e23 / density_gcc * 1e24   # A^3
     box = vol ** (1.0 / 3.0)
     box = max(box, 2.0 * rcut_book + 0.5)
-    nsite = int(math.ceil(nmol ** (1.0 / 3.0)))
+    nbig = sum(1 for k in kinds if k != "water")
+    nwat = nmol - nbig
+    nsite = int(math.ceil(max(nbig, 1) ** (1.0 / 3.0)))
     sites = _lattice_sites(nsite, nsite, nsite, (0, 0, 0), (box, box, box), rng, 0.2)
-    sites = sites[rng.permutation(len(sites))][:nmol]
+    sites = sites[rng.permutation(len(sites))][:nbig]
     b = _Builder()
     bx = np.array([box, box, box])
     # larger molecules first (orientation search against what is placed), then
-    # waters, dropping any water that would sit on top of them
-    big = [(k, c) for k, c in zip(kinds, sites) if k != "water"]
-    small = [c for k, c in zip(kinds, sites) if k == "water"]
-    for k, c in big:
+    # waters by random sequential addition on a fine jittered grid, so the
+    # replica keeps its drawn composition and size
+    big = [k for k in kinds if k != "water"]
+    for k, c in zip(big, sites):
         b.add_checked(tmpls[k], c, rng, bx, dmin=2.5, tries=400)
     heavy = np.concatenate(b.pos, axis=0) if b.pos else np.zeros((0, 3))
-    for c in small:
-        if heavy.shape[0]:
-            d = heavy - c
-            d -= bx * np.round(d / bx)
-            if np.min((d ** 2).sum(-1)) < 2.6 ** 2:
-                continue
+    if nbig == 0:   # pure water: a jittered lattice holds every molecule
+        m = int(math.ceil(nwat ** (1.0 / 3.0)))
+        wsites = _lattice_sites(m, m, m, (0, 0, 0), (box, box, box), rng, 0.2)
+        wsites = wsites[rng.permutation(len(wsites))][:nwat]
+    else:
+        wsites = _fill_sites(nwat, box, heavy, rng)
+    for c in wsites:
         b.add(tmpls["water"], c, _random_rotation(rng))
     g = fft_friendly(int(math.ceil(box / spacing)))
     return _finish(b, (box, box, box), alpha=alpha, rrcut=rcut, rcut=rcut,
                    rcut_book=rcut_book, nfft=(g, g, g), order=order,
                    torsion_form=torsion_form, name=name, seed=seed)
+
+
+def _fill_sites(n, box, solute, rng, d_solute=2.1, d_water=2.3, grid=0.9):
+    """Up to `n` centres in the periodic cube [0, box)^3 picked in random order
+    from a jittered grid of spacing ~`grid`, each at least `d_solute` from every
+    solute atom and `d_water` from every centre already taken (a spatial hash
+    of cell size >= both distances keeps each test local)."""
+    m = max(1, int(box / grid))
+    cand = _lattice_sites(m, m, m, (0, 0, 0), (box, box, box), rng, 0.3 * box / m)
+    cand = np.mod(cand[rng.permutation(len(cand))], box)
+    nc = max(1, int(box / max(d_solute, d_water)))
+    cw = box / nc
+    cells = {}
+
+    def key(p):
+        return tuple((np.floor(p / cw).astype(int)) % nc)
+
+    def near(p, dmin, kind):
+        k = key(p)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q, t in cells.get(((k[0] + dx) % nc, (k[1] + dy) % nc,
+                                           (k[2] + dz) % nc), ()):
+                        d = q - p
+                        d -= box * np.round(d / box)
+                        lim = d_solute if t else d_water
+                        if d @ d < lim * lim:
+                            return True
+        return False
+
+    for q in solute:
+        q = np.mod(q, box)
+        cells.setdefault(key(q), []).append((q, True))
+    out = []
+    for p in cand:
+        if len(out) == n:
+            break
+        if near(p, d_water, False):
+            continue
+        cells.setdefault(key(p), []).append((p, False))
+        out.append(p)
+    return out
 
 
 def fft_friendly(n: int) -> int:
@@ -515,12 +561,13 @@ def _replica(args):
     return mixture_box(n, kind, seed=rseed, alcohol_frac=frac, name=name)
 
 
-def replica_ensemble(nrep=64, nmin=1500, nmax=12000, seed=4, workers=1):
+def replica_ensemble(nrep=64, nmin=1500, nmax=12000, seed=4, workers=1, mp_context="fork"):
     """Uneven Green-Kubo replicas: sizes log-uniform in [nmin, nmax], each a
     water, n-butane or water/propanol (5-20 mol%) box with its own PME grid.
     The replicas' parameters are drawn in order from one generator, so
     `workers` > 1 (forked processes building replicas side by side) gives the
-    same replicas; fork before anything touches the GPU."""
+    same replicas.  Use `mp_context` "fork" only before anything touches the
+    GPU; "spawn" starts fresh interpreters and is safe afterwards."""
     rng = np.random.default_rng(seed)
     jobs = []
     for r in range(nrep):
@@ -533,7 +580,7 @@ def replica_ensemble(nrep=64, nmin=1500, nmax=12000, seed=4, workers=1):
     import multiprocessing as mp
     # largest first, so the pool's tail is short
     order = sorted(range(nrep), key=lambda r: -jobs[r][0])
-    with mp.get_context("fork").Pool(workers) as pool:
+    with mp.get_context(mp_context).Pool(workers) as pool:
         built = pool.map(_replica, [jobs[r] for r in order], chunksize=1)
     reps = [None] * nrep
     for r, sysd in zip(order, built):
