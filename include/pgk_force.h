@@ -351,6 +351,15 @@ int pgk_ewaldk_hf(pgk_ctx *ctx, int32_t kmax, double yratio, double zratio,
  * (fft_pme.F90:872-937 with B-splines in place of their derivatives), so
  * sum_i e_i = pot_ewk.  e_atom [natom], eref; the self term is not included. */
 int pgk_recip_energy_atom(pgk_ctx *ctx, double *e_atom);
+/* pgk_recip_energy_atom without any device allocation (no reference
+ * counterpart; for callers that cannot allocate: code under stream capture,
+ * tight sampling loops, hosts that forbid allocation after set-up).  Same
+ * arguments, result and errors; the per-atom energies are staged in a fixed
+ * device array of PGK_NOALLOC_MAX_ATOMS = 1,048,576 doubles (8 MiB per
+ * device, held until the process ends), so natom above that returns
+ * PGK_EINVAL.  All calls on one device share that array: calls that overlap
+ * on one device (other host threads or streams) race.                    */
+int pgk_recip_energy_atom_noalloc(pgk_ctx *ctx, double *e_atom);
 /* Per-atom reciprocal (SPME) virial tensors of the same evaluation (no
  * reference counterpart: the SPME form of ewaldkp_hf's virielct_atm):
  * W_i^ab = q_i phi_ab(r_i) / 2, with phi_ab the charge grid convolved with

@@ -525,9 +525,15 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
       pgk_set_error("pgk_create: atom type out of range");
       return PGK_EINVAL;
     }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {   // the allocation-free entry points' scratch address on this device
+    const int rc0 = pgk_noalloc_init_device(dev);
+    if (rc0) return rc0;
+  }
   pgk_ctx *c = new pgk_ctx();
   c->precision = precision;
-  (void)hipGetDevice(&c->device);
+  c->device = dev;
   c->natom = n;
   c->ntype = nt;
   c->charge = copyv(d->charge, n);
@@ -847,7 +853,10 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
                   c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_vgrid,
-                  c->d_vir6};
+                  c->d_vir6, c->d_hj_vel, c->d_hj_mass, c->d_hj_molevel,
+                  c->d_hj_part, c->d_hj_out, c->d_hj_mol_start, c->d_hj_mol_atoms,
+                  c->d_hj_jcol, c->d_hj_jpart, c->d_md_hfseries, c->d_md_hfmass,
+                  c->d_hj_fscratch, c->d_md_fshort};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sum) (void)hipHostFree(c->h_sum);

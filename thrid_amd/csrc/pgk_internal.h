@@ -329,6 +329,9 @@ int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_export(pgk_ctx *c);
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
 int pgk_pme_atom_energy(pgk_ctx *c, double *d_e);
+// atoms the allocation-free entry points' fixed scratch holds (8 MiB of doubles)
+#define PGK_NOALLOC_MAX_ATOMS (1 << 20)
+int pgk_noalloc_init_device(int dev);     // once per device, that device current
 int pgk_pme_atom_virial(pgk_ctx *c, double *d_w);   // (3,3,natom) into d_w
 // grow a device buffer to at least `need` bytes (contents not kept)
 extern "C" int grow_dev(void **p, size_t *cap, size_t need);

@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_nhc.argtypes = [vp, C.POINTER(abi.NhcDesc)]
     L.pgk_md_respa.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts)]
     L.pgk_recip_energy_atom.argtypes = [vp, vp]
+    L.pgk_recip_energy_atom_noalloc.argtypes = [vp, vp]
     L.pgk_recip_virial_atom.argtypes = [vp, vp]
     L.pgk_md_heatf.argtypes = [vp, C.POINTER(abi.HfDesc), C.c_int32]
     L.pgk_md_heatf_series.argtypes = [vp, vp, C.c_int32, vp]
@@ -82,7 +83,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
               "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
-              "pgk_md_respa", "pgk_recip_energy_atom", "pgk_recip_virial_atom", "pgk_md_langevin",
+              "pgk_md_respa", "pgk_recip_energy_atom", "pgk_recip_energy_atom_noalloc", "pgk_recip_virial_atom", "pgk_md_langevin",
               "pgk_md_langevin_state"):
         getattr(L, f).restype = C.c_int
     if L.pgk_abi_version() != abi.ABI_VERSION:
@@ -343,6 +344,14 @@ class ForceContext:
         evaluation at the current positions (sum = pot_ewk)."""
         e = np.zeros(self.n)
         _check(self.L.pgk_recip_energy_atom(self.h, e.ctypes.data), "pgk_recip_energy_atom")
+        return e
+
+    def recip_energy_atom_noalloc(self):
+        """recip_energy_atom through the entry point that allocates nothing
+        (pgk_recip_energy_atom_noalloc: fixed device scratch)."""
+        e = np.zeros(self.n)
+        _check(self.L.pgk_recip_energy_atom_noalloc(self.h, e.ctypes.data),
+               "pgk_recip_energy_atom_noalloc")
         return e
 
     def recip_virial_atom(self):
