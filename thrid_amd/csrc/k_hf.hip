@@ -340,7 +340,7 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.rrsq = c->rrsq_cut;
     const int nblk = pgk_grid_blocks(c->ncl, 4);
     if (!h.flux) {
-      k_hf_pairs<AH><<<nblk, 256, 0, st>>>(p, h);
+      return pgk_hf_w_launch(c);   // per-atom arrays: k_hf_w.hip
     } else if (h.mode == PGK_HF_BULK) {
       k_hf_pairs_flux<AH, true><<<nblk, 256, 0, st>>>(p, h);
     } else {

@@ -630,6 +630,7 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
   c->npair_cl = tot;
   c->npairs = -1;   // counted on request
   c->have_list = true;
+  c->list_serial++;
   for (int d = 0; d < 3; d++) c->list_box[d] = c->box[d];
   return PGK_OK;
 }

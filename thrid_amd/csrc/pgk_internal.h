@@ -215,6 +215,15 @@ struct pgk_ctx {
   bool hf_ready = false;           // a pgk_calforce_hf call left its arrays on the device
   bool hf_export = true;           // run_calforce writes the caller-layout HF arrays
   bool hf_pairs_only = false;      // run_calforce: pair heat-flux pass without pair forces
+  // W-array pair pass (k_hf_w.hip): per-slot FP64 positions, centres and
+  // molecule ids, and the transposed list (entries grouped by half-cluster)
+  double4 *d_hfw_xq = nullptr, *d_hfw_cen = nullptr;
+  int32_t *d_hfw_mol = nullptr;
+  size_t hfw_xq_bytes = 0, hfw_cen_bytes = 0, hfw_mol_bytes = 0;
+  int32_t *d_tl_start = nullptr, *d_tl_src = nullptr;
+  uint32_t *d_tl_mask = nullptr;
+  size_t tl_start_bytes = 0, tl_src_bytes = 0, tl_mask_bytes = 0;
+  int64_t list_serial = 0, tl_serial = -1;   // list builds; the build the transpose is of
   bool pme_grid_ready = false;     // d_pgrid holds the potential of the current positions
   // per-atom reciprocal virial (pgk_recip_virial_atom): one component's
   // potential grid at a time, and the six per-atom components
@@ -326,6 +335,9 @@ int pgk_pme_prep(pgk_ctx *c, bool pack);
 int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
                        int *seg_kind, int *nseg, bool hf);
 int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
+// regions the W-array pair pass holds per wave in LDS (volume / surface forms)
+#define PGK_HF_W_MAXREG 64
+int pgk_hf_w_launch(pgk_ctx *c);
 int pgk_hf_export(pgk_ctx *c);
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
 int pgk_pme_atom_energy(pgk_ctx *c, double *d_e);
