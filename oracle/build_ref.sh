@@ -82,5 +82,29 @@ printf '%s\n' "${SRCS[@]}" | xargs -P "$JOBS" -I{} bash -c \
 compile "$HERE/ref_harness.F90" "$OUT/obj/zz_ref_harness.o"
 
 "$FC" -shared -fPIC -o "$OUT/libpgkref.so" "$OUT"/obj/*.o
+
+# The same reference with the drop-in shim (thrid_amd/fortran/calforce_gpu.F90)
+# in place of calforce.o, calforce_hf.o, mklist.o and mklist_cell.o, linked to
+# the MI355X library: the link itself checks that the shim satisfies every
+# caller of those four routines.
+PKG="$HERE/../thrid_amd"
+if [ -f "$PKG/libpgkforce.so" ]; then
+  mkdir -p "$OUT/shim"
+  "$FC" -O2 -fPIC -cpp -DHF -module-dir "$OUT/shim" -I"$OUT/mod" \
+    -c "$PKG/fortran/pgk_force_mod.F90" -o "$OUT/shim/pgk_force_mod.o"
+  "$FC" -O2 -fPIC -cpp -DHF -module-dir "$OUT/shim" -I"$OUT/mod" -I"$OUT/shim" \
+    -c "$PKG/fortran/calforce_gpu.F90" -o "$OUT/shim/calforce_gpu.o"
+  OBJS=()
+  for o in "$OUT"/obj/*.o; do
+    case "$(basename "$o")" in
+      calforce.o|calforce_hf.o|mklist.o|mklist_cell.o) ;;
+      *) OBJS+=("$o");;
+    esac
+  done
+  "$FC" -shared -fPIC -o "$OUT/libpgkref_gpu.so" "${OBJS[@]}" "$OUT/shim/pgk_force_mod.o" \
+    "$OUT/shim/calforce_gpu.o" -L"$PKG" -lpgkforce -Wl,-rpath,'$ORIGIN/../../thrid_amd' \
+    -Wl,--no-undefined -Wl,--allow-shlib-undefined
+  echo "build_ref: built $OUT/libpgkref_gpu.so"
+fi
 rm -f "$OUT"/obj/*.log
 echo "build_ref: built $OUT/libpgkref.so"

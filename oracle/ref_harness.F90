@@ -26,7 +26,7 @@ module pgkref_harness
   use spme_global
   use mpi_global
   use interface_mdtech, only: calforce, calforce_hf, mklist2a, mklist_cell, &
-       &                      erf_corr_cutoff, calheatf
+       &                      erf_corr_cutoff, calheatf, rattle_c, rattlep_v, nhcint
   use interface_preproc, only: MPI_loopinit, malloc_larray, mkexcl, &
        &                       spline_interp_init, fft_pme_init, prepewk
   use interface_interact, only: ewaldkp_hf
@@ -533,5 +533,77 @@ contains
     wv(:,:,:) = w4(:,:,:,1)
     deallocate(fvc, w4, mc, ht)
   end subroutine pgkref_ewaldk_hf
+
+  ! ---- integrator pieces (the deterministic parts of the MD step) ----------
+  ! Constraints as mkconst leaves them (mkconst.F90:110-164): nconst pairs
+  ! (iconst, jconst, dconst), grouped per molecule by index_mconst /
+  ! list_mconst with ascending constraint order inside a group; masses and
+  ! velocities into atmmass / atmvel (streaming velocities zero).
+  subroutine pgkref_set_md(nc, ng, gstart, ic, jc, dc, mass, vel) &
+       &     bind(C, name='pgkref_set_md')
+    integer(c_int), value :: nc, ng
+    integer(c_int), intent(in) :: gstart(ng+1), ic(nc), jc(nc)
+    real(c_double), intent(in) :: dc(nc), mass(natom), vel(3,natom)
+    integer :: k
+    nconst = nc
+    iconst(1:nc) = ic(1:nc)
+    jconst(1:nc) = jc(1:nc)
+    dconst(1:nc) = dc(1:nc)
+    index_mconst(:) = 0
+    index_mconst(1:ng+1) = gstart(1:ng+1) + 1
+    do k = 1, nc
+       list_mconst(k) = k
+    end do
+    atmmass(1:natom) = mass(1:natom)
+    atmvel(1:3,1:natom) = vel(1:3,1:natom)
+    atmvel_strm(1:3,1:natom) = 0.0d0
+  end subroutine pgkref_set_md
+
+  subroutine pgkref_get_md(cor, vel) bind(C, name='pgkref_get_md')
+    real(c_double), intent(out) :: cor(3,natom), vel(3,natom)
+    cor(1:3,1:natom) = atmcor(1:3,1:natom)
+    vel(1:3,1:natom) = atmvel(1:3,1:natom)
+  end subroutine pgkref_get_md
+
+  ! rattle_c (rattle.F90:6-272): positions atmcor (already drifted) and the
+  ! half-step velocities against the constraints, from the positions at T
+  subroutine pgkref_rattle_c(cor_old, eps, dt) bind(C, name='pgkref_rattle_c')
+    real(c_double), intent(in) :: cor_old(3,natom)
+    real(c_double), value :: eps, dt
+    real(8), allocatable :: old(:,:)
+    integer :: one
+    allocate(old(3,natom))
+    old(1:3,1:natom) = cor_old(1:3,1:natom)
+    one = 1
+    call rattle_c(old, eps, dt, dt, one, one)
+    deallocate(old)
+  end subroutine pgkref_rattle_c
+
+  ! rattlep_v (rattle_mtk.F90:278-...): velocities against the constraints
+  subroutine pgkref_rattlep_v(eps, dt) bind(C, name='pgkref_rattlep_v')
+    real(c_double), value :: eps, dt
+    real(8) :: vc, vt(3,3)
+    vc = 0.0d0
+    vt = 0.0d0
+    call rattlep_v(eps, dt, dt, 1, 1, 1.0d0, vc, vt)
+  end subroutine pgkref_rattlep_v
+
+  ! nhcint (nhcint.F90:8-140) on the chain (qmass, xlogs, vlogs) with the
+  ! Yoshida-Suzuki weights w; kT = text (the reference's non-dimensional
+  ! temperature), the kinetic energy from atmmass / atmvel
+  subroutine pgkref_nhcint(mchain, nyosh, nexti, degfree, dt, text, w, q, x, v) &
+       &     bind(C, name='pgkref_nhcint')
+    integer(c_int), value :: mchain, nyosh, nexti, degfree
+    real(c_double), value :: dt, text
+    real(c_double), intent(in) :: w(nyosh), q(mchain)
+    real(c_double), intent(inout) :: x(mchain), v(mchain)
+    yscoeff(1:nyosh) = w(1:nyosh)
+    qmass(1:mchain) = q(1:mchain)
+    xlogs(1:mchain) = x(1:mchain)
+    vlogs(1:mchain) = v(1:mchain)
+    call nhcint(degfree, dt, mchain, text, nexti, nyosh)
+    x(1:mchain) = xlogs(1:mchain)
+    v(1:mchain) = vlogs(1:mchain)
+  end subroutine pgkref_nhcint
 
 end module pgkref_harness
