@@ -225,6 +225,55 @@ def run_extras(sysd, state, precision, steps=200):
     return out
 
 
+def run_interface(steps, precs):
+    """The ~1M-atom two-sided alcohol/water interface (300,000 waters + 8,000
+    1-propanols, 200 x 200 x 300 A, SPME 192 x 192 x 288 order 6, 12 A cutoff):
+    after a short FP64 quench of the generated start, `steps` MD steps per form
+    with Langevin thermostats (320 K / 280 K) on the 20 A at each end of the
+    slab, a Nose-Hoover chain (300 K) on the rest, and the 1 A binned heat flux
+    (300 volume regions, waters by molecule centre) sampled every step.
+    Returns ms per step and atom-steps/s per form."""
+    from thrid_amd import abi, md, systems
+    from thrid_amd.pgk import ForceContext
+    t0 = time.perf_counter()
+    s = systems.interface_slab()
+    cons = md.rigid_constraints(s)
+    c = ForceContext(s, "fp64")
+    s["pos"] = md.relax(c, s, cons, schedule=((0.02, 10), (0.1, 10), (0.5, 10)))
+    c.close()
+    t_setup = time.perf_counter() - t0
+    n = s["natom"]
+    lx, ly, lz = (float(b) for b in s["box"])
+    ij, d, gs = cons
+    ht = np.ones(n, np.int32)
+    ht[np.isin(s["atype"], [systems.TYPE_INDEX["OB"], systems.TYPE_INDEX["HB"]])] = abi.HFTYP_MOLE
+    z = np.arange(0.0, lz + 0.5, 1.0)
+    zmin, zmax = float(s["pos"][:, 2].min()), float(s["pos"][:, 2].max())
+    ends = np.array([[0.0, lx, 0.0, ly, zmin - 1.0, zmin + 20.0],
+                     [0.0, lx, 0.0, ly, zmax - 20.0, zmax + 1.0]])
+    out = {"atoms": n, "setup_s": t_setup, "steps": steps, "heat_flux_bins": len(z) - 1}
+    for prec in precs:
+        c = ForceContext(s, prec)
+        c.md_setup(s["mass"], ij, d, gs, DT_FS)
+        c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=55))
+        c.md_langevin(ends, [320.0, 280.0], [0.01, 0.01], seed=5)
+        c.md_nhc(300.0, 0.01, 3 * n - len(d) - 3)
+        c.md_heatf(dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:], hftyp=ht), 1)
+        c.md_run(2, NSTEP_BOOK, s["rcut_book"], 2)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.md_run(steps, NSTEP_BOOK, s["rcut_book"], steps)
+        c.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        rows = c.md_heatf_series()
+        out[prec] = {"ms_per_step": sec * 1e3, "atom_steps_per_s": n / sec,
+                     "ns_per_day": 86400.0 / sec * DT_FS * 1e-6,
+                     "flux_rows": int(rows.shape[0]),
+                     "flux_finite": bool(np.all(np.isfinite(rows)))}
+        c.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (scale for the HBM byte counters)."""
     import torch
@@ -257,6 +306,10 @@ def cpu_baseline(sysd, budget_s=20.0):
     from thrid_amd import abi
     kind = "reference" if pyoracle.reference_available() else "port"
     m = pyoracle.Reference(sysd) if kind == "reference" else pyoracle.Oracle(sysd)
+    timed = kind == "reference" and os.path.abspath(pyoracle.REF_SO) == \
+        os.path.abspath(pyoracle.REF_TIMED_SO)
+    if timed:
+        m.times(reset=True)
     t0 = time.perf_counter()
     m.mklist(sysd["rcut_book"], 1)
     t_list = time.perf_counter() - t0
@@ -269,10 +322,31 @@ def cpu_baseline(sysd, budget_s=20.0):
         force = r["force"]
     per_step = t_force / ncall + t_list / NSTEP_BOOK
     base = dict(value=sysd["natom"] / per_step, unit="atom-steps/s", cores=1, kind=kind,
+                cpu=cpu_model(),
                 sample=f"{ncall} calforce calls + 1 cell-list build (amortised over "
                        f"{NSTEP_BOOK} steps) of the same 100,001-atom box, serial, "
                        f"list {t_list:.2f} s, calforce {t_force / ncall:.2f} s/call")
+    if timed:
+        # the reference's own -DTIME_M split (time_method.F90:157-265), CPU
+        # seconds per calforce call (drct, rcpr, bond) and per list build (table)
+        t = m.times()
+        base["time_m_split_s"] = {k: (t[k] / ncall if k != "table_time" else t[k])
+                                  for k in ("drct_time", "rcpr_time", "bond_time", "table_time")}
     return base, force
+
+
+def cpu_model():
+    """The host CPU's model name and the logical CPUs this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({len(os.sched_getaffinity(0))} logical CPUs available, 1 used)"
 
 
 def cpu_leg_subprocess():
@@ -282,9 +356,13 @@ def cpu_leg_subprocess():
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         log = os.path.join(d, "cpu.log")
+        env = dict(os.environ)
+        timed = os.path.join(ROOT, "oracle", "_ref", "libpgkref_timed.so")
+        if os.path.exists(timed):   # the reference with its -DTIME_M timers
+            env["PGK_REF_SO"] = timed
         with open(log, "w") as lf:
             r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", d],
-                               stdout=lf, stderr=subprocess.STDOUT)
+                               stdout=lf, stderr=subprocess.STDOUT, env=env)
         if r.returncode != 0:
             sys.stderr.write(open(log).read()[-4000:])
             raise RuntimeError(f"CPU baseline leg failed ({r.returncode})")
@@ -362,7 +440,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--precision", choices=["all", "both", "mixed", "fp64"], default="all",
+                    help="all: fp64, mixed and fp32 (the three rungs of the precision "
+                         "ladder); both: fp64 and mixed")
     ap.add_argument("--drift-steps", type=int, default=3000,
                     help="NVE steps per form for the energy drift")
     ap.add_argument("--copy-calib", action="store_true",
@@ -373,6 +453,8 @@ def main():
     ap.add_argument("--ensemble-hf", type=int, default=1, metavar="K",
                     help="heat-flux sample every K ensemble steps (0: none)")
     ap.add_argument("--no-heatf", action="store_true", help="skip the heat-flux leg")
+    ap.add_argument("--interface", type=int, default=20, metavar="STEPS",
+                    help="MD steps of the ~1M-atom interface leg per form (0 skips it)")
     ap.add_argument("--extras", action="store_true",
                     help="add the region Langevin and per-atom virial cost leg")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
@@ -400,7 +482,8 @@ def main():
     n = sysd["natom"]
     state = equilibrated_state(sysd)
     res = {}
-    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    precs = {"all": ("fp64", "mixed", "fp32"), "both": ("fp64", "mixed")}.get(
+        args.precision, (args.precision,))
     for prec in precs:
         sec, prof, info, drift, _rows = run_md(sysd, state, prec, args.steps, args.warmup,
                                                args.drift_steps)
@@ -456,6 +539,10 @@ def main():
         out["fp64"]["roofline_pairs"]["frac"] = (out["fp64"]["roofline_pairs"]["achieved"]
                                                  / PEAK["fp64"])
         out["speedup_mixed_vs_fp64"] = f64["sec"] / res["mixed"]["sec"]
+    if "fp32" in res:
+        f32 = res["fp32"]
+        out["fp32"] = {"atom_steps_per_s": n / f32["sec"], "ms_per_step": f32["sec"] * 1e3,
+                       "ns_per_day": 86400.0 / f32["sec"] * DT_FS * 1e-6}
     if args.extras:
         out["extras"] = {p: run_extras(sysd, state, p) for p in precs}
         for p, e in out["extras"].items():
@@ -463,13 +550,21 @@ def main():
                   f"{e['langevin_ms_per_step']:.4f} ms; per-atom reciprocal virial "
                   f"{e['recip_virial_atom_ms']:.3f} ms", file=sys.stderr, flush=True)
     if not args.no_heatf:
-        out["heatf"] = {p: run_heatf(sysd, state, p) for p in precs}
+        out["heatf"] = {p: run_heatf(sysd, state, p) for p in precs if p != "fp32"}
         for p, h in out["heatf"].items():
             print(f"# heatf {p}: " + "; ".join(
                 f"{k}: calforce_hf {v['calforce_hf_ms']:.2f} ms + calheatf {v['calheatf_ms']:.3f} "
                 f"ms (with host arrays {v['calforce_hf_with_host_arrays_ms']:.2f} ms; fused "
                 f"{v['fused_calforce_heatf_ms']:.2f} ms)"
                 for k, v in h.items()), file=sys.stderr, flush=True)
+    if args.interface > 0:
+        iprecs = [p for p in precs if p != "fp32"]     # the interface runs FP64 and mixed
+        out["interface"] = run_interface(args.interface, iprecs)
+        for p in iprecs:
+            e = out["interface"][p]
+            print(f"# interface {p}: {out['interface']['atoms']} atoms, {e['ms_per_step']:.3f} "
+                  f"ms/step ({e['atom_steps_per_s']:.3e} atom-steps/s) with both thermostats and "
+                  f"300-bin heat flux every step", file=sys.stderr, flush=True)
     if reps:
         out["ensemble"] = run_ensemble(reps, args.ensemble_steps, 10, args.ensemble_hf)
         e = out["ensemble"]

@@ -83,6 +83,24 @@ compile "$HERE/ref_harness.F90" "$OUT/obj/zz_ref_harness.o"
 
 "$FC" -shared -fPIC -o "$OUT/libpgkref.so" "$OUT"/obj/*.o
 
+# The same reference with its own per-term timers on (-DTIME_M in calforce and
+# calforce_hf, as the reference's makefiles allow): the CPU baseline's split.
+mkdir -p "$OUT/timed"
+for f in calforce calforce_hf; do
+  "$FC" $FFLAGS -DTIME_M -c "$REF/$f.F90" -o "$OUT/timed/$f.o" 2> "$OUT/timed/$f.log" ||
+    { cat "$OUT/timed/$f.log" >&2; exit 1; }
+done
+TOBJS=()
+for o in "$OUT"/obj/*.o; do
+  case "$(basename "$o")" in
+    calforce.o|calforce_hf.o) ;;
+    *) TOBJS+=("$o");;
+  esac
+done
+"$FC" -shared -fPIC -o "$OUT/libpgkref_timed.so" "${TOBJS[@]}" "$OUT/timed/calforce.o" \
+  "$OUT/timed/calforce_hf.o"
+rm -f "$OUT"/timed/*.log
+
 # The same reference with the drop-in shim (thrid_amd/fortran/calforce_gpu.F90)
 # in place of calforce.o, calforce_hf.o, mklist.o and mklist_cell.o, linked to
 # the MI355X library: the link itself checks that the shim satisfies every

@@ -29,6 +29,7 @@ ORACLE_SO = os.path.join(HERE, "libpgkoracle.so")
 # (e.g. _ref/libpgkref_gpu.so: the reference with the drop-in GPU shim)
 REF_SO = os.environ.get("PGK_REF_SO") or os.path.join(HERE, "_ref", "libpgkref.so")
 REF_GPU_SO = os.path.join(HERE, "_ref", "libpgkref_gpu.so")
+REF_TIMED_SO = os.path.join(HERE, "_ref", "libpgkref_timed.so")   # -DTIME_M calforce
 
 
 def build_oracle(force: bool = False) -> str:
@@ -404,6 +405,14 @@ class Reference:
                         float(s["rcut"]), int(s["nspltbl"]), nf[0], nf[1], nf[2],
                         int(s["pme_order"]), C.byref(ewc))
         self.pot_ewc = ewc.value
+
+    def times(self, reset=False):
+        """The reference's own timers (time_global, CPU seconds): drct, rcpr,
+        bond (accumulated by calforce in a -DTIME_M build) and table (the
+        list builds), heatf; reset=True zeroes them afterwards."""
+        t = np.zeros(5)
+        self.L.pgkref_times(C.c_void_p(t.ctypes.data), C.c_int32(1 if reset else 0))
+        return dict(zip(("drct_time", "rcpr_time", "bond_time", "table_time", "heatf_time"), t))
 
     # ---- integrator pieces (rattle_c, rattlep_v, nhcint) ----
     def set_md(self, cons_ij, cons_d, group_start, mass, vel):

@@ -30,6 +30,8 @@ module pgkref_harness
   use interface_preproc, only: MPI_loopinit, malloc_larray, mkexcl, &
        &                       spline_interp_init, fft_pme_init, prepewk
   use interface_interact, only: ewaldkp_hf
+  use interface_timer, only: get_timecount, get_timecount_barrier
+  use time_global
   implicit none
   private
 
@@ -239,13 +241,37 @@ contains
     real(c_double), value :: rcut_book
     integer(c_int), value :: mode
     integer(c_int) :: nl
+    ! the list build is timed as moldyn.F90:1063-1244 times it (table_time)
+    call get_timecount_barrier(table_time, table_time_elps, 'START')
     if (mode == 0) then
        call mklist2a(rcut_book, box_s(1), box_s(2), box_s(3), .true.)
     else
        call mklist_cell(rcut_book, box_s(1), box_s(2), box_s(3))
     end if
+    call get_timecount(table_time, table_time_elps, 'STOP')
     nl = nlistnball
   end function pgkref_mklist
+
+  ! the reference's own per-term timers (time_global): in a build with
+  ! -DTIME_M calforce accumulates drct_time (ewaldr), rcpr_time (SPME) and
+  ! bond_time (bonded terms) itself; table_time is the list builds above.
+  ! Seconds of process CPU time (etime in the serial build).
+  subroutine pgkref_times(t, reset) bind(C, name='pgkref_times')
+    real(c_double), intent(out) :: t(5)
+    integer(c_int), value :: reset
+    t(1) = drct_time
+    t(2) = rcpr_time
+    t(3) = bond_time
+    t(4) = table_time
+    t(5) = heatf_time
+    if (reset /= 0) then
+       drct_time = 0.0d0
+       rcpr_time = 0.0d0
+       bond_time = 0.0d0
+       table_time = 0.0d0
+       heatf_time = 0.0d0
+    end if
+  end subroutine pgkref_times
 
   subroutine pgkref_get_list(idx, lst) bind(C, name='pgkref_get_list')
     integer(c_int), intent(out) :: idx(*), lst(*)

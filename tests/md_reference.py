@@ -192,7 +192,7 @@ class Nhc:
 
 
 def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=None,
-              record=None):
+              record=None, lang=None):
     """nstep NVT steps in the order of moldyn_nhc.F90 (nhcint, thermostatted
     long-force velocity update, drift, rattle_c, forces, velocity update,
     rattlep_v, nhcint) with one short step per long step.  force(pos) returns
@@ -203,27 +203,49 @@ def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=No
     f, pe = force(pos)
     if record:
         record(pos, vel, pe)
-    for _ in range(nstep):
-        nhc.half((mass[:, None] * vel * vel).sum() / MD_CONV, dt)
-        s2, bbv = nhc.kick_factors(dt)
+    # with a region Langevin thermostat (lang) the chain acts on the atoms
+    # outside every region only; the others take plain half kicks
+    chain = np.ones(len(mass), bool)
+    if lang:
+        lang.register(pos)
+
+    def kint():
+        return (mass[chain, None] * vel[chain] * vel[chain]).sum() / MD_CONV
+
+    def kick(vel, f, s2, bbv):
         acc = f * MD_CONV * minv[:, None]
-        vel = vel * s2 + acc * bbv
+        out = vel * s2 + acc * bbv
+        if lang:
+            plain = vel + 0.5 * dt * MD_CONV * minv[:, None] * f
+            out[~chain] = plain[~chain]
+        return out
+
+    for _ in range(nstep):
+        if lang:
+            lang.half(vel, True)
+            chain = lang.reg < 0
+        nhc.half(kint(), dt)
+        s2, bbv = nhc.kick_factors(dt)
+        vel = kick(vel, f, s2, bbv)
         old = pos.copy()
         pos = pos + dt * vel
         if shake_fn:
             shake_fn(pos, vel, old)
         f, pe = force(pos)
-        acc = f * MD_CONV * minv[:, None]
-        vel = vel * s2 + acc * bbv
+        vel = kick(vel, f, s2, bbv)
         if rattle_fn:
             rattle_fn(pos, vel)
-        nhc.half((mass[:, None] * vel * vel).sum() / MD_CONV, dt)
+        nhc.half(kint(), dt)
+        if lang:
+            lang.register(pos)
+            lang.half(vel, False)
         if record:
             record(pos, vel, pe)
     return pos, vel
 
 
-def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, record=None):
+def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, record=None,
+            lang=None):
     """nhc_steps on the oracle's forces with the rigid-water constraints
     (the list built once at the start, nstep < nstep_book)."""
     mass = np.asarray(sysd["mass"], np.float64)
@@ -243,7 +265,7 @@ def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, rec
 
     return nhc_steps(force, pos, vel, minv, dt, nstep, nhc,
                      lambda p, v, o: shake(p, v, o, minv, cons_ij, d2, groups, dt),
-                     lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record)
+                     lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record, lang)
 
 
 def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, nstep, mts,

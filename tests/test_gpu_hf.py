@@ -219,6 +219,6 @@ def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode):
         assert np.max(np.abs(fused["hf"][:, t] - full[:, t])) <= tol * max(ts, 1e-3 * scale), \
             abi.HFJ_NAMES[t]
     if mode == abi.HF_VOLUME:
-        z = np.linspace(0.0, lz, 258)
+        z = np.linspace(0.0, lz, 1026)   # more regions than the pass holds in LDS
         with pytest.raises(PgkError):
             c.calforce_heatf(dict(mode=mode, z1=z[:-1], z2=z[1:]), vel)

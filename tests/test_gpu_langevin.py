@@ -201,3 +201,36 @@ def test_langevin_heatf_series_sees_thermostatted_state():
     scale = np.max(np.abs(end))
     assert np.max(np.abs(series[-1] - end)) <= 1e-11 * scale
     c.close()
+
+
+def test_langevin_ends_with_chain_middle_matches_reference(oracle_mod):
+    """Region Langevin thermostats at the two ends of the box (hot and cold
+    slabs in z) and a Nose-Hoover chain on every atom outside them, the
+    thermostat arrangement of the two-sided interface workload: 5 FP64 steps
+    against the NumPy restatement (the chain's kinetic energy over the atoms
+    outside the regions, plain half kicks inside them)."""
+    s = water648()
+    dt = 1.0
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=31)
+    c, (ij, d, gs) = _ctx(s, "fp64", dt)
+    c.md_set_velocities(v0)
+    lx, ly, lz = (float(b) for b in s["box"])
+    reg = np.array([[0.0, lx, 0.0, ly, 0.0, 0.2 * lz], [0.0, lx, 0.0, ly, 0.8 * lz, lz]])
+    temp, damp = [320.0, 280.0], [0.02, 0.02]
+    nfree = 3 * s["natom"] - len(d) - 3
+    c.md_langevin(reg, temp, damp, seed=77)
+    c.md_nhc(300.0, 0.01, nfree, mchain=4, nyosh=3, next=1)
+    c.md_run(5, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    x, vx = c.md_nhc_state()
+    det, cnt = c.md_langevin_state()
+    lt = md_reference.Langevin(reg, temp, damp, s["molid"], s["mass"], s["box"], dt, seed=77)
+    nhc = md_reference.Nhc(4, 300.0, 0.01, nfree, 3, 1)
+    rp, rv = md_reference.run_nhc(oracle_mod.Oracle(s), s, v0, ij, d, gs, dt, 5, nhc, lang=lt)
+    assert min(cnt) > 0 and sum(cnt) < s["natom"]
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    assert np.max(np.abs(vx - nhc.v)) < 1e-9 * np.max(np.abs(nhc.v))
+    assert np.max(np.abs(x - nhc.x)) < 1e-9 * max(np.max(np.abs(nhc.x)), 1e-12)
+    np.testing.assert_allclose(det, lt.det, rtol=1e-8, atol=1e-9 * np.abs(lt.det).max())
+    assert list(cnt) == [int(np.sum(lt.reg == k)) for k in range(2)]

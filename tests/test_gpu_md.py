@@ -206,11 +206,35 @@ def test_md_nhc_thermalises_and_conserves(prec):
 
 
 
+def _calcom(s, pos):
+    """calcom (calcom.F90:1-147): sum m x over each molecule's atoms in index
+    order times 1/M (a one-atom molecule: its position), wrapped once."""
+    mass = np.asarray(s["mass"], np.float64)
+    molid = np.asarray(s["molid"])
+    box = np.asarray(s["box"], np.float64)
+    com = np.zeros((int(molid.max()), 3))
+    for m in range(com.shape[0]):
+        atoms = np.nonzero(molid == m + 1)[0]
+        if len(atoms) == 1:
+            c = pos[atoms[0]].copy()
+        else:
+            mm, c = 0.0, np.zeros(3)
+            for i in atoms:
+                mm = mm + mass[i]
+                c = c + mass[i] * pos[i]
+            c = c * (1.0 / mm)
+        c = np.where(c < 0.0, c + box, np.where(c >= box, c - box, c))
+        com[m] = c
+    return com
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("mode", ["bulk", "volume"])
+@pytest.mark.parametrize("mode", ["bulk", "volume", "volume_mole"])
 def test_md_heatf_series_matches_fused_call(prec, mode):
     """The heat-flux rows of the MD loop (steps 0, 5, 10) against the fused
-    calforce_heatf call on the same state: the start, and the end of the run."""
+    calforce_heatf call on the same state: the start, and the end of the run.
+    volume_mole: every water by its molecule's centre of mass, which the MD
+    loop computes on the device and the fused call takes from the host."""
     from thrid_amd import abi
     s = water648()
     c, _cons = _ctx(s, prec, 1.0)
@@ -222,6 +246,8 @@ def test_md_heatf_series_matches_fused_call(prec, mode):
     else:
         z = np.linspace(0.0, lz, 5)
         hf = dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:])
+        if mode == "volume_mole":
+            hf["hftyp"] = np.full(s["natom"], abi.HFTYP_MOLE, np.int32)
     c.md_heatf(hf, 5)
     rows = c.md_run(10, 20, s["rcut_book"], 5)
     series = c.md_heatf_series()
@@ -239,8 +265,12 @@ def test_md_heatf_series_matches_fused_call(prec, mode):
     for a, b in ((rows, rows2), (p1, p2), (v1, v2), (ene[0], ene2[0])):
         assert np.max(np.abs(a - b)) <= tol * max(1.0, np.max(np.abs(b)))
     m = s["mass"] / md.MD_CONV          # m v^2 / 2 in eref, as the MD loop uses
+    if mode == "volume_mole":
+        hf["molecom"] = _calcom(s, p1)
     end = c.calforce_heatf(hf, v1, m)["hf"]
     c.set_positions(p0)
+    if mode == "volume_mole":
+        hf["molecom"] = _calcom(s, p0)
     start = c.calforce_heatf(hf, v0)["hf"]
     for got, ref in ((series[2], end), (series[0], start)):
         scale = np.max(np.abs(ref))

@@ -57,6 +57,46 @@ __global__ __launch_bounds__(256) void k_molevel(int nmol, const int32_t *__rest
   molevel[3 * m + 2] = p2 * inv;
 }
 
+// Molecule centres of mass as calcom (calcom.F90:1-147): sum m x over the
+// molecule's atoms in index order, times 1/M; a one-atom molecule takes its
+// position; then wrapped once into [0, L) per axis.
+__global__ __launch_bounds__(256) void k_calcom(int nmol, const int32_t *__restrict__ start,
+                                                const int32_t *__restrict__ atoms,
+                                                const double *__restrict__ mass,
+                                                const double *__restrict__ pos, double3 L,
+                                                double *__restrict__ com) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  double c[3];
+  if (start[m + 1] - start[m] == 1) {
+    const int i = atoms[start[m]];
+    c[0] = pos[3 * i];
+    c[1] = pos[3 * i + 1];
+    c[2] = pos[3 * i + 2];
+  } else {
+    double mm = 0.0;
+    c[0] = c[1] = c[2] = 0.0;
+    for (int k = start[m]; k < start[m + 1]; k++) {
+      const int i = atoms[k];
+      mm = mm + mass[i];
+      c[0] = c[0] + mass[i] * pos[3 * i];
+      c[1] = c[1] + mass[i] * pos[3 * i + 1];
+      c[2] = c[2] + mass[i] * pos[3 * i + 2];
+    }
+    const double inv = 1.0 / mm;
+    c[0] = c[0] * inv;
+    c[1] = c[1] * inv;
+    c[2] = c[2] * inv;
+  }
+  const double Ls[3] = {L.x, L.y, L.z};
+  for (int d = 0; d < 3; d++) {
+    if (c[d] < 0.0) c[d] = c[d] + Ls[d];
+    else if (c[d] >= Ls[d]) c[d] = c[d] - Ls[d];
+    com[3 * m + d] = c[d];
+  }
+}
+
 // transport part: block (chunk, region); record [kNterm][3]
 template <class AH>
 __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__restrict__ part) {
@@ -251,6 +291,15 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
   }
   if (rc) return rc;
   k_heatf_final<<<h.nreg, 256, 0, c->stream>>>(a, ptr, pcol, c->d_hj_out);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+int pgk_calcom_launch(pgk_ctx *c, const double *d_mass, double *d_com) {
+  if (c->hj_nmol <= 0) return PGK_OK;
+  k_calcom<<<pgk_grid_blocks(c->hj_nmol, 256), 256, 0, c->stream>>>(
+      c->hj_nmol, c->d_hj_mol_start, c->d_hj_mol_atoms, d_mass, c->d_pos,
+      make_double3(c->box[0], c->box[1], c->box[2]), d_com);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

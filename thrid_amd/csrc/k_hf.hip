@@ -134,7 +134,9 @@ __device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp,
                                              double svc, int ql, double svl, double *sl) {
   if (!fp.seg || (svc == 0.0 && svl == 0.0)) return;
   const double lo = fmin(fp.ciz, fp.cjz), hi = fmax(fp.ciz, fp.cjz);
-  for (int ir = 0; ir < h.nreg; ir++) {
+  int r0, r1;
+  hf_region_window(h, lo, hi, r0, r1);
+  for (int ir = r0; ir <= r1; ir++) {
     if (hi < h.z1[ir] || lo > h.z2[ir]) continue;   // the segment misses the region
     const double ff = hf_volume_weight(h, ir, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
     if (ff == 0.0) continue;
@@ -345,6 +347,8 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
       k_hf_pairs_flux<AH, true><<<nblk, 256, 0, st>>>(p, h);
     } else {
       const size_t lds = sizeof(double) * 16 * (size_t)h.nreg;
+      HIP_CHECK(hipFuncSetAttribute((const void *)k_hf_pairs_flux<AH, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       k_hf_pairs_flux<AH, false><<<nblk, 256, lds, st>>>(p, h);
     }
     HIP_CHECK(hipGetLastError());

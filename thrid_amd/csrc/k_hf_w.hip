@@ -238,7 +238,9 @@ __global__ __launch_bounds__(256) void k_hf_w(HfWArgs p, HfDev h) {
       const double flx = (double)fl * rx, fly = (double)fl * ry, flz = (double)fl * rz;
       AH *st = sl + (size_t)tl * 2 * nreg * 3;
       const double lo = fmin(ct3[2], cs3[2]), hi = fmax(ct3[2], cs3[2]);
-      for (int ir = 0; ir < nreg; ir++) {
+      int r0 = 0, r1 = nreg - 1;
+      if (MODE == PGK_HF_VOLUME) hf_region_window(h, lo, hi, r0, r1);
+      for (int ir = r0; ir <= r1; ir++) {
         double ff;
         if (MODE == PGK_HF_SURFACE) {
           const double ri1 = ct3[2] - h.z1[ir], rj1 = cs3[2] - h.z1[ir];

@@ -53,6 +53,13 @@ __device__ __forceinline__ double kickv(double v, double f, double minv, double 
                                         const Kick &k) {
   return k.nhc ? kick_nhc(v, f, minv, k) : kick1(v, f, minv, dt);
 }
+// With the region Langevin thermostat on as well, the chain acts on the atoms
+// outside every Langevin region only: an atom registered in a region
+// (ltreg[i] >= 0) takes the plain kick, the region thermostat its own update.
+__device__ __forceinline__ Kick kick_at(const Kick &k, const int32_t *ltreg, int i) {
+  if (k.nhc && ltreg && ltreg[i] >= 0) return Kick{false, 1.0, 0.0};
+  return k;
+}
 
 // RESPA (moldyn.F90:1465-1771): the long force's half kick (dt_l) comes
 // before the short one on the first inner step and after it on the last;
@@ -64,10 +71,11 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict
                                                        const double *__restrict__ minv, double dt,
                                                        const double *__restrict__ nhc,
                                                        const double *__restrict__ force_l,
-                                                       double dt_l) {
+                                                       double dt_l,
+                                                       const int32_t *__restrict__ ltreg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Kick k = kick_of(nhc);
+  const Kick k = kick_at(kick_of(nhc), ltreg, i);
   for (int d = 0; d < 3; d++) {
     double v0 = vel[3 * i + d];
     if (force_l) v0 = kick1(v0, force_l[3 * i + d], minv[i], dt_l);
@@ -82,10 +90,11 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double *__restrict__ vel
                                                  const double *__restrict__ force,
                                                  const double *__restrict__ minv, double dt,
                                                  const double *__restrict__ nhc,
-                                                 const double *__restrict__ force_l, double dt_l) {
+                                                 const double *__restrict__ force_l, double dt_l,
+                                                 const int32_t *__restrict__ ltreg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Kick k = kick_of(nhc);
+  const Kick k = kick_at(kick_of(nhc), ltreg, i);
   for (int d = 0; d < 3; d++) {
     double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
     if (force_l) v = kick1(v, force_l[3 * i + d], minv[i], dt_l);
@@ -108,6 +117,7 @@ struct ConsArgs {
   const double *nhc;       // thermostat factors of the half kicks, or null
   const double *force_l;   // RESPA: long force of this inner step's extra kick, or null
   double dt_l;             // RESPA: its time step (the outer one)
+  const int32_t *ltreg;    // chain + region Langevin: region of each atom (-1 none), or null
 };
 
 // Triangle groups (three atoms held by three constraints: the rigid 3-site
@@ -268,9 +278,9 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     double3 o0, o1, o2;
-    kick_drift3(force, a.minv, i0, a.dt, k, p0, v0, o0, a.force_l, a.dt_l);
-    kick_drift3(force, a.minv, i1, a.dt, k, p1, v1, o1, a.force_l, a.dt_l);
-    kick_drift3(force, a.minv, i2, a.dt, k, p2, v2, o2, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, i0), p0, v0, o0, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, i1), p1, v1, o1, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, i2), p2, v2, o2, a.force_l, a.dt_l);
     const bool done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 p = ld3(a.pos, i), v = ld3(a.vel, i), o;
-    kick_drift3(force, a.minv, i, a.dt, k, p, v, o, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, i), p, v, o, a.force_l, a.dt_l);
     st3(a.pos, i, p);
     st3(a.vel, i, v);
   }
@@ -300,9 +310,9 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
-    kick3(force, a.minv, i0, a.dt, k, v0, a.force_l, a.dt_l);
-    kick3(force, a.minv, i1, a.dt, k, v1, a.force_l, a.dt_l);
-    kick3(force, a.minv, i2, a.dt, k, v2, a.force_l, a.dt_l);
+    kick3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, i0), v0, a.force_l, a.dt_l);
+    kick3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, i1), v1, a.force_l, a.dt_l);
+    kick3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, i2), v2, a.force_l, a.dt_l);
     const bool done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
@@ -311,7 +321,7 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 v = ld3(a.vel, i);
-    kick3(force, a.minv, i, a.dt, k, v, a.force_l, a.dt_l);
+    kick3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, i), v, a.force_l, a.dt_l);
     st3(a.vel, i, v);
   }
 }
@@ -391,13 +401,15 @@ __global__ __launch_bounds__(64) void k_rattle_v(ConsArgs a) {
   if (!done) atomicOr(a.fail, 2);
 }
 
-// kinetic energy records: per block 0.5 sum m v^2 (fixed-order tree)
+// kinetic energy records: per block 0.5 sum m v^2 (fixed-order tree); with
+// skip, the atoms registered in a Langevin region are left out
 __global__ __launch_bounds__(256) void k_md_ke(int n, const double *__restrict__ vel,
                                                const double *__restrict__ minv,
-                                               double *__restrict__ rec) {
+                                               double *__restrict__ rec,
+                                               const int32_t *__restrict__ skip) {
   double ev[1] = {0.0};
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  if (i < n && !(skip && skip[i] >= 0)) {
     const double v2 = vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] +
                       vel[3 * i + 2] * vel[3 * i + 2];
     ev[0] = 0.5 * v2 / minv[i] / PGK_MD_CONV;
@@ -511,6 +523,7 @@ ConsArgs cons_args(pgk_ctx *c) {
   a.nhc = c->md_nhc_m ? c->d_md_nhc + 3 * c->md_nhc_m : nullptr;
   a.force_l = nullptr;
   a.dt_l = 0.0;
+  a.ltreg = (c->md_nhc_m && c->md_lt_nreg) ? c->d_lt_reg : nullptr;
   return a;
 }
 
@@ -533,7 +546,7 @@ int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, 
     return PGK_OK;
   }
   k_md_kick_drift<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(
-      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, f, c->d_md_minv, dt, a.nhc, fl, dtl);
+      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, f, c->d_md_minv, dt, a.nhc, fl, dtl, a.ltreg);
   HIP_CHECK(hipGetLastError());
   if (c->md_ngroup > 0) {
     k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(a);
@@ -558,7 +571,7 @@ int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double
     return PGK_OK;
   }
   k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, f, c->d_md_minv,
-                                                            dt, a.nhc, fl, dtl);
+                                                            dt, a.nhc, fl, dtl, a.ltreg);
   HIP_CHECK(hipGetLastError());
   return pgk_md_rattle_v(c);
 }
@@ -571,15 +584,18 @@ int pgk_md_rattle_v(pgk_ctx *c) {
   return PGK_OK;
 }
 
-// queue the kinetic-energy records into rec (pgk_grid_blocks(natom, 256) records)
-int pgk_md_ke(pgk_ctx *c, double *rec) {
+// queue the kinetic-energy records into rec (pgk_grid_blocks(natom, 256)
+// records); the chain's records (nhc) leave out the atoms of Langevin regions
+int pgk_md_ke(pgk_ctx *c, double *rec, bool nhc) {
   const int n = c->natom;
-  k_md_ke<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_md_minv, rec);
+  const int32_t *skip = (nhc && c->md_lt_nreg) ? c->d_lt_reg : nullptr;
+  k_md_ke<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_md_minv, rec,
+                                                          skip);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
 
-// one nhcint call on the kinetic-energy records already in d_md_ke
+// one nhcint call on the kinetic-energy records already in d_md_kenhc
 int pgk_md_nhc_half(pgk_ctx *c) {
   NhcArgs p;
   p.m = c->md_nhc_m;
@@ -594,7 +610,7 @@ int pgk_md_nhc_half(pgk_ctx *c) {
     p.w4[k] = p.w2[k] / 2.0;
     p.w8[k] = p.w4[k] / 2.0;
   }
-  k_nhc_half<<<1, 256, 0, c->stream>>>(p, c->d_md_ke, c->d_md_nhc);
+  k_nhc_half<<<1, 256, 0, c->stream>>>(p, c->d_md_kenhc, c->d_md_nhc);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

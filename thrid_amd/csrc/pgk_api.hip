@@ -851,6 +851,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_hf_pe_out, c->d_hf_w_out, c->d_md_gstart, c->d_md_cij, c->d_md_fail,
                   c->d_md_small,
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
+                  c->d_md_kenhc,
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
                   c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_vgrid,
                   c->d_vir6, c->d_hfw_xq, c->d_hfw_cen, c->d_hfw_mol, c->d_tl_start,
@@ -1050,8 +1051,10 @@ int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
 // regions, hftyp and molecule centres, clears the per-atom sums and points
 // hfdev at them.  flux: no W arrays; the collision part is summed into
 // d_hj_jcol / d_hj_jpart with the velocities already in d_hj_vel.
+// dev_com: the molecule centres are computed on the device (k_calcom) rather
+// than given by the caller
 static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *who,
-                    bool upload = true) {
+                    bool upload = true, bool dev_com = false) {
   if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
       !hf->z2 || !hf->hftyp) {
     pgk_set_error(std::string(who) + ": invalid heat-flux description");
@@ -1068,7 +1071,7 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   }
   int maxmol = 0;
   for (int i = 0; i < n; i++) maxmol = std::max(maxmol, c->molid[i]);
-  if (need_com && (!hf->molecom || hf->nmol < maxmol)) {
+  if (need_com && !dev_com && (!hf->molecom || hf->nmol < maxmol)) {
     pgk_set_error(std::string(who) + ": MOLE atoms need molecom for every molecule");
     return PGK_EINVAL;
   }
@@ -1089,8 +1092,9 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
                 (rc = grow_dev((void **)&c->d_hf_pe_out, &peo_cap, pe_b)) ||
                 (rc = grow_dev((void **)&c->d_hf_w_out, &c->hf_wout_cap, wout_b))))
     return rc;
+  const int ncom = dev_com ? maxmol : hf->nmol;
   if (need_com &&
-      (rc = grow_dev((void **)&c->d_hf_com, &c->hf_com_cap, sizeof(double) * 3 * hf->nmol)))
+      (rc = grow_dev((void **)&c->d_hf_com, &c->hf_com_cap, sizeof(double) * 3 * ncom)))
     return rc;
   if (upload) {
     HIP_CHECK(hipMemcpyAsync(c->d_hf_z, hf->z1, sizeof(double) * nreg, hipMemcpyHostToDevice,
@@ -1100,7 +1104,7 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
     HIP_CHECK(hipMemcpyAsync(c->d_hf_typ, hf->hftyp, sizeof(int32_t) * n,
                              hipMemcpyHostToDevice, st));
   }
-  if (need_com)
+  if (need_com && !dev_com)
     HIP_CHECK(hipMemcpyAsync(c->d_hf_com, hf->molecom, sizeof(double) * 3 * hf->nmol,
                              hipMemcpyHostToDevice, st));
   HIP_CHECK(hipMemsetAsync(c->d_hf_pe, 0, pe_b, st));
@@ -1112,6 +1116,16 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   h.nw = (int)nw;
   h.z1 = c->d_hf_z;
   h.z2 = c->d_hf_z + nreg;
+  {   // equal bins tiling [z1[0], z2[nreg-1]]
+    const double w = hf->z2[0] - hf->z1[0];
+    bool ub = nreg >= 2 && w > 0.0;
+    for (int r = 0; ub && r < nreg; r++)
+      ub = fabs(hf->z1[r] - (hf->z1[0] + r * w)) <= 1e-9 * w &&
+           fabs(hf->z2[r] - hf->z1[r] - w) <= 1e-9 * w;
+    h.ubin = ub ? 1 : 0;
+    h.ubz0 = hf->z1[0];
+    h.ubw = ub ? w : 1.0;
+  }
   h.hftyp = c->d_hf_typ;
   h.molid = c->d_molid;
   h.molecom = need_com ? c->d_hf_com : nullptr;
@@ -1372,7 +1386,7 @@ int pgk_md_setup(pgk_ctx *c, const pgk_md_desc *md) {
       (rc = upload(&c->d_md_fail, one)))
     return rc;
   std::vector<double> rec((size_t)pgk_grid_blocks(n, 256) * PGK_REC, 0.0);
-  if ((rc = upload(&c->d_md_ke, rec))) return rc;
+  if ((rc = upload(&c->d_md_ke, rec)) || (rc = upload(&c->d_md_kenhc, rec))) return rc;
   // every group a triangle: the half steps run as one kernel each, with the
   // atoms outside every group listed for their own threads
   bool all_tri = true;
@@ -1462,6 +1476,8 @@ static int md_sample(pgk_ctx *c, double *row) {
   return PGK_OK;
 }
 
+static int hj_molecules(pgk_ctx *c);
+
 // ---- heat-flux time series of the MD loop ----------------------------------
 // the bonded terms (their heat-flux parts) and the pair heat-flux pass; the
 // pair and reciprocal forces of the step are already known
@@ -1482,9 +1498,10 @@ static int md_hf_begin(pgk_ctx *c, int nstep) {
     std::vector<double> z((size_t)3 * c->natom, 0.0);
     if ((rc = upload(&c->d_hj_fscratch, z))) return rc;
   }
+  if (c->md_hf_mole && (rc = hj_molecules(c))) return rc;
   const pgk_hf_desc d = {c->md_hf_mode, nreg, c->md_hf_z1.data(), c->md_hf_z2.data(),
                          c->md_hf_typ.data(), 0, nullptr};
-  return hf_setup(c, &d, true, "pgk_md_run");
+  return hf_setup(c, &d, true, "pgk_md_run", true, true);
 }
 
 // at step s (a multiple of the interval): the fused heat-flux pass at the
@@ -1496,7 +1513,9 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   const pgk_hf_desc d = {c->md_hf_mode, nreg, c->md_hf_z1.data(), c->md_hf_z2.data(),
                          c->md_hf_typ.data(), 0, nullptr};
   int rc;
-  if ((rc = hf_setup(c, &d, true, "pgk_md_run", false))) return rc;
+  if ((rc = hf_setup(c, &d, true, "pgk_md_run", false, true))) return rc;
+  // MOLE centres: the molecules' centres of mass at this state (calcom.F90)
+  if (c->md_hf_mole && (rc = pgk_calcom_launch(c, c->d_md_hfmass, c->d_hf_com))) return rc;
   c->hfdev.vel = c->d_md_vel;
   c->hf_export = false;
   c->hf_pairs_only = true;
@@ -1552,11 +1571,20 @@ struct MdRunner {
   // variables; the atoms feel it through the half kicks), so the closing
   // call's kinetic-energy records serve the next step's opening call, and
   // begin() writes them for the first.
-  int nhc_open() { return c->md_nhc_m ? pgk_md_nhc_half(c) : PGK_OK; }
+  // With the region Langevin thermostat as well, the chain couples to the
+  // atoms outside every region; the region half that opens the step has just
+  // changed the others, and a registration may have moved atoms between the
+  // sets, so the opening call takes fresh records of the chain's atoms.
+  int nhc_open() {
+    int r;
+    if (!c->md_nhc_m) return PGK_OK;
+    if (c->md_lt_nreg && (r = pgk_md_ke(c, c->d_md_kenhc, true))) return r;
+    return pgk_md_nhc_half(c);
+  }
   int nhc_close() {
     int r;
     if (!c->md_nhc_m) return PGK_OK;
-    if ((r = pgk_md_ke(c, c->d_md_ke))) return r;
+    if ((r = pgk_md_ke(c, c->d_md_kenhc, true))) return r;
     return pgk_md_nhc_half(c);
   }
   int eager(bool ene) {
@@ -1629,7 +1657,7 @@ struct MdRunner {
           sizeof(double) * PGK_LT_MAXREG * (size_t)pgk_grid_blocks(c->natom, 256), c->stream));
       if ((r = pgk_md_lt_register(c))) return r;
     }
-    if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_ke))) return r;
+    if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_kenhc, true))) return r;
     if (respa()) {
       if ((r = short_forces(true))) return r;
       return run_calforce(c, PGK_MTS_LONG, &c->md_mts, 0, false, c->d_force, nullptr, nullptr,
@@ -1702,10 +1730,6 @@ int pgk_md_run(pgk_ctx *c, int32_t nstep, int32_t nstep_book, double rcut_book,
     pgk_set_error("pgk_md_run: RESPA with the Nose-Hoover chain is not offered");
     return PGK_EINVAL;
   }
-  if (c->md_lt_nreg && c->md_nhc_m) {
-    pgk_set_error("pgk_md_run: the Langevin thermostat with the Nose-Hoover chain is not offered");
-    return PGK_EINVAL;
-  }
   int rc = set_device_ctx(c);
   if (rc) return rc;
   c->hf_ready = false;
@@ -1745,11 +1769,6 @@ int pgk_md_run_batch(pgk_ctx *const *ctxs, int32_t n, int32_t nstep, int32_t nst
     pgk_ctx *c = ctxs[r];
     if (c && c->md_nshort > 1 && c->md_nhc_m) {
       pgk_set_error("pgk_md_run_batch: RESPA with the Nose-Hoover chain is not offered");
-      return PGK_EINVAL;
-    }
-    if (c && c->md_lt_nreg && c->md_nhc_m) {
-      pgk_set_error("pgk_md_run_batch: the Langevin thermostat with the Nose-Hoover chain is "
-                    "not offered");
       return PGK_EINVAL;
     }
     if (!c || !c->md_ready || !c->have_pos) {
@@ -2110,11 +2129,14 @@ int pgk_md_heatf(pgk_ctx *c, const pgk_hf_desc *hf, int32_t interval) {
                   " regions) or bulk form needed");
     return PGK_EINVAL;
   }
-  for (int i = 0; hf->hftyp && i < n; i++)
-    if (hf->hftyp[i] != PGK_HFTYP_ATOM) {
-      pgk_set_error("pgk_md_heatf: every atom must be its own centre (hftyp ATOM)");
+  bool mole = false;
+  for (int i = 0; hf->hftyp && i < n; i++) {
+    if (hf->hftyp[i] != PGK_HFTYP_ATOM && hf->hftyp[i] != PGK_HFTYP_MOLE) {
+      pgk_set_error("pgk_md_heatf: hftyp must be ATOM (1) or MOLE (2)");
       return PGK_EINVAL;
     }
+    mole = mole || hf->hftyp[i] == PGK_HFTYP_MOLE;
+  }
   int rc = set_device_ctx(c);
   if (rc) return rc;
   std::vector<double> m(n);
@@ -2122,7 +2144,9 @@ int pgk_md_heatf(pgk_ctx *c, const pgk_hf_desc *hf, int32_t interval) {
   if ((rc = upload(&c->d_md_hfmass, m))) return rc;
   c->md_hf_z1.assign(hf->z1, hf->z1 + hf->nregion);
   c->md_hf_z2.assign(hf->z2, hf->z2 + hf->nregion);
-  c->md_hf_typ.assign(n, PGK_HFTYP_ATOM);
+  if (hf->hftyp) c->md_hf_typ.assign(hf->hftyp, hf->hftyp + n);
+  else c->md_hf_typ.assign(n, PGK_HFTYP_ATOM);
+  c->md_hf_mole = mole;
   c->md_hf_mode = hf->mode;
   c->md_hf_interval = interval;
   c->md_hf_rows = 0;

@@ -76,6 +76,19 @@ __device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, doubl
   return (r2 >= 0.0) ? ff * nbc : -ff * nbc;
 }
 
+// the regions a z segment [lo, hi] can touch: with equal bins the bins of its
+// ends (widened by one for rounding; the exact weight decides), else all
+__device__ __forceinline__ void hf_region_window(const HfDev &h, double lo, double hi, int &r0,
+                                                 int &r1) {
+  if (h.ubin) {
+    r0 = max(0, (int)floor((lo - h.ubz0) / h.ubw) - 1);
+    r1 = min(h.nreg - 1, (int)floor((hi - h.ubz0) / h.ubw) + 1);
+  } else {
+    r0 = 0;
+    r1 = h.nreg - 1;
+  }
+}
+
 // flux form of calvirihf: the interaction's share of sum_a W(a,n,.,r) v(a),
 // i.e. the region weight times s = fi . v_i - fj . v_j, added to
 // jcol[cls][r][n] (global FP64 atomics; the bonded terms use this, the pair
@@ -99,7 +112,9 @@ __device__ void calvirihf_flux(const HfDev &h, int i, int j, const double *fi, c
   }
   const double r2 = ci[2] - cj[2], rz = fabs(r2);
   if (rz * h.Li[2] >= 0.5) return;
-  for (int ir = 0; ir < h.nreg; ir++) {
+  int r0, r1;
+  hf_region_window(h, fmin(ci[2], cj[2]), fmax(ci[2], cj[2]), r0, r1);
+  for (int ir = r0; ir <= r1; ir++) {
     const double ff = hf_volume_weight(h, ir, ci[2], cj[2], r2, rz, nbc);
     if (ff != 0.0) atomicAdd(&J[3 * ir + 2], ff * s);
   }
@@ -156,7 +171,9 @@ __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, co
     }
     return;
   }
-  for (int ir = 0; ir < h.nreg; ir++) {
+  int r0, r1;
+  hf_region_window(h, fmin(ci[2], cj[2]), fmax(ci[2], cj[2]), r0, r1);
+  for (int ir = r0; ir <= r1; ir++) {
     const double ff = hf_volume_weight(h, ir, ci[2], cj[2], r[2], rz, nbc);
     if (ff == 0.0) continue;
     AH *wi = W + (ir * N + i) * 3, *wj = W + (ir * N + j) * 3;

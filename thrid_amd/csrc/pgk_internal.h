@@ -62,7 +62,7 @@ void pgk_set_error(const std::string &msg);
 
 // volume regions of the flux form: the pair pass keeps [4][nreg] doubles of
 // LDS per wave
-#define PGK_HEATF_MAXREG 256
+#define PGK_HEATF_MAXREG 1024
 
 // device view of one calforce_hf call's heat-flux set-up and accumulators
 // (layout: pe[class][atom]; w[class][region][atom][nw], nw = 9 in bulk mode,
@@ -70,6 +70,10 @@ void pgk_set_error(const std::string &msg);
 struct HfDev {
   int mode = 0, nreg = 0, natom = 0, nw = 3;
   const double *z1 = nullptr, *z2 = nullptr;   // [nreg]
+  // regions that tile [ubz0, ubz0 + nreg ubw) in equal bins (ubin = 1): a
+  // segment's regions are then found from its ends instead of by a walk
+  int ubin = 0;
+  double ubz0 = 0.0, ubw = 1.0;
   const int32_t *hftyp = nullptr;              // [natom] 1 = ATOM, 2 = MOLE
   const int32_t *molid = nullptr;              // [natom] 1-based
   const double *molecom = nullptr;             // [3*nmol]
@@ -241,6 +245,7 @@ struct pgk_ctx {
   std::vector<double> md_hf_z1, md_hf_z2;
   std::vector<int32_t> md_hf_typ;
   int md_hf_mode = 0;
+  bool md_hf_mole = false;         // some atoms are MOLE centres (device calcom)
   double *d_md_hfseries = nullptr, *d_md_hfmass = nullptr, *d_hj_fscratch = nullptr;
   size_t md_hfseries_cap = 0;
   std::vector<double> md_mass;     // masses (amu) given to pgk_md_setup
@@ -269,6 +274,7 @@ struct pgk_ctx {
   int32_t *d_md_free = nullptr;
   double *d_md_d2 = nullptr, *d_md_minv = nullptr, *d_md_vel = nullptr, *d_md_pos_old = nullptr;
   double *d_md_ke = nullptr;       // kinetic-energy block records
+  double *d_md_kenhc = nullptr;    // the chain's records (atoms outside Langevin regions)
   // Nose-Hoover chain (pgk_md_nhc); off when md_nhc_m == 0
   int md_nhc_m = 0, md_nhc_nyosh = 1, md_nhc_next = 1;
   double md_nhc_gkt = 0.0, md_nhc_gnkt = 0.0;
@@ -340,6 +346,8 @@ int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_w_launch(pgk_ctx *c);
 int pgk_hf_export(pgk_ctx *c);
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
+// molecule centres of mass from d_pos (calcom.F90:1-147) into d_com [3*nmol]
+int pgk_calcom_launch(pgk_ctx *c, const double *d_mass, double *d_com);
 int pgk_pme_atom_energy(pgk_ctx *c, double *d_e);
 // atoms the allocation-free entry points' fixed scratch holds (8 MiB of doubles)
 #define PGK_NOALLOC_MAX_ATOMS (1 << 20)
@@ -350,7 +358,7 @@ extern "C" int grow_dev(void **p, size_t *cap, size_t need);
 int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl);
 int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double dtl);
 int pgk_md_rattle_v(pgk_ctx *c);
-int pgk_md_ke(pgk_ctx *c, double *rec);
+int pgk_md_ke(pgk_ctx *c, double *rec, bool nhc = false);
 int pgk_md_nhc_half(pgk_ctx *c);
 int pgk_md_lt_register(pgk_ctx *c);      // assign atoms to the Langevin regions
 int pgk_md_lt_half(pgk_ctx *c, bool first);
