@@ -248,10 +248,20 @@ int pgk_calforce_hf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
  * each divided by the region volume Lx Ly (z2 - z1), and their total.
  * Volume mode: z components only (the others are 0); bulk mode: all three.
  * vel [3*natom] and mass [natom] in units where m v^2 / 2 is in the energy
- * unit of the force path (the reference's non-dimensional atmvel, atmmass);
- * mass NULL reuses the last masses given.  hf [nregion*PGK_NHFJ*3].  The
+ * unit of the force path (the reference's non-dimensional atmvel, atmmass),
+ * lab-frame (streaming velocities: pgk_set_atmvel_strm); mass NULL reuses
+ * the last masses given.  hf [nregion*PGK_NHFJ*3].  The
  * surface mode keeps state across steps and is not offered.                 */
 int pgk_calheatf(pgk_ctx *ctx, const double *vel, const double *mass, double *hf);
+
+/* atmvel_strm (md_global.F90:443, assigned by ass_strmvel.F90:8-295): the
+ * streaming velocity of each atom, [3*natom] in the units of vel, kept by
+ * the context until changed.  pgk_calheatf and pgk_calforce_heatf take each
+ * atom's kinetic energy as m (v - vs)^2 / 2 (calheatf.F90:368-369, 677-681);
+ * the centre velocities they transport and the collision part use vel as
+ * given (calheatf.F90:564, 644, 1023).  NULL clears it (zero).  The MD loop
+ * uses the streaming slabs of pgk_md_langevin instead.                      */
+int pgk_set_atmvel_strm(pgk_ctx *ctx, const double *vstrm);
 
 /* calforce_hf followed by calheatf in one pass (no reference counterpart as
  * one call: the fused z-binned heat-flux reduction for hosts that only need
@@ -259,7 +269,7 @@ int pgk_calheatf(pgk_ctx *ctx, const double *vel, const double *mass, double *hf
  * sum_a W(a,n,i,r) v_i(a) directly, fi.v_i - fj.v_j times the region weight
  * of calvirihf, so the (3,3,natom,nregion) arrays are never formed.  Forces,
  * energies and virials as pgk_calforce_hf; hf as pgk_calheatf.  Volume form
- * up to 256 regions, or the bulk form; the order of the FP64 additions is
+ * up to 1024 regions, or the bulk form; the order of the FP64 additions is
  * not fixed (hf agrees with pgk_calforce_hf + pgk_calheatf to rounding).    */
 int pgk_calforce_heatf(pgk_ctx *ctx, int32_t mts_flag, const pgk_mts *mts,
                        const pgk_hf_desc *hf, const double *vel, const double *mass,
@@ -435,11 +445,15 @@ int pgk_md_nhc(pgk_ctx *ctx, const pgk_nhc_desc *nhc);
  * numbers come from SplitMix64 at counter positions
  * seed + (4 (draw natom + i) + k + 1) 0x9E3779B97F4A7C15, mapped to
  * ((z >> 11) + 1) 2^-53 in (0, 1], in place of the reference's SFMT stream;
- * draw counts the steps run since pgk_md_langevin.  Not combined with the
- * Nose-Hoover chain (as in the reference).  lt NULL switches it off.       */
+ * draw counts the steps run since pgk_md_langevin.  With the Nose-Hoover
+ * chain on as well, the chain acts on the atoms outside every region (its
+ * kinetic energy and its half kicks) and the regions keep their own
+ * thermostat.  lt NULL switches the thermostat and the streaming slabs off. */
 #define PGK_LT_MAXREG 16
 typedef struct pgk_langevin_desc {
-  int32_t nregion;             /* 1..PGK_LT_MAXREG (nlangeregion)             */
+  int32_t nregion;             /* 0..PGK_LT_MAXREG (nlangeregion); 0: no      *
+                                * thermostat, streaming slabs only (needs     *
+                                * nstream >= 1; the region arrays may be NULL)*/
   const double *x1, *x2;       /* [nregion] ltxpos1 / ltxpos2 (A)             */
   const double *y1, *y2;       /* [nregion] ltypos1 / ltypos2 (A)             */
   const double *z1, *z2;       /* [nregion] ltzpos1 / ltzpos2 (A)             */
@@ -453,7 +467,14 @@ typedef struct pgk_langevin_desc {
   /* streaming velocity (ass_strmvel.F90:8-295, ifstrmvel): nstream slabs
    * [z1,z2) in the wrapped z of the same registration units; an atom whose
    * unit lies in slab s has streaming velocity strm_vel[3s..3s+2] (A/fs),
-   * and the thermostat acts on v minus it (langevin_region.F90:81-82, 190).
+   * assigned after every drift at the new positions (moldyn_nhc.F90:1412).
+   * Everything that measures or controls the temperature sees v minus it:
+   * the region thermostat (langevin_region.F90:81-82, 190), the chain's
+   * kinetic energy (nhcint.F90:53-54) and half kicks, which scale v - vs
+   * and add vs back (moldyn_nhc.F90:1280-1310, 1585-1598), the kinetic
+   * energy of the sampled rows (calkin.F90:98) and the atoms' kinetic
+   * energies in the MD loop's heat flux (calheatf.F90:368-369, whose
+   * transport velocities and collision part stay lab-frame).
    * nstream 0 (or NULL arrays): none.                                       */
   int32_t nstream;             /* 0..PGK_LT_MAXREG                            */
   const double *strm_z1, *strm_z2;   /* [nstream] strmzpos1 / strmzpos2 (A)   */

@@ -1330,14 +1330,18 @@ int pgko_calforce_hf(pgko_sys *s, int32_t mts_flag, const pgk_mts *mts,
  * [0, Lz) (calheatf.F90:650-655); velocity of the centre: the atom's, or
  * the molecule's mass-weighted mean over its atoms in index order
  * (calheatf.F90:548-570).  Transport part (:615-870), atoms outer, regions
- * inner: for z1 <= z <= z2, hfkin += (m v^2 / 2) v_c and hfpot_c += e_c v_c.
+ * inner: for z1 <= z <= z2, hfkin += (m u^2 / 2) v_c and hfpot_c += e_c v_c,
+ * u = v - vstrm (vstrm NULL: zero): the reference subtracts atmvel_strm from
+ * atmvel (:368-369) and uses that u for the kinetic energy (:677-681) and
+ * in the collision part (:1027-1330), while the transported centre
+ * velocities v_c come from the lab-frame copy (atmvel_tmp, :564, 588, 644).
  * Collision part (:1027-1330), regions outer, atoms inner:
  * hfviri_c(n) += W(1,n) v1 + W(2,n) v2 + W(3,n) v3.  Only the z component
  * outside the bulk form.  Each term times 1/(Lx Ly (z2 - z1)) (:1334-1380),
  * then hftot = the terms in the reference's order (:1420-1436; the classes
  * the force path does not have are zero). */
 int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
-                  const double *mass, const double *pot_atm,
+                  const double *vstrm, const double *mass, const double *pot_atm,
                   const double *viri_atm, double *hfout) {
   const int n = s->natom, nreg = hf->nregion;
   const int bulk = hf->mode == PGK_HF_BULK;
@@ -1371,7 +1375,8 @@ int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
     }
     if (cz < 0.0) cz = cz + s->box[2];
     else if (cz >= s->box[2]) cz = cz - s->box[2];
-    const double *v = vel + 3 * i;
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = vel[3 * i + d] - (vstrm ? vstrm[3 * i + d] : 0.0);
     for (int r = 0; r < nreg; r++) {
       if (!(hf->z1[r] <= cz && cz <= hf->z2[r])) continue;
       const double ke = 0.5 * mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -1385,7 +1390,8 @@ int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
   }
   for (int r = 0; r < nreg; r++)
     for (int i = 0; i < n; i++) {
-      const double *v = vel + 3 * i;
+      double v[3];
+      for (int d = 0; d < 3; d++) v[d] = vel[3 * i + d] - (vstrm ? vstrm[3 * i + d] : 0.0);
       for (int k = 0; k < PGK_NHF; k++) {
         /* Fortran (3,3,natom,nreg) per class: W(a,n) at a + 3n */
         const double *w = viri_atm + (((size_t)k * nreg + r) * n + i) * 9;

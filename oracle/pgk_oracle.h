@@ -41,9 +41,10 @@ int pgko_calforce_hf(pgko_sys *s, int32_t mts_flag, const pgk_mts *mts,
                      pgk_hf_out *hfo);
 
 /* calheatf (calheatf.F90:8-1470), volume or bulk mode, over per-atom arrays
- * in the pgk_hf_out layout: hf[nregion*PGK_NHFJ*3] (see pgk_calheatf). */
+ * in the pgk_hf_out layout: hf[nregion*PGK_NHFJ*3] (see pgk_calheatf);
+ * vstrm: the streaming velocities (atmvel_strm) or NULL. */
 int pgko_calheatf(const pgko_sys *s, const pgk_hf_desc *hf, const double *vel,
-                  const double *mass, const double *pot_atm,
+                  const double *vstrm, const double *mass, const double *pot_atm,
                   const double *viri_atm, double *hfout);
 
 /* prepewk + ewaldkp_hf (prepewk.F90:9-158, ewaldk_hf.F90:8-304): standard

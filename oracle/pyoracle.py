@@ -73,7 +73,7 @@ def _oracle_lib():
         L.pgko_ewaldk_hf.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double,
                                      C.c_int32] + [C.c_void_p] * 6
         L.pgko_calheatf.restype = C.c_int
-        L.pgko_calheatf.argtypes = [C.c_void_p, C.POINTER(abi.HfDesc)] + [C.c_void_p] * 5
+        L.pgko_calheatf.argtypes = [C.c_void_p, C.POINTER(abi.HfDesc)] + [C.c_void_p] * 6
         L.pgko_fft3d.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         _lib = L
     return _lib
@@ -236,16 +236,19 @@ class Oracle:
                     virt=vt.reshape(3, 3).T.copy(), pot_atm=pe,
                     viri_atm=wv.reshape(n, 3, 3))
 
-    def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
+    def calheatf(self, hf, vel, mass, pot_atm, viri_atm, vstrm=None):
         """calheatf over calforce_hf's per-atom arrays (as returned by
-        calforce_hf); returns hf[nregion, PGK_NHFJ, 3]."""
+        calforce_hf), with streaming velocities vstrm [natom, 3] (atmvel_strm)
+        or none; returns hf[nregion, PGK_NHFJ, 3]."""
         hs, keep = _hf_struct(self.sysd, hf)
         v = np.ascontiguousarray(vel, np.float64)
+        vs = None if vstrm is None else np.ascontiguousarray(vstrm, np.float64)
         m = np.ascontiguousarray(mass, np.float64)
         pe = np.ascontiguousarray(pot_atm, np.float64)
         wv = np.ascontiguousarray(viri_atm, np.float64)
         out = np.zeros((hs.nregion, abi.NHFJ, 3))
-        rc = self.L.pgko_calheatf(self.h, C.byref(hs), v.ctypes.data, m.ctypes.data,
+        rc = self.L.pgko_calheatf(self.h, C.byref(hs), v.ctypes.data,
+                                  None if vs is None else vs.ctypes.data, m.ctypes.data,
                                   pe.ctypes.data, wv.ctypes.data, out.ctypes.data)
         if rc:
             raise RuntimeError(f"oracle calheatf failed: {rc}")
@@ -543,9 +546,10 @@ class Reference:
                     virt=vt.reshape(3, 3).T.copy(), pot_atm=pe,
                     viri_atm=wv.reshape(n, 3, 3))
 
-    def calheatf(self, hf, vel, mass, pot_atm, viri_atm):
+    def calheatf(self, hf, vel, mass, pot_atm, viri_atm, vstrm=None):
         """The reference's calheatf (volume-based) on the given per-atom
-        arrays; returns hf[nregion, PGK_NHFJ, 3]."""
+        arrays, with atmvel = vel and atmvel_strm = vstrm (zero when None);
+        returns hf[nregion, PGK_NHFJ, 3]."""
         z1 = np.ascontiguousarray(hf["z1"], np.float64)
         z2 = np.ascontiguousarray(hf["z2"], np.float64)
         nreg = len(z1)
@@ -558,11 +562,12 @@ class Reference:
         m = np.ascontiguousarray(mass, np.float64)
         pe = np.ascontiguousarray(pot_atm, np.float64)
         wv = np.ascontiguousarray(viri_atm, np.float64)
+        vs = np.ascontiguousarray(np.zeros_like(v) if vstrm is None else vstrm, np.float64)
         out = np.zeros((nreg, abi.NHFJ, 3))
         _big_stack_call(self.L.pgkref_calheatf, C.c_int32(nreg), C.c_void_p(z1.ctypes.data),
                         C.c_void_p(z2.ctypes.data), C.c_void_p(ht.ctypes.data),
                         C.c_int32(mc.shape[1]), C.c_void_p(mc.ctypes.data),
                         C.c_void_p(m.ctypes.data), C.c_void_p(v.ctypes.data),
-                        C.c_void_p(pe.ctypes.data), C.c_void_p(wv.ctypes.data),
-                        C.c_void_p(out.ctypes.data))
+                        C.c_void_p(vs.ctypes.data), C.c_void_p(pe.ctypes.data),
+                        C.c_void_p(wv.ctypes.data), C.c_void_p(out.ctypes.data))
         return out

@@ -455,12 +455,12 @@ contains
   ! restores it afterwards.
   ! hf(3,18,nreg): kin, pot bo an to 14 elin ljin elc lj, viri (same
   ! order), tot.
-  subroutine pgkref_calheatf(nreg, z1, z2, hftyp, nmol, molecom, mass, vel, &
+  subroutine pgkref_calheatf(nreg, z1, z2, hftyp, nmol, molecom, mass, vel, strm, &
        &                     pe, wv, hf) bind(C, name='pgkref_calheatf')
     integer(c_int), value :: nreg, nmol
     integer(c_int), intent(in) :: hftyp(*)
     real(c_double), intent(in) :: z1(nreg), z2(nreg), molecom(3,nmol)
-    real(c_double), intent(in) :: mass(natom), vel(3,natom)
+    real(c_double), intent(in) :: mass(natom), vel(3,natom), strm(3,natom)
     real(c_double), intent(in) :: pe(natom,8), wv(3,3,natom,nreg,8)
     real(c_double), intent(out) :: hf(3,18,nreg)
 
@@ -484,7 +484,7 @@ contains
     com_hf = 0.0d0
     atmmass(1:natom) = mass(1:natom)
     atmvel(1:3,1:natom) = vel(1:3,1:natom)
-    atmvel_strm(1:3,1:natom) = 0.0d0
+    atmvel_strm(1:3,1:natom) = strm(1:3,1:natom)
     ! molept lists: the atoms of each molecule in index order
     nmole = maxval(irmolept_list(1:natom))
     k = 1
@@ -527,6 +527,7 @@ contains
        end do
        hf(:,18,k) = tot(:,k)
     end do
+    atmvel_strm(1:3,1:natom) = 0.0d0
     deallocate(z, zx, w0, wx, hp, hv, hz, hx, hkx, hpx, hvx, tot, cor_hf, com_hf, typ_save)
   end subroutine pgkref_calheatf
 

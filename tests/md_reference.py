@@ -85,13 +85,15 @@ def run(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, lang=None):
     rows = []
 
     def energies(r):
-        ke = 0.5 * (mass[:, None] * vel ** 2).sum() / MD_CONV
+        # kinetic energy relative to the streaming velocities (calkin.F90:98)
+        u = vel - lang.vstrm if lang else vel
+        ke = 0.5 * (mass[:, None] * u ** 2).sum() / MD_CONV
         pe = r["ene"][abi.ENE_NAMES.index("pot_tot")]
         return [ke, pe, ke + pe]
 
-    rows.append(energies(r))
     if lang:
         lang.register(pos)
+    rows.append(energies(r))
     for _ in range(nstep):
         if lang:
             lang.half(vel, True)
@@ -192,15 +194,22 @@ class Nhc:
 
 
 def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=None,
-              record=None, lang=None):
+              record=None, lang=None, strm=None):
     """nstep NVT steps in the order of moldyn_nhc.F90 (nhcint, thermostatted
-    long-force velocity update, drift, rattle_c, forces, velocity update,
-    rattlep_v, nhcint) with one short step per long step.  force(pos) returns
-    (F in eref/A, PE in eref); shake_fn(pos, vel, old) and rattle_fn(pos, vel)
-    apply the constraints in place; record(pos, vel, pe) is called at step 0
-    and after every step.  Returns the final (pos, vel)."""
+    long-force velocity update, drift, rattle_c, streaming velocities,
+    forces, velocity update, rattlep_v, nhcint) with one short step per long
+    step.  force(pos) returns (F in eref/A, PE in eref); shake_fn(pos, vel,
+    old) and rattle_fn(pos, vel) apply the constraints in place; record(pos,
+    vel, pe) is called at step 0 and after every step.  strm: an object whose
+    register(pos) assigns its per-atom streaming velocities .vstrm
+    (ass_strmvel.F90, e.g. a Langevin with streaming slabs): the chain's
+    kinetic energy and half kicks then act on v - vstrm
+    (moldyn_nhc.F90:1280-1310, 1412, nhcint.F90:53-54).  Returns the final
+    (pos, vel)."""
     mass = 1.0 / minv
     f, pe = force(pos)
+    if strm:
+        strm.register(pos)
     if record:
         record(pos, vel, pe)
     # with a region Langevin thermostat (lang) the chain acts on the atoms
@@ -209,12 +218,16 @@ def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=No
     if lang:
         lang.register(pos)
 
+    def vs():
+        return strm.vstrm if strm else 0.0
+
     def kint():
-        return (mass[chain, None] * vel[chain] * vel[chain]).sum() / MD_CONV
+        u = (vel - vs())[chain]
+        return (mass[chain, None] * u * u).sum() / MD_CONV
 
     def kick(vel, f, s2, bbv):
         acc = f * MD_CONV * minv[:, None]
-        out = vel * s2 + acc * bbv
+        out = ((vel - vs()) * s2 + acc * bbv) + vs() if strm else vel * s2 + acc * bbv
         if lang:
             plain = vel + 0.5 * dt * MD_CONV * minv[:, None] * f
             out[~chain] = plain[~chain]
@@ -231,6 +244,8 @@ def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=No
         pos = pos + dt * vel
         if shake_fn:
             shake_fn(pos, vel, old)
+        if strm:
+            strm.register(pos)
         f, pe = force(pos)
         vel = kick(vel, f, s2, bbv)
         if rattle_fn:
@@ -245,7 +260,7 @@ def nhc_steps(force, pos, vel, minv, dt, nstep, nhc, shake_fn=None, rattle_fn=No
 
 
 def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, record=None,
-            lang=None):
+            lang=None, strm=None):
     """nhc_steps on the oracle's forces with the rigid-water constraints
     (the list built once at the start, nstep < nstep_book)."""
     mass = np.asarray(sysd["mass"], np.float64)
@@ -265,7 +280,8 @@ def run_nhc(oracle, sysd, vel, cons_ij, cons_d, group_start, dt, nstep, nhc, rec
 
     return nhc_steps(force, pos, vel, minv, dt, nstep, nhc,
                      lambda p, v, o: shake(p, v, o, minv, cons_ij, d2, groups, dt),
-                     lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record, lang)
+                     lambda p, v: rattle_v(p, v, minv, cons_ij, d2, groups), record, lang,
+                     strm)
 
 
 def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, nstep, mts,
@@ -295,13 +311,14 @@ def run_respa(oracle, sysd, vel, cons_ij, cons_d, group_start, dt_long, nshort, 
     def row():
         # each call's pot_tot holds the terms it evaluated plus the Ewald self
         # energy (a constant), which the step counts once
-        ke = 0.5 * (mass[:, None] * vel ** 2).sum() / MD_CONV
+        u = vel - lang.vstrm if lang else vel
+        ke = 0.5 * (mass[:, None] * u ** 2).sum() / MD_CONV
         pe = rl["ene"][itot] + (rs["ene"][itot] - rs["ene"][iewc])
         return [ke, pe, ke + pe]
 
-    rows = [row()]
     if lang:
         lang.register(pos)
+    rows = [row()]
     for _ in range(nstep):
         if lang:
             lang.half(vel, True)
@@ -361,7 +378,8 @@ class Langevin:
     def __init__(self, regions, temperature, damp, molid, mass, box, dt, mol_com=None, seed=0,
                  stream=None):
         import math
-        self.reg_box = np.atleast_2d(np.asarray(regions, np.float64))
+        rb = np.asarray(regions, np.float64)
+        self.reg_box = rb.reshape(0, 6) if rb.size == 0 else np.atleast_2d(rb)
         nr = self.reg_box.shape[0]
         self.temp = np.broadcast_to(np.asarray(temperature, np.float64), (nr,))
         self.damp = np.broadcast_to(np.asarray(damp, np.float64), (nr,))

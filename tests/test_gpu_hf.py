@@ -165,6 +165,19 @@ def test_calheatf_matches_oracle(oracle_mod, prec, mode):
     c.calforce_hf(hf, per_atom=False)
     assert np.array_equal(c.calheatf(vel), j) if prec == "mixed" else \
         np.max(np.abs(c.calheatf(vel) - j)) <= 1e-12 * scale
+    # streaming velocities (atmvel_strm): the kinetic energies in their frame,
+    # against the oracle given the same (pinned to the compiled reference)
+    molid = np.asarray(s["molid"]) - 1
+    vs = rng.normal(scale=0.01, size=(int(molid.max()) + 1, 3))[molid]
+    c.set_atmvel_strm(vs)
+    js = c.calheatf(vel)
+    c.set_atmvel_strm(None)
+    same_s = o.calheatf(hf, vel, mass, got["pot_atm"],
+                        got["viri_atm"].reshape(abi.NHF, 4, -1, 9), vstrm=vs)
+    assert np.max(np.abs(js - same_s)) <= 1e-12 * np.max(np.abs(same_s))
+    kin = abi.HFJ_KIN
+    assert np.max(np.abs(js[:, kin] - j[:, kin])) > 1e-3 * np.max(np.abs(j[:, kin]))
+    assert np.max(np.abs(c.calheatf(vel) - j)) <= 1e-12 * scale   # cleared again
 
 
 def test_calheatf_order_of_calls():

@@ -33,7 +33,8 @@ def _regions(s):
                      [0.0, 0.5 * lx, 0.0, ly, 0.0, lz]])
 
 
-@pytest.mark.parametrize("by_com,streaming", [(True, False), (False, False), (True, True)])
+@pytest.mark.parametrize("by_com,streaming",
+                         [(True, False), (False, False), (True, True), (False, True)])
 def test_langevin_matches_reference(oracle_mod, by_com, streaming):
     s = water648()
     dt = 1.0
@@ -65,10 +66,13 @@ def test_langevin_matches_reference(oracle_mod, by_com, streaming):
     assert np.max(np.abs(c2.md_state()[1] - v)) > 1e-4
 
 
-def test_langevin_respa_matches_reference(oracle_mod):
+@pytest.mark.parametrize("streaming", [False, True])
+def test_langevin_respa_matches_reference(oracle_mod, streaming):
     """RESPA (3 inner steps, bonded terms short) on the water/propanol slab
     with the thermostat, waters registered by centre of mass and propanol
-    atom by atom (the reference's iftcratom split)."""
+    atom by atom (the reference's iftcratom split); with streaming slabs, the
+    thermostat and the kinetic-energy rows act on v minus the streaming
+    velocity."""
     from helpers import small_slab
     from thrid_amd import abi
     from thrid_amd.pgk import ForceContext
@@ -85,12 +89,15 @@ def test_langevin_respa_matches_reference(oracle_mod):
     c.md_respa(3, mts)
     c.md_set_velocities(v0)
     reg, temp, damp = _regions(s), [250.0, 400.0, 320.0], [0.02, 0.05, 0.01]
-    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=77)
+    lz = float(s["box"][2])
+    stream = ([0.1 * lz, 0.5 * lz], [0.5 * lz, 0.9 * lz],
+              [[0.003, 0.0, -0.001], [-0.002, 0.001, 0.0]]) if streaming else None
+    c.md_langevin(reg, temp, damp, mol_com=mol_com, seed=77, stream=stream)
     rows = c.md_run(3, 20, s["rcut_book"], 1)
     p, v = c.md_state()
     det, cnt = c.md_langevin_state()
     lt = md_reference.Langevin(reg, temp, damp, molid, s["mass"], s["box"], 1.5,
-                               mol_com=mol_com, seed=77)
+                               mol_com=mol_com, seed=77, stream=stream)
     rp, rv, rrows = md_reference.run_respa(oracle_mod.Oracle(s), s, v0, ij, d, gs, 1.5, 3, 3,
                                            mts, lang=lt)
     assert np.max(np.abs(p - rp)) < 1e-9
@@ -234,3 +241,91 @@ def test_langevin_ends_with_chain_middle_matches_reference(oracle_mod):
     assert np.max(np.abs(x - nhc.x)) < 1e-9 * max(np.max(np.abs(nhc.x)), 1e-12)
     np.testing.assert_allclose(det, lt.det, rtol=1e-8, atol=1e-9 * np.abs(lt.det).max())
     assert list(cnt) == [int(np.sum(lt.reg == k)) for k in range(2)]
+
+
+def _stream_slabs(s):
+    lz = float(s["box"][2])
+    return ([0.0, 0.45 * lz], [0.45 * lz, 0.9 * lz],
+            [[0.004, 0.0, 0.0], [-0.003, 0.002, 0.0]])
+
+
+def test_chain_with_streaming_slabs_matches_reference(oracle_mod):
+    """Nose-Hoover chain with streaming slabs and no region thermostat: the
+    chain's kinetic energy and half kicks act on v minus each atom's
+    streaming velocity, assigned after every drift (moldyn_nhc.F90:1280-1310,
+    1412; nhcint.F90:53-54); the sampled kinetic energy is taken in the same
+    frame (calkin.F90:98).  5 FP64 steps against the NumPy restatement."""
+    s = water648()
+    dt = 1.0
+    v0 = md.maxwell_boltzmann(s, 300.0, seed=41)
+    c, (ij, d, gs) = _ctx(s, "fp64", dt)
+    c.md_set_velocities(v0)
+    nfree = 3 * s["natom"] - len(d) - 3
+    stream = _stream_slabs(s)
+    c.md_langevin([], stream=stream)
+    c.md_nhc(250.0, 0.01, nfree, mchain=4, nyosh=3, next=2)
+    rows = c.md_run(5, 20, s["rcut_book"], 1)
+    p, v = c.md_state()
+    x, vx = c.md_nhc_state()
+    mass = np.asarray(s["mass"], np.float64)
+    strm = md_reference.Langevin(np.zeros((0, 6)), [], [], s["molid"], s["mass"], s["box"], dt,
+                                 stream=stream)
+    nhc = md_reference.Nhc(4, 250.0, 0.01, nfree, 3, 2)
+    rrows = []
+
+    def record(pos, vel, pe):
+        u = vel - strm.vstrm
+        ke = 0.5 * (mass[:, None] * u * u).sum() / md.MD_CONV
+        rrows.append([ke, pe, ke + pe + nhc.conserved_extra()])
+
+    rp, rv = md_reference.run_nhc(oracle_mod.Oracle(s), s, v0, ij, d, gs, dt, 5, nhc,
+                                  record=record, strm=strm)
+    assert 0 < np.count_nonzero(strm.vstrm[:, 0]) < s["natom"]
+    assert np.max(np.abs(p - rp)) < 1e-9
+    assert np.max(np.abs(v - rv)) < 1e-9 * np.max(np.abs(rv))
+    assert np.max(np.abs(vx - nhc.v)) < 1e-9 * np.max(np.abs(nhc.v))
+    assert np.max(np.abs(x - nhc.x)) < 1e-9 * np.max(np.abs(nhc.x))
+    np.testing.assert_allclose(rows, np.array(rrows), rtol=1e-9, atol=1e-9)
+    # the slabs did matter: the lab-frame chain gives another trajectory
+    nhc2 = md_reference.Nhc(4, 250.0, 0.01, nfree, 3, 2)
+    _p2, rv2 = md_reference.run_nhc(oracle_mod.Oracle(s), s, v0, ij, d, gs, dt, 5, nhc2)
+    assert np.max(np.abs(rv2 - rv)) > 1e3 * 1e-9 * np.max(np.abs(rv))
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_heatf_series_in_streaming_frame(prec):
+    """With streaming slabs, the MD loop's heat-flux rows take the atoms'
+    kinetic energies and the collision part relative to their streaming
+    velocities (calheatf.F90:368-369): the last row equals the fused call
+    given the same streaming velocities through set_atmvel_strm, and differs
+    from the call without them."""
+    from thrid_amd import abi
+    s = water648()
+    c, _ = _ctx(s, prec, 1.0)
+    c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=43))
+    stream = _stream_slabs(s)
+    c.md_langevin([], stream=stream)
+    lz = s["box"][2]
+    z = np.linspace(0.0, lz, 5)
+    hf = dict(mode=abi.HF_VOLUME, z1=z[:-1], z2=z[1:])
+    c.md_heatf(hf, 4)
+    rows = c.md_run(8, 20, s["rcut_book"], 4)
+    series = c.md_heatf_series()
+    p1, v1 = c.md_state()
+    strm = md_reference.Langevin(np.zeros((0, 6)), [], [], s["molid"], s["mass"], s["box"], 1.0,
+                                 stream=stream)
+    strm.register(p1)
+    m = s["mass"] / md.MD_CONV
+    lab = c.calforce_heatf(hf, v1, m)["hf"]
+    c.set_atmvel_strm(strm.vstrm)
+    want = c.calforce_heatf(hf, v1, m)["hf"]
+    c.set_atmvel_strm(None)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(series[-1] - want)) <= 1e-11 * scale
+    kin = abi.HFJ_KIN
+    assert np.max(np.abs(lab[:, kin] - want[:, kin])) > 1e-3 * np.max(np.abs(want[:, kin]))
+    u = v1 - strm.vstrm
+    ke = 0.5 * (np.asarray(s["mass"])[:, None] * u * u).sum() / md.MD_CONV
+    assert abs(rows[-1][0] - ke) <= 1e-9 * ke
+    c.close()

@@ -54,6 +54,43 @@ def test_oracle_calheatf_matches_reference(oracle_mod):
     np.testing.assert_array_equal(a, b)
 
 
+def test_reference_calheatf_subtracts_streaming_velocity(oracle_mod):
+    """With atmvel_strm set, calheatf works with u = v - vs
+    (calheatf.F90:368-369) for each atom's kinetic energy m u^2 / 2
+    (:677-681) and in the collision part (:1027-1330), while the centre
+    velocities it transports stay lab-frame (atmvel_tmp, :564, 588, 644):
+    the restatement given the same vs agrees bit for bit; the potential
+    transport terms equal the call without vs; giving v - vs for
+    everything (the simpler reading) is wrong."""
+    if not oracle_mod.reference_available():
+        pytest.skip("reference library not built (oracle/build_ref.sh)")
+    s = small_slab("rb")
+    lz = s["box"][2]
+    hf = dict(mode=abi.HF_VOLUME, z1=np.array([0.0, 0.3 * lz, 0.45 * lz]),
+              z2=np.array([0.5 * lz, 0.6 * lz, lz]), hftyp=_mixed_centres(s),
+              molecom=_molecom(s))
+    ref = oracle_mod.Reference(s)
+    ref.mklist(s["rcut_book"], 1)
+    r = ref.calforce_hf(hf)
+    vel = _velocities(s)
+    mass = s["mass"] * 0.37
+    rng = np.random.default_rng(8)
+    molid = np.asarray(s["molid"]) - 1
+    vs = rng.normal(scale=0.01, size=(int(molid.max()) + 1, 3))[molid]
+    a = ref.calheatf(hf, vel, mass, r["pot_atm"], r["viri_atm"], vstrm=vs)
+    o = oracle_mod.Oracle(s)
+    np.testing.assert_array_equal(a, o.calheatf(hf, vel, mass, r["pot_atm"], r["viri_atm"],
+                                                vstrm=vs))
+    lab = ref.calheatf(hf, vel, mass, r["pot_atm"], r["viri_atm"])
+    kin, pot = abi.HFJ_KIN, list(range(abi.HFJ_POT, abi.HFJ_VIRI))
+    np.testing.assert_array_equal(a[:, pot], lab[:, pot])
+    for t in [kin] + list(range(abi.HFJ_VIRI, abi.HFJ_TOT)):
+        if np.max(np.abs(lab[:, t])) > 0:
+            assert np.max(np.abs(a[:, t] - lab[:, t])) > 1e-6 * np.max(np.abs(lab[:, t])), t
+    wrong = ref.calheatf(hf, vel - vs, mass, r["pot_atm"], r["viri_atm"])
+    assert np.max(np.abs(wrong[:, pot] - a[:, pot])) > 1e-3 * np.max(np.abs(a[:, pot]))
+
+
 def _oracle_hf(oracle_mod, s, hf):
     o = oracle_mod.Oracle(s)
     o.mklist(s["rcut_book"], 1)

@@ -5,8 +5,10 @@
 // Per region r, with the centre of atom i (the atom, or its molecule's
 // centre of mass for hftyp MOLE, z wrapped once into [0, Lz)) and the
 // velocity of that centre (the atom's, or the molecule's mass-weighted mean):
-//   transport   sum_{z1 <= z_i <= z2} (m_i v_i^2 / 2) vc_i   and   e_c(i) vc_i
-//   collision   sum_i sum_a W_c(a, n, i, r) v_i(a)
+//   transport   sum_{z1 <= z_i <= z2} (m_i u_i^2 / 2) vc_i   and   e_c(i) vc_i
+//   collision   sum_i sum_a W_c(a, n, i, r) u_i(a)
+// with u_i = v_i minus the atom's streaming velocity (zero without one) and
+// vc_i lab-frame (calheatf.F90:368-369, 564, 644, 677-681, 1027-1330)
 // each times 1 / (Lx Ly (z2 - z1)), and their total in the reference's order.
 // The volume form has only z components (W holds column n = 3); the bulk
 // form all three.
@@ -29,6 +31,8 @@ struct HeatfArgs {
   const double *z1, *z2;
   const int32_t *hftyp, *molid;
   const double *molecom, *pos, *vel, *mass, *molevel;
+  const double *vstrm;   // streaming velocities for the kinetic energies, or null
+  const double *cvel;    // collision-part velocities: vel - vstrm, or vel
   const void *pe, *w;
   double L[3];
 };
@@ -122,7 +126,11 @@ __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__
     if (cz < 0.0) cz = cz + a.L[2];
     else if (cz >= a.L[2]) cz = cz - a.L[2];
     if (!(z1 <= cz && cz <= z2)) continue;
-    const double *v = a.vel + 3 * i;
+    // kinetic energy relative to the streaming velocity (calheatf.F90:368-369,
+    // 677-681); the transported centre velocity stays lab-frame
+    double v[3] = {a.vel[3 * i], a.vel[3 * i + 1], a.vel[3 * i + 2]};
+    if (a.vstrm)
+      for (int d = 0; d < 3; d++) v[d] = v[d] - a.vstrm[3 * i + d];
     const double ke = 0.5 * a.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const double vc[3] = {a.bulk ? pv[0] : 0.0, a.bulk ? pv[1] : 0.0, pv[2]};
 #pragma unroll
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(256) void k_heatf_collision(HeatfArgs a, double *__
   const AH *W = reinterpret_cast<const AH *>(a.w) + (size_t)row * a.natom * a.nw;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
-    const double v0 = a.vel[3 * i], v1 = a.vel[3 * i + 1], v2 = a.vel[3 * i + 2];
+    const double v0 = a.cvel[3 * i], v1 = a.cvel[3 * i + 1], v2 = a.cvel[3 * i + 2];
     const AH *w = W + (size_t)i * a.nw;
     if (a.nw == 9) {
 #pragma unroll
@@ -223,6 +231,14 @@ __global__ __launch_bounds__(256) void k_heatf_collect(int nreg, int bulk, int n
   if (threadIdx.x == 0) pcol[(size_t)row * 3 + c] = jcol[(size_t)row * 3 + c] + rec[0];
 }
 
+// velocities relative to the streaming velocities
+__global__ __launch_bounds__(256) void k_peculiar(int n3, const double *__restrict__ vel,
+                                                  const double *__restrict__ vstrm,
+                                                  double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n3) out[k] = vel[k] - vstrm[k];
+}
+
 template <int PREC>
 int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   using AH = typename HfAcc<PREC>::type;
@@ -241,10 +257,28 @@ int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
 
 }  // namespace
 
+// The velocities of the collision part: d_vel, or (with streaming
+// velocities) d_vel - d_vstrm queued into the context's buffer.
+int pgk_heatf_collision_vel(pgk_ctx *c, const double *d_vel, const double *d_vstrm,
+                            const double **out) {
+  if (!d_vstrm) {
+    *out = d_vel;
+    return PGK_OK;
+  }
+  const int n3 = 3 * c->natom;
+  int rc;
+  if ((rc = grow_dev((void **)&c->d_hj_vpec, &c->hj_vpec_cap, sizeof(double) * n3))) return rc;
+  k_peculiar<<<pgk_grid_blocks(n3, 256), 256, 0, c->stream>>>(n3, d_vel, d_vstrm, c->d_hj_vpec);
+  HIP_CHECK(hipGetLastError());
+  *out = c->d_hj_vpec;
+  return PGK_OK;
+}
+
 // Queue calheatf over the context's heat-flux arrays; velocities and masses
 // are already in d_vel / d_mass (device, natom entries), the result goes to
 // c->d_hj_out.
-int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
+int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
+                     const double *d_vstrm) {
   const HfDev &h = c->hfdev;
   const int n = c->natom;
   HeatfArgs a;
@@ -266,6 +300,8 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
   a.molecom = h.molecom;
   a.pos = h.pos;
   a.vel = d_vel;
+  a.vstrm = d_vstrm;
+  a.cvel = d_vel;
   a.mass = d_mass;
   a.molevel = c->d_hj_molevel;
   a.pe = h.pe;
@@ -274,6 +310,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass) {
   const size_t ntr = (size_t)h.nreg * nsplit * kNterm * 3;
   const size_t ncol = (size_t)rows * a.nsplit_col * 3;
   int rc;
+  if (!h.flux && (rc = pgk_heatf_collision_vel(c, d_vel, d_vstrm, &a.cvel))) return rc;
   if ((rc = grow_dev((void **)&c->d_hj_part, &c->hj_part_cap, sizeof(double) * (ntr + ncol))) ||
       (rc = grow_dev((void **)&c->d_hj_out, &c->hj_out_cap,
                      sizeof(double) * (size_t)h.nreg * PGK_NHFJ * 3)))

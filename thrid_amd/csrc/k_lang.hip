@@ -58,15 +58,12 @@ __device__ __forceinline__ bool lt_in(double g, double lo, double hi, double l) 
   return (lo <= g && g < hi) || (lo <= g - l && g - l < hi);
 }
 
-// one thread per registration unit: the unit's point (centre of mass or the
-// atom), wrapped, against the regions in order; the first that holds it
-// takes every atom of the unit (langevin_regist.F90:112-140)
-__global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__restrict__ pos) {
+// the point of registration unit u (its centre of mass or its one atom),
+// each coordinate wrapped once into [0, L)
+__device__ __forceinline__ void lt_unit_point(const LtArgs &a, const double *__restrict__ pos,
+                                              int u, double (&g)[3]) {
 #pragma clang fp contract(off)
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= a.nunit) return;
   const int s = a.ustart[u], e = a.ustart[u + 1];
-  double g[3];
   if (a.ucom[u]) {
     double mm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     for (int j = s; j < e; j++) {
@@ -87,6 +84,24 @@ __global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__r
     g[2] = pos[3 * i + 2];
   }
   for (int d = 0; d < 3; d++) g[d] = lt_wrap(g[d], a.box[d]);
+}
+
+// the streaming slab of a wrapped z (ass_strmvel.F90: z only, first match)
+__device__ __forceinline__ int lt_strm_slab(const LtArgs &a, double z) {
+  for (int k = 0; k < a.nstrm; k++)
+    if (a.sz[0][k] <= z && z < a.sz[1][k]) return k;
+  return -1;
+}
+
+// one thread per registration unit: the unit's point against the regions in
+// order; the first that holds it takes every atom of the unit
+// (langevin_regist.F90:112-140)
+__global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__restrict__ pos) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= a.nunit) return;
+  const int s = a.ustart[u], e = a.ustart[u + 1];
+  double g[3];
+  lt_unit_point(a, pos, u, g);
   int r = -1;
   for (int k = 0; k < a.nreg; k++) {
     if (lt_in(g[0], a.lo[0][k], a.hi[0][k], a.box[0]) &&
@@ -96,17 +111,25 @@ __global__ __launch_bounds__(256) void k_lt_register(LtArgs a, const double *__r
       break;
     }
   }
-  // streaming slab of the same point (ass_strmvel.F90: z only, first match)
-  int sr = -1;
-  for (int k = 0; k < a.nstrm; k++) {
-    if (a.sz[0][k] <= g[2] && g[2] < a.sz[1][k]) {
-      sr = k;
-      break;
-    }
-  }
+  const int sr = lt_strm_slab(a, g[2]);
   for (int j = s; j < e; j++) {
     a.reg[a.uatom[j]] = r;
     a.sreg[a.uatom[j]] = sr;
+  }
+}
+
+// ass_strmvel (ass_strmvel.F90:8-295): every atom of a unit whose point lies
+// in streaming slab k takes strmvel(k), the others zero; one thread per unit
+__global__ __launch_bounds__(256) void k_strm_assign(LtArgs a, const double *__restrict__ pos,
+                                                     double *__restrict__ vstrm) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= a.nunit) return;
+  double g[3];
+  lt_unit_point(a, pos, u, g);
+  const int sr = lt_strm_slab(a, g[2]);
+  for (int j = a.ustart[u]; j < a.ustart[u + 1]; j++) {
+    const int i = a.uatom[j];
+    for (int d = 0; d < 3; d++) vstrm[3 * i + d] = sr >= 0 ? a.sv[d][sr] : 0.0;
   }
 }
 
@@ -228,6 +251,17 @@ LtArgs lt_args(pgk_ctx *c) {
 int pgk_md_lt_register(pgk_ctx *c) {
   const LtArgs a = lt_args(c);
   k_lt_register<<<pgk_grid_blocks(a.nunit, 256), 256, 0, c->stream>>>(a, c->d_pos);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+// the atoms' streaming velocities at the current positions (no-op without
+// streaming slabs)
+int pgk_md_strm_assign(pgk_ctx *c) {
+  if (!c->md_lt_nstrm) return PGK_OK;
+  const LtArgs a = lt_args(c);
+  k_strm_assign<<<pgk_grid_blocks(a.nunit, 256), 256, 0, c->stream>>>(a, c->d_pos,
+                                                                     c->d_md_vstrm);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

@@ -33,32 +33,45 @@ __device__ __forceinline__ double drift1(double x, double v, double dt) { return
 // v exp(-dt vxi1/2) + F/m bbv (moldyn_nhc.F90:1301-1302, 1589-1590); nhc
 // points at the two factors {exp(-dt vxi1/2), bbv} that k_nhc_half keeps on
 // the device, or is null (plain NVE).
+// With streaming velocities (strm), the chain scales the velocity relative to
+// the atom's streaming velocity vs and adds vs back: ((v - vs) exp(-dt
+// vxi1/2) + F/m bbv) + vs (moldyn_nhc.F90:1280-1310, 1585-1598).
 struct Kick {
-  bool nhc;
+  bool nhc, strm;
   double s2, bbv;
+  double vs[3];
 };
 __device__ __forceinline__ Kick kick_of(const double *nhc) {
-  Kick k{nhc != nullptr, 1.0, 0.0};
+  Kick k{nhc != nullptr, false, 1.0, 0.0, {0.0, 0.0, 0.0}};
   if (nhc) {
     k.s2 = nhc[0];
     k.bbv = nhc[1];
   }
   return k;
 }
-__device__ __forceinline__ double kick_nhc(double v, double f, double minv, const Kick &k) {
+__device__ __forceinline__ double kick_nhc(double v, double f, double minv, const Kick &k,
+                                           int d) {
 #pragma clang fp contract(off)
+  if (k.strm) return ((v - k.vs[d]) * k.s2 + f * PGK_MD_CONV * minv * k.bbv) + k.vs[d];
   return v * k.s2 + f * PGK_MD_CONV * minv * k.bbv;
 }
 __device__ __forceinline__ double kickv(double v, double f, double minv, double dt,
-                                        const Kick &k) {
-  return k.nhc ? kick_nhc(v, f, minv, k) : kick1(v, f, minv, dt);
+                                        const Kick &k, int d) {
+  return k.nhc ? kick_nhc(v, f, minv, k, d) : kick1(v, f, minv, dt);
 }
 // With the region Langevin thermostat on as well, the chain acts on the atoms
 // outside every Langevin region only: an atom registered in a region
 // (ltreg[i] >= 0) takes the plain kick, the region thermostat its own update.
-__device__ __forceinline__ Kick kick_at(const Kick &k, const int32_t *ltreg, int i) {
-  if (k.nhc && ltreg && ltreg[i] >= 0) return Kick{false, 1.0, 0.0};
-  return k;
+// vstrm: the atoms' streaming velocities [3*natom] under the chain, or null.
+__device__ __forceinline__ Kick kick_at(const Kick &k, const int32_t *ltreg,
+                                        const double *vstrm, int i) {
+  if (k.nhc && ltreg && ltreg[i] >= 0) return Kick{false, false, 1.0, 0.0, {0.0, 0.0, 0.0}};
+  Kick r = k;
+  if (k.nhc && vstrm) {
+    r.strm = true;
+    for (int d = 0; d < 3; d++) r.vs[d] = vstrm[3 * i + d];
+  }
+  return r;
 }
 
 // RESPA (moldyn.F90:1465-1771): the long force's half kick (dt_l) comes
@@ -72,14 +85,15 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(int n, double *__restrict
                                                        const double *__restrict__ nhc,
                                                        const double *__restrict__ force_l,
                                                        double dt_l,
-                                                       const int32_t *__restrict__ ltreg) {
+                                                       const int32_t *__restrict__ ltreg,
+                                                       const double *__restrict__ vstrm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Kick k = kick_at(kick_of(nhc), ltreg, i);
+  const Kick k = kick_at(kick_of(nhc), ltreg, vstrm, i);
   for (int d = 0; d < 3; d++) {
     double v0 = vel[3 * i + d];
     if (force_l) v0 = kick1(v0, force_l[3 * i + d], minv[i], dt_l);
-    const double v = kickv(v0, force[3 * i + d], minv[i], dt, k);
+    const double v = kickv(v0, force[3 * i + d], minv[i], dt, k, d);
     vel[3 * i + d] = v;
     pos_old[3 * i + d] = pos[3 * i + d];
     pos[3 * i + d] = drift1(pos[3 * i + d], v, dt);
@@ -91,12 +105,13 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double *__restrict__ vel
                                                  const double *__restrict__ minv, double dt,
                                                  const double *__restrict__ nhc,
                                                  const double *__restrict__ force_l, double dt_l,
-                                                 const int32_t *__restrict__ ltreg) {
+                                                 const int32_t *__restrict__ ltreg,
+                                                 const double *__restrict__ vstrm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Kick k = kick_at(kick_of(nhc), ltreg, i);
+  const Kick k = kick_at(kick_of(nhc), ltreg, vstrm, i);
   for (int d = 0; d < 3; d++) {
-    double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k);
+    double v = kickv(vel[3 * i + d], force[3 * i + d], minv[i], dt, k, d);
     if (force_l) v = kick1(v, force_l[3 * i + d], minv[i], dt_l);
     vel[3 * i + d] = v;
   }
@@ -118,6 +133,7 @@ struct ConsArgs {
   const double *force_l;   // RESPA: long force of this inner step's extra kick, or null
   double dt_l;             // RESPA: its time step (the outer one)
   const int32_t *ltreg;    // chain + region Langevin: region of each atom (-1 none), or null
+  const double *vstrm;     // chain + streaming slabs: streaming velocity of each atom, or null
 };
 
 // Triangle groups (three atoms held by three constraints: the rigid 3-site
@@ -243,9 +259,9 @@ __device__ __forceinline__ void kick_drift3(const double *force, const double *m
     v.y = kick1(v.y, force_l[3 * i + 1], m, dt_l);
     v.z = kick1(v.z, force_l[3 * i + 2], m, dt_l);
   }
-  v.x = kickv(v.x, force[3 * i], m, dt, k);
-  v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
-  v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
+  v.x = kickv(v.x, force[3 * i], m, dt, k, 0);
+  v.y = kickv(v.y, force[3 * i + 1], m, dt, k, 1);
+  v.z = kickv(v.z, force[3 * i + 2], m, dt, k, 2);
   p.x = drift1(p.x, v.x, dt);
   p.y = drift1(p.y, v.y, dt);
   p.z = drift1(p.z, v.z, dt);
@@ -254,9 +270,9 @@ __device__ __forceinline__ void kick3(const double *force, const double *minv, i
                                       const Kick &k, double3 &v,
                                       const double *force_l = nullptr, double dt_l = 0.0) {
   const double m = minv[i];
-  v.x = kickv(v.x, force[3 * i], m, dt, k);
-  v.y = kickv(v.y, force[3 * i + 1], m, dt, k);
-  v.z = kickv(v.z, force[3 * i + 2], m, dt, k);
+  v.x = kickv(v.x, force[3 * i], m, dt, k, 0);
+  v.y = kickv(v.y, force[3 * i + 1], m, dt, k, 1);
+  v.z = kickv(v.z, force[3 * i + 2], m, dt, k, 2);
   if (force_l) {
     v.x = kick1(v.x, force_l[3 * i], m, dt_l);
     v.y = kick1(v.y, force_l[3 * i + 1], m, dt_l);
@@ -278,9 +294,12 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
     double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
     double3 o0, o1, o2;
-    kick_drift3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, i0), p0, v0, o0, a.force_l, a.dt_l);
-    kick_drift3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, i1), p1, v1, o1, a.force_l, a.dt_l);
-    kick_drift3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, i2), p2, v2, o2, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, a.vstrm, i0), p0,
+                v0, o0, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, a.vstrm, i1), p1,
+                v1, o1, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, a.vstrm, i2), p2,
+                v2, o2, a.force_l, a.dt_l);
     const bool done = shake_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2, o0, o1, o2);
     st3(a.pos, i0, p0);
     st3(a.pos, i1, p1);
@@ -292,7 +311,8 @@ __global__ __launch_bounds__(64) void k_md_first_half(ConsArgs a, const double *
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 p = ld3(a.pos, i), v = ld3(a.vel, i), o;
-    kick_drift3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, i), p, v, o, a.force_l, a.dt_l);
+    kick_drift3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, a.vstrm, i), p,
+                v, o, a.force_l, a.dt_l);
     st3(a.pos, i, p);
     st3(a.vel, i, v);
   }
@@ -310,9 +330,9 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
     const int i0 = a.cij[2 * c0], i1 = a.cij[2 * c0 + 1], i2 = a.cij[2 * c0 + 3];
     const double3 p0 = ld3(a.pos, i0), p1 = ld3(a.pos, i1), p2 = ld3(a.pos, i2);
     double3 v0 = ld3(a.vel, i0), v1 = ld3(a.vel, i1), v2 = ld3(a.vel, i2);
-    kick3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, i0), v0, a.force_l, a.dt_l);
-    kick3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, i1), v1, a.force_l, a.dt_l);
-    kick3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, i2), v2, a.force_l, a.dt_l);
+    kick3(force, a.minv, i0, a.dt, kick_at(k, a.ltreg, a.vstrm, i0), v0, a.force_l, a.dt_l);
+    kick3(force, a.minv, i1, a.dt, kick_at(k, a.ltreg, a.vstrm, i1), v1, a.force_l, a.dt_l);
+    kick3(force, a.minv, i2, a.dt, kick_at(k, a.ltreg, a.vstrm, i2), v2, a.force_l, a.dt_l);
     const bool done = rattle_triangle(a, c0, i0, i1, i2, p0, p1, p2, v0, v1, v2);
     st3(a.vel, i0, v0);
     st3(a.vel, i1, v1);
@@ -321,7 +341,7 @@ __global__ __launch_bounds__(64) void k_md_second_half(ConsArgs a,
   } else if (g < a.ngroup + nfree) {
     const int i = free_atoms[g - a.ngroup];
     double3 v = ld3(a.vel, i);
-    kick3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, i), v, a.force_l, a.dt_l);
+    kick3(force, a.minv, i, a.dt, kick_at(k, a.ltreg, a.vstrm, i), v, a.force_l, a.dt_l);
     st3(a.vel, i, v);
   }
 }
@@ -401,21 +421,27 @@ __global__ __launch_bounds__(64) void k_rattle_v(ConsArgs a) {
   if (!done) atomicOr(a.fail, 2);
 }
 
-// kinetic energy records: per block 0.5 sum m v^2 (fixed-order tree); with
-// skip, the atoms registered in a Langevin region are left out
+// kinetic energy records: per block 0.5 sum m v^2 (fixed-order tree), v
+// relative to the streaming velocity when vstrm is given (calkin.F90:98,
+// nhcint.F90:53-54); with skip, the atoms registered in a Langevin region
+// are left out
 __global__ __launch_bounds__(256) void k_md_ke(int n, const double *__restrict__ vel,
                                                const double *__restrict__ minv,
                                                double *__restrict__ rec,
-                                               const int32_t *__restrict__ skip) {
+                                               const int32_t *__restrict__ skip,
+                                               const double *__restrict__ vstrm) {
   double ev[1] = {0.0};
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && !(skip && skip[i] >= 0)) {
-    const double v2 = vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] +
-                      vel[3 * i + 2] * vel[3 * i + 2];
+    double v[3] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    if (vstrm)
+      for (int d = 0; d < 3; d++) v[d] = v[d] - vstrm[3 * i + d];
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     ev[0] = 0.5 * v2 / minv[i] / PGK_MD_CONV;
   }
   block_record<1, 4>(ev, rec + (size_t)blockIdx.x * PGK_REC);
 }
+
 
 // ---- Nose-Hoover chain ----------------------------------------------------
 #define PGK_NHC_MAX 10   // maxnhc (md_global.F90:58)
@@ -524,6 +550,7 @@ ConsArgs cons_args(pgk_ctx *c) {
   a.force_l = nullptr;
   a.dt_l = 0.0;
   a.ltreg = (c->md_nhc_m && c->md_lt_nreg) ? c->d_lt_reg : nullptr;
+  a.vstrm = (c->md_nhc_m && c->md_lt_nstrm) ? c->d_md_vstrm : nullptr;
   return a;
 }
 
@@ -543,16 +570,17 @@ int pgk_md_kick_drift(pgk_ctx *c, const double *f, double dt, const double *fl, 
     k_md_first_half<<<pgk_grid_blocks(nt, 64), 64, 0, c->stream>>>(a, f, c->d_md_free,
                                                                    c->md_nfree);
     HIP_CHECK(hipGetLastError());
-    return PGK_OK;
+    return pgk_md_strm_assign(c);   // at the new positions (moldyn_nhc.F90:1412)
   }
   k_md_kick_drift<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(
-      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, f, c->d_md_minv, dt, a.nhc, fl, dtl, a.ltreg);
+      n, c->d_pos, c->d_md_pos_old, c->d_md_vel, f, c->d_md_minv, dt, a.nhc, fl, dtl, a.ltreg,
+      a.vstrm);
   HIP_CHECK(hipGetLastError());
   if (c->md_ngroup > 0) {
     k_shake<<<pgk_grid_blocks(c->md_ngroup, 64), 64, 0, c->stream>>>(a);
     HIP_CHECK(hipGetLastError());
   }
-  return PGK_OK;
+  return pgk_md_strm_assign(c);
 }
 
 // kick (force f over dt, then the long force fl over dtl when given) and the
@@ -571,7 +599,8 @@ int pgk_md_kick(pgk_ctx *c, const double *f, double dt, const double *fl, double
     return PGK_OK;
   }
   k_md_kick<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, f, c->d_md_minv,
-                                                            dt, a.nhc, fl, dtl, a.ltreg);
+                                                            dt, a.nhc, fl, dtl, a.ltreg,
+                                                            a.vstrm);
   HIP_CHECK(hipGetLastError());
   return pgk_md_rattle_v(c);
 }
@@ -589,8 +618,9 @@ int pgk_md_rattle_v(pgk_ctx *c) {
 int pgk_md_ke(pgk_ctx *c, double *rec, bool nhc) {
   const int n = c->natom;
   const int32_t *skip = (nhc && c->md_lt_nreg) ? c->d_lt_reg : nullptr;
+  const double *vs = c->md_lt_nstrm ? c->d_md_vstrm : nullptr;
   k_md_ke<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(n, c->d_md_vel, c->d_md_minv, rec,
-                                                          skip);
+                                                          skip, vs);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

@@ -853,7 +853,8 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_md_d2, c->d_md_minv, c->d_md_vel, c->d_md_pos_old, c->d_md_free, c->d_md_ke,
                   c->d_md_kenhc,
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
-                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_vgrid,
+                  c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_md_vstrm,
+                  c->d_hf_vstrm, c->d_hj_vpec, c->d_vgrid,
                   c->d_vir6, c->d_hfw_xq, c->d_hfw_cen, c->d_hfw_mol, c->d_tl_start,
                   c->d_tl_src, c->d_tl_mask, c->d_hj_vel, c->d_hj_mass, c->d_hj_molevel,
                   c->d_hj_part, c->d_hj_out, c->d_hj_mol_start, c->d_hj_mol_atoms,
@@ -1237,6 +1238,27 @@ static int hj_molecules(pgk_ctx *c) {
   return PGK_OK;
 }
 
+// the streaming velocities the standalone heat-flux calls subtract, or null
+static const double *hj_vstrm(pgk_ctx *c) { return c->hf_vstrm_on ? c->d_hf_vstrm : nullptr; }
+
+int pgk_set_atmvel_strm(pgk_ctx *c, const double *vstrm) {
+  if (!c) {
+    pgk_set_error("pgk_set_atmvel_strm: null context");
+    return PGK_EINVAL;
+  }
+  if (!vstrm) {
+    c->hf_vstrm_on = false;
+    return PGK_OK;
+  }
+  int rc = set_device_ctx(c);
+  if (rc) return rc;
+  if ((rc = upload(&c->d_hf_vstrm,
+                   std::vector<double>(vstrm, vstrm + (size_t)3 * c->natom))))
+    return rc;
+  c->hf_vstrm_on = true;
+  return PGK_OK;
+}
+
 // calforce_hf and calheatf in one pass, without the per-atom W arrays
 int pgk_calforce_heatf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const pgk_hf_desc *hf,
                        const double *vel, const double *mass, pgk_force_out *out, double *hfj) {
@@ -1256,14 +1278,15 @@ int pgk_calforce_heatf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const p
   int rc = set_device_ctx(c);
   if (rc) return rc;
   if ((rc = hj_upload(c, vel, mass, "pgk_calforce_heatf")) || (rc = hj_molecules(c)) ||
-      (rc = hf_setup(c, hf, true, "pgk_calforce_heatf")))
+      (rc = hf_setup(c, hf, true, "pgk_calforce_heatf")) ||
+      (rc = pgk_heatf_collision_vel(c, c->d_hj_vel, hj_vstrm(c), &c->hfdev.vel)))
     return rc;
   c->hf_ready = false;
   c->hf_export = false;
   rc = run_calforce(c, mts_flag, mts, 1, out->for_viri_coul && out->for_viri_lj, c->d_force,
                     c->d_fvc, c->d_fvl, out->ene, out->virt, true);
   if (rc) return rc;
-  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass))) return rc;
+  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass, hj_vstrm(c)))) return rc;
   if ((rc = copy_force_out(c, out))) return rc;
   HIP_CHECK(hipMemcpyAsync(hfj, c->d_hj_out, sizeof(double) * (size_t)hf->nregion * PGK_NHFJ * 3,
                            hipMemcpyDeviceToHost, c->stream));
@@ -1289,7 +1312,7 @@ int pgk_calheatf(pgk_ctx *c, const double *vel, const double *mass, double *hf) 
   int rc = set_device_ctx(c);
   if (rc) return rc;
   if ((rc = hj_upload(c, vel, mass, "pgk_calheatf")) || (rc = hj_molecules(c))) return rc;
-  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass))) return rc;
+  if ((rc = pgk_heatf_launch(c, c->d_hj_vel, c->d_hj_mass, hj_vstrm(c)))) return rc;
   HIP_CHECK(hipMemcpyAsync(hf, c->d_hj_out,
                            sizeof(double) * (size_t)c->hfdev.nreg * PGK_NHFJ * 3,
                            hipMemcpyDeviceToHost, c->stream));
@@ -1516,7 +1539,10 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   if ((rc = hf_setup(c, &d, true, "pgk_md_run", false, true))) return rc;
   // MOLE centres: the molecules' centres of mass at this state (calcom.F90)
   if (c->md_hf_mole && (rc = pgk_calcom_launch(c, c->d_md_hfmass, c->d_hf_com))) return rc;
-  c->hfdev.vel = c->d_md_vel;
+  // with streaming slabs the collision part and the kinetic energies are
+  // taken relative to the streaming velocities (calheatf.F90:368-369)
+  const double *vs = c->md_lt_nstrm ? c->d_md_vstrm : nullptr;
+  if ((rc = pgk_heatf_collision_vel(c, c->d_md_vel, vs, &c->hfdev.vel))) return rc;
   c->hf_export = false;
   c->hf_pairs_only = true;
   // the energy records of the step stay the ones pgk_energies reports
@@ -1534,7 +1560,7 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   c->last_mts = last_mts;
   c->last_pending = last_pending;
   if (rc) return rc;
-  if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass))) return rc;
+  if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass, vs))) return rc;
   c->hfdev.flux = 0;
   const size_t row = (size_t)nreg * PGK_NHFJ * 3;
   HIP_CHECK(hipMemcpyAsync(c->d_md_hfseries + (size_t)c->md_hf_rows * row, c->d_hj_out,
@@ -1657,6 +1683,7 @@ struct MdRunner {
           sizeof(double) * PGK_LT_MAXREG * (size_t)pgk_grid_blocks(c->natom, 256), c->stream));
       if ((r = pgk_md_lt_register(c))) return r;
     }
+    if ((r = pgk_md_strm_assign(c))) return r;
     if (c->md_nhc_m && (r = pgk_md_ke(c, c->d_md_kenhc, true))) return r;
     if (respa()) {
       if ((r = short_forces(true))) return r;
@@ -2011,17 +2038,20 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
   }
   if (!lt) {
     c->md_lt_nreg = 0;
+    c->md_lt_nstrm = 0;
     return PGK_OK;
   }
   const int nr = lt->nregion;
-  if (nr < 1 || nr > PGK_LT_MAXREG || !lt->x1 || !lt->x2 || !lt->y1 || !lt->y2 || !lt->z1 ||
-      !lt->z2 || !lt->temperature || !lt->damp) {
-    pgk_set_error("pgk_md_langevin: need 1 <= nregion <= PGK_LT_MAXREG and every region array");
-    return PGK_EINVAL;
-  }
   const int ns = (lt->strm_z1 && lt->strm_z2 && lt->strm_vel) ? lt->nstream : 0;
   if (ns < 0 || ns > PGK_LT_MAXREG) {
     pgk_set_error("pgk_md_langevin: need 0 <= nstream <= PGK_LT_MAXREG");
+    return PGK_EINVAL;
+  }
+  if (nr < 0 || nr > PGK_LT_MAXREG || (nr == 0 && ns == 0) ||
+      (nr > 0 && (!lt->x1 || !lt->x2 || !lt->y1 || !lt->y2 || !lt->z1 || !lt->z2 ||
+                  !lt->temperature || !lt->damp))) {
+    pgk_set_error("pgk_md_langevin: need 1 <= nregion <= PGK_LT_MAXREG and every region array"
+                  " (or nregion 0 with streaming slabs only)");
     return PGK_EINVAL;
   }
   for (int k = 0; k < nr; k++)
@@ -2056,7 +2086,7 @@ int pgk_md_langevin(pgk_ctx *c, const pgk_langevin_desc *lt) {
   if ((rc = upload(&c->d_lt_ustart, ustart)) || (rc = upload(&c->d_lt_uatom, uatom)) ||
       (rc = upload(&c->d_lt_ucom, ucom)) || (rc = upload(&c->d_lt_mass, c->md_mass)) ||
       (rc = upload(&c->d_lt_frand, zero)) || (rc = upload(&c->d_lt_reg, none)) ||
-      (rc = upload(&c->d_lt_sreg, none)) ||
+      (rc = upload(&c->d_lt_sreg, none)) || (rc = upload(&c->d_md_vstrm, zero)) ||
       (rc = upload(&c->d_lt_det, std::vector<double>(
                                      (size_t)pgk_grid_blocks(n, 256) * PGK_LT_MAXREG, 0.0))) ||
       (rc = upload(&c->d_lt_cnt, std::vector<int32_t>(PGK_LT_MAXREG, 0))))

@@ -290,6 +290,12 @@ struct pgk_ctx {
   int md_lt_nstrm = 0;             // streaming-velocity slabs
   double md_lt_sz[2][PGK_LT_MAXREG] = {}, md_lt_sv[3][PGK_LT_MAXREG] = {};
   int32_t *d_lt_sreg = nullptr;    // streaming slab of each atom (-1 none)
+  double *d_md_vstrm = nullptr;    // [3*natom] streaming velocity of each atom
+  // atmvel_strm of the standalone heat-flux calls (pgk_set_atmvel_strm)
+  double *d_hf_vstrm = nullptr;
+  bool hf_vstrm_on = false;
+  double *d_hj_vpec = nullptr;     // velocities minus the streaming ones
+  size_t hj_vpec_cap = 0;
   int32_t *d_lt_ustart = nullptr, *d_lt_uatom = nullptr, *d_lt_ucom = nullptr;
   int32_t *d_lt_reg = nullptr, *d_lt_cnt = nullptr;   // region of each atom (-1 none)
   double *d_lt_mass = nullptr, *d_lt_frand = nullptr;
@@ -345,7 +351,12 @@ int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 #define PGK_HF_W_MAXREG 64
 int pgk_hf_w_launch(pgk_ctx *c);
 int pgk_hf_export(pgk_ctx *c);
-int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass);
+// d_vstrm: streaming velocities [3*natom] (kinetic energies and collision
+// part relative to them), or null
+int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
+                     const double *d_vstrm);
+int pgk_heatf_collision_vel(pgk_ctx *c, const double *d_vel, const double *d_vstrm,
+                            const double **out);
 // molecule centres of mass from d_pos (calcom.F90:1-147) into d_com [3*nmol]
 int pgk_calcom_launch(pgk_ctx *c, const double *d_mass, double *d_com);
 int pgk_pme_atom_energy(pgk_ctx *c, double *d_e);
@@ -361,6 +372,7 @@ int pgk_md_rattle_v(pgk_ctx *c);
 int pgk_md_ke(pgk_ctx *c, double *rec, bool nhc = false);
 int pgk_md_nhc_half(pgk_ctx *c);
 int pgk_md_lt_register(pgk_ctx *c);      // assign atoms to the Langevin regions
+int pgk_md_strm_assign(pgk_ctx *c);      // streaming velocities at the current positions
 int pgk_md_lt_half(pgk_ctx *c, bool first);
 int pgk_md_lt_count(pgk_ctx *c);         // natom_lt into d_lt_cnt
 

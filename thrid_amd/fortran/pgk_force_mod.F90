@@ -173,6 +173,14 @@ module pgk_force_mod
        integer(c_int) :: rc
      end function pgk_calheatf
 
+     ! atmvel_strm (md_global.F90:443) for pgk_calheatf / pgk_calforce_heatf:
+     ! vstrm = c_loc(atmvel_strm), or c_null_ptr to clear it
+     function pgk_set_atmvel_strm(ctx, vstrm) bind(C, name='pgk_set_atmvel_strm') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, vstrm
+       integer(c_int) :: rc
+     end function pgk_set_atmvel_strm
+
      ! calforce_hf and calheatf in one pass, without the per-atom arrays
      function pgk_calforce_heatf(ctx, mts_flag, mts, hf, vel, mass, out, hfj) &
           & bind(C, name='pgk_calforce_heatf') result(rc)

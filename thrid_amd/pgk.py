@@ -58,6 +58,7 @@ def load_library(path: str = LIB_PATH):
     L.pgk_md_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, vp]
     L.pgk_md_use_graphs.argtypes = [vp, C.c_int32]
     L.pgk_calheatf.argtypes = [vp, vp, vp, vp]
+    L.pgk_set_atmvel_strm.argtypes = [vp, vp]
     L.pgk_ewaldk_hf.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_int32] + [vp] * 6
     L.pgk_calforce_heatf.argtypes = [vp, C.c_int32, C.POINTER(abi.Mts), C.POINTER(abi.HfDesc),
                                      vp, vp, C.POINTER(abi.ForceOut), vp]
@@ -82,6 +83,7 @@ def load_library(path: str = LIB_PATH):
               "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
+              "pgk_set_atmvel_strm",
               "pgk_md_heatf", "pgk_md_heatf_series", "pgk_ewaldk_hf",
               "pgk_md_respa", "pgk_recip_energy_atom", "pgk_recip_energy_atom_noalloc", "pgk_recip_virial_atom", "pgk_md_langevin",
               "pgk_md_langevin_state"):
@@ -279,6 +281,17 @@ class ForceContext:
                                    out.ctypes.data), "pgk_calheatf")
         return out
 
+    def set_atmvel_strm(self, vstrm):
+        """Streaming velocities [natom, 3] (atmvel_strm) that calheatf and
+        calforce_heatf subtract for the kinetic energies; None clears them."""
+        if vstrm is None:
+            _check(self.L.pgk_set_atmvel_strm(self.h, None), "pgk_set_atmvel_strm")
+            return
+        vs = np.ascontiguousarray(vstrm, np.float64)
+        if vs.shape != (self.n, 3):
+            raise ValueError("set_atmvel_strm needs [natom, 3]")
+        _check(self.L.pgk_set_atmvel_strm(self.h, vs.ctypes.data), "pgk_set_atmvel_strm")
+
     # ---- device-resident NVE step ------------------------------------------
     def md_setup(self, mass, cons_ij, cons_d, group_start, dt, tol=1e-10):
         """mass [natom] amu; cons_ij (ncons,2) 1-based; cons_d (ncons,) A;
@@ -397,18 +410,22 @@ class ForceContext:
         [nregion] in 1/fs, mol_com [nmol] 1 where a molecule is registered
         by its centre of mass (None: every molecule); stream = (z1, z2,
         vel [nslab, 3] in A/fs) streaming-velocity slabs (ass_strmvel) or
-        None; regions=None switches it off."""
+        None; regions=None switches it off, regions=[] with stream sets the
+        streaming slabs alone (no thermostat)."""
         if regions is None:
             _check(self.L.pgk_md_langevin(self.h, None), "pgk_md_langevin")
             self._lt_nreg = 0
             return
-        reg = np.ascontiguousarray(np.atleast_2d(regions), np.float64)
+        reg = np.asarray(regions, np.float64)
+        reg = reg.reshape(0, 6) if reg.size == 0 else np.ascontiguousarray(np.atleast_2d(reg))
         nr = reg.shape[0]
-        if reg.shape[1] != 6 or temperature is None or damp is None:
+        if reg.shape[1] != 6 or (nr > 0 and (temperature is None or damp is None)):
             raise ValueError("md_langevin needs regions [nregion, 6], temperature and damp")
-        cols = [np.ascontiguousarray(reg[:, k]) for k in range(6)]
-        t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperature, np.float64), (nr,)))
-        g = np.ascontiguousarray(np.broadcast_to(np.asarray(damp, np.float64), (nr,)))
+        cols = [np.ascontiguousarray(reg[:, k]) if nr else None for k in range(6)]
+        t = g = None
+        if nr:
+            t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperature, np.float64), (nr,)))
+            g = np.ascontiguousarray(np.broadcast_to(np.asarray(damp, np.float64), (nr,)))
         mc = None if mol_com is None else np.ascontiguousarray(mol_com, np.int32)
         sz1 = sz2 = sv = None
         if stream is not None:
