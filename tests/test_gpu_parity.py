@@ -174,6 +174,12 @@ def test_mixed_is_deterministic():
     d = c2.calforce(abi.MTS_LONG)
     assert np.array_equal(a["force"], b["force"]) and np.array_equal(a["force"], d["force"])
     assert np.array_equal(a["ene"], d["ene"])
+    # with stage timing on, the reciprocal chain runs after the pair kernel
+    # instead of beside it: the same sums
+    c.profile(1)
+    e = c.calforce(abi.MTS_LONG)
+    c.profile(0)
+    assert np.array_equal(a["force"], e["force"]) and np.array_equal(a["ene"], e["ene"])
 
 
 def test_rebuild_after_motion(oracle_mod):

@@ -318,6 +318,7 @@ struct PmeGeom {
 };
 
 #define PGK_MAXSRC 125   // source bins per tile (5 per axis)
+#define PGK_SPREAD_MAXVIS 512   // kept atom visits per tile listed in LDS
 
 struct TileGeom {
   int T[3], nt[3], nsrc[3];   // tile size, tiles per axis, source bins per axis
@@ -581,54 +582,109 @@ __global__ __launch_bounds__(256) void k_tile_fill(int nitem, const int4 *__rest
 template <int PREC> struct SpreadAcc { using type = typename Prec<PREC>::real; };
 template <> struct SpreadAcc<PGK_PREC_MIXED> { using type = unsigned long long; };
 
+// the stencil points of one axis that fall in the tile [o, o + T): bit a set
+// when point b + a (wrapped once into [0, K)) does
+template <int P>
+__device__ __forceinline__ unsigned stencil_bits(int b, int o, int T, int K) {
+  unsigned m = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    int i = b + a;
+    if (i < 0) i += K;
+    if ((unsigned)(i - o) < (unsigned)T) m |= 1u << a;
+  }
+  return m;
+}
+
+// One wave per tile of the charge grid (owner computes: no global atomics).
+// The atoms that can reach the tile are those whose stencil origin lies in
+// its home bin or in the nsrc - 1 bins below it along each axis.  The wave
+// first keeps, of those visits, the ones whose stencil overlaps the tile at
+// all (about a third: an atom of a lower bin reaches up only from its last
+// P - 1 planes), then each lane takes one kept visit and adds exactly the
+// stencil points inside the tile to the LDS tile.  Mixed form: the points
+// are 2^-40 fixed-point integers, so the sums do not depend on the order of
+// the additions; the tile is then written to the grid once.
 template <int PREC, int P>
-__global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__restrict__ q,
-                                                     PmeGeom g, TileGeom tg,
-                                                     const int32_t *__restrict__ start,
-                                                     const int32_t *__restrict__ items,
-                                                     const typename Prec<PREC>::real *__restrict__ th,
-                                                     const int4 *__restrict__ base,
-                                                     typename Prec<PREC>::real *__restrict__ grid) {
+__global__ __launch_bounds__(64) void k_spread_tile(int nitem, const double *__restrict__ q,
+                                                    PmeGeom g, TileGeom tg,
+                                                    const int32_t *__restrict__ start,
+                                                    const int32_t *__restrict__ items,
+                                                    const typename Prec<PREC>::real *__restrict__ th,
+                                                    const int4 *__restrict__ base,
+                                                    typename Prec<PREC>::real *__restrict__ grid) {
   using R = typename Prec<PREC>::real;
   using S = typename SpreadAcc<PREC>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   S *acc = reinterpret_cast<S *>(smem);
   __shared__ int s_lo[PGK_MAXSRC], s_n[PGK_MAXSRC + 1];
+  __shared__ int s_vis[PGK_SPREAD_MAXVIS];
+  __shared__ int s_nv;
   const int T0 = tg.T[0], T1 = tg.T[1], T2 = tg.T[2];
   const int npt = T0 * T1 * T2;
   const int tid = xcd_block();
   if (tid >= tg.ntile) return;
+  const int lane = threadIdx.x;
   const int tx = tid % tg.nt[0], ty = (tid / tg.nt[0]) % tg.nt[1], tz = tid / (tg.nt[0] * tg.nt[1]);
   const int ox = tx * T0, oy = ty * T1, oz = tz * T2;
-  for (int i = threadIdx.x; i < npt; i += blockDim.x) acc[i] = S(0);
+  for (int i = lane; i < npt; i += 64) acc[i] = S(0);
   // the source bins of this tile, flattened into one index space
   const int nsrc = tg.nsrc[0] * tg.nsrc[1] * tg.nsrc[2];
-  if (threadIdx.x == 0) {
-    int tot = 0, u = 0;
-    for (int dz = 0; dz < tg.nsrc[2]; dz++)
-      for (int dy = 0; dy < tg.nsrc[1]; dy++)
-        for (int dx = 0; dx < tg.nsrc[0]; dx++) {
-          const int sx = (tx - dx + tg.nt[0]) % tg.nt[0];
-          const int sy = (ty - dy + tg.nt[1]) % tg.nt[1];
-          const int sz = (tz - dz + tg.nt[2]) % tg.nt[2];
-          const int bin = (sz * tg.nt[1] + sy) * tg.nt[0] + sx;
-          s_lo[u] = start[bin];
-          s_n[u] = tot;
-          tot += start[bin + 1] - start[bin];
-          u++;
-        }
-    s_n[nsrc] = tot;
+  for (int u = lane; u < nsrc; u += 64) {
+    const int dx = u % tg.nsrc[0], dy = (u / tg.nsrc[0]) % tg.nsrc[1],
+              dz = u / (tg.nsrc[0] * tg.nsrc[1]);
+    const int sx = (tx - dx + tg.nt[0]) % tg.nt[0];
+    const int sy = (ty - dy + tg.nt[1]) % tg.nt[1];
+    const int sz = (tz - dz + tg.nt[2]) % tg.nt[2];
+    const int bin = (sz * tg.nt[1] + sy) * tg.nt[0] + sx;
+    s_lo[u] = start[bin];
+    s_n[u + 1] = start[bin + 1] - start[bin];
+  }
+  if (lane == 0) s_nv = 0;
+  __syncthreads();
+  if (lane == 0) {
+    s_n[0] = 0;
+    for (int u = 0; u < nsrc; u++) s_n[u + 1] += s_n[u];
   }
   __syncthreads();
   const int total = s_n[nsrc];
-  const size_t st = (size_t)nitem;
-  // one thread per atom visit: its weights in registers, then the p^3 points
-  // (splitting a visit's z planes over threads measured slower)
-  for (int v = threadIdx.x; v < total; v += blockDim.x) {
+  // keep the visits whose stencil meets the tile (the kept order does not
+  // matter: the additions below commute exactly in the mixed form)
+  for (int v = lane; v < total; v += 64) {
     int u = 0;
     while (u + 1 < nsrc && v >= s_n[u + 1]) u++;
     const int k = items[s_lo[u] + (v - s_n[u])];
     const int4 b = base[k];
+    if (stencil_bits<P>(b.x, ox, T0, g.K[0]) && stencil_bits<P>(b.y, oy, T1, g.K[1]) &&
+        stencil_bits<P>(b.z, oz, T2, g.K[2])) {
+      const int w = atomicAdd(&s_nv, 1);
+      if (w < PGK_SPREAD_MAXVIS) s_vis[w] = k;
+    }
+  }
+  __syncthreads();
+  const int nv = min(s_nv, PGK_SPREAD_MAXVIS);
+  const size_t st = (size_t)nitem;
+  // the rare tile with more kept visits than fit in the list takes the rest
+  // with a second walk over all visits (same test, no list)
+  const bool overflow = s_nv > PGK_SPREAD_MAXVIS;
+  for (int w = lane; w < (overflow ? total + nv : nv); w += 64) {
+    int k;
+    if (w < nv) {
+      k = s_vis[w];
+    } else {
+      const int v = w - nv;
+      int u = 0;
+      while (u + 1 < nsrc && v >= s_n[u + 1]) u++;
+      k = items[s_lo[u] + (v - s_n[u])];
+      bool listed = false;
+      for (int x = 0; x < nv && !listed; x++) listed = s_vis[x] == k;
+      if (listed) continue;
+    }
+    const int4 b = base[k];
+    const unsigned mx = stencil_bits<P>(b.x, ox, T0, g.K[0]);
+    const unsigned my = stencil_bits<P>(b.y, oy, T1, g.K[1]);
+    const unsigned mz = stencil_bits<P>(b.z, oz, T2, g.K[2]);
+    if (!(mx && my && mz)) continue;
     const R qi = (R)q[b.w];
     R w1[P], w2[P], w3[P];
 #pragma unroll
@@ -639,25 +695,24 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
     }
 #pragma unroll
     for (int a3 = 0; a3 < P; a3++) {
+      if (!((mz >> a3) & 1u)) continue;
       int iz = b.z + a3;
       if (iz < 0) iz += g.K[2];
-      const unsigned lz = (unsigned)(iz - oz);
-      if (lz >= (unsigned)T2) continue;
+      const int lz = iz - oz;
 #pragma unroll
       for (int a2 = 0; a2 < P; a2++) {
+        if (!((my >> a2) & 1u)) continue;
         int iy = b.y + a2;
         if (iy < 0) iy += g.K[1];
-        const unsigned ly = (unsigned)(iy - oy);
-        if (ly >= (unsigned)T1) continue;
+        const int ly = iy - oy;
         const R w23 = w2[a2] * w3[a3];
 #pragma unroll
         for (int a1 = 0; a1 < P; a1++) {
+          if (!((mx >> a1) & 1u)) continue;
           int ix = b.x + a1;
           if (ix < 0) ix += g.K[0];
-          const unsigned lx = (unsigned)(ix - ox);
-          if (lx >= (unsigned)T0) continue;
           const R val = w1[a1] * w23;
-          S *dst = &acc[(lz * T1 + ly) * T0 + lx];
+          S *dst = &acc[(lz * T1 + ly) * T0 + (ix - ox)];
           if (PREC == PGK_PREC_MIXED)
             atomicAdd(dst, (S)fx_of(val * 256.0f));   // 2^-40 fixed point
           else
@@ -667,7 +722,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(int nitem, const double *__
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < npt; i += blockDim.x) {
+  for (int i = lane; i < npt; i += 64) {
     const int lx = i % T0, ly = (i / T0) % T1, lz = i / (T0 * T1);
     const size_t idx = ((size_t)(oz + lz) * g.K[1] + (oy + ly)) * g.K[0] + (ox + lx);
     if (PREC == PGK_PREC_MIXED)
@@ -1459,7 +1514,7 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
     R *gr = reinterpret_cast<R *>(c->d_grid);
     const int32_t *start = L.start, *items = L.items;
 #define PGK_SPREAD(P)                                                                         \
-  k_spread_tile<PREC, P><<<xcd_grid(tg.ntile), 256, lds, st>>>(nitem, c->d_q, g, tg, start,   \
+  k_spread_tile<PREC, P><<<xcd_grid(tg.ntile), 64, lds, st>>>(nitem, c->d_q, g, tg, start,    \
                                                                 items, th, base, gr)
     switch (c->order) {
       case 3: PGK_SPREAD(3); break;

@@ -296,6 +296,24 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     if ((rc = pgk_pme_prep(c, do_pair))) return rc;
     pgk_prof_end(c, PGK_PROF_OTHER);
   }
+  // The reciprocal chain (spread, FFT, gather) and the pair kernel read the
+  // same positions and write different accumulators: with both due, the
+  // chain goes to the second stream and the pair and bonded kernels run
+  // beside it (timing any stage but the pair kernel keeps them one after
+  // the other)
+  const bool timed_other = c->prof && (c->prof_mask & ~(1u << PGK_PROF_PAIRS)) != 0;
+  const bool overlap = do_pair && do_ewk && !hf && !timed_other && c->pme_overlap;
+  if (overlap) {
+    HIP_CHECK(hipEventRecord(c->ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    c->stream = c->stream2;
+    rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
+    c->stream = st;
+    if (rc) return rc;
+    seg_kind[nseg++] = SEG_PME;
+    c->pme_grid_ready = true;
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+  }
   if (do_pair) {
     size_t ns = (size_t)c->ncl * PGK_CL;
     if (!do_ewk) {
@@ -312,7 +330,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     pgk_prof_end(c, PGK_PROF_PAIRS);
     seg_kind[nseg++] = SEG_PAIR;
   }
-  if (do_ewk) {
+  if (do_ewk && !overlap) {
     if ((rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
     seg_kind[nseg++] = SEG_PME;
     c->pme_grid_ready = true;
@@ -320,6 +338,7 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   pgk_prof_begin(c, PGK_PROF_BONDED);
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
+  if (overlap) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
   if (hf_pairs && (rc = pgk_hf_launch_pairs(c, true))) return rc;
   // forces: accumulators + excluded-pair correction -> FP64 output
   double *fout = d_force_out ? d_force_out : c->d_force;
@@ -640,8 +659,14 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
     c->rkcut = 0.9 * rk;
   }
   hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (he != hipSuccess) {
     pgk_set_error(std::string("pgk_create: stream: ") + hipGetErrorString(he));
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PGK_EDEVICE;
   }
@@ -866,6 +891,9 @@ int pgk_destroy(pgk_ctx *c) {
   if (c->h_list_meta) (void)hipHostFree(c->h_list_meta);
   for (hipEvent_t e : c->pev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PGK_OK;

@@ -97,6 +97,11 @@ struct DevBuf {
 struct pgk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the reciprocal-space chain runs on stream2 beside the pair kernel when
+  // nothing needs the stages one after the other (fork / join events)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool pme_overlap = true;
   int precision = PGK_PREC_FP64;
 
   // system (host copies)
