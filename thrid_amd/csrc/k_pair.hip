@@ -350,17 +350,20 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
     a.red[(size_t)blockIdx.x * PGK_REC + threadIdx.x] = 0.0;
 }
 
-// ---- force combination + excluded pairs ------------------------------------
-// One thread per atom: the excluded-pair correction (ewaldr.F90:744-828:
-// E -= qq S'_e(r), f = -qq S'_f(r) r_ij with the erf tables, no cutoff) over
-// the atom's symmetric exclusion list, then the slot-order pair forces and
-// the atom-order accumulators (PME, bonded) are read, summed into the FP64
-// output and cleared for the next call.  Each pair is evaluated in the
-// reference's orientation (lower index first), so both atoms see the same
-// value; the lower atom records the pair's energy and virial.  FP64 in every
-// form; no atomics.
+// ---- excluded pairs, then the force combination -----------------------------
+// k_excl: one thread per atom, the excluded-pair correction
+// (ewaldr.F90:744-828: E -= qq S'_e(r), f = -qq S'_f(r) r_ij with the erf
+// tables, no cutoff) over the atom's symmetric exclusion list, added to the
+// atom-order accumulators (and the Coulomb class's).  Each pair is evaluated
+// in the reference's orientation (lower index first), so both atoms see the
+// same value; the lower atom records the pair's energy and virial.  FP64
+// arithmetic in every form.  It reads only positions and charges, so it can
+// run beside the pair kernel.
+// k_combine: one thread per atom, the slot-order pair forces and the
+// atom-order accumulators (PME, bonded, excluded pairs) summed into the FP64
+// output and cleared for the next call.  No atomics.
 struct CombineArgs {
-  int n, use_cl, do_excl;
+  int n, use_cl;
   const int32_t *slot_of;
   const int32_t *xs_start, *xs_j;   // symmetric exclusion CSR, 0-based
   const double *pos, *q, *splx;
@@ -371,67 +374,76 @@ struct CombineArgs {
 };
 
 template <class A, bool SPLIT>
+__global__ __launch_bounds__(256) void k_excl(CombineArgs a, A *__restrict__ fat,
+                                              A *__restrict__ fatc) {
+  double ev[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n && a.xs_start[i] < a.xs_start[i + 1]) {
+    double fe[3] = {0.0, 0.0, 0.0};
+    for (int k = a.xs_start[i]; k < a.xs_start[i + 1]; k++) {
+      const int j = a.xs_j[k];
+      const int lo = min(i, j), hi = max(i, j);
+      double rx = a.pos[3 * lo] - a.pos[3 * hi], ry = a.pos[3 * lo + 1] - a.pos[3 * hi + 1],
+             rz = a.pos[3 * lo + 2] - a.pos[3 * hi + 2];
+      rx = rx - a.L.x * fnint(rx * a.Li.x);
+      ry = ry - a.L.y * fnint(ry * a.Li.y);
+      rz = rz - a.L.z * fnint(rz * a.Li.z);
+      const double r = sqrt(r2_nc(rx, ry, rz));
+      double sf, se;
+      spline2(a.splx, r, a.h, a.hinv, a.xmax, a.nspl, sf, se);
+      const double qq = a.q[lo] * a.q[hi];
+      const double ft = -sf * qq;
+      const double fx = ft * rx, fy = ft * ry, fz = ft * rz;
+      if (i == lo) {
+        fe[0] += fx;
+        fe[1] += fy;
+        fe[2] += fz;
+        ev[0] -= se * qq;
+        ev[1] += fx * rx; ev[2] += fy * ry; ev[3] += fz * rz;
+        ev[4] += fx * ry; ev[5] += fx * rz; ev[6] += fy * rz;
+      } else {
+        fe[0] -= fx;
+        fe[1] -= fy;
+        fe[2] -= fz;
+      }
+    }
+    for (int d = 0; d < 3; d++) {
+      acc_add(&fat[3 * i + d], fe[d]);
+      if (SPLIT) acc_add(&fatc[3 * i + d], fe[d]);
+    }
+  }
+  block_record<7, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+}
+
+template <class A, bool SPLIT>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs a, A *__restrict__ fcl,
                                                  A *__restrict__ fat, A *__restrict__ fclc,
                                                  A *__restrict__ fatc, double *__restrict__ force,
                                                  double *__restrict__ fvc,
                                                  double *__restrict__ fvl) {
-  double ev[7] = {0, 0, 0, 0, 0, 0, 0};
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.n) {
-    double fe[3] = {0.0, 0.0, 0.0};
-    if (a.do_excl) {
-      for (int k = a.xs_start[i]; k < a.xs_start[i + 1]; k++) {
-        const int j = a.xs_j[k];
-        const int lo = min(i, j), hi = max(i, j);
-        double rx = a.pos[3 * lo] - a.pos[3 * hi], ry = a.pos[3 * lo + 1] - a.pos[3 * hi + 1],
-               rz = a.pos[3 * lo + 2] - a.pos[3 * hi + 2];
-        rx = rx - a.L.x * fnint(rx * a.Li.x);
-        ry = ry - a.L.y * fnint(ry * a.Li.y);
-        rz = rz - a.L.z * fnint(rz * a.Li.z);
-        const double r = sqrt(r2_nc(rx, ry, rz));
-        double sf, se;
-        spline2(a.splx, r, a.h, a.hinv, a.xmax, a.nspl, sf, se);
-        const double qq = a.q[lo] * a.q[hi];
-        const double ft = -sf * qq;
-        const double fx = ft * rx, fy = ft * ry, fz = ft * rz;
-        if (i == lo) {
-          fe[0] += fx;
-          fe[1] += fy;
-          fe[2] += fz;
-          ev[0] -= se * qq;
-          ev[1] += fx * rx; ev[2] += fy * ry; ev[3] += fz * rz;
-          ev[4] += fx * ry; ev[5] += fx * rz; ev[6] += fy * rz;
-        } else {
-          fe[0] -= fx;
-          fe[1] -= fy;
-          fe[2] -= fz;
-        }
-      }
+  if (i >= a.n) return;
+  const int s = a.use_cl ? a.slot_of[i] : -1;
+  // every accumulator is read once here and left zeroed for the next call
+  for (int d = 0; d < 3; d++) {
+    double fp = 0.0;
+    if (s >= 0) {
+      fp = acc_load(&fcl[3 * s + d]);
+      fcl[3 * s + d] = A(0);
     }
-    const int s = a.use_cl ? a.slot_of[i] : -1;
-    // every accumulator is read once here and left zeroed for the next call
-    for (int d = 0; d < 3; d++) {
-      double fp = 0.0;
+    force[3 * i + d] = fp + acc_load(&fat[3 * i + d]);
+    fat[3 * i + d] = A(0);
+    if (SPLIT) {
+      double fc = 0.0;
       if (s >= 0) {
-        fp = acc_load(&fcl[3 * s + d]);
-        fcl[3 * s + d] = A(0);
+        fc = acc_load(&fclc[3 * s + d]);
+        fclc[3 * s + d] = A(0);
       }
-      force[3 * i + d] = fp + acc_load(&fat[3 * i + d]) + fe[d];
-      fat[3 * i + d] = A(0);
-      if (SPLIT) {
-        double fc = 0.0;
-        if (s >= 0) {
-          fc = acc_load(&fclc[3 * s + d]);
-          fclc[3 * s + d] = A(0);
-        }
-        fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]) + fe[d];
-        fatc[3 * i + d] = A(0);
-        fvl[3 * i + d] = fp - fc;
-      }
+      fvc[3 * i + d] = fc + acc_load(&fatc[3 * i + d]);
+      fatc[3 * i + d] = A(0);
+      fvl[3 * i + d] = fp - fc;
     }
   }
-  if (a.do_excl) block_record<7, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 template <int PREC>
@@ -507,13 +519,10 @@ int pgk_pairs_launch(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   }
 }
 
-template <class A>
-static int launch_combine(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
-                          double *fvl, int *off, int *nrec) {
+static CombineArgs combine_args(pgk_ctx *c, bool do_pair) {
   CombineArgs a;
   a.n = c->natom;
   a.use_cl = do_pair ? 1 : 0;
-  a.do_excl = (do_pair && c->nexclpair > 0) ? 1 : 0;
   a.slot_of = c->d_slot_of;
   a.xs_start = c->d_xs_start;
   a.xs_j = c->d_xs_j;
@@ -526,15 +535,45 @@ static int launch_combine(pgk_ctx *c, bool do_pair, bool split, double *fout, do
   a.nspl = c->nspl;
   a.L = make_double3(c->box[0], c->box[1], c->box[2]);
   a.Li = make_double3(1.0 / c->box[0], 1.0 / c->box[1], 1.0 / c->box[2]);
-  const int nblk = pgk_grid_blocks(c->natom, 256);
-  *nrec = 0;
   a.red = nullptr;
-  if (a.do_excl) {
-    *off = c->red_used;
-    *nrec = nblk;
-    a.red = c->d_red + (size_t)c->red_used * PGK_REC;
-    c->red_used += nblk;
+  return a;
+}
+
+template <class A>
+static int launch_excl(pgk_ctx *c, bool split, int *off, int *nrec) {
+  CombineArgs a = combine_args(c, true);
+  const int nblk = pgk_grid_blocks(c->natom, 256);
+  *off = c->red_used;
+  *nrec = nblk;
+  a.red = c->d_red + (size_t)c->red_used * PGK_REC;
+  c->red_used += nblk;
+  A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
+  if (split)
+    k_excl<A, true><<<nblk, 256, 0, c->stream>>>(a, fat, fatc);
+  else
+    k_excl<A, false><<<nblk, 256, 0, c->stream>>>(a, fat, fatc);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
+
+// The excluded-pair correction of the listed pair terms into the atom-order
+// accumulators, with its energy/virial records (*nrec = 0 when the system
+// has no exclusions).
+int pgk_excl_launch(pgk_ctx *c, bool split, int *off, int *nrec) {
+  *nrec = 0;
+  if (c->nexclpair <= 0) return PGK_OK;
+  switch (c->precision) {
+    case PGK_PREC_FP64: return launch_excl<double>(c, split, off, nrec);
+    case PGK_PREC_MIXED: return launch_excl<unsigned long long>(c, split, off, nrec);
+    default: return launch_excl<float>(c, split, off, nrec);
   }
+}
+
+template <class A>
+static int launch_combine(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
+                          double *fvl) {
+  const CombineArgs a = combine_args(c, do_pair);
+  const int nblk = pgk_grid_blocks(c->natom, 256);
   A *fcl = reinterpret_cast<A *>(c->d_fcl), *fat = reinterpret_cast<A *>(c->d_fat);
   A *fclc = reinterpret_cast<A *>(c->d_fcl_c), *fatc = reinterpret_cast<A *>(c->d_fat_c);
   if (split)
@@ -546,16 +585,13 @@ static int launch_combine(pgk_ctx *c, bool do_pair, bool split, double *fout, do
 }
 
 // Sum every force accumulator into fout (and the per-class forces when
-// split), adding the excluded-pair correction when the pair terms ran; the
-// correction's energy/virial records are appended when *nrec > 0.
+// split).
 int pgk_combine_launch(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
-                       double *fvl, int *off, int *nrec) {
+                       double *fvl) {
   switch (c->precision) {
-    case PGK_PREC_FP64:
-      return launch_combine<double>(c, do_pair, split, fout, fvc, fvl, off, nrec);
+    case PGK_PREC_FP64: return launch_combine<double>(c, do_pair, split, fout, fvc, fvl);
     case PGK_PREC_MIXED:
-      return launch_combine<unsigned long long>(c, do_pair, split, fout, fvc, fvl, off, nrec);
-    default:
-      return launch_combine<float>(c, do_pair, split, fout, fvc, fvl, off, nrec);
+      return launch_combine<unsigned long long>(c, do_pair, split, fout, fvc, fvl);
+    default: return launch_combine<float>(c, do_pair, split, fout, fvc, fvl);
   }
 }

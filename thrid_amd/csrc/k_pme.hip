@@ -435,6 +435,11 @@ __global__ __launch_bounds__(256) void k_prep(int nitem, const int32_t *__restri
   if (k < nitem) rank[k] = (t >= 0) ? s_base[slot] + r : -1;
 }
 
+// Forces from the potential grid (grad_sum, fft_pme.F90:515-590): four
+// lanes per slot, lane s taking the stencil planes a3 = s, s + 4, ... (four
+// times the threads of one lane per atom, so the grid loads of more atoms are
+// in flight at once); the lanes' partial forces are added in a fixed tree and
+// the first lane of the four adds the force to the atom's accumulator.
 template <int PREC, bool SPLIT, int P>
 __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__restrict__ order,
                                                   const double *__restrict__ pos,
@@ -443,7 +448,10 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
                                                   typename Prec<PREC>::acc *fat,
                                                   typename Prec<PREC>::acc *fat_c) {
   using R = typename Prec<PREC>::real;
-  const int k = xcd_block() * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  const int k = t >> 2, sub = t & 3;
+  // whole groups of four lanes leave together, so the shuffles below see
+  // their partners
   if (k >= nitem) return;
   const int i = order ? order[k] : k;
   if (i < 0) return;
@@ -460,18 +468,24 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
     int ii = i0[0] + a1;
     ix[a1 - 1] = (ii < 0) ? ii + g.K[0] : ii;
   }
-#pragma unroll
-  for (int a3 = 1; a3 <= P; a3++) {
+  for (int a3 = 1 + sub; a3 <= P; a3 += 4) {
     int kk = i0[2] + a3;
     if (kk < 0) kk += g.K[2];
+    R t3 = 0, d3 = 0;
+#pragma unroll
+    for (int a = 1; a <= P; a++)
+      if (a == a3) {
+        t3 = th[2][a];
+        d3 = dth[2][a];
+      }
 #pragma unroll
     for (int a2 = 1; a2 <= P; a2++) {
       int jj = i0[1] + a2;
       if (jj < 0) jj += g.K[1];
       const R *row = grid + ((size_t)kk * g.K[1] + jj) * g.K[0];
-      const R t23 = th[1][a2] * th[2][a3];
-      const R d23 = dth[1][a2] * th[2][a3];
-      const R t2d3 = th[1][a2] * dth[2][a3];
+      const R t23 = th[1][a2] * t3;
+      const R d23 = dth[1][a2] * t3;
+      const R t2d3 = th[1][a2] * d3;
 #pragma unroll
       for (int a1 = 1; a1 <= P; a1++) {
         const R term = qi * row[ix[a1 - 1]];
@@ -481,6 +495,13 @@ __global__ __launch_bounds__(256) void k_gather_t(int nitem, const int32_t *__re
       }
     }
   }
+  f1 += __shfl_xor(f1, 1, 64);
+  f2 += __shfl_xor(f2, 1, 64);
+  f3 += __shfl_xor(f3, 1, 64);
+  f1 += __shfl_xor(f1, 2, 64);
+  f2 += __shfl_xor(f2, 2, 64);
+  f3 += __shfl_xor(f3, 2, 64);
+  if (sub != 0) return;
   double fx = g.recip[0] * (double)f1, fy = g.recip[1] * (double)f2, fz = g.recip[2] * (double)f3;
   acc_add(&fat[3 * i + 0], fx);
   acc_add(&fat[3 * i + 1], fy);
@@ -1612,7 +1633,7 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   pgk_prof_begin(c, PGK_PROF_GATHER);
   A *fat = reinterpret_cast<A *>(c->d_fat), *fatc = reinterpret_cast<A *>(c->d_fat_c);
   {
-    const int gb = pgk_grid_blocks(nitem, 256);
+    const int gb = pgk_grid_blocks((int64_t)nitem * 4, 256);   // four lanes per slot
     const int gbx = xcd_grid(gb);
 #define PGK_GATHER(SP, P)                                                                 \
   k_gather_t<PREC, SP, P><<<gbx, 256, 0, st>>>(nitem, ord, c->d_pos, c->d_q, g, c->d_pgrid, \

@@ -308,9 +308,13 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     c->stream = c->stream2;
     rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
+    if (!rc) {
+      seg_kind[nseg++] = SEG_PME;
+      rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
+      if (!rc && seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
+    }
     c->stream = st;
     if (rc) return rc;
-    seg_kind[nseg++] = SEG_PME;
     c->pme_grid_ready = true;
     HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
   }
@@ -338,27 +342,23 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   pgk_prof_begin(c, PGK_PROF_BONDED);
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
-  if (overlap) HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  if (overlap) {
+    HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+  } else if (do_pair) {
+    pgk_prof_begin(c, PGK_PROF_EXCL);
+    if ((rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+    pgk_prof_end(c, PGK_PROF_EXCL);
+    if (seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
+  }
   if (hf_pairs && (rc = pgk_hf_launch_pairs(c, true))) return rc;
   // forces: accumulators + excluded-pair correction -> FP64 output
   double *fout = d_force_out ? d_force_out : c->d_force;
-  {
-    pgk_prof_begin(c, PGK_PROF_EXCL);
-    int eo = 0, en = 0;
-    if ((rc = pgk_combine_launch(c, do_pair, split, fout, split ? d_fvc : nullptr,
-                                 split ? d_fvl : nullptr, &eo, &en)))
-      return rc;
-    pgk_prof_end(c, PGK_PROF_EXCL);
-    if (en > 0) {
-      seg_off[nseg] = eo;
-      seg_n[nseg] = en;
-      seg_kind[nseg++] = SEG_EXCL;
-    }
-  }
+  if ((rc = pgk_combine_launch(c, do_pair, split, fout, split ? d_fvc : nullptr,
+                               split ? d_fvl : nullptr)))
+    return rc;
   // energies: fixed-order segment sums into d_sum, copied to pinned h_sum
-  // (a dynamics step that samples nothing sums them on the device but skips
-  // the copy)
-  if (nseg > 0) {
+  // (a dynamics step that samples nothing skips both: nothing reads them)
+  if (nseg > 0 && energies) {
     // table slots: the MD loop's heat-flux pass and RESPA's short-level calls
     // alternate with the step's own call without re-uploading any table
     const int slot = c->hf_pairs_only ? 1 : (mts_flag == PGK_MTS_SHORT ? 2 : 0);
@@ -379,9 +379,8 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     }
     k_reduce_segments<<<nseg * PGK_REC, 256, 0, st>>>(c->d_red, dseg, dseg + 16, c->d_sum);
     HIP_CHECK(hipGetLastError());
-    if (energies)
-      HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
-                               hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(c->h_sum, c->d_sum, sizeof(double) * nseg * PGK_REC,
+                             hipMemcpyDeviceToHost, st));
   }
   if (hf && c->hf_export && (rc = pgk_hf_export(c))) return rc;
   c->last_nseg = nseg;
@@ -420,7 +419,15 @@ int finalize_energies(pgk_ctx *c, double *ene, double *virt) {
   E[PGK_V_COUL] = E[PGK_V_LJ] = 0.0;
   E[PGK_V_BOND] = E[PGK_V_ANGL] = E[PGK_V_TORS] = E[PGK_V_14] = 0.0;
   bool tors_done = false;
-  for (int k = 0; k < nseg; k++) {
+  // the pair record sets the Coulomb and LJ sums that the excluded-pair
+  // record then corrects, whichever stream queued them first
+  int korder[16], nk = 0;
+  for (int k = 0; k < nseg; k++)
+    if (seg_kind[k] == SEG_PAIR) korder[nk++] = k;
+  for (int k = 0; k < nseg; k++)
+    if (seg_kind[k] != SEG_PAIR) korder[nk++] = k;
+  for (int kk = 0; kk < nseg; kk++) {
+    const int k = korder[kk];
     const double *s = segsum[k];
     switch (seg_kind[k]) {
       case SEG_PAIR:

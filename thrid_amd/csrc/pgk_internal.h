@@ -345,7 +345,8 @@ int pgk_list_count_pairs(pgk_ctx *c);   // fills c->npairs (counted on request)
 int pgk_list_count_pairs(pgk_ctx *c);   // fills c->npairs (counted lazily)
 int pgk_pairs_launch(pgk_ctx *c, bool want_fsplit, bool energies, int *red_off, int *red_n);
 int pgk_combine_launch(pgk_ctx *c, bool do_pair, bool split, double *fout, double *fvc,
-                       double *fvl, int *red_off, int *red_n);
+                       double *fvl);
+int pgk_excl_launch(pgk_ctx *c, bool split, int *off, int *nrec);
 int pgk_pme_launch(pgk_ctx *c, bool want_fsplit, int *red_off, int *red_n);
 int pgk_pme_prepare(pgk_ctx *c);
 int pgk_pme_prep(pgk_ctx *c, bool pack);
