@@ -476,6 +476,11 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   size_t ljb = (size_t)2 * c->ntype * c->ntype * sizeof(R);
   a.lj_lds = (lds + ljb <= 96 * 1024) ? 1 : 0;
   if (a.lj_lds) lds += ljb;
+  // beside the reciprocal chain, the pair blocks leave room on every CU: an
+  // LDS request above 1/(B+1) of the CU's LDS caps them at B blocks (of NW
+  // waves) per CU, so the chain's short kernels find free slots at once
+  if (c->pair_cap_blocks > 0 && c->lds_per_cu > 0)
+    lds = std::max(lds, (size_t)c->lds_per_cu / (size_t)(c->pair_cap_blocks + 1) + 1024);
   for (int d = 0; d < 3; d++) {
     a.L[d] = (R)c->box[d];
     a.Linv[d] = (R)(1.0 / c->box[d]);

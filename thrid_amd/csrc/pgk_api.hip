@@ -330,7 +330,11 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipGetLastError());
     }
     pgk_prof_begin(c, PGK_PROF_PAIRS);
-    if ((rc = pgk_pairs_launch(c, split, energies, &seg_off[nseg], &seg_n[nseg]))) return rc;
+    // beside the chain, at most three pair blocks per CU
+    c->pair_cap_blocks = overlap ? 3 : 0;
+    rc = pgk_pairs_launch(c, split, energies, &seg_off[nseg], &seg_n[nseg]);
+    c->pair_cap_blocks = 0;
+    if (rc) return rc;
     pgk_prof_end(c, PGK_PROF_PAIRS);
     seg_kind[nseg++] = SEG_PAIR;
   }
@@ -669,6 +673,10 @@ int pgk_create(const pgk_system_desc *d, int32_t precision, pgk_ctx **out) {
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  if (he == hipSuccess &&
+      hipDeviceGetAttribute(&c->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                            c->device) != hipSuccess)
+    c->lds_per_cu = 0;
   if (he != hipSuccess) {
     pgk_set_error(std::string("pgk_create: stream: ") + hipGetErrorString(he));
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);

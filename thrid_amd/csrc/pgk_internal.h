@@ -102,6 +102,8 @@ struct pgk_ctx {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool pme_overlap = true;
+  int pair_cap_blocks = 0;         // >0: pair blocks per CU while the chain runs beside
+  int lds_per_cu = 0;              // bytes of LDS per CU
   int precision = PGK_PREC_FP64;
 
   // system (host copies)
