@@ -347,8 +347,8 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
       k_hf_pairs_flux<AH, true><<<nblk, 256, 0, st>>>(p, h);
     } else {
       const size_t lds = sizeof(double) * 16 * (size_t)h.nreg;
-      HIP_CHECK(hipFuncSetAttribute((const void *)k_hf_pairs_flux<AH, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      const int lr = pgk_lds_limit((const void *)k_hf_pairs_flux<AH, false>, lds);
+      if (lr) return lr;
       k_hf_pairs_flux<AH, false><<<nblk, 256, lds, st>>>(p, h);
     }
     HIP_CHECK(hipGetLastError());

@@ -381,10 +381,9 @@ int launch_w_mode(pgk_ctx *c, const HfWArgs &pa, const HfWArgs &pb) {
   const size_t lds_a = (MODE == PGK_HF_BULK) ? 0 : (size_t)4 * 8 * 2 * h.nreg * 3 * sizeof(AH);
   const size_t lds_b = (MODE == PGK_HF_BULK) ? 0 : (size_t)4 * 4 * 2 * h.nreg * 3 * sizeof(AH);
   if (lds_a > 0) {
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_hf_w<AH, MODE, false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_a));
-    HIP_CHECK(hipFuncSetAttribute((const void *)k_hf_w<AH, MODE, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    int lr = pgk_lds_limit((const void *)k_hf_w<AH, MODE, false>, lds_a);
+    if (!lr) lr = pgk_lds_limit((const void *)k_hf_w<AH, MODE, true>, lds_b);
+    if (lr) return lr;
   }
   k_hf_w<AH, MODE, false><<<pgk_grid_blocks(pa.nunit, 4), 256, lds_a, st>>>(pa, h);
   k_hf_w<AH, MODE, true><<<pgk_grid_blocks(pb.nunit, 4), 256, lds_b, st>>>(pb, h);

@@ -500,8 +500,10 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
 #define PGK_LAUNCH_PAIRS(SP, LL, EN)                                                    \
   {                                                                                      \
     auto kf = k_pairs<PREC, SP, LL, NW, EN>;                                             \
-    HIP_CHECK(hipFuncSetAttribute((const void *)kf,                                      \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    {                                                                                    \
+      const int lr = pgk_lds_limit((const void *)kf, lds);                               \
+      if (lr) return lr;                                                                 \
+    }                                                                                    \
     kf<<<nblk, 64 * NW, lds, c->stream>>>(a);                                            \
   }
   if (split) {   // the per-class forces come with their energies

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -20,6 +21,19 @@
 
 static thread_local std::string g_err;
 void pgk_set_error(const std::string &msg) { g_err = msg; }
+
+int pgk_lds_limit(const void *f, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> set;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &cur = set[std::make_pair(dev, f)];
+  if (lds <= cur) return PGK_OK;
+  HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  cur = lds;
+  return PGK_OK;
+}
 
 void pgk_prof_begin(pgk_ctx *c, int k) {
   if (!c->prof || !((c->prof_mask >> k) & 1u)) return;

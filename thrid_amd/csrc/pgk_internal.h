@@ -384,6 +384,11 @@ int pgk_md_strm_assign(pgk_ctx *c);      // streaming velocities at the current 
 int pgk_md_lt_half(pgk_ctx *c, bool first);
 int pgk_md_lt_count(pgk_ctx *c);         // natom_lt into d_lt_cnt
 
+// raise kernel f's dynamic LDS limit to at least lds bytes: the runtime call
+// is made once per kernel, device and larger size (it costs host time on
+// every launch otherwise)
+int pgk_lds_limit(const void *f, size_t lds);
+
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);
 }
