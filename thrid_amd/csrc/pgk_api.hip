@@ -1111,8 +1111,10 @@ int pgk_energies(pgk_ctx *c, double *ene, double *virt) {
 // d_hj_jcol / d_hj_jpart with the velocities already in d_hj_vel.
 // dev_com: the molecule centres are computed on the device (k_calcom) rather
 // than given by the caller
+// checked: the description was validated before (the MD loop's samples of
+// the description md_hf_begin checked), so the per-atom scans are skipped
 static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *who,
-                    bool upload = true, bool dev_com = false) {
+                    bool upload = true, bool dev_com = false, bool checked = false) {
   if (hf->mode < PGK_HF_VOLUME || hf->mode > PGK_HF_BULK || hf->nregion < 1 || !hf->z1 ||
       !hf->z2 || !hf->hftyp) {
     pgk_set_error(std::string(who) + ": invalid heat-flux description");
@@ -1120,15 +1122,20 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   }
   const int n = c->natom, nreg = hf->nregion;
   bool need_com = false;
-  for (int i = 0; i < n; i++) {
-    if (hf->hftyp[i] != PGK_HFTYP_ATOM && hf->hftyp[i] != PGK_HFTYP_MOLE) {
-      pgk_set_error(std::string(who) + ": hftyp must be ATOM (1) or MOLE (2)");
-      return PGK_EINVAL;
-    }
-    if (hf->hftyp[i] == PGK_HFTYP_MOLE) need_com = true;
-  }
   int maxmol = 0;
-  for (int i = 0; i < n; i++) maxmol = std::max(maxmol, c->molid[i]);
+  if (checked) {
+    need_com = c->md_hf_mole;
+    maxmol = c->hj_nmol;
+  } else {
+    for (int i = 0; i < n; i++) {
+      if (hf->hftyp[i] != PGK_HFTYP_ATOM && hf->hftyp[i] != PGK_HFTYP_MOLE) {
+        pgk_set_error(std::string(who) + ": hftyp must be ATOM (1) or MOLE (2)");
+        return PGK_EINVAL;
+      }
+      if (hf->hftyp[i] == PGK_HFTYP_MOLE) need_com = true;
+    }
+    for (int i = 0; i < n; i++) maxmol = std::max(maxmol, c->molid[i]);
+  }
   if (need_com && !dev_com && (!hf->molecom || hf->nmol < maxmol)) {
     pgk_set_error(std::string(who) + ": MOLE atoms need molecom for every molecule");
     return PGK_EINVAL;
@@ -1593,7 +1600,7 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   const pgk_hf_desc d = {c->md_hf_mode, nreg, c->md_hf_z1.data(), c->md_hf_z2.data(),
                          c->md_hf_typ.data(), 0, nullptr};
   int rc;
-  if ((rc = hf_setup(c, &d, true, "pgk_md_run", false, true))) return rc;
+  if ((rc = hf_setup(c, &d, true, "pgk_md_run", false, true, true))) return rc;
   // MOLE centres: the molecules' centres of mass at this state (calcom.F90)
   if (c->md_hf_mole && (rc = pgk_calcom_launch(c, c->d_md_hfmass, c->d_hf_com))) return rc;
   // with streaming slabs the collision part and the kinetic energies are
