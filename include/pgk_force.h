@@ -570,6 +570,22 @@ int pgk_profile(pgk_ctx *ctx, int32_t enable, double *ms, int64_t *calls);
 /* Stages that record events while timing is on: bit k for stage k (default
  * all).  Timing fewer stages keeps the host's enqueue rate up.               */
 int pgk_profile_stages(pgk_ctx *ctx, uint32_t mask);
+/* Launch-shape choices (no reference counterpart: tuning).  None changes a
+ * result beyond the rounding of the mixed and FP32 forms' FP32 partial sums
+ * (the FP64 form's order of atomic additions is not fixed anyway).
+ *   PGK_TUNE_PAIR_CHUNK_BELOW: systems with fewer 8-atom clusters than this
+ *     run the pair kernel in its chunked form (each wave takes 64 list entries
+ *     of one cluster instead of the cluster's whole list), which fills the
+ *     device for small systems; 0 never, a huge value always.  Default
+ *     PGK_TUNE_DEFAULT_CHUNK_BELOW, the crossover thrid_amd/autotune.py
+ *     measured (thrid_amd/tuning.json).
+ *   PGK_TUNE_PME_OVERLAP: 1 (default) runs the reciprocal chain on a second
+ *     stream beside the pair kernel when both are due, 0 one after the other.
+ *   PGK_TUNE_PAIR_CAP_BLOCKS: pair blocks per CU while the chain runs beside
+ *     them (default 3; 0 no cap).                                          */
+enum { PGK_TUNE_PAIR_CHUNK_BELOW = 1, PGK_TUNE_PME_OVERLAP = 2, PGK_TUNE_PAIR_CAP_BLOCKS = 3 };
+#define PGK_TUNE_DEFAULT_CHUNK_BELOW 4352
+int pgk_tune(pgk_ctx *ctx, int32_t knob, int64_t value);
 /* Shape of the current list (no reference counterpart: instrumentation):
  * info[0] 8-atom clusters, info[1] list entries (an i-cluster against half a
  * j-cluster), info[2] listed atom pairs, info[3] pair slots per entry.       */

@@ -78,7 +78,11 @@ def test_constants_match_header(tmp_path):
              "PGK_PREC_FP32": abi.PREC["fp32"], "PGK_NENE": abi.NENE,
              "PGK_NTENS": abi.NTENS, "PGK_NHF": abi.NHF,
              "PGK_HF_VOLUME": abi.HF_VOLUME, "PGK_HF_SURFACE": abi.HF_SURFACE,
-             "PGK_HF_BULK": abi.HF_BULK}
+             "PGK_HF_BULK": abi.HF_BULK,
+             "PGK_TUNE_PAIR_CHUNK_BELOW": abi.TUNE_PAIR_CHUNK_BELOW,
+             "PGK_TUNE_PME_OVERLAP": abi.TUNE_PME_OVERLAP,
+             "PGK_TUNE_PAIR_CAP_BLOCKS": abi.TUNE_PAIR_CAP_BLOCKS,
+             "PGK_TUNE_DEFAULT_CHUNK_BELOW": abi.TUNE_DEFAULT_CHUNK_BELOW}
     lines = ["#include <stdio.h>", '#include "pgk_force.h"', "int main(void) {"]
     for k in names:
         lines.append(f'  printf("{k} %lld\\n", (long long)({k}));')
@@ -121,3 +125,31 @@ def test_no_device_is_an_error_not_a_fallback():
     from helpers import water648
     with pytest.raises(pgk.PgkError, match="device"):
         pgk.ForceContext(water648(), "fp64")
+
+
+def test_tune_validates_before_the_device():
+    lib = pgk.load_library()
+    assert lib.pgk_tune(None, abi.TUNE_PAIR_CHUNK_BELOW, 0) == abi.PGK_EINVAL
+
+
+def test_pair_form_crossover_rule():
+    from thrid_amd import autotune
+
+    def row(ncl, prec, whole, chunked):
+        return {"clusters": ncl, "precision": prec, "per_cluster_us": whole, "chunked_us": chunked}
+    rows = [row(200, "mixed", 90, 30), row(200, "fp64", 95, 40),
+            row(800, "mixed", 100, 35), row(800, "fp64", 110, 50),
+            row(3200, "mixed", 150, 80), row(3200, "fp64", 160, 170),
+            row(12800, "mixed", 240, 290), row(12800, "fp64", 300, 330)]
+    # chunked wins in both precisions up to 800 clusters, not at 3200
+    assert autotune.crossover(rows) == 1600
+    assert autotune.crossover(rows[:4]) == 1600
+    assert autotune.crossover([row(100, "mixed", 10, 20)]) == 0
+
+
+def test_committed_tuning_table_matches_the_compiled_default():
+    from thrid_amd import autotune
+    t = autotune.load_table()
+    assert t is not None
+    assert t["pair_chunk_below"] == abi.TUNE_DEFAULT_CHUNK_BELOW
+    assert t["pair_chunk_below"] == autotune.crossover(t["measurements"])

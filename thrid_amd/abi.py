@@ -15,6 +15,8 @@ PGK_OK, PGK_EINVAL, PGK_EDEVICE, PGK_ENOMEM, PGK_ESTATE, PGK_EOVERFLOW = 0, -1, 
 PREC = {"fp64": 0, "mixed": 1, "fp32": 2}
 MTS_LONG, MTS_MED, MTS_SHORT = 1, 2, 3
 HF_VOLUME, HF_SURFACE, HF_BULK = 0, 1, 2
+TUNE_PAIR_CHUNK_BELOW, TUNE_PME_OVERLAP, TUNE_PAIR_CAP_BLOCKS = 1, 2, 3
+TUNE_DEFAULT_CHUNK_BELOW = 4352
 HFTYP_ATOM, HFTYP_MOLE = 1, 2
 
 ENE_NAMES = ["pot_tot", "pot_nonbon", "pot_vdw", "pot_elc", "pot_ewk", "pot_ewc",

@@ -408,13 +408,35 @@ __global__ void k_compact_pairs(int ncl_max, const int32_t *__restrict__ ci_coun
   }
 }
 
-// cluster count, overflow flags and entry count in one record for the host
+// Work items of the pair kernel's chunked form: chunk k (entries 64 k ..
+// 64 k + 63 of its list) of cluster ci, coded ci * 32 + k (a cluster holds at
+// most PGK_MAXJ = 2048 entries, 32 chunks).
+__global__ void k_chunk_count(int ncl_max, const int32_t *__restrict__ ci_count,
+                              int32_t *__restrict__ wcount) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci <= ncl_max) wcount[ci] = (ci < ncl_max) ? (ci_count[ci] + 63) / 64 : 0;
+}
+__global__ void k_chunk_fill(int ncl_max, const int32_t *__restrict__ wcount,
+                             const int32_t *__restrict__ wstart, int32_t *__restrict__ witem,
+                             int64_t cap) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncl_max) return;
+  for (int k = 0; k < wcount[ci]; k++) {
+    const int64_t w = (int64_t)wstart[ci] + k;
+    if (w < cap) witem[w] = ci * 32 + k;
+  }
+}
+
+// cluster count, overflow flags, entry count and work-item count in one
+// record for the host
 __global__ void k_list_meta(const int32_t *__restrict__ ncl, const int32_t *__restrict__ ovf,
-                            const int32_t *__restrict__ tot, int32_t *__restrict__ meta) {
+                            const int32_t *__restrict__ tot, const int32_t *__restrict__ wtot,
+                            int32_t *__restrict__ meta) {
   if (threadIdx.x == 0) {
     meta[0] = *ncl;
     meta[1] = *ovf;
     meta[2] = *tot;
+    meta[3] = *wtot;
   }
 }
 
@@ -512,6 +534,9 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
         (rc = dev_alloc(&c->d_tmp_mask, (size_t)ncl_max * PGK_MAXJ)) ||
         (rc = dev_alloc(&c->d_ci_count, (size_t)ncl_max + 4)) ||
         (rc = dev_alloc(&c->d_ci_start, (size_t)ncl_max + 4)) ||
+        (rc = dev_alloc(&c->d_wcount, (size_t)ncl_max + 4)) ||
+        (rc = dev_alloc(&c->d_wstart, (size_t)ncl_max + 4)) ||
+        (rc = dev_alloc(&c->d_witem, (size_t)ncl_max * (PGK_MAXJ / 64))) ||
         (rc = dev_alloc((int32_t **)&c->d_ct, ns)))
       return rc;
     size_t xqb = (c->precision == PGK_PREC_FP64) ? 32 : 16;
@@ -603,12 +628,21 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
   k_compact_pairs<<<(ncl_max + 3) / 4, 256, 0, st>>>(ncl_max, c->d_ci_count, c->d_ci_start,
                                                      c->d_tmp_cj, c->d_tmp_mask, c->d_cj,
                                                      c->d_mask, c->cap_pair_cl, c->d_overflow);
-  k_list_meta<<<1, 64, 0, st>>>(d_ncl, c->d_overflow, c->d_ci_start + ncl_max, c->d_list_meta);
+  k_chunk_count<<<pgk_grid_blocks(ncl_max + 1, 256), 256, 0, st>>>(ncl_max, c->d_ci_count,
+                                                                   c->d_wcount);
+  tb = c->scan_tmp_bytes;
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, tb, c->d_wcount, c->d_wstart,
+                                             ncl_max + 1, st));
+  k_chunk_fill<<<pgk_grid_blocks(ncl_max, 256), 256, 0, st>>>(
+      ncl_max, c->d_wcount, c->d_wstart, c->d_witem, (int64_t)ncl_max * (PGK_MAXJ / 64));
+  k_list_meta<<<1, 64, 0, st>>>(d_ncl, c->d_overflow, c->d_ci_start + ncl_max,
+                                c->d_wstart + ncl_max, c->d_list_meta);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(c->h_list_meta, c->d_list_meta, 3 * sizeof(int32_t),
+  HIP_CHECK(hipMemcpyAsync(c->h_list_meta, c->d_list_meta, 4 * sizeof(int32_t),
                            hipMemcpyDeviceToHost, st));
   HIP_CHECK(hipStreamSynchronize(st));
   const int32_t ncl = c->h_list_meta[0], ovf = c->h_list_meta[1], tot = c->h_list_meta[2];
+  c->nwi = c->h_list_meta[3];
   if (ovf & 1) {
     pgk_set_error("pgk_build_list: more than 2048 neighbour candidates or half-cluster "
                   "entries for one cluster (rcut_book too large for the cluster list)");

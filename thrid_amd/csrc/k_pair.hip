@@ -22,6 +22,7 @@
 
 #include "pgk_device.h"
 
+
 template <int PREC>
 struct PairArgs {
   using R = typename Prec<PREC>::real;
@@ -35,6 +36,8 @@ struct PairArgs {
   const R *lj;           // ntype^2 x 2: eps (0 unless INTTYPE_VDW), rmin^2
   A *fcl, *fcl_c;
   double *red;
+  const int32_t *witem;  // chunked form: work items (cluster * 32 + chunk)
+  int nwi;
   int ncl, ntype, nspl_lds, lj_lds;
   R L[3], Linv[3];
   R sqcut, sqcut2, rrsq;
@@ -157,7 +160,12 @@ __device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_i
 
 // ENE: energies and virial tensors (a call whose energies nobody reads, such
 // as an MD step that samples nothing, skips them and writes a zero record).
-template <int PREC, bool SPLIT, bool LJLDS, int NW, bool ENE>
+// CH (chunked form): a wave takes one work item, 64 entries of one cluster's
+// list, instead of the whole list, so a small system's few clusters still
+// spread over the device; the cluster's i-forces are then added once per
+// chunk (fixed point in the mixed form, so the sums do not depend on which
+// wave adds them).
+template <int PREC, bool SPLIT, bool LJLDS, int NW, bool ENE, bool CH>
 __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   using R = typename Prec<PREC>::real;
   using R4 = typename Prec<PREC>::real4;
@@ -177,7 +185,16 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
   const R *ljt = LJLDS ? s_lj : a.lj;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = uni32(blockIdx.x * NW + wave);
+  int ci = uni32(blockIdx.x * NW + wave), chunk = -1;
+  if (CH) {
+    const int w = ci;
+    ci = a.ncl;
+    if (w < a.nwi) {
+      const int item = uni32(a.witem[w]);
+      ci = item >> 5;
+      chunk = item & 31;
+    }
+  }
   double ev[14];
 #pragma unroll
   for (int k = 0; k < 14; k++) ev[k] = 0.0;
@@ -191,7 +208,11 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
     IA fix = 0, fiy = 0, fiz = 0, cix = 0, ciy = 0, ciz = 0;
     R e_elc = 0, e_vdw = 0;
     R tc[6] = {0, 0, 0, 0, 0, 0}, tl[6] = {0, 0, 0, 0, 0, 0};
-    const int e0 = uni32(a.ci_start[ci]), e1 = uni32(a.ci_start[ci + 1]);
+    int e0 = uni32(a.ci_start[ci]), e1 = uni32(a.ci_start[ci + 1]);
+    if (CH) {
+      e0 += 64 * chunk;
+      e1 = min(e1, e0 + 64);
+    }
     // one pass: two half-cluster entries, masks combined as m (bits 0-31 the
     // first, 32-63 the second), hc the lane's half-cluster, xj tj its j-slot
     auto entry = [&](const int hc, const uint64_t m, const R4 xj, const int tj) {
@@ -467,6 +488,8 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   }
   a.fcl = reinterpret_cast<A *>(c->d_fcl);
   a.fcl_c = reinterpret_cast<A *>(c->d_fcl_c);
+  a.witem = c->d_witem;
+  a.nwi = c->nwi;
   a.ncl = c->ncl;
   a.ntype = c->ntype;
   // only the table rows up to rrcut are ever read
@@ -492,19 +515,26 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   a.rrsq = (R)c->rrsq_cut;
   a.h = (R)c->spl_int;
   a.hinv = (R)(1.0 / c->spl_int);
-  int nblk = pgk_grid_blocks(c->ncl, NW);
+  // chunked form for small systems, whose clusters do not fill the device
+  // with one wave each
+  const bool ch = (int64_t)c->ncl < c->pair_chunk_below;
+  int nblk = pgk_grid_blocks(ch ? c->nwi : c->ncl, NW);
   *off = c->red_used;
   *nrec = nblk;
   a.red = c->d_red + (size_t)c->red_used * PGK_REC;
   c->red_used += nblk;
-#define PGK_LAUNCH_PAIRS(SP, LL, EN)                                                    \
+#define PGK_LAUNCH_PAIRS_C(SP, LL, EN, CHK)                                             \
   {                                                                                      \
-    auto kf = k_pairs<PREC, SP, LL, NW, EN>;                                             \
+    auto kf = k_pairs<PREC, SP, LL, NW, EN, CHK>;                                        \
     {                                                                                    \
       const int lr = pgk_lds_limit((const void *)kf, lds);                               \
       if (lr) return lr;                                                                 \
     }                                                                                    \
     kf<<<nblk, 64 * NW, lds, c->stream>>>(a);                                            \
+  }
+#define PGK_LAUNCH_PAIRS(SP, LL, EN)                                                    \
+  {                                                                                      \
+    if (ch) PGK_LAUNCH_PAIRS_C(SP, LL, EN, true) else PGK_LAUNCH_PAIRS_C(SP, LL, EN, false) \
   }
   if (split) {   // the per-class forces come with their energies
     if (a.lj_lds) PGK_LAUNCH_PAIRS(true, true, true) else PGK_LAUNCH_PAIRS(true, false, true)
@@ -514,6 +544,7 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
     if (a.lj_lds) PGK_LAUNCH_PAIRS(false, true, false) else PGK_LAUNCH_PAIRS(false, false, false)
   }
 #undef PGK_LAUNCH_PAIRS
+#undef PGK_LAUNCH_PAIRS_C
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

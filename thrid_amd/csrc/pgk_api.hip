@@ -274,7 +274,8 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   const size_t ab = (c->precision == PGK_PREC_FP32) ? 4 : 8;
   // capacity of the record buffer: every launch appends its blocks
   int need = 16;
-  if (do_pair) need += pgk_grid_blocks(c->ncl, 4) + pgk_grid_blocks(c->natom, 256) + 1;
+  if (do_pair)
+    need += pgk_grid_blocks(std::max(c->ncl, c->nwi), 4) + pgk_grid_blocks(c->natom, 256) + 1;
   {
     int H = c->nfft[0] / 2 + 1;
     need += c->nfft[1] * H + 16;
@@ -344,8 +345,8 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipGetLastError());
     }
     pgk_prof_begin(c, PGK_PROF_PAIRS);
-    // beside the chain, at most three pair blocks per CU
-    c->pair_cap_blocks = overlap ? 3 : 0;
+    // beside the chain, a cap on the pair blocks per CU
+    c->pair_cap_blocks = overlap ? c->pair_cap_default : 0;
     rc = pgk_pairs_launch(c, split, energies, &seg_off[nseg], &seg_n[nseg]);
     c->pair_cap_blocks = 0;
     if (rc) return rc;
@@ -897,7 +898,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_atom_bin, c->d_atom_slot, c->d_sorted, c->d_col_ncl, c->d_col_clstart,
                   c->d_cl_atom, c->d_cl_col, c->d_slot_of, c->d_bbox, c->d_posc, c->d_tmp_cj,
                   c->d_list_meta,
-                  c->d_tmp_mask, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
+                  c->d_tmp_mask, c->d_wcount, c->d_wstart, c->d_witem, c->d_ci_count, c->d_ci_start, c->d_cj, c->d_mask,
                   c->d_overflow, c->d_scan_tmp, c->d_xq, c->d_ct, c->d_fcl, c->d_fat,
                   c->d_fcl_c, c->d_fat_c, c->d_force, c->d_fvc, c->d_fvl, c->d_red, c->d_sum,
                   c->d_grid, c->d_pgrid, c->d_cgrid, c->d_tw, c->d_twf, c->d_theta, c->d_tile,
@@ -2296,6 +2297,32 @@ int pgk_profile_stages(pgk_ctx *c, uint32_t mask) {
   if (!c) return PGK_EINVAL;
   c->prof_mask = mask;
   return PGK_OK;
+}
+
+int pgk_tune(pgk_ctx *c, int32_t knob, int64_t value) {
+  if (!c) {
+    pgk_set_error("pgk_tune: null context");
+    return PGK_EINVAL;
+  }
+  switch (knob) {
+    case PGK_TUNE_PAIR_CHUNK_BELOW:
+      if (value < 0) break;
+      c->pair_chunk_below = value;
+      return PGK_OK;
+    case PGK_TUNE_PME_OVERLAP:
+      if (value != 0 && value != 1) break;
+      c->pme_overlap = value != 0;
+      return PGK_OK;
+    case PGK_TUNE_PAIR_CAP_BLOCKS:
+      if (value < 0 || value > 16) break;
+      c->pair_cap_default = (int)value;
+      return PGK_OK;
+    default:
+      pgk_set_error("pgk_tune: unknown knob " + std::to_string(knob));
+      return PGK_EINVAL;
+  }
+  pgk_set_error("pgk_tune: value out of range for knob " + std::to_string(knob));
+  return PGK_EINVAL;
 }
 
 int pgk_list_info(pgk_ctx *c, int64_t *info) {

@@ -76,11 +76,12 @@ def load_library(path: str = LIB_PATH):
     L.pgk_profile.argtypes = [vp, C.c_int32, vp, vp]
     L.pgk_list_info.argtypes = [vp, vp]
     L.pgk_profile_stages.argtypes = [vp, C.c_uint32]
+    L.pgk_tune.argtypes = [vp, C.c_int32, C.c_int64]
     for f in ("pgk_create", "pgk_destroy", "pgk_set_box", "pgk_set_positions",
               "pgk_set_positions_device", "pgk_build_list", "pgk_list_npairs",
               "pgk_list_pairs", "pgk_calforce", "pgk_calforce_hf", "pgk_calforce_device",
               "pgk_calforce_async", "pgk_energies", "pgk_synchronize", "pgk_profile",
-              "pgk_list_info", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
+              "pgk_list_info", "pgk_tune", "pgk_md_setup", "pgk_md_set_velocities", "pgk_md_get_state",
               "pgk_md_run", "pgk_md_use_graphs", "pgk_md_run_batch", "pgk_profile_stages",
               "pgk_md_nhc", "pgk_md_nhc_state", "pgk_calheatf", "pgk_calforce_heatf",
               "pgk_set_atmvel_strm",
@@ -100,13 +101,26 @@ def _check(rc, what):
         raise PgkError(f"{what} failed ({rc}): {msg}")
 
 
+_TUNED = []
+
+
+def _tuned_chunk_below():
+    """The pair-form crossover of the tuning table written by
+    thrid_amd.autotune (None without a table)."""
+    if not _TUNED:
+        from thrid_amd import autotune
+        t = autotune.load_table()
+        _TUNED.append(None if t is None else int(t["pair_chunk_below"]))
+    return _TUNED[0]
+
+
 class ForceContext:
     """One system on the current GPU.
 
     precision: 'fp64' (the reference's arithmetic), 'mixed' (FP32 pair/FFT,
     64-bit fixed-point force sums) or 'fp32'."""
 
-    def __init__(self, sysd, precision="fp64", excl=None, ifnetqcorrp=0):
+    def __init__(self, sysd, precision="fp64", excl=None, ifnetqcorrp=0, tuned=True):
         self.L = load_library()
         self.sysd = sysd
         self.n = int(sysd["natom"])
@@ -116,6 +130,10 @@ class ForceContext:
         _check(self.L.pgk_create(C.byref(self._desc), abi.PREC[precision], C.byref(h)),
                "pgk_create")
         self.h = h
+        if tuned:
+            below = _tuned_chunk_below()
+            if below is not None and below != abi.TUNE_DEFAULT_CHUNK_BELOW:
+                self.tune(abi.TUNE_PAIR_CHUNK_BELOW, below)
         self.set_box(sysd["box"])
         self.set_positions(sysd["pos"])
 
@@ -474,6 +492,10 @@ class ForceContext:
         _check(self.L.pgk_profile(self.h, int(enable), ms.ctypes.data, calls.ctypes.data),
                "pgk_profile")
         return ms, calls
+
+    def tune(self, knob, value):
+        """Set a launch-shape choice (abi.TUNE_*); see pgk_tune."""
+        _check(self.L.pgk_tune(self.h, int(knob), int(value)), "pgk_tune")
 
     def profile_stages(self, mask):
         """Stages timed while profiling is on: bit k = stage k (PAIRS = 0)."""
