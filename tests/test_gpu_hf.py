@@ -38,12 +38,14 @@ def regions(box_z, n):
     return z[:-1], z[1:]
 
 
-def _run(oracle_mod, s, hf, prec):
+def _run(oracle_mod, s, hf, prec, chunk_below=None):
     from thrid_amd.pgk import ForceContext
     o = oracle_mod.Oracle(s)
     o.mklist(s["rcut_book"], 1)
     ref = o.calforce_hf(hf)
     c = ForceContext(s, prec)
+    if chunk_below is not None:
+        c.tune(abi.TUNE_PAIR_CHUNK_BELOW, chunk_below)
     c.mklist(s["rcut_book"])
     got = c.calforce_hf(hf)
     return ref, got, c
@@ -199,19 +201,22 @@ def test_calheatf_order_of_calls():
         c.calheatf(vel)                       # positions changed since the call
 
 
+@pytest.mark.parametrize("form", ["chunked", "per_cluster"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("mode", [abi.HF_VOLUME, abi.HF_BULK])
-def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode):
+def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode, form):
     """The fused pass (no W arrays: each interaction's fi.v_i - fj.v_j times
     its region weights) against calforce_hf + calheatf on the same context,
-    and end to end against the oracle; forces and energies are calforce_hf's."""
+    and end to end against the oracle; forces and energies are calforce_hf's.
+    Both forms of the passes over the list: one wave per 64 entries of a
+    cluster (this system's default) and one wave per cluster."""
     from thrid_amd.pgk import PgkError
     s = small_slab("rb")
     lz = s["box"][2]
     hf = dict(mode=mode, z1=np.array([0.0, 0.3 * lz, 0.45 * lz, 0.2 * lz]),
               z2=np.array([0.5 * lz, 0.6 * lz, lz, 0.7 * lz]), hftyp=_hf_centres(s),
               molecom=molecom(s))
-    ref, got, c = _run(oracle_mod, s, hf, prec)
+    ref, got, c = _run(oracle_mod, s, hf, prec, 1 << 40 if form == "chunked" else 0)
     rng = np.random.default_rng(23)
     vel = rng.normal(scale=0.02, size=(s["natom"], 3))
     mass = s["mass"] * 0.37

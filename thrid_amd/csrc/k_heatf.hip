@@ -247,7 +247,8 @@ int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   HIP_CHECK(hipGetLastError());
   if (c->hfdev.flux) {
     k_heatf_collect<<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
-        a.nreg, a.bulk, pgk_grid_blocks(c->ncl, 4), c->hfdev.jcol, c->hfdev.jpart, pcol);
+        a.nreg, a.bulk, pgk_grid_blocks(pgk_pair_waves(c), 4), c->hfdev.jcol, c->hfdev.jpart,
+        pcol);
   } else {
     k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
   }

@@ -1,5 +1,6 @@
-// k_hf.hip - heat-flux pass of calforce_hf over the listed pairs, and the
-// export of the per-atom arrays.
+// k_hf.hip - fused heat-flux pass over the listed pairs (pgk_calforce_heatf
+// and the MD loop's samples), and the export of calforce_hf's per-atom arrays
+// (whose pair pass is k_hf_w.hip).
 //
 // Restates the per-atom part of ewaldrp_hf (ewaldr_hf.F90:8-670) in the
 // default build: for every listed pair with r <= rrcut the bare Coulomb
@@ -10,10 +11,12 @@
 // weight 1/2.  Excluded pairs and the reciprocal sum add no per-atom terms in
 // this build (ewaldr_hf.F90:637-653 only under _HF_BULK_EWK).
 //
-// Arithmetic is FP64 in every precision form (the cutoff tests are the
-// reference's: no contraction, r = sqrt(r^2), r <= rrcut).  The pair terms
+// The cutoff tests are the reference's in every precision form (FP64, no
+// contraction, r = sqrt(r^2), r <= rrcut); the pair terms are FP64, or FP32
+// in the mixed form.  The pair terms
 // are symmetric under i <-> j, so the cluster list's orientation of a pair
-// does not matter.  One 64-lane wave per i-cluster, lane = ii + 8 jj.
+// does not matter.  One 64-lane wave per i-cluster (or per 64 entries of one
+// in the chunked form), lane = ii + 8 jj.
 #include <type_traits>
 
 #include "pgk_hf.h"
@@ -26,7 +29,8 @@ struct HfPairArgs {
   const double *q;
   const int32_t *type, *molid, *inttyp;
   const double *eps, *rmin;
-  int ncl, ntype;
+  const int32_t *witem;   // chunked form: work items (cluster * 32 + chunk of 64 entries)
+  int ncl, nwi, ntype;
   double sqcut, sqcut2, rrcut;
   double rrsq;   // largest r^2 with sqrt(r^2) <= rrcut: r2 <= rrsq decides r <= rrcut exactly
 };
@@ -34,63 +38,6 @@ struct HfPairArgs {
 __device__ __forceinline__ double r2_exact(const double *r) {
 #pragma clang fp contract(off)
   return r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-}
-
-template <class AH>
-__global__ __launch_bounds__(256) void k_hf_pairs(HfPairArgs p, HfDev h) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
-  if (ci >= p.ncl) return;
-  // lane = ii + 8 jq + 32 hs: i-slot ii, j-slot jq of entry e + hs
-  const int ii = lane & 7, jq = (lane >> 3) & 3, hs = lane >> 5;
-  const int ai = p.cl_atom[ci * PGK_CL + ii];
-  const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
-  for (int e2 = e0; e2 < e1; e2 += 2) {
-    const int e = e2 + hs;
-    if (e >= e1) continue;
-    const uint32_t m = p.mask[e];
-    if (!((m >> (lane & 31)) & 1u)) continue;
-    const int aj = p.cl_atom[p.cj[e] * 4 + jq];
-    double r[3];
-    for (int d = 0; d < 3; d++) {
-      r[d] = h.pos[3 * ai + d] - h.pos[3 * aj + d];
-      r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
-    }
-    const double r2 = r2_exact(r);
-    if (r2 > p.sqcut) continue;
-    const double rinv2 = 1.0 / r2;
-    const double ra = sqrt(r2);
-    const double rinv1 = 1.0 / ra;
-    const bool same = p.molid[ai] == p.molid[aj];
-    if (ra <= p.rrcut) {
-      const double qq = p.q[ai] * p.q[aj];
-      const double ph = qq * rinv1;
-      const double fh = ph * rinv2;
-      const double fi[3] = {fh * r[0], fh * r[1], fh * r[2]};
-      const double fj[3] = {-fi[0], -fi[1], -fi[2]};
-      const int cls = same ? PGK_HF_ELIN : PGK_HF_ELC;
-      hf_pe_add<AH>(h, cls, ai, 0.5 * ph);
-      hf_pe_add<AH>(h, cls, aj, 0.5 * ph);
-      calvirihf_dev<AH>(h, ai, aj, fi, fj, 0.5, cls);
-    }
-    if (r2 <= p.sqcut2) {
-      const size_t tij = (size_t)(p.type[ai] - 1) + (size_t)p.ntype * (p.type[aj] - 1);
-      if (p.inttyp[tij] != 1) continue;
-      const double rad = p.rmin[tij];
-      const double t2 = rinv2 * (rad * rad);
-      const double t6 = t2 * t2 * t2;
-      const double t12 = t6 * t6;
-      const double ep = p.eps[tij];
-      const double pot = ep * (t12 - 2.0 * t6);
-      const double ft = 12.0 * ep * (t12 - t6) * rinv2;
-      const double fi[3] = {ft * r[0], ft * r[1], ft * r[2]};
-      const double fj[3] = {-fi[0], -fi[1], -fi[2]};
-      const int cls = same ? PGK_HF_LJIN : PGK_HF_VDW;
-      hf_pe_add<AH>(h, cls, ai, 0.5 * pot);
-      hf_pe_add<AH>(h, cls, aj, 0.5 * pot);
-      calvirihf_dev<AH>(h, ai, aj, fi, fj, 0.5, cls);
-    }
-  }
 }
 
 // Flux form of the pass (pgk_calforce_heatf): the same pairs, energies and
@@ -153,14 +100,26 @@ template <> __device__ __forceinline__ float hf_rinv<float>(double r2) {
   return __builtin_amdgcn_rsqf((float)r2);
 }
 
-template <class AH, bool BULK>
+// CH: the chunked form, one wave per work item (64 list entries of one
+// cluster) instead of one wave per cluster; the i atom's energies are then
+// added once per item
+template <class AH, bool BULK, bool CH>
 __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
   // the mixed form (fixed-point accumulators) computes the pair terms in FP32
   using R = typename std::conditional<std::is_same<AH, unsigned long long>::value, float,
                                       double>::type;
   extern __shared__ double lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
+  int ci = blockIdx.x * 4 + wave, chunk = 0;
+  if (CH) {
+    const int w = ci;
+    ci = p.ncl;
+    if (w < p.nwi) {
+      const int item = p.witem[w];
+      ci = item >> 5;
+      chunk = item & 31;
+    }
+  }
   const int nreg = h.nreg;
   double acc[4][3] = {};
   double *sl = lds + (size_t)wave * 4 * nreg;   // volume: this wave's slice
@@ -189,7 +148,11 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
     // list; the j atom's are summed over the 8 lanes that share it (lanes
     // ii = 0..7 of one (jq, hs) group) and added once per entry
     double ei[4] = {0.0, 0.0, 0.0, 0.0};   // ELIN, LJIN, ELC, VDW
-    const int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
+    int e0 = p.ci_start[ci], e1 = p.ci_start[ci + 1];
+    if (CH) {
+      e0 += 64 * chunk;
+      e1 = min(e1, e0 + 64);
+    }
     for (int e2 = e0; e2 < e1; e2 += 2) {
       const int e = e2 + hs;
       const bool ein = e < e1;
@@ -333,23 +296,31 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.inttyp = c->d_inttyp;
     p.eps = c->d_eps;
     p.rmin = c->d_rmin;
+    p.witem = c->d_witem;
     p.ncl = c->ncl;
+    p.nwi = c->nwi;
     p.ntype = c->ntype;
     double sq = c->rrcut * c->rrcut, sq2 = c->rcut * c->rcut;
     p.sqcut = sq2 > sq ? sq2 : sq;
     p.sqcut2 = sq2;
     p.rrcut = c->rrcut;
     p.rrsq = c->rrsq_cut;
-    const int nblk = pgk_grid_blocks(c->ncl, 4);
+    // the block count sizes jpart (hf_setup) and the collect pass reads it
+    const int nblk = pgk_grid_blocks(pgk_pair_waves(c), 4);
+    const bool ch = pgk_pair_chunked(c);
     if (!h.flux) {
       return pgk_hf_w_launch(c);   // per-atom arrays: k_hf_w.hip
     } else if (h.mode == PGK_HF_BULK) {
-      k_hf_pairs_flux<AH, true><<<nblk, 256, 0, st>>>(p, h);
+      if (ch)
+        k_hf_pairs_flux<AH, true, true><<<nblk, 256, 0, st>>>(p, h);
+      else
+        k_hf_pairs_flux<AH, true, false><<<nblk, 256, 0, st>>>(p, h);
     } else {
       const size_t lds = sizeof(double) * 16 * (size_t)h.nreg;
-      const int lr = pgk_lds_limit((const void *)k_hf_pairs_flux<AH, false>, lds);
+      auto kf = ch ? k_hf_pairs_flux<AH, false, true> : k_hf_pairs_flux<AH, false, false>;
+      const int lr = pgk_lds_limit((const void *)kf, lds);
       if (lr) return lr;
-      k_hf_pairs_flux<AH, false><<<nblk, 256, lds, st>>>(p, h);
+      kf<<<nblk, 256, lds, st>>>(p, h);
     }
     HIP_CHECK(hipGetLastError());
   }

@@ -517,8 +517,8 @@ static int launch_pairs(pgk_ctx *c, bool split, bool ene, int *off, int *nrec) {
   a.hinv = (R)(1.0 / c->spl_int);
   // chunked form for small systems, whose clusters do not fill the device
   // with one wave each
-  const bool ch = (int64_t)c->ncl < c->pair_chunk_below;
-  int nblk = pgk_grid_blocks(ch ? c->nwi : c->ncl, NW);
+  const bool ch = pgk_pair_chunked(c);
+  int nblk = pgk_grid_blocks(pgk_pair_waves(c), NW);
   *off = c->red_used;
   *nrec = nblk;
   a.red = c->d_red + (size_t)c->red_used * PGK_REC;

@@ -1208,7 +1208,7 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   h.jpart = nullptr;
   if (flux) {
     const size_t jcol_b = sizeof(double) * PGK_NHF * (size_t)nreg * 3;
-    const size_t nblk = (size_t)pgk_grid_blocks(std::max(c->ncl, 1), 4);
+    const size_t nblk = (size_t)pgk_grid_blocks(std::max<int64_t>(pgk_pair_waves(c), 1), 4);
     const size_t jpart_b =
         sizeof(double) * nblk * (hf->mode == PGK_HF_BULK ? 12 : 4 * (size_t)nreg);
     if ((rc = grow_dev((void **)&c->d_hj_jcol, &c->hj_jcol_cap, jcol_b)) ||

@@ -398,3 +398,15 @@ int pgk_lds_limit(const void *f, size_t lds);
 static inline int pgk_grid_blocks(int64_t n, int bs) {
   return (int)((n + bs - 1) / bs);
 }
+
+// the pair passes over the list (the force pass and the fused heat-flux pass)
+// run one wave per 64 entries of a cluster instead of one wave per cluster
+// when the system has too few clusters to fill the device
+static inline bool pgk_pair_chunked(const pgk_ctx *c) {
+  return (int64_t)c->ncl < c->pair_chunk_below;
+}
+
+// waves of a pass over the current list in the form pgk_pair_chunked picks
+static inline int64_t pgk_pair_waves(const pgk_ctx *c) {
+  return pgk_pair_chunked(c) ? (int64_t)c->nwi : (int64_t)c->ncl;
+}
