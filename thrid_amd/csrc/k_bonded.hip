@@ -42,15 +42,25 @@ struct BArgs {
 };
 
 // rec: E, V, T(9 in Fortran (a,n) order)
-template <class A, class AH, bool HF>
+template <class A, class AH, int HFM>
 __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restrict__ bl, int ok,
                                                int orr, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
+  extern __shared__ __align__(8) unsigned char bl_lds[];
+  AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
+  AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
+  if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = bl[3 * t], j = bl[3 * t + 1], ty = bl[3 * t + 2];
+    HfAt at[2];
+    constexpr bool fx = HFM >= 2, fb = HFM == 2;   // flux form (bulk): atoms loaded up front
+    if constexpr (fx) {
+      hf_at_load<fb>(h, i, at[0]);
+      hf_at_load<fb>(h, j, at[1]);
+    }
     double K = a.par[ok + ty], r0 = a.par[orr + ty];
     if (fabs(K) >= 1.0e-16) {
       double r[3];
@@ -62,29 +72,43 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
       double f[3] = {fa * r[0] / ra, fa * r[1] / ra, fa * r[2] / ra};
       addf(fat, i, f, 1.0);
       addf(fat, j, f, -1.0);
-      if (HF) {   // calbondp_hf (calbond_hf.F90:6-164)
+      if (HFM) {   // calbondp_hf (calbond_hf.F90:6-164)
         const double fj[3] = {-f[0], -f[1], -f[2]};
         hf_pe_add<AH>(h, PGK_HF_BO, i, 0.5 * ev[0]);
         hf_pe_add<AH>(h, PGK_HF_BO, j, 0.5 * ev[0]);
-        calvirihf_dev<AH>(h, i, j, f, fj, 0.5, PGK_HF_BO);
+        if constexpr (fx) calvirihf_flux_at<AH, fb>(h, at[0], at[1], f, fj, 0.5, jb, sj);
+        else calvirihf_dev<AH>(h, i, j, f, fj, 0.5, PGK_HF_BO);
       }
       ev[1] = dot3(f, r);
       for (int nn = 0; nn < 3; nn++)
         for (int aa = 0; aa < 3; aa++) ev[2 + aa + 3 * nn] = f[aa] * r[nn];
     }
   }
+  if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_BO);
+  if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_BO);
   block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
-template <class A, class AH, bool HF>
+template <class A, class AH, int HFM>
 __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restrict__ al, int ok,
                                                 int ot, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
+  extern __shared__ __align__(8) unsigned char bl_lds[];
+  AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
+  AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
+  if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = al[4 * t], j = al[4 * t + 1], k = al[4 * t + 2], ty = al[4 * t + 3];
+    HfAt at[3];
+    constexpr bool fx = HFM >= 2, fb = HFM == 2;   // flux form (bulk): atoms loaded up front
+    if constexpr (fx) {
+      hf_at_load<fb>(h, i, at[0]);
+      hf_at_load<fb>(h, j, at[1]);
+      hf_at_load<fb>(h, k, at[2]);
+    }
     double K = a.par[ok + ty], t0 = a.par[ot + ty];
     if (fabs(K) >= 1.0e-32) {
       double rij[3], rkj[3];
@@ -112,14 +136,20 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
       addf(fat, i, fi, 1.0);
       addf(fat, j, fj, 1.0);
       addf(fat, k, fk, 1.0);
-      if (HF) {   // calanglp_hf (calangl_hf.F90:6-255): 1/3 per atom, three scatters
+      if (HFM) {   // calanglp_hf (calangl_hf.F90:6-255): 1/3 per atom, three scatters
         const double c3 = 1.0 / 3.0;
         hf_pe_add<AH>(h, PGK_HF_AN, i, c3 * ev[0]);
         hf_pe_add<AH>(h, PGK_HF_AN, j, c3 * ev[0]);
         hf_pe_add<AH>(h, PGK_HF_AN, k, c3 * ev[0]);
-        calvirihf_dev<AH>(h, i, j, fi, fj, c3, PGK_HF_AN);
-        calvirihf_dev<AH>(h, j, k, fj, fk, c3, PGK_HF_AN);
-        calvirihf_dev<AH>(h, k, i, fk, fi, c3, PGK_HF_AN);
+        if constexpr (fx) {
+          calvirihf_flux_at<AH, fb>(h, at[0], at[1], fi, fj, c3, jb, sj);
+          calvirihf_flux_at<AH, fb>(h, at[1], at[2], fj, fk, c3, jb, sj);
+          calvirihf_flux_at<AH, fb>(h, at[2], at[0], fk, fi, c3, jb, sj);
+        } else {
+          calvirihf_dev<AH>(h, i, j, fi, fj, c3, PGK_HF_AN);
+          calvirihf_dev<AH>(h, j, k, fj, fk, c3, PGK_HF_AN);
+          calvirihf_dev<AH>(h, k, i, fk, fi, c3, PGK_HF_AN);
+        }
       }
       const double *pi = a.pos + 3 * i, *pj = a.pos + 3 * j, *pk = a.pos + 3 * k;
       ev[1] = dot3(fi, pi) + dot3(fj, pj) + dot3(fk, pk);
@@ -128,6 +158,8 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
           ev[2 + aa + 3 * nn] = fi[aa] * pi[nn] + fj[aa] * pj[nn] + fk[aa] * pk[nn];
     }
   }
+  if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_AN);
+  if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_AN);
   block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
@@ -190,15 +222,27 @@ __device__ __forceinline__ void tforces(TGeo &g, double fa, double *fi, double *
 }
 
 // RB = true: C_0..C_5 at par[oc + 6*ty]; false: periodic (div, barhig, phase, period)
-template <class A, class AH, bool HF, bool RB>
+template <class A, class AH, int HFM, bool RB>
 __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict__ tl, int o0,
                                               int o1, int o2, int o3, A *fat, HfDev h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
+  extern __shared__ __align__(8) unsigned char bl_lds[];
+  AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
+  AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
+  if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = tl[5 * t], j = tl[5 * t + 1], k = tl[5 * t + 2], l = tl[5 * t + 3], ty = tl[5 * t + 4];
+    HfAt at[4];
+    constexpr bool fx = HFM >= 2, fb = HFM == 2;   // flux form (bulk): atoms loaded up front
+    if constexpr (fx) {
+      hf_at_load<fb>(h, i, at[0]);
+      hf_at_load<fb>(h, j, at[1]);
+      hf_at_load<fb>(h, k, at[2]);
+      hf_at_load<fb>(h, l, at[3]);
+    }
     TGeo g;
     tgeo(a.pos, i, j, k, l, a.L, a.Li, g);
     double fa, dp;
@@ -234,18 +278,27 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
     addf(fat, j, fj, 1.0);
     addf(fat, k, fk, 1.0);
     addf(fat, l, fl, 1.0);
-    if (HF) {   // caltors(rb)p_hf: 1/4 per atom, six pair scatters
+    if (HFM) {   // caltors(rb)p_hf: 1/4 per atom, six pair scatters
       const double c4 = 0.25;
       hf_pe_add<AH>(h, PGK_HF_TO, i, c4 * dp);
       hf_pe_add<AH>(h, PGK_HF_TO, j, c4 * dp);
       hf_pe_add<AH>(h, PGK_HF_TO, k, c4 * dp);
       hf_pe_add<AH>(h, PGK_HF_TO, l, c4 * dp);
-      calvirihf_dev<AH>(h, i, j, fi, fj, c4, PGK_HF_TO);
-      calvirihf_dev<AH>(h, i, k, fi, fk, c4, PGK_HF_TO);
-      calvirihf_dev<AH>(h, i, l, fi, fl, c4, PGK_HF_TO);
-      calvirihf_dev<AH>(h, j, k, fj, fk, c4, PGK_HF_TO);
-      calvirihf_dev<AH>(h, j, l, fj, fl, c4, PGK_HF_TO);
-      calvirihf_dev<AH>(h, k, l, fk, fl, c4, PGK_HF_TO);
+      if constexpr (fx) {
+        calvirihf_flux_at<AH, fb>(h, at[0], at[1], fi, fj, c4, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[0], at[2], fi, fk, c4, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[0], at[3], fi, fl, c4, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[1], at[2], fj, fk, c4, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[1], at[3], fj, fl, c4, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[2], at[3], fk, fl, c4, jb, sj);
+      } else {
+        calvirihf_dev<AH>(h, i, j, fi, fj, c4, PGK_HF_TO);
+        calvirihf_dev<AH>(h, i, k, fi, fk, c4, PGK_HF_TO);
+        calvirihf_dev<AH>(h, i, l, fi, fl, c4, PGK_HF_TO);
+        calvirihf_dev<AH>(h, j, k, fj, fk, c4, PGK_HF_TO);
+        calvirihf_dev<AH>(h, j, l, fj, fl, c4, PGK_HF_TO);
+        calvirihf_dev<AH>(h, k, l, fk, fl, c4, PGK_HF_TO);
+      }
     }
     const double *P[4] = {a.pos + 3 * i, a.pos + 3 * j, a.pos + 3 * k, a.pos + 3 * l};
     const double *F[4] = {fi, fj, fk, fl};
@@ -258,11 +311,13 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
         ev[2 + aa + 3 * nn] = s;
       }
   }
+  if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_TO);
+  if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_TO);
   block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 // 1-4 pairs; rec: E_elc14, E_vdw14, V, T(9)
-template <class A, class AH, bool HF>
+template <class A, class AH, int HFM>
 __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restrict__ pl,
                                                const double *__restrict__ pf,
                                                const double *__restrict__ q,
@@ -273,9 +328,19 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
   double ev[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) ev[k] = 0.0;
+  extern __shared__ __align__(8) unsigned char bl_lds[];
+  AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
+  AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
+  if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = pl[2 * t], j = pl[2 * t + 1];
+    HfAt at[2];
+    constexpr bool fx = HFM >= 2, fb = HFM == 2;   // flux form (bulk): atoms loaded up front
+    if constexpr (fx) {
+      hf_at_load<fb>(h, i, at[0]);
+      hf_at_load<fb>(h, j, at[1]);
+    }
     double dv = pf[2 * t], de = pf[2 * t + 1];
     int tp = (type[i] - 1) + ntype * (type[j] - 1);
     double r[3];
@@ -295,16 +360,21 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
     double f[3] = {fe[0] + ft * r[0], fe[1] + ft * r[1], fe[2] + ft * r[2]};
     addf(fat, i, f, 1.0);
     addf(fat, j, f, -1.0);
-    if (HF) {   // calnon14p_hf (calnon14_hf.F90:8-386): Coulomb and LJ parts apart
+    if (HFM) {   // calnon14p_hf (calnon14_hf.F90:8-386): Coulomb and LJ parts apart
       const double fej[3] = {-fe[0], -fe[1], -fe[2]};
       const double fv[3] = {ft * r[0], ft * r[1], ft * r[2]};
       const double fvj[3] = {-fv[0], -fv[1], -fv[2]};
       hf_pe_add<AH>(h, PGK_HF_14, i, 0.5 * pel);
       hf_pe_add<AH>(h, PGK_HF_14, j, 0.5 * pel);
-      calvirihf_dev<AH>(h, i, j, fe, fej, 0.5, PGK_HF_14);
       hf_pe_add<AH>(h, PGK_HF_14, i, 0.5 * pv);
       hf_pe_add<AH>(h, PGK_HF_14, j, 0.5 * pv);
-      calvirihf_dev<AH>(h, i, j, fv, fvj, 0.5, PGK_HF_14);
+      if constexpr (fx) {
+        calvirihf_flux_at<AH, fb>(h, at[0], at[1], fe, fej, 0.5, jb, sj);
+        calvirihf_flux_at<AH, fb>(h, at[0], at[1], fv, fvj, 0.5, jb, sj);
+      } else {
+        calvirihf_dev<AH>(h, i, j, fe, fej, 0.5, PGK_HF_14);
+        calvirihf_dev<AH>(h, i, j, fv, fvj, 0.5, PGK_HF_14);
+      }
     }
     ev[0] = pel;
     ev[1] = pv;
@@ -312,6 +382,8 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
     for (int nn = 0; nn < 3; nn++)
       for (int aa = 0; aa < 3; aa++) ev[3 + aa + 3 * nn] = f[aa] * r[nn];
   }
+  if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_14);
+  if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_14);
   block_record<12, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
@@ -320,7 +392,10 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
 // Launch every bonded term on the MTS level mts_flag.  Records are appended
 // per term; the segment table (kind per segment) is written to segs.  With
 // hf the calforce_hf variants also scatter the per-atom heat-flux terms.
-template <int PREC, bool HF>
+// HFM: 0 forces only, 1 the per-atom heat-flux arrays (calforce_hf), 2 and 3
+// the flux form (the block's flux sums) in the bulk and the volume mode; each
+// form is its own kernel, so a flux form carries none of the others' code
+template <int PREC, int HFM>
 static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
                          int *seg_kind, int *nseg) {
   using A = typename Prec<PREC>::acc;
@@ -333,6 +408,8 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
   a.L = make_double3(c->box[0], c->box[1], c->box[2]);
   a.Li = make_double3(1.0 / c->box[0], 1.0 / c->box[1], 1.0 / c->box[2]);
   hipStream_t st = c->stream;
+  // the flux form's per-block sums (at most 1024 regions: 8 KiB)
+  const size_t lds = (HFM >= 2) ? sizeof(AH) * (size_t)hf_flux_block_len(h) : 0;
   auto seg = [&](int nblk, int kind) {
     seg_off[*nseg] = c->red_used;
     seg_n[*nseg] = nblk;
@@ -346,21 +423,21 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     a.n = nb;
     int g = pgk_grid_blocks(nb, 256);
     seg(g, 1);
-    k_bonds<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat, h);
+    k_bonds<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat, h);
   }
   int na = c->nangl_act;
   if (m->angl == mts_flag && na > 0) {
     a.n = na;
     int g = pgk_grid_blocks(na, 256);
     seg(g, 2);
-    k_angles<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_angl, c->off_ak, c->off_at, fat, h);
+    k_angles<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_angl, c->off_ak, c->off_at, fat, h);
   }
   int nt = (int)(c->tors_type.size());
   if (m->tors == mts_flag && nt > 0) {
     a.n = nt;
     int g = pgk_grid_blocks(nt, 256);
     seg(g, 3);
-    k_tors<A, AH, HF, false><<<g, 256, 0, st>>>(a, c->d_tors, c->off_tdiv, c->off_tb,
+    k_tors<A, AH, HFM, false><<<g, 256, lds, st>>>(a, c->d_tors, c->off_tdiv, c->off_tb,
                                                 c->off_tp, c->off_tper, fat, h);
   }
   int nr = (int)(c->rb_type.size());
@@ -368,13 +445,13 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     a.n = nr;
     int g = pgk_grid_blocks(nr, 256);
     seg(g, 4);
-    k_tors<A, AH, HF, true><<<g, 256, 0, st>>>(a, c->d_rb, c->off_rbc, 0, 0, 0, fat, h);
+    k_tors<A, AH, HFM, true><<<g, 256, lds, st>>>(a, c->d_rb, c->off_rbc, 0, 0, 0, fat, h);
   }
   if (m->nonb14 == mts_flag && c->n14 > 0) {
     a.n = c->n14;
     int g = pgk_grid_blocks(c->n14, 256);
     seg(g, 5);
-    k_non14<A, AH, HF><<<g, 256, 0, st>>>(a, c->d_p14, c->d_p14f, c->d_q, c->d_type, c->d_eps,
+    k_non14<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_p14, c->d_p14f, c->d_q, c->d_type, c->d_eps,
                                            c->d_rmin, c->ntype, fat, h);
   }
   HIP_CHECK(hipGetLastError());
@@ -383,9 +460,12 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
 
 int pgk_bonded_launch2(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_off, int *seg_n,
                        int *seg_kind, int *nseg, bool hf) {
-#define PGK_BL(P)                                                                      \
-  return hf ? launch_bonded<P, true>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)   \
-            : launch_bonded<P, false>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)
+  const int hfm = !hf ? 0 : !c->hfdev.flux ? 1 : c->hfdev.mode == PGK_HF_BULK ? 2 : 3;
+#define PGK_BL(P)                                                                          \
+  return hfm == 3   ? launch_bonded<P, 3>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)   \
+         : hfm == 2 ? launch_bonded<P, 2>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)   \
+         : hfm == 1 ? launch_bonded<P, 1>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)   \
+                    : launch_bonded<P, 0>(c, m, mts_flag, seg_off, seg_n, seg_kind, nseg)
   switch (c->precision) {
     case PGK_PREC_FP64: PGK_BL(PGK_PREC_FP64);
     case PGK_PREC_MIXED: PGK_BL(PGK_PREC_MIXED);

@@ -209,11 +209,13 @@ __global__ __launch_bounds__(256) void k_heatf_final(HeatfArgs a, const double *
   }
 }
 
-// flux form: collision row (class, region) = the bonded terms' atomics in
-// jcol plus, for the pair classes, the pair pass's per-block records in
-// block order (block (row, component) per launch block, fixed-order tree)
+// flux form: collision row (class, region) = the bonded terms' sums in jcol
+// (heat-flux accumulator type AH) plus, for the pair classes, the pair pass's
+// per-block records in block order (block (row, component) per launch block,
+// fixed-order tree)
+template <class AH>
 __global__ __launch_bounds__(256) void k_heatf_collect(int nreg, int bulk, int nblk,
-                                                       const double *__restrict__ jcol,
+                                                       const AH *__restrict__ jcol,
                                                        const double *__restrict__ jpart,
                                                        double *__restrict__ pcol) {
   const int row = blockIdx.x / 3, c = blockIdx.x % 3;   // row = class * nreg + region
@@ -228,7 +230,20 @@ __global__ __launch_bounds__(256) void k_heatf_collect(int nreg, int bulk, int n
   __shared__ double rec[PGK_REC];
   block_record<1, 4>(v, rec);
   __syncthreads();
-  if (threadIdx.x == 0) pcol[(size_t)row * 3 + c] = jcol[(size_t)row * 3 + c] + rec[0];
+  if (threadIdx.x == 0) pcol[(size_t)row * 3 + c] = acc_load(&jcol[(size_t)row * 3 + c]) + rec[0];
+}
+
+// clears up to three 8-byte arrays in one launch (the heat-flux sums a pass
+// adds to), in place of one fill per array
+__global__ __launch_bounds__(256) void k_zero3(uint64_t *__restrict__ p0, size_t n0,
+                                               uint64_t *__restrict__ p1, size_t n1,
+                                               uint64_t *__restrict__ p2, size_t n2) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n0 + n1 + n2; k += stride) {
+    if (k < n0) p0[k] = 0;
+    else if (k < n0 + n1) p1[k - n0] = 0;
+    else p2[k - n0 - n1] = 0;
+  }
 }
 
 // velocities relative to the streaming velocities
@@ -246,9 +261,9 @@ int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   k_heatf_transport<AH><<<dim3(a.nsplit, a.nreg), 256, 0, st>>>(a, ptr);
   HIP_CHECK(hipGetLastError());
   if (c->hfdev.flux) {
-    k_heatf_collect<<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
-        a.nreg, a.bulk, pgk_grid_blocks(pgk_pair_waves(c), 4), c->hfdev.jcol, c->hfdev.jpart,
-        pcol);
+    k_heatf_collect<AH><<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
+        a.nreg, a.bulk, pgk_grid_blocks(pgk_pair_waves(c), 4),
+        reinterpret_cast<const AH *>(c->hfdev.jcol), c->hfdev.jpart, pcol);
   } else {
     k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
   }
@@ -257,6 +272,16 @@ int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
 }
 
 }  // namespace
+
+int pgk_zero3_launch(hipStream_t st, void *p0, size_t n0, void *p1, size_t n1, void *p2,
+                     size_t n2) {
+  const size_t n = n0 + n1 + n2;
+  if (n == 0) return PGK_OK;
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 2048);
+  k_zero3<<<nb, 256, 0, st>>>((uint64_t *)p0, n0, (uint64_t *)p1, n1, (uint64_t *)p2, n2);
+  HIP_CHECK(hipGetLastError());
+  return PGK_OK;
+}
 
 // The velocities of the collision part: d_vel, or (with streaming
 // velocities) d_vel - d_vstrm queued into the context's buffer.

@@ -92,6 +92,31 @@ __device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp,
   }
 }
 
+// Cross-lane exchange inside aligned groups of 8 lanes (DPP, no LDS):
+// quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E, row_half_mirror
+// (lane k <-> 7 - k) = 0x141.
+template <int CTRL>
+__device__ __forceinline__ double dpp8d(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// e[0..3] summed over each aligned group of 8 lanes (k = lane & 7) in four
+// exchanges: afterwards lanes k and k^1 hold the sum of e[k >> 1].  Step 1
+// pairs k with 7 - k (the low half keeps e0 and e1, the high half e2 and
+// e3), step 2 pairs k with k^2 (k & 2 keeps the second of its two), step 3
+// k with k^1.  The group's whole wave must be active.
+__device__ __forceinline__ double sum4x8(const double (&e)[4], int k) {
+  const bool hi = (k & 4) != 0;
+  const double a0 = (hi ? e[2] : e[0]) + dpp8d<0x141>(hi ? e[0] : e[2]);
+  const double a1 = (hi ? e[3] : e[1]) + dpp8d<0x141>(hi ? e[1] : e[3]);
+  const bool b1 = (k & 2) != 0;
+  const double b = (b1 ? a1 : a0) + dpp8d<0x4E>(b1 ? a0 : a1);
+  return b + dpp8d<0xB1>(b);
+}
+
 // 1/sqrt(r^2) in the compute type: FP64 divide of the root, or FP32's
 // hardware reciprocal root (about one ulp)
 template <class R> __device__ __forceinline__ R hf_rinv(double r2);
@@ -213,17 +238,11 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           if (!BULK) flux_regions(h, fp, same ? 0 : 2, svc, same ? 1 : 3, svl, sl);
         }
       }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        ej[q] += __shfl_xor(ej[q], 1, 64);
-        ej[q] += __shfl_xor(ej[q], 2, 64);
-        ej[q] += __shfl_xor(ej[q], 4, 64);
-      }
-      if (ii == 0 && aj >= 0) {
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (ej[q] != 0.0) hf_pe_add<AH>(h, kPairCls0 + q, aj, ej[q]);
-      }
+      // the j atom's four class energies summed over its 8 lanes; lanes
+      // ii and ii^1 then hold class (ii >> 1) and one of them adds it
+      const double sj4 = sum4x8(ej, ii);
+      if (aj >= 0 && (ii & 1) == 0 && sj4 != 0.0)
+        hf_pe_add<AH>(h, kPairCls0 + (ii >> 1), aj, sj4);
     }
     if (ai >= 0) {
 #pragma unroll

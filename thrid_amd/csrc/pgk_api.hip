@@ -1173,8 +1173,10 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   if (need_com && !dev_com)
     HIP_CHECK(hipMemcpyAsync(c->d_hf_com, hf->molecom, sizeof(double) * 3 * hf->nmol,
                              hipMemcpyHostToDevice, st));
-  HIP_CHECK(hipMemsetAsync(c->d_hf_pe, 0, pe_b, st));
-  if (!flux) HIP_CHECK(hipMemsetAsync(c->d_hf_w, 0, w_b, st));
+  if (!flux) {
+    HIP_CHECK(hipMemsetAsync(c->d_hf_pe, 0, pe_b, st));
+    HIP_CHECK(hipMemsetAsync(c->d_hf_w, 0, w_b, st));
+  }
   HfDev &h = c->hfdev;
   h.mode = hf->mode;
   h.nreg = nreg;
@@ -1214,8 +1216,10 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
     if ((rc = grow_dev((void **)&c->d_hj_jcol, &c->hj_jcol_cap, jcol_b)) ||
         (rc = grow_dev((void **)&c->d_hj_jpart, &c->hj_jpart_cap, jpart_b)))
       return rc;
-    HIP_CHECK(hipMemsetAsync(c->d_hj_jcol, 0, jcol_b, st));
-    HIP_CHECK(hipMemsetAsync(c->d_hj_jpart, 0, jpart_b, st));
+    // the per-atom energies and both flux sums start at zero: one launch
+    if ((rc = pgk_zero3_launch(st, c->d_hf_pe, pe_b / 8, c->d_hj_jcol, jcol_b / 8, c->d_hj_jpart,
+                               jpart_b / 8)))
+      return rc;
     h.jcol = c->d_hj_jcol;
     h.jpart = c->d_hj_jpart;
   }

@@ -89,25 +89,35 @@ __device__ __forceinline__ void hf_region_window(const HfDev &h, double lo, doub
   }
 }
 
-// flux form of calvirihf: the interaction's share of sum_a W(a,n,.,r) v(a),
-// i.e. the region weight times s = fi . v_i - fj . v_j, added to
-// jcol[cls][r][n] (global FP64 atomics; the bonded terms use this, the pair
-// pass keeps its own per-block sums)
-__device__ void calvirihf_flux(const HfDev &h, int i, int j, const double *fi, const double *fj,
-                               double nbc, int cls, const double *ci, const double *cj) {
-  const bool bulk = (h.mode == PGK_HF_BULK);
-  const double *vi = h.vel + 3 * i, *vj = h.vel + 3 * j;
-  const double s = (fi[0] * vi[0] + fi[1] * vi[1] + fi[2] * vi[2]) -
-                   (fj[0] * vj[0] + fj[1] * vj[1] + fj[2] * vj[2]);
-  double *J = h.jcol + (size_t)cls * h.nreg * 3;
-  if (bulk) {
+// Flux form of calvirihf: the interaction's share of sum_a W(a,n,.,r) v(a),
+// i.e. the region weight times s = fi . v_i - fj . v_j, in the heat-flux
+// accumulator type.  Bulk mode (the same 3 components for every region):
+// each thread sums its terms in registers (jb), the block adds its waves'
+// sums once (hf_flux_block_end_bulk).  An add of every lane to one address
+// would be serialised lane by lane.  Volume mode: the block's z components
+// per region in LDS (sj), added to with LDS atomics (the lanes' regions
+// differ) and flushed once per block (hf_flux_block_end_vol).  The pair pass
+// keeps its own per-block sums.
+
+// one term in the accumulator type: the fixed-point value of the mixed form
+// is the one acc_add would add, so the sums are bit for bit the same
+template <class AH> __device__ __forceinline__ AH acc_val(double v);
+template <> __device__ __forceinline__ double acc_val<double>(double v) { return v; }
+template <> __device__ __forceinline__ unsigned long long acc_val<unsigned long long>(double v) {
+  return (unsigned long long)__double2ll_rn(v * PGK_FX_SCALE);
+}
+
+template <class AH, bool BULK>
+__device__ __forceinline__ void calvirihf_flux_s(const HfDev &h, double s, const double *ci,
+                                                 const double *cj, double nbc, AH *jb,
+                                                 AH *sj) {
+  if (BULK) {
     double r[3];
     for (int d = 0; d < 3; d++) {
       r[d] = ci[d] - cj[d];
       r[d] = r[d] - h.L[d] * round(r[d] * h.Li[d]);
     }
-    for (int ir = 0; ir < h.nreg; ir++)
-      for (int n = 0; n < 3; n++) atomicAdd(&J[3 * ir + n], r[n] * nbc * s);
+    for (int n = 0; n < 3; n++) jb[n] += acc_val<AH>(r[n] * nbc * s);
     return;
   }
   const double r2 = ci[2] - cj[2], rz = fabs(r2);
@@ -116,10 +126,90 @@ __device__ void calvirihf_flux(const HfDev &h, int i, int j, const double *fi, c
   hf_region_window(h, fmin(ci[2], cj[2]), fmax(ci[2], cj[2]), r0, r1);
   for (int ir = r0; ir <= r1; ir++) {
     const double ff = hf_volume_weight(h, ir, ci[2], cj[2], r2, rz, nbc);
-    if (ff != 0.0) atomicAdd(&J[3 * ir + 2], ff * s);
+    if (ff != 0.0) acc_add(&sj[ir], ff * s);
   }
 }
 
+// What the flux form needs of one atom of a bonded term: its centre,
+// velocity, molecule and centre kind.  A bonded kernel loads all of a term's
+// atoms at once, before the term's geometry, so that the loads' latency
+// overlaps the arithmetic instead of following it pair by pair.
+struct HfAt {
+  double c[3], v[3];
+  int mol, atom;   // atom: 1 for an ATOM centre, 0 for a MOLE centre
+};
+
+template <bool BULK>
+__device__ __forceinline__ void hf_at_load(const HfDev &h, int i, HfAt &a) {
+  a.mol = h.molid[i];
+  a.atom = hf_centre(h, i, BULK, a.c);
+  a.v[0] = h.vel[3 * i];
+  a.v[1] = h.vel[3 * i + 1];
+  a.v[2] = h.vel[3 * i + 2];
+}
+
+// calvirihf's flux form on loaded atoms (BULK: the bulk mode, else volume)
+template <class AH, bool BULK>
+__device__ __forceinline__ void calvirihf_flux_at(const HfDev &h, const HfAt &A, const HfAt &B,
+                                                  const double *fi, const double *fj,
+                                                  double nbc, AH *jb, AH *sj) {
+  if (A.atom + B.atom == 0 && A.mol == B.mol) return;
+  const double s = (fi[0] * A.v[0] + fi[1] * A.v[1] + fi[2] * A.v[2]) -
+                   (fj[0] * B.v[0] + fj[1] * B.v[1] + fj[2] * B.v[2]);
+  calvirihf_flux_s<AH, BULK>(h, s, A.c, B.c, nbc, jb, sj);
+}
+
+// entries of a block's flux sums in LDS (the dynamic LDS a bonded kernel
+// needs in the flux form is hf_flux_block_len * sizeof(accumulator)): bulk,
+// 3 per wave of a 256-thread block; volume, one per region
+__host__ __device__ __forceinline__ int hf_flux_block_len(const HfDev &h) {
+  return h.mode == PGK_HF_BULK ? 12 : h.nreg;
+}
+
+__device__ __forceinline__ void acc_add_raw(double *p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void acc_add_raw(unsigned long long *p, unsigned long long v) {
+  atomicAdd(p, v);
+}
+
+// volume mode: every thread of the block calls both (they synchronise it)
+template <class AH>
+__device__ __forceinline__ void hf_flux_block_begin_vol(const HfDev &h, AH *sj) {
+  for (int k = threadIdx.x; k < h.nreg; k += blockDim.x) sj[k] = (AH)0;
+  __syncthreads();
+}
+
+template <class AH>
+__device__ __forceinline__ void hf_flux_block_end_vol(const HfDev &h, const AH *sj, int cls) {
+  __syncthreads();
+  AH *J = reinterpret_cast<AH *>(h.jcol) + (size_t)cls * h.nreg * 3;
+  for (int k = threadIdx.x; k < h.nreg; k += blockDim.x) {
+    const AH v = sj[k];
+    if (v != (AH)0) acc_add_raw(&J[3 * k + 2], v);
+  }
+}
+
+// bulk mode: the threads' register sums jb over the wave (butterfly), the
+// 4 waves in order, then one add per component and region; every thread of
+// the 256-thread block calls it
+template <class AH>
+__device__ __forceinline__ void hf_flux_block_end_bulk(const HfDev &h, AH (&jb)[3], AH *sj,
+                                                       int cls) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int n = 0; n < 3; n++)
+    for (int o = 32; o >= 1; o >>= 1) jb[n] += __shfl_xor(jb[n], o, 64);
+  if (lane == 0)
+    for (int n = 0; n < 3; n++) sj[3 * wave + n] = jb[n];
+  __syncthreads();
+  AH *J = reinterpret_cast<AH *>(h.jcol) + (size_t)cls * h.nreg * 3;
+  for (int k = threadIdx.x; k < 3 * h.nreg; k += blockDim.x) {
+    const int n = k % 3;
+    const AH v = ((sj[n] + sj[3 + n]) + sj[6 + n]) + sj[9 + n];
+    if (v != (AH)0) acc_add_raw(&J[k], v);
+  }
+}
+
+// the per-atom arrays (calforce_hf); the flux form is calvirihf_flux_at
 template <class AH>
 __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, const double *fj,
                               double nbc, int cls) {
@@ -127,10 +217,6 @@ __device__ void calvirihf_dev(const HfDev &h, int i, int j, const double *fi, co
   double ci[3], cj[3];
   const int atomflag = hf_centre(h, i, bulk, ci) + hf_centre(h, j, bulk, cj);
   if (atomflag == 0 && h.molid[i] == h.molid[j]) return;
-  if (h.flux) {
-    calvirihf_flux(h, i, j, fi, fj, nbc, cls, ci, cj);
-    return;
-  }
   double r[3] = {0.0, 0.0, 0.0}, rz = 0.0;
   if (bulk) {
     r[0] = ci[0] - cj[0];

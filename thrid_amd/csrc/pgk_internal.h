@@ -85,7 +85,7 @@ struct HfDev {
   // part of calheatf, sum_a W(a,n,i,r) v_i(a), is accumulated directly
   int flux = 0;
   const double *vel = nullptr;                 // [3*natom] atom velocities
-  double *jcol = nullptr;                      // [PGK_NHF][nreg][3] FP64 sums
+  double *jcol = nullptr;   // [PGK_NHF][nreg][3] sums, heat-flux accumulator type (int64 fixed point in the mixed form)
   double *jpart = nullptr;                     // pair pass: per-block records
 };
 
@@ -369,6 +369,10 @@ int pgk_hf_export(pgk_ctx *c);
 // part relative to them), or null
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
                      const double *d_vstrm);
+// zero n0, n1 and n2 8-byte words at p0, p1 and p2 (a null pointer with a
+// zero count is skipped) in one kernel launch on st
+int pgk_zero3_launch(hipStream_t st, void *p0, size_t n0, void *p1, size_t n1, void *p2,
+                     size_t n2);
 int pgk_heatf_collision_vel(pgk_ctx *c, const double *d_vel, const double *d_vstrm,
                             const double **out);
 // molecule centres of mass from d_pos (calcom.F90:1-147) into d_com [3*nmol]
