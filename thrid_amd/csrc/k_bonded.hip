@@ -26,8 +26,10 @@ __device__ __forceinline__ void cross3(const double *x, const double *y, double 
   o[2] = x[0] * y[1] - x[1] * y[0];
 }
 
+// f == nullptr: the call wants no forces (a heat-flux sample)
 template <class A>
 __device__ __forceinline__ void addf(A *f, int i, const double *v, double s) {
+  if (!f) return;
   acc_add(&f[3 * i + 0], s * v[0]);
   acc_add(&f[3 * i + 1], s * v[1]);
   acc_add(&f[3 * i + 2], s * v[2]);
@@ -38,7 +40,7 @@ struct BArgs {
   const double *par;
   double3 L, Li;
   int n;
-  double *red;
+  double *red;   // energy and virial records; nullptr: none (a heat-flux sample)
 };
 
 // rec: E, V, T(9 in Fortran (a,n) order)
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_BO);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_BO);
-  block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 template <class A, class AH, int HFM>
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_AN);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_AN);
-  block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 // shared torsion geometry (calijkl / calijklrb)
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_TO);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_TO);
-  block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 // 1-4 pairs; rec: E_elc14, E_vdw14, V, T(9)
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_14);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_14);
-  block_record<12, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<12, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
 }
 
 }  // namespace
@@ -401,7 +403,9 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
   using A = typename Prec<PREC>::acc;
   using AH = typename HfAcc<PREC>::type;
   const HfDev &h = c->hfdev;
-  A *fat = reinterpret_cast<A *>(c->d_fat);
+  // a heat-flux sample of the MD loop wants the per-atom energies and the
+  // flux only: no forces, no records
+  A *fat = c->hf_pairs_only ? nullptr : reinterpret_cast<A *>(c->d_fat);
   BArgs a;
   a.pos = c->d_pos;
   a.par = c->d_bpar;
@@ -417,6 +421,7 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     (*nseg)++;
     a.red = c->d_red + (size_t)c->red_used * PGK_REC;
     c->red_used += nblk;
+    if (c->hf_pairs_only) a.red = nullptr;   // a heat-flux sample reads no records
   };
   int nb = c->nbond_act;
   if (m->bond == mts_flag && nb > 0) {

@@ -76,16 +76,19 @@ __device__ __forceinline__ void flux_add(const FluxPair &fp, double e, double f,
 }
 
 // volume form: the pair's Coulomb (class qc) and LJ (class ql) shares times
-// the weight of every region its centres' segment crosses
+// the weight of every region its centres' segment crosses (zr: the regions'
+// bounds, z1 then z2, staged in LDS)
 __device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp, int qc,
-                                             double svc, int ql, double svl, double *sl) {
+                                             double svc, int ql, double svl, double *sl,
+                                             const double *zr) {
   if (!fp.seg || (svc == 0.0 && svl == 0.0)) return;
   const double lo = fmin(fp.ciz, fp.cjz), hi = fmax(fp.ciz, fp.cjz);
   int r0, r1;
   hf_region_window(h, lo, hi, r0, r1);
   for (int ir = r0; ir <= r1; ir++) {
-    if (hi < h.z1[ir] || lo > h.z2[ir]) continue;   // the segment misses the region
-    const double ff = hf_volume_weight(h, ir, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
+    const double z1 = zr[ir], z2 = zr[h.nreg + ir];
+    if (hi < z1 || lo > z2) continue;   // the segment misses the region
+    const double ff = hf_volume_weight_z(z1, z2, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
     if (ff == 0.0) continue;
     if (svc != 0.0) atomicAdd(&sl[qc * h.nreg + ir], ff * svc);
     if (svl != 0.0) atomicAdd(&sl[ql * h.nreg + ir], ff * svl);
@@ -148,8 +151,13 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
   const int nreg = h.nreg;
   double acc[4][3] = {};
   double *sl = lds + (size_t)wave * 4 * nreg;   // volume: this wave's slice
+  double *zr = lds + (size_t)16 * nreg;           // volume: the regions' bounds
   if (!BULK) {
     for (int k = lane; k < 4 * nreg; k += 64) sl[k] = 0.0;
+    for (int k = threadIdx.x; k < nreg; k += 256) {
+      zr[k] = h.z1[k];
+      zr[nreg + k] = h.z2[k];
+    }
     __syncthreads();
   }
   if (ci < p.ncl) {
@@ -198,16 +206,23 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           const R rinv2 = rinv1 * rinv1;
           const bool same = moli == p.molid[aj];
           // the centres' separation and, in the volume form, whether it counts
-          double caj[3];
-          const int fl = fla + hf_centre(h, aj, BULK, caj);
-          bool seg = !(fl == 0 && same);
-          double rc[3], rz = 0.0;
-          for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
-          if (BULK) {
-            for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
+          double caj[3] = {0.0, 0.0, 0.0}, rc[3], rz = 0.0;
+          bool seg = true;
+          if (BULK && fla == 1 && h.hftyp[aj] != PGK_HFTYP_MOLE) {
+            // two atom centres in the bulk form: the centres' minimum image
+            // is the pair's own r (round and rint differ only at a
+            // half-box tie, which no listed pair reaches)
+            for (int d = 0; d < 3; d++) rc[d] = r[d];
           } else {
-            rz = fabs(rc[2]);
-            seg = seg && rz * h.Li[2] < 0.5;
+            const int fl = fla + hf_centre(h, aj, BULK, caj);
+            seg = !(fl == 0 && same);
+            for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
+            if (BULK) {
+              for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
+            } else {
+              rz = fabs(rc[2]);
+              seg = seg && rz * h.Li[2] < 0.5;
+            }
           }
           // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
           const double *vjp = h.vel + 3 * aj;
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
               else flux_add<3, BULK>(fp, pot, ft, ei, ej, acc, svl);
             }
           }
-          if (!BULK) flux_regions(h, fp, same ? 0 : 2, svc, same ? 1 : 3, svl, sl);
+          if (!BULK) flux_regions(h, fp, same ? 0 : 2, svc, same ? 1 : 3, svl, sl, zr);
         }
       }
       // the j atom's four class energies summed over its 8 lanes; lanes
@@ -335,7 +350,7 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
       else
         k_hf_pairs_flux<AH, true, false><<<nblk, 256, 0, st>>>(p, h);
     } else {
-      const size_t lds = sizeof(double) * 16 * (size_t)h.nreg;
+      const size_t lds = sizeof(double) * 18 * (size_t)h.nreg;   // 4 wave slices + bounds
       auto kf = ch ? k_hf_pairs_flux<AH, false, true> : k_hf_pairs_flux<AH, false, false>;
       const int lr = pgk_lds_limit((const void *)kf, lds);
       if (lr) return lr;

@@ -370,9 +370,11 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     if (seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
   }
   if (hf_pairs && (rc = pgk_hf_launch_pairs(c, true))) return rc;
-  // forces: accumulators + excluded-pair correction -> FP64 output
+  // forces: accumulators + excluded-pair correction -> FP64 output (a
+  // heat-flux sample of the MD loop adds no forces and has none to sum)
   double *fout = d_force_out ? d_force_out : c->d_force;
-  if ((rc = pgk_combine_launch(c, do_pair, split, fout, split ? d_fvc : nullptr,
+  if (!c->hf_pairs_only &&
+      (rc = pgk_combine_launch(c, do_pair, split, fout, split ? d_fvc : nullptr,
                                split ? d_fvl : nullptr)))
     return rc;
   // energies: fixed-order segment sums into d_sum, copied to pinned h_sum

@@ -51,10 +51,9 @@ __device__ __forceinline__ int hf_centre(const HfDev &h, int i, bool bulk, doubl
 // ciz, cjz (r2 = ciz - cjz, rz = |r2|) in the volume form: the signed length
 // of the part of the segment inside [z1, z2], times nbc (calvirihf.F90:201-357).
 // Returns 0 when the segment misses the region.
-__device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, double ciz,
-                                                   double cjz, double r2, double rz,
-                                                   double nbc) {
-  const double z1 = h.z1[ir], z2 = h.z2[ir];
+__device__ __forceinline__ double hf_volume_weight_z(double z1, double z2, double ciz,
+                                                     double cjz, double r2, double rz,
+                                                     double nbc) {
   const double ri1 = ciz - z1, rj1 = cjz - z1, ri2 = ciz - z2, rj2 = cjz - z2;
   int np = 0;
   bool p1 = false;
@@ -74,6 +73,12 @@ __device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, doubl
     ff = z2 - z1;
   }
   return (r2 >= 0.0) ? ff * nbc : -ff * nbc;
+}
+
+__device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, double ciz,
+                                                   double cjz, double r2, double rz,
+                                                   double nbc) {
+  return hf_volume_weight_z(h.z1[ir], h.z2[ir], ciz, cjz, r2, rz, nbc);
 }
 
 // the regions a z segment [lo, hi] can touch: with equal bins the bins of its
