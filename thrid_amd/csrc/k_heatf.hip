@@ -27,7 +27,9 @@ constexpr int kNterm = 1 + PGK_NHF;   // kinetic + potential classes
 
 struct HeatfArgs {
   int natom, nreg, nw, bulk, nsplit, chunk;
-  int nsplit_col;   // partial records per collision row
+  int nsplit_col;             // partial records per collision row
+  int nsplit_tr, chunk_tr;    // transport: atom chunks per region and their size
+  const double *cz;           // the atoms' centre z (wrapped), or null: computed in place
   const double *z1, *z2;
   const int32_t *hftyp, *molid;
   const double *molecom, *pos, *vel, *mass, *molevel;
@@ -105,8 +107,8 @@ __global__ __launch_bounds__(256) void k_calcom(int nmol, const int32_t *__restr
 template <class AH>
 __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__restrict__ part) {
   const int r = blockIdx.y, split = blockIdx.x;
-  const int i0 = split * a.chunk;
-  const int i1 = min(a.natom, i0 + a.chunk);
+  const int i0 = split * a.chunk_tr;
+  const int i1 = min(a.natom, i0 + a.chunk_tr);
   const double z1 = a.z1[r], z2 = a.z2[r];
   const AH *pe = reinterpret_cast<const AH *>(a.pe);
   double acc[kNterm * 3];
@@ -114,18 +116,16 @@ __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__
   for (int t = 0; t < kNterm * 3; t++) acc[t] = 0.0;
   for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
     double cz;
-    const double *pv;
-    if (a.hftyp[i] == PGK_HFTYP_MOLE) {
-      const int m = a.molid[i] - 1;
-      cz = a.molecom[3 * m + 2];
-      pv = a.molevel + 3 * m;
+    if (a.cz) {
+      cz = a.cz[i];
     } else {
-      cz = a.pos[3 * i + 2];
-      pv = a.vel + 3 * i;
+      cz = a.hftyp[i] == PGK_HFTYP_MOLE ? a.molecom[3 * (a.molid[i] - 1) + 2] : a.pos[3 * i + 2];
+      if (cz < 0.0) cz = cz + a.L[2];
+      else if (cz >= a.L[2]) cz = cz - a.L[2];
     }
-    if (cz < 0.0) cz = cz + a.L[2];
-    else if (cz >= a.L[2]) cz = cz - a.L[2];
     if (!(z1 <= cz && cz <= z2)) continue;
+    const double *pv = a.hftyp[i] == PGK_HFTYP_MOLE ? a.molevel + 3 * (a.molid[i] - 1)
+                                                    : a.vel + 3 * i;
     // kinetic energy relative to the streaming velocity (calheatf.F90:368-369,
     // 677-681); the transported centre velocity stays lab-frame
     double v[3] = {a.vel[3 * i], a.vel[3 * i + 1], a.vel[3 * i + 2]};
@@ -142,7 +142,19 @@ __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__
       for (int c = 0; c < 3; c++) acc[3 * (1 + k) + c] += e * vc[c];
     }
   }
-  block_record<kNterm * 3, 4>(acc, part + ((size_t)r * a.nsplit + split) * (kNterm * 3));
+  block_record<kNterm * 3, 4>(acc, part + ((size_t)r * a.nsplit_tr + split) * (kNterm * 3));
+}
+
+// the atoms' centre z, wrapped into [0, Lz) as the transport part tests it
+// (ATOM: the atom, MOLE: its molecule's centre of mass), once per pass
+// instead of once per region
+__global__ __launch_bounds__(256) void k_centre_z(HeatfArgs a, double *__restrict__ cz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.natom) return;
+  double z = a.hftyp[i] == PGK_HFTYP_MOLE ? a.molecom[3 * (a.molid[i] - 1) + 2] : a.pos[3 * i + 2];
+  if (z < 0.0) z = z + a.L[2];
+  else if (z >= a.L[2]) z = z - a.L[2];
+  cz[i] = z;
 }
 
 // collision part: block (chunk, class-region row); record [3]
@@ -184,8 +196,8 @@ __global__ __launch_bounds__(256) void k_heatf_final(HeatfArgs a, const double *
     const int t = lane / 3, c = lane % 3;
     double s = 0.0;
     if (t < kNterm) {
-      for (int k = wave; k < a.nsplit; k += 4)
-        s += ptr[((size_t)r * a.nsplit + k) * (kNterm * 3) + 3 * t + c];
+      for (int k = wave; k < a.nsplit_tr; k += 4)
+        s += ptr[((size_t)r * a.nsplit_tr + k) * (kNterm * 3) + 3 * t + c];
     } else {
       const size_t row = (size_t)(t - kNterm) * a.nreg + r;
       for (int k = wave; k < a.nsplit_col; k += 4) s += pcol[(row * a.nsplit_col + k) * 3 + c];
@@ -258,7 +270,7 @@ template <int PREC>
 int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   using AH = typename HfAcc<PREC>::type;
   hipStream_t st = c->stream;
-  k_heatf_transport<AH><<<dim3(a.nsplit, a.nreg), 256, 0, st>>>(a, ptr);
+  k_heatf_transport<AH><<<dim3(a.nsplit_tr, a.nreg), 256, 0, st>>>(a, ptr);
   HIP_CHECK(hipGetLastError());
   if (c->hfdev.flux) {
     k_heatf_collect<AH><<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
@@ -319,6 +331,12 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
   a.nsplit = nsplit;
   a.chunk = (n + nsplit - 1) / nsplit;
   a.nsplit_col = h.flux ? 1 : nsplit;
+  // transport: every region's blocks scan the atoms, about 2048 blocks in all
+  int nsplit_tr = (2048 + h.nreg - 1) / h.nreg;
+  nsplit_tr = std::max(1, std::min(nsplit_tr, (n + 255) / 256));
+  a.nsplit_tr = nsplit_tr;
+  a.chunk_tr = (n + nsplit_tr - 1) / nsplit_tr;
+  a.cz = nullptr;
   a.z1 = h.z1;
   a.z2 = h.z2;
   a.hftyp = h.hftyp;
@@ -333,7 +351,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
   a.pe = h.pe;
   a.w = h.w;
   for (int d = 0; d < 3; d++) a.L[d] = h.L[d];
-  const size_t ntr = (size_t)h.nreg * nsplit * kNterm * 3;
+  const size_t ntr = (size_t)h.nreg * nsplit_tr * kNterm * 3;
   const size_t ncol = (size_t)rows * a.nsplit_col * 3;
   int rc;
   if (!h.flux && (rc = pgk_heatf_collision_vel(c, d_vel, d_vstrm, &a.cvel))) return rc;
@@ -345,6 +363,12 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
     k_molevel<<<pgk_grid_blocks(c->hj_nmol, 256), 256, 0, c->stream>>>(
         c->hj_nmol, c->d_hj_mol_start, c->d_hj_mol_atoms, d_mass, d_vel, c->d_hj_molevel);
     HIP_CHECK(hipGetLastError());
+  }
+  if (h.nreg > 1) {   // many regions: the centres once, not once per region
+    if ((rc = grow_dev((void **)&c->d_hj_cz, &c->hj_cz_cap, sizeof(double) * n))) return rc;
+    k_centre_z<<<pgk_grid_blocks(n, 256), 256, 0, c->stream>>>(a, c->d_hj_cz);
+    HIP_CHECK(hipGetLastError());
+    a.cz = c->d_hj_cz;
   }
   double *ptr = c->d_hj_part, *pcol = c->d_hj_part + ntr;
   switch (c->precision) {

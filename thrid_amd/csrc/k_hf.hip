@@ -83,6 +83,40 @@ __device__ __forceinline__ void flux_regions(const HfDev &h, const FluxPair &fp,
                                              const double *zr) {
   if (!fp.seg || (svc == 0.0 && svl == 0.0)) return;
   const double lo = fmin(fp.ciz, fp.cjz), hi = fmax(fp.ciz, fp.cjz);
+  if (h.tiled) {
+    // ka, kb: the last region with z1 <= lo, hi (-1: none).  Regions below
+    // ka - 1 end below lo, regions above kb start above hi: the weight
+    // misses them.  ka - 1 ends at z1[ka]: it is weighed only when that is
+    // lo itself.  Regions strictly between ka and kb have lo < z1 < z2 <= hi,
+    // so both planes are crossed (or the upper plane holds the upper centre)
+    // and the weight is their signed width, as hf_volume_weight_z gives it.
+    const int nreg = h.nreg;
+    int ka = min(nreg - 1, max(-1, (int)floor((lo - h.ubz0) * h.ubwi)));
+    while (ka + 1 < nreg && zr[ka + 1] <= lo) ka++;
+    while (ka >= 0 && zr[ka] > lo) ka--;
+    int kb = min(nreg - 1, max(ka, (int)floor((hi - h.ubz0) * h.ubwi)));
+    while (kb + 1 < nreg && zr[kb + 1] <= hi) kb++;
+    while (kb > ka && zr[kb] > hi) kb--;
+    auto exact = [&](int ir) {
+      const double z1 = zr[ir], z2 = zr[nreg + ir];
+      if (hi < z1 || lo > z2) return;
+      const double ff = hf_volume_weight_z(z1, z2, fp.ciz, fp.cjz, fp.rc[2], fp.rz, 1.0);
+      if (ff == 0.0) return;
+      if (svc != 0.0) atomicAdd(&sl[qc * nreg + ir], ff * svc);
+      if (svl != 0.0) atomicAdd(&sl[ql * nreg + ir], ff * svl);
+    };
+    if (ka >= 1 && zr[ka] == lo) exact(ka - 1);
+    if (ka >= 0) exact(ka);
+    const bool neg = !(fp.rc[2] >= 0.0);
+    for (int ir = max(ka + 1, 0); ir < kb; ir++) {
+      const double wd = zr[nreg + ir] - zr[ir];
+      const double ff = neg ? -wd : wd;
+      if (svc != 0.0) atomicAdd(&sl[qc * nreg + ir], ff * svc);
+      if (svl != 0.0) atomicAdd(&sl[ql * nreg + ir], ff * svl);
+    }
+    if (kb > ka) exact(kb);
+    return;
+  }
   int r0, r1;
   hf_region_window(h, lo, hi, r0, r1);
   for (int ir = r0; ir <= r1; ir++) {

@@ -914,6 +914,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_hf_vstrm, c->d_hj_vpec, c->d_vgrid,
                   c->d_vir6, c->d_hfw_xq, c->d_hfw_cen, c->d_hfw_mol, c->d_tl_start,
                   c->d_tl_src, c->d_tl_mask, c->d_hj_vel, c->d_hj_mass, c->d_hj_molevel,
+                  c->d_hj_cz,
                   c->d_hj_part, c->d_hj_out, c->d_hj_mol_start, c->d_hj_mol_atoms,
                   c->d_hj_jcol, c->d_hj_jpart, c->d_md_hfseries, c->d_md_hfmass,
                   c->d_hj_fscratch, c->d_md_fshort};
@@ -1195,6 +1196,11 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
     h.ubin = ub ? 1 : 0;
     h.ubz0 = hf->z1[0];
     h.ubw = ub ? w : 1.0;
+    h.ubwi = 1.0 / h.ubw;
+    bool tiled = ub;
+    for (int r = 0; tiled && r < nreg; r++)
+      tiled = hf->z1[r] < hf->z2[r] && (r + 1 == nreg || hf->z2[r] == hf->z1[r + 1]);
+    h.tiled = tiled ? 1 : 0;
   }
   h.hftyp = c->d_hf_typ;
   h.molid = c->d_molid;

@@ -82,12 +82,13 @@ __device__ __forceinline__ double hf_volume_weight(const HfDev &h, int ir, doubl
 }
 
 // the regions a z segment [lo, hi] can touch: with equal bins the bins of its
-// ends (widened by one for rounding; the exact weight decides), else all
+// ends (widened by one for the rounding of the product by 1/width and the
+// bins' deviation from equal; the exact weight decides), else all
 __device__ __forceinline__ void hf_region_window(const HfDev &h, double lo, double hi, int &r0,
                                                  int &r1) {
   if (h.ubin) {
-    r0 = max(0, (int)floor((lo - h.ubz0) / h.ubw) - 1);
-    r1 = min(h.nreg - 1, (int)floor((hi - h.ubz0) / h.ubw) + 1);
+    r0 = max(0, (int)floor((lo - h.ubz0) * h.ubwi) - 1);
+    r1 = min(h.nreg - 1, (int)floor((hi - h.ubz0) * h.ubwi) + 1);
   } else {
     r0 = 0;
     r1 = h.nreg - 1;

@@ -71,9 +71,12 @@ struct HfDev {
   int mode = 0, nreg = 0, natom = 0, nw = 3;
   const double *z1 = nullptr, *z2 = nullptr;   // [nreg]
   // regions that tile [ubz0, ubz0 + nreg ubw) in equal bins (ubin = 1): a
-  // segment's regions are then found from its ends instead of by a walk
-  int ubin = 0;
-  double ubz0 = 0.0, ubw = 1.0;
+  // segment's regions are then found from its ends instead of by a walk;
+  // tiled = 1 when moreover each region ends exactly where the next begins
+  // (z2[r] == z1[r + 1]) and z1 increases strictly: a segment's regions
+  // strictly between its end regions then take its full width
+  int ubin = 0, tiled = 0;
+  double ubz0 = 0.0, ubw = 1.0, ubwi = 1.0;
   const int32_t *hftyp = nullptr;              // [natom] 1 = ATOM, 2 = MOLE
   const int32_t *molid = nullptr;              // [natom] 1-based
   const double *molecom = nullptr;             // [3*nmol]
@@ -250,6 +253,8 @@ struct pgk_ctx {
   size_t vir6_bytes = 0;
   // calheatf (pgk_calheatf): velocities, masses, molecule lists, partial records
   double *d_hj_vel = nullptr, *d_hj_mass = nullptr, *d_hj_molevel = nullptr;
+  double *d_hj_cz = nullptr;   // centre z per atom (transport part, many regions)
+  size_t hj_cz_cap = 0;
   double *d_hj_part = nullptr, *d_hj_out = nullptr;
   int32_t *d_hj_mol_start = nullptr, *d_hj_mol_atoms = nullptr;   // atoms of each molecule
   double *d_hj_jcol = nullptr, *d_hj_jpart = nullptr;   // flux form sums
