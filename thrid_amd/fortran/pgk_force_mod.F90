@@ -26,6 +26,10 @@ module pgk_force_mod
   integer(c_int32_t), parameter :: PGK_MTS_LONG = 1, PGK_MTS_MED = 2, PGK_MTS_SHORT = 3
   integer(c_int32_t), parameter :: PGK_HF_VOLUME = 0, PGK_HF_SURFACE = 1, PGK_HF_BULK = 2
   integer(c_int32_t), parameter :: PGK_HFTYP_ATOM = 1, PGK_HFTYP_MOLE = 2
+  ! pgk_tune knobs and the compiled default of the pair-form crossover
+  integer(c_int32_t), parameter :: PGK_TUNE_PAIR_CHUNK_BELOW = 1, PGK_TUNE_PME_OVERLAP = 2, &
+       &                           PGK_TUNE_PAIR_CAP_BLOCKS = 3
+  integer(c_int64_t), parameter :: PGK_TUNE_DEFAULT_CHUNK_BELOW = 4352
   integer, parameter :: PGK_NENE = 19, PGK_NTENS = 6, PGK_NHF = 8, PGK_NHFJ = 18
   ! energy record (ene(1:PGK_NENE), 1-based here; C index + 1)
   integer, parameter :: PGK_E_TOT = 1, PGK_E_NONBON = 2, PGK_E_VDW = 3, PGK_E_ELC = 4, &
@@ -180,6 +184,15 @@ module pgk_force_mod
        type(c_ptr), value :: ctx, vstrm
        integer(c_int) :: rc
      end function pgk_set_atmvel_strm
+
+     ! launch-shape choices of one context (no reference counterpart)
+     function pgk_tune(ctx, knob, value) bind(C, name='pgk_tune') result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: knob
+       integer(c_int64_t), value :: value
+       integer(c_int) :: rc
+     end function pgk_tune
 
      ! calforce_hf and calheatf in one pass, without the per-atom arrays
      function pgk_calforce_heatf(ctx, mts_flag, mts, hf, vel, mass, out, hfj) &
