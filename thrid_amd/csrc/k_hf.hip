@@ -26,9 +26,11 @@ namespace {
 struct HfPairArgs {
   const int32_t *ci_start, *cj, *cl_atom;   // cj: half-cluster index of each entry
   const uint32_t *mask;
-  const double *q;
-  const int32_t *type, *molid, *inttyp;
-  const double *eps, *rmin;
+  // per cluster slot (k_hf_pack): position and charge, centre (w = 1 ATOM,
+  // 0 MOLE), collision-part velocity, molecule id, 0-based type
+  const double4 *xq, *cen, *v4;
+  const int32_t *mol, *ct;
+  const double *lj;       // ntype^2 x (eps, 0 unless INTTYPE_VDW; rmin^2)
   const int32_t *witem;   // chunked form: work items (cluster * 32 + chunk of 64 entries)
   int ncl, nwi, ntype;
   double sqcut, sqcut2, rrcut;
@@ -196,21 +198,12 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
   }
   if (ci < p.ncl) {
     const int ii = lane & 7, jq = (lane >> 3) & 3, hs = lane >> 5;
-    const int ai = p.cl_atom[ci * PGK_CL + ii];
-    // padding slots (ai = -1) have no mask bits and read nothing
-    double cai[3] = {0.0, 0.0, 0.0}, vi[3] = {0.0, 0.0, 0.0}, pi[3] = {0.0, 0.0, 0.0};
-    double qi = 0.0;
-    int fla = 0, moli = 0, typi = 1;
-    if (ai >= 0) {   // the i atom's data, loaded once for the whole list
-      fla = hf_centre(h, ai, BULK, cai);
-      for (int d = 0; d < 3; d++) {
-        vi[d] = h.vel[3 * ai + d];
-        pi[d] = h.pos[3 * ai + d];
-      }
-      qi = p.q[ai];
-      moli = p.molid[ai];
-      typi = p.type[ai];
-    }
+    // the i atom's slot, loaded once for the whole list (padding slots, ai =
+    // -1, have no mask bits)
+    const int islot = ci * PGK_CL + ii;
+    const int ai = p.cl_atom[islot];
+    const double4 xi = p.xq[islot], cci = p.cen[islot], vi = p.v4[islot];
+    const int moli = p.mol[islot], typi = p.ct[islot];
     // per-atom energies: the i atom's shares stay in registers for the whole
     // list; the j atom's are summed over the 8 lanes that share it (lanes
     // ii = 0..7 of one (jq, hs) group) and added once per entry
@@ -224,33 +217,34 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
       const int e = e2 + hs;
       const bool ein = e < e1;
       const uint32_t m = ein ? p.mask[e] : 0u;
-      const int aj = ein ? p.cl_atom[p.cj[e] * 4 + jq] : -1;
+      const int js = ein ? p.cj[e] * 4 + jq : 0;
+      const int aj = ein ? p.cl_atom[js] : -1;
       double ej[4] = {0.0, 0.0, 0.0, 0.0};
       if ((m >> (lane & 31)) & 1u) {
-        double r[3];
-        for (int d = 0; d < 3; d++) {
-          r[d] = pi[d] - h.pos[3 * aj + d];
-          r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
-        }
+        const double4 xj = p.xq[js];
+        double r[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
+        for (int d = 0; d < 3; d++) r[d] = r[d] - h.L[d] * rint(r[d] * h.Li[d]);
         const double r2 = r2_exact(r);
         if (r2 <= p.sqcut) {
           // the cutoff tests stay exact FP64 decisions; the pair terms are in
           // the compute type R (FP32 in the mixed form: one hardware rsqrt)
           const R rinv1 = hf_rinv<R>(r2);
           const R rinv2 = rinv1 * rinv1;
-          const bool same = moli == p.molid[aj];
+          const bool same = moli == p.mol[js];
+          const double4 ccj = p.cen[js];
           // the centres' separation and, in the volume form, whether it counts
-          double caj[3] = {0.0, 0.0, 0.0}, rc[3], rz = 0.0;
+          double rc[3], rz = 0.0;
           bool seg = true;
-          if (BULK && fla == 1 && h.hftyp[aj] != PGK_HFTYP_MOLE) {
+          if (BULK && cci.w != 0.0 && ccj.w != 0.0) {
             // two atom centres in the bulk form: the centres' minimum image
             // is the pair's own r (round and rint differ only at a
             // half-box tie, which no listed pair reaches)
             for (int d = 0; d < 3; d++) rc[d] = r[d];
           } else {
-            const int fl = fla + hf_centre(h, aj, BULK, caj);
-            seg = !(fl == 0 && same);
-            for (int d = 0; d < 3; d++) rc[d] = cai[d] - caj[d];
+            seg = !(cci.w == 0.0 && ccj.w == 0.0 && same);
+            rc[0] = cci.x - ccj.x;
+            rc[1] = cci.y - ccj.y;
+            rc[2] = cci.z - ccj.z;
             if (BULK) {
               for (int d = 0; d < 3; d++) rc[d] = rc[d] - h.L[d] * round(rc[d] * h.Li[d]);
             } else {
@@ -259,25 +253,23 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
             }
           }
           // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
-          const double *vjp = h.vel + 3 * aj;
-          const double w[3] = {vi[0] + vjp[0], vi[1] + vjp[1], vi[2] + vjp[2]};
-          const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
-          const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cai[2], caj[2], rz};
+          const double4 vj = p.v4[js];
+          const double rw = r[0] * (vi.x + vj.x) + r[1] * (vi.y + vj.y) + r[2] * (vi.z + vj.z);
+          const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cci.z, ccj.z, rz};
           double svc = 0.0, svl = 0.0;
           if (r2 <= p.rrsq) {
-            const R qq = (R)qi * (R)p.q[aj];
+            const R qq = (R)xi.w * (R)xj.w;
             const R ph = qq * rinv1;
             if (same) flux_add<0, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
             else flux_add<2, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
           }
           if (r2 <= p.sqcut2) {
-            const size_t tij = (size_t)(typi - 1) + (size_t)p.ntype * (p.type[aj] - 1);
-            if (p.inttyp[tij] == 1) {
-              const R rad = (R)p.rmin[tij];
-              const R t2 = rinv2 * (rad * rad);
+            const int tp = typi + p.ntype * p.ct[js];
+            const R ep = (R)p.lj[2 * tp];
+            if (ep != R(0)) {   // INTTYPE_VDW pairs only
+              const R t2 = rinv2 * (R)p.lj[2 * tp + 1];
               const R t6 = t2 * t2 * t2;
               const R t12 = t6 * t6;
-              const R ep = (R)p.eps[tij];
               const R pot = ep * (t12 - (R)2 * t6);
               const R ft = (R)12 * ep * (t12 - t6) * rinv2;
               if (same) flux_add<1, BULK>(fp, pot, ft, ei, ej, acc, svl);
@@ -358,12 +350,6 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.cj = c->d_cj;
     p.cl_atom = c->d_cl_atom;
     p.mask = c->d_mask;
-    p.q = c->d_q;
-    p.type = c->d_type;
-    p.molid = c->d_molid;
-    p.inttyp = c->d_inttyp;
-    p.eps = c->d_eps;
-    p.rmin = c->d_rmin;
     p.witem = c->d_witem;
     p.ncl = c->ncl;
     p.nwi = c->nwi;
@@ -376,9 +362,26 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     // the block count sizes jpart (hf_setup) and the collect pass reads it
     const int nblk = pgk_grid_blocks(pgk_pair_waves(c), 4);
     const bool ch = pgk_pair_chunked(c);
-    if (!h.flux) {
-      return pgk_hf_w_launch(c);   // per-atom arrays: k_hf_w.hip
-    } else if (h.mode == PGK_HF_BULK) {
+    if (!h.flux) return pgk_hf_w_launch(c);   // per-atom arrays: k_hf_w.hip
+    // the flux form reads its atoms' data per cluster slot
+    const int nslot = c->ncl * PGK_CL;
+    int rc;
+    if ((rc = grow_dev((void **)&c->d_hfw_xq, &c->hfw_xq_bytes, sizeof(double4) * nslot)) ||
+        (rc = grow_dev((void **)&c->d_hfw_cen, &c->hfw_cen_bytes, sizeof(double4) * nslot)) ||
+        (rc = grow_dev((void **)&c->d_hfw_mol, &c->hfw_mol_bytes, sizeof(int32_t) * nslot)) ||
+        (rc = grow_dev((void **)&c->d_hfw_v, &c->hfw_v_bytes, sizeof(double4) * nslot)))
+      return rc;
+    k_hf_pack<<<pgk_grid_blocks(nslot, 256), 256, 0, st>>>(nslot, c->d_cl_atom, h, c->d_q,
+                                                             c->d_hfw_xq, c->d_hfw_cen,
+                                                             c->d_hfw_mol, c->d_hfw_v);
+    HIP_CHECK(hipGetLastError());
+    p.xq = c->d_hfw_xq;
+    p.cen = c->d_hfw_cen;
+    p.v4 = c->d_hfw_v;
+    p.mol = c->d_hfw_mol;
+    p.ct = c->d_ct;
+    p.lj = c->d_ljp64;
+    if (h.mode == PGK_HF_BULK) {
       if (ch)
         k_hf_pairs_flux<AH, true, true><<<nblk, 256, 0, st>>>(p, h);
       else

@@ -316,10 +316,14 @@ __global__ __launch_bounds__(256) void k_hf_w(HfWArgs p, HfDev h) {
   }
 }
 
-// per slot: FP64 position and charge, heat-flux centre, molecule id
+}  // namespace
+
+// per slot: FP64 position and charge, heat-flux centre (w = 1 ATOM, 0 MOLE),
+// molecule id, and with vel4 the collision-part velocity (h.vel)
 __global__ void k_hf_pack(int nslot, const int32_t *__restrict__ cl_atom, HfDev h,
                           const double *__restrict__ q, double4 *__restrict__ xq,
-                          double4 *__restrict__ cen, int32_t *__restrict__ mol) {
+                          double4 *__restrict__ cen, int32_t *__restrict__ mol,
+                          double4 *__restrict__ vel4) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
   const int a = cl_atom[s];
@@ -327,6 +331,7 @@ __global__ void k_hf_pack(int nslot, const int32_t *__restrict__ cl_atom, HfDev 
     xq[s] = make_double4(0.0, 0.0, 0.0, 0.0);
     cen[s] = make_double4(0.0, 0.0, 0.0, 1.0);
     mol[s] = 0;
+    if (vel4) vel4[s] = make_double4(0.0, 0.0, 0.0, 0.0);
     return;
   }
   xq[s] = make_double4(h.pos[3 * a], h.pos[3 * a + 1], h.pos[3 * a + 2], q[a]);
@@ -334,7 +339,10 @@ __global__ void k_hf_pack(int nslot, const int32_t *__restrict__ cl_atom, HfDev 
   const int atomc = hf_centre(h, a, h.mode == PGK_HF_BULK, c);
   cen[s] = make_double4(c[0], c[1], c[2], atomc ? 1.0 : 0.0);
   mol[s] = h.molid[a];
+  if (vel4) vel4[s] = make_double4(h.vel[3 * a], h.vel[3 * a + 1], h.vel[3 * a + 2], 0.0);
 }
+
+namespace {
 
 // transposed list: entries grouped by half-cluster, each holding its
 // cluster and the mask with bit ii + 8 jq moved to bit jq + 4 ii
@@ -414,7 +422,7 @@ int pgk_hf_w_launch(pgk_ctx *c) {
     return rc;
   k_hf_pack<<<pgk_grid_blocks(nslot, 256), 256, 0, st>>>(nslot, c->d_cl_atom, h, c->d_q,
                                                            c->d_hfw_xq, c->d_hfw_cen,
-                                                           c->d_hfw_mol);
+                                                           c->d_hfw_mol, nullptr);
   HIP_CHECK(hipGetLastError());
   if (c->tl_serial != c->list_serial) {   // transposed list of the current list
     if ((rc = grow_dev((void **)&c->d_tl_start, &c->tl_start_bytes,

@@ -912,7 +912,7 @@ int pgk_destroy(pgk_ctx *c) {
                   c->d_md_nhc, c->d_lt_ustart, c->d_lt_uatom, c->d_lt_ucom, c->d_lt_reg,
                   c->d_lt_cnt, c->d_lt_mass, c->d_lt_frand, c->d_lt_det, c->d_lt_sreg, c->d_md_vstrm,
                   c->d_hf_vstrm, c->d_hj_vpec, c->d_vgrid,
-                  c->d_vir6, c->d_hfw_xq, c->d_hfw_cen, c->d_hfw_mol, c->d_tl_start,
+                  c->d_vir6, c->d_hfw_xq, c->d_hfw_cen, c->d_hfw_mol, c->d_hfw_v, c->d_tl_start,
                   c->d_tl_src, c->d_tl_mask, c->d_hj_vel, c->d_hj_mass, c->d_hj_molevel,
                   c->d_hj_cz,
                   c->d_hj_part, c->d_hj_out, c->d_hj_mol_start, c->d_hj_mol_atoms,

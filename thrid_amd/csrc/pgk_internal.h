@@ -240,6 +240,8 @@ struct pgk_ctx {
   double4 *d_hfw_xq = nullptr, *d_hfw_cen = nullptr;
   int32_t *d_hfw_mol = nullptr;
   size_t hfw_xq_bytes = 0, hfw_cen_bytes = 0, hfw_mol_bytes = 0;
+  double4 *d_hfw_v = nullptr;   // per slot: collision-part velocity (fused flux pass)
+  size_t hfw_v_bytes = 0;
   int32_t *d_tl_start = nullptr, *d_tl_src = nullptr;
   uint32_t *d_tl_mask = nullptr;
   size_t tl_start_bytes = 0, tl_src_bytes = 0, tl_mask_bytes = 0;
@@ -378,6 +380,12 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
 // zero count is skipped) in one kernel launch on st
 int pgk_zero3_launch(hipStream_t st, void *p0, size_t n0, void *p1, size_t n1, void *p2,
                      size_t n2);
+// per cluster slot: position and charge, heat-flux centre, molecule id and
+// (vel4 non-null) the collision-part velocity, for the heat-flux pair passes
+__global__ void k_hf_pack(int nslot, const int32_t *__restrict__ cl_atom, HfDev h,
+                          const double *__restrict__ q, double4 *__restrict__ xq,
+                          double4 *__restrict__ cen, int32_t *__restrict__ mol,
+                          double4 *__restrict__ vel4);
 int pgk_heatf_collision_vel(pgk_ctx *c, const double *d_vel, const double *d_vstrm,
                             const double **out);
 // molecule centres of mass from d_pos (calcom.F90:1-147) into d_com [3*nmol]
