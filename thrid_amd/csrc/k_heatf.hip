@@ -316,7 +316,7 @@ int pgk_heatf_collision_vel(pgk_ctx *c, const double *d_vel, const double *d_vst
 // are already in d_vel / d_mass (device, natom entries), the result goes to
 // c->d_hj_out.
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
-                     const double *d_vstrm) {
+                     const double *d_vstrm, double *out) {
   const HfDev &h = c->hfdev;
   const int n = c->natom;
   HeatfArgs a;
@@ -377,7 +377,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
     default: rc = launch_heatf<PGK_PREC_FP32>(c, a, ptr, pcol); break;
   }
   if (rc) return rc;
-  k_heatf_final<<<h.nreg, 256, 0, c->stream>>>(a, ptr, pcol, c->d_hj_out);
+  k_heatf_final<<<h.nreg, 256, 0, c->stream>>>(a, ptr, pcol, out ? out : c->d_hj_out);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }

@@ -1637,11 +1637,12 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   c->last_mts = last_mts;
   c->last_pending = last_pending;
   if (rc) return rc;
-  if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass, vs))) return rc;
-  c->hfdev.flux = 0;
+  // the rows go straight to the series
   const size_t row = (size_t)nreg * PGK_NHFJ * 3;
-  HIP_CHECK(hipMemcpyAsync(c->d_md_hfseries + (size_t)c->md_hf_rows * row, c->d_hj_out,
-                           sizeof(double) * row, hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = pgk_heatf_launch(c, c->d_md_vel, c->d_md_hfmass, vs,
+                             c->d_md_hfseries + (size_t)c->md_hf_rows * row)))
+    return rc;
+  c->hfdev.flux = 0;
   c->md_hf_rows++;
   return PGK_OK;
 }

@@ -373,9 +373,10 @@ int pgk_hf_launch_pairs(pgk_ctx *c, bool do_pairs);
 int pgk_hf_w_launch(pgk_ctx *c);
 int pgk_hf_export(pgk_ctx *c);
 // d_vstrm: streaming velocities [3*natom] (kinetic energies and collision
-// part relative to them), or null
+// part relative to them), or null; the rows go to out (nreg x PGK_NHFJ x 3
+// doubles on the device), or to c->d_hj_out when out is null
 int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
-                     const double *d_vstrm);
+                     const double *d_vstrm, double *out = nullptr);
 // zero n0, n1 and n2 8-byte words at p0, p1 and p2 (a null pointer with a
 // zero count is skipped) in one kernel launch on st
 int pgk_zero3_launch(hipStream_t st, void *p0, size_t n0, void *p1, size_t n1, void *p2,
