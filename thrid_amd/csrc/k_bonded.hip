@@ -45,8 +45,9 @@ struct BArgs {
 
 // rec: E, V, T(9 in Fortran (a,n) order)
 template <class A, class AH, int HFM>
-__global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restrict__ bl, int ok,
-                                               int orr, A *fat, HfDev h) {
+__device__ __forceinline__ void bonds_body(int bid, const BArgs &a,
+                                           const int32_t *__restrict__ bl, int ok, int orr,
+                                           A *fat, const HfDev &h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
   AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
   AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
   if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bid * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = bl[3 * t], j = bl[3 * t + 1], ty = bl[3 * t + 2];
     HfAt at[2];
@@ -88,12 +89,13 @@ __global__ __launch_bounds__(256) void k_bonds(BArgs a, const int32_t *__restric
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_BO);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_BO);
-  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)bid * PGK_REC);
 }
 
 template <class A, class AH, int HFM>
-__global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restrict__ al, int ok,
-                                                int ot, A *fat, HfDev h) {
+__device__ __forceinline__ void angles_body(int bid, const BArgs &a,
+                                            const int32_t *__restrict__ al, int ok, int ot,
+                                            A *fat, const HfDev &h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
   AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
   AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
   if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bid * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = al[4 * t], j = al[4 * t + 1], k = al[4 * t + 2], ty = al[4 * t + 3];
     HfAt at[3];
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(256) void k_angles(BArgs a, const int32_t *__restri
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_AN);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_AN);
-  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)bid * PGK_REC);
 }
 
 // shared torsion geometry (calijkl / calijklrb)
@@ -225,8 +227,9 @@ __device__ __forceinline__ void tforces(TGeo &g, double fa, double *fi, double *
 
 // RB = true: C_0..C_5 at par[oc + 6*ty]; false: periodic (div, barhig, phase, period)
 template <class A, class AH, int HFM, bool RB>
-__global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict__ tl, int o0,
-                                              int o1, int o2, int o3, A *fat, HfDev h) {
+__device__ __forceinline__ void tors_body(int bid, const BArgs &a,
+                                          const int32_t *__restrict__ tl, int o0, int o1, int o2,
+                                          int o3, A *fat, const HfDev &h) {
   double ev[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) ev[k] = 0.0;
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
   AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
   AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
   if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bid * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = tl[5 * t], j = tl[5 * t + 1], k = tl[5 * t + 2], l = tl[5 * t + 3], ty = tl[5 * t + 4];
     HfAt at[4];
@@ -315,18 +318,19 @@ __global__ __launch_bounds__(256) void k_tors(BArgs a, const int32_t *__restrict
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_TO);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_TO);
-  if (a.red) block_record<11, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<11, 4>(ev, a.red + (size_t)bid * PGK_REC);
 }
 
 // 1-4 pairs; rec: E_elc14, E_vdw14, V, T(9)
 template <class A, class AH, int HFM>
-__global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restrict__ pl,
-                                               const double *__restrict__ pf,
-                                               const double *__restrict__ q,
-                                               const int32_t *__restrict__ type,
-                                               const double *__restrict__ eps,
-                                               const double *__restrict__ rmin, int ntype,
-                                               A *fat, HfDev h) {
+__device__ __forceinline__ void non14_body(int bid, const BArgs &a,
+                                           const int32_t *__restrict__ pl,
+                                           const double *__restrict__ pf,
+                                           const double *__restrict__ q,
+                                           const int32_t *__restrict__ type,
+                                           const double *__restrict__ eps,
+                                           const double *__restrict__ rmin, int ntype, A *fat,
+                                           const HfDev &h) {
   double ev[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) ev[k] = 0.0;
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
   AH *sj = reinterpret_cast<AH *>(bl_lds);   // flux form: the block's sums
   AH jb[3] = {(AH)0, (AH)0, (AH)0};   // bulk flux form: this thread's sums
   if (HFM == 3) hf_flux_block_begin_vol<AH>(h, sj);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = bid * blockDim.x + threadIdx.x;
   if (t < a.n) {
     int i = pl[2 * t], j = pl[2 * t + 1];
     HfAt at[2];
@@ -386,7 +390,40 @@ __global__ __launch_bounds__(256) void k_non14(BArgs a, const int32_t *__restric
   }
   if (HFM == 2) hf_flux_block_end_bulk<AH>(h, jb, sj, PGK_HF_14);
   if (HFM == 3) hf_flux_block_end_vol<AH>(h, sj, PGK_HF_14);
-  if (a.red) block_record<12, 4>(ev, a.red + (size_t)blockIdx.x * PGK_REC);
+  if (a.red) block_record<12, 4>(ev, a.red + (size_t)bid * PGK_REC);
+}
+
+// Every bonded term of one call in one launch: blocks [0, g1) bonds, then
+// angles, periodic torsions, RB torsions and 1-4 pairs up to g5; each block
+// runs its term's body with its index within the term.
+template <class A>
+struct BondedLaunch {
+  BArgs b, an, t, rb, p14;
+  const int32_t *bl, *al, *tl, *rl, *pl;
+  int ob_k, ob_r, oa_k, oa_t, ot_div, ot_b, ot_p, ot_per, orb_c;
+  const double *p14f, *q, *eps, *rmin;
+  const int32_t *type;
+  int ntype;
+  int g1, g2, g3, g4, g5;
+  A *fat;
+};
+
+template <class A, class AH, int HFM>
+__global__ __launch_bounds__(256) void k_bonded(BondedLaunch<A> L, HfDev h) {
+  const int b = blockIdx.x;
+  if (b < L.g1) {
+    bonds_body<A, AH, HFM>(b, L.b, L.bl, L.ob_k, L.ob_r, L.fat, h);
+  } else if (b < L.g2) {
+    angles_body<A, AH, HFM>(b - L.g1, L.an, L.al, L.oa_k, L.oa_t, L.fat, h);
+  } else if (b < L.g3) {
+    tors_body<A, AH, HFM, false>(b - L.g2, L.t, L.tl, L.ot_div, L.ot_b, L.ot_p, L.ot_per, L.fat,
+                                 h);
+  } else if (b < L.g4) {
+    tors_body<A, AH, HFM, true>(b - L.g3, L.rb, L.rl, L.orb_c, 0, 0, 0, L.fat, h);
+  } else {
+    non14_body<A, AH, HFM>(b - L.g4, L.p14, L.pl, L.p14f, L.q, L.type, L.eps, L.rmin, L.ntype,
+                           L.fat, h);
+  }
 }
 
 }  // namespace
@@ -423,42 +460,74 @@ static int launch_bonded(pgk_ctx *c, const pgk_mts *m, int mts_flag, int *seg_of
     c->red_used += nblk;
     if (c->hf_pairs_only) a.red = nullptr;   // a heat-flux sample reads no records
   };
+  BondedLaunch<A> L;
+  L.fat = fat;
+  int gend = 0;
   int nb = c->nbond_act;
   if (m->bond == mts_flag && nb > 0) {
     a.n = nb;
-    int g = pgk_grid_blocks(nb, 256);
+    const int g = pgk_grid_blocks(nb, 256);
     seg(g, 1);
-    k_bonds<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_bond, c->off_bk, c->off_br, fat, h);
+    L.b = a;
+    gend += g;
   }
+  L.g1 = gend;
+  L.bl = c->d_bond;
+  L.ob_k = c->off_bk;
+  L.ob_r = c->off_br;
   int na = c->nangl_act;
   if (m->angl == mts_flag && na > 0) {
     a.n = na;
-    int g = pgk_grid_blocks(na, 256);
+    const int g = pgk_grid_blocks(na, 256);
     seg(g, 2);
-    k_angles<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_angl, c->off_ak, c->off_at, fat, h);
+    L.an = a;
+    gend += g;
   }
+  L.g2 = gend;
+  L.al = c->d_angl;
+  L.oa_k = c->off_ak;
+  L.oa_t = c->off_at;
   int nt = (int)(c->tors_type.size());
   if (m->tors == mts_flag && nt > 0) {
     a.n = nt;
-    int g = pgk_grid_blocks(nt, 256);
+    const int g = pgk_grid_blocks(nt, 256);
     seg(g, 3);
-    k_tors<A, AH, HFM, false><<<g, 256, lds, st>>>(a, c->d_tors, c->off_tdiv, c->off_tb,
-                                                c->off_tp, c->off_tper, fat, h);
+    L.t = a;
+    gend += g;
   }
+  L.g3 = gend;
+  L.tl = c->d_tors;
+  L.ot_div = c->off_tdiv;
+  L.ot_b = c->off_tb;
+  L.ot_p = c->off_tp;
+  L.ot_per = c->off_tper;
   int nr = (int)(c->rb_type.size());
   if (m->torsrb == mts_flag && nr > 0) {
     a.n = nr;
-    int g = pgk_grid_blocks(nr, 256);
+    const int g = pgk_grid_blocks(nr, 256);
     seg(g, 4);
-    k_tors<A, AH, HFM, true><<<g, 256, lds, st>>>(a, c->d_rb, c->off_rbc, 0, 0, 0, fat, h);
+    L.rb = a;
+    gend += g;
   }
+  L.g4 = gend;
+  L.rl = c->d_rb;
+  L.orb_c = c->off_rbc;
   if (m->nonb14 == mts_flag && c->n14 > 0) {
     a.n = c->n14;
-    int g = pgk_grid_blocks(c->n14, 256);
+    const int g = pgk_grid_blocks(c->n14, 256);
     seg(g, 5);
-    k_non14<A, AH, HFM><<<g, 256, lds, st>>>(a, c->d_p14, c->d_p14f, c->d_q, c->d_type, c->d_eps,
-                                           c->d_rmin, c->ntype, fat, h);
+    L.p14 = a;
+    gend += g;
   }
+  L.g5 = gend;
+  L.pl = c->d_p14;
+  L.p14f = c->d_p14f;
+  L.q = c->d_q;
+  L.type = c->d_type;
+  L.eps = c->d_eps;
+  L.rmin = c->d_rmin;
+  L.ntype = c->ntype;
+  if (gend > 0) k_bonded<A, AH, HFM><<<gend, 256, lds, st>>>(L, h);
   HIP_CHECK(hipGetLastError());
   return PGK_OK;
 }
