@@ -318,6 +318,10 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   // the other)
   const bool timed_other = c->prof && (c->prof_mask & ~(1u << PGK_PROF_PAIRS)) != 0;
   const bool overlap = do_pair && do_ewk && !hf && !timed_other && c->pme_overlap;
+  // beside the chain the excluded-pair correction goes to the shorter stream:
+  // the chain's for large systems, the pair kernel's for small ones (chunked
+  // form), whose reciprocal chain outlasts the pair and bonded kernels
+  const bool excl_main = overlap && pgk_pair_chunked(c);
   if (overlap) {
     HIP_CHECK(hipEventRecord(c->ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
@@ -325,8 +329,10 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
     rc = pgk_pme_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
     if (!rc) {
       seg_kind[nseg++] = SEG_PME;
-      rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
-      if (!rc && seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
+      if (!excl_main) {
+        rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]);
+        if (!rc && seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
+      }
     }
     c->stream = st;
     if (rc) return rc;
@@ -362,6 +368,10 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   if ((rc = pgk_bonded_launch2(c, m, mts_flag, seg_off, seg_n, seg_kind, &nseg, hf))) return rc;
   pgk_prof_end(c, PGK_PROF_BONDED);
   if (overlap) {
+    if (excl_main) {
+      if ((rc = pgk_excl_launch(c, split, &seg_off[nseg], &seg_n[nseg]))) return rc;
+      if (seg_n[nseg] > 0) seg_kind[nseg++] = SEG_EXCL;
+    }
     HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
   } else if (do_pair) {
     pgk_prof_begin(c, PGK_PROF_EXCL);
