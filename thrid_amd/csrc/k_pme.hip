@@ -916,6 +916,118 @@ __global__ __launch_bounds__(256) void k_fft_y(void *cgrid, int H, int K2, int T
   }
 }
 
+// Small grids: the x and y passes of one z plane in one block, the plane
+// held in LDS between them (forward: real rows -> x half spectra -> y lines;
+// backward the reverse).  The arithmetic per line is the separate passes'
+// (k_fft_x_fwd / k_fft_y / k_fft_x_bwd), so the results are the same; the
+// reciprocal chain of a small system loses two launches per force call.
+// LDS: row pairs [NL][K1] x 2, y lines [H][K2 + 1] x 2, twiddles K1 + K2.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_xy_fwd(const void *grid, void *cgrid, int K1,
+                                                     int K2, int H, FftPlan p1, const void *tw1v,
+                                                     FftPlan p2, const void *tw2v) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int NL = (K2 + 1) / 2, ly = K2 + 1;
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + NL * K1;
+  C *qa = pb + NL * K1;
+  C *qb = qa + H * ly;
+  C *tw1 = qb + H * ly;
+  C *tw2 = tw1 + K1;
+  for (int t = threadIdx.x; t < K1; t += blockDim.x) tw1[t] = reinterpret_cast<const C *>(tw1v)[t];
+  for (int t = threadIdx.x; t < K2; t += blockDim.x) tw2[t] = reinterpret_cast<const C *>(tw2v)[t];
+  const int k3 = blockIdx.x;
+  const R *gr = reinterpret_cast<const R *>(grid) + (size_t)k3 * K2 * K1;
+  const float r1 = 1.0f / (float)K1, rh = 1.0f / (float)H, r2 = 1.0f / (float)K2;
+  for (int t = threadIdx.x; t < NL * K1; t += blockDim.x) {
+    const int l = fdiv(t, K1, r1), k = t - l * K1;
+    const int ra = 2 * l, rb = ra + 1;
+    C z;
+    z.x = gr[(size_t)ra * K1 + k];
+    z.y = (rb < K2) ? gr[(size_t)rb * K1 + k] : R(0);
+    pa[t] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, NL, K1, p1, tw1, -1);
+  for (int t = threadIdx.x; t < NL * H; t += blockDim.x) {
+    const int l = fdiv(t, H, rh), k = t - l * H;
+    const int ra = 2 * l, rb = ra + 1;
+    const C zk = pa[l * K1 + k];
+    const C zn = pa[l * K1 + ((K1 - k) % K1)];
+    // A = (Z_k + conj Z_{n-k})/2,  B = (Z_k - conj Z_{n-k})/(2i)
+    C A, B;
+    A.x = R(0.5) * (zk.x + zn.x);
+    A.y = R(0.5) * (zk.y - zn.y);
+    B.x = R(0.5) * (zk.y + zn.y);
+    B.y = R(0.5) * (zn.x - zk.x);
+    qa[k * ly + ra] = A;
+    if (rb < K2) qa[k * ly + rb] = B;
+  }
+  __syncthreads();
+  fft_lds<C>(qa, qb, H, ly, p2, tw2, -1);
+  C *cg = reinterpret_cast<C *>(cgrid) + (size_t)k3 * H * K2;
+  for (int t = threadIdx.x; t < H * K2; t += blockDim.x) {
+    const int h = fdiv(t, K2, r2), k2 = t - h * K2;
+    cg[(size_t)h * K2 + k2] = qa[h * ly + k2];
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_fft_yx_bwd(const void *cgrid, void *grid, int K1,
+                                                     int K2, int H, FftPlan p1, const void *tw1v,
+                                                     FftPlan p2, const void *tw2v) {
+  using R = typename Prec<PREC>::real;
+  using C = typename C2T<R>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int NL = (K2 + 1) / 2, ly = K2 + 1;
+  C *pa = reinterpret_cast<C *>(smem);
+  C *pb = pa + NL * K1;
+  C *qa = pb + NL * K1;
+  C *qb = qa + H * ly;
+  C *tw1 = qb + H * ly;
+  C *tw2 = tw1 + K1;
+  for (int t = threadIdx.x; t < K1; t += blockDim.x) tw1[t] = reinterpret_cast<const C *>(tw1v)[t];
+  for (int t = threadIdx.x; t < K2; t += blockDim.x) tw2[t] = reinterpret_cast<const C *>(tw2v)[t];
+  const int k3 = blockIdx.x;
+  const C *cg = reinterpret_cast<const C *>(cgrid) + (size_t)k3 * H * K2;
+  const float r1 = 1.0f / (float)K1, r2 = 1.0f / (float)K2;
+  for (int t = threadIdx.x; t < H * K2; t += blockDim.x) {
+    const int h = fdiv(t, K2, r2), k2 = t - h * K2;
+    qa[h * ly + k2] = cg[(size_t)h * K2 + k2];
+  }
+  __syncthreads();
+  fft_lds<C>(qa, qb, H, ly, p2, tw2, +1);
+  for (int t = threadIdx.x; t < NL * K1; t += blockDim.x) {
+    const int l = fdiv(t, K1, r1), k = t - l * K1;
+    const int ra = 2 * l, rb = ra + 1;
+    const int kk = (k < H) ? k : K1 - k;
+    C A = qa[kk * ly + ra], B;
+    B.x = B.y = R(0);
+    if (rb < K2) B = qa[kk * ly + rb];
+    if (k >= H) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    // Z = A + i B
+    C z;
+    z.x = A.x - B.y;
+    z.y = A.y + B.x;
+    pa[t] = z;
+  }
+  __syncthreads();
+  fft_lds<C>(pa, pb, NL, K1, p1, tw1, +1);
+  R *gr = reinterpret_cast<R *>(grid) + (size_t)k3 * K2 * K1;
+  for (int t = threadIdx.x; t < NL * K1; t += blockDim.x) {
+    const int l = fdiv(t, K1, r1), k = t - l * K1;
+    const int ra = 2 * l, rb = ra + 1;
+    const C z = pa[t];
+    gr[(size_t)ra * K1 + k] = z.x;
+    if (rb < K2) gr[(size_t)rb * K1 + k] = z.y;
+  }
+}
+
 // y pass with the register FFT: one wavefront per block, LPW lines (h fastest
 // across lines, then k3), so consecutive lanes read consecutive h
 template <int PREC, class F, int SGN>
@@ -1413,6 +1525,15 @@ int lines_for(int n, size_t csize, int maxl) {
 }
 size_t fft_lds_bytes(int nl, int n, size_t csize) { return (size_t)(2 * nl * (n + 1) + n) * csize; }
 
+// LDS of the fused plane kernels (k_fft_xy_fwd / k_fft_yx_bwd), 0 when they
+// do not apply: a register FFT on x or y, or a plane above 64 KiB
+size_t fft_plane_bytes(int K1, int K2, size_t csize) {
+  if (with_reg_fft(K1, [](auto) {}) || with_reg_fft(K2, [](auto) {})) return 0;
+  const int NL = (K2 + 1) / 2, H = K1 / 2 + 1;
+  const size_t b = (size_t)(2 * NL * K1 + 2 * H * (K2 + 1) + K1 + K2) * csize;
+  return b <= 65536 ? b : 0;
+}
+
 // Device layout of the per-slot PME data: slots are the list's spatial order
 // when a list exists (ncl*8 with padding), else atom order.
 struct PmeLayout {
@@ -1574,16 +1695,26 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   // register FFT where the line length has one (32, 64, 128, 256), batched
   // Stockham in LDS otherwise; each axis chooses on its own
   const void *tw1 = twc + c->tw_off[0], *tw2 = twc + c->tw_off[1], *tw3 = twc + c->tw_off[2];
+  // small grids: x and y of a z plane in one block
+  const size_t plane_lds = fft_plane_bytes(K1, K2, csz);
   const int nlx = lines_for(K1, csz, 16);
   const int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
-  const bool rx = with_reg_fft(K1, [&](auto f) {
-    using F = decltype(f);
-    const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
-    k_fft_x_fwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_grid, c->d_cgrid, nrow, tw1, H, K2);
-  });
-  if (!rx)
-    k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow,
-                                                                 nlx, p1, tw1, H, K2);
+  bool rx = false;
+  if (plane_lds) {
+    const int lr = pgk_lds_limit((const void *)k_fft_xy_fwd<PREC>, plane_lds);
+    if (lr) return lr;
+    k_fft_xy_fwd<PREC><<<K3, 256, plane_lds, st>>>(c->d_grid, c->d_cgrid, K1, K2, H, p1, tw1,
+                                                   p2, tw2);
+  } else {
+    rx = with_reg_fft(K1, [&](auto f) {
+      using F = decltype(f);
+      const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+      k_fft_x_fwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_grid, c->d_cgrid, nrow, tw1, H, K2);
+    });
+    if (!rx)
+      k_fft_x_fwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_grid, c->d_cgrid, nrow,
+                                                                   nlx, p1, tw1, H, K2);
+  }
   HIP_CHECK(hipGetLastError());
   const int twy = lines_for(K2, csz, 16);
   const int by = K3 * ((H + twy - 1) / twy);
@@ -1598,7 +1729,7 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
       k_fft_y<PREC><<<by, 256, fft_lds_bytes(twy, K2, csz), st>>>(c->d_cgrid, H, K2, twy, p2,
                                                                   tw2, SGN);
   };
-  fft_y(std::integral_constant<int, -1>{});
+  if (!plane_lds) fft_y(std::integral_constant<int, -1>{});
   HIP_CHECK(hipGetLastError());
   *off = c->red_used;
   double *red = c->d_red + (size_t)c->red_used * PGK_REC;
@@ -1617,17 +1748,24 @@ int launch_pme(pgk_ctx *c, bool split, int *off, int *nrec) {
   }
   c->red_used += *nrec;
   HIP_CHECK(hipGetLastError());
-  fft_y(std::integral_constant<int, +1>{});
   // potential grid back into the real grid buffer (compute type)
-  if (!rx)
-    k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_pgrid, nrow,
-                                                                 nlx, p1, tw1, H, K2);
-  else
-    with_reg_fft(K1, [&](auto f) {
-      using F = decltype(f);
-      const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
-      k_fft_x_bwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_cgrid, c->d_pgrid, nrow, tw1, H, K2);
-    });
+  if (plane_lds) {
+    const int lr = pgk_lds_limit((const void *)k_fft_yx_bwd<PREC>, plane_lds);
+    if (lr) return lr;
+    k_fft_yx_bwd<PREC><<<K3, 256, plane_lds, st>>>(c->d_cgrid, c->d_pgrid, K1, K2, H, p1, tw1,
+                                                   p2, tw2);
+  } else {
+    fft_y(std::integral_constant<int, +1>{});
+    if (!rx)
+      k_fft_x_bwd<PREC><<<bx, 256, (2 * nlx + 1) * K1 * csz, st>>>(c->d_cgrid, c->d_pgrid, nrow,
+                                                                   nlx, p1, tw1, H, K2);
+    else
+      with_reg_fft(K1, [&](auto f) {
+        using F = decltype(f);
+        const int nb = ((nrow + 1) / 2 + F::LPW - 1) / F::LPW;
+        k_fft_x_bwd_reg<PREC, F><<<nb, 64, 0, st>>>(c->d_cgrid, c->d_pgrid, nrow, tw1, H, K2);
+      });
+  }
   HIP_CHECK(hipGetLastError());
   pgk_prof_end(c, PGK_PROF_FFT);
   pgk_prof_begin(c, PGK_PROF_GATHER);
@@ -1787,6 +1925,8 @@ static int launch_atom_virial(pgk_ctx *c, double *d_w) {
   ca.H = H;
   ca.ifnetq = 0;
   const void *tw1 = twc + c->tw_off[0], *tw2 = twc + c->tw_off[1], *tw3 = twc + c->tw_off[2];
+  // small grids: x and y of a z plane in one block
+  const size_t plane_lds = fft_plane_bytes(K1, K2, csz);
   const int nlx = lines_for(K1, csz, 16);
   const int bx = (nrow + 2 * nlx - 1) / (2 * nlx);
   const int twy = lines_for(K2, csz, 16);
