@@ -1555,8 +1555,11 @@ static int pme_layout(pgk_ctx *c, size_t rsize, PmeLayout *L) {
   L->g.order = c->order;
   TileGeom &tg = L->tg;
   tg.ntile = 1;
+  // spreading tiles of up to 8^3 points (one wave each); small grids take
+  // 4^3 tiles, whose eight times as many waves fill the device
+  const int tmax = ((int64_t)c->nfft[0] * c->nfft[1] * c->nfft[2] <= 64 * 64 * 64) ? 4 : 8;
   for (int d = 0; d < 3; d++) {
-    int T = 8;
+    int T = tmax;
     while (c->nfft[d] % T) T--;
     tg.T[d] = T;
     tg.nt[d] = c->nfft[d] / T;
