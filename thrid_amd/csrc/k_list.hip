@@ -215,10 +215,13 @@ __device__ __forceinline__ int zsearch(const float *__restrict__ bbox, int c0, i
 // Phase B decides every atom pair of every surviving cluster pair with the
 // reference's exact FP64 test, lane = ii + 8*jj, PGK_UNROLL candidates per
 // iteration, and writes (cj, 64-bit mask) for nonzero masks.
-__global__ __launch_bounds__(256) void k_find_pairs(ListArgs a) {
-  __shared__ int32_t s_cand[4][PGK_MAXJ];
+// NW waves per block: 4, or 1 for small systems, whose few clusters then
+// spread over all CUs instead of a quarter as many blocks
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_find_pairs(ListArgs a) {
+  __shared__ int32_t s_cand[NW][PGK_MAXJ];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
+  const int ci = blockIdx.x * NW + wave;
   const int ncl = a.col_clstart[a.ncolumn];
   const bool active = ci < ncl;
   int32_t *cand = s_cand[wave];
@@ -620,7 +623,10 @@ int pgk_list_build(pgk_ctx *c, double rcut_book) {
   // a margin for the bounding-box padding and float rounding
   a.zwin = rf + (float)g.zbw + 0.01f;
   // blocks cover ci = 0 .. ncl_max so that the count past the end is cleared
-  k_find_pairs<<<(ncl_max + 1 + 3) / 4, 256, 0, st>>>(a);
+  if ((int64_t)ncl_max < c->pair_chunk_below)
+    k_find_pairs<1><<<ncl_max + 1, 64, 0, st>>>(a);
+  else
+    k_find_pairs<4><<<(ncl_max + 1 + 3) / 4, 256, 0, st>>>(a);
   HIP_CHECK(hipGetLastError());
   tb = c->scan_tmp_bytes;
   HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, tb, c->d_ci_count, c->d_ci_start,
