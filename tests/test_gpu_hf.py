@@ -155,7 +155,8 @@ def test_calheatf_matches_oracle(oracle_mod, prec, mode):
     scale = np.max(np.abs(same))
     assert scale > 0
     assert np.max(np.abs(j - same)) <= 1e-12 * scale
-    full = o.calheatf(hf, vel, mass, ref["pot_atm"], ref["viri_atm"].reshape(abi.NHF, 4, -1, 9))
+    full = o.calheatf(hf, vel, mass, ref["pot_atm"],
+                      ref["viri_atm"].reshape(abi.NHF, len(hf["z1"]), -1, 9))
     tol = 1e-10 if prec == "fp64" else 1e-4
     for t in range(abi.NHFJ):
         ts = np.max(np.abs(full[:, t]))
@@ -203,19 +204,26 @@ def test_calheatf_order_of_calls():
 
 @pytest.mark.parametrize("form", ["chunked", "per_cluster"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("mode", [abi.HF_VOLUME, abi.HF_BULK])
-def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode, form):
+@pytest.mark.parametrize("mode,regions", [(abi.HF_VOLUME, "four"), (abi.HF_BULK, "four"),
+                                          (abi.HF_BULK, "box")])
+def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode, regions, form):
     """The fused pass (no W arrays: each interaction's fi.v_i - fj.v_j times
     its region weights) against calforce_hf + calheatf on the same context,
     and end to end against the oracle; forces and energies are calforce_hf's.
     Both forms of the passes over the list: one wave per 64 entries of a
-    cluster (this system's default) and one wave per cluster."""
+    cluster (this system's default) and one wave per cluster.  "box": one
+    bulk region over the whole box with atom centres, where the pass sums the
+    pair energies' transport share itself instead of per-atom energies."""
     from thrid_amd.pgk import PgkError
     s = small_slab("rb")
     lz = s["box"][2]
-    hf = dict(mode=mode, z1=np.array([0.0, 0.3 * lz, 0.45 * lz, 0.2 * lz]),
-              z2=np.array([0.5 * lz, 0.6 * lz, lz, 0.7 * lz]), hftyp=_hf_centres(s),
-              molecom=molecom(s))
+    if regions == "box":
+        hf = dict(mode=mode, z1=np.array([0.0]), z2=np.array([lz]),
+                  hftyp=np.full(s["natom"], abi.HFTYP_ATOM, np.int32))
+    else:
+        hf = dict(mode=mode, z1=np.array([0.0, 0.3 * lz, 0.45 * lz, 0.2 * lz]),
+                  z2=np.array([0.5 * lz, 0.6 * lz, lz, 0.7 * lz]), hftyp=_hf_centres(s),
+                  molecom=molecom(s))
     ref, got, c = _run(oracle_mod, s, hf, prec, 1 << 40 if form == "chunked" else 0)
     rng = np.random.default_rng(23)
     vel = rng.normal(scale=0.02, size=(s["natom"], 3))
@@ -230,7 +238,8 @@ def test_calforce_heatf_matches_two_step(oracle_mod, prec, mode, form):
         np.testing.assert_allclose(fused["force"], got["force"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(fused["ene"], got["ene"], rtol=1e-12, atol=1e-12)
     o = oracle_mod.Oracle(s)
-    full = o.calheatf(hf, vel, mass, ref["pot_atm"], ref["viri_atm"].reshape(abi.NHF, 4, -1, 9))
+    full = o.calheatf(hf, vel, mass, ref["pot_atm"],
+                      ref["viri_atm"].reshape(abi.NHF, len(hf["z1"]), -1, 9))
     tol = 1e-10 if prec == "fp64" else 1e-4
     for t in range(abi.NHFJ):
         ts = np.max(np.abs(full[:, t]))

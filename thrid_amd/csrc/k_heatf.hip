@@ -29,6 +29,8 @@ struct HeatfArgs {
   int natom, nreg, nw, bulk, nsplit, chunk;
   int nsplit_col;             // partial records per collision row
   int nsplit_tr, chunk_tr;    // transport: atom chunks per region and their size
+  int ntr_slot;               // transport records per region: nsplit_tr, + 1 with the
+                              // pair pass's own transport sums (direct)
   const double *cz;           // the atoms' centre z (wrapped), or null: computed in place
   const double *z1, *z2;
   const int32_t *hftyp, *molid;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void k_heatf_transport(HeatfArgs a, double *__
       for (int c = 0; c < 3; c++) acc[3 * (1 + k) + c] += e * vc[c];
     }
   }
-  block_record<kNterm * 3, 4>(acc, part + ((size_t)r * a.nsplit_tr + split) * (kNterm * 3));
+  block_record<kNterm * 3, 4>(acc, part + ((size_t)r * a.ntr_slot + split) * (kNterm * 3));
 }
 
 // the atoms' centre z, wrapped into [0, Lz) as the transport part tests it
@@ -196,8 +198,8 @@ __global__ __launch_bounds__(256) void k_heatf_final(HeatfArgs a, const double *
     const int t = lane / 3, c = lane % 3;
     double s = 0.0;
     if (t < kNterm) {
-      for (int k = wave; k < a.nsplit_tr; k += 4)
-        s += ptr[((size_t)r * a.nsplit_tr + k) * (kNterm * 3) + 3 * t + c];
+      for (int k = wave; k < a.ntr_slot; k += 4)
+        s += ptr[((size_t)r * a.ntr_slot + k) * (kNterm * 3) + 3 * t + c];
     } else {
       const size_t row = (size_t)(t - kNterm) * a.nreg + r;
       for (int k = wave; k < a.nsplit_col; k += 4) s += pcol[(row * a.nsplit_col + k) * 3 + c];
@@ -224,25 +226,38 @@ __global__ __launch_bounds__(256) void k_heatf_final(HeatfArgs a, const double *
 // flux form: collision row (class, region) = the bonded terms' sums in jcol
 // (heat-flux accumulator type AH) plus, for the pair classes, the pair pass's
 // per-block records in block order (block (row, component) per launch block,
-// fixed-order tree)
+// fixed-order tree).  direct (bulk, one region): the pair pass's transport
+// sums too, into the transport record slot after the atom chunks' (tslot),
+// zero for the other terms.
 template <class AH>
 __global__ __launch_bounds__(256) void k_heatf_collect(int nreg, int bulk, int nblk,
                                                        const AH *__restrict__ jcol,
                                                        const double *__restrict__ jpart,
-                                                       double *__restrict__ pcol) {
+                                                       double *__restrict__ pcol, int direct,
+                                                       double *__restrict__ tslot) {
   const int row = blockIdx.x / 3, c = blockIdx.x % 3;   // row = class * nreg + region
   const int k = row / nreg, r = row % nreg;
   const int q = k - PGK_HF_ELIN;
-  double v[1] = {0.0};
-  if (q >= 0 && q < 4 && (bulk || c == 2)) {
-    const int stride = bulk ? 12 : 4 * nreg;
+  const bool pair = q >= 0 && q < 4;
+  double v[2] = {0.0, 0.0};
+  if (pair && (bulk || c == 2)) {
+    const int stride = bulk ? 24 : 4 * nreg;
     const int off = bulk ? 3 * q + c : q * nreg + r;
-    for (int b = threadIdx.x; b < nblk; b += 256) v[0] += jpart[(size_t)b * stride + off];
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+      v[0] += jpart[(size_t)b * stride + off];
+      if (direct) v[1] += jpart[(size_t)b * stride + 12 + off];
+    }
   }
   __shared__ double rec[PGK_REC];
-  block_record<1, 4>(v, rec);
+  block_record<2, 4>(v, rec);
   __syncthreads();
-  if (threadIdx.x == 0) pcol[(size_t)row * 3 + c] = acc_load(&jcol[(size_t)row * 3 + c]) + rec[0];
+  if (threadIdx.x == 0) {
+    pcol[(size_t)row * 3 + c] = acc_load(&jcol[(size_t)row * 3 + c]) + rec[0];
+    if (direct) {   // term 1 + k of the transport record (term 0: kinetic)
+      tslot[3 * (1 + k) + c] = pair ? rec[1] : 0.0;
+      if (k == 0) tslot[c] = 0.0;
+    }
+  }
 }
 
 // clears up to three 8-byte arrays in one launch (the heat-flux sums a pass
@@ -273,9 +288,11 @@ int launch_heatf(pgk_ctx *c, const HeatfArgs &a, double *ptr, double *pcol) {
   k_heatf_transport<AH><<<dim3(a.nsplit_tr, a.nreg), 256, 0, st>>>(a, ptr);
   HIP_CHECK(hipGetLastError());
   if (c->hfdev.flux) {
+    const int direct = a.ntr_slot > a.nsplit_tr;   // one region: its slot after the chunks'
     k_heatf_collect<AH><<<PGK_NHF * a.nreg * 3, 256, 0, st>>>(
         a.nreg, a.bulk, pgk_grid_blocks(pgk_pair_waves(c), 4),
-        reinterpret_cast<const AH *>(c->hfdev.jcol), c->hfdev.jpart, pcol);
+        reinterpret_cast<const AH *>(c->hfdev.jcol), c->hfdev.jpart, pcol, direct,
+        ptr + (size_t)a.nsplit_tr * (kNterm * 3));
   } else {
     k_heatf_collision<AH><<<dim3(a.nsplit, a.nreg * PGK_NHF), 256, 0, st>>>(a, pcol);
   }
@@ -336,6 +353,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
   nsplit_tr = std::max(1, std::min(nsplit_tr, (n + 255) / 256));
   a.nsplit_tr = nsplit_tr;
   a.chunk_tr = (n + nsplit_tr - 1) / nsplit_tr;
+  a.ntr_slot = nsplit_tr + ((h.flux && h.direct_tr) ? 1 : 0);
   a.cz = nullptr;
   a.z1 = h.z1;
   a.z2 = h.z2;
@@ -351,7 +369,7 @@ int pgk_heatf_launch(pgk_ctx *c, const double *d_vel, const double *d_mass,
   a.pe = h.pe;
   a.w = h.w;
   for (int d = 0; d < 3; d++) a.L[d] = h.L[d];
-  const size_t ntr = (size_t)h.nreg * nsplit_tr * kNterm * 3;
+  const size_t ntr = (size_t)h.nreg * a.ntr_slot * kNterm * 3;
   const size_t ncol = (size_t)rows * a.nsplit_col * 3;
   int rc;
   if (!h.flux && (rc = pgk_heatf_collision_vel(c, d_vel, d_vstrm, &a.cvel))) return rc;

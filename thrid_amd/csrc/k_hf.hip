@@ -54,19 +54,30 @@ constexpr int kPairCls0 = PGK_HF_ELIN;   // ELIN, LJIN, ELC, VDW are 4..7
 struct FluxPair {
   bool seg;          // the centres' segment counts (not two MOLE atoms of one molecule...)
   double rw;         // r . (v_i + v_j)
+  double w[3];       // v_i + v_j
   double rc[3];      // centre separation (minimum image in the bulk form)
   double ciz, cjz, rz;
 };
 
 // one interaction of pair class Q (0 ELIN, 1 LJIN, 2 ELC, 3 VDW) with energy
 // e and force coefficient f (force on i = f r): half the energy to each
-// atom, and f (r . (v_i + v_j)) / 2 times the region weights to the flux
-template <int Q, bool BULK>
+// atom, and f (r . (v_i + v_j)) / 2 times the region weights to the flux.
+// DT (bulk form over the whole box, atom centres, transport velocities =
+// the pair's velocities): the energies' share of the transport part,
+// e/2 (v_i + v_j), is summed here (tr) instead of per atom.
+template <int Q, bool BULK, bool DT>
 __device__ __forceinline__ void flux_add(const FluxPair &fp, double e, double f,
                                          double (&ei)[4], double (&ej)[4],
-                                         double (&acc)[4][3], double &sv_out) {
-  ei[Q] += 0.5 * e;
-  ej[Q] += 0.5 * e;
+                                         double (&acc)[4][3], double (&tr)[4][3],
+                                         double &sv_out) {
+  if (DT) {
+    const double he = 0.5 * e;
+#pragma unroll
+    for (int n = 0; n < 3; n++) tr[Q][n] += he * fp.w[n];
+  } else {
+    ei[Q] += 0.5 * e;
+    ej[Q] += 0.5 * e;
+  }
   if (!fp.seg) return;
   const double sv = f * fp.rw * 0.5;
   if (BULK) {
@@ -167,7 +178,7 @@ template <> __device__ __forceinline__ float hf_rinv<float>(double r2) {
 // CH: the chunked form, one wave per work item (64 list entries of one
 // cluster) instead of one wave per cluster; the i atom's energies are then
 // added once per item
-template <class AH, bool BULK, bool CH>
+template <class AH, bool BULK, bool CH, bool DT>
 __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
   // the mixed form (fixed-point accumulators) computes the pair terms in FP32
   using R = typename std::conditional<std::is_same<AH, unsigned long long>::value, float,
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
     }
   }
   const int nreg = h.nreg;
-  double acc[4][3] = {};
+  double acc[4][3] = {}, tr[4][3] = {};
   double *sl = lds + (size_t)wave * 4 * nreg;   // volume: this wave's slice
   double *zr = lds + (size_t)16 * nreg;           // volume: the regions' bounds
   if (!BULK) {
@@ -254,14 +265,15 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
           }
           // s = fi . v_i - fj . v_j with fj = -fi, per unit force coefficient
           const double4 vj = p.v4[js];
-          const double rw = r[0] * (vi.x + vj.x) + r[1] * (vi.y + vj.y) + r[2] * (vi.z + vj.z);
-          const FluxPair fp{seg, rw, {rc[0], rc[1], rc[2]}, cci.z, ccj.z, rz};
+          const double w[3] = {vi.x + vj.x, vi.y + vj.y, vi.z + vj.z};
+          const double rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
+          const FluxPair fp{seg, rw, {w[0], w[1], w[2]}, {rc[0], rc[1], rc[2]}, cci.z, ccj.z, rz};
           double svc = 0.0, svl = 0.0;
           if (r2 <= p.rrsq) {
             const R qq = (R)xi.w * (R)xj.w;
             const R ph = qq * rinv1;
-            if (same) flux_add<0, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
-            else flux_add<2, BULK>(fp, ph, ph * rinv2, ei, ej, acc, svc);
+            if (same) flux_add<0, BULK, DT>(fp, ph, ph * rinv2, ei, ej, acc, tr, svc);
+            else flux_add<2, BULK, DT>(fp, ph, ph * rinv2, ei, ej, acc, tr, svc);
           }
           if (r2 <= p.sqcut2) {
             const int tp = typi + p.ntype * p.ct[js];
@@ -272,37 +284,44 @@ __global__ __launch_bounds__(256) void k_hf_pairs_flux(HfPairArgs p, HfDev h) {
               const R t12 = t6 * t6;
               const R pot = ep * (t12 - (R)2 * t6);
               const R ft = (R)12 * ep * (t12 - t6) * rinv2;
-              if (same) flux_add<1, BULK>(fp, pot, ft, ei, ej, acc, svl);
-              else flux_add<3, BULK>(fp, pot, ft, ei, ej, acc, svl);
+              if (same) flux_add<1, BULK, DT>(fp, pot, ft, ei, ej, acc, tr, svl);
+              else flux_add<3, BULK, DT>(fp, pot, ft, ei, ej, acc, tr, svl);
             }
           }
           if (!BULK) flux_regions(h, fp, same ? 0 : 2, svc, same ? 1 : 3, svl, sl, zr);
         }
       }
-      // the j atom's four class energies summed over its 8 lanes; lanes
-      // ii and ii^1 then hold class (ii >> 1) and one of them adds it
-      const double sj4 = sum4x8(ej, ii);
-      if (aj >= 0 && (ii & 1) == 0 && sj4 != 0.0)
-        hf_pe_add<AH>(h, kPairCls0 + (ii >> 1), aj, sj4);
+      if (!DT) {
+        // the j atom's four class energies summed over its 8 lanes; lanes
+        // ii and ii^1 then hold class (ii >> 1) and one of them adds it
+        const double sj4 = sum4x8(ej, ii);
+        if (aj >= 0 && (ii & 1) == 0 && sj4 != 0.0)
+          hf_pe_add<AH>(h, kPairCls0 + (ii >> 1), aj, sj4);
+      }
     }
-    if (ai >= 0) {
+    if (!DT && ai >= 0) {
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (ei[q] != 0.0) hf_pe_add<AH>(h, kPairCls0 + q, ai, ei[q]);
     }
   }
   if (BULK) {
-    __shared__ double wsum[4][12];
+    // per block: the 12 flux sums, then (DT) the 12 transport sums, else 0
+    __shared__ double wsum[4][24];
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int n = 0; n < 3; n++) {
         const double t = wave_sum(acc[q][n]);
-        if (lane == 0) wsum[wave][3 * q + n] = t;
+        const double u = DT ? wave_sum(tr[q][n]) : 0.0;
+        if (lane == 0) {
+          wsum[wave][3 * q + n] = t;
+          wsum[wave][12 + 3 * q + n] = u;
+        }
       }
     __syncthreads();
-    if (threadIdx.x < 12)
-      h.jpart[(size_t)blockIdx.x * 12 + threadIdx.x] =
+    if (threadIdx.x < 24)
+      h.jpart[(size_t)blockIdx.x * 24 + threadIdx.x] =
           ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) +
           wsum[3][threadIdx.x];
   } else {
@@ -381,14 +400,19 @@ int launch_hf(pgk_ctx *c, bool do_pairs) {
     p.mol = c->d_hfw_mol;
     p.ct = c->d_ct;
     p.lj = c->d_ljp64;
+    // the pair energies' transport share summed in the pass (see flux_add)
+    const bool dt = h.mode == PGK_HF_BULK && h.direct_ok && h.vel == h.tvel;
+    c->hfdev.direct_tr = dt ? 1 : 0;
     if (h.mode == PGK_HF_BULK) {
-      if (ch)
-        k_hf_pairs_flux<AH, true, true><<<nblk, 256, 0, st>>>(p, h);
-      else
-        k_hf_pairs_flux<AH, true, false><<<nblk, 256, 0, st>>>(p, h);
+      auto kf = ch ? (dt ? k_hf_pairs_flux<AH, true, true, true>
+                         : k_hf_pairs_flux<AH, true, true, false>)
+                   : (dt ? k_hf_pairs_flux<AH, true, false, true>
+                         : k_hf_pairs_flux<AH, true, false, false>);
+      kf<<<nblk, 256, 0, st>>>(p, h);
     } else {
       const size_t lds = sizeof(double) * 18 * (size_t)h.nreg;   // 4 wave slices + bounds
-      auto kf = ch ? k_hf_pairs_flux<AH, false, true> : k_hf_pairs_flux<AH, false, false>;
+      auto kf = ch ? k_hf_pairs_flux<AH, false, true, false>
+                   : k_hf_pairs_flux<AH, false, false, false>;
       const int lr = pgk_lds_limit((const void *)kf, lds);
       if (lr) return lr;
       kf<<<nblk, 256, lds, st>>>(p, h);

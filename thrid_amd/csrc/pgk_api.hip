@@ -1224,13 +1224,20 @@ static int hf_setup(pgk_ctx *c, const pgk_hf_desc *hf, bool flux, const char *wh
   h.w = flux ? nullptr : c->d_hf_w;
   h.flux = flux ? 1 : 0;
   h.vel = flux ? c->d_hj_vel : nullptr;
+  h.tvel = nullptr;
+  h.direct_ok = (flux && hf->mode == PGK_HF_BULK && nreg == 1 && !need_com &&
+                 hf->z1[0] <= 0.0 && hf->z2[0] >= c->box[2])
+                    ? 1
+                    : 0;
+  h.direct_tr = 0;
   h.jcol = nullptr;
   h.jpart = nullptr;
   if (flux) {
     const size_t jcol_b = sizeof(double) * PGK_NHF * (size_t)nreg * 3;
     const size_t nblk = (size_t)pgk_grid_blocks(std::max<int64_t>(pgk_pair_waves(c), 1), 4);
+    // bulk: 12 flux sums and 12 transport sums per block; volume: [4][nreg]
     const size_t jpart_b =
-        sizeof(double) * nblk * (hf->mode == PGK_HF_BULK ? 12 : 4 * (size_t)nreg);
+        sizeof(double) * nblk * (hf->mode == PGK_HF_BULK ? 24 : 4 * (size_t)nreg);
     if ((rc = grow_dev((void **)&c->d_hj_jcol, &c->hj_jcol_cap, jcol_b)) ||
         (rc = grow_dev((void **)&c->d_hj_jpart, &c->hj_jpart_cap, jpart_b)))
       return rc;
@@ -1368,6 +1375,7 @@ int pgk_calforce_heatf(pgk_ctx *c, int32_t mts_flag, const pgk_mts *mts, const p
       (rc = hf_setup(c, hf, true, "pgk_calforce_heatf")) ||
       (rc = pgk_heatf_collision_vel(c, c->d_hj_vel, hj_vstrm(c), &c->hfdev.vel)))
     return rc;
+  c->hfdev.tvel = c->d_hj_vel;   // what pgk_heatf_launch transports below
   c->hf_ready = false;
   c->hf_export = false;
   rc = run_calforce(c, mts_flag, mts, 1, out->for_viri_coul && out->for_viri_lj, c->d_force,
@@ -1630,6 +1638,7 @@ static int md_hf_sample(pgk_ctx *c, int s) {
   // taken relative to the streaming velocities (calheatf.F90:368-369)
   const double *vs = c->md_lt_nstrm ? c->d_md_vstrm : nullptr;
   if ((rc = pgk_heatf_collision_vel(c, c->d_md_vel, vs, &c->hfdev.vel))) return rc;
+  c->hfdev.tvel = c->d_md_vel;   // what pgk_heatf_launch transports below
   c->hf_export = false;
   c->hf_pairs_only = true;
   // the energy records of the step stay the ones pgk_energies reports

@@ -87,7 +87,13 @@ struct HfDev {
   // flux form (pgk_calforce_heatf): instead of the W arrays, the collision
   // part of calheatf, sum_a W(a,n,i,r) v_i(a), is accumulated directly
   int flux = 0;
-  const double *vel = nullptr;                 // [3*natom] atom velocities
+  const double *vel = nullptr;                 // [3*natom] collision-part velocities
+  // the transport part's velocities (set by the flux form's callers); with
+  // one bulk region over the whole box, atom centres only and tvel == vel
+  // (direct_ok and the pointers equal) the pair pass sums the pair
+  // energies' transport share itself (direct_tr) instead of per-atom energies
+  const double *tvel = nullptr;
+  int direct_ok = 0, direct_tr = 0;
   double *jcol = nullptr;   // [PGK_NHF][nreg][3] sums, heat-flux accumulator type (int64 fixed point in the mixed form)
   double *jpart = nullptr;                     // pair pass: per-block records
 };
