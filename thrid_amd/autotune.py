@@ -4,7 +4,8 @@ The pair kernel has two forms (pgk_tune, PGK_TUNE_PAIR_CHUNK_BELOW): one wave
 per 8-atom i-cluster walking the cluster's whole list, or the chunked form,
 one wave per 64 list entries of a cluster.  The first wastes nothing on large
 systems; the second fills the device when a system has too few clusters for
-one wave each (config 4's replicas, the 18k-atom slab).  This module times
+one wave each (the 1.5k-12k-atom replicas of a heat-flux ensemble, the
+18k-atom slab).  This module times
 both forms on a ladder of water boxes, takes the crossover in cluster count,
 and writes it with the measurements to tuning.json beside this file, which
 ForceContext applies at creation (the library's compiled default,

@@ -2,7 +2,7 @@
 the heat-flux sample, one context at a time (the workload a kernel profile
 of the heat-flux path is taken on).
 
-    python -m thrid_amd.mdprobe replicas   # three config-4-like replicas, bulk flux
+    python -m thrid_amd.mdprobe replicas   # three 1.6k-11k-atom replicas, bulk flux
     python -m thrid_amd.mdprobe slab       # the 18.4k water/propanol slab, bulk and 1 A bins
     python -m thrid_amd.mdprobe interface  # the ~1M-atom interface, 300 bins, both thermostats
 
@@ -42,8 +42,9 @@ def _mole_waters(s):
 
 
 def replicas(prec="mixed", nstep=50):
-    """Three replicas of config 4's range (water 1.6k, water/propanol 5k,
-    n-butane 11k atoms): no flux, then the bulk flux every step."""
+    """Three replicas spanning the 64-replica ensemble's 1.5k-12k-atom range
+    (water 1.6k, water/propanol 5k, n-butane 11k atoms): no flux, then the
+    bulk flux every step."""
     reps = [systems.mixture_box(1600, "water", seed=21, name="water1600"),
             systems.mixture_box(5000, "mix", seed=22, alcohol_frac=0.12, name="mix5000"),
             systems.mixture_box(11000, "butane", seed=23, name="butane11000")]
@@ -59,7 +60,7 @@ def replicas(prec="mixed", nstep=50):
 
 
 def slab(prec="mixed", nstep=20):
-    """The 18.4k-atom water/propanol slab (config 2's box): no flux, the bulk
+    """The 18.4k-atom water/propanol slab (50 x 50 x 150 A): no flux, the bulk
     flux, and 1 A bins over the box (waters by molecule centre), every step."""
     s = systems.propanol_water_slab()
     cons = _relaxed(s)
@@ -79,8 +80,8 @@ def slab(prec="mixed", nstep=20):
 
 
 def interface(prec="mixed", nstep=5):
-    """Config 5: the ~1M-atom interface with Langevin ends, the chain on the
-    rest and the 300-bin flux every step."""
+    """The ~1M-atom two-sided alcohol/water interface (200 x 200 x 300 A) with
+    Langevin ends, the chain on the rest and the 300-bin flux every step."""
     s = systems.interface_slab()
     cons = _relaxed(s, schedule=((0.02, 10), (0.1, 10), (0.5, 10)))
     n = s["natom"]
