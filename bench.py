@@ -12,7 +12,10 @@ Prints one JSON line.  value = atom-steps/s of the mixed form; beside it the
 FP64 form, ns/day, the force-only step rate, the pair kernel's roofline (HIP
 events on the library stream), each form's force error against the
 reference's FP64 forces, NVE energy drift per form and the reference CPU
-baseline.
+baseline.  Further legs (each with its own flag): the heat-flux kernels on
+the water box, the 18.4k-atom water/propanol slab and the ~1M-atom interface
+under their thermostats with the heat flux every step, and the replica
+ensemble.
 """
 import argparse
 import json
@@ -274,6 +277,45 @@ def run_interface(steps, precs):
     return out
 
 
+def run_slab(steps, precs):
+    """The 18.4k-atom water/propanol slab (4096 waters, 512 1-propanols,
+    50 x 50 x 150 A, SPME 64 x 64 x 192 order 4, 12 A cutoff): after a short
+    FP64 quench, `steps` MD steps per form under a Nose-Hoover chain at 300 K
+    with the box's heat-flux vector (bulk form, one region over the box, atom
+    centres) sampled every step.  Returns ms per step and atom-steps/s per form."""
+    from thrid_amd import abi, md, systems
+    from thrid_amd.pgk import ForceContext
+    t0 = time.perf_counter()
+    s = systems.propanol_water_slab()
+    cons = md.rigid_constraints(s)
+    c = ForceContext(s, "fp64")
+    s["pos"] = md.relax(c, s, cons)
+    c.close()
+    n = s["natom"]
+    ij, d, gs = cons
+    lz = float(s["box"][2])
+    out = {"atoms": n, "setup_s": time.perf_counter() - t0, "steps": steps}
+    for prec in precs:
+        c = ForceContext(s, prec)
+        c.md_setup(s["mass"], ij, d, gs, DT_FS)
+        c.md_set_velocities(md.maxwell_boltzmann(s, 300.0, seed=57))
+        c.md_nhc(300.0, 0.01, 3 * n - len(d) - 3)
+        c.md_heatf(dict(mode=abi.HF_BULK, z1=np.array([0.0]), z2=np.array([lz])), 1)
+        c.md_run(5, NSTEP_BOOK, s["rcut_book"], 5)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.md_run(steps, NSTEP_BOOK, s["rcut_book"], steps)
+        c.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        rows = c.md_heatf_series()
+        out[prec] = {"ms_per_step": sec * 1e3, "atom_steps_per_s": n / sec,
+                     "ns_per_day": 86400.0 / sec * DT_FS * 1e-6,
+                     "flux_rows": int(rows.shape[0]),
+                     "flux_finite": bool(np.all(np.isfinite(rows)))}
+        c.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (scale for the HBM byte counters)."""
     import torch
@@ -453,6 +495,8 @@ def main():
     ap.add_argument("--ensemble-hf", type=int, default=1, metavar="K",
                     help="heat-flux sample every K ensemble steps (0: none)")
     ap.add_argument("--no-heatf", action="store_true", help="skip the heat-flux leg")
+    ap.add_argument("--slab", type=int, default=200, metavar="STEPS",
+                    help="MD steps of the 18.4k-atom slab leg per form (0 skips it)")
     ap.add_argument("--interface", type=int, default=20, metavar="STEPS",
                     help="MD steps of the ~1M-atom interface leg per form (0 skips it)")
     ap.add_argument("--extras", action="store_true",
@@ -557,6 +601,14 @@ def main():
                 f"ms (with host arrays {v['calforce_hf_with_host_arrays_ms']:.2f} ms; fused "
                 f"{v['fused_calforce_heatf_ms']:.2f} ms)"
                 for k, v in h.items()), file=sys.stderr, flush=True)
+    if args.slab > 0:
+        sprecs = [p for p in precs if p != "fp32"]     # the slab runs FP64 and mixed
+        out["slab"] = run_slab(args.slab, sprecs)
+        for p in sprecs:
+            e = out["slab"][p]
+            print(f"# slab {p}: {out['slab']['atoms']} atoms, {e['ms_per_step']:.3f} ms/step "
+                  f"({e['atom_steps_per_s']:.3e} atom-steps/s) with the chain and the heat flux "
+                  f"every step", file=sys.stderr, flush=True)
     if args.interface > 0:
         iprecs = [p for p in precs if p != "fp32"]     # the interface runs FP64 and mixed
         out["interface"] = run_interface(args.interface, iprecs)
