@@ -316,14 +316,22 @@ __global__ __launch_bounds__(64 * NW) void k_pairs(PairArgs<PREC> a) {
         d.tj = a.ct[slot];
       };
       // two passes in flight: each buffer is refilled right after its pass
-      // is computed, while the other buffer's pass is computed
+      // is computed, while the other buffer's pass is computed.  Without
+      // energies the second pass is not guarded by k + 2 < nin: past the
+      // chunk's end its mask is 0, so it adds exact zeros and no j-atomics,
+      // and the loop body stays one straight path.  With a guard, the
+      // wait-count analysis merges the two paths and makes the first pass
+      // wait for the second buffer's loads (and the previous pass's
+      // j-atomics) as well as its own: the MD step's kernel takes 10 % longer.
+      // The energy kernels keep the guard: unguarded, their interleaved
+      // passes need more registers and run at lower occupancy.
       Ent A, B;
       fetch(A, 0);
       fetch(B, 2);
       for (int k = 0; k < nin; k += 4) {
         entry(A.hc, A.m, A.xj, A.tj);
         fetch(A, k + 4);
-        if (k + 2 < nin) {
+        if (!ENE || k + 2 < nin) {
           entry(B.hc, B.m, B.xj, B.tj);
           fetch(B, k + 6);
         }
