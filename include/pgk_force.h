@@ -582,7 +582,7 @@ int pgk_profile_stages(pgk_ctx *ctx, uint32_t mask);
  *   PGK_TUNE_PME_OVERLAP: 1 (default) runs the reciprocal chain on a second
  *     stream beside the pair kernel when both are due, 0 one after the other.
  *   PGK_TUNE_PAIR_CAP_BLOCKS: pair blocks per CU while the chain runs beside
- *     them (default 3; 0 no cap).                                          */
+ *     them (default 2; 0 no cap); the chunked form keeps 3 unless 0.        */
 enum { PGK_TUNE_PAIR_CHUNK_BELOW = 1, PGK_TUNE_PME_OVERLAP = 2, PGK_TUNE_PAIR_CAP_BLOCKS = 3 };
 #define PGK_TUNE_DEFAULT_CHUNK_BELOW 4352
 int pgk_tune(pgk_ctx *ctx, int32_t knob, int64_t value);

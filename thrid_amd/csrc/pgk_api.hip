@@ -319,9 +319,12 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
   const bool timed_other = c->prof && (c->prof_mask & ~(1u << PGK_PROF_PAIRS)) != 0;
   const bool overlap = do_pair && do_ewk && !hf && !timed_other && c->pme_overlap;
   // beside the chain the excluded-pair correction goes to the shorter stream:
-  // the chain's for large systems, the pair kernel's for small ones (chunked
-  // form), whose reciprocal chain outlasts the pair and bonded kernels
-  const bool excl_main = overlap && pgk_pair_chunked(c);
+  // the pair kernel's where the chain outlasts the pair and bonded kernels
+  // (small systems in the chunked form; the FP64 form, whose chain runs in
+  // FP64: 0.376 -> 0.368 ms per step on the 100k water box), the chain's
+  // otherwise (mixed and FP32 on large systems: 0.291 ms there against 0.296)
+  const bool excl_main =
+      overlap && (pgk_pair_chunked(c) || c->precision == PGK_PREC_FP64);
   if (overlap) {
     HIP_CHECK(hipEventRecord(c->ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
@@ -351,8 +354,12 @@ int run_calforce(pgk_ctx *c, int32_t mts_flag, const pgk_mts *m, int32_t preatom
       HIP_CHECK(hipGetLastError());
     }
     pgk_prof_begin(c, PGK_PROF_PAIRS);
-    // beside the chain, a cap on the pair blocks per CU
-    c->pair_cap_blocks = overlap ? c->pair_cap_default : 0;
+    // beside the chain, a cap on the pair blocks per CU (the chunked form's
+    // few hundred blocks fill the device in one round at three per CU)
+    c->pair_cap_blocks =
+        !overlap ? 0
+                 : (pgk_pair_chunked(c) && c->pair_cap_default > 0 ? c->pair_cap_chunked
+                                                                  : c->pair_cap_default);
     rc = pgk_pairs_launch(c, split, energies, &seg_off[nseg], &seg_n[nseg]);
     c->pair_cap_blocks = 0;
     if (rc) return rc;

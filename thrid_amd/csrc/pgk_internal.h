@@ -190,7 +190,8 @@ struct pgk_ctx {
   int32_t *d_wcount = nullptr, *d_wstart = nullptr, *d_witem = nullptr;
   int nwi = 0;
   int64_t pair_chunk_below = PGK_TUNE_DEFAULT_CHUNK_BELOW;   // chunked form below this ncl
-  int pair_cap_default = 3;        // pair blocks per CU beside the chain (0: no cap)
+  int pair_cap_default = 2;        // pair blocks per CU beside the chain (0: no cap)
+  int pair_cap_chunked = 3;        // the same for the chunked form
   int32_t *d_cj = nullptr;         // entries: half-cluster index 2 cj + h
   uint32_t *d_mask = nullptr;      // entries: bit ii + 8 jq for i-slot ii, j-slot 4 h + jq
   int64_t npair_cl = 0, cap_pair_cl = 0;
