@@ -153,3 +153,18 @@ def test_committed_tuning_table_matches_the_compiled_default():
     assert t is not None
     assert t["pair_chunk_below"] == abi.TUNE_DEFAULT_CHUNK_BELOW
     assert t["pair_chunk_below"] == autotune.crossover(t["measurements"])
+
+
+def test_documented_launch_defaults_match_the_library_source():
+    """The header's pgk_tune documentation states the defaults the context
+    starts with; the context's fields in the library source hold them."""
+    import re
+    header = open(HEADER).read()
+    internal = open(os.path.join(os.path.dirname(pgk.__file__), "csrc", "pgk_internal.h")).read()
+    doc = re.search(r"PGK_TUNE_PAIR_CAP_BLOCKS:(.*?)\*/", header, re.S).group(1)
+    m = re.search(r"default (\d+); 0 no cap\); the chunked form keeps (\d+)", " ".join(doc.split()))
+    assert m, doc
+    assert int(m.group(1)) == int(re.search(r"int pair_cap_default = (\d+);", internal).group(1))
+    assert int(m.group(2)) == int(re.search(r"int pair_cap_chunked = (\d+);", internal).group(1))
+    assert re.search(r"bool pme_overlap = true;", internal)
+    assert "PGK_TUNE_PME_OVERLAP: 1 (default)" in header
